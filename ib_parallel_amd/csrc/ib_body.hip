@@ -1,0 +1,381 @@
+// ib_body.hip -- immersed-boundary coupling and torsional bodies on CDNA4.
+//
+// reg (E^T) is a scatter in the reference (VecSetValues ADD_VALUES in tap
+// order, src/fsinterface.F90:14-47).  Here it is an atomics-free gather: the
+// (entry, tap) pairs are sorted once per weight rebuild by destination face
+// with a stable radix sort, run-length encoded, and one thread per touched
+// face sums its contributions in tap order (k, then l, then p) -- the
+// reference's own addition order, so the result is deterministic run to run.
+#include <hipcub/hipcub.hpp>
+
+#include "ib_kernels.h"
+
+namespace ibg {
+
+// deltafnc of src/grid.F90:955-1003.  With f32 set, the exponent-less literals
+// take their REAL(4) values as in the reference source.
+__device__ double deltafnc(double x1, double x2, double dr, int f32) {
+  const double r = fabs(x1 - x2);
+  const double r1 = r / dr, r2 = r1 * r1, r3 = r2 * r1, r4 = r3 * r1;
+  double d;
+#define F(lit) (f32 ? (double)(lit##f) : (lit))
+  if (r1 <= 1.0) {
+    const double a5 = asin((1.0 / 2.0) * sqrt(3.0) * (2.0 * r1 - 1.0));
+    const double a8 = sqrt(1.0 - 12.0 * r2 + 12.0 * r1);
+    d = 0.4166666667e-1 * r4 + (F(-.1388888889) + 0.3472222222e-1 * a8) * r3 +
+        (-0.7121664902e-1 - 0.5208333333e-1 * a8 + F(0.2405626122) * a5) * r2 +
+        (F(-.2405626122) * a5 + F(-.3792313933) + F(.1012731481) * a8) * r1 + 0.8018753741e-1 * a5 -
+        0.4195601852e-1 * a8 + F(.6485698427);
+  } else if (r1 <= 2.0) {
+    const double a6 = asin((1.0 / 2.0) * sqrt(3.0) * (-3.0 + 2.0 * r1));
+    const double a9 = sqrt(-23.0 + 36.0 * r1 - 12.0 * r2);
+    d = -0.6250000000e-1 * r4 + (F(.4861111111) - 0.1736111111e-1 * a9) * r3 +
+        (F(-1.143175026) + 0.7812500000e-1 * a9 + F(-.1202813061) * a6) * r2 +
+        (F(.8751991178) + F(.3608439183) * a6 + F(-.1548032407) * a9) * r1 + F(-.2806563809) * a6 +
+        0.822848104e-2 + F(.1150173611) * a9;
+  } else if (r1 <= 3.0) {
+    const double a1 = asin((1.0 / 2.0 * (2.0 * r1 - 5.0)) * sqrt(3.0));
+    const double a7 = sqrt(-71.0 - 12.0 * r2 + 60.0 * r1);
+    d = 0.2083333333e-1 * r4 + (0.3472222222e-2 * a7 + F(-.2638888889)) * r3 +
+        (F(1.214391675) - 0.2604166667e-1 * a7 + 0.2405626122e-1 * a1) * r2 +
+        (F(-.1202813061) * a1 + F(-2.449273192) + 0.7262731481e-1 * a7) * r1 + F(.1523563211) * a1 +
+        F(1.843201677) - 0.7306134259e-1 * a7;
+  } else {
+    d = 0.0;
+  }
+#undef F
+  return d / dr;
+}
+
+// setup_reg: src/grid.F90:459-528, one thread per (entry, tap)
+__global__ void k_setup_reg(Geom g, int nf, const double* __restrict__ xb, const double* __restrict__ yb,
+                            const int* __restrict__ comp, double delta, double offx, double offy, int f32,
+                            double* __restrict__ w, int* __restrict__ key, int* __restrict__ err) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nf * kTaps) return;
+  const int k = idx / kTaps, t = idx - k * kTaps;
+  const int l = t / 7 - 3, p = t % 7 - 3;
+  const double d2 = 0.5 * delta;
+  const int ix = (int)((xb[k] + offx) / delta) + 1;
+  const int iy = (int)((yb[k] + offy) / delta) + 1;
+  const int ii = ix + p, jj = iy + l;
+  double x, y;
+  int kk;
+  bool ok;
+  if (comp[k] == 1) {
+    x = delta * (ix - 1 + p) - offx;
+    y = delta * (iy - 1 + l) - offy + d2;
+    ok = ii >= 1 && ii <= g.m + 1 && jj >= 1 && jj <= g.n;
+    kk = (g.m + 1) * (jj - 1) + (ii - 1);
+  } else {
+    x = delta * (ix - 1 + p) - offx + d2;
+    y = delta * (iy - 1 + l) - offy;
+    ok = ii >= 1 && ii <= g.m && jj >= 1 && jj <= g.n + 1;
+    kk = g.Fx + g.m * (jj - 1) + (ii - 1);
+  }
+  if (!ok) {
+    *err = 1;
+    w[idx] = 0.0;
+    key[idx] = 0;
+    return;
+  }
+  w[idx] = delta * delta * deltafnc(x, xb[k], delta, f32) * deltafnc(y, yb[k], delta, f32);
+  key[idx] = kk;
+}
+
+void setup_reg(hipStream_t st, const Geom& g, int nf, const double* xb, const double* yb, const int* comp,
+               double delta, double offx, double offy, int f32lit, double* w, int* key, int* err) {
+  const int nt = nf * kTaps;
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_setup_reg, dim3(ceil_div(nt, 256)), dim3(256), 0, st, g, nf, xb, yb, comp, delta, offx, offy,
+                     f32lit, w, key, err);
+  IB_LAUNCH_CHECK();
+}
+
+__global__ void k_iota(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+
+__global__ void k_zero_int(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+void RegCSR::alloc(int ntaps_, int max_key) {
+  ntaps = ntaps_;
+  key_bits = 1;
+  while ((1LL << key_bits) <= (long long)max_key) ++key_bits;
+  const size_t b = sizeof(int) * (size_t)(ntaps > 0 ? ntaps : 1);
+  IB_HIP(hipMalloc(&key, b));
+  IB_HIP(hipMalloc(&key_sorted, b));
+  IB_HIP(hipMalloc(&tap, b));
+  IB_HIP(hipMalloc(&tap_sorted, b));
+  IB_HIP(hipMalloc(&uniq, b));
+  IB_HIP(hipMalloc(&cnt, b));
+  IB_HIP(hipMalloc(&off, b));
+  IB_HIP(hipMalloc(&nruns, sizeof(int)));
+  size_t t1 = 0, t2 = 0, t3 = 0;
+  IB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, key, key_sorted, tap, tap_sorted, ntaps, 0, key_bits));
+  IB_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, t2, key_sorted, uniq, cnt, nruns, ntaps));
+  IB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t3, cnt, off, ntaps));
+  tmp_bytes = std::max(t1, std::max(t2, t3));
+  IB_HIP(hipMalloc(&tmp, tmp_bytes > 0 ? tmp_bytes : 16));
+  if (ntaps > 0) {
+    hipLaunchKernelGGL(k_iota, dim3(ceil_div(ntaps, 256)), dim3(256), 0, 0, tap, ntaps);
+    IB_LAUNCH_CHECK();
+    IB_HIP(hipDeviceSynchronize());
+  }
+}
+
+void RegCSR::release() {
+  int* ps[] = {key, key_sorted, tap, tap_sorted, uniq, cnt, off, nruns};
+  for (int* p : ps)
+    if (p) (void)hipFree(p);
+  if (tmp) (void)hipFree(tmp);
+  key = key_sorted = tap = tap_sorted = uniq = cnt = off = nruns = nullptr;
+  tmp = nullptr;
+}
+
+void RegCSR::build(hipStream_t st) {
+  if (ntaps <= 0) return;
+  size_t b = tmp_bytes;
+  IB_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, b, key, key_sorted, tap, tap_sorted, ntaps, 0, key_bits, st));
+  hipLaunchKernelGGL(k_zero_int, dim3(ceil_div(ntaps, 256)), dim3(256), 0, st, cnt, ntaps);
+  IB_LAUNCH_CHECK();
+  b = tmp_bytes;
+  IB_HIP(hipcub::DeviceRunLengthEncode::Encode(tmp, b, key_sorted, uniq, cnt, nruns, ntaps, st));
+  b = tmp_bytes;
+  IB_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, b, cnt, off, ntaps, st));
+}
+
+// reg: one thread per touched face, contributions summed in tap order
+__global__ void k_reg(Geom g, int ntaps, const int* __restrict__ nruns, const int* __restrict__ uniq,
+                      const int* __restrict__ off, const int* __restrict__ cnt, const int* __restrict__ taps,
+                      const double* __restrict__ w, const double* __restrict__ f, double* __restrict__ hx,
+                      double* __restrict__ hy) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= ntaps || r >= *nruns) return;
+  const int o = off[r], c = cnt[r], key = uniq[r];
+  double acc = 0.0;
+  for (int q = o; q < o + c; ++q) {
+    const int tp = taps[q];
+    acc = acc + w[tp] * f[tp / kTaps];
+  }
+  if (key < g.Fx) hx[key] = acc;
+  else hy[key - g.Fx] = acc;
+}
+
+void reg(hipStream_t st, const Geom& g, const RegCSR& c, const double* w, const double* f, double* hx, double* hy) {
+  if (c.ntaps <= 0) return;
+  hipLaunchKernelGGL(k_reg, dim3(ceil_div(c.ntaps, 256)), dim3(256), 0, st, g, c.ntaps, c.nruns, c.uniq, c.off, c.cnt,
+                     c.tap_sorted, w, f, hx, hy);
+  IB_LAUNCH_CHECK();
+}
+
+// regT: src/fsinterface.F90:51-80, one thread per force entry
+__global__ void k_regT(Geom g, int nf, const double* __restrict__ w, const int* __restrict__ key,
+                       const double* __restrict__ hx, const double* __restrict__ hy, double shift_x,
+                       const double* __restrict__ base, double* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nf) return;
+  double acc = base ? base[k] : 0.0;
+  for (int t = 0; t < kTaps; ++t) {
+    const int kk = key[k * kTaps + t];
+    const double v = kk < g.Fx ? hx[kk] + shift_x : hy[kk - g.Fx];
+    acc = acc + w[k * kTaps + t] * v;
+  }
+  out[k] = acc;
+}
+
+void regT(hipStream_t st, const Geom& g, int nf, const double* w, const int* key, const double* hx, const double* hy,
+          double shift_x, const double* base, double* out) {
+  if (nf <= 0) return;
+  hipLaunchKernelGGL(k_regT, dim3(ceil_div(nf, 128)), dim3(128), 0, st, g, nf, w, key, hx, hy, shift_x, base, out);
+  IB_LAUNCH_CHECK();
+}
+
+// redistribute2 on the touched faces: finter = 1/wght * frc where wght > 1e-10
+__global__ void k_redist(Geom g, int ntaps, const int* __restrict__ nruns, const int* __restrict__ uniq,
+                         const int* __restrict__ off, const int* __restrict__ cnt, const int* __restrict__ taps,
+                         const double* __restrict__ w, const double* __restrict__ f, double* __restrict__ fx,
+                         double* __restrict__ fy) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= ntaps || r >= *nruns) return;
+  const int o = off[r], c = cnt[r], key = uniq[r];
+  double ws = 0.0, fs = 0.0;
+  for (int q = o; q < o + c; ++q) {
+    const int tp = taps[q];
+    ws = ws + w[tp] * 1.0;
+    fs = fs + w[tp] * f[tp / kTaps];
+  }
+  const double v = ws > 1e-10 ? 1.0 / ws * fs : 0.0;
+  if (key < g.Fx) fx[key] = v;
+  else fy[key - g.Fx] = v;
+}
+
+void redist_faces(hipStream_t st, const Geom& g, const RegCSR& c, const double* w, const double* f, double* fx_out,
+                  double* fy_out) {
+  if (c.ntaps <= 0) return;
+  hipLaunchKernelGGL(k_redist, dim3(ceil_div(c.ntaps, 256)), dim3(256), 0, st, g, c.ntaps, c.nruns, c.uniq, c.off,
+                     c.cnt, c.tap_sorted, w, f, fx_out, fy_out);
+  IB_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------------
+// torsional bodies (family 2), src/operators_struct.F90
+// ------------------------------------------------------------------------
+
+__device__ __forceinline__ double BR(const TorsBody& b, int j, double si, double ci) {
+  // BR_build (:319-338), j = 1..2*npts
+  return j <= b.npts ? (j - 1) * b.deltas * si : -(j - b.npts - 1) * b.deltas * ci;
+}
+
+__device__ double block_sum(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] = sh[t] + sh[t + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// b_times2 (:14-127) as the rank-1 product c * BR (BR^T x) / den
+__global__ void __launch_bounds__(256) k_btimes(const TorsBody* __restrict__ tbs, const double* __restrict__ theta,
+                                                double dt, double delta, const double* __restrict__ x,
+                                                double* __restrict__ y) {
+  __shared__ double sh[256];
+  const TorsBody b = tbs[blockIdx.x];
+  const double th = theta[blockIdx.x];
+  const double si = sin(b.gamma0 - th), ci = cos(b.gamma0 - th);
+  double part = 0.0;
+  for (int j = threadIdx.x + 1; j <= 2 * b.npts; j += blockDim.x) part += BR(b, j, si, ci) * x[b.off + j - 1];
+  const double dot = block_sum(part, sh);
+  const double den = 4.0 / (dt * dt) * b.M + 2.0 / dt * b.C + b.K;
+  const double sc = (delta * delta / b.deltas / dt) * (2 * b.deltas / dt);
+  for (int j = threadIdx.x + 1; j <= 2 * b.npts; j += blockDim.x)
+    y[b.off + j - 1] = y[b.off + j - 1] + BR(b, j, si, ci) * dot / den * sc;
+}
+
+void body_btimes(hipStream_t st, int nt, const TorsBody* tb, const double* theta, double dt, double delta,
+                 const double* x, double* y) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_btimes, dim3(nt), dim3(256), 0, st, tb, theta, dt, delta, x, y);
+  IB_LAUNCH_CHECK();
+}
+
+// compute_rhs (:342-398) for torsional entries + regT of q + q0 (rhsf of the
+// FSI loop, src/timestepper.F90:322-358); one thread per force entry
+__global__ void k_fsi_rhs(Geom g, int nf, const int* __restrict__ tors_of, const TorsBody* __restrict__ tbs, int nt,
+                          const double* __restrict__ th, double dt, double delta, const double* __restrict__ w,
+                          const int* __restrict__ key, const double* __restrict__ qx, const double* __restrict__ qy,
+                          double q0x, double* __restrict__ rhsf) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nf) return;
+  double v = 0.0;
+  const int t = tors_of[k];
+  if (t >= 0) {
+    const TorsBody b = tbs[t];
+    const double theta = th[t], thetad = th[nt + t];
+    const double theta0 = th[3 * nt + t], thetad0 = th[4 * nt + t], thetadd0 = th[5 * nt + t];
+    const double rth = thetad0 + thetad + 2.0 / dt * (theta0 - theta);
+    const double den = 4.0 / (dt * dt) * b.M + 2.0 / dt * b.C + b.K;
+    const double jr = 2.0 / dt / den *
+                      (b.M * (4.0 / (dt * dt) * theta0 + 4.0 / dt * thetad0 + thetadd0) +
+                       b.C * (thetad0 + 2.0 / dt * theta0) - (4.0 / (dt * dt) * b.M + 2.0 / dt * b.C + b.K) * theta +
+                       0.0 + 0.0);
+    const double sc = delta * (rth - jr - thetad);
+    const double si = sin(b.gamma0 - theta), ci = cos(b.gamma0 - theta);
+    v = BR(b, k - b.off + 1, si, ci) * sc;
+  }
+  for (int tp = 0; tp < kTaps; ++tp) {
+    const int kk = key[k * kTaps + tp];
+    const double u = kk < g.Fx ? qx[kk] + q0x : qy[kk - g.Fx] + 0.0;
+    v = v + w[k * kTaps + tp] * u;
+  }
+  rhsf[k] = v;
+}
+
+void fsi_rhs(hipStream_t st, const Geom& g, int nf, const int* tors_of_entry, const TorsBody* tb, int nt,
+             const double* th, double dt, double delta, const double* w, const int* key, const double* qx,
+             const double* qy, double q0x, double* rhsf) {
+  if (nf <= 0) return;
+  hipLaunchKernelGGL(k_fsi_rhs, dim3(ceil_div(nf, 128)), dim3(128), 0, st, g, nf, tors_of_entry, tb, nt, th, dt, delta,
+                     w, key, qx, qy, q0x, rhsf);
+  IB_LAUNCH_CHECK();
+}
+
+// compute_dx (:402-505) + update_xb (:600-620); one workgroup per body
+__global__ void __launch_bounds__(256) k_body_dx(const TorsBody* __restrict__ tbs, double* __restrict__ th, double dt,
+                                                 double delta, const double* __restrict__ f, double* __restrict__ xb,
+                                                 double* __restrict__ yb, double* __restrict__ dabs) {
+  __shared__ double sh[256];
+  __shared__ double s_theta;
+  const int nt = gridDim.x, t = blockIdx.x;
+  const TorsBody b = tbs[t];
+  const double theta = th[t];
+  const double theta0 = th[3 * nt + t], thetad0 = th[4 * nt + t], thetadd0 = th[5 * nt + t];
+  const double si = sin(b.gamma0 - theta), ci = cos(b.gamma0 - theta);
+  double part = 0.0;
+  for (int j = threadIdx.x + 1; j <= 2 * b.npts; j += blockDim.x) part += BR(b, j, si, ci) * f[b.off + j - 1];
+  double brf = block_sum(part, sh);
+  if (threadIdx.x == 0) {
+    const double jr = 1.0 / (4.0 / (dt * dt) * b.M + 2.0 / dt * b.C + b.K) *
+                      (b.M * (4 / (dt * dt) * theta0 + 4 / dt * thetad0 + thetadd0) +
+                       b.C * (thetad0 + 2.0 / dt * theta0) - (4 / (dt * dt) * b.M + 2.0 / dt * b.C + b.K) * theta +
+                       0.0 + 0.0);
+    brf = b.deltas / (4 / (dt * dt) * b.M + b.K) * brf * (delta / b.deltas / dt);
+    const double dth = jr + brf;
+    const double thn = theta + dth;
+    th[t] = thn;
+    th[nt + t] = -thetad0 + 2 / dt * (thn - theta0);
+    th[2 * nt + t] = -thetadd0 - 4 / dt * thetad0 + 4 / (dt * dt) * (thn - theta0);
+    dabs[t] = fabs(dth);
+    s_theta = thn;
+  }
+  __syncthreads();
+  const double c = cos(b.gamma0 - s_theta), s = sin(b.gamma0 - s_theta);
+  for (int j = threadIdx.x + 1; j <= 2 * b.npts; j += blockDim.x) {
+    const int jj = j <= b.npts ? j : j - b.npts;
+    xb[b.off + j - 1] = b.xbf0 + (jj - 1) * b.deltas * c;
+    yb[b.off + j - 1] = b.ybf0 + (jj - 1) * b.deltas * s;
+  }
+}
+
+void body_dx(hipStream_t st, int nt, const TorsBody* tb, double* th, double dt, double delta, const double* f,
+             double* xb, double* yb, double* dtheta_abs) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_body_dx, dim3(nt), dim3(256), 0, st, tb, th, dt, delta, f, xb, yb, dtheta_abs);
+  IB_LAUNCH_CHECK();
+}
+
+// initial_acc (:233-294) with Ftint = ktheta*theta (var_update, :207-229)
+__global__ void __launch_bounds__(256) k_initial_acc(const TorsBody* __restrict__ tbs, double* __restrict__ th,
+                                                     double dt, double delta, const double* __restrict__ f) {
+  __shared__ double sh[256];
+  const int nt = gridDim.x, t = blockIdx.x;
+  const TorsBody b = tbs[t];
+  const double theta = th[t], thetad = th[nt + t];
+  const double si = sin(b.gamma0 - theta), ci = cos(b.gamma0 - theta);
+  double part = 0.0;
+  for (int j = threadIdx.x + 1; j <= 2 * b.npts; j += blockDim.x) part += BR(b, j, si, ci) * f[b.off + j - 1];
+  double brf = block_sum(part, sh);
+  if (threadIdx.x == 0) {
+    brf = brf * delta / b.deltas / dt;
+    const double ftint = b.K * theta;
+    th[2 * nt + t] = 1 / b.M * (-b.C * thetad - ftint + brf * b.deltas + 0.0 + 0.0);
+  }
+}
+
+void body_initial_acc(hipStream_t st, int nt, const TorsBody* tb, double* th, double dt, double delta,
+                      const double* f) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_initial_acc, dim3(nt), dim3(256), 0, st, tb, th, dt, delta, f);
+  IB_LAUNCH_CHECK();
+}
+
+}  // namespace ibg

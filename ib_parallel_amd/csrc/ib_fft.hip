@@ -1,0 +1,415 @@
+// ib_fft.hip -- unnormalised 2-D DST-I (FFTW RODFT00 x RODFT00) on CDNA4.
+//
+// Replaces the FFTW-MPI transforms of src/myfft.F90:168-213.  A 2-D DST is a
+// row (x) pass and a column (y) pass.  Each pass evaluates two real DST-I of
+// length N at once with one complex FFT of length L = 2(N+1): the pair
+// (x_a, x_b) is packed as z = x_a + i x_b and extended oddly
+// (z_0 = z_M = 0, z_{2M-t} = -z_t), then
+//     Z = -2i S(x_a) + 2 S(x_b)   =>   S(x_a) = -Im Z,  S(x_b) = Re Z.
+// The FFT is a mixed-radix Stockham autosort run entirely in LDS with two
+// ping-pong buffers per transform: radix 4/2/3/5 butterflies live in
+// registers, any larger prime factor uses one thread per output.  Fused
+// pre/post operations keep the ctci/ainv/predictor spectral products inside
+// the column pass (Y_SCALE2, Y_PRED) and apply_bc / the corrector subtraction
+// inside the row pass (X_APPLYBC, X_SUBFROM).
+#include <cmath>
+#include <vector>
+
+#include "ib_kernels.h"
+
+namespace ibg {
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+template <int R>
+__device__ __forceinline__ void dft(double2* v);
+
+template <>
+__device__ __forceinline__ void dft<2>(double2* v) {
+  double2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <>
+__device__ __forceinline__ void dft<3>(double2* v) {
+  const double s3 = 0.86602540378443864676;  // sin(2 pi / 3)
+  double2 t = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+  double2 mm = make_double2(v[0].x - 0.5 * t.x, v[0].y - 0.5 * t.y);
+  double2 jd = make_double2(s3 * d.y, -s3 * d.x);  // -i s3 d
+  v[0] = cadd(v[0], t);
+  v[1] = cadd(mm, jd);
+  v[2] = csub(mm, jd);
+}
+
+template <>
+__device__ __forceinline__ void dft<4>(double2* v) {
+  double2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+  double2 c = cadd(v[1], v[3]), d = csub(v[1], v[3]);
+  v[0] = cadd(a, c);
+  v[2] = csub(a, c);
+  v[1] = make_double2(b.x + d.y, b.y - d.x);  // b - i d
+  v[3] = make_double2(b.x - d.y, b.y + d.x);  // b + i d
+}
+
+template <>
+__device__ __forceinline__ void dft<5>(double2* v) {
+  const double c1 = 0.30901699437494742410;   // cos(2 pi/5)
+  const double c2 = -0.80901699437494742410;  // cos(4 pi/5)
+  const double s1 = 0.95105651629515357212;   // sin(2 pi/5)
+  const double s2 = 0.58778525229247312917;   // sin(4 pi/5)
+  double2 p1 = cadd(v[1], v[4]), m1 = csub(v[1], v[4]);
+  double2 p2 = cadd(v[2], v[3]), m2 = csub(v[2], v[3]);
+  double2 a1 = make_double2(v[0].x + c1 * p1.x + c2 * p2.x, v[0].y + c1 * p1.y + c2 * p2.y);
+  double2 a2 = make_double2(v[0].x + c2 * p1.x + c1 * p2.x, v[0].y + c2 * p1.y + c1 * p2.y);
+  double2 b1 = make_double2(s1 * m1.x + s2 * m2.x, s1 * m1.y + s2 * m2.y);
+  double2 b2 = make_double2(s2 * m1.x - s1 * m2.x, s2 * m1.y - s1 * m2.y);
+  v[0] = make_double2(v[0].x + p1.x + p2.x, v[0].y + p1.y + p2.y);
+  v[1] = make_double2(a1.x + b1.y, a1.y - b1.x);  // a1 - i b1
+  v[4] = make_double2(a1.x - b1.y, a1.y + b1.x);  // a1 + i b1
+  v[2] = make_double2(a2.x + b2.y, a2.y - b2.x);  // a2 - i b2
+  v[3] = make_double2(a2.x - b2.y, a2.y + b2.x);  // a2 + i b2
+}
+
+// One Stockham stage of radix R over nf transforms of length L stored at
+// in[s*L + t]; Ns = product of the radices already applied.
+template <int R>
+__device__ __forceinline__ void stage_fixed(const double2* __restrict__ in, double2* __restrict__ out, int nf,
+                                            int L, int Ns, const double2* __restrict__ W) {
+  const int LR = L / R;
+  const int tws = LR / Ns;
+  const int total = nf * LR;
+  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    const int s = idx / LR;
+    const int j = idx - s * LR;
+    const double2* ip = in + s * L;
+    double2* op = out + s * L;
+    const int k = j % Ns;
+    double2 v[R];
+    v[0] = ip[j];
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(ip[j + r * LR], W[k * r * tws]);
+    dft<R>(v);
+    const int base = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) op[base + r * Ns] = v[r];
+  }
+}
+
+__device__ void stage_generic(const double2* __restrict__ in, double2* __restrict__ out, int nf, int L, int Ns,
+                              int R, const double2* __restrict__ W) {
+  const int LR = L / R;
+  const int tws = LR / Ns;
+  const int total = nf * L;
+  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    const int s = idx / L;
+    const int rem = idx - s * L;
+    const int q = rem / LR;
+    const int j = rem - q * LR;
+    const int k = j % Ns;
+    const double2* ip = in + s * L;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int r = 0; r < R; ++r) {
+      int tw = k * r * tws + ((r * q) % R) * LR;
+      if (tw >= L) tw -= L;
+      acc = cadd(acc, cmul(ip[j + r * LR], W[tw]));
+    }
+    out[s * L + (j - k) * R + k + q * Ns] = acc;
+  }
+}
+
+// Runs the plan on nf transforms held in a (scratch b); returns the buffer
+// that holds the result.  Caller synchronises before the call.
+__device__ double2* fft_run(double2* a, double2* b, int nf, const FFTPlan& pl) {
+  int Ns = 1;
+  for (int st = 0; st < pl.nst; ++st) {
+    const int R = pl.rad[st];
+    switch (R) {
+      case 4: stage_fixed<4>(a, b, nf, pl.L, Ns, pl.W); break;
+      case 2: stage_fixed<2>(a, b, nf, pl.L, Ns, pl.W); break;
+      case 3: stage_fixed<3>(a, b, nf, pl.L, Ns, pl.W); break;
+      case 5: stage_fixed<5>(a, b, nf, pl.L, Ns, pl.W); break;
+      default: stage_generic(a, b, nf, pl.L, Ns, R, pl.W); break;
+    }
+    __syncthreads();
+    double2* t = a;
+    a = b;
+    b = t;
+    Ns *= R;
+  }
+  return a;
+}
+
+struct XArgs {
+  const double* in[kMaxBatch];
+  double* out[kMaxBatch];
+  const double* bc;
+  Geom g;
+  int rp;
+};
+
+struct YArgs {
+  const double* in[kMaxBatch];
+  const double* in2[kMaxBatch];
+  double* out[kMaxBatch];
+  const double* tab;
+  const double* tab2;
+  double div;
+  Geom g;
+  int cp;
+};
+
+// Row pass.  Rows j0 = 0..n-2 (node j = j0+2) of length N = m-1.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_dst_x(XArgs a, FFTPlan pl) {
+  extern __shared__ double2 sm[];
+  const int L = pl.L, M = pl.M, N = pl.N, rp = a.rp;
+  const int nrows = a.g.n - 1;
+  double2* A = sm;
+  double2* B = sm + rp * L;
+  const double* __restrict__ in = a.in[blockIdx.y];
+  double* out = a.out[blockIdx.y];
+  const int pair0 = blockIdx.x * rp;
+  for (int idx = threadIdx.x; idx < rp * N; idx += blockDim.x) {
+    const int s = idx / N;
+    const int t = idx - s * N;
+    const int ra = 2 * (pair0 + s), rb = ra + 1;
+    double xa = 0.0, xb = 0.0;
+    if (ra < nrows) xa = in[(long)ra * N + t];
+    if (rb < nrows) xb = in[(long)rb * N + t];
+    if (MODE == X_APPLYBC) {
+      // apply_bc(vort, sbc, 1.d0), src/operators_fluid.F90:539-549
+      const int i = t + 2;
+      const Geom& g = a.g;
+      const double* bc = a.bc;
+      if (ra < nrows) {
+        const int j = ra + 2;
+        if (i == 2) xa = xa + bc[g.left + j - 1];
+        if (i == g.m) xa = xa + bc[g.right + j - 1];
+        if (j == 2) xa = xa + bc[g.bottom + i - 1];
+        if (j == g.n) xa = xa + bc[g.top + i - 1];
+      }
+      if (rb < nrows) {
+        const int j = rb + 2;
+        if (i == 2) xb = xb + bc[g.left + j - 1];
+        if (i == g.m) xb = xb + bc[g.right + j - 1];
+        if (j == 2) xb = xb + bc[g.bottom + i - 1];
+        if (j == g.n) xb = xb + bc[g.top + i - 1];
+      }
+    }
+    A[s * L + t + 1] = make_double2(xa, xb);
+    A[s * L + L - (t + 1)] = make_double2(-xa, -xb);
+  }
+  for (int s = threadIdx.x; s < rp; s += blockDim.x) {
+    A[s * L] = make_double2(0.0, 0.0);
+    A[s * L + M] = make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  const double2* Z = fft_run(A, B, rp, pl);
+  for (int idx = threadIdx.x; idx < rp * N; idx += blockDim.x) {
+    const int s = idx / N;
+    const int t = idx - s * N;
+    const int ra = 2 * (pair0 + s), rb = ra + 1;
+    const double2 z = Z[s * L + t + 1];
+    if (MODE == X_SUBFROM) {
+      if (ra < nrows) out[(long)ra * N + t] = out[(long)ra * N + t] - (-z.y);
+      if (rb < nrows) out[(long)rb * N + t] = out[(long)rb * N + t] - z.x;
+    } else {
+      if (ra < nrows) out[(long)ra * N + t] = -z.y;
+      if (rb < nrows) out[(long)rb * N + t] = z.x;
+    }
+  }
+}
+
+// Column pass.  Columns i0 = 0..m-2 (stride m-1) of length N = n-1.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_dst_y(YArgs a, FFTPlan pl) {
+  extern __shared__ double2 sm[];
+  const int L = pl.L, M = pl.M, N = pl.N, cp = a.cp;
+  const int ncols = a.g.m - 1;
+  const int nfft = (MODE == Y_PRED) ? 2 * cp : cp;
+  double2* A = sm;
+  double2* B = sm + nfft * L;
+  const double* __restrict__ in = a.in[blockIdx.y];
+  const double* __restrict__ in2 = a.in2[blockIdx.y];
+  double* __restrict__ out = a.out[blockIdx.y];
+  const int col0 = 2 * blockIdx.x * cp;
+  const int w2 = 2 * cp;  // columns handled by this workgroup
+  for (int idx = threadIdx.x; idx < N * w2; idx += blockDim.x) {
+    const int t = idx / w2;
+    const int c = idx - t * w2;
+    const int col = col0 + c;
+    const int s = c >> 1;
+    double v = 0.0, v2 = 0.0;
+    if (col < ncols) {
+      v = in[(long)t * ncols + col];
+      if (MODE == Y_PRED) v2 = in2[(long)t * ncols + col];
+    }
+    double* pa = reinterpret_cast<double*>(&A[s * L + t + 1]);
+    double* pb = reinterpret_cast<double*>(&A[s * L + L - (t + 1)]);
+    pa[c & 1] = v;
+    pb[c & 1] = -v;
+    if (MODE == Y_PRED) {
+      double* qa = reinterpret_cast<double*>(&A[(s + cp) * L + t + 1]);
+      double* qb = reinterpret_cast<double*>(&A[(s + cp) * L + L - (t + 1)]);
+      qa[c & 1] = v2;
+      qb[c & 1] = -v2;
+    }
+  }
+  for (int s = threadIdx.x; s < nfft; s += blockDim.x) {
+    A[s * L] = make_double2(0.0, 0.0);
+    A[s * L + M] = make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  double2* Z = fft_run(A, B, nfft, pl);
+  if (MODE == Y_SCALE2 || MODE == Y_PRED) {
+    double2* O = (Z == A) ? B : A;
+    (void)O;
+    for (int idx = threadIdx.x; idx < cp * N; idx += blockDim.x) {
+      const int s = idx / N;
+      const int t = idx - s * N;  // frequency index t+1
+      const int ca = col0 + 2 * s, cb = ca + 1;
+      const double2 za = Z[s * L + t + 1];
+      double ya = -za.y, yb = za.x;
+      if (MODE == Y_SCALE2) {
+        ya = (ca < ncols) ? a.tab[(long)t * ncols + ca] * ya / a.div : 0.0;
+        yb = (cb < ncols) ? a.tab[(long)t * ncols + cb] * yb / a.div : 0.0;
+      } else {
+        const double2 zb = Z[(s + cp) * L + t + 1];
+        const double ba = -zb.y, bb = zb.x;
+        // lam1i * (dst(rhs terms) + lam1 * dst(omega)), src/operators_fluid.F90:657-661
+        ya = (ca < ncols) ? a.tab2[(long)t * ncols + ca] * (ya + a.tab[(long)t * ncols + ca] * ba) : 0.0;
+        yb = (cb < ncols) ? a.tab2[(long)t * ncols + cb] * (yb + a.tab[(long)t * ncols + cb] * bb) : 0.0;
+      }
+      Z[s * L + t + 1] = make_double2(ya, yb);
+      Z[s * L + L - (t + 1)] = make_double2(-ya, -yb);
+    }
+    for (int s = threadIdx.x; s < cp; s += blockDim.x) {
+      Z[s * L] = make_double2(0.0, 0.0);
+      Z[s * L + M] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    double2* other = (Z == A) ? B : A;
+    Z = fft_run(Z, other, cp, pl);
+  }
+  for (int idx = threadIdx.x; idx < N * w2; idx += blockDim.x) {
+    const int t = idx / w2;
+    const int c = idx - t * w2;
+    const int col = col0 + c;
+    const int s = c >> 1;
+    if (col < ncols) {
+      const double2 z = Z[s * L + t + 1];
+      out[(long)t * ncols + col] = (c & 1) ? z.x : -z.y;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------
+
+static std::vector<int> factorize(int L) {
+  std::vector<int> r;
+  const int small[] = {4, 2, 3, 5};
+  for (int q : small)
+    while (L % q == 0) {
+      r.push_back(q);
+      L /= q;
+    }
+  for (int p = 7; L > 1; p += 2)
+    while (L % p == 0) {
+      r.push_back(p);
+      L /= p;
+    }
+  return r;
+}
+
+void HostFFTPlan::init(int N) {
+  dev.N = N;
+  dev.M = N + 1;
+  dev.L = 2 * (N + 1);
+  std::vector<int> rad = factorize(dev.L);
+  if ((int)rad.size() > kMaxStages) throw HipError("FFT length has too many factors");
+  dev.nst = (int)rad.size();
+  for (int i = 0; i < dev.nst; ++i) dev.rad[i] = rad[i];
+  std::vector<double2> w(dev.L);
+  for (int t = 0; t < dev.L; ++t) {
+    long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)dev.L;
+    w[t] = make_double2((double)cosl(ang), (double)-sinl(ang));
+  }
+  IB_HIP(hipMalloc(&W, sizeof(double2) * dev.L));
+  IB_HIP(hipMemcpy(W, w.data(), sizeof(double2) * dev.L, hipMemcpyHostToDevice));
+  dev.W = W;
+}
+
+void HostFFTPlan::release() {
+  if (W) (void)hipFree(W);
+  W = nullptr;
+}
+
+size_t dst_lds_bytes(const FFTPlan& pl, int nfft) { return (size_t)2 * nfft * pl.L * sizeof(double2); }
+
+void dst_set_max_lds(size_t bytes) {
+  const void* fns[] = {(const void*)k_dst_x<X_PLAIN>, (const void*)k_dst_x<X_APPLYBC>,
+                       (const void*)k_dst_x<X_SUBFROM>, (const void*)k_dst_y<Y_PLAIN>,
+                       (const void*)k_dst_y<Y_SCALE2>, (const void*)k_dst_y<Y_PRED>};
+  for (const void* f : fns) IB_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+void dst_x(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, const Geom& g, int rp,
+           size_t max_lds) {
+  XArgs a{};
+  for (int b = 0; b < l.batch; ++b) {
+    a.in[b] = l.in[b];
+    a.out[b] = l.out[b];
+  }
+  a.bc = l.bc;
+  a.g = g;
+  while (rp > 1 && dst_lds_bytes(pl, rp) > max_lds) rp /= 2;
+  a.rp = rp;
+  const int npairs = ceil_div(g.n - 1, 2);
+  dim3 grid(ceil_div(npairs, rp), l.batch);
+  const size_t lds = dst_lds_bytes(pl, rp);
+  if (lds > max_lds) throw HipError("DST row pass needs more LDS than the device allows");
+  switch (mode) {
+    case X_PLAIN: hipLaunchKernelGGL(k_dst_x<X_PLAIN>, grid, dim3(256), lds, st, a, pl); break;
+    case X_APPLYBC: hipLaunchKernelGGL(k_dst_x<X_APPLYBC>, grid, dim3(256), lds, st, a, pl); break;
+    case X_SUBFROM: hipLaunchKernelGGL(k_dst_x<X_SUBFROM>, grid, dim3(256), lds, st, a, pl); break;
+    default: throw HipError("bad DST row mode");
+  }
+  IB_LAUNCH_CHECK();
+}
+
+void dst_y(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, const Geom& g, int cp,
+           size_t max_lds) {
+  YArgs a{};
+  for (int b = 0; b < l.batch; ++b) {
+    a.in[b] = l.in[b];
+    a.in2[b] = l.in2[b];
+    a.out[b] = l.out[b];
+  }
+  a.tab = l.tab;
+  a.tab2 = l.tab2;
+  a.div = l.div;
+  a.g = g;
+  const int mult = (mode == Y_PRED) ? 2 : 1;
+  while (cp > 1 && dst_lds_bytes(pl, mult * cp) > max_lds) cp /= 2;
+  a.cp = cp;
+  const int npairs = ceil_div(g.m - 1, 2);
+  dim3 grid(ceil_div(npairs, cp), l.batch);
+  const size_t lds = dst_lds_bytes(pl, mult * cp);
+  if (lds > max_lds) throw HipError("DST column pass needs more LDS than the device allows");
+  switch (mode) {
+    case Y_PLAIN: hipLaunchKernelGGL(k_dst_y<Y_PLAIN>, grid, dim3(256), lds, st, a, pl); break;
+    case Y_SCALE2: hipLaunchKernelGGL(k_dst_y<Y_SCALE2>, grid, dim3(256), lds, st, a, pl); break;
+    case Y_PRED: hipLaunchKernelGGL(k_dst_y<Y_PRED>, grid, dim3(256), lds, st, a, pl); break;
+    default: throw HipError("bad DST column mode");
+  }
+  IB_LAUNCH_CHECK();
+}
+
+}  // namespace ibg

@@ -1,0 +1,126 @@
+// ib_kernels.h -- host-side launchers of the HIP kernels (one translation
+// unit per kernel family; all launches go to the caller's stream).
+#pragma once
+
+#include <vector>
+
+#include "ib_common.h"
+
+namespace ibg {
+
+// ---------------------------------------------------------------- DST-I ---
+// Row (x) pass modes
+enum { X_PLAIN = 0, X_APPLYBC = 1, X_SUBFROM = 2 };
+// Column (y) pass modes
+enum { Y_PLAIN = 0, Y_SCALE2 = 1, Y_PRED = 2 };
+
+constexpr int kMaxBatch = 4;
+
+struct DstLaunch {
+  int batch = 1;
+  const double* in[kMaxBatch] = {};
+  const double* in2[kMaxBatch] = {};  // Y_PRED: second input b
+  double* out[kMaxBatch] = {};
+  const double* bc = nullptr;         // X_APPLYBC boundary vector
+  const double* tab = nullptr;        // Y_SCALE2 table / Y_PRED lam1
+  const double* tab2 = nullptr;       // Y_PRED lam1i
+  double div = 1.0;                   // Y_SCALE2 divisor
+};
+
+// Builds the Stockham plan (radix list) and twiddle table for L = 2(N+1).
+struct HostFFTPlan {
+  FFTPlan dev{};
+  double2* W = nullptr;
+  void init(int N);
+  void release();
+};
+
+// x pass over rows of length m-1 (n-1 rows); rp = row pairs per workgroup
+void dst_x(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& a, const Geom& g, int rp,
+           size_t max_lds);
+// y pass over columns of length n-1 (m-1 columns); cp = column pairs per workgroup
+void dst_y(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& a, const Geom& g, int cp,
+           size_t max_lds);
+// LDS bytes a launch needs
+size_t dst_lds_bytes(const FFTPlan& pl, int nfft);
+// Raises the dynamic-LDS limit of the DST kernels to `bytes` (call once).
+void dst_set_max_lds(size_t bytes);
+
+// ------------------------------------------------------------- stencils ---
+void fill(hipStream_t st, double* p, long n, double v);
+void copy(hipStream_t st, double* dst, const double* src, long n);
+// rot (C^T): vort = hy(i,j)-hy(i-1,j)-hx(i,j)+hx(i,j-1)
+void rot(hipStream_t st, const Geom& g, const double* hx, const double* hy, double* vort);
+// coarsify fine level -> centre of coarse level
+void coarsify(hipStream_t st, const Geom& g, const double* fine, double* coarse);
+// get_bc of `nlev` coarse levels at once: bc[l] = fac * get_bc(src[l])
+void get_bc(hipStream_t st, const Geom& g, const double* src, long src_stride, double* bc, long bc_stride,
+            int nlev, double fac);
+// curl (C) of s with boundary sbc into qx,qy; if bc_out: bc_out = get_bc(s, 1)
+void curl_getbc(hipStream_t st, const Geom& g, const double* s, const double* sbc, double* qx, double* qy,
+                double* bc_out);
+// nonlinear + rot + apply_bc + AB2/Euler combine for the predictor of one level.
+// r = con1*rhs + con2*rold + rhsbc; rhs_old <- rhs.  raw_rhs (optional) gets rhs.
+void nl_rhs(hipStream_t st, const Geom& g, const double* omega, const double* qx, const double* qy,
+            double q0x, const double* lastbc, const double* omegabc, double vfac, double con1, double con2,
+            int first_step, double* rhs_old, double* r, double* raw_rhs);
+void sub_inplace(hipStream_t st, double* a, const double* b, long n);
+
+// ---------------------------------------------------- immersed boundary ---
+// weights of setup_reg / deltafnc for every (entry, tap); face key per tap:
+// x-face index for x entries, Fx + y-face index for y entries; err flag set on
+// a stencil leaving the grid (key then clamped to 0, weight 0).
+void setup_reg(hipStream_t st, const Geom& g, int nf, const double* xb, const double* yb, const int* comp,
+               double delta, double offx, double offy, int f32lit, double* w, int* key, int* err);
+// sort taps by key (stable) and run-length encode: uniq keys, offsets, count
+struct RegCSR {
+  int ntaps = 0;
+  int *key = nullptr, *key_sorted = nullptr, *tap = nullptr, *tap_sorted = nullptr;
+  int *uniq = nullptr, *cnt = nullptr, *off = nullptr, *nruns = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  int key_bits = 32;
+  void alloc(int ntaps, int max_key);
+  void release();
+  void build(hipStream_t st);
+};
+// reg (E^T): hx/hy[face] = sum over taps of w*f[tap/49] in tap order; faces not
+// touched are left as they are (callers zero them).
+void reg(hipStream_t st, const Geom& g, const RegCSR& c, const double* w, const double* f, double* hx,
+         double* hy);
+// regT (E): out[k] = base[k] + sum_t w[k,t]*(field[key] + shift) in tap order;
+// shift_x is added to x-face values (q0 of level 1), base may be null (0).
+void regT(hipStream_t st, const Geom& g, int nf, const double* w, const int* key, const double* hx,
+          const double* hy, double shift_x, const double* base, double* out);
+// redistribute (src/operators_struct.F90:735-834) in CSR form: writes
+// finter = sum(w f)/sum(w) (if sum(w) > 1e-10, else 0) into touched faces.
+void redist_faces(hipStream_t st, const Geom& g, const RegCSR& c, const double* w, const double* f,
+                  double* fx_out, double* fy_out);
+
+// ---------------------------------------------------- torsional bodies ---
+// y[body entries] += b_times2(x); one workgroup per body
+void body_btimes(hipStream_t st, int nt, const TorsBody* tb, const double* theta, double dt, double delta,
+                 const double* x, double* y);
+// rhsf = compute_rhs (torsional entries, 0 elsewhere) + E qh
+void fsi_rhs(hipStream_t st, const Geom& g, int nf, const int* tors_of_entry, const TorsBody* tb, int nt,
+             const double* th, double dt, double delta, const double* w, const int* key, const double* qx,
+             const double* qy, double q0x, double* rhsf);
+// compute_dx + update_xb; th = [theta, thetad, thetadd, theta0, thetad0, thetadd0]
+void body_dx(hipStream_t st, int nt, const TorsBody* tb, double* th, double dt, double delta, const double* f,
+             double* xb, double* yb, double* dtheta_abs);
+void body_initial_acc(hipStream_t st, int nt, const TorsBody* tb, double* th, double dt, double delta,
+                      const double* f);
+
+// ------------------------------------------------------------------ GMRES ---
+struct GmresDev {
+  double ttol, rn0, bnorm, res;
+  int its, it, reason, first;
+};
+void gm_start(hipStream_t st, int nf, int kmax, const double* b, const double* Ax, double* V, GmresDev* s,
+              double* H, double* grs, double rtol, double atol);
+void gm_arnoldi(hipStream_t st, int nf, int kmax, double* V, GmresDev* s, double* H, double* cc, double* ss,
+                double* grs);
+void gm_buildsol(hipStream_t st, int nf, int kmax, const double* V, GmresDev* s, const double* H,
+                 const double* grs, double* x, double* scratch);
+
+}  // namespace ibg

@@ -1,0 +1,773 @@
+// ib_solver.hip -- device-resident advance(itime) and the C ABI (include/ib_gpu.h).
+//
+// The step mirrors src/timestepper.F90:18-556 for num_stat = F, sub_domain = F:
+//   1. predictor on every level, coarse to fine (:100-236): fused
+//      nonlinear+rot+apply_bc+AB2 kernel, then x pass (2 arrays), fused
+//      y-DST / lam1 / lam1i / y-DST column pass, x pass;
+//   2. vort2flux (:239-245) and the FSI fixed point (:249-456) with
+//      device-resident GMRES on the Schur operator a_times2 + b_times2;
+//   3. corrector (:488-553): reg, rot, ainv with the subtraction fused into its
+//      last row pass, vort2flux.
+// All state lives in HBM for the handle's lifetime; the host reads back only
+// GMRES/FSI status words.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ib_gpu.h"
+#include "ib_kernels.h"
+
+using namespace ibg;
+
+namespace {
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t n_) {
+    release();
+    n = n_;
+    IB_HIP(hipMalloc(&p, sizeof(T) * (n > 0 ? n : 1)));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { release(); }
+  operator T*() const { return p; }
+};
+
+struct KTimer {
+  bool on = false;
+  int64_t count = 0;
+  double total_ms = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, spare;
+  hipEvent_t start_ev = nullptr;
+  void begin(hipStream_t st) {
+    if (!on) return;
+    std::pair<hipEvent_t, hipEvent_t> e;
+    if (!spare.empty()) {
+      e = spare.back();
+      spare.pop_back();
+    } else {
+      IB_HIP(hipEventCreate(&e.first));
+      IB_HIP(hipEventCreate(&e.second));
+    }
+    IB_HIP(hipEventRecord(e.first, st));
+    pending.push_back(e);
+  }
+  void end(hipStream_t st) {
+    if (!on) return;
+    IB_HIP(hipEventRecord(pending.back().second, st));
+  }
+  void harvest() {
+    for (auto& e : pending) {
+      float ms = 0;
+      IB_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+      total_ms += ms;
+      ++count;
+      spare.push_back(e);
+    }
+    pending.clear();
+  }
+  void destroy() {
+    for (auto& e : pending) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    for (auto& e : spare) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    pending.clear();
+    spare.clear();
+  }
+};
+
+std::string g_create_err;
+
+}  // namespace
+
+struct ib_handle {
+  ib_params prm{};
+  std::string err;
+  int dev = 0;
+  hipStream_t st = nullptr;
+  size_t max_lds = 65536;
+  Geom g{};
+  int L = 0;
+  double delta = 0, dt = 0, Re = 0, offx = 0, offy = 0, normalize = 0;
+  HostFFTPlan px, py;
+  DBuf<double> laminv, lam1, lam1i;
+  std::vector<double> vfac, con1, con2, q0x;
+  DBuf<double> omega, s, qx, qy, rhs_old;
+  DBuf<double> t0, t1, t2;
+  DBuf<double> vv, sn, vx, vy, hx, hy;
+  DBuf<double> lastbc, obc, sbcA, sbcB, zbc;
+  // bodies
+  int nb = 0, nf = 0, nbodies = 0, ntors = 0;
+  std::vector<int> h_cbody, h_ccomp, h_tors_of;
+  std::vector<TorsBody> h_tb;
+  DBuf<double> xb, yb, f, f_rdst, w, rhsf, gw;
+  DBuf<int> comp, tors_of, err_flag;
+  RegCSR csr;
+  DBuf<TorsBody> tb;
+  DBuf<double> th, dabs;
+  // GMRES
+  int kmax = 30;
+  DBuf<double> V, H, cc, ss, grs;
+  GmresDev* gst = nullptr;
+  GmresDev* hst = nullptr;  // pinned
+  double* hbuf = nullptr;   // pinned readback
+  int* hflag = nullptr;     // pinned
+  int itime = 0, ksp_iter = 0, fsi_iter = 0;
+  double err_fsi = 0;
+  KTimer tim[2];
+
+  // ------------------------------------------------------------------ util
+  void sync() {
+    IB_HIP(hipStreamSynchronize(st));
+    tim[0].harvest();
+    tim[1].harvest();
+  }
+
+  void xpass(int mode, const DstLaunch& a) {
+    tim[0].begin(st);
+    dst_x(st, px.dev, mode, a, g, (px.dev.L <= 512 ? 2 : 1), max_lds);
+    tim[0].end(st);
+  }
+  void ypass(int mode, const DstLaunch& a) {
+    tim[1].begin(st);
+    dst_y(st, py.dev, mode, a, g, 4, max_lds);
+    tim[1].end(st);
+  }
+
+  // s = ctci(w [+ apply_bc(bc)]): src/myfft.F90:177-188
+  void ctci_dev(const double* win, double* out, const double* bc) {
+    DstLaunch a;
+    a.in[0] = win; a.out[0] = t0; a.bc = bc;
+    xpass(bc ? X_APPLYBC : X_PLAIN, a);
+    DstLaunch b;
+    b.in[0] = t0; b.out[0] = t1; b.tab = laminv; b.div = 1.0;
+    ypass(Y_SCALE2, b);
+    DstLaunch c;
+    c.in[0] = t1; c.out[0] = out;
+    xpass(X_PLAIN, c);
+  }
+
+  // ainv: src/myfft.F90:161-173; with subfrom, target -= ainv(v) (corrector)
+  void ainv_dev(const double* v, double* out, bool subfrom) {
+    DstLaunch a;
+    a.in[0] = v; a.out[0] = t0;
+    xpass(X_PLAIN, a);
+    DstLaunch b;
+    b.in[0] = t0; b.out[0] = t1; b.tab = lam1i; b.div = normalize;
+    ypass(Y_SCALE2, b);
+    DstLaunch c;
+    c.in[0] = t1; c.out[0] = out;
+    xpass(subfrom ? X_SUBFROM : X_PLAIN, c);
+  }
+
+  // vort2flux: src/operators_fluid.F90:237-377
+  void vort2flux_dev(double* wv, double* sv, double* qxv, double* qyv) {
+    const long P = g.P;
+    for (int k = 1; k < L; ++k) coarsify(st, g, wv + (k - 1) * P, wv + k * P);
+    const double* sbc = zbc;
+    for (int k = L - 1; k >= 0; --k) {
+      ctci_dev(wv + k * P, sv + k * P, k == L - 1 ? nullptr : sbc);
+      double* bout = nullptr;
+      if (k > 0) bout = (sbc == sbcA) ? sbcB.p : sbcA.p;
+      curl_getbc(st, g, sv + k * P, sbc, qxv + (long)k * g.Fx, qyv + (long)k * g.Fy, bout);
+      sbc = bout;
+    }
+  }
+
+  // ab_times = a_times2 + b_times2: src/operators_fluid.F90:14-120
+  void schur_dev(const double* x, double* y) {
+    fill(st, hx, g.Fx, 0.0);
+    fill(st, hy, g.Fy, 0.0);
+    reg(st, g, csr, w, x, hx, hy);
+    rot(st, g, hx, hy, vv);
+    if (L > 1) fill(st, vv + g.P, (long)(L - 1) * g.P, 0.0);
+    ainv_dev(vv, vv, false);
+    vort2flux_dev(vv, sn, vx, vy);
+    regT(st, g, nf, w, csr.key, vx, vy, 0.0, nullptr, y);
+    body_btimes(st, ntors, tb, th, dt, delta, x, y);
+  }
+
+  // setup_reg + the sorted tap list; returns false if a stencil left the grid
+  void rebuild_weights() {
+    setup_reg(st, g, nf, xb, yb, comp, delta, offx, offy, prm.deltafnc_f32_literals, w, csr.key, err_flag);
+    csr.build(st);
+  }
+
+  int read_err_flag() {
+    IB_HIP(hipMemcpyAsync(hflag, err_flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    sync();
+    return *hflag;
+  }
+
+  void read_gmres() {
+    IB_HIP(hipMemcpyAsync(hst, gst, sizeof(GmresDev), hipMemcpyDeviceToHost, st));
+    sync();
+  }
+
+  // PETSc-style GMRES(kmax) on the Schur operator, x = initial guess / result
+  int gmres_dev(const double* b, double* x, int* its_out) {
+    GmresDev init{};
+    init.first = 1;
+    *hst = init;
+    IB_HIP(hipMemcpyAsync(gst, hst, sizeof(GmresDev), hipMemcpyHostToDevice, st));
+    int reason = 0, its = 0;
+    const int maxit = prm.gmres_max_it;
+    while (true) {
+      schur_dev(x, gw);
+      gm_start(st, nf, kmax, b, gw, V, gst, H, grs, prm.rtol, prm.atol);
+      read_gmres();
+      reason = hst->reason;
+      if (reason) break;
+      int it = 0;
+      while (true) {
+        schur_dev(V + (size_t)it * nf, V + (size_t)(it + 1) * nf);
+        gm_arnoldi(st, nf, kmax, V, gst, H, cc, ss, grs);
+        read_gmres();
+        reason = hst->reason;
+        its = hst->its;
+        ++it;
+        if (reason || it >= kmax || its >= maxit) break;
+      }
+      gm_buildsol(st, nf, kmax, V, gst, H, grs, x, nullptr);
+      if (reason) break;
+      if (its >= maxit) {
+        reason = -2;
+        break;
+      }
+    }
+    *its_out = its;
+    return reason > 0 ? IB_OK : IB_E_NOCONV_GMRES;
+  }
+
+  // ---------------------------------------------------------------- advance
+  int advance() {
+    itime += 1;
+    const long P = g.P, nbc = g.nbc;
+    // step 1: lastbc(k) = 0.25 get_bc(omega_{k+1}^n) (src/timestepper.F90:103-117)
+    if (L > 1) get_bc(st, g, omega + P, P, lastbc, nbc, L - 1, 0.25);
+    fill(st, lastbc + (L - 1) * nbc, nbc, 0.0);
+    for (int k = L - 1; k >= 0; --k) {
+      const double* ob = zbc;
+      if (k < L - 1) {
+        get_bc(st, g, omega + (k + 1) * P, P, obc, nbc, 1, 0.25);
+        ob = obc;
+      }
+      nl_rhs(st, g, omega + k * P, qx + (long)k * g.Fx, qy + (long)k * g.Fy, q0x[k], lastbc + k * nbc, ob, vfac[k],
+             con1[k], con2[k], itime == 1, rhs_old + k * P, t2, nullptr);
+      DstLaunch a;
+      a.batch = 2;
+      a.in[0] = t2; a.out[0] = t0;
+      a.in[1] = omega + k * P; a.out[1] = t1;
+      xpass(X_PLAIN, a);
+      DstLaunch b;
+      b.in[0] = t0; b.in2[0] = t1; b.out[0] = t2; b.tab = lam1 + k * P; b.tab2 = lam1i + k * P;
+      ypass(Y_PRED, b);
+      DstLaunch c;
+      c.in[0] = t2; c.out[0] = omega + k * P;
+      xpass(X_PLAIN, c);
+    }
+    vort2flux_dev(omega, s, qx, qy);
+
+    // step 2: FSI fixed point (src/timestepper.F90:249-456)
+    int status = IB_OK;
+    int fsi_count = 0;
+    double err_glob = 10.0;
+    ksp_iter = 0;
+    if (ntors > 0) {
+      if (itime == 1) body_initial_acc(st, ntors, tb, th, dt, delta, f);
+      IB_HIP(hipMemcpyAsync(th + 3 * ntors, th.p, sizeof(double) * 3 * ntors, hipMemcpyDeviceToDevice, st));
+    }
+    while (err_glob >= prm.fsi_tol && fsi_count <= prm.fsi_max_count) {
+      ++fsi_count;
+      fsi_rhs(st, g, nf, tors_of, tb, ntors, th, dt, delta, w, csr.key, qx, qy, q0x[0], rhsf);
+      int its = 0;
+      int rc = gmres_dev(rhsf, f, &its);
+      ksp_iter += its;
+      if (rc != IB_OK) status = rc;
+      // redistribute (output only)
+      redist_faces(st, g, csr, w, f, hx, hy);
+      regT(st, g, nf, w, csr.key, hx, hy, 0.0, nullptr, f_rdst);
+      if (ntors > 0) {
+        body_dx(st, ntors, tb, th, dt, delta, f, xb, yb, dabs);
+        rebuild_weights();
+        IB_HIP(hipMemcpyAsync(hbuf, dabs.p, sizeof(double) * ntors, hipMemcpyDeviceToHost, st));
+        if (read_err_flag()) {
+          err = "body point regularisation stencil leaves the grid";
+          return IB_E_INVAL;
+        }
+        err_glob = 0.0;
+        for (int t = 0; t < ntors; ++t) err_glob = std::max(err_glob, hbuf[t]);
+      } else {
+        err_glob = 0.0;  // rigid bodies only: positions and weights unchanged
+      }
+    }
+    if (err_glob >= prm.fsi_tol && status == IB_OK) status = IB_E_NOCONV_FSI;
+    fsi_iter = fsi_count;
+    err_fsi = err_glob;
+
+    // step 3: corrector (src/timestepper.F90:488-553)
+    fill(st, hx, g.Fx, 0.0);
+    fill(st, hy, g.Fy, 0.0);
+    reg(st, g, csr, w, f, hx, hy);
+    rot(st, g, hx, hy, vv);
+    ainv_dev(vv, omega, true);
+    vort2flux_dev(omega, s, qx, qy);
+    sync();
+    return status;
+  }
+
+  // ------------------------------------------------------------------ setup
+  int fail(int code, const std::string& m) {
+    err = m;
+    return code;
+  }
+
+  int init(const ib_params* p, int nbod, const ib_body* bodies) {
+    prm = *p;
+    if (prm.gmres_restart <= 0) prm.gmres_restart = 30;
+    if (prm.gmres_max_it <= 0) prm.gmres_max_it = 10000;
+    if (prm.fsi_max_count <= 0) prm.fsi_max_count = 1000;
+    const int m = prm.m, n = prm.n;
+    L = prm.mgridlev;
+    if (m < 8 || n < 8 || m % 4 || n % 4) return fail(IB_E_INVAL, "grid must be divisible by 4 (m, n >= 8)");
+    if (L < 1) return fail(IB_E_INVAL, "mgridlev must be >= 1");
+    if (prm.num_stat || prm.sub_domain) return fail(IB_E_UNSUPPORTED, "num_stat and sub_domain paths are not built");
+    if (prm.istart != 0) return fail(IB_E_UNSUPPORTED, "restart from .var is not built");
+    if (prm.gmres_restart > 250) return fail(IB_E_INVAL, "gmres_restart must be <= 250");
+    kmax = prm.gmres_restart;
+    dt = prm.dt; Re = prm.re; offx = prm.offsetx; offy = prm.offsety;
+    delta = prm.len / (double)m;
+    g.m = m; g.n = n; g.L = L;
+    g.P = (m - 1) * (n - 1); g.Fx = (m + 1) * n; g.Fy = m * (n + 1);
+    g.nbc = 2 * (m + 1) + 2 * (n + 1);
+    g.left = 0; g.right = n + 1; g.bottom = 2 * (n + 1); g.top = 2 * (n + 1) + m + 1;
+
+    // bodies (src/grid.F90:239-433)
+    if (nbod < 0) return fail(IB_E_INVAL, "negative body count");
+    bool seen_tors = false;
+    std::vector<int> nbb(1, 0);
+    std::vector<double> hxb, hyb;
+    for (int b = 0; b < nbod; ++b) {
+      const ib_body& B = bodies[b];
+      if (B.npts < 2) return fail(IB_E_INVAL, "a body needs at least 2 points");
+      if (B.fam != 1 && B.fam != 2)
+        return fail(IB_E_UNSUPPORTED, "only body families 1 (rigid) and 2 (torsional) exist in the reference");
+      if (B.fam == 1 && seen_tors) return fail(IB_E_INVAL, "rigid bodies must precede torsional bodies (src/grid.F90:1226)");
+      if (B.fam == 2) {
+        seen_tors = true;
+        if (B.itheta <= 0) return fail(IB_E_INVAL, "torsional body needs itheta > 0");
+        TorsBody t{};
+        t.off = nbb.back();
+        t.npts = B.npts;
+        t.gamma0 = B.gamma0;
+        t.deltas = std::sqrt((B.x[1] - B.x[0]) * (B.x[1] - B.x[0]) + (B.y[1] - B.y[0]) * (B.y[1] - B.y[0]));
+        t.xbf0 = B.x[0]; t.ybf0 = B.y[0];
+        t.M = B.itheta; t.K = B.ktheta; t.C = B.ctheta;
+        h_tb.push_back(t);
+      }
+      for (int j = 0; j < 2 * B.npts; ++j) {
+        const int pnt = j < B.npts ? j : j - B.npts;
+        h_cbody.push_back(b);
+        h_ccomp.push_back(j < B.npts ? 1 : 2);
+        h_tors_of.push_back(B.fam == 2 ? (int)h_tb.size() - 1 : -1);
+        hxb.push_back(B.x[pnt]);
+        hyb.push_back(B.y[pnt]);
+      }
+      nbb.push_back(nbb.back() + 2 * B.npts);
+      nb += B.npts;
+    }
+    nbodies = nbod;
+    nf = 2 * nb;
+    ntors = (int)h_tb.size();
+
+    IB_HIP(hipGetDevice(&dev));
+    IB_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    int lds = 0, optin = 0;
+    IB_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess) optin = 0;
+    (void)hipGetLastError();
+    max_lds = (size_t)std::max(lds, optin);
+    if (max_lds > 65536) dst_set_max_lds(max_lds);
+
+    px.init(m - 1);
+    py.init(n - 1);
+    if (dst_lds_bytes(px.dev, 1) > max_lds || dst_lds_bytes(py.dev, 2) > max_lds)
+      return fail(IB_E_UNSUPPORTED, "grid too large for the single-workgroup DST (LDS)");
+
+    // setup_fft tables: src/myfft.F90:262-289
+    const double pi = 4.0 * std::atan(1.0);
+    normalize = 4.0 * (double)(float)(m * n);
+    const double del2 = delta * delta;
+    vfac.resize(L); con1.resize(L); con2.resize(L); q0x.resize(L);
+    for (int k = 0; k < L; ++k) {
+      const double del22 = del2 * std::pow(4.0, (double)k);
+      vfac[k] = 0.5 * dt / Re / del22 / normalize;
+      con1[k] = 1.5 * dt / del22 / normalize;
+      con2[k] = -0.5 * dt / del22 / normalize;
+      // motion_potential with motion_grid = (-1, 0, ...): q0x = -fac * (-1) (src/variables.F90:299-314)
+      const double fac = delta * std::pow(2.0, (double)k);
+      q0x[k] = -fac * (-1.0) + 0.0;
+    }
+    const long P = g.P;
+    std::vector<double> hl(P), hl1((size_t)P * L), hl1i((size_t)P * L);
+    for (int j = 2; j <= n; ++j)
+      for (int i = 2; i <= m; ++i) {
+        const double lam = -2.0 * (std::cos(pi * (double)(i - 1) / (double)m) + std::cos(pi * (double)(j - 1) / (double)n) - 2.0);
+        const long q = (long)(i - 2) + (long)(m - 1) * (j - 2);
+        hl[q] = 1.0 / lam / normalize;
+        for (int k = 0; k < L; ++k) {
+          const double del22 = del2 * std::pow(4.0, (double)k);
+          hl1[(size_t)k * P + q] = (1.0 - 0.5 * dt * lam / del22 / Re) / normalize;
+          hl1i[(size_t)k * P + q] = 1.0 / (1.0 + 0.5 * dt * lam / del22 / Re);
+        }
+      }
+    laminv.alloc(P); lam1.alloc((size_t)P * L); lam1i.alloc((size_t)P * L);
+    IB_HIP(hipMemcpy(laminv.p, hl.data(), sizeof(double) * P, hipMemcpyHostToDevice));
+    IB_HIP(hipMemcpy(lam1.p, hl1.data(), sizeof(double) * P * L, hipMemcpyHostToDevice));
+    IB_HIP(hipMemcpy(lam1i.p, hl1i.data(), sizeof(double) * P * L, hipMemcpyHostToDevice));
+
+    // state (initial_condition: all zero, src/variables.F90:262-295)
+    omega.alloc((size_t)P * L); s.alloc((size_t)P * L); rhs_old.alloc((size_t)P * L);
+    qx.alloc((size_t)g.Fx * L); qy.alloc((size_t)g.Fy * L);
+    t0.alloc(P); t1.alloc(P); t2.alloc(P);
+    vv.alloc((size_t)P * L); sn.alloc((size_t)P * L); vx.alloc((size_t)g.Fx * L); vy.alloc((size_t)g.Fy * L);
+    hx.alloc(g.Fx); hy.alloc(g.Fy);
+    lastbc.alloc((size_t)g.nbc * L); obc.alloc(g.nbc); sbcA.alloc(g.nbc); sbcB.alloc(g.nbc); zbc.alloc(g.nbc);
+    double* zero_these[] = {omega, s, rhs_old, qx, qy, zbc};
+    size_t zero_n[] = {omega.n, s.n, rhs_old.n, qx.n, qy.n, zbc.n};
+    for (int i = 0; i < 6; ++i) fill(st, zero_these[i], (long)zero_n[i], 0.0);
+
+    xb.alloc(nf); yb.alloc(nf); f.alloc(nf); f_rdst.alloc(nf); rhsf.alloc(nf); gw.alloc(nf);
+    w.alloc((size_t)nf * kTaps);
+    comp.alloc(nf); tors_of.alloc(nf); err_flag.alloc(1);
+    if (nf > 0) {
+      IB_HIP(hipMemcpy(xb.p, hxb.data(), sizeof(double) * nf, hipMemcpyHostToDevice));
+      IB_HIP(hipMemcpy(yb.p, hyb.data(), sizeof(double) * nf, hipMemcpyHostToDevice));
+      IB_HIP(hipMemcpy(comp.p, h_ccomp.data(), sizeof(int) * nf, hipMemcpyHostToDevice));
+      IB_HIP(hipMemcpy(tors_of.p, h_tors_of.data(), sizeof(int) * nf, hipMemcpyHostToDevice));
+    }
+    fill(st, f, nf, 0.0);
+    fill(st, f_rdst, nf, 0.0);
+    IB_HIP(hipMemsetAsync(err_flag.p, 0, sizeof(int), st));
+    tb.alloc(std::max(1, ntors));
+    if (ntors > 0) IB_HIP(hipMemcpy(tb.p, h_tb.data(), sizeof(TorsBody) * ntors, hipMemcpyHostToDevice));
+    th.alloc((size_t)6 * std::max(1, ntors));
+    dabs.alloc(std::max(1, ntors));
+    fill(st, th, (long)th.n, 0.0);
+
+    V.alloc((size_t)(kmax + 1) * std::max(1, nf));
+    H.alloc((size_t)(kmax + 1) * kmax);
+    cc.alloc(kmax); ss.alloc(kmax); grs.alloc(kmax + 1);
+    IB_HIP(hipMalloc(&gst, sizeof(GmresDev)));
+    IB_HIP(hipHostMalloc(&hst, sizeof(GmresDev), hipHostMallocDefault));
+    IB_HIP(hipHostMalloc(&hbuf, sizeof(double) * std::max(64, ntors + 1), hipHostMallocDefault));
+    IB_HIP(hipHostMalloc(&hflag, sizeof(int), hipHostMallocDefault));
+
+    csr.alloc(nf * kTaps, g.Fx + g.Fy);
+    rebuild_weights();
+    if (read_err_flag()) return fail(IB_E_INVAL, "body point regularisation stencil leaves the grid");
+    itime = prm.istart;
+    sync();
+    return IB_OK;
+  }
+
+  ~ib_handle() {
+    if (st) (void)hipStreamSynchronize(st);
+    tim[0].destroy();
+    tim[1].destroy();
+    csr.release();
+    px.release();
+    py.release();
+    if (gst) (void)hipFree(gst);
+    if (hst) (void)hipHostFree(hst);
+    if (hbuf) (void)hipHostFree(hbuf);
+    if (hflag) (void)hipHostFree(hflag);
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  // -------------------------------------------------------------- host I/O
+  void h2d(double* d, const double* h, size_t n) {
+    IB_HIP(hipMemcpyAsync(d, h, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  }
+  void d2h(double* h, const double* d, size_t n) {
+    IB_HIP(hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  }
+};
+
+// ------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------
+
+#define API_TRY try {
+#define API_CATCH(h)                           \
+  }                                            \
+  catch (const std::exception& e) {            \
+    if (h) (h)->err = e.what();                \
+    else g_create_err = e.what();              \
+    return IB_E_HIP;                           \
+  }
+
+extern "C" {
+
+int ib_create(const ib_params* p, int32_t nbodies, const ib_body* bodies, ib_handle** out) {
+  if (!p || !out) return IB_E_INVAL;
+  *out = nullptr;
+  ib_handle* h = nullptr;
+  try {
+    h = new ib_handle();
+    int rc = h->init(p, nbodies, bodies);
+    if (rc != IB_OK) {
+      g_create_err = h->err;
+      delete h;
+      return rc;
+    }
+  } catch (const std::exception& e) {
+    g_create_err = e.what();
+    delete h;
+    return IB_E_HIP;
+  }
+  *out = h;
+  return IB_OK;
+}
+
+int ib_destroy(ib_handle* h) {
+  delete h;
+  return IB_OK;
+}
+
+int ib_get_sizes(const ib_handle* h, ib_sizes* o) {
+  if (!h || !o) return IB_E_INVAL;
+  o->m = h->g.m; o->n = h->g.n; o->mgridlev = h->L; o->nbc = h->g.nbc;
+  o->n_nodes = h->g.P; o->n_xfaces = h->g.Fx; o->n_yfaces = h->g.Fy;
+  o->nb = h->nb; o->nf = h->nf; o->nbodies = h->nbodies; o->ntors = h->ntors;
+  return IB_OK;
+}
+
+int ib_last_error(const ib_handle* h, char* buf, int32_t len) {
+  const std::string& e = h ? h->err : g_create_err;
+  if (buf && len > 0) std::snprintf(buf, (size_t)len, "%s", e.c_str());
+  return (int)e.size();
+}
+
+int ib_advance(ib_handle* h, int32_t* itime, ib_step_info* info) {
+  if (!h) return IB_E_INVAL;
+  API_TRY
+  int rc = h->advance();
+  if (itime) *itime = h->itime;
+  if (info) {
+    info->itime = h->itime; info->ksp_iter = h->ksp_iter; info->fsi_iter = h->fsi_iter;
+    info->status = rc; info->err_fsi = h->err_fsi;
+  }
+  return rc;
+  API_CATCH(h)
+}
+
+int ib_get_level_fields(ib_handle* h, int32_t k, double* omega, double* s, double* qx, double* qy, double* rhs_old) {
+  if (!h || k < 1 || k > h->L) return IB_E_INVAL;
+  API_TRY
+  const size_t P = h->g.P, q = k - 1;
+  if (omega) h->d2h(omega, h->omega + q * P, P);
+  if (s) h->d2h(s, h->s + q * P, P);
+  if (qx) h->d2h(qx, h->qx + q * h->g.Fx, h->g.Fx);
+  if (qy) h->d2h(qy, h->qy + q * h->g.Fy, h->g.Fy);
+  if (rhs_old) h->d2h(rhs_old, h->rhs_old + q * P, P);
+  h->sync();
+  return IB_OK;
+  API_CATCH(h)
+}
+
+int ib_set_level_fields(ib_handle* h, int32_t k, const double* omega, const double* s, const double* qx,
+                        const double* qy, const double* rhs_old) {
+  if (!h || k < 1 || k > h->L) return IB_E_INVAL;
+  API_TRY
+  const size_t P = h->g.P, q = k - 1;
+  if (omega) h->h2d(h->omega + q * P, omega, P);
+  if (s) h->h2d(h->s + q * P, s, P);
+  if (qx) h->h2d(h->qx + q * h->g.Fx, qx, h->g.Fx);
+  if (qy) h->h2d(h->qy + q * h->g.Fy, qy, h->g.Fy);
+  if (rhs_old) h->h2d(h->rhs_old + q * P, rhs_old, P);
+  h->sync();
+  return IB_OK;
+  API_CATCH(h)
+}
+
+int ib_get_body(ib_handle* h, double* f, double* f_rdst, double* xb, double* yb, double* theta, double* thetad,
+                double* thetadd) {
+  if (!h) return IB_E_INVAL;
+  API_TRY
+  const size_t nf = h->nf, nt = h->ntors;
+  if (f && nf) h->d2h(f, h->f, nf);
+  if (f_rdst && nf) h->d2h(f_rdst, h->f_rdst, nf);
+  if (xb && nf) h->d2h(xb, h->xb, nf);
+  if (yb && nf) h->d2h(yb, h->yb, nf);
+  if (theta && nt) h->d2h(theta, h->th, nt);
+  if (thetad && nt) h->d2h(thetad, h->th + nt, nt);
+  if (thetadd && nt) h->d2h(thetadd, h->th + 2 * nt, nt);
+  h->sync();
+  return IB_OK;
+  API_CATCH(h)
+}
+
+// write_force2: src/operators_struct.F90:711-731
+int ib_get_forces(ib_handle* h, int32_t which, double* fxo, double* fyo) {
+  if (!h || !fxo || !fyo) return IB_E_INVAL;
+  API_TRY
+  std::vector<double> fv(std::max(1, h->nf));
+  if (h->nf) h->d2h(fv.data(), which ? h->f_rdst.p : h->f.p, h->nf);
+  h->sync();
+  std::vector<double> ax(h->nbodies, 0.0), ay(h->nbodies, 0.0);
+  for (int k = 0; k < h->nf; ++k) {
+    const int b = h->h_cbody[k];
+    if (h->h_ccomp[k] == 1) ax[b] = ax[b] + fv[k] / h->dt;
+    else ay[b] = ay[b] + fv[k] / h->dt;
+  }
+  for (int b = 0; b < h->nbodies; ++b) {
+    fxo[b] = 2 * h->delta * ax[b];
+    fyo[b] = 2 * h->delta * ay[b];
+  }
+  return IB_OK;
+  API_CATCH(h)
+}
+
+int ib_op_dst2d(ib_handle* h, int32_t batch, const double* in, double* out) {
+  if (!h || batch < 0) return IB_E_INVAL;
+  API_TRY
+  const size_t P = h->g.P;
+  for (int b = 0; b < batch; ++b) {
+    h->h2d(h->t2, in + b * P, P);
+    DstLaunch a;
+    a.in[0] = h->t2; a.out[0] = h->t0;
+    h->xpass(X_PLAIN, a);
+    DstLaunch c;
+    c.in[0] = h->t0; c.out[0] = h->t1;
+    h->ypass(Y_PLAIN, c);
+    h->d2h(out + b * P, h->t1, P);
+    h->sync();
+  }
+  return IB_OK;
+  API_CATCH(h)
+}
+
+int ib_op_ctci(ib_handle* h, const double* omega, double* s) {
+  if (!h) return IB_E_INVAL;
+  API_TRY
+  const size_t P = h->g.P;
+  h->h2d(h->t2, omega, P);
+  h->ctci_dev(h->t2, h->t2, nullptr);
+  h->d2h(s, h->t2, P);
+  h->sync();
+  return IB_OK;
+  API_CATCH(h)
+}
+
+int ib_op_ainv(ib_handle* h, const double* in, double* out) {
+  if (!h) return IB_E_INVAL;
+  API_TRY
+  const size_t P = h->g.P;
+  h->h2d(h->t2, in, P);
+  h->ainv_dev(h->t2, h->t2, false);
+  h->d2h(out, h->t2, P);
+  h->sync();
+  return IB_OK;
+  API_CATCH(h)
+}
+
+int ib_op_vort2flux(ib_handle* h, double* omega, double* s, double* qx, double* qy) {
+  if (!h) return IB_E_INVAL;
+  API_TRY
+  const size_t P = h->g.P, L = h->L;
+  h->h2d(h->vv, omega, P * L);
+  h->vort2flux_dev(h->vv, h->sn, h->vx, h->vy);
+  h->d2h(omega, h->vv, P * L);
+  if (s) h->d2h(s, h->sn, P * L);
+  if (qx) h->d2h(qx, h->vx, (size_t)h->g.Fx * L);
+  if (qy) h->d2h(qy, h->vy, (size_t)h->g.Fy * L);
+  h->sync();
+  return IB_OK;
+  API_CATCH(h)
+}
+
+int ib_op_reg(ib_handle* h, const double* fin, double* hxo, double* hyo) {
+  if (!h) return IB_E_INVAL;
+  API_TRY
+  if (h->nf) h->h2d(h->gw, fin, h->nf);
+  fill(h->st, h->hx, h->g.Fx, 0.0);
+  fill(h->st, h->hy, h->g.Fy, 0.0);
+  reg(h->st, h->g, h->csr, h->w, h->gw, h->hx, h->hy);
+  h->d2h(hxo, h->hx, h->g.Fx);
+  h->d2h(hyo, h->hy, h->g.Fy);
+  h->sync();
+  return IB_OK;
+  API_CATCH(h)
+}
+
+int ib_op_regT(ib_handle* h, const double* hxi, const double* hyi, const double* f_in, double* f_out) {
+  if (!h) return IB_E_INVAL;
+  API_TRY
+  h->h2d(h->hx, hxi, h->g.Fx);
+  h->h2d(h->hy, hyi, h->g.Fy);
+  if (h->nf) h->h2d(h->gw, f_in, h->nf);
+  regT(h->st, h->g, h->nf, h->w, h->csr.key, h->hx, h->hy, 0.0, h->gw, h->rhsf);
+  if (h->nf) h->d2h(f_out, h->rhsf, h->nf);
+  h->sync();
+  return IB_OK;
+  API_CATCH(h)
+}
+
+int ib_op_schur_matvec(ib_handle* h, const double* x, double* y) {
+  if (!h) return IB_E_INVAL;
+  API_TRY
+  if (h->nf) h->h2d(h->rhsf, x, h->nf);
+  h->schur_dev(h->rhsf, h->gw);
+  if (h->nf) h->d2h(y, h->gw, h->nf);
+  h->sync();
+  return IB_OK;
+  API_CATCH(h)
+}
+
+int ib_op_nonlinear_rot(ib_handle* h, int32_t k, const double* omega, const double* qx, const double* qy,
+                        const double* bc, double* rhs) {
+  if (!h || k < 1 || k > h->L) return IB_E_INVAL;
+  API_TRY
+  const size_t P = h->g.P;
+  h->h2d(h->t0, omega, P);
+  h->h2d(h->hx, qx, h->g.Fx);
+  h->h2d(h->hy, qy, h->g.Fy);
+  h->h2d(h->obc, bc, h->g.nbc);
+  nl_rhs(h->st, h->g, h->t0, h->hx, h->hy, h->q0x[k - 1], h->obc, h->zbc, 0.0, 0.0, 0.0, 0, nullptr, nullptr,
+         h->t1);
+  h->d2h(rhs, h->t1, P);
+  h->sync();
+  return IB_OK;
+  API_CATCH(h)
+}
+
+int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable) {
+  if (!h || kclass < 0 || kclass > 1) return IB_E_INVAL;
+  API_TRY
+  h->sync();
+  h->tim[kclass].on = enable != 0;
+  h->tim[kclass].count = 0;
+  h->tim[kclass].total_ms = 0;
+  return IB_OK;
+  API_CATCH(h)
+}
+
+int ib_kernel_timing_read(ib_handle* h, int32_t kclass, int64_t* launches, double* total_ms) {
+  if (!h || kclass < 0 || kclass > 1) return IB_E_INVAL;
+  API_TRY
+  h->sync();
+  if (launches) *launches = h->tim[kclass].count;
+  if (total_ms) *total_ms = h->tim[kclass].total_ms;
+  return IB_OK;
+  API_CATCH(h)
+}
+
+}  // extern "C"

@@ -1,0 +1,153 @@
+"""Readers for the reference's input decks and the named workloads.
+
+* ``read_ib_inp``: namelist &READ_PARAMETERS of input/ib.inp
+  (src/parameters.F90:51-78).
+* ``read_body``: one input/body.NNN.inp (src/grid.F90:252-288): npts, fam,
+  [fam 2: gamma0 (deg), alpha0 (deg), itheta, ktheta, ctheta], then x y per line.
+* ``read_case``: ib.inp + body.001.inp, body.002.inp, ... until one is missing.
+* ``cylinder_case``, ``plate_case``, ``tiny_case``: synthetic workloads with the
+  shapes of the named configurations (circular cylinder 200x200x3; hinged
+  plate 400x400x4), generated here because the reference ships no such decks.
+"""
+import math
+import os
+import re
+
+import numpy as np
+
+from .abi import BodySpec, make_params
+
+_NML_KEYS = ["istart", "istop", "isave", "irestart", "m", "n", "dt", "re", "fsi_tol", "atol", "rtol", "len",
+             "offsetx", "offsety", "mgridlev", "compute_pressure", "dimen", "num_stat", "motion_prescribed",
+             "sub_domain", "sub_domain_full_rectangular", "sub_domain_precomputed", "sdxi", "sdxe", "sdyi",
+             "sdye", "delta_sd"]
+
+
+def _fortran_value(tok):
+    t = tok.strip().rstrip(",").strip()
+    u = t.upper()
+    if u in ("T", ".TRUE.", "TRUE"):
+        return True
+    if u in ("F", ".FALSE.", "FALSE"):
+        return False
+    u = u.replace("D", "E")
+    if re.fullmatch(r"[+-]?\d+", u):
+        return int(u)
+    return float(u)
+
+
+def read_ib_inp(path):
+    """Parse the namelist group READ_PARAMETERS; keys are lower-cased."""
+    txt = open(path).read()
+    m = re.search(r"&\s*read_parameters(.*?)/", txt, re.IGNORECASE | re.DOTALL)
+    if not m:
+        raise ValueError(f"{path}: no &READ_PARAMETERS group")
+    body = m.group(1)
+    out = {}
+    for key, val in re.findall(r"([A-Za-z_][A-Za-z0-9_]*)\s*=\s*([^=\n]+?)(?=,?\s*(?:[A-Za-z_][A-Za-z0-9_]*\s*=|$))",
+                               body, re.DOTALL):
+        k = key.lower()
+        if k not in _NML_KEYS:
+            raise ValueError(f"{path}: unknown namelist key {key}")
+        out[k] = _fortran_value(val)
+    return out
+
+
+def read_body(path):
+    vals = open(path).read().split()
+    npts = int(vals[0]); fam = int(vals[1]); pos = 2
+    gamma0 = alpha0 = itheta = ktheta = ctheta = 0.0
+    if fam == 2:
+        gamma0 = float(vals[2].replace("D", "E")) * math.pi / 180
+        alpha0 = float(vals[3].replace("D", "E")) * math.pi / 180
+        itheta, ktheta, ctheta = (float(v.replace("D", "E")) for v in vals[4:7])
+        pos = 7
+    xy = np.array([float(v.replace("D", "E")) for v in vals[pos:pos + 2 * npts]]).reshape(npts, 2)
+    return BodySpec(xy[:, 0], xy[:, 1], fam, gamma0, alpha0, itheta, ktheta, ctheta)
+
+
+def read_case(input_dir, **overrides):
+    nml = read_ib_inp(os.path.join(input_dir, "ib.inp"))
+    nml.update(overrides)
+    bodies = []
+    i = 1
+    while os.path.exists(os.path.join(input_dir, f"body.{i:03d}.inp")):
+        bodies.append(read_body(os.path.join(input_dir, f"body.{i:03d}.inp")))
+        i += 1
+    p = make_params(nml["m"], nml["n"], nml["mgridlev"], nml["dt"], nml["re"], nml["len"], nml["offsetx"],
+                    nml["offsety"], fsi_tol=nml.get("fsi_tol", 1e-7), atol=nml.get("atol", 1e-7),
+                    rtol=nml.get("rtol", 1e-7), istart=nml.get("istart", 0),
+                    num_stat=int(bool(nml.get("num_stat", False))), sub_domain=int(bool(nml.get("sub_domain", False))))
+    return p, bodies, nml
+
+
+def write_body(path, b):
+    with open(path, "w") as fh:
+        fh.write(f"{b.npts}\n{b.fam}\n")
+        if b.fam == 2:
+            fh.write(f"{b.gamma0 * 180 / math.pi:.16f}\n{b.alpha0 * 180 / math.pi:.16f}\n")
+            fh.write(f"{b.itheta:.16e}\n{b.ktheta:.16e}\n{b.ctheta:.16e}\n")
+        for x, y in zip(b.x, b.y):
+            fh.write(f"{x:.16f}   {y:.16f}\n")
+
+
+def write_ib_inp(path, p, istop=10, isave=10):
+    with open(path, "w") as fh:
+        fh.write("&READ_PARAMETERS\n")
+        fh.write(f"ISTART = {p.istart},\nISTOP = {istop},\nISAVE = {isave},\nIRESTART = 0,\n")
+        fh.write(f"M = {p.m},\nN = {p.n},\nDT = {p.dt!r},\nRe = {p.re!r},\n")
+        fh.write(f"FSI_TOL = {p.fsi_tol!r},\nATOL = {p.atol!r},\nRTOL = {p.rtol!r},\n")
+        fh.write(f"LEN = {p.len!r},\nOFFSETX = {p.offsetx!r},\nOFFSETY = {p.offsety!r},\nMGRIDLEV = {p.mgridlev},\n")
+        fh.write("COMPUTE_PRESSURE = F,\ndimen = 2,\nnum_stat=F,\nmotion_prescribed=F,\nsub_domain = F,\n/\n")
+
+
+# --------------------------------------------------------------------------
+# named synthetic workloads
+# --------------------------------------------------------------------------
+
+def cylinder_body(npts=157, radius=0.5):
+    t = 2 * np.pi * np.arange(npts) / npts
+    return BodySpec(radius * np.cos(t), radius * np.sin(t), fam=1)
+
+
+def cylinder_case(m=200, n=200, mgridlev=3, dt=0.004, re=100.0, npts=157, tol=1e-7, **kw):
+    """Rigid circular cylinder D=1, Re=100, finest domain [-1,3]x[-2,2] (dx = 4/m)."""
+    p = make_params(m, n, mgridlev, dt, re, 4.0, 1.0, 2.0, fsi_tol=tol, atol=tol, rtol=tol, **kw)
+    return p, [cylinder_body(npts)]
+
+
+def plate_body(npts=101, length=1.0, pitch_deg=10.0, mass_ratio=0.2, k_theta=0.1, c_theta=0.0):
+    """Flat plate pinned at its leading edge (origin) as a torsional body.
+
+    Points j = 0..npts-1 at j*ds along angle gamma0 = -pitch (trailing edge
+    below the hinge, flow in +x).  Rotational inertia about the hinge of a thin
+    plate with mass ratio M = rho_s h / (rho_f L): I = M L^3 / 3 (unit fluid
+    density); the spring stiffness is taken as k_theta directly.
+    """
+    g = -math.radians(pitch_deg)
+    ds = length / (npts - 1)
+    r = ds * np.arange(npts)
+    return BodySpec(r * math.cos(g), r * math.sin(g), fam=2, gamma0=g, alpha0=0.0,
+                    itheta=mass_ratio * length ** 3 / 3.0, ktheta=k_theta, ctheta=c_theta)
+
+
+def plate_case(m=400, n=400, mgridlev=4, dt=0.001, re=200.0, npts=101, tol=1e-10, **kw):
+    """Hinged flexible-plate FSI workload: 400x400 cells, 4 levels, dx=0.01."""
+    p = make_params(m, n, mgridlev, dt, re, 4.0, 1.0, 2.0, fsi_tol=tol, atol=tol, rtol=tol, **kw)
+    return p, [plate_body(npts)]
+
+
+def tiny_case(m=32, n=24, mgridlev=2, torsional=True, tol=1e-12, dt=0.01, re=50.0, **kw):
+    """Small grid for oracle-vs-GPU parity: a coarse cylinder plus a hinged plate."""
+    p = make_params(m, n, mgridlev, dt, re, 4.0, 1.6, 1.5, fsi_tol=tol, atol=tol, rtol=tol, **kw)
+    dx = 4.0 / m
+    nc = max(8, int(round(np.pi * 0.6 / dx)))
+    t = 2 * np.pi * np.arange(nc) / nc
+    bodies = [BodySpec(0.3 * np.cos(t) - 0.4, 0.3 * np.sin(t), fam=1)]
+    if torsional:
+        npl = max(4, int(round(0.6 / dx)) + 1)
+        g = -math.radians(15.0)
+        r = np.linspace(0, 0.6, npl)
+        bodies.append(BodySpec(0.2 + r * math.cos(g), 0.05 + r * math.sin(g), fam=2, gamma0=g,
+                               itheta=0.05, ktheta=0.2, ctheta=0.0))
+    return p, bodies

@@ -1,0 +1,149 @@
+/*
+ * ib_gpu.h -- C ABI of the MI355X immersed-boundary projection time step.
+ *
+ * One handle = one simulation (nested multi-domain grid + immersed bodies),
+ * all state resident in HBM of the current HIP device.  Every entry point
+ * returns an int status (IB_OK == 0).  Host buffers passed in are borrowed for
+ * the duration of the call only.  Arrays use the reference layout: Fortran
+ * column-major, i (x) fastest, with the index ranges of the reference's DMDAs:
+ *
+ *   omega, s, rhs_old : nodes  i = 2..m,   j = 2..n      (m-1)*(n-1) values
+ *   qx (x-face flux)  : faces  i = 1..m+1, j = 1..n      (m+1)*n     values
+ *   qy (y-face flux)  : faces  i = 1..m,   j = 1..n+1    m*(n+1)     values
+ *   force vector f    : [body1 x(1..n1), body1 y(1..n1), body2 x, body2 y, ...]
+ *
+ * Reference seams replaced (file:line in nirmaljp6/IB_Parallel):
+ *   ib_create           setup (src/grid.F90:117-234) + setup_variables
+ *                       (src/variables.F90:84-257) + initial_condition
+ *                       (src/variables.F90:262-295)
+ *   ib_advance          advance(itime) (src/timestepper.F90:18-556)
+ *   ib_get_level_fields / ib_set_level_fields
+ *                       the per-level Vecs written by write_variables
+ *                       (src/variables.F90:709-803) / read by read_variables
+ *                       (src/variables.F90:807-913)
+ *   ib_get_body         fvec, f_rdstvec, xbvec, ybvec, theta* (src/grid.F90:61,
+ *                       src/variables.F90:205,213-214)
+ *   ib_get_forces       write_force2 (src/operators_struct.F90:711-731)
+ *   ib_op_*             single operators of src/myfft.F90,
+ *                       src/operators_fluid.F90, src/fsinterface.F90 and the
+ *                       MatShell callback ab_times (src/timestepper.F90:774-782)
+ */
+#ifndef IB_GPU_H
+#define IB_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  IB_OK = 0,
+  IB_E_INVAL = 1,          /* bad parameters / deck (e.g. m or n not divisible by 4) */
+  IB_E_NOCONV_GMRES = 2,   /* Krylov solve hit gmres_max_it or diverged */
+  IB_E_NOCONV_FSI = 3,     /* FSI fixed point exceeded fsi_max_count */
+  IB_E_HIP = 4,            /* HIP runtime error */
+  IB_E_UNSUPPORTED = 5     /* option present in the reference deck but not built */
+};
+
+/* Mirrors namelist READ_PARAMETERS (src/parameters.F90:55-59) plus solver knobs
+ * the reference takes from PETSc defaults. */
+typedef struct ib_params {
+  int32_t m, n;            /* cells in x and y on every level (multiples of 4) */
+  int32_t mgridlev;        /* number of nested grid levels */
+  int32_t istart;          /* initial time index (0 = cold start) */
+  double dt, re;           /* time step, Reynolds number */
+  double len;              /* x-extent of the finest domain; delta = len/m */
+  double offsetx, offsety; /* finest domain is [-offsetx, len-offsetx] x [-offsety, ...] */
+  double fsi_tol;          /* FSI fixed point tolerance on max|dtheta| */
+  double atol, rtol;       /* GMRES absolute / relative tolerance */
+  int32_t gmres_restart;   /* PETSc default 30 */
+  int32_t gmres_max_it;    /* PETSc default 10000 */
+  int32_t fsi_max_count;   /* loop runs while count <= this (reference: 1000) */
+  int32_t deltafnc_f32_literals; /* 1: exponent-less literals of deltafnc are
+                                    REAL(4) as written (src/grid.F90:972-993) */
+  int32_t num_stat;        /* 1 would select the Cholesky path: not built */
+  int32_t sub_domain;      /* 1 would select the sub-domain path: not built */
+  int32_t reserved[6];
+} ib_params;
+
+/* One body deck input/body.NNN.inp (src/grid.F90:252-288), angles in radians. */
+typedef struct ib_body {
+  int32_t npts;
+  int32_t fam;             /* 1 rigid, 2 torsional (hinged at point 1) */
+  double gamma0, alpha0;   /* equilibrium angle, angle w.r.t. previous body */
+  double itheta, ktheta, ctheta; /* inertia, stiffness, damping (fam 2) */
+  const double *x, *y;     /* npts coordinates */
+} ib_body;
+
+typedef struct ib_step_info {
+  int32_t itime;           /* time index after the step */
+  int32_t ksp_iter;        /* GMRES iterations summed over FSI iterations */
+  int32_t fsi_iter;        /* FSI iterations */
+  int32_t status;          /* IB_OK or IB_E_NOCONV_* of this step */
+  double err_fsi;          /* last max|dtheta| */
+} ib_step_info;
+
+typedef struct ib_sizes {
+  int32_t m, n, mgridlev, nbc;
+  int32_t n_nodes, n_xfaces, n_yfaces; /* (m-1)(n-1), (m+1)n, m(n+1) */
+  int32_t nb, nf, nbodies, ntors;      /* points, forces (=2nb), bodies, torsional bodies */
+} ib_sizes;
+
+typedef struct ib_handle ib_handle;
+
+/* --- lifetime (setup: src/grid.F90:117-234, src/variables.F90:84-295) --- */
+int ib_create(const ib_params* p, int32_t nbodies, const ib_body* bodies, ib_handle** out);
+int ib_destroy(ib_handle* h);
+int ib_get_sizes(const ib_handle* h, ib_sizes* out);
+/* Copies the message of the last failing call on h (or of ib_create when h is
+ * NULL) into buf; returns its length. */
+int ib_last_error(const ib_handle* h, char* buf, int32_t len);
+
+/* --- one time step: advance(itime), src/timestepper.F90:18-556.
+ * Increments *itime like the reference; info may be NULL. --- */
+int ib_advance(ib_handle* h, int32_t* itime, ib_step_info* info);
+
+/* --- state access (level k = 1..mgridlev); any pointer may be NULL --- */
+int ib_get_level_fields(ib_handle* h, int32_t k, double* omega, double* s,
+                        double* qx, double* qy, double* rhs_old);
+int ib_set_level_fields(ib_handle* h, int32_t k, const double* omega, const double* s,
+                        const double* qx, const double* qy, const double* rhs_old);
+/* theta* have one entry per torsional body, in deck order */
+int ib_get_body(ib_handle* h, double* f, double* f_rdst, double* xb, double* yb,
+                double* theta, double* thetad, double* thetadd);
+/* write_force2: per body fx = 2*delta*sum(f_x)/dt, fy likewise (drag, lift).
+ * which = 0: fvec, 1: f_rdstvec. fx, fy hold nbodies entries. */
+int ib_get_forces(ib_handle* h, int32_t which, double* fx, double* fy);
+
+/* --- single operators on the handle's grid (host buffers) --- */
+/* unnormalised 2-D DST-I (FFTW RODFT00 x RODFT00) of `batch` node arrays,
+ * src/myfft.F90:207-216 */
+int ib_op_dst2d(ib_handle* h, int32_t batch, const double* in, double* out);
+/* ctci: s = dst(laminv*dst(omega)), src/myfft.F90:177-188 */
+int ib_op_ctci(ib_handle* h, const double* omega, double* s);
+/* ainv: vort = dst(lam1i(:,:,1)*dst(vort)/normalize), src/myfft.F90:161-173 */
+int ib_op_ainv(ib_handle* h, const double* in, double* out);
+/* vort2flux on all levels, src/operators_fluid.F90:237-377.  omega holds
+ * mgridlev node arrays and is updated in place (coarsify); s, qx, qy out. */
+int ib_op_vort2flux(ib_handle* h, double* omega, double* s, double* qx, double* qy);
+/* reg = E^T (src/fsinterface.F90:14-47): hx, hy are overwritten with E^T f */
+int ib_op_reg(ib_handle* h, const double* f, double* hx, double* hy);
+/* regT = E (src/fsinterface.F90:51-80): f_out = f_in + E (hx, hy) */
+int ib_op_regT(ib_handle* h, const double* hx, const double* hy, const double* f_in, double* f_out);
+/* ab_times: y = a_times2(x) + b_times2(x), src/timestepper.F90:774-782 */
+int ib_op_schur_matvec(ib_handle* h, const double* x, double* y);
+/* nonlinear + rot of one level: rhs = C^T N(omega_k, q_k + q0_k, bc),
+ * src/operators_fluid.F90:553-587 and :203-233; bc has nbc entries */
+int ib_op_nonlinear_rot(ib_handle* h, int32_t k, const double* omega, const double* qx,
+                        const double* qy, const double* bc, double* rhs);
+
+/* --- instrumentation: per-kernel-class timing with HIP events on the
+ * handle's stream.  class 0 = DST row pass, 1 = DST column pass. --- */
+int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable);
+int ib_kernel_timing_read(ib_handle* h, int32_t kclass, int64_t* launches, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IB_GPU_H */

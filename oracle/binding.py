@@ -1,0 +1,112 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg.  The product package never imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+from ib_parallel_amd import abi
+from ib_parallel_amd.sim import Simulation
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+_fns = None
+
+EXTRA = {
+    "op_rot": (C.c_int, [C.c_void_p, abi._D, abi._D, abi._D]),
+    "op_curl": (C.c_int, [C.c_void_p, abi._D, abi._D, abi._D, abi._D]),
+    "op_coarsify": (C.c_int, [C.c_void_p, abi._D, abi._D]),
+    "op_get_bc": (C.c_int, [C.c_void_p, abi._D, C.c_double, abi._D]),
+    "op_apply_bc": (C.c_int, [C.c_void_p, abi._D, abi._D, C.c_double]),
+    "op_weights": (C.c_int, [C.c_void_p, abi._D, abi._I, abi._I]),
+    "deltafnc": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_int32]),
+    "dst1d": (C.c_int, [C.c_int32, C.c_int32, abi._D, abi._D]),
+    "set_threads": (C.c_int, [C.c_int32]),
+    "get_threads": (C.c_int, []),
+}
+
+
+def build():
+    """Compile liboracle.so in place (make; g++)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib, _fns
+    if _fns is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = C.CDLL(LIB)
+        _fns = abi.bind(_lib, "orc_", EXTRA)
+    return _fns
+
+
+class OracleSim(Simulation):
+    """Simulation on the CPU restatement, plus the oracle-only operators."""
+
+    def __init__(self, params, bodies):
+        super().__init__(load(), params, bodies, prefix="orc_")
+
+    def rot(self, hx, hy):
+        import numpy as np
+        v = np.empty(self.P)
+        self._check(self._f["op_rot"](self.h, abi.dptr(np.ascontiguousarray(hx, float)),
+                                      abi.dptr(np.ascontiguousarray(hy, float)), abi.dptr(v)))
+        return v
+
+    def curl(self, s, sbc):
+        import numpy as np
+        qx = np.empty(self.Fx); qy = np.empty(self.Fy)
+        self._check(self._f["op_curl"](self.h, abi.dptr(np.ascontiguousarray(s, float)),
+                                       abi.dptr(np.ascontiguousarray(sbc, float)), abi.dptr(qx), abi.dptr(qy)))
+        return qx, qy
+
+    def coarsify(self, fine, coarse):
+        import numpy as np
+        out = np.array(coarse, dtype=np.float64, copy=True)
+        self._check(self._f["op_coarsify"](self.h, abi.dptr(np.ascontiguousarray(fine, float)), abi.dptr(out)))
+        return out
+
+    def get_bc(self, coarse, fac):
+        import numpy as np
+        bc = np.full(self.nbc, np.nan)
+        self._check(self._f["op_get_bc"](self.h, abi.dptr(np.ascontiguousarray(coarse, float)), fac, abi.dptr(bc)))
+        return bc
+
+    def apply_bc(self, vort, bc, fac):
+        import numpy as np
+        out = np.array(vort, dtype=np.float64, copy=True)
+        self._check(self._f["op_apply_bc"](self.h, abi.dptr(out), abi.dptr(np.ascontiguousarray(bc, float)), fac))
+        return out
+
+    def weights(self):
+        import numpy as np
+        w = np.empty(49 * self.nf)
+        # x entries and y entries each get 49 taps
+        b = self.body()
+        nfx = int(round(self.nf / 2))
+        ix = np.empty(49 * self.nf, dtype=np.int32); iy = np.empty(49 * self.nf, dtype=np.int32)
+        self._check(self._f["op_weights"](self.h, abi.dptr(w), ix.ctypes.data_as(abi._I), iy.ctypes.data_as(abi._I)))
+        return w.reshape(self.nf, 49), ix, iy, b
+
+
+def deltafnc(x1, x2, dr, f32=True):
+    return load()["deltafnc"](x1, x2, dr, 1 if f32 else 0)
+
+
+def dst1d(x):
+    import numpy as np
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    N = x.shape[-1]
+    cnt = x.size // N
+    out = np.empty_like(x)
+    load()["dst1d"](N, cnt, abi.dptr(x), abi.dptr(out))
+    return out
+
+
+def set_threads(n):
+    load()["set_threads"](int(n))

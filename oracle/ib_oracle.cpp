@@ -1,0 +1,1079 @@
+/*
+ * ib_oracle.cpp -- FP64 CPU restatement of IB_Parallel's advance(itime) path.
+ *
+ * TEST INFRASTRUCTURE ONLY: parity checker and CPU baseline.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load this library.
+ *
+ * Every routine names the reference routine (file:line in nirmaljp6/IB_Parallel)
+ * whose arithmetic and loop order it follows.  Single-process semantics: one
+ * "rank" owns the whole grid, so PETSc VecSetValues(ADD_VALUES) is a sequential
+ * add in insertion order and every DMDA ghost at a physical boundary stays 0
+ * unless the reference writes it explicitly.
+ *
+ * The unnormalised DST-I (FFTW RODFT00) is computed through a complex FFT of
+ * length 2M (M = N+1) of an odd-extended pair of real vectors: with
+ * z = x_a + i x_b extended oddly, Z = -2i S(x_a) + 2 S(x_b), so
+ * RODFT00(x_a) = -Im Z and RODFT00(x_b) = Re Z.  The FFT is a mixed-radix
+ * Stockham autosort (radices 4, 2, 3, 5, 7 and a generic odd radix).
+ *
+ * Parity pins (tests/test_oracle_*.py): DST-I against scipy.fft.dst golden
+ * vectors and brute force; ctci/ainv against the 5-point operator they invert;
+ * rot/curl adjointness and C^T C = 5-point Laplacian; reg/regT adjointness;
+ * deltafnc partition of unity, first moment and continuity; coarsify/get_bc on
+ * constant and bilinear fields; GMRES against a dense LAPACK solve of the
+ * matrix assembled column by column from the matvec; torsional B rank-1 and the
+ * free-spring closed form.
+ */
+#include "ib_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+typedef std::complex<double> cplx;
+
+/* ------------------------------------------------------------------------ */
+/* FFT / DST-I                                                               */
+/* ------------------------------------------------------------------------ */
+
+std::vector<int> factorize(int L) {
+  std::vector<int> r;
+  const int small[] = {4, 2, 3, 5, 7};
+  for (int q : small)
+    while (L % q == 0) { r.push_back(q); L /= q; }
+  for (int p = 11; L > 1; p += 2)
+    while (L % p == 0) { r.push_back(p); L /= p; }
+  return r;
+}
+
+struct FFTPlan {
+  int L = 0;
+  std::vector<int> rad;
+  std::vector<cplx> W; /* W[t] = exp(-2 pi i t / L) */
+  void init(int L_) {
+    L = L_;
+    rad = factorize(L);
+    W.resize(L);
+    for (int t = 0; t < L; ++t) {
+      long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)L;
+      W[t] = cplx((double)cosl(a), (double)-sinl(a));
+    }
+  }
+  /* Stockham autosort; result lands in a (tmp is scratch of length L). */
+  void exec(cplx* a, cplx* tmp) const {
+    cplx* in = a;
+    cplx* out = tmp;
+    int Ns = 1;
+    cplx v[128], V[128];
+    std::vector<cplx> vbig, Vbig;
+    for (int R : rad) {
+      cplx* vv = v;
+      cplx* VV = V;
+      if (R > 128) { vbig.resize(R); Vbig.resize(R); vv = vbig.data(); VV = Vbig.data(); }
+      const int LR = L / R;
+      const int tws = L / (Ns * R);
+      for (int j = 0; j < LR; ++j) {
+        const int k = j % Ns;
+        for (int r = 0; r < R; ++r) vv[r] = in[j + r * LR] * W[(k * r * tws) % L];
+        for (int s = 0; s < R; ++s) {
+          cplx acc = 0.0;
+          for (int r = 0; r < R; ++r) acc += vv[r] * W[((r * s) % R) * LR];
+          VV[s] = acc;
+        }
+        const int base = (j / Ns) * Ns * R + k;
+        for (int s = 0; s < R; ++s) out[base + s * Ns] = VV[s];
+      }
+      Ns *= R;
+      std::swap(in, out);
+    }
+    if (in != a) std::memcpy(a, in, sizeof(cplx) * L);
+  }
+};
+
+/* RODFT00 of length N on two strided real vectors at once (xb may be null). */
+struct DST1 {
+  int N = 0, M = 0;
+  FFTPlan plan;
+  void init(int N_) { N = N_; M = N + 1; plan.init(2 * M); }
+  void pair(const double* xa, const double* xb, long stride, double* ya, double* yb,
+            long ostride, cplx* z, cplx* tmp) const {
+    z[0] = 0.0;
+    z[M] = 0.0;
+    for (int t = 1; t <= N; ++t) {
+      cplx c(xa[(long)(t - 1) * stride], xb ? xb[(long)(t - 1) * stride] : 0.0);
+      z[t] = c;
+      z[2 * M - t] = -c;
+    }
+    plan.exec(z, tmp);
+    for (int k = 1; k <= N; ++k) {
+      ya[(long)(k - 1) * ostride] = -z[k].imag();
+      if (yb) yb[(long)(k - 1) * ostride] = z[k].real();
+    }
+  }
+};
+
+int g_threads = 1;
+
+/* ------------------------------------------------------------------------ */
+/* simulation state                                                          */
+/* ------------------------------------------------------------------------ */
+
+struct Body {
+  int npts = 0, fam = 1;
+  double gamma0 = 0, alpha0 = 0, itheta = 0, ktheta = 0, ctheta = 0;
+  double deltas = 0, xbf0 = 0, ybf0 = 0;
+  std::vector<double> x, y;
+};
+
+}  // namespace
+
+struct orc_handle {
+  ib_params prm;
+  std::string err;
+  int m = 0, n = 0, L = 0;
+  double delta = 0, dt = 0, Re = 0, len = 0, offx = 0, offy = 0, pi = 0;
+  int P = 0, Fx = 0, Fy = 0, nbc = 0, left = 0, right = 0, bottom = 0, top = 0;
+  double normalize = 0;
+  std::vector<double> laminv, lam1, lam1i; /* lam1, lam1i: L consecutive P-blocks */
+  std::vector<double> vfac, con1, con2;
+  DST1 dx, dy;
+  /* state, level index 0..L-1 (= reference k-1) */
+  std::vector<std::vector<double>> omega, s, qx, qy, rhs_old, q0x, q0y;
+  /* bodies */
+  std::vector<Body> bdy;
+  int nb = 0, nf = 0;
+  std::vector<int> nb_ind, nbb_ind; /* nbb_ind[b] = end offset of body b in fvec (nbb_ind[0] = 0) */
+  std::vector<int> cbody, ccomp;    /* per force entry: body (0-based), component 1=x 2=y */
+  std::vector<double> xb, yb, f, f_rdst;
+  std::vector<int> tors; /* torsional bodies in deck order */
+  std::vector<double> theta, thetad, thetadd, theta0, thetad0, thetadd0, dtheta, Mt, Jt, Ct, Ftint;
+  /* regularisation (src/grid.F90:36-40) */
+  int support2 = 3;
+  std::vector<double> weight; /* 49 per force entry */
+  std::vector<int> reg_ix, reg_iy;
+  int itime = 0;
+  int ksp_iter = 0, fsi_iter = 0;
+  double err_fsi = 0;
+
+  /* ---------------- index helpers ---------------- */
+  inline long nd(int i, int j) const { return (long)(i - 2) + (long)(m - 1) * (j - 2); }
+  inline long fx(int i, int j) const { return (long)(i - 1) + (long)(m + 1) * (j - 1); }
+  inline long fy(int i, int j) const { return (long)(i - 1) + (long)m * (j - 1); }
+
+  int fail(int code, const std::string& msg) { err = msg; return code; }
+
+  /* ---------------- spectral operators (src/myfft.F90) ---------------- */
+
+  /* dst: src/myfft.F90:207-216; FFTW 2-D RODFT00 on (m-1) x (n-1) */
+  void dst2d(double* a) const {
+    const int N1 = m - 1, N2 = n - 1;
+    /* x pass: rows j (contiguous, length N1), two rows per transform */
+#pragma omp parallel num_threads(g_threads)
+    {
+      std::vector<cplx> z(2 * std::max(dx.M, dy.M)), t(2 * std::max(dx.M, dy.M));
+#pragma omp for schedule(static)
+      for (int jp = 0; jp < (N2 + 1) / 2; ++jp) {
+        int ja = 2 * jp, jb = ja + 1;
+        double* ra = a + (long)ja * N1;
+        double* rb = jb < N2 ? a + (long)jb * N1 : nullptr;
+        dx.pair(ra, rb, 1, ra, rb, 1, z.data(), t.data());
+      }
+      /* y pass: columns i (stride N1, length N2) */
+#pragma omp for schedule(static)
+      for (int ip = 0; ip < (N1 + 1) / 2; ++ip) {
+        int ia = 2 * ip, ib = ia + 1;
+        double* ca = a + ia;
+        double* cb = ib < N1 ? a + ib : nullptr;
+        dy.pair(ca, cb, N1, ca, cb, N1, z.data(), t.data());
+      }
+    }
+  }
+
+  /* ctci: src/myfft.F90:177-188 */
+  void ctci(const double* w, double* out) const {
+    std::vector<double> in(w, w + P);
+    dst2d(in.data());
+    for (int q = 0; q < P; ++q) in[q] = laminv[q] * in[q];
+    dst2d(in.data());
+    std::memcpy(out, in.data(), sizeof(double) * P);
+  }
+
+  /* ainv: src/myfft.F90:161-173 */
+  void ainv(double* v) const {
+    dst2d(v);
+    for (int q = 0; q < P; ++q) v[q] = lam1i[q] * v[q] / normalize;
+    dst2d(v);
+  }
+
+  /* ---------------- stencils (src/operators_fluid.F90) ---------------- */
+
+  /* rot_int + rot_bc: src/operators_fluid.F90:203-233 */
+  void rot(const double* hx, const double* hy, double* vort) const {
+    for (int j = 2; j <= n; ++j)
+      for (int i = 2; i <= m; ++i)
+        vort[nd(i, j)] = hy[fy(i, j)] - hy[fy(i - 1, j)] - hx[fx(i, j)] + hx[fx(i, j - 1)];
+  }
+
+  /* node value with boundary ghosts taken from a bc vector, in the order the
+   * reference overwrites them (left, right, bottom, top; corners end up from
+   * bottom/top): src/operators_fluid.F90:448-451 and :563-566 */
+  inline double ext(const double* a, const double* bc, int i, int j) const {
+    if (j == 1) return bc[bottom + i - 1];
+    if (j == n + 1) return bc[top + i - 1];
+    if (i == 1) return bc[left + j - 1];
+    if (i == m + 1) return bc[right + j - 1];
+    return a[nd(i, j)];
+  }
+
+  /* curl_int + curl_bc: src/operators_fluid.F90:423-471 */
+  void curl(const double* sv, const double* sbc, double* qxo, double* qyo) const {
+    for (int j = 1; j <= n; ++j)
+      for (int i = 1; i <= m + 1; ++i)
+        qxo[fx(i, j)] = ext(sv, sbc, i, j + 1) - ext(sv, sbc, i, j);
+    for (int j = 1; j <= n + 1; ++j)
+      for (int i = 1; i <= m; ++i)
+        qyo[fy(i, j)] = ext(sv, sbc, i, j) - ext(sv, sbc, i + 1, j);
+  }
+
+  /* coarsify: src/operators_fluid.F90:381-419 (fine odd nodes I,J = 3..m-1) */
+  void coarsify(const double* wf, double* wc) const {
+    for (int J = 3; J <= n - 1; J += 2)
+      for (int I = 3; I <= m - 1; I += 2) {
+        int ic = m / 4 + (I + 1) / 2, jc = n / 4 + (J + 1) / 2;
+        double v = wf[nd(I, J)] +
+                   0.5 * (wf[nd(I + 1, J)] + wf[nd(I, J + 1)] + wf[nd(I - 1, J)] + wf[nd(I, J - 1)]) +
+                   0.25 * (wf[nd(I + 1, J + 1)] + wf[nd(I + 1, J - 1)] + wf[nd(I - 1, J - 1)] +
+                           wf[nd(I - 1, J + 1)]);
+        wc[nd(ic, jc)] = v;
+      }
+  }
+
+  /* get_bc: src/operators_fluid.F90:475-535 with the slot map of
+   * setup_getbc (src/grid.F90:1091-1140) */
+  void get_bc(const double* sc, double* bc, double fac) const {
+    const int i0 = m / 4 + 1, i1 = 3 * m / 4 + 1, j0 = n / 4 + 1, j1 = 3 * n / 4 + 1;
+    for (int i = i0; i <= i1; ++i) {
+      int slot = 2 * (i - 1 - m / 4);
+      bc[bottom + slot] = sc[nd(i, j0)] * fac;
+      bc[top + slot] = sc[nd(i, j1)] * fac;
+      if (i != i1) {
+        bc[bottom + slot + 1] = (0.5 * (sc[nd(i, j0)] + sc[nd(i + 1, j0)])) * fac;
+        bc[top + slot + 1] = (0.5 * (sc[nd(i, j1)] + sc[nd(i + 1, j1)])) * fac;
+      }
+    }
+    for (int j = j0; j <= j1; ++j) {
+      int slot = 2 * (j - 1 - n / 4);
+      bc[left + slot] = sc[nd(i0, j)] * fac;
+      bc[right + slot] = sc[nd(i1, j)] * fac;
+      if (j != j1) {
+        bc[left + slot + 1] = (0.5 * (sc[nd(i0, j)] + sc[nd(i0, j + 1)])) * fac;
+        bc[right + slot + 1] = (0.5 * (sc[nd(i1, j)] + sc[nd(i1, j + 1)])) * fac;
+      }
+    }
+  }
+
+  /* apply_bc: src/operators_fluid.F90:539-549 */
+  void apply_bc(double* v, const double* bc, double fac) const {
+    for (int j = 2; j <= n; ++j) v[nd(2, j)] = v[nd(2, j)] + fac * bc[left + j - 1];
+    for (int j = 2; j <= n; ++j) v[nd(m, j)] = v[nd(m, j)] + fac * bc[right + j - 1];
+    for (int i = 2; i <= m; ++i) v[nd(i, 2)] = v[nd(i, 2)] + fac * bc[bottom + i - 1];
+    for (int i = 2; i <= m; ++i) v[nd(i, n)] = v[nd(i, n)] + fac * bc[top + i - 1];
+  }
+
+  /* nonlinear: src/operators_fluid.F90:553-587.  qhx/qhy are q + q0 on the
+   * DMDA interior; their physical-boundary ghosts are 0 (never written). */
+  void nonlinear(const double* w, const double* qhx, const double* qhy, const double* bc,
+                 double* nlx, double* nly) const {
+    auto QX = [&](int i, int j) -> double { return (j < 1 || j > n) ? 0.0 : qhx[fx(i, j)]; };
+    auto QY = [&](int i, int j) -> double { return (i < 1 || i > m) ? 0.0 : qhy[fy(i, j)]; };
+    auto uavg = [&](int i, int j) { return 0.5 * (QX(i, j) + QX(i, j - 1)); };
+    auto vavg = [&](int i, int j) { return 0.5 * (QY(i, j) + QY(i - 1, j)); };
+    for (int j = 1; j <= n; ++j)
+      for (int i = 1; i <= m + 1; ++i)
+        nlx[fx(i, j)] = 0.5 * (vavg(i, j + 1) * ext(w, bc, i, j + 1) + vavg(i, j) * ext(w, bc, i, j));
+    for (int j = 1; j <= n + 1; ++j)
+      for (int i = 1; i <= m; ++i)
+        nly[fy(i, j)] = -0.5 * (uavg(i + 1, j) * ext(w, bc, i + 1, j) + uavg(i, j) * ext(w, bc, i, j));
+  }
+
+  /* fluid_predictor: src/operators_fluid.F90:649-663 (level index k0 = k-1) */
+  void predictor(double* w, const double* rhs, const double* rold, const double* rhsbc, int k0) const {
+    std::vector<double> a(P), b(w, w + P);
+    for (int q = 0; q < P; ++q) a[q] = con1[k0] * rhs[q] + con2[k0] * rold[q] + rhsbc[q];
+    dst2d(a.data());
+    dst2d(b.data());
+    const double* l1 = &lam1[(long)k0 * P];
+    const double* l1i = &lam1i[(long)k0 * P];
+    for (int q = 0; q < P; ++q) a[q] = l1i[q] * (a[q] + l1[q] * b[q]);
+    dst2d(a.data());
+    std::memcpy(w, a.data(), sizeof(double) * P);
+  }
+
+  /* vort2flux: src/operators_fluid.F90:237-377.  w is modified by coarsify. */
+  void vort2flux(std::vector<std::vector<double>>& w, std::vector<std::vector<double>>& so,
+                 std::vector<std::vector<double>>& qxo, std::vector<std::vector<double>>& qyo) const {
+    for (int k = 1; k < L; ++k) coarsify(w[k - 1].data(), w[k].data());
+    std::vector<double> sbc(nbc, 0.0), wk(P);
+    ctci(w[L - 1].data(), so[L - 1].data());
+    curl(so[L - 1].data(), sbc.data(), qxo[L - 1].data(), qyo[L - 1].data());
+    if (L > 1) get_bc(so[L - 1].data(), sbc.data(), 1.0);
+    for (int k = L - 2; k >= 0; --k) {
+      wk.assign(w[k].begin(), w[k].end());
+      apply_bc(wk.data(), sbc.data(), 1.0);
+      ctci(wk.data(), so[k].data());
+      curl(so[k].data(), sbc.data(), qxo[k].data(), qyo[k].data());
+      if (k > 0) get_bc(so[k].data(), sbc.data(), 1.0);
+    }
+  }
+
+  /* ---------------- IB coupling (src/grid.F90, src/fsinterface.F90) -------- */
+
+  /* deltafnc: src/grid.F90:955-1003 */
+  static double deltafnc(double x1, double x2, double dr, bool f32) {
+    double r = std::fabs(x1 - x2);
+    double r1 = r / dr, r2 = r1 * r1, r3 = r2 * r1, r4 = r3 * r1;
+    double d;
+    /* Exponent-less literals are default REAL(4) in the reference source;
+     * F(x) yields that value (f32) or the decimal value as double. */
+#define F(lit) (f32 ? (double)(lit##f) : (lit))
+    if (r1 <= 1.0) {
+      double a5 = std::asin((1.0 / 2.0) * std::sqrt(3.0) * (2.0 * r1 - 1.0));
+      double a8 = std::sqrt(1.0 - 12.0 * r2 + 12.0 * r1);
+      d = 0.4166666667e-1 * r4 + (F(-.1388888889) + 0.3472222222e-1 * a8) * r3 +
+          (-0.7121664902e-1 - 0.5208333333e-1 * a8 + F(0.2405626122) * a5) * r2 +
+          (F(-.2405626122) * a5 + F(-.3792313933) + F(.1012731481) * a8) * r1 + 0.8018753741e-1 * a5 -
+          0.4195601852e-1 * a8 + F(.6485698427);
+    } else if (r1 <= 2.0) {
+      double a6 = std::asin((1.0 / 2.0) * std::sqrt(3.0) * (-3.0 + 2.0 * r1));
+      double a9 = std::sqrt(-23.0 + 36.0 * r1 - 12.0 * r2);
+      d = -0.6250000000e-1 * r4 + (F(.4861111111) - 0.1736111111e-1 * a9) * r3 +
+          (F(-1.143175026) + 0.7812500000e-1 * a9 + F(-.1202813061) * a6) * r2 +
+          (F(.8751991178) + F(.3608439183) * a6 + F(-.1548032407) * a9) * r1 + F(-.2806563809) * a6 +
+          0.822848104e-2 + F(.1150173611) * a9;
+    } else if (r1 <= 3.0) {
+      double a1 = std::asin((1.0 / 2.0 * (2.0 * r1 - 5.0)) * std::sqrt(3.0));
+      double a7 = std::sqrt(-71.0 - 12.0 * r2 + 60.0 * r1);
+      d = 0.2083333333e-1 * r4 + (0.3472222222e-2 * a7 + F(-.2638888889)) * r3 +
+          (F(1.214391675) - 0.2604166667e-1 * a7 + 0.2405626122e-1 * a1) * r2 +
+          (F(-.1202813061) * a1 + F(-2.449273192) + 0.7262731481e-1 * a7) * r1 + F(.1523563211) * a1 +
+          F(1.843201677) - 0.7306134259e-1 * a7;
+    } else {
+      d = 0.0;
+    }
+#undef F
+    return d / dr;
+  }
+
+  /* setup_reg: src/grid.F90:459-528 */
+  int setup_reg() {
+    const int s2 = support2, nt = (2 * s2 + 1) * (2 * s2 + 1);
+    const double d2 = 0.5 * delta;
+    const bool f32 = prm.deltafnc_f32_literals != 0;
+    weight.assign((size_t)nt * nf, 0.0);
+    reg_ix.clear();
+    reg_iy.clear();
+    for (int k = 0; k < nf; ++k) {
+      int ix = (int)((xb[k] + offx) / delta) + 1;
+      int iy = (int)((yb[k] + offy) / delta) + 1;
+      int next = 0;
+      for (int l = -s2; l <= s2; ++l)
+        for (int p = -s2; p <= s2; ++p) {
+          double x, y;
+          if (ccomp[k] == 1) {
+            x = delta * (ix - 1 + p) - offx;
+            y = delta * (iy - 1 + l) - offy + d2;
+            int ii = ix + p, jj = iy + l; /* face (ii, jj) */
+            if (ii < 1 || ii > m + 1 || jj < 1 || jj > n)
+              return fail(IB_E_INVAL, "body point regularisation stencil leaves the x-face grid");
+            reg_ix.push_back((m + 1) * (iy + l - 1) + (ix + p - 1));
+          } else {
+            x = delta * (ix - 1 + p) - offx + d2;
+            y = delta * (iy - 1 + l) - offy;
+            int ii = ix + p, jj = iy + l;
+            if (ii < 1 || ii > m || jj < 1 || jj > n + 1)
+              return fail(IB_E_INVAL, "body point regularisation stencil leaves the y-face grid");
+            reg_iy.push_back(m * (iy + l - 1) + (ix + p - 1));
+          }
+          weight[(size_t)k * nt + next] = delta * delta * deltafnc(x, xb[k], delta, f32) * deltafnc(y, yb[k], delta, f32);
+          ++next;
+        }
+    }
+    return IB_OK;
+  }
+
+  /* reg: src/fsinterface.F90:14-47; hx, hy accumulate (caller zeroes) */
+  void reg(const double* h0, double* hx, double* hy) const {
+    const int nt = (2 * support2 + 1) * (2 * support2 + 1);
+    size_t itx = 0, ity = 0;
+    for (int k = 0; k < nf; ++k)
+      for (int t = 0; t < nt; ++t) {
+        double v = weight[(size_t)k * nt + t] * h0[k];
+        if (ccomp[k] == 1) { hx[reg_ix[itx]] += v; ++itx; }
+        else { hy[reg_iy[ity]] += v; ++ity; }
+      }
+  }
+
+  /* regT: src/fsinterface.F90:51-80; h0 accumulates */
+  void regT(const double* hx, const double* hy, double* h0) const {
+    const int nt = (2 * support2 + 1) * (2 * support2 + 1);
+    size_t itx = 0, ity = 0;
+    for (int k = 0; k < nf; ++k)
+      for (int t = 0; t < nt; ++t) {
+        if (ccomp[k] == 1) { h0[k] = h0[k] + weight[(size_t)k * nt + t] * hx[reg_ix[itx]]; ++itx; }
+        else { h0[k] = h0[k] + weight[(size_t)k * nt + t] * hy[reg_iy[ity]]; ++ity; }
+      }
+  }
+
+  /* ---------------- torsional bodies (src/operators_struct.F90) ---------- */
+
+  /* BR_build: src/operators_struct.F90:319-338 (j = 1..2*npts within body) */
+  double BRval(const Body& b, int j, double th) const {
+    if (j <= b.npts) return (j - 1) * b.deltas * std::sin(b.gamma0 - th);
+    return -(j - b.npts - 1) * b.deltas * std::cos(b.gamma0 - th);
+  }
+
+  /* b_times2: src/operators_struct.F90:14-127 (rank-1 dense form evaluated
+   * row by row as in MatMult of sol_Jmat) */
+  void b_times2(const double* x, double* y) const {
+    for (int t = 0; t < (int)tors.size(); ++t) {
+      const Body& b = bdy[tors[t]];
+      const int off = nbb_ind[tors[t]];
+      const int nn = 2 * b.npts;
+      const double si = std::sin(b.gamma0 - theta[t]), ci = std::cos(b.gamma0 - theta[t]);
+      const double den = 4.0 / std::pow(dt, 2.0) * Mt[t] + 2.0 / dt * Ct[t] + Jt[t];
+      const double sc = (delta * delta / b.deltas / dt) * (2 * b.deltas / dt);
+      for (int j = 1; j <= nn; ++j) {
+        const int im = j - 1;
+        double acc = 0.0;
+        for (int k = 1; k <= nn; ++k) {
+          const int in = k - 1;
+          double v;
+          if (j <= b.npts) {
+            if (k <= b.npts) v = (double)(im * in) * (si * si) * (b.deltas * b.deltas);
+            else v = -(double)(im * (in - b.npts)) * si * ci * (b.deltas * b.deltas);
+          } else {
+            if (k <= b.npts) v = -(double)((im - b.npts) * in) * si * ci * (b.deltas * b.deltas);
+            else v = (double)((im - b.npts) * (in - b.npts)) * (ci * ci) * (b.deltas * b.deltas);
+          }
+          v = v / den;
+          acc += v * x[off + k - 1];
+        }
+        y[off + j - 1] = acc * sc;
+      }
+    }
+  }
+
+  /* compute_rhs: src/operators_struct.F90:342-398 (gravity = dynamics = 0,
+   * src/user.F90 compute_gravity/compute_dynamics with g_param = 0) */
+  void compute_rhs(double* y) const {
+    for (int t = 0; t < (int)tors.size(); ++t) {
+      const Body& b = bdy[tors[t]];
+      const int off = nbb_ind[tors[t]];
+      const double grav = 0.0, dyn = 0.0;
+      double rth = thetad0[t] + thetad[t] + 2.0 / dt * (theta0[t] - theta[t]);
+      double den = 4.0 / std::pow(dt, 2.0) * Mt[t] + 2.0 / dt * Ct[t] + Jt[t];
+      double jr = 2.0 / dt / den *
+                  (Mt[t] * (4.0 / (dt * dt) * theta0[t] + 4.0 / dt * thetad0[t] + thetadd0[t]) +
+                   Ct[t] * (thetad0[t] + 2.0 / dt * theta0[t]) -
+                   (4.0 / (dt * dt) * Mt[t] + 2.0 / dt * Ct[t] + Jt[t]) * theta[t] + grav + dyn);
+      double sc = delta * (rth - jr - thetad[t]);
+      for (int j = 1; j <= 2 * b.npts; ++j) y[off + j - 1] = BRval(b, j, theta[t]) * sc;
+    }
+  }
+
+  /* compute_dx + update_xb: src/operators_struct.F90:402-505, :600-620 */
+  double compute_dx() {
+    double e = 0.0;
+    for (int t = 0; t < (int)tors.size(); ++t) {
+      Body& b = bdy[tors[t]];
+      const int off = nbb_ind[tors[t]];
+      const double grav = 0.0, dyn = 0.0;
+      double jr = 1.0 / (4.0 / (dt * dt) * Mt[t] + 2.0 / dt * Ct[t] + Jt[t]) *
+                  (Mt[t] * (4 / (dt * dt) * theta0[t] + 4 / dt * thetad0[t] + thetadd0[t]) +
+                   Ct[t] * (thetad0[t] + 2.0 / dt * theta0[t]) -
+                   (4 / (dt * dt) * Mt[t] + 2.0 / dt * Ct[t] + Jt[t]) * theta[t] + grav + dyn);
+      double brf = 0.0;
+      for (int j = 1; j <= 2 * b.npts; ++j) brf += BRval(b, j, theta[t]) * f[off + j - 1];
+      brf = b.deltas / (4 / (dt * dt) * Mt[t] + Jt[t]) * brf * (delta / b.deltas / dt);
+      dtheta[t] = jr + brf;
+      e = std::max(e, std::fabs(dtheta[t]));
+      theta[t] = theta[t] + dtheta[t];
+      thetad[t] = -thetad0[t] + 2 / dt * (theta[t] - theta0[t]);
+      thetadd[t] = -thetadd0[t] - 4 / dt * thetad0[t] + 4 / (dt * dt) * (theta[t] - theta0[t]);
+      const double c = std::cos(b.gamma0 - theta[t]), s_ = std::sin(b.gamma0 - theta[t]);
+      for (int j = 1; j <= 2 * b.npts; ++j) {
+        int jj = j <= b.npts ? j : j - b.npts;
+        xb[off + j - 1] = b.xbf0 + (jj - 1) * b.deltas * c;
+        yb[off + j - 1] = b.ybf0 + (jj - 1) * b.deltas * s_;
+      }
+    }
+    return e;
+  }
+
+  /* initial_acc: src/operators_struct.F90:233-294 */
+  void initial_acc() {
+    for (int t = 0; t < (int)tors.size(); ++t) {
+      const Body& b = bdy[tors[t]];
+      const int off = nbb_ind[tors[t]];
+      double brf = 0.0;
+      for (int j = 1; j <= 2 * b.npts; ++j) brf += BRval(b, j, theta[t]) * f[off + j - 1];
+      brf = brf * delta / b.deltas / dt;
+      thetadd[t] = 1 / Mt[t] * (-Ct[t] * thetad[t] - Ftint[t] + brf * b.deltas + 0.0 + 0.0);
+    }
+  }
+
+  /* ---------------- Schur matvec and GMRES ---------------- */
+
+  /* a_times2 + ab_times: src/operators_fluid.F90:14-120, src/timestepper.F90:774-782 */
+  void schur(const double* x, double* y) const {
+    std::vector<double> hx(Fx, 0.0), hy(Fy, 0.0), y2(nf, 0.0), y1(nf, 0.0);
+    reg(x, hx.data(), hy.data());
+    b_times2(x, y2.data());
+    std::vector<std::vector<double>> vv(L, std::vector<double>(P, 0.0)), sv(L, std::vector<double>(P)),
+        vx(L, std::vector<double>(Fx)), vy(L, std::vector<double>(Fy));
+    rot(hx.data(), hy.data(), vv[0].data());
+    ainv(vv[0].data());
+    vort2flux(vv, sv, vx, vy);
+    regT(vx[0].data(), vy[0].data(), y1.data());
+    for (int k = 0; k < nf; ++k) y[k] = 1.0 * y1[k] + y2[k];
+  }
+
+  static double dot(const double* a, const double* b, int N) {
+    double s = 0.0;
+    for (int i = 0; i < N; ++i) s += a[i] * b[i];
+    return s;
+  }
+
+  /* PETSc 3.11 KSPGMRES as configured in setup_ksp (src/variables.F90:565-574):
+   * PCNONE, unpreconditioned norm, nonzero initial guess, classical
+   * Gram-Schmidt without refinement, restart gmres_restart, convergence
+   * ||r|| <= max(rtol*||b||, atol) on the Givens residual estimate. */
+  int gmres(const double* b, double* x, int* its_out) const {
+    const int N = nf, kmax = prm.gmres_restart, maxit = prm.gmres_max_it;
+    std::vector<std::vector<double>> V(kmax + 1, std::vector<double>(N));
+    std::vector<double> H((size_t)(kmax + 1) * kmax, 0.0), cc(kmax), ss(kmax), grs(kmax + 1), w(N);
+    auto HH = [&](int r, int c) -> double& { return H[(size_t)c * (kmax + 1) + r]; };
+    const double bnorm = std::sqrt(dot(b, b, N));
+    double ttol = 0.0, rn0 = 0.0;
+    int its = 0;
+    int reason = 0; /* 0 iterating, 1 converged, -1 diverged */
+    bool first = true;
+    while (!reason) {
+      schur(x, w.data());
+      for (int i = 0; i < N; ++i) V[0][i] = b[i] - w[i];
+      double res = std::sqrt(dot(V[0].data(), V[0].data(), N));
+      grs[0] = res;
+      if (first) {
+        rn0 = bnorm > 0 ? bnorm : res;
+        ttol = std::max(prm.rtol * rn0, prm.atol);
+        first = false;
+      }
+      if (res == 0.0) { reason = 1; break; }
+      if (res <= ttol) { reason = 1; break; }
+      if (res >= 1e4 * rn0) { reason = -1; break; }
+      for (int i = 0; i < N; ++i) V[0][i] /= res;
+      int it = 0;
+      bool hapend = false;
+      while (!reason && it < kmax && its < maxit) {
+        schur(V[it].data(), V[it + 1].data());
+        double lhh[1024];
+        std::vector<double> lbig;
+        double* lh = lhh;
+        if (it + 1 > 1024) { lbig.resize(it + 1); lh = lbig.data(); }
+        for (int j = 0; j <= it; ++j) lh[j] = dot(V[it + 1].data(), V[j].data(), N);
+        for (int i = 0; i < N; ++i) {
+          double acc = V[it + 1][i];
+          for (int j = 0; j <= it; ++j) acc += -lh[j] * V[j][i];
+          V[it + 1][i] = acc;
+        }
+        for (int j = 0; j <= it; ++j) HH(j, it) = lh[j];
+        double tt = std::sqrt(dot(V[it + 1].data(), V[it + 1].data(), N));
+        HH(it + 1, it) = tt;
+        double hapbnd = std::fabs(tt / grs[it]);
+        if (hapbnd > 1e-30) hapbnd = 1e-30;
+        if (tt < hapbnd) hapend = true;
+        else for (int i = 0; i < N; ++i) V[it + 1][i] = V[it + 1][i] * (1.0 / tt);
+        /* KSPGMRESUpdateHessenberg */
+        for (int j = 0; j < it; ++j) {
+          double t0 = HH(j, it);
+          HH(j, it) = cc[j] * t0 + ss[j] * HH(j + 1, it);
+          HH(j + 1, it) = -ss[j] * t0 + cc[j] * HH(j + 1, it);
+        }
+        if (!hapend) {
+          double t1 = std::sqrt(HH(it, it) * HH(it, it) + HH(it + 1, it) * HH(it + 1, it));
+          cc[it] = HH(it, it) / t1;
+          ss[it] = HH(it + 1, it) / t1;
+          grs[it + 1] = -ss[it] * grs[it];
+          grs[it] = cc[it] * grs[it];
+          HH(it, it) = cc[it] * HH(it, it) + ss[it] * HH(it + 1, it);
+          res = std::fabs(grs[it + 1]);
+        } else {
+          res = 0.0;
+        }
+        ++it;
+        ++its;
+        if (res <= ttol) reason = 1;
+        else if (res >= 1e4 * rn0) reason = -1;
+        if (hapend) { if (!reason) reason = 1; break; }
+      }
+      /* KSPGMRESBuildSoln */
+      if (it > 0) {
+        std::vector<double> nrs(it);
+        int top = it - 1;
+        nrs[top] = grs[top] / HH(top, top);
+        for (int ii = 1; ii <= top; ++ii) {
+          int k = top - ii;
+          double t = grs[k];
+          for (int j = k + 1; j <= top; ++j) t = t - HH(k, j) * nrs[j];
+          nrs[k] = t / HH(k, k);
+        }
+        for (int i = 0; i < N; ++i) {
+          double acc = 0.0;
+          for (int j = 0; j <= top; ++j) acc += nrs[j] * V[j][i];
+          x[i] = x[i] + acc;
+        }
+      }
+      if (!reason && its >= maxit) { reason = -2; break; }
+    }
+    *its_out = its;
+    return reason > 0 ? IB_OK : IB_E_NOCONV_GMRES;
+  }
+
+  /* redistribute + redistribute2: src/operators_struct.F90:735-834 */
+  void redistribute() {
+    std::vector<double> ones(nf, 1.0), wx(Fx, 0.0), wy(Fy, 0.0), frx(Fx, 0.0), fry(Fy, 0.0), fix(Fx, 0.0),
+        fiy(Fy, 0.0);
+    reg(ones.data(), wx.data(), wy.data());
+    reg(f.data(), frx.data(), fry.data());
+    for (int q = 0; q < Fx; ++q)
+      if (wx[q] > 1e-10) fix[q] = 1.0 / wx[q] * frx[q];
+    for (int q = 0; q < Fy; ++q)
+      if (wy[q] > 1e-10) fiy[q] = 1.0 / wy[q] * fry[q];
+    std::fill(f_rdst.begin(), f_rdst.end(), 0.0);
+    regT(fix.data(), fiy.data(), f_rdst.data());
+  }
+
+  /* ---------------- one time step: src/timestepper.F90:18-556 ----------- */
+  int advance() {
+    itime += 1;
+    /* motion_potential/motion_rotation (src/variables.F90:299-360) with
+     * motion_grid = (-1, 0, 0, 0, 0) (src/user.F90:13-31) */
+    for (int k = 0; k < L; ++k) {
+      double fac = delta * std::pow(2.0, (double)k);
+      std::fill(q0x[k].begin(), q0x[k].end(), -fac * (-1.0) + 0.0);
+      std::fill(q0y[k].begin(), q0y[k].end(), -fac * 0.0 + 0.0);
+    }
+    /* step 1: predictor on every level, coarse to fine */
+    std::vector<std::vector<double>> lastbc(L, std::vector<double>(nbc, 0.0));
+    for (int k = 0; k < L - 1; ++k) get_bc(omega[k + 1].data(), lastbc[k].data(), 0.25);
+    std::vector<double> omegabc(nbc), rhs(P), rhsbc(P), nlx(Fx), nly(Fy), qhx(Fx), qhy(Fy);
+    for (int k = L - 1; k >= 0; --k) {
+      for (int q = 0; q < Fx; ++q) qhx[q] = qx[k][q] + q0x[k][q];
+      for (int q = 0; q < Fy; ++q) qhy[q] = qy[k][q] + q0y[k][q];
+      if (k < L - 1) get_bc(omega[k + 1].data(), omegabc.data(), 0.25);
+      else std::fill(omegabc.begin(), omegabc.end(), 0.0);
+      nonlinear(omega[k].data(), qhx.data(), qhy.data(), lastbc[k].data(), nlx.data(), nly.data());
+      std::fill(rhsbc.begin(), rhsbc.end(), 0.0);
+      apply_bc(rhsbc.data(), lastbc[k].data(), vfac[k]);
+      apply_bc(rhsbc.data(), omegabc.data(), vfac[k]);
+      rot(nlx.data(), nly.data(), rhs.data());
+      if (itime == 1) rhs_old[k] = rhs;
+      predictor(omega[k].data(), rhs.data(), rhs_old[k].data(), rhsbc.data(), k);
+      rhs_old[k] = rhs;
+    }
+    vort2flux(omega, s, qx, qy);
+    for (int q = 0; q < Fx; ++q) qhx[q] = qx[0][q] + q0x[0][q];
+    for (int q = 0; q < Fy; ++q) qhy[q] = qy[0][q] + q0y[0][q];
+
+    /* step 2: FSI fixed point with GMRES (num_stat = F, sub_domain = F) */
+    int status = IB_OK;
+    int fsi_count = 0;
+    double err_glob = 10.0;
+    ksp_iter = 0;
+    for (size_t t = 0; t < tors.size(); ++t) {
+      const Body& b = bdy[tors[t]];
+      Mt[t] = b.itheta;          /* get_M */
+      Jt[t] = b.ktheta;          /* get_JK */
+      Ct[t] = b.ctheta;          /* get_C */
+      Ftint[t] = Jt[t] * theta[t]; /* var_update */
+    }
+    if (itime == 1) initial_acc();
+    theta0 = theta;
+    thetad0 = thetad;
+    thetadd0 = thetadd;
+    std::vector<double> rhsf(nf), yb_(nf);
+    while (err_glob >= prm.fsi_tol && fsi_count <= prm.fsi_max_count) {
+      ++fsi_count;
+      std::fill(rhsf.begin(), rhsf.end(), 0.0);
+      std::fill(yb_.begin(), yb_.end(), 0.0);
+      compute_rhs(yb_.data());
+      for (int k = 0; k < nf; ++k) rhsf[k] += yb_[k];
+      regT(qhx.data(), qhy.data(), rhsf.data());
+      int its = 0;
+      int rc = gmres(rhsf.data(), f.data(), &its);
+      ksp_iter += its;
+      if (rc != IB_OK) status = rc;
+      redistribute();
+      double e = compute_dx();
+      err_glob = e;
+      if (err_glob >= prm.fsi_tol)
+        for (size_t t = 0; t < tors.size(); ++t) Ftint[t] = Jt[t] * theta[t];
+      int r2 = setup_reg();
+      if (r2 != IB_OK) return r2;
+    }
+    if (err_glob >= prm.fsi_tol && status == IB_OK) status = IB_E_NOCONV_FSI;
+    fsi_iter = fsi_count;
+    err_fsi = err_glob;
+
+    /* step 3: corrector on level 1 */
+    std::vector<double> hx(Fx, 0.0), hy(Fy, 0.0), vort(P);
+    reg(f.data(), hx.data(), hy.data());
+    rot(hx.data(), hy.data(), vort.data());
+    ainv(vort.data());
+    for (int q = 0; q < P; ++q) omega[0][q] = omega[0][q] - vort[q];
+    vort2flux(omega, s, qx, qy);
+    return status;
+  }
+
+  /* ---------------- setup: src/grid.F90:117-434, src/myfft.F90:237-298 --- */
+  int init(const ib_params* p, int nbodies, const ib_body* bodies) {
+    prm = *p;
+    if (prm.gmres_restart <= 0) prm.gmres_restart = 30;
+    if (prm.gmres_max_it <= 0) prm.gmres_max_it = 10000;
+    if (prm.fsi_max_count <= 0) prm.fsi_max_count = 1000;
+    m = prm.m; n = prm.n; L = prm.mgridlev;
+    if (m < 8 || n < 8 || m % 4 || n % 4) return fail(IB_E_INVAL, "grid must be divisible by 4 (m, n >= 8)");
+    if (L < 1) return fail(IB_E_INVAL, "mgridlev must be >= 1");
+    if (prm.num_stat || prm.sub_domain) return fail(IB_E_UNSUPPORTED, "num_stat and sub_domain paths are not built");
+    if (prm.istart != 0) return fail(IB_E_UNSUPPORTED, "restart from .var is not built");
+    pi = 4.0 * std::atan(1.0);
+    dt = prm.dt; Re = prm.re; len = prm.len; offx = prm.offsetx; offy = prm.offsety;
+    delta = len / (double)m;
+    P = (m - 1) * (n - 1); Fx = (m + 1) * n; Fy = m * (n + 1);
+    nbc = 2 * (m + 1) + 2 * (n + 1);
+    left = 0; right = n + 1; bottom = 2 * (n + 1); top = 2 * (n + 1) + m + 1;
+    dx.init(m - 1);
+    dy.init(n - 1);
+    /* setup_fft tables (src/myfft.F90:262-289) */
+    normalize = 4.0 * (double)(float)(m * n);
+    const double del2 = delta * delta;
+    vfac.resize(L); con1.resize(L); con2.resize(L);
+    for (int k = 0; k < L; ++k) {
+      double del22 = del2 * std::pow(4.0, (double)k);
+      vfac[k] = 0.5 * dt / Re / del22 / normalize;
+      con1[k] = 1.5 * dt / del22 / normalize;
+      con2[k] = -0.5 * dt / del22 / normalize;
+    }
+    laminv.resize(P); lam1.resize((size_t)P * L); lam1i.resize((size_t)P * L);
+    for (int j = 2; j <= n; ++j)
+      for (int i = 2; i <= m; ++i) {
+        double lam = -2.0 * (std::cos(pi * (double)(i - 1) / (double)m) + std::cos(pi * (double)(j - 1) / (double)n) - 2.0);
+        long q = nd(i, j);
+        laminv[q] = 1.0 / lam / normalize;
+        for (int k = 0; k < L; ++k) {
+          double del22 = del2 * std::pow(4.0, (double)k);
+          lam1[(size_t)k * P + q] = (1.0 - 0.5 * dt * lam / del22 / Re) / normalize;
+          lam1i[(size_t)k * P + q] = 1.0 / (1.0 + 0.5 * dt * lam / del22 / Re);
+        }
+      }
+    /* bodies: setup_geometry (src/grid.F90:239-398) */
+    if (nbodies < 0) return fail(IB_E_INVAL, "negative body count");
+    bool seen_tors = false;
+    nbb_ind.assign(1, 0);
+    for (int b = 0; b < nbodies; ++b) {
+      Body B;
+      B.npts = bodies[b].npts;
+      B.fam = bodies[b].fam;
+      if (B.npts < 2) return fail(IB_E_INVAL, "a body needs at least 2 points");
+      if (B.fam != 1 && B.fam != 2) return fail(IB_E_UNSUPPORTED, "only body families 1 (rigid) and 2 (torsional) exist in the reference");
+      if (B.fam == 1 && seen_tors) return fail(IB_E_INVAL, "rigid bodies must precede torsional bodies (src/grid.F90:1226)");
+      if (B.fam == 2) seen_tors = true;
+      B.gamma0 = bodies[b].gamma0; B.alpha0 = bodies[b].alpha0;
+      B.itheta = bodies[b].itheta; B.ktheta = bodies[b].ktheta; B.ctheta = bodies[b].ctheta;
+      B.x.assign(bodies[b].x, bodies[b].x + B.npts);
+      B.y.assign(bodies[b].y, bodies[b].y + B.npts);
+      B.deltas = std::sqrt((B.x[1] - B.x[0]) * (B.x[1] - B.x[0]) + (B.y[1] - B.y[0]) * (B.y[1] - B.y[0]));
+      B.xbf0 = B.x[0]; B.ybf0 = B.y[0];
+      if (B.fam == 2 && B.itheta <= 0) return fail(IB_E_INVAL, "torsional body needs itheta > 0");
+      if (B.fam == 2) tors.push_back(b);
+      bdy.push_back(B);
+      nb_ind.push_back(B.npts);
+      nbb_ind.push_back(nbb_ind.back() + 2 * B.npts);
+      nb += B.npts;
+    }
+    nf = 2 * nb;
+    /* collect_bodies: src/grid.F90:403-433 */
+    xb.resize(nf); yb.resize(nf); cbody.resize(nf); ccomp.resize(nf);
+    for (int b = 0; b < nbodies; ++b)
+      for (int j = 0; j < 2 * bdy[b].npts; ++j) {
+        int k = nbb_ind[b] + j;
+        int pnt = j < bdy[b].npts ? j : j - bdy[b].npts;
+        cbody[k] = b;
+        ccomp[k] = j < bdy[b].npts ? 1 : 2;
+        xb[k] = bdy[b].x[pnt];
+        yb[k] = bdy[b].y[pnt];
+      }
+    f.assign(nf, 0.0); f_rdst.assign(nf, 0.0);
+    size_t nt = tors.size();
+    theta.assign(nt, 0.0); thetad.assign(nt, 0.0); thetadd.assign(nt, 0.0);
+    theta0 = theta; thetad0 = theta; thetadd0 = theta; dtheta = theta;
+    Mt = theta; Jt = theta; Ct = theta; Ftint = theta;
+    /* get_M, get_JK, get_C, var_update (src/operators_struct.F90:131-229) hold
+     * deck constants; set them now so single operators work before a step */
+    for (size_t t = 0; t < nt; ++t) {
+      Mt[t] = bdy[tors[t]].itheta; Jt[t] = bdy[tors[t]].ktheta; Ct[t] = bdy[tors[t]].ctheta;
+      Ftint[t] = Jt[t] * theta[t];
+    }
+    int rc = setup_reg();
+    if (rc != IB_OK) return rc;
+    /* initial_condition: src/variables.F90:262-295 */
+    omega.assign(L, std::vector<double>(P, 0.0));
+    s.assign(L, std::vector<double>(P, 0.0));
+    rhs_old.assign(L, std::vector<double>(P, 0.0));
+    qx.assign(L, std::vector<double>(Fx, 0.0));
+    qy.assign(L, std::vector<double>(Fy, 0.0));
+    q0x.assign(L, std::vector<double>(Fx, 0.0));
+    q0y.assign(L, std::vector<double>(Fy, 0.0));
+    itime = prm.istart;
+    return IB_OK;
+  }
+};
+
+/* ------------------------------------------------------------------------ */
+/* C ABI                                                                     */
+/* ------------------------------------------------------------------------ */
+
+static std::string g_create_err;
+
+extern "C" {
+
+int orc_create(const ib_params* p, int32_t nbodies, const ib_body* bodies, orc_handle** out) {
+  if (!p || !out) return IB_E_INVAL;
+  orc_handle* h = new (std::nothrow) orc_handle();
+  if (!h) return IB_E_INVAL;
+  int rc = h->init(p, nbodies, bodies);
+  if (rc != IB_OK) {
+    g_create_err = h->err;
+    delete h;
+    *out = nullptr;
+    return rc;
+  }
+  *out = h;
+  return IB_OK;
+}
+
+int orc_destroy(orc_handle* h) {
+  delete h;
+  return IB_OK;
+}
+
+int orc_get_sizes(const orc_handle* h, ib_sizes* o) {
+  o->m = h->m; o->n = h->n; o->mgridlev = h->L; o->nbc = h->nbc;
+  o->n_nodes = h->P; o->n_xfaces = h->Fx; o->n_yfaces = h->Fy;
+  o->nb = h->nb; o->nf = h->nf; o->nbodies = (int)h->bdy.size(); o->ntors = (int)h->tors.size();
+  return IB_OK;
+}
+
+int orc_last_error(const orc_handle* h, char* buf, int32_t len) {
+  const std::string& e = h ? h->err : g_create_err;
+  if (buf && len > 0) {
+    std::snprintf(buf, (size_t)len, "%s", e.c_str());
+  }
+  return (int)e.size();
+}
+
+int orc_advance(orc_handle* h, int32_t* itime, ib_step_info* info) {
+  int rc = h->advance();
+  if (itime) *itime = h->itime;
+  if (info) {
+    info->itime = h->itime; info->ksp_iter = h->ksp_iter; info->fsi_iter = h->fsi_iter;
+    info->status = rc; info->err_fsi = h->err_fsi;
+  }
+  return rc;
+}
+
+int orc_get_level_fields(orc_handle* h, int32_t k, double* omega, double* s, double* qx, double* qy,
+                         double* rhs_old) {
+  if (k < 1 || k > h->L) return IB_E_INVAL;
+  int q = k - 1;
+  if (omega) std::memcpy(omega, h->omega[q].data(), sizeof(double) * h->P);
+  if (s) std::memcpy(s, h->s[q].data(), sizeof(double) * h->P);
+  if (qx) std::memcpy(qx, h->qx[q].data(), sizeof(double) * h->Fx);
+  if (qy) std::memcpy(qy, h->qy[q].data(), sizeof(double) * h->Fy);
+  if (rhs_old) std::memcpy(rhs_old, h->rhs_old[q].data(), sizeof(double) * h->P);
+  return IB_OK;
+}
+
+int orc_set_level_fields(orc_handle* h, int32_t k, const double* omega, const double* s, const double* qx,
+                         const double* qy, const double* rhs_old) {
+  if (k < 1 || k > h->L) return IB_E_INVAL;
+  int q = k - 1;
+  if (omega) std::memcpy(h->omega[q].data(), omega, sizeof(double) * h->P);
+  if (s) std::memcpy(h->s[q].data(), s, sizeof(double) * h->P);
+  if (qx) std::memcpy(h->qx[q].data(), qx, sizeof(double) * h->Fx);
+  if (qy) std::memcpy(h->qy[q].data(), qy, sizeof(double) * h->Fy);
+  if (rhs_old) std::memcpy(h->rhs_old[q].data(), rhs_old, sizeof(double) * h->P);
+  return IB_OK;
+}
+
+int orc_get_body(orc_handle* h, double* f, double* f_rdst, double* xb, double* yb, double* theta,
+                 double* thetad, double* thetadd) {
+  size_t nf = h->nf, nt = h->tors.size();
+  if (f) std::memcpy(f, h->f.data(), sizeof(double) * nf);
+  if (f_rdst) std::memcpy(f_rdst, h->f_rdst.data(), sizeof(double) * nf);
+  if (xb) std::memcpy(xb, h->xb.data(), sizeof(double) * nf);
+  if (yb) std::memcpy(yb, h->yb.data(), sizeof(double) * nf);
+  if (theta && nt) std::memcpy(theta, h->theta.data(), sizeof(double) * nt);
+  if (thetad && nt) std::memcpy(thetad, h->thetad.data(), sizeof(double) * nt);
+  if (thetadd && nt) std::memcpy(thetadd, h->thetadd.data(), sizeof(double) * nt);
+  return IB_OK;
+}
+
+/* write_force2: src/operators_struct.F90:711-731 */
+int orc_get_forces(orc_handle* h, int32_t which, double* fxo, double* fyo) {
+  const std::vector<double>& fv = which ? h->f_rdst : h->f;
+  int nbod = (int)h->bdy.size();
+  std::vector<double> ax(nbod, 0.0), ay(nbod, 0.0);
+  for (int k = 0; k < h->nf; ++k) {
+    if (h->ccomp[k] == 1) ax[h->cbody[k]] = ax[h->cbody[k]] + fv[k] / h->dt;
+    else ay[h->cbody[k]] = ay[h->cbody[k]] + fv[k] / h->dt;
+  }
+  for (int b = 0; b < nbod; ++b) {
+    fxo[b] = 2 * h->delta * ax[b];
+    fyo[b] = 2 * h->delta * ay[b];
+  }
+  return IB_OK;
+}
+
+int orc_op_dst2d(orc_handle* h, int32_t batch, const double* in, double* out) {
+  for (int b = 0; b < batch; ++b) {
+    std::memcpy(out + (size_t)b * h->P, in + (size_t)b * h->P, sizeof(double) * h->P);
+    h->dst2d(out + (size_t)b * h->P);
+  }
+  return IB_OK;
+}
+
+int orc_op_ctci(orc_handle* h, const double* omega, double* s) {
+  h->ctci(omega, s);
+  return IB_OK;
+}
+
+int orc_op_ainv(orc_handle* h, const double* in, double* out) {
+  std::memcpy(out, in, sizeof(double) * h->P);
+  h->ainv(out);
+  return IB_OK;
+}
+
+int orc_op_vort2flux(orc_handle* h, double* omega, double* s, double* qx, double* qy) {
+  int L = h->L, P = h->P, Fx = h->Fx, Fy = h->Fy;
+  std::vector<std::vector<double>> w(L), sv(L, std::vector<double>(P)), vx(L, std::vector<double>(Fx)),
+      vy(L, std::vector<double>(Fy));
+  for (int k = 0; k < L; ++k) w[k].assign(omega + (size_t)k * P, omega + (size_t)(k + 1) * P);
+  h->vort2flux(w, sv, vx, vy);
+  for (int k = 0; k < L; ++k) {
+    std::memcpy(omega + (size_t)k * P, w[k].data(), sizeof(double) * P);
+    if (s) std::memcpy(s + (size_t)k * P, sv[k].data(), sizeof(double) * P);
+    if (qx) std::memcpy(qx + (size_t)k * Fx, vx[k].data(), sizeof(double) * Fx);
+    if (qy) std::memcpy(qy + (size_t)k * Fy, vy[k].data(), sizeof(double) * Fy);
+  }
+  return IB_OK;
+}
+
+int orc_op_reg(orc_handle* h, const double* f, double* hx, double* hy) {
+  std::fill(hx, hx + h->Fx, 0.0);
+  std::fill(hy, hy + h->Fy, 0.0);
+  h->reg(f, hx, hy);
+  return IB_OK;
+}
+
+int orc_op_regT(orc_handle* h, const double* hx, const double* hy, const double* f_in, double* f_out) {
+  std::memcpy(f_out, f_in, sizeof(double) * h->nf);
+  h->regT(hx, hy, f_out);
+  return IB_OK;
+}
+
+int orc_op_schur_matvec(orc_handle* h, const double* x, double* y) {
+  h->schur(x, y);
+  return IB_OK;
+}
+
+int orc_op_nonlinear_rot(orc_handle* h, int32_t k, const double* omega, const double* qx, const double* qy,
+                         const double* bc, double* rhs) {
+  if (k < 1 || k > h->L) return IB_E_INVAL;
+  std::vector<double> qhx(h->Fx), qhy(h->Fy), nlx(h->Fx), nly(h->Fy);
+  double fac = h->delta * std::pow(2.0, (double)(k - 1));
+  for (int q = 0; q < h->Fx; ++q) qhx[q] = qx[q] + (-fac * (-1.0) + 0.0);
+  for (int q = 0; q < h->Fy; ++q) qhy[q] = qy[q] + (-fac * 0.0 + 0.0);
+  h->nonlinear(omega, qhx.data(), qhy.data(), bc, nlx.data(), nly.data());
+  h->rot(nlx.data(), nly.data(), rhs);
+  return IB_OK;
+}
+
+int orc_op_rot(orc_handle* h, const double* hx, const double* hy, double* vort) {
+  h->rot(hx, hy, vort);
+  return IB_OK;
+}
+
+int orc_op_curl(orc_handle* h, const double* s, const double* sbc, double* qx, double* qy) {
+  h->curl(s, sbc, qx, qy);
+  return IB_OK;
+}
+
+int orc_op_coarsify(orc_handle* h, const double* fine, double* coarse) {
+  h->coarsify(fine, coarse);
+  return IB_OK;
+}
+
+int orc_op_get_bc(orc_handle* h, const double* coarse, double fac, double* bc) {
+  h->get_bc(coarse, bc, fac);
+  return IB_OK;
+}
+
+int orc_op_apply_bc(orc_handle* h, double* vort, const double* bc, double fac) {
+  h->apply_bc(vort, bc, fac);
+  return IB_OK;
+}
+
+int orc_op_weights(orc_handle* h, double* weight, int32_t* reg_ix, int32_t* reg_iy) {
+  if (weight) std::memcpy(weight, h->weight.data(), sizeof(double) * h->weight.size());
+  if (reg_ix) for (size_t i = 0; i < h->reg_ix.size(); ++i) reg_ix[i] = h->reg_ix[i];
+  if (reg_iy) for (size_t i = 0; i < h->reg_iy.size(); ++i) reg_iy[i] = h->reg_iy[i];
+  return IB_OK;
+}
+
+double orc_deltafnc(double x1, double x2, double dr, int32_t f32) {
+  return orc_handle::deltafnc(x1, x2, dr, f32 != 0);
+}
+
+int orc_dst1d(int32_t N, int32_t count, const double* in, double* out) {
+  DST1 d;
+  d.init(N);
+  std::vector<cplx> z(2 * d.M), t(2 * d.M);
+  for (int c = 0; c < count; c += 2) {
+    const double* xa = in + (size_t)c * N;
+    const double* xb = c + 1 < count ? in + (size_t)(c + 1) * N : nullptr;
+    double* ya = out + (size_t)c * N;
+    double* yb2 = c + 1 < count ? out + (size_t)(c + 1) * N : nullptr;
+    d.pair(xa, xb, 1, ya, yb2, 1, z.data(), t.data());
+  }
+  return IB_OK;
+}
+
+int orc_set_threads(int32_t nthreads) {
+  g_threads = nthreads > 0 ? nthreads : 1;
+  return IB_OK;
+}
+
+int orc_get_threads(void) { return g_threads; }
+
+}  // extern "C"

@@ -1,0 +1,60 @@
+/*
+ * ib_oracle.h -- CPU restatement of IB_Parallel's advance() path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * implementation and the CPU baseline timed beside it; the product path
+ * (libibgpu.so) never links, loads or calls it.
+ *
+ * The entry points mirror include/ib_gpu.h one to one with an orc_ prefix, so
+ * a test can drive the oracle and the GPU library with the same arguments.
+ */
+#ifndef IB_ORACLE_H
+#define IB_ORACLE_H
+#include "../include/ib_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_handle orc_handle;
+
+int orc_create(const ib_params* p, int32_t nbodies, const ib_body* bodies, orc_handle** out);
+int orc_destroy(orc_handle* h);
+int orc_get_sizes(const orc_handle* h, ib_sizes* out);
+int orc_last_error(const orc_handle* h, char* buf, int32_t len);
+int orc_advance(orc_handle* h, int32_t* itime, ib_step_info* info);
+int orc_get_level_fields(orc_handle* h, int32_t k, double* omega, double* s,
+                         double* qx, double* qy, double* rhs_old);
+int orc_set_level_fields(orc_handle* h, int32_t k, const double* omega, const double* s,
+                         const double* qx, const double* qy, const double* rhs_old);
+int orc_get_body(orc_handle* h, double* f, double* f_rdst, double* xb, double* yb,
+                 double* theta, double* thetad, double* thetadd);
+int orc_get_forces(orc_handle* h, int32_t which, double* fx, double* fy);
+int orc_op_dst2d(orc_handle* h, int32_t batch, const double* in, double* out);
+int orc_op_ctci(orc_handle* h, const double* omega, double* s);
+int orc_op_ainv(orc_handle* h, const double* in, double* out);
+int orc_op_vort2flux(orc_handle* h, double* omega, double* s, double* qx, double* qy);
+int orc_op_reg(orc_handle* h, const double* f, double* hx, double* hy);
+int orc_op_regT(orc_handle* h, const double* hx, const double* hy, const double* f_in, double* f_out);
+int orc_op_schur_matvec(orc_handle* h, const double* x, double* y);
+int orc_op_nonlinear_rot(orc_handle* h, int32_t k, const double* omega, const double* qx,
+                         const double* qy, const double* bc, double* rhs);
+
+/* oracle-only helpers used by the tests */
+int orc_op_rot(orc_handle* h, const double* hx, const double* hy, double* vort);
+int orc_op_curl(orc_handle* h, const double* s, const double* sbc, double* qx, double* qy);
+int orc_op_coarsify(orc_handle* h, const double* fine, double* coarse);
+int orc_op_get_bc(orc_handle* h, const double* coarse, double fac, double* bc);
+int orc_op_apply_bc(orc_handle* h, double* vort, const double* bc, double fac);
+int orc_op_weights(orc_handle* h, double* weight, int32_t* reg_ix, int32_t* reg_iy);
+/* phi(r) of deltafnc for r = |x1-x2|/dr with dr = 1 (src/grid.F90:955-1003) */
+double orc_deltafnc(double x1, double x2, double dr, int32_t f32_literals);
+/* 1-D unnormalised DST-I (FFTW RODFT00) of `count` vectors of length N */
+int orc_dst1d(int32_t N, int32_t count, const double* in, double* out);
+int orc_set_threads(int32_t nthreads);
+int orc_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
