@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Throughput of the immersed-boundary projection time step on one MI355X.
+
+A "step" is one call of advance(itime) (src/timestepper.F90:18-556): the
+predictor on every grid level, vort2flux, the FSI fixed point with its GMRES
+Schur solves, and the corrector.  The workload is the hinged flexible plate
+(400x400 cells, 4 levels, dx = 0.01, dt = 0.001, Re = 200, 101 body points,
+torsional family-2 body, GMRES/FSI tolerance 1e-10), synthetic and generated
+in ib_parallel_amd.decks.plate_case.  State is resident in HBM before timing.
+
+Metric: Mgridpt-steps/s = m*n*mgridlev*steps / seconds / 1e6 (FP64).
+
+Also reported:
+  roofline      dominant DST pass kernel class: algorithmic bytes per launch
+                (each FP64 array/table read once, output written once) over
+                the launch duration measured with HIP events on the library's
+                stream during the timed region; peak = 8 TB/s HBM.
+  cpu_baseline  the CPU oracle (oracle/, a C++ restatement of the same
+                algorithm, OpenMP over DST rows) on the host cores, on a
+                bounded sample of the same workload from the same cold start.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def gridpts(sim):
+    return sim.m * sim.n * sim.L
+
+
+def make_case(name):
+    from ib_parallel_amd import decks
+    if name == "plate":
+        p, b = decks.plate_case()
+        return p, b, "hinged plate FSI 400x400x4, 101 pts, Re=200, dt=1e-3, tol=1e-10"
+    if name == "cylinder":
+        p, b = decks.cylinder_case()
+        return p, b, "rigid cylinder 200x200x3, 157 pts, Re=100, dt=4e-3, tol=1e-7"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def device_sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def run_gpu(args):
+    from ib_parallel_amd import GpuSim
+    p, bodies, desc = make_case(args.workload)
+    sim = GpuSim(p, bodies)
+    for _ in range(args.warmup):
+        sim.advance(allow_noconv=True)
+    for kc in (0, 1):
+        sim.kernel_timing(kc, True)
+    its = fsi = 0
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        info = sim.advance(allow_noconv=True)
+        its += info.ksp_iter
+        fsi += info.fsi_iter
+    device_sync()
+    t1 = time.perf_counter()
+    timing = {kc: sim.kernel_timing_read(kc) for kc in (0, 1)}
+    for kc in (0, 1):
+        sim.kernel_timing(kc, False)
+    secs = t1 - t0
+    dom = max(timing, key=lambda kc: timing[kc][1])
+    n_l, ms, by = timing[dom]
+    achieved = (by / n_l) / (ms / n_l * 1e-3) / 1e9 if n_l and ms > 0 else 0.0
+    res = {
+        "sim": sim, "desc": desc, "secs": secs, "its": its, "fsi": fsi,
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "dst_x (row pass)" if dom == 0 else "dst_y (column pass)",
+            "launches": n_l, "avg_us": round(ms / max(n_l, 1) * 1e3, 3),
+            "bytes_per_launch": round(by / max(n_l, 1)),
+            "other_class": {"kernel": "dst_y (column pass)" if dom == 0 else "dst_x (row pass)",
+                            "launches": timing[1 - dom][0],
+                            "avg_us": round(timing[1 - dom][1] / max(timing[1 - dom][0], 1) * 1e3, 3),
+                            "share_of_step": round(timing[1 - dom][1] * 1e-3 / secs, 4)},
+            "share_of_step": round(ms * 1e-3 / secs, 4),
+        },
+    }
+    return res
+
+
+def run_cpu(args, gpu_its_per_step):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import binding as ob
+    threads = max(1, min(16, os.cpu_count() or 1))
+    ob.set_threads(threads)
+    p, bodies, _ = make_case(args.workload)
+    sim = ob.OracleSim(p, bodies)
+    steps = its = 0
+    t0 = time.perf_counter()
+    while True:
+        info = sim.advance(allow_noconv=True)
+        steps += 1
+        its += info.ksp_iter
+        if time.perf_counter() - t0 >= args.cpu_seconds or steps >= args.steps + args.warmup:
+            break
+    secs = time.perf_counter() - t0
+    value = gridpts(sim) * steps / secs / 1e6
+    return {
+        "value": round(value, 4), "unit": "Mgridpt-steps/s", "cores": threads, "kind": "port",
+        "sample": (f"oracle (C++ FP64 restatement, OpenMP {threads} threads) advancing steps 1..{steps} of the "
+                   f"same workload from the same cold start in {secs:.1f} s; {its} GMRES iterations "
+                   f"({its / steps:.1f}/step vs {gpu_its_per_step:.1f}/step in the GPU timed steps)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="plate", choices=["plate", "cylinder"])
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path",
+              file=sys.stderr)
+        sys.exit(2)
+
+    g = run_gpu(args)
+    sim = g["sim"]
+    secs = g["secs"]
+    value = gridpts(sim) * args.steps / secs / 1e6
+    out = {
+        "metric": "Mgridpt-steps/s FP64 at 1e-10 rel err; batched & adjoint vs plain GPU",
+        "value": round(value, 4),
+        "unit": "Mgridpt-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(secs / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": g["desc"], "m": sim.m, "n": sim.n, "mgridlev": sim.L, "nb": sim.nf // 2,
+                   "gmres_its_per_step": round(g["its"] / args.steps, 2),
+                   "fsi_its_per_step": round(g["fsi"] / args.steps, 2), "form": "plain"},
+        "roofline": g["roofline"],
+    }
+    if not args.no_cpu:
+        cpu = run_cpu(args, g["its"] / args.steps)
+        out["cpu_baseline"] = cpu
+        out["gpu_over_cpu"] = round(value / cpu["value"], 2) if cpu["value"] > 0 else None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

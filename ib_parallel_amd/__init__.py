@@ -45,10 +45,12 @@ class GpuSim(Simulation):
         self._check(self._f["kernel_timing"](self.h, kclass, 1 if enable else 0))
 
     def kernel_timing_read(self, kclass):
+        """(launches, total_ms, total algorithmic bytes) of the timed launches."""
         n = C.c_int64()
         ms = C.c_double()
-        self._check(self._f["kernel_timing_read"](self.h, kclass, C.byref(n), C.byref(ms)))
-        return n.value, ms.value
+        by = C.c_double()
+        self._check(self._f["kernel_timing_read"](self.h, kclass, C.byref(n), C.byref(ms), C.byref(by)))
+        return n.value, ms.value, by.value
 
 
 __all__ = ["GpuSim", "IBError", "Simulation", "abi", "load", "library"]

@@ -66,44 +66,52 @@ __global__ void __launch_bounds__(kGmT) k_gm_start(int nf, const double* __restr
   }
 }
 
+// wavefront-level sum (64 lanes on CDNA4)
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = warpSize / 2; o > 0; o >>= 1) v += __shfl_down(v, o);
+  return v;
+}
+
 // One Arnoldi step on w = V[it+1] (already A V[it]): CGS, normalisation,
 // Hessenberg column update with Givens rotations (KSPGMRESUpdateHessenberg).
+// The it+1 dot products are spread over the workgroup's wavefronts, one dot
+// product per wavefront at a time, so no block barrier is needed per product.
 __global__ void __launch_bounds__(kGmT) k_gm_arnoldi(int nf, int kmax, double* __restrict__ V, GmresDev* s,
                                                      double* __restrict__ H, double* __restrict__ cc,
                                                      double* __restrict__ ss, double* __restrict__ grs) {
   __shared__ double sh[kGmT];
-  __shared__ double h[256];
   __shared__ int s_hap;
   __shared__ double s_tt;
   const int it = s->it;
+  const int ld = kmax + 1;
   double* w = V + (long)(it + 1) * nf;
+  double* hcol = H + (long)it * ld;
+  const int lane = threadIdx.x % warpSize, wave = threadIdx.x / warpSize, nwave = blockDim.x / warpSize;
   // h_j = <w, v_j>, all from the unmodified w (classical Gram-Schmidt)
-  for (int j = 0; j <= it; ++j) {
+  for (int j = wave; j <= it; j += nwave) {
     const double* vj = V + (long)j * nf;
     double p = 0.0;
-    for (int i = threadIdx.x; i < nf; i += blockDim.x) p += w[i] * vj[i];
-    const double d = gm_block_sum(p, sh);
-    if (threadIdx.x == 0) h[j] = d;
+    for (int i = lane; i < nf; i += warpSize) p += w[i] * vj[i];
+    p = wave_sum(p);
+    if (lane == 0) hcol[j] = p;
   }
   __syncthreads();
   double pn = 0.0;
   for (int i = threadIdx.x; i < nf; i += blockDim.x) {
     double acc = w[i];
-    for (int j = 0; j <= it; ++j) acc += -h[j] * V[(long)j * nf + i];
+    for (int j = 0; j <= it; ++j) acc += -hcol[j] * V[(long)j * nf + i];
     w[i] = acc;
     pn += acc * acc;
   }
   const double tt = sqrt(gm_block_sum(pn, sh));
-  const int ld = kmax + 1;
   if (threadIdx.x == 0) {
-    for (int j = 0; j <= it; ++j) H[(long)it * ld + j] = h[j];
-    H[(long)it * ld + it + 1] = tt;
+    hcol[it + 1] = tt;
     double hapbnd = fabs(tt / grs[it]);
     if (hapbnd > 1e-30) hapbnd = 1e-30;
     const int hap = tt < hapbnd;
     s_hap = hap;
     s_tt = tt;
-    double* hh = H + (long)it * ld;
+    double* hh = hcol;
     for (int j = 0; j < it; ++j) {
       const double t0 = hh[j];
       hh[j] = cc[j] * t0 + ss[j] * hh[j + 1];
@@ -140,8 +148,7 @@ __global__ void __launch_bounds__(kGmT) k_gm_arnoldi(int nf, int kmax, double* _
 // KSPGMRESBuildSoln: back substitution on the rotated Hessenberg, x += V y
 __global__ void __launch_bounds__(kGmT) k_gm_buildsol(int nf, int kmax, const double* __restrict__ V, GmresDev* s,
                                                       const double* __restrict__ H, const double* __restrict__ grs,
-                                                      double* __restrict__ x) {
-  __shared__ double nrs[256];
+                                                      double* __restrict__ x, double* __restrict__ nrs) {
   const int it = s->it;  // number of Arnoldi vectors used
   if (it <= 0) return;
   const int ld = kmax + 1;
@@ -179,8 +186,7 @@ void gm_arnoldi(hipStream_t st, int nf, int kmax, double* V, GmresDev* s, double
 
 void gm_buildsol(hipStream_t st, int nf, int kmax, const double* V, GmresDev* s, const double* H, const double* grs,
                  double* x, double* scratch) {
-  (void)scratch;
-  hipLaunchKernelGGL(k_gm_buildsol, dim3(1), dim3(kGmT), 0, st, nf, kmax, V, s, H, grs, x);
+  hipLaunchKernelGGL(k_gm_buildsol, dim3(1), dim3(kGmT), 0, st, nf, kmax, V, s, H, grs, x, scratch);
   IB_LAUNCH_CHECK();
 }
 

@@ -46,10 +46,12 @@ struct KTimer {
   bool on = false;
   int64_t count = 0;
   double total_ms = 0;
+  double total_bytes = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, spare;
   hipEvent_t start_ev = nullptr;
-  void begin(hipStream_t st) {
+  void begin(hipStream_t st, double bytes) {
     if (!on) return;
+    total_bytes += bytes;
     std::pair<hipEvent_t, hipEvent_t> e;
     if (!spare.empty()) {
       e = spare.back();
@@ -114,7 +116,7 @@ struct ib_handle {
   DBuf<double> th, dabs;
   // GMRES
   int kmax = 30;
-  DBuf<double> V, H, cc, ss, grs;
+  DBuf<double> V, H, cc, ss, grs, nrs;
   GmresDev* gst = nullptr;
   GmresDev* hst = nullptr;  // pinned
   double* hbuf = nullptr;   // pinned readback
@@ -130,13 +132,17 @@ struct ib_handle {
     tim[1].harvest();
   }
 
+  // algorithmic bytes of one DST pass launch: arrays read + tables + arrays written
+  double pass_bytes(int nread, int nwrite, int batch) const {
+    return 8.0 * (double)g.P * (double)(nread + nwrite) * (double)batch;
+  }
   void xpass(int mode, const DstLaunch& a) {
-    tim[0].begin(st);
+    tim[0].begin(st, pass_bytes(mode == X_SUBFROM ? 2 : 1, 1, a.batch));
     dst_x(st, px.dev, mode, a, g, (px.dev.L <= 512 ? 2 : 1), max_lds);
     tim[0].end(st);
   }
   void ypass(int mode, const DstLaunch& a) {
-    tim[1].begin(st);
+    tim[1].begin(st, pass_bytes(mode == Y_PRED ? 4 : (mode == Y_SCALE2 ? 2 : 1), 1, a.batch));
     dst_y(st, py.dev, mode, a, g, 4, max_lds);
     tim[1].end(st);
   }
@@ -235,7 +241,7 @@ struct ib_handle {
         ++it;
         if (reason || it >= kmax || its >= maxit) break;
       }
-      gm_buildsol(st, nf, kmax, V, gst, H, grs, x, nullptr);
+      gm_buildsol(st, nf, kmax, V, gst, H, grs, x, nrs);
       if (reason) break;
       if (its >= maxit) {
         reason = -2;
@@ -340,7 +346,7 @@ struct ib_handle {
     if (L < 1) return fail(IB_E_INVAL, "mgridlev must be >= 1");
     if (prm.num_stat || prm.sub_domain) return fail(IB_E_UNSUPPORTED, "num_stat and sub_domain paths are not built");
     if (prm.istart != 0) return fail(IB_E_UNSUPPORTED, "restart from .var is not built");
-    if (prm.gmres_restart > 250) return fail(IB_E_INVAL, "gmres_restart must be <= 250");
+    if (prm.gmres_restart > 4096) return fail(IB_E_INVAL, "gmres_restart must be <= 4096");
     kmax = prm.gmres_restart;
     dt = prm.dt; Re = prm.re; offx = prm.offsetx; offy = prm.offsety;
     delta = prm.len / (double)m;
@@ -464,7 +470,7 @@ struct ib_handle {
 
     V.alloc((size_t)(kmax + 1) * std::max(1, nf));
     H.alloc((size_t)(kmax + 1) * kmax);
-    cc.alloc(kmax); ss.alloc(kmax); grs.alloc(kmax + 1);
+    cc.alloc(kmax); ss.alloc(kmax); grs.alloc(kmax + 1); nrs.alloc(kmax + 1);
     IB_HIP(hipMalloc(&gst, sizeof(GmresDev)));
     IB_HIP(hipHostMalloc(&hst, sizeof(GmresDev), hipHostMallocDefault));
     IB_HIP(hipHostMalloc(&hbuf, sizeof(double) * std::max(64, ntors + 1), hipHostMallocDefault));
@@ -756,16 +762,18 @@ int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable) {
   h->tim[kclass].on = enable != 0;
   h->tim[kclass].count = 0;
   h->tim[kclass].total_ms = 0;
+  h->tim[kclass].total_bytes = 0;
   return IB_OK;
   API_CATCH(h)
 }
 
-int ib_kernel_timing_read(ib_handle* h, int32_t kclass, int64_t* launches, double* total_ms) {
+int ib_kernel_timing_read(ib_handle* h, int32_t kclass, int64_t* launches, double* total_ms, double* total_bytes) {
   if (!h || kclass < 0 || kclass > 1) return IB_E_INVAL;
   API_TRY
   h->sync();
   if (launches) *launches = h->tim[kclass].count;
   if (total_ms) *total_ms = h->tim[kclass].total_ms;
+  if (total_bytes) *total_bytes = h->tim[kclass].total_bytes;
   return IB_OK;
   API_CATCH(h)
 }

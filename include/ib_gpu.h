@@ -138,10 +138,14 @@ int ib_op_schur_matvec(ib_handle* h, const double* x, double* y);
 int ib_op_nonlinear_rot(ib_handle* h, int32_t k, const double* omega, const double* qx,
                         const double* qy, const double* bc, double* rhs);
 
-/* --- instrumentation: per-kernel-class timing with HIP events on the
- * handle's stream.  class 0 = DST row pass, 1 = DST column pass. --- */
+/* --- instrumentation (no reference counterpart): per-kernel-class timing
+ * with HIP events recorded around each launch on the handle's stream.
+ * class 0 = DST row pass, 1 = DST column pass.  total_bytes accumulates the
+ * algorithmic HBM bytes of the timed launches (each FP64 input array, table
+ * and output array read or written once). --- */
 int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable);
-int ib_kernel_timing_read(ib_handle* h, int32_t kclass, int64_t* launches, double* total_ms);
+int ib_kernel_timing_read(ib_handle* h, int32_t kclass, int64_t* launches, double* total_ms,
+                          double* total_bytes);
 
 #ifdef __cplusplus
 }
