@@ -84,6 +84,7 @@ SIGNATURES = {
     "op_regT": (C.c_int, [_P, _D, _D, _D, _D]),
     "op_schur_matvec": (C.c_int, [_P, _D, _D]),
     "op_nonlinear_rot": (C.c_int, [_P, C.c_int32, _D, _D, _D, _D, _D]),
+    "op_gmres": (C.c_int, [_P, _D, _D, _I]),
 }
 
 GPU_ONLY = {

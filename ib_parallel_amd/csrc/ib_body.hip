@@ -13,8 +13,12 @@
 namespace ibg {
 
 // deltafnc of src/grid.F90:955-1003.  With f32 set, the exponent-less literals
-// take their REAL(4) values as in the reference source.
+// take their REAL(4) values as in the reference source.  The kernel jumps by
+// ~1e-6 at r = 1 and 2 with those literals, and body points on grid lines
+// land exactly there, so the weight build evaluates every operation as
+// written (no FMA contraction) to pick the same branch as the source order.
 __device__ double deltafnc(double x1, double x2, double dr, int f32) {
+#pragma clang fp contract(off)
   const double r = fabs(x1 - x2);
   const double r1 = r / dr, r2 = r1 * r1, r3 = r2 * r1, r4 = r3 * r1;
   double d;
@@ -51,6 +55,7 @@ __device__ double deltafnc(double x1, double x2, double dr, int f32) {
 __global__ void k_setup_reg(Geom g, int nf, const double* __restrict__ xb, const double* __restrict__ yb,
                             const int* __restrict__ comp, double delta, double offx, double offy, int f32,
                             double* __restrict__ w, int* __restrict__ key, int* __restrict__ err) {
+#pragma clang fp contract(off)
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nf * kTaps) return;
   const int k = idx / kTaps, t = idx - k * kTaps;

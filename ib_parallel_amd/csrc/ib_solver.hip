@@ -738,6 +738,22 @@ int ib_op_schur_matvec(ib_handle* h, const double* x, double* y) {
   API_CATCH(h)
 }
 
+int ib_op_gmres(ib_handle* h, const double* b, double* x, int32_t* its) {
+  if (!h) return IB_E_INVAL;
+  API_TRY
+  if (h->nf) {
+    h->h2d(h->rhsf, b, h->nf);
+    h->h2d(h->f_rdst, x, h->nf);
+  }
+  int it = 0;
+  int rc = h->gmres_dev(h->rhsf, h->f_rdst, &it);
+  if (h->nf) h->d2h(x, h->f_rdst, h->nf);
+  h->sync();
+  if (its) *its = it;
+  return rc;
+  API_CATCH(h)
+}
+
 int ib_op_nonlinear_rot(ib_handle* h, int32_t k, const double* omega, const double* qx, const double* qy,
                         const double* bc, double* rhs) {
   if (!h || k < 1 || k > h->L) return IB_E_INVAL;

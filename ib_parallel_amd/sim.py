@@ -139,6 +139,15 @@ class Simulation:
         self._check(self._f["op_schur_matvec"](self.h, abi.dptr(x), abi.dptr(y)))
         return y
 
+    def gmres(self, b, x0=None, allow_noconv=False):
+        """Solve the Schur system (ab_times) by GMRES from x0; returns (x, its)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.nf) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+        its = C.c_int32()
+        rc = self._f["op_gmres"](self.h, abi.dptr(b), abi.dptr(x), C.byref(its))
+        self._check(rc, (abi.IB_E_NOCONV_GMRES,) if allow_noconv else ())
+        return x, its.value
+
     def nonlinear_rot(self, k, omega, qx, qy, bc):
         c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
         omega, qx, qy, bc = map(c, (omega, qx, qy, bc))

@@ -133,6 +133,10 @@ int ib_op_reg(ib_handle* h, const double* f, double* hx, double* hy);
 int ib_op_regT(ib_handle* h, const double* hx, const double* hy, const double* f_in, double* f_out);
 /* ab_times: y = a_times2(x) + b_times2(x), src/timestepper.F90:774-782 */
 int ib_op_schur_matvec(ib_handle* h, const double* x, double* y);
+/* KSPSolve(ksp_fsi, AB, b, x) (src/timestepper.F90:363-366): GMRES on the
+ * Schur operator with x as the initial guess (in) and solution (out); *its
+ * gets the iteration count.  Returns IB_E_NOCONV_GMRES if not converged. */
+int ib_op_gmres(ib_handle* h, const double* b, double* x, int32_t* its);
 /* nonlinear + rot of one level: rhs = C^T N(omega_k, q_k + q0_k, bc),
  * src/operators_fluid.F90:553-587 and :203-233; bc has nbc entries */
 int ib_op_nonlinear_rot(ib_handle* h, int32_t k, const double* omega, const double* qx,

@@ -25,6 +25,9 @@ EXTRA = {
     "op_weights": (C.c_int, [C.c_void_p, abi._D, abi._I, abi._I]),
     "deltafnc": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_int32]),
     "dst1d": (C.c_int, [C.c_int32, C.c_int32, abi._D, abi._D]),
+    "op_btimes": (C.c_int, [C.c_void_p, abi._D, abi._D]),
+    "set_theta": (C.c_int, [C.c_void_p, abi._D, abi._D, abi._D]),
+    "op_body_update": (C.c_int, [C.c_void_p, abi._D, abi._D]),
     "set_threads": (C.c_int, [C.c_int32]),
     "get_threads": (C.c_int, []),
 }
@@ -92,6 +95,29 @@ class OracleSim(Simulation):
         ix = np.empty(49 * self.nf, dtype=np.int32); iy = np.empty(49 * self.nf, dtype=np.int32)
         self._check(self._f["op_weights"](self.h, abi.dptr(w), ix.ctypes.data_as(abi._I), iy.ctypes.data_as(abi._I)))
         return w.reshape(self.nf, 49), ix, iy, b
+
+
+def _arr(a):
+    import numpy as np
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def btimes(sim, x):
+    import numpy as np
+    y = np.empty(sim.nf)
+    sim._check(sim._f["op_btimes"](sim.h, abi.dptr(_arr(x)), abi.dptr(y)))
+    return y
+
+
+def set_theta(sim, theta=None, thetad=None, thetadd=None):
+    sim._check(sim._f["set_theta"](sim.h, abi.dptr(_arr(theta)), abi.dptr(_arr(thetad)), abi.dptr(_arr(thetadd))))
+
+
+def body_update(sim, f):
+    import numpy as np
+    d = np.empty(max(1, sim.ntors))
+    sim._check(sim._f["op_body_update"](sim.h, abi.dptr(_arr(f)), abi.dptr(d)))
+    return d[: sim.ntors]
 
 
 def deltafnc(x1, x2, dr, f32=True):

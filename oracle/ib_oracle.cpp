@@ -1019,6 +1019,39 @@ int orc_op_nonlinear_rot(orc_handle* h, int32_t k, const double* omega, const do
   return IB_OK;
 }
 
+int orc_op_gmres(orc_handle* h, const double* b, double* x, int32_t* its) {
+  int it = 0;
+  int rc = h->gmres(b, x, &it);
+  if (its) *its = it;
+  return rc;
+}
+
+int orc_op_btimes(orc_handle* h, const double* x, double* y) {
+  std::fill(y, y + h->nf, 0.0);
+  h->b_times2(x, y);
+  return IB_OK;
+}
+
+int orc_set_theta(orc_handle* h, const double* theta, const double* thetad, const double* thetadd) {
+  size_t nt = h->tors.size();
+  for (size_t t = 0; t < nt; ++t) {
+    if (theta) h->theta[t] = theta[t];
+    if (thetad) h->thetad[t] = thetad[t];
+    if (thetadd) h->thetadd[t] = thetadd[t];
+  }
+  return IB_OK;
+}
+
+int orc_op_body_update(orc_handle* h, const double* f, double* dtheta) {
+  h->theta0 = h->theta;
+  h->thetad0 = h->thetad;
+  h->thetadd0 = h->thetadd;
+  std::memcpy(h->f.data(), f, sizeof(double) * h->nf);
+  h->compute_dx();
+  for (size_t t = 0; t < h->tors.size(); ++t) dtheta[t] = h->dtheta[t];
+  return IB_OK;
+}
+
 int orc_op_rot(orc_handle* h, const double* hx, const double* hy, double* vort) {
   h->rot(hx, hy, vort);
   return IB_OK;

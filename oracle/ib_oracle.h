@@ -37,10 +37,18 @@ int orc_op_vort2flux(orc_handle* h, double* omega, double* s, double* qx, double
 int orc_op_reg(orc_handle* h, const double* f, double* hx, double* hy);
 int orc_op_regT(orc_handle* h, const double* hx, const double* hy, const double* f_in, double* f_out);
 int orc_op_schur_matvec(orc_handle* h, const double* x, double* y);
+int orc_op_gmres(orc_handle* h, const double* b, double* x, int32_t* its);
 int orc_op_nonlinear_rot(orc_handle* h, int32_t k, const double* omega, const double* qx,
                          const double* qy, const double* bc, double* rhs);
 
 /* oracle-only helpers used by the tests */
+/* y = b_times2(x) on torsional entries, 0 elsewhere */
+int orc_op_btimes(orc_handle* h, const double* x, double* y);
+/* overwrite the torsional state (one entry per torsional body) */
+int orc_set_theta(orc_handle* h, const double* theta, const double* thetad, const double* thetadd);
+/* one Newmark/FSI body update with force f: theta0 <- theta, compute_dx;
+ * dtheta gets the increments */
+int orc_op_body_update(orc_handle* h, const double* f, double* dtheta);
 int orc_op_rot(orc_handle* h, const double* hx, const double* hy, double* vort);
 int orc_op_curl(orc_handle* h, const double* s, const double* sbc, double* qx, double* qy);
 int orc_op_coarsify(orc_handle* h, const double* fine, double* coarse);

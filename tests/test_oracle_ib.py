@@ -1,0 +1,140 @@
+"""Oracle immersed-boundary coupling and torsional-body algebra.
+
+deltafnc: partition of unity, zero first moment, symmetry and continuity at
+r = 1, 2, 3 (the defining properties of the 6-point kernel); the REAL(4)
+literal variant satisfies them only to ~1e-6.  reg/regT: exact adjointness,
+weights summing to one per force entry.  b_times2: rank one, symmetric, equal
+to c BR BR^T / den assembled densely.  Newmark body update with zero fluid
+force: the average-acceleration recurrence of a free torsional spring.
+GMRES: the solution of the dense Schur matrix assembled column by column from
+the matvec (the reference's num_stat preprocess) solved by LAPACK.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel
+from ib_parallel_amd import decks
+from oracle import binding as ob
+
+
+def phi(r, f32):
+    return np.array([ob.deltafnc(float(x), 0.0, 1.0, f32) for x in np.atleast_1d(r)])
+
+
+@pytest.mark.parametrize("f32,tol", ((False, 1e-8), (True, 3e-6)))
+def test_deltafnc_moments(f32, tol):
+    for r in np.linspace(0.0, 1.0, 23, endpoint=False):
+        k = np.arange(-3, 4)
+        x = r + k
+        w = phi(np.abs(x), f32)
+        assert abs(w.sum() - 1.0) < tol
+        assert abs((x * w).sum()) < tol
+
+
+def test_deltafnc_continuity_and_support():
+    # the kernel's coefficients carry 10 significant digits (1e-8 level);
+    # the REAL(4) literals leave ~1e-6
+    for f32, tol in ((False, 1e-8), (True, 3e-6)):
+        for r0 in (1.0, 2.0):
+            lo, hi = phi([r0 - 1e-12, r0 + 1e-12], f32)
+            assert abs(lo - hi) < tol
+        assert abs(phi([3.0 - 1e-12], f32)[0]) < tol
+        assert phi([3.0 + 1e-9], f32)[0] == 0.0
+
+
+def test_deltafnc_f32_literals_change_the_kernel():
+    r = np.linspace(0, 2.9, 59)
+    a, b = phi(r, False), phi(r, True)
+    d = np.abs(a - b)[np.abs(a) > 1e-3] / np.abs(a[np.abs(a) > 1e-3])
+    assert 1e-6 < d.max() < 1e-3
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    p, b = decks.tiny_case(m=40, n=32, mgridlev=2)
+    return ob.OracleSim(p, b)
+
+
+def test_reg_regT_adjoint(tiny):
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal(tiny.nf)
+    ux, uy = rng.standard_normal(tiny.Fx), rng.standard_normal(tiny.Fy)
+    hx, hy = tiny.reg(f)
+    lhs = hx @ ux + hy @ uy
+    rhs = f @ tiny.regT(ux, uy)
+    assert abs(lhs - rhs) < 1e-13 * (np.abs(f) @ np.abs(tiny.regT(np.abs(ux), np.abs(uy))))
+
+
+def test_reg_weights_sum_to_one(tiny):
+    for k in range(tiny.nf):
+        e = np.zeros(tiny.nf); e[k] = 1.0
+        hx, hy = tiny.reg(e)
+        assert abs(hx.sum() + hy.sum() - 1.0) < 3e-6   # f32-literal kernel
+
+
+def test_regT_of_constant_field(tiny):
+    out = tiny.regT(np.full(tiny.Fx, 2.0), np.full(tiny.Fy, 2.0))
+    assert np.allclose(out, 2.0, rtol=3e-6)
+
+
+def test_btimes_rank_one(tiny):
+    b = tiny.body()
+    n = tiny.nf
+    B = np.stack([ob.btimes(tiny, np.eye(n)[i]) for i in range(n)], 1)
+    assert np.linalg.matrix_rank(B, tol=1e-12 * np.abs(B).max()) == 1
+    assert rel(B, B.T) < 1e-14
+    # dense c BR BR^T / den on the torsional body's slice
+    body = [bd for bd in decks.tiny_case(m=40, n=32, mgridlev=2)[1] if bd.fam == 2][0]
+    npts = body.npts
+    p = tiny.params
+    d = p.len / p.m
+    ds = np.hypot(body.x[1] - body.x[0], body.y[1] - body.y[0])
+    j = np.arange(npts)
+    BR = np.concatenate([j * ds * np.sin(body.gamma0), -j * ds * np.cos(body.gamma0)])
+    den = 4 / p.dt ** 2 * body.itheta + 2 / p.dt * body.ctheta + body.ktheta
+    c = (d * d / ds / p.dt) * (2 * ds / p.dt)
+    off = n - 2 * npts
+    assert rel(B[off:, off:], c * np.outer(BR, BR) / den) < 1e-13
+
+
+def test_free_spring_newmark(tiny):
+    p = tiny.params
+    body = [bd for bd in decks.tiny_case(m=40, n=32, mgridlev=2)[1] if bd.fam == 2][0]
+    M, K, dt = body.itheta, body.ktheta, p.dt
+    th, thd = 0.01, -0.02
+    thdd = -K / M * th
+    ob.set_theta(tiny, [th], [thd], [thdd])
+    for step in range(5):
+        ob.body_update(tiny, np.zeros(tiny.nf))
+        # average acceleration: th1 (1 + dt^2 K/(4M)) = th + dt thd + dt^2/4 thdd
+        th1 = (th + dt * thd + dt * dt / 4 * thdd) / (1 + dt * dt * K / (4 * M))
+        thdd1 = -K / M * th1
+        thd1 = thd + dt / 2 * (thdd + thdd1)
+        th, thd, thdd = th1, thd1, thdd1
+        b = tiny.body()
+        assert b["theta"][0] == pytest.approx(th, rel=1e-12)
+        assert b["thetad"][0] == pytest.approx(thd, rel=1e-10)
+        assert b["thetadd"][0] == pytest.approx(thdd, rel=1e-10)
+
+
+def test_gmres_matches_dense_solve():
+    p, b = decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-13, gmres_restart=200)
+    sim = ob.OracleSim(p, b)
+    n = sim.nf
+    A = np.stack([sim.schur_matvec(np.eye(n)[i]) for i in range(n)], 1)
+    rhs = A @ np.random.default_rng(8).standard_normal(n)
+    x, its = sim.gmres(rhs)
+    assert its <= n + 1
+    assert rel(x, np.linalg.solve(A, rhs)) < 1e-8
+    assert rel(A @ x, rhs) < 1e-12
+
+
+def test_gmres_restart_is_petsc_gmres30():
+    """Restarted GMRES(30) on the same system needs more iterations than full GMRES."""
+    p, b = decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-9, gmres_restart=30)
+    sim = ob.OracleSim(p, b)
+    rhs = sim.schur_matvec(np.random.default_rng(9).standard_normal(sim.nf))
+    _, its30 = sim.gmres(rhs)
+    p2, b2 = decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-9, gmres_restart=200)
+    _, its_full = ob.OracleSim(p2, b2).gmres(rhs)
+    assert its30 > its_full
