@@ -178,25 +178,34 @@ void reg(hipStream_t st, const Geom& g, const RegCSR& c, const double* w, const 
   IB_LAUNCH_CHECK();
 }
 
-// regT: src/fsinterface.F90:51-80, one thread per force entry
+// regT: src/fsinterface.F90:51-80.  One wavefront per force entry: lane t
+// forms w_t * u(face_t) for tap t, then the 49 products are added in tap
+// order (the reference's sequential accumulation) through lane shuffles.
+__device__ __forceinline__ double taps_in_order(double acc, double prod) {
+  for (int t = 0; t < kTaps; ++t) acc = acc + __shfl(prod, t);
+  return acc;
+}
+
 __global__ void k_regT(Geom g, int nf, const double* __restrict__ w, const int* __restrict__ key,
                        const double* __restrict__ hx, const double* __restrict__ hy, double shift_x,
                        const double* __restrict__ base, double* __restrict__ out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = (blockIdx.x * blockDim.x + threadIdx.x) / warpSize;
+  const int lane = threadIdx.x % warpSize;
   if (k >= nf) return;
-  double acc = base ? base[k] : 0.0;
-  for (int t = 0; t < kTaps; ++t) {
-    const int kk = key[k * kTaps + t];
+  double prod = 0.0;
+  if (lane < kTaps) {
+    const int kk = key[k * kTaps + lane];
     const double v = kk < g.Fx ? hx[kk] + shift_x : hy[kk - g.Fx];
-    acc = acc + w[k * kTaps + t] * v;
+    prod = w[k * kTaps + lane] * v;
   }
-  out[k] = acc;
+  const double acc = taps_in_order(base ? base[k] : 0.0, prod);
+  if (lane == 0) out[k] = acc;
 }
 
 void regT(hipStream_t st, const Geom& g, int nf, const double* w, const int* key, const double* hx, const double* hy,
           double shift_x, const double* base, double* out) {
   if (nf <= 0) return;
-  hipLaunchKernelGGL(k_regT, dim3(ceil_div(nf, 128)), dim3(128), 0, st, g, nf, w, key, hx, hy, shift_x, base, out);
+  hipLaunchKernelGGL(k_regT, dim3(ceil_div(nf, 4)), dim3(256), 0, st, g, nf, w, key, hx, hy, shift_x, base, out);
   IB_LAUNCH_CHECK();
 }
 
@@ -274,12 +283,13 @@ void body_btimes(hipStream_t st, int nt, const TorsBody* tb, const double* theta
 }
 
 // compute_rhs (:342-398) for torsional entries + regT of q + q0 (rhsf of the
-// FSI loop, src/timestepper.F90:322-358); one thread per force entry
+// FSI loop, src/timestepper.F90:322-358); one wavefront per force entry
 __global__ void k_fsi_rhs(Geom g, int nf, const int* __restrict__ tors_of, const TorsBody* __restrict__ tbs, int nt,
                           const double* __restrict__ th, double dt, double delta, const double* __restrict__ w,
                           const int* __restrict__ key, const double* __restrict__ qx, const double* __restrict__ qy,
                           double q0x, double* __restrict__ rhsf) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = (blockIdx.x * blockDim.x + threadIdx.x) / warpSize;
+  const int lane = threadIdx.x % warpSize;
   if (k >= nf) return;
   double v = 0.0;
   const int t = tors_of[k];
@@ -297,19 +307,21 @@ __global__ void k_fsi_rhs(Geom g, int nf, const int* __restrict__ tors_of, const
     const double si = sin(b.gamma0 - theta), ci = cos(b.gamma0 - theta);
     v = BR(b, k - b.off + 1, si, ci) * sc;
   }
-  for (int tp = 0; tp < kTaps; ++tp) {
-    const int kk = key[k * kTaps + tp];
+  double prod = 0.0;
+  if (lane < kTaps) {
+    const int kk = key[k * kTaps + lane];
     const double u = kk < g.Fx ? qx[kk] + q0x : qy[kk - g.Fx] + 0.0;
-    v = v + w[k * kTaps + tp] * u;
+    prod = w[k * kTaps + lane] * u;
   }
-  rhsf[k] = v;
+  v = taps_in_order(v, prod);
+  if (lane == 0) rhsf[k] = v;
 }
 
 void fsi_rhs(hipStream_t st, const Geom& g, int nf, const int* tors_of_entry, const TorsBody* tb, int nt,
              const double* th, double dt, double delta, const double* w, const int* key, const double* qx,
              const double* qy, double q0x, double* rhsf) {
   if (nf <= 0) return;
-  hipLaunchKernelGGL(k_fsi_rhs, dim3(ceil_div(nf, 128)), dim3(128), 0, st, g, nf, tors_of_entry, tb, nt, th, dt, delta,
+  hipLaunchKernelGGL(k_fsi_rhs, dim3(ceil_div(nf, 4)), dim3(256), 0, st, g, nf, tors_of_entry, tb, nt, th, dt, delta,
                      w, key, qx, qy, q0x, rhsf);
   IB_LAUNCH_CHECK();
 }

@@ -75,6 +75,27 @@ __device__ __forceinline__ void dft<5>(double2* v) {
   v[3] = make_double2(a2.x - b2.y, a2.y + b2.x);  // a2 + i b2
 }
 
+template <>
+__device__ __forceinline__ void dft<8>(double2* v) {
+  const double c = 0.70710678118654752440;  // cos(pi/4)
+  double2 e[4] = {v[0], v[2], v[4], v[6]};
+  double2 o[4] = {v[1], v[3], v[5], v[7]};
+  dft<4>(e);
+  dft<4>(o);
+  // o[k] *= exp(-2 pi i k / 8)
+  const double2 o1 = make_double2(c * (o[1].x + o[1].y), c * (o[1].y - o[1].x));
+  const double2 o2 = make_double2(o[2].y, -o[2].x);
+  const double2 o3 = make_double2(c * (o[3].y - o[3].x), -c * (o[3].x + o[3].y));
+  v[0] = cadd(e[0], o[0]);
+  v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);
+  v[5] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);
+  v[6] = csub(e[2], o2);
+  v[3] = cadd(e[3], o3);
+  v[7] = csub(e[3], o3);
+}
+
 // One Stockham stage of radix R over nf transforms of length L stored at
 // in[s*L + t]; Ns = product of the radices already applied.
 template <int R>
@@ -122,18 +143,20 @@ __device__ void stage_generic(const double2* __restrict__ in, double2* __restric
   }
 }
 
-// Runs the plan on nf transforms held in a (scratch b); returns the buffer
-// that holds the result.  Caller synchronises before the call.
-__device__ double2* fft_run(double2* a, double2* b, int nf, const FFTPlan& pl) {
+// Runs the plan on nf transforms held in a (scratch b) with the twiddle table
+// W staged in LDS; returns the buffer that holds the result.  Caller
+// synchronises before the call.
+__device__ double2* fft_run(double2* a, double2* b, int nf, const FFTPlan& pl, const double2* W) {
   int Ns = 1;
   for (int st = 0; st < pl.nst; ++st) {
     const int R = pl.rad[st];
     switch (R) {
-      case 4: stage_fixed<4>(a, b, nf, pl.L, Ns, pl.W); break;
-      case 2: stage_fixed<2>(a, b, nf, pl.L, Ns, pl.W); break;
-      case 3: stage_fixed<3>(a, b, nf, pl.L, Ns, pl.W); break;
-      case 5: stage_fixed<5>(a, b, nf, pl.L, Ns, pl.W); break;
-      default: stage_generic(a, b, nf, pl.L, Ns, R, pl.W); break;
+      case 8: stage_fixed<8>(a, b, nf, pl.L, Ns, W); break;
+      case 4: stage_fixed<4>(a, b, nf, pl.L, Ns, W); break;
+      case 2: stage_fixed<2>(a, b, nf, pl.L, Ns, W); break;
+      case 3: stage_fixed<3>(a, b, nf, pl.L, Ns, W); break;
+      case 5: stage_fixed<5>(a, b, nf, pl.L, Ns, W); break;
+      default: stage_generic(a, b, nf, pl.L, Ns, R, W); break;
     }
     __syncthreads();
     double2* t = a;
@@ -148,6 +171,8 @@ struct XArgs {
   const double* in[kMaxBatch];
   double* out[kMaxBatch];
   const double* bc;
+  const double* hx;  // X_ROT: the node value is rot(hx, hy)
+  const double* hy;
   Geom g;
   int rp;
 };
@@ -161,7 +186,13 @@ struct YArgs {
   double div;
   Geom g;
   int cp;
+  int chunk;  // column-pair tiles per XCD (XCD-aware tile order)
 };
+
+// Stage the twiddle table in LDS (read by every stage of every transform).
+__device__ __forceinline__ void load_twiddles(double2* Wl, const FFTPlan& pl) {
+  for (int t = threadIdx.x; t < pl.L; t += blockDim.x) Wl[t] = pl.W[t];
+}
 
 // Row pass.  Rows j0 = 0..n-2 (node j = j0+2) of length N = m-1.
 template <int MODE>
@@ -171,16 +202,34 @@ __global__ void __launch_bounds__(256) k_dst_x(XArgs a, FFTPlan pl) {
   const int nrows = a.g.n - 1;
   double2* A = sm;
   double2* B = sm + rp * L;
+  double2* Wl = sm + 2 * rp * L;
   const double* __restrict__ in = a.in[blockIdx.y];
   double* out = a.out[blockIdx.y];
   const int pair0 = blockIdx.x * rp;
+  load_twiddles(Wl, pl);
   for (int idx = threadIdx.x; idx < rp * N; idx += blockDim.x) {
     const int s = idx / N;
     const int t = idx - s * N;
     const int ra = 2 * (pair0 + s), rb = ra + 1;
     double xa = 0.0, xb = 0.0;
-    if (ra < nrows) xa = in[(long)ra * N + t];
-    if (rb < nrows) xb = in[(long)rb * N + t];
+    if (MODE == X_ROT) {
+      // rot_int + rot_bc: vort(i,j) = hy(i,j) - hy(i-1,j) - hx(i,j) + hx(i,j-1)
+      const Geom& g = a.g;
+      const int i = t + 2;
+      if (ra < nrows) {
+        const int j = ra + 2;
+        xa = a.hy[(i - 1) + g.m * (j - 1)] - a.hy[(i - 2) + g.m * (j - 1)] - a.hx[(i - 1) + (g.m + 1) * (j - 1)] +
+             a.hx[(i - 1) + (g.m + 1) * (j - 2)];
+      }
+      if (rb < nrows) {
+        const int j = rb + 2;
+        xb = a.hy[(i - 1) + g.m * (j - 1)] - a.hy[(i - 2) + g.m * (j - 1)] - a.hx[(i - 1) + (g.m + 1) * (j - 1)] +
+             a.hx[(i - 1) + (g.m + 1) * (j - 2)];
+      }
+    } else {
+      if (ra < nrows) xa = in[(long)ra * N + t];
+      if (rb < nrows) xb = in[(long)rb * N + t];
+    }
     if (MODE == X_APPLYBC) {
       // apply_bc(vort, sbc, 1.d0), src/operators_fluid.F90:539-549
       const int i = t + 2;
@@ -209,7 +258,7 @@ __global__ void __launch_bounds__(256) k_dst_x(XArgs a, FFTPlan pl) {
     A[s * L + M] = make_double2(0.0, 0.0);
   }
   __syncthreads();
-  const double2* Z = fft_run(A, B, rp, pl);
+  const double2* Z = fft_run(A, B, rp, pl, Wl);
   for (int idx = threadIdx.x; idx < rp * N; idx += blockDim.x) {
     const int s = idx / N;
     const int t = idx - s * N;
@@ -234,11 +283,17 @@ __global__ void __launch_bounds__(256) k_dst_y(YArgs a, FFTPlan pl) {
   const int nfft = (MODE == Y_PRED) ? 2 * cp : cp;
   double2* A = sm;
   double2* B = sm + nfft * L;
+  double2* Wl = sm + 2 * nfft * L;
   const double* __restrict__ in = a.in[blockIdx.y];
   const double* __restrict__ in2 = a.in2[blockIdx.y];
   double* __restrict__ out = a.out[blockIdx.y];
-  const int col0 = 2 * blockIdx.x * cp;
+  // workgroups are dealt to the 8 XCDs round-robin: give each XCD a
+  // contiguous band of column tiles so neighbouring tiles share its L2
+  const int tile = (blockIdx.x % 8) * a.chunk + blockIdx.x / 8;
+  const int col0 = 2 * tile * cp;
+  if (col0 >= ncols) return;
   const int w2 = 2 * cp;  // columns handled by this workgroup
+  load_twiddles(Wl, pl);
   for (int idx = threadIdx.x; idx < N * w2; idx += blockDim.x) {
     const int t = idx / w2;
     const int c = idx - t * w2;
@@ -265,7 +320,7 @@ __global__ void __launch_bounds__(256) k_dst_y(YArgs a, FFTPlan pl) {
     A[s * L + M] = make_double2(0.0, 0.0);
   }
   __syncthreads();
-  double2* Z = fft_run(A, B, nfft, pl);
+  double2* Z = fft_run(A, B, nfft, pl, Wl);
   if (MODE == Y_SCALE2 || MODE == Y_PRED) {
     double2* O = (Z == A) ? B : A;
     (void)O;
@@ -294,7 +349,7 @@ __global__ void __launch_bounds__(256) k_dst_y(YArgs a, FFTPlan pl) {
     }
     __syncthreads();
     double2* other = (Z == A) ? B : A;
-    Z = fft_run(Z, other, cp, pl);
+    Z = fft_run(Z, other, cp, pl, Wl);
   }
   for (int idx = threadIdx.x; idx < N * w2; idx += blockDim.x) {
     const int t = idx / w2;
@@ -314,7 +369,7 @@ __global__ void __launch_bounds__(256) k_dst_y(YArgs a, FFTPlan pl) {
 
 static std::vector<int> factorize(int L) {
   std::vector<int> r;
-  const int small[] = {4, 2, 3, 5};
+  const int small[] = {8, 4, 2, 3, 5};
   for (int q : small)
     while (L % q == 0) {
       r.push_back(q);
@@ -351,11 +406,12 @@ void HostFFTPlan::release() {
   W = nullptr;
 }
 
-size_t dst_lds_bytes(const FFTPlan& pl, int nfft) { return (size_t)2 * nfft * pl.L * sizeof(double2); }
+size_t dst_lds_bytes(const FFTPlan& pl, int nfft) { return (size_t)(2 * nfft + 1) * pl.L * sizeof(double2); }
 
 void dst_set_max_lds(size_t bytes) {
   const void* fns[] = {(const void*)k_dst_x<X_PLAIN>, (const void*)k_dst_x<X_APPLYBC>,
-                       (const void*)k_dst_x<X_SUBFROM>, (const void*)k_dst_y<Y_PLAIN>,
+                       (const void*)k_dst_x<X_SUBFROM>, (const void*)k_dst_x<X_ROT>,
+                       (const void*)k_dst_y<Y_PLAIN>,
                        (const void*)k_dst_y<Y_SCALE2>, (const void*)k_dst_y<Y_PRED>};
   for (const void* f : fns) IB_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
@@ -368,6 +424,8 @@ void dst_x(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, cons
     a.out[b] = l.out[b];
   }
   a.bc = l.bc;
+  a.hx = l.hx;
+  a.hy = l.hy;
   a.g = g;
   while (rp > 1 && dst_lds_bytes(pl, rp) > max_lds) rp /= 2;
   a.rp = rp;
@@ -379,6 +437,7 @@ void dst_x(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, cons
     case X_PLAIN: hipLaunchKernelGGL(k_dst_x<X_PLAIN>, grid, dim3(256), lds, st, a, pl); break;
     case X_APPLYBC: hipLaunchKernelGGL(k_dst_x<X_APPLYBC>, grid, dim3(256), lds, st, a, pl); break;
     case X_SUBFROM: hipLaunchKernelGGL(k_dst_x<X_SUBFROM>, grid, dim3(256), lds, st, a, pl); break;
+    case X_ROT: hipLaunchKernelGGL(k_dst_x<X_ROT>, grid, dim3(256), lds, st, a, pl); break;
     default: throw HipError("bad DST row mode");
   }
   IB_LAUNCH_CHECK();
@@ -400,7 +459,9 @@ void dst_y(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, cons
   while (cp > 1 && dst_lds_bytes(pl, mult * cp) > max_lds) cp /= 2;
   a.cp = cp;
   const int npairs = ceil_div(g.m - 1, 2);
-  dim3 grid(ceil_div(npairs, cp), l.batch);
+  const int ntiles = ceil_div(npairs, cp);
+  a.chunk = ceil_div(ntiles, 8);
+  dim3 grid(8 * a.chunk, l.batch);
   const size_t lds = dst_lds_bytes(pl, mult * cp);
   if (lds > max_lds) throw HipError("DST column pass needs more LDS than the device allows");
   switch (mode) {

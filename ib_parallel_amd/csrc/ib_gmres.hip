@@ -6,9 +6,12 @@
 // Givens-rotation residual estimate, convergence when
 // ||r|| <= max(rtol*||b||, atol), divergence when ||r|| >= 1e4*||b||, happy
 // breakdown tolerance 1e-30.  The Krylov basis, Hessenberg matrix and
-// rotations stay in HBM; the vectors have nf entries, so every kernel here is
-// one workgroup and the host only reads the small status block between
-// iterations.  The Schur matvec between the kernels is the expensive part.
+// rotations stay in HBM; the host reads the small status block only.
+//
+// Every kernel returns at once when the status block already carries a
+// convergence reason, so the host may enqueue several iterations ahead of
+// the status it has read (the extra matvecs then write only into basis
+// columns the solution never uses).
 #include "ib_kernels.h"
 
 namespace ibg {
@@ -26,6 +29,11 @@ __device__ double gm_block_sum(double v, double* sh) {
   const double r = sh[0];
   __syncthreads();
   return r;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = warpSize / 2; o > 0; o >>= 1) v += __shfl_down(v, o);
+  return v;
 }
 
 // V0 = b - A x; residual norm, initial tolerance and convergence tests
@@ -66,43 +74,78 @@ __global__ void __launch_bounds__(kGmT) k_gm_start(int nf, const double* __restr
   }
 }
 
-// wavefront-level sum (64 lanes on CDNA4)
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int o = warpSize / 2; o > 0; o >>= 1) v += __shfl_down(v, o);
-  return v;
+// h_j = <w, v_j> for j = 0..it, w = V[it+1] unmodified (classical
+// Gram-Schmidt); one wavefront per dot product, many workgroups.
+__global__ void __launch_bounds__(256) k_gm_dots(int nf, int kmax, const double* __restrict__ V,
+                                                 const GmresDev* __restrict__ s, double* __restrict__ H) {
+  if (s->reason) return;
+  const int it = s->it;
+  const int j = (blockIdx.x * blockDim.x + threadIdx.x) / warpSize;
+  const int lane = threadIdx.x % warpSize;
+  if (j > it) return;
+  const double* w = V + (long)(it + 1) * nf;
+  const double* vj = V + (long)j * nf;
+  double p = 0.0;
+  for (int i = lane; i < nf; i += warpSize) p += w[i] * vj[i];
+  p = wave_sum(p);
+  if (lane == 0) H[(long)it * (kmax + 1) + j] = p;
 }
 
-// One Arnoldi step on w = V[it+1] (already A V[it]): CGS, normalisation,
-// Hessenberg column update with Givens rotations (KSPGMRESUpdateHessenberg).
-// The it+1 dot products are spread over the workgroup's wavefronts, one dot
-// product per wavefront at a time, so no block barrier is needed per product.
-__global__ void __launch_bounds__(kGmT) k_gm_arnoldi(int nf, int kmax, double* __restrict__ V, GmresDev* s,
-                                                     double* __restrict__ H, double* __restrict__ cc,
-                                                     double* __restrict__ ss, double* __restrict__ grs) {
+// w -= sum_j h_j v_j, ||w||, normalisation and the Givens update of the new
+// Hessenberg column (KSPGMRESUpdateHessenberg) on LDS copies.
+__global__ void __launch_bounds__(kGmT) k_gm_update(int nf, int kmax, double* __restrict__ V, GmresDev* s,
+                                                    double* __restrict__ H, double* __restrict__ cc,
+                                                    double* __restrict__ ss, double* __restrict__ grs) {
   __shared__ double sh[kGmT];
-  __shared__ int s_hap;
+  extern __shared__ double dyn[];  // hcol[kmax+1] | cc[kmax] | ss[kmax]
+  __shared__ int s_hap, s_skip;
   __shared__ double s_tt;
+  if (threadIdx.x == 0) s_skip = s->reason != 0;
+  __syncthreads();
+  if (s_skip) return;
   const int it = s->it;
   const int ld = kmax + 1;
+  double* hcol = dyn;
+  double* lcc = dyn + ld;
+  double* lss = lcc + kmax;
   double* w = V + (long)(it + 1) * nf;
-  double* hcol = H + (long)it * ld;
-  const int lane = threadIdx.x % warpSize, wave = threadIdx.x / warpSize, nwave = blockDim.x / warpSize;
-  // h_j = <w, v_j>, all from the unmodified w (classical Gram-Schmidt)
-  for (int j = wave; j <= it; j += nwave) {
-    const double* vj = V + (long)j * nf;
-    double p = 0.0;
-    for (int i = lane; i < nf; i += warpSize) p += w[i] * vj[i];
-    p = wave_sum(p);
-    if (lane == 0) hcol[j] = p;
+  double* hg = H + (long)it * ld;
+  for (int j = threadIdx.x; j <= it; j += blockDim.x) {
+    hcol[j] = hg[j];
+    if (j < it) {
+      lcc[j] = cc[j];
+      lss[j] = ss[j];
+    }
   }
   __syncthreads();
+  // w_i -= sum_j h_j v_j[i]: the j range is split over `nseg` thread groups
+  // (partials in LDS, summed in segment order) to keep more loads in flight
+  double* part = sh;  // reuses the reduction buffer: nseg * nf <= kGmT doubles
+  const int nseg = nf <= kGmT ? (kGmT / nf < 8 ? kGmT / nf : 8) : 1;
+  const int jn = it + 1;
+  const int per = (jn + nseg - 1) / nseg;
+  if (nseg > 1) {
+    const int seg = threadIdx.x / nf, i = threadIdx.x - seg * nf;
+    if (seg < nseg) {
+      const int j0 = seg * per, j1 = j0 + per < jn ? j0 + per : jn;
+      double acc = 0.0;
+      for (int j = j0; j < j1; ++j) acc += -hcol[j] * V[(long)j * nf + i];
+      part[seg * nf + i] = acc;
+    }
+    __syncthreads();
+  }
   double pn = 0.0;
   for (int i = threadIdx.x; i < nf; i += blockDim.x) {
     double acc = w[i];
-    for (int j = 0; j <= it; ++j) acc += -hcol[j] * V[(long)j * nf + i];
+    if (nseg > 1) {
+      for (int sg = 0; sg < nseg; ++sg) acc += part[sg * nf + i];
+    } else {
+      for (int j = 0; j < jn; ++j) acc += -hcol[j] * V[(long)j * nf + i];
+    }
     w[i] = acc;
     pn += acc * acc;
   }
+  __syncthreads();
   const double tt = sqrt(gm_block_sum(pn, sh));
   if (threadIdx.x == 0) {
     hcol[it + 1] = tt;
@@ -111,20 +154,20 @@ __global__ void __launch_bounds__(kGmT) k_gm_arnoldi(int nf, int kmax, double* _
     const int hap = tt < hapbnd;
     s_hap = hap;
     s_tt = tt;
-    double* hh = hcol;
     for (int j = 0; j < it; ++j) {
-      const double t0 = hh[j];
-      hh[j] = cc[j] * t0 + ss[j] * hh[j + 1];
-      hh[j + 1] = -ss[j] * t0 + cc[j] * hh[j + 1];
+      const double t0 = hcol[j];
+      hcol[j] = lcc[j] * t0 + lss[j] * hcol[j + 1];
+      hcol[j + 1] = -lss[j] * t0 + lcc[j] * hcol[j + 1];
     }
     double res;
     if (!hap) {
-      const double t1 = sqrt(hh[it] * hh[it] + hh[it + 1] * hh[it + 1]);
-      cc[it] = hh[it] / t1;
-      ss[it] = hh[it + 1] / t1;
-      grs[it + 1] = -ss[it] * grs[it];
-      grs[it] = cc[it] * grs[it];
-      hh[it] = cc[it] * hh[it] + ss[it] * hh[it + 1];
+      const double t1 = sqrt(hcol[it] * hcol[it] + hcol[it + 1] * hcol[it + 1]);
+      const double c = hcol[it] / t1, sn = hcol[it + 1] / t1;
+      cc[it] = c;
+      ss[it] = sn;
+      grs[it + 1] = -sn * grs[it];
+      grs[it] = c * grs[it];
+      hcol[it] = c * hcol[it] + sn * hcol[it + 1];
       res = fabs(grs[it + 1]);
     } else {
       res = 0.0;
@@ -139,33 +182,57 @@ __global__ void __launch_bounds__(kGmT) k_gm_arnoldi(int nf, int kmax, double* _
     s->reason = reason;
   }
   __syncthreads();
+  for (int j = threadIdx.x; j <= it + 1; j += blockDim.x) hg[j] = hcol[j];
   if (!s_hap) {
     const double inv = 1.0 / s_tt;
     for (int i = threadIdx.x; i < nf; i += blockDim.x) w[i] = w[i] * inv;
   }
 }
 
-// KSPGMRESBuildSoln: back substitution on the rotated Hessenberg, x += V y
-__global__ void __launch_bounds__(kGmT) k_gm_buildsol(int nf, int kmax, const double* __restrict__ V, GmresDev* s,
-                                                      const double* __restrict__ H, const double* __restrict__ grs,
-                                                      double* __restrict__ x, double* __restrict__ nrs) {
+// KSPGMRESBuildSoln: back substitution on the rotated Hessenberg, x += V y.
+// Columns of H are staged in LDS in chunks of `bs` columns so the dependent
+// chain of the substitution runs on LDS values.
+static int bs_chunk(int kmax) {
+  int c = 6000 / (kmax + 1) - 2;  // keep (c + 2)(kmax + 1) doubles under 48 KB
+  return c > 16 ? 16 : (c < 1 ? 1 : c);
+}
+
+__global__ void __launch_bounds__(kGmT) k_gm_buildsol(int nf, int kmax, int kBsChunk, const double* __restrict__ V,
+                                                      GmresDev* s, const double* __restrict__ H,
+                                                      const double* __restrict__ grs, double* __restrict__ x,
+                                                      double* __restrict__ nrs) {
+  extern __shared__ double dyn[];  // t[kmax+1] | y[kmax+1] | chunk[kBsChunk][kmax+1]
   const int it = s->it;  // number of Arnoldi vectors used
   if (it <= 0) return;
   const int ld = kmax + 1;
-  if (threadIdx.x == 0) {
-    const int top = it - 1;
-    nrs[top] = grs[top] / H[(long)top * ld + top];
-    for (int ii = 1; ii <= top; ++ii) {
-      const int k = top - ii;
-      double t = grs[k];
-      for (int j = k + 1; j <= top; ++j) t = t - H[(long)j * ld + k] * nrs[j];
-      nrs[k] = t / H[(long)k * ld + k];
+  double* t = dyn;
+  double* y = dyn + ld;
+  double* chunk = dyn + 2 * ld;
+  const int top = it - 1;
+  for (int j = threadIdx.x; j <= top; j += blockDim.x) t[j] = grs[j];
+  __syncthreads();
+  for (int k1 = top; k1 >= 0; k1 -= kBsChunk) {
+    const int k0 = k1 - kBsChunk + 1 < 0 ? 0 : k1 - kBsChunk + 1;
+    const int nc = k1 - k0 + 1;
+    // stage columns k0..k1, rows 0..k1
+    for (int q = threadIdx.x; q < nc * (k1 + 1); q += blockDim.x) {
+      const int c = q / (k1 + 1), r = q - c * (k1 + 1);
+      chunk[c * ld + r] = H[(long)(k0 + c) * ld + r];
+    }
+    __syncthreads();
+    for (int k = k1; k >= k0; --k) {
+      const double* col = chunk + (k - k0) * ld;
+      if (threadIdx.x == 0) y[k] = t[k] / col[k];
+      __syncthreads();
+      const double yk = y[k];
+      for (int j = threadIdx.x; j < k; j += blockDim.x) t[j] = t[j] - col[j] * yk;
+      __syncthreads();
     }
   }
-  __syncthreads();
+  for (int j = threadIdx.x; j <= top; j += blockDim.x) nrs[j] = y[j];
   for (int i = threadIdx.x; i < nf; i += blockDim.x) {
     double acc = 0.0;
-    for (int j = 0; j < it; ++j) acc += nrs[j] * V[(long)j * nf + i];
+    for (int j = 0; j <= top; ++j) acc += y[j] * V[(long)j * nf + i];
     x[i] = x[i] + acc;
   }
 }
@@ -180,13 +247,19 @@ void gm_start(hipStream_t st, int nf, int kmax, const double* b, const double* A
 
 void gm_arnoldi(hipStream_t st, int nf, int kmax, double* V, GmresDev* s, double* H, double* cc, double* ss,
                 double* grs) {
-  hipLaunchKernelGGL(k_gm_arnoldi, dim3(1), dim3(kGmT), 0, st, nf, kmax, V, s, H, cc, ss, grs);
+  // kmax+1 dot products at most: one wavefront each, 4 per workgroup
+  hipLaunchKernelGGL(k_gm_dots, dim3(ceil_div(kmax + 1, 4)), dim3(256), 0, st, nf, kmax, V, s, H);
+  IB_LAUNCH_CHECK();
+  const size_t lds = sizeof(double) * (size_t)(3 * kmax + 1);
+  hipLaunchKernelGGL(k_gm_update, dim3(1), dim3(kGmT), lds, st, nf, kmax, V, s, H, cc, ss, grs);
   IB_LAUNCH_CHECK();
 }
 
 void gm_buildsol(hipStream_t st, int nf, int kmax, const double* V, GmresDev* s, const double* H, const double* grs,
                  double* x, double* scratch) {
-  hipLaunchKernelGGL(k_gm_buildsol, dim3(1), dim3(kGmT), 0, st, nf, kmax, V, s, H, grs, x, scratch);
+  const int c = bs_chunk(kmax);
+  const size_t lds = sizeof(double) * (size_t)(c + 2) * (kmax + 1);
+  hipLaunchKernelGGL(k_gm_buildsol, dim3(1), dim3(kGmT), lds, st, nf, kmax, c, V, s, H, grs, x, scratch);
   IB_LAUNCH_CHECK();
 }
 

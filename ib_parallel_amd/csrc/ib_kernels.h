@@ -10,7 +10,7 @@ namespace ibg {
 
 // ---------------------------------------------------------------- DST-I ---
 // Row (x) pass modes
-enum { X_PLAIN = 0, X_APPLYBC = 1, X_SUBFROM = 2 };
+enum { X_PLAIN = 0, X_APPLYBC = 1, X_SUBFROM = 2, X_ROT = 3 };
 // Column (y) pass modes
 enum { Y_PLAIN = 0, Y_SCALE2 = 1, Y_PRED = 2 };
 
@@ -22,6 +22,8 @@ struct DstLaunch {
   const double* in2[kMaxBatch] = {};  // Y_PRED: second input b
   double* out[kMaxBatch] = {};
   const double* bc = nullptr;         // X_APPLYBC boundary vector
+  const double* hx = nullptr;         // X_ROT: node values are rot(hx, hy)
+  const double* hy = nullptr;
   const double* tab = nullptr;        // Y_SCALE2 table / Y_PRED lam1
   const double* tab2 = nullptr;       // Y_PRED lam1i
   double div = 1.0;                   // Y_SCALE2 divisor
@@ -118,6 +120,8 @@ struct GmresDev {
 };
 void gm_start(hipStream_t st, int nf, int kmax, const double* b, const double* Ax, double* V, GmresDev* s,
               double* H, double* grs, double rtol, double atol);
+// one Arnoldi step: wavefront-parallel CGS dot products over several
+// workgroups, then orthogonalisation, norm and Givens update in one workgroup
 void gm_arnoldi(hipStream_t st, int nf, int kmax, double* V, GmresDev* s, double* H, double* cc, double* ss,
                 double* grs);
 void gm_buildsol(hipStream_t st, int nf, int kmax, const double* V, GmresDev* s, const double* H,

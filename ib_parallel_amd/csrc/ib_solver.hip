@@ -12,6 +12,7 @@
 // GMRES/FSI status words.
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -109,7 +110,7 @@ struct ib_handle {
   int nb = 0, nf = 0, nbodies = 0, ntors = 0;
   std::vector<int> h_cbody, h_ccomp, h_tors_of;
   std::vector<TorsBody> h_tb;
-  DBuf<double> xb, yb, f, f_rdst, w, rhsf, gw;
+  DBuf<double> xb, yb, f, f_rdst, w, rhsf, gw, opx;
   DBuf<int> comp, tors_of, err_flag;
   RegCSR csr;
   DBuf<TorsBody> tb;
@@ -124,6 +125,8 @@ struct ib_handle {
   int itime = 0, ksp_iter = 0, fsi_iter = 0;
   double err_fsi = 0;
   KTimer tim[2];
+  // captured GMRES batches, keyed by (first basis column, batch length)
+  std::map<std::pair<int, int>, hipGraphExec_t> gm_graphs;
 
   // ------------------------------------------------------------------ util
   void sync() {
@@ -143,7 +146,7 @@ struct ib_handle {
   }
   void ypass(int mode, const DstLaunch& a) {
     tim[1].begin(st, pass_bytes(mode == Y_PRED ? 4 : (mode == Y_SCALE2 ? 2 : 1), 1, a.batch));
-    dst_y(st, py.dev, mode, a, g, 4, max_lds);
+    dst_y(st, py.dev, mode, a, g, 1, max_lds);
     tim[1].end(st);
   }
 
@@ -160,11 +163,14 @@ struct ib_handle {
     xpass(X_PLAIN, c);
   }
 
-  // ainv: src/myfft.F90:161-173; with subfrom, target -= ainv(v) (corrector)
+  // ainv: src/myfft.F90:161-173; with subfrom, target -= ainv(v) (corrector).
+  // v == nullptr: the input is rot(hx, hy) (rot_int + rot_bc), formed in the
+  // load of the first row pass.
   void ainv_dev(const double* v, double* out, bool subfrom) {
     DstLaunch a;
     a.in[0] = v; a.out[0] = t0;
-    xpass(X_PLAIN, a);
+    a.hx = hx; a.hy = hy;
+    xpass(v ? X_PLAIN : X_ROT, a);
     DstLaunch b;
     b.in[0] = t0; b.out[0] = t1; b.tab = lam1i; b.div = normalize;
     ypass(Y_SCALE2, b);
@@ -187,21 +193,23 @@ struct ib_handle {
     }
   }
 
-  // ab_times = a_times2 + b_times2: src/operators_fluid.F90:14-120
+  // ab_times = a_times2 + b_times2: src/operators_fluid.F90:14-120.
+  // hx/hy are zero outside the faces the current weights touch and reg
+  // rewrites every touched face, so they need no clearing per product; the
+  // coarse levels of vv are zero outside the centres coarsify rewrites.
   void schur_dev(const double* x, double* y) {
-    fill(st, hx, g.Fx, 0.0);
-    fill(st, hy, g.Fy, 0.0);
     reg(st, g, csr, w, x, hx, hy);
-    rot(st, g, hx, hy, vv);
-    if (L > 1) fill(st, vv + g.P, (long)(L - 1) * g.P, 0.0);
-    ainv_dev(vv, vv, false);
+    ainv_dev(nullptr, vv, false);
     vort2flux_dev(vv, sn, vx, vy);
     regT(st, g, nf, w, csr.key, vx, vy, 0.0, nullptr, y);
     body_btimes(st, ntors, tb, th, dt, delta, x, y);
   }
 
-  // setup_reg + the sorted tap list; returns false if a stencil left the grid
+  // setup_reg + the sorted tap list; the old touched faces of hx/hy are
+  // cleared first (the invariant schur_dev relies on)
   void rebuild_weights() {
+    fill(st, hx, g.Fx, 0.0);
+    fill(st, hy, g.Fy, 0.0);
     setup_reg(st, g, nf, xb, yb, comp, delta, offx, offy, prm.deltafnc_f32_literals, w, csr.key, err_flag);
     csr.build(st);
   }
@@ -217,7 +225,49 @@ struct ib_handle {
     sync();
   }
 
-  // PETSc-style GMRES(kmax) on the Schur operator, x = initial guess / result
+  // PETSc-style GMRES(kmax) on the Schur operator, x = initial guess / result.
+  // Iterations are enqueued in batches of up to kGmBatch between status
+  // reads; once the device status holds a reason the remaining Arnoldi
+  // kernels of the batch return at once, and the surplus matvecs only write
+  // basis columns past the converged one, which the solution never reads.
+  static constexpr int kGmBatch = 8;
+
+  // nb Arnoldi iterations starting at basis column it0.  Every argument of
+  // these launches is fixed for the handle's lifetime except the column
+  // pointers, so each (it0, nb) is captured once as a HIP graph and replayed.
+  // Kernel timing records events per launch and therefore uses plain
+  // launches.
+  void gmres_batch_direct(int it0, int nb) {
+    for (int q = 0; q < nb; ++q) {
+      schur_dev(V + (size_t)(it0 + q) * nf, V + (size_t)(it0 + q + 1) * nf);
+      gm_arnoldi(st, nf, kmax, V, gst, H, cc, ss, grs);
+    }
+  }
+  void gmres_batch(int it0, int nb) {
+    if (prm.launch_mode != 0 || tim[0].on || tim[1].on) {
+      gmres_batch_direct(it0, nb);
+      return;
+    }
+    auto key = std::make_pair(it0, nb);
+    auto itr = gm_graphs.find(key);
+    if (itr == gm_graphs.end()) {
+      hipGraph_t gr = nullptr;
+      hipGraphExec_t ge = nullptr;
+      IB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        gmres_batch_direct(it0, nb);
+      } catch (...) {
+        (void)hipStreamEndCapture(st, &gr);
+        if (gr) (void)hipGraphDestroy(gr);
+        throw;
+      }
+      IB_HIP(hipStreamEndCapture(st, &gr));
+      IB_HIP(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
+      IB_HIP(hipGraphDestroy(gr));
+      itr = gm_graphs.emplace(key, ge).first;
+    }
+    IB_HIP(hipGraphLaunch(itr->second, st));
+  }
   int gmres_dev(const double* b, double* x, int* its_out) {
     GmresDev init{};
     init.first = 1;
@@ -233,12 +283,12 @@ struct ib_handle {
       if (reason) break;
       int it = 0;
       while (true) {
-        schur_dev(V + (size_t)it * nf, V + (size_t)(it + 1) * nf);
-        gm_arnoldi(st, nf, kmax, V, gst, H, cc, ss, grs);
+        const int nb = std::min(kGmBatch, std::min(kmax - it, maxit - its));
+        gmres_batch(it, nb);
         read_gmres();
         reason = hst->reason;
         its = hst->its;
-        ++it;
+        it = hst->it;
         if (reason || it >= kmax || its >= maxit) break;
       }
       gm_buildsol(st, nf, kmax, V, gst, H, grs, x, nrs);
@@ -319,11 +369,8 @@ struct ib_handle {
     err_fsi = err_glob;
 
     // step 3: corrector (src/timestepper.F90:488-553)
-    fill(st, hx, g.Fx, 0.0);
-    fill(st, hy, g.Fy, 0.0);
     reg(st, g, csr, w, f, hx, hy);
-    rot(st, g, hx, hy, vv);
-    ainv_dev(vv, omega, true);
+    ainv_dev(nullptr, omega, true);
     vort2flux_dev(omega, s, qx, qy);
     sync();
     return status;
@@ -346,7 +393,7 @@ struct ib_handle {
     if (L < 1) return fail(IB_E_INVAL, "mgridlev must be >= 1");
     if (prm.num_stat || prm.sub_domain) return fail(IB_E_UNSUPPORTED, "num_stat and sub_domain paths are not built");
     if (prm.istart != 0) return fail(IB_E_UNSUPPORTED, "restart from .var is not built");
-    if (prm.gmres_restart > 4096) return fail(IB_E_INVAL, "gmres_restart must be <= 4096");
+    if (prm.gmres_restart > 1000) return fail(IB_E_INVAL, "gmres_restart must be <= 1000");
     kmax = prm.gmres_restart;
     dt = prm.dt; Re = prm.re; offx = prm.offsetx; offy = prm.offsety;
     delta = prm.len / (double)m;
@@ -446,11 +493,11 @@ struct ib_handle {
     vv.alloc((size_t)P * L); sn.alloc((size_t)P * L); vx.alloc((size_t)g.Fx * L); vy.alloc((size_t)g.Fy * L);
     hx.alloc(g.Fx); hy.alloc(g.Fy);
     lastbc.alloc((size_t)g.nbc * L); obc.alloc(g.nbc); sbcA.alloc(g.nbc); sbcB.alloc(g.nbc); zbc.alloc(g.nbc);
-    double* zero_these[] = {omega, s, rhs_old, qx, qy, zbc};
-    size_t zero_n[] = {omega.n, s.n, rhs_old.n, qx.n, qy.n, zbc.n};
-    for (int i = 0; i < 6; ++i) fill(st, zero_these[i], (long)zero_n[i], 0.0);
+    double* zero_these[] = {omega, s, rhs_old, qx, qy, zbc, vv, hx, hy};
+    size_t zero_n[] = {omega.n, s.n, rhs_old.n, qx.n, qy.n, zbc.n, vv.n, hx.n, hy.n};
+    for (int i = 0; i < 9; ++i) fill(st, zero_these[i], (long)zero_n[i], 0.0);
 
-    xb.alloc(nf); yb.alloc(nf); f.alloc(nf); f_rdst.alloc(nf); rhsf.alloc(nf); gw.alloc(nf);
+    xb.alloc(nf); yb.alloc(nf); f.alloc(nf); f_rdst.alloc(nf); rhsf.alloc(nf); gw.alloc(nf); opx.alloc(nf);
     w.alloc((size_t)nf * kTaps);
     comp.alloc(nf); tors_of.alloc(nf); err_flag.alloc(1);
     if (nf > 0) {
@@ -486,6 +533,7 @@ struct ib_handle {
 
   ~ib_handle() {
     if (st) (void)hipStreamSynchronize(st);
+    for (auto& kv : gm_graphs) (void)hipGraphExecDestroy(kv.second);
     tim[0].destroy();
     tim[1].destroy();
     csr.release();
@@ -695,6 +743,7 @@ int ib_op_vort2flux(ib_handle* h, double* omega, double* s, double* qx, double* 
   if (s) h->d2h(s, h->sn, P * L);
   if (qx) h->d2h(qx, h->vx, (size_t)h->g.Fx * L);
   if (qy) h->d2h(qy, h->vy, (size_t)h->g.Fy * L);
+  fill(h->st, h->vv, (long)(P * L), 0.0);  // restore the zero coarse-level margins
   h->sync();
   return IB_OK;
   API_CATCH(h)
@@ -703,10 +752,8 @@ int ib_op_vort2flux(ib_handle* h, double* omega, double* s, double* qx, double* 
 int ib_op_reg(ib_handle* h, const double* fin, double* hxo, double* hyo) {
   if (!h) return IB_E_INVAL;
   API_TRY
-  if (h->nf) h->h2d(h->gw, fin, h->nf);
-  fill(h->st, h->hx, h->g.Fx, 0.0);
-  fill(h->st, h->hy, h->g.Fy, 0.0);
-  reg(h->st, h->g, h->csr, h->w, h->gw, h->hx, h->hy);
+  if (h->nf) h->h2d(h->opx, fin, h->nf);
+  reg(h->st, h->g, h->csr, h->w, h->opx, h->hx, h->hy);
   h->d2h(hxo, h->hx, h->g.Fx);
   h->d2h(hyo, h->hy, h->g.Fy);
   h->sync();
@@ -717,11 +764,11 @@ int ib_op_reg(ib_handle* h, const double* fin, double* hxo, double* hyo) {
 int ib_op_regT(ib_handle* h, const double* hxi, const double* hyi, const double* f_in, double* f_out) {
   if (!h) return IB_E_INVAL;
   API_TRY
-  h->h2d(h->hx, hxi, h->g.Fx);
-  h->h2d(h->hy, hyi, h->g.Fy);
-  if (h->nf) h->h2d(h->gw, f_in, h->nf);
-  regT(h->st, h->g, h->nf, h->w, h->csr.key, h->hx, h->hy, 0.0, h->gw, h->rhsf);
-  if (h->nf) h->d2h(f_out, h->rhsf, h->nf);
+  h->h2d(h->vx, hxi, h->g.Fx);
+  h->h2d(h->vy, hyi, h->g.Fy);
+  if (h->nf) h->h2d(h->opx, f_in, h->nf);
+  regT(h->st, h->g, h->nf, h->w, h->csr.key, h->vx, h->vy, 0.0, h->opx, h->gw);
+  if (h->nf) h->d2h(f_out, h->gw, h->nf);
   h->sync();
   return IB_OK;
   API_CATCH(h)
@@ -730,8 +777,8 @@ int ib_op_regT(ib_handle* h, const double* hxi, const double* hyi, const double*
 int ib_op_schur_matvec(ib_handle* h, const double* x, double* y) {
   if (!h) return IB_E_INVAL;
   API_TRY
-  if (h->nf) h->h2d(h->rhsf, x, h->nf);
-  h->schur_dev(h->rhsf, h->gw);
+  if (h->nf) h->h2d(h->opx, x, h->nf);
+  h->schur_dev(h->opx, h->gw);
   if (h->nf) h->d2h(y, h->gw, h->nf);
   h->sync();
   return IB_OK;
@@ -743,11 +790,11 @@ int ib_op_gmres(ib_handle* h, const double* b, double* x, int32_t* its) {
   API_TRY
   if (h->nf) {
     h->h2d(h->rhsf, b, h->nf);
-    h->h2d(h->f_rdst, x, h->nf);
+    h->h2d(h->opx, x, h->nf);
   }
   int it = 0;
-  int rc = h->gmres_dev(h->rhsf, h->f_rdst, &it);
-  if (h->nf) h->d2h(x, h->f_rdst, h->nf);
+  int rc = h->gmres_dev(h->rhsf, h->opx, &it);
+  if (h->nf) h->d2h(x, h->opx, h->nf);
   h->sync();
   if (its) *its = it;
   return rc;
@@ -760,10 +807,10 @@ int ib_op_nonlinear_rot(ib_handle* h, int32_t k, const double* omega, const doub
   API_TRY
   const size_t P = h->g.P;
   h->h2d(h->t0, omega, P);
-  h->h2d(h->hx, qx, h->g.Fx);
-  h->h2d(h->hy, qy, h->g.Fy);
+  h->h2d(h->vx, qx, h->g.Fx);
+  h->h2d(h->vy, qy, h->g.Fy);
   h->h2d(h->obc, bc, h->g.nbc);
-  nl_rhs(h->st, h->g, h->t0, h->hx, h->hy, h->q0x[k - 1], h->obc, h->zbc, 0.0, 0.0, 0.0, 0, nullptr, nullptr,
+  nl_rhs(h->st, h->g, h->t0, h->vx, h->vy, h->q0x[k - 1], h->obc, h->zbc, 0.0, 0.0, 0.0, 0, nullptr, nullptr,
          h->t1);
   h->d2h(rhs, h->t1, P);
   h->sync();

@@ -64,7 +64,9 @@ typedef struct ib_params {
                                     REAL(4) as written (src/grid.F90:972-993) */
   int32_t num_stat;        /* 1 would select the Cholesky path: not built */
   int32_t sub_domain;      /* 1 would select the sub-domain path: not built */
-  int32_t reserved[6];
+  int32_t launch_mode;     /* 0: GMRES iterations replayed as captured HIP
+                              graphs (default); 1: plain stream launches */
+  int32_t reserved[5];
 } ib_params;
 
 /* One body deck input/body.NNN.inp (src/grid.F90:252-288), angles in radians. */

@@ -37,7 +37,7 @@ def test_oracle_exports_mirror_of_abi():
 
 def test_struct_layout_matches_header():
     # offsets the Fortran bind(C) module and the C header agree on
-    assert C.sizeof(abi.IBParams) == 4 * 4 + 8 * 8 + 4 * 6 + 4 * 6
+    assert C.sizeof(abi.IBParams) == 4 * 4 + 8 * 8 + 4 * 7 + 4 * 5
     assert abi.IBParams.dt.offset == 16 and abi.IBParams.gmres_restart.offset == 80
     assert C.sizeof(abi.IBBody) == 8 + 8 * 5 + 16
     assert C.sizeof(abi.IBStepInfo) == 24
