@@ -33,7 +33,7 @@ class IBParams(C.Structure):
         ("fsi_tol", C.c_double), ("atol", C.c_double), ("rtol", C.c_double),
         ("gmres_restart", C.c_int32), ("gmres_max_it", C.c_int32), ("fsi_max_count", C.c_int32),
         ("deltafnc_f32_literals", C.c_int32), ("num_stat", C.c_int32), ("sub_domain", C.c_int32),
-        ("launch_mode", C.c_int32), ("reserved", C.c_int32 * 5),
+        ("launch_mode", C.c_int32), ("schur_mode", C.c_int32), ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -83,6 +83,7 @@ SIGNATURES = {
     "op_reg": (C.c_int, [_P, _D, _D, _D]),
     "op_regT": (C.c_int, [_P, _D, _D, _D, _D]),
     "op_schur_matvec": (C.c_int, [_P, _D, _D]),
+    "op_schur_matrix": (C.c_int, [_P, _D]),
     "op_nonlinear_rot": (C.c_int, [_P, C.c_int32, _D, _D, _D, _D, _D]),
     "op_gmres": (C.c_int, [_P, _D, _D, _I]),
 }
@@ -118,7 +119,7 @@ def dptr(a):
 
 def make_params(m, n, mgridlev, dt, re, len_, offsetx, offsety, fsi_tol=1e-7, atol=1e-7, rtol=1e-7,
                 gmres_restart=30, gmres_max_it=10000, fsi_max_count=1000, deltafnc_f32_literals=1,
-                istart=0, num_stat=0, sub_domain=0, launch_mode=0):
+                istart=0, num_stat=0, sub_domain=0, launch_mode=0, schur_mode=0):
     p = IBParams()
     p.m, p.n, p.mgridlev, p.istart = m, n, mgridlev, istart
     p.dt, p.re, p.len, p.offsetx, p.offsety = dt, re, len_, offsetx, offsety
@@ -126,6 +127,7 @@ def make_params(m, n, mgridlev, dt, re, len_, offsetx, offsety, fsi_tol=1e-7, at
     p.gmres_restart, p.gmres_max_it, p.fsi_max_count = gmres_restart, gmres_max_it, fsi_max_count
     p.deltafnc_f32_literals, p.num_stat, p.sub_domain = deltafnc_f32_literals, num_stat, sub_domain
     p.launch_mode = launch_mode
+    p.schur_mode = schur_mode
     return p
 
 

@@ -8,6 +8,7 @@
 // reference's own addition order, so the result is deterministic run to run.
 #include <hipcub/hipcub.hpp>
 
+#include "ib_device.h"
 #include "ib_kernels.h"
 
 namespace ibg {
@@ -206,6 +207,49 @@ void regT(hipStream_t st, const Geom& g, int nf, const double* w, const int* key
           double shift_x, const double* base, double* out) {
   if (nf <= 0) return;
   hipLaunchKernelGGL(k_regT, dim3(ceil_div(nf, 4)), dim3(256), 0, st, g, nf, w, key, hx, hy, shift_x, base, out);
+  IB_LAUNCH_CHECK();
+}
+
+// Columns of the Schur matrix: S(r, j) = E C s1_j (regT of the level-1 flux,
+// curl taken at the tap faces) + b_times2(e_j)(r).  One wavefront per (row r,
+// column j): tap-order sum as in k_regT, then y = y1 + y2 as in ab_times
+// (src/timestepper.F90:780).
+__device__ __forceinline__ double BRv(const TorsBody& b, int j, double si, double ci) {
+  return j <= b.npts ? (j - 1) * b.deltas * si : -(j - b.npts - 1) * b.deltas * ci;
+}
+
+__global__ void k_schur_cols(Geom g, int nf, int c0, const double* __restrict__ w, const int* __restrict__ key,
+                             const double* __restrict__ s1, long s_stride, const double* __restrict__ bc1,
+                             long bc_stride, const int* __restrict__ tors_of, const TorsBody* __restrict__ tbs,
+                             const double* __restrict__ theta, double dt, double delta, double* __restrict__ S) {
+  const int r = (blockIdx.x * blockDim.x + threadIdx.x) / warpSize;
+  const int lane = threadIdx.x % warpSize;
+  if (r >= nf) return;
+  const int jl = blockIdx.y, j = c0 + jl;
+  const double* s = s1 + (long)jl * s_stride;
+  const double* bc = bc1 + (long)jl * bc_stride;
+  double prod = 0.0;
+  if (lane < kTaps) prod = w[r * kTaps + lane] * curl_at(g, s, bc, key[r * kTaps + lane]);
+  double y1 = 0.0;
+  for (int t = 0; t < kTaps; ++t) y1 = y1 + __shfl(prod, t);
+  double y2 = 0.0;
+  const int tr = tors_of[r];
+  if (tr >= 0 && tr == tors_of[j]) {
+    const TorsBody b = tbs[tr];
+    const double si = sin(b.gamma0 - theta[tr]), ci = cos(b.gamma0 - theta[tr]);
+    const double den = 4.0 / (dt * dt) * b.M + 2.0 / dt * b.C + b.K;
+    const double sc = (delta * delta / b.deltas / dt) * (2 * b.deltas / dt);
+    y2 = BRv(b, r - b.off + 1, si, ci) * BRv(b, j - b.off + 1, si, ci) / den * sc;
+  }
+  if (lane == 0) S[(long)j * nf + r] = 1.0 * y1 + y2;
+}
+
+void schur_columns(hipStream_t st, const Geom& g, int nf, int c0, int nc, const double* w, const int* key,
+                   const double* s1, long s_stride, const double* bc1, long bc_stride, const int* tors_of,
+                   const TorsBody* tb, const double* theta, double dt, double delta, double* S) {
+  if (nf <= 0 || nc <= 0) return;
+  hipLaunchKernelGGL(k_schur_cols, dim3(ceil_div(nf, 4), nc), dim3(256), 0, st, g, nf, c0, w, key, s1, s_stride, bc1,
+                     bc_stride, tors_of, tb, theta, dt, delta, S);
   IB_LAUNCH_CHECK();
 }
 

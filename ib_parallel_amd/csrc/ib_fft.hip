@@ -173,14 +173,37 @@ struct XArgs {
   const double* bc;
   const double* hx;  // X_ROT: the node value is rot(hx, hy)
   const double* hy;
+  long s_in, s_out, s_bc;  // strided: batch b at base[0] + b * stride
+  int strided;              // 0: batch b at in[b], out[b] (b < kMaxBatch)
+  int c0;                  // X_POINTROT: force entry of batch element 0
+  const double* w;         // X_POINTROT: weights, 49 per entry
+  const int* key;          // X_POINTROT: face keys, 49 per entry
   Geom g;
   int rp;
+};
+
+// X_POINTROT: rot(E^T e_k) at node (i, j), formed from the 49 taps of force
+// entry k (the x-face or y-face patch centred on (ix, iy)).
+struct PointPatch {
+  int comp, ix, iy;
+  const double* w;
+  __device__ double face(int c, int i, int j) const {
+    if (c != comp) return 0.0;
+    const int p = i - ix, l = j - iy;
+    if (p < -3 || p > 3 || l < -3 || l > 3) return 0.0;
+    return w[(l + 3) * 7 + (p + 3)];
+  }
+  __device__ double rot(int i, int j) const {
+    return face(2, i, j) - face(2, i - 1, j) - face(1, i, j) + face(1, i, j - 1);
+  }
 };
 
 struct YArgs {
   const double* in[kMaxBatch];
   const double* in2[kMaxBatch];
   double* out[kMaxBatch];
+  long s_in, s_in2, s_out;  // strided: batch b at base[0] + b * stride
+  int strided;              // 0: batch b at in[b], out[b] (b < kMaxBatch)
   const double* tab;
   const double* tab2;
   double div;
@@ -203,16 +226,42 @@ __global__ void __launch_bounds__(256) k_dst_x(XArgs a, FFTPlan pl) {
   double2* A = sm;
   double2* B = sm + rp * L;
   double2* Wl = sm + 2 * rp * L;
-  const double* __restrict__ in = a.in[blockIdx.y];
-  double* out = a.out[blockIdx.y];
+  const int bsel = a.strided ? 0 : blockIdx.y;
+  const double* __restrict__ in = a.in[bsel] ? a.in[bsel] + (long)blockIdx.y * a.s_in : nullptr;
+  double* out = a.out[bsel] + (long)blockIdx.y * a.s_out;
+  const double* bcb = a.bc ? a.bc + (long)blockIdx.y * a.s_bc : nullptr;
   const int pair0 = blockIdx.x * rp;
+  PointPatch pt{};
+  if (MODE == X_POINTROT) {
+    const int k = a.c0 + blockIdx.y;
+    const int ck = a.key[k * kTaps + kTaps / 2];  // centre tap (p = l = 0)
+    const Geom& g = a.g;
+    if (ck < g.Fx) { pt.comp = 1; pt.ix = ck % (g.m + 1) + 1; pt.iy = ck / (g.m + 1) + 1; }
+    else { pt.comp = 2; pt.ix = (ck - g.Fx) % g.m + 1; pt.iy = (ck - g.Fx) / g.m + 1; }
+    pt.w = a.w + (long)k * kTaps;
+    // rows whose nodes see no tap transform to zero: write zeros and leave
+    const int j0 = 2 * pair0 + 2, j1 = 2 * (pair0 + rp) + 1;  // node rows of this workgroup
+    if (j1 < pt.iy - 3 || j0 > pt.iy + 4) {
+      for (int idx = threadIdx.x; idx < rp * N; idx += blockDim.x) {
+        const int s = idx / N, t = idx - s * N;
+        const int ra = 2 * (pair0 + s), rb = ra + 1;
+        if (ra < nrows) out[(long)ra * N + t] = 0.0;
+        if (rb < nrows) out[(long)rb * N + t] = 0.0;
+      }
+      return;
+    }
+  }
   load_twiddles(Wl, pl);
   for (int idx = threadIdx.x; idx < rp * N; idx += blockDim.x) {
     const int s = idx / N;
     const int t = idx - s * N;
     const int ra = 2 * (pair0 + s), rb = ra + 1;
     double xa = 0.0, xb = 0.0;
-    if (MODE == X_ROT) {
+    if (MODE == X_POINTROT) {
+      const int i = t + 2;
+      if (ra < nrows) xa = pt.rot(i, ra + 2);
+      if (rb < nrows) xb = pt.rot(i, rb + 2);
+    } else if (MODE == X_ROT) {
       // rot_int + rot_bc: vort(i,j) = hy(i,j) - hy(i-1,j) - hx(i,j) + hx(i,j-1)
       const Geom& g = a.g;
       const int i = t + 2;
@@ -234,7 +283,7 @@ __global__ void __launch_bounds__(256) k_dst_x(XArgs a, FFTPlan pl) {
       // apply_bc(vort, sbc, 1.d0), src/operators_fluid.F90:539-549
       const int i = t + 2;
       const Geom& g = a.g;
-      const double* bc = a.bc;
+      const double* bc = bcb;
       if (ra < nrows) {
         const int j = ra + 2;
         if (i == 2) xa = xa + bc[g.left + j - 1];
@@ -284,9 +333,10 @@ __global__ void __launch_bounds__(256) k_dst_y(YArgs a, FFTPlan pl) {
   double2* A = sm;
   double2* B = sm + nfft * L;
   double2* Wl = sm + 2 * nfft * L;
-  const double* __restrict__ in = a.in[blockIdx.y];
-  const double* __restrict__ in2 = a.in2[blockIdx.y];
-  double* __restrict__ out = a.out[blockIdx.y];
+  const int bsel = a.strided ? 0 : blockIdx.y;
+  const double* __restrict__ in = a.in[bsel] + (long)blockIdx.y * a.s_in;
+  const double* __restrict__ in2 = a.in2[bsel] ? a.in2[bsel] + (long)blockIdx.y * a.s_in2 : nullptr;
+  double* __restrict__ out = a.out[bsel] + (long)blockIdx.y * a.s_out;
   // workgroups are dealt to the 8 XCDs round-robin: give each XCD a
   // contiguous band of column tiles so neighbouring tiles share its L2
   const int tile = (blockIdx.x % 8) * a.chunk + blockIdx.x / 8;
@@ -411,6 +461,7 @@ size_t dst_lds_bytes(const FFTPlan& pl, int nfft) { return (size_t)(2 * nfft + 1
 void dst_set_max_lds(size_t bytes) {
   const void* fns[] = {(const void*)k_dst_x<X_PLAIN>, (const void*)k_dst_x<X_APPLYBC>,
                        (const void*)k_dst_x<X_SUBFROM>, (const void*)k_dst_x<X_ROT>,
+                       (const void*)k_dst_x<X_POINTROT>,
                        (const void*)k_dst_y<Y_PLAIN>,
                        (const void*)k_dst_y<Y_SCALE2>, (const void*)k_dst_y<Y_PRED>};
   for (const void* f : fns) IB_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
@@ -419,10 +470,20 @@ void dst_set_max_lds(size_t bytes) {
 void dst_x(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, const Geom& g, int rp,
            size_t max_lds) {
   XArgs a{};
-  for (int b = 0; b < l.batch; ++b) {
+  const bool strided = l.strided;
+  if (!strided && l.batch > kMaxBatch) throw HipError("DST row pass: batch > 4 needs strides");
+  if (!strided && (l.s_in || l.s_out || l.s_bc)) throw HipError("DST row pass: strides on an unstrided batch");
+  a.strided = strided;
+  for (int b = 0; b < (strided ? 1 : l.batch); ++b) {
     a.in[b] = l.in[b];
     a.out[b] = l.out[b];
   }
+  a.s_in = l.s_in;
+  a.s_out = l.s_out;
+  a.s_bc = l.s_bc;
+  a.c0 = l.c0;
+  a.w = l.w;
+  a.key = l.key;
   a.bc = l.bc;
   a.hx = l.hx;
   a.hy = l.hy;
@@ -438,6 +499,7 @@ void dst_x(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, cons
     case X_APPLYBC: hipLaunchKernelGGL(k_dst_x<X_APPLYBC>, grid, dim3(256), lds, st, a, pl); break;
     case X_SUBFROM: hipLaunchKernelGGL(k_dst_x<X_SUBFROM>, grid, dim3(256), lds, st, a, pl); break;
     case X_ROT: hipLaunchKernelGGL(k_dst_x<X_ROT>, grid, dim3(256), lds, st, a, pl); break;
+    case X_POINTROT: hipLaunchKernelGGL(k_dst_x<X_POINTROT>, grid, dim3(256), lds, st, a, pl); break;
     default: throw HipError("bad DST row mode");
   }
   IB_LAUNCH_CHECK();
@@ -446,11 +508,18 @@ void dst_x(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, cons
 void dst_y(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, const Geom& g, int cp,
            size_t max_lds) {
   YArgs a{};
-  for (int b = 0; b < l.batch; ++b) {
+  const bool strided = l.strided;
+  if (!strided && l.batch > kMaxBatch) throw HipError("DST column pass: batch > 4 needs strides");
+  if (!strided && (l.s_in || l.s_in2 || l.s_out)) throw HipError("DST column pass: strides on an unstrided batch");
+  a.strided = strided;
+  for (int b = 0; b < (strided ? 1 : l.batch); ++b) {
     a.in[b] = l.in[b];
     a.in2[b] = l.in2[b];
     a.out[b] = l.out[b];
   }
+  a.s_in = l.s_in;
+  a.s_in2 = l.s_in2;
+  a.s_out = l.s_out;
   a.tab = l.tab;
   a.tab2 = l.tab2;
   a.div = l.div;

@@ -7,22 +7,10 @@
 // face fields are 0 (never written by DMGlobalToLocal); node ghosts come from
 // boundary vectors, corners from the bottom/top part (the order in which
 // curl_bc and nonlinear overwrite them).
+#include "ib_device.h"
 #include "ib_kernels.h"
 
 namespace ibg {
-
-__device__ __forceinline__ long ND(const Geom& g, int i, int j) { return (long)(i - 2) + (long)(g.m - 1) * (j - 2); }
-__device__ __forceinline__ long FXI(const Geom& g, int i, int j) { return (long)(i - 1) + (long)(g.m + 1) * (j - 1); }
-__device__ __forceinline__ long FYI(const Geom& g, int i, int j) { return (long)(i - 1) + (long)g.m * (j - 1); }
-
-// node value with ghosts from a boundary vector (src/operators_fluid.F90:448-451, 563-566)
-__device__ __forceinline__ double ext(const Geom& g, const double* a, const double* bc, int i, int j) {
-  if (j == 1) return bc[g.bottom + i - 1];
-  if (j == g.n + 1) return bc[g.top + i - 1];
-  if (i == 1) return bc[g.left + j - 1];
-  if (i == g.m + 1) return bc[g.right + j - 1];
-  return a[ND(g, i, j)];
-}
 
 __global__ void k_fill(double* p, long n, double v) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -80,9 +68,11 @@ void rot(hipStream_t st, const Geom& g, const double* hx, const double* hy, doub
 }
 
 // coarsify: src/operators_fluid.F90:381-419 (fine odd nodes I,J = 3..m-1)
-__global__ void k_coarsify(Geom g, const double* __restrict__ wf, double* __restrict__ wc) {
+__global__ void k_coarsify(Geom g, const double* __restrict__ wf, double* __restrict__ wc, long stride) {
   const int nx = (g.m - 2) / 2, ny = (g.n - 2) / 2;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  wf += blockIdx.y * stride;
+  wc += blockIdx.y * stride;
   if (q >= nx * ny) return;
   const int I = 3 + 2 * (q % nx), J = 3 + 2 * (q / nx);
   const int ic = g.m / 4 + (I + 1) / 2, jc = g.n / 4 + (J + 1) / 2;
@@ -93,9 +83,9 @@ __global__ void k_coarsify(Geom g, const double* __restrict__ wf, double* __rest
   wc[ND(g, ic, jc)] = v;
 }
 
-void coarsify(hipStream_t st, const Geom& g, const double* fine, double* coarse) {
+void coarsify(hipStream_t st, const Geom& g, const double* fine, double* coarse, int batch, long stride) {
   const int nt = ((g.m - 2) / 2) * ((g.n - 2) / 2);
-  hipLaunchKernelGGL(k_coarsify, dim3(ceil_div(nt, 256)), dim3(256), 0, st, g, fine, coarse);
+  hipLaunchKernelGGL(k_coarsify, dim3(ceil_div(nt, 256), batch), dim3(256), 0, st, g, fine, coarse, stride);
   IB_LAUNCH_CHECK();
 }
 

@@ -237,6 +237,30 @@ __global__ void __launch_bounds__(kGmT) k_gm_buildsol(int nf, int kmax, int kBsC
   }
 }
 
+// y = S x, S dense nf x nf column major.  A workgroup owns 64 rows (one
+// per lane) and splits the columns over its 4 wavefronts; the 4 partial sums
+// are added in wavefront order.
+__global__ void __launch_bounds__(256) k_gemv(int nf, const double* __restrict__ S, const double* __restrict__ x,
+                                              double* __restrict__ y) {
+  __shared__ double part[4][64];
+  const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+  const int r = blockIdx.x * 64 + lane;
+  const int per = (nf + 3) / 4;
+  const int j0 = wv * per, j1 = j0 + per < nf ? j0 + per : nf;
+  double acc = 0.0;
+  if (r < nf)
+    for (int j = j0; j < j1; ++j) acc += S[(long)j * nf + r] * x[j];
+  part[wv][lane] = acc;
+  __syncthreads();
+  if (wv == 0 && r < nf) y[r] = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+}
+
+void gemv(hipStream_t st, int nf, const double* S, const double* x, double* y) {
+  if (nf <= 0) return;
+  hipLaunchKernelGGL(k_gemv, dim3(ceil_div(nf, 64)), dim3(256), 0, st, nf, S, x, y);
+  IB_LAUNCH_CHECK();
+}
+
 void gm_start(hipStream_t st, int nf, int kmax, const double* b, const double* Ax, double* V, GmresDev* s, double* H,
               double* grs, double rtol, double atol) {
   (void)kmax;

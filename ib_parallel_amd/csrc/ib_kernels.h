@@ -10,14 +10,21 @@ namespace ibg {
 
 // ---------------------------------------------------------------- DST-I ---
 // Row (x) pass modes
-enum { X_PLAIN = 0, X_APPLYBC = 1, X_SUBFROM = 2, X_ROT = 3 };
+enum { X_PLAIN = 0, X_APPLYBC = 1, X_SUBFROM = 2, X_ROT = 3, X_POINTROT = 4 };
 // Column (y) pass modes
 enum { Y_PLAIN = 0, Y_SCALE2 = 1, Y_PRED = 2 };
 
 constexpr int kMaxBatch = 4;
 
+// A launch covers `batch` arrays: in[b]/out[b] for b < kMaxBatch, or, when a
+// stride is non-zero, base + b * stride (any batch size up to 65535).
 struct DstLaunch {
   int batch = 1;
+  bool strided = false;  // batch b at in[0] + b*s_in, out[0] + b*s_out, ...
+  long s_in = 0, s_in2 = 0, s_out = 0, s_bc = 0;
+  int c0 = 0;                         // X_POINTROT: force entry of batch 0
+  const double* w = nullptr;          // X_POINTROT: weights (49 per entry)
+  const int* key = nullptr;           // X_POINTROT: face keys (49 per entry)
   const double* in[kMaxBatch] = {};
   const double* in2[kMaxBatch] = {};  // Y_PRED: second input b
   double* out[kMaxBatch] = {};
@@ -53,8 +60,8 @@ void fill(hipStream_t st, double* p, long n, double v);
 void copy(hipStream_t st, double* dst, const double* src, long n);
 // rot (C^T): vort = hy(i,j)-hy(i-1,j)-hx(i,j)+hx(i,j-1)
 void rot(hipStream_t st, const Geom& g, const double* hx, const double* hy, double* vort);
-// coarsify fine level -> centre of coarse level
-void coarsify(hipStream_t st, const Geom& g, const double* fine, double* coarse);
+// coarsify fine level -> centre of coarse level (batch arrays `stride` apart)
+void coarsify(hipStream_t st, const Geom& g, const double* fine, double* coarse, int batch = 1, long stride = 0);
 // get_bc of `nlev` coarse levels at once: bc[l] = fac * get_bc(src[l])
 void get_bc(hipStream_t st, const Geom& g, const double* src, long src_stride, double* bc, long bc_stride,
             int nlev, double fac);
@@ -94,6 +101,14 @@ void reg(hipStream_t st, const Geom& g, const RegCSR& c, const double* w, const 
 // shift_x is added to x-face values (q0 of level 1), base may be null (0).
 void regT(hipStream_t st, const Geom& g, int nf, const double* w, const int* key, const double* hx,
           const double* hy, double shift_x, const double* base, double* out);
+// Columns c0..c0+nc-1 of the Schur matrix (ab_times applied to unit
+// vectors): S(r, j) = sum_t w[r,t] * curl(s1_j, bc1_j)(face) + b_times2(e_j)(r),
+// with s1_j the level-1 streamfunction of column j; S is nf x nf, column major
+void schur_columns(hipStream_t st, const Geom& g, int nf, int c0, int nc, const double* w, const int* key,
+                   const double* s1, long s_stride, const double* bc1, long bc_stride, const int* tors_of,
+                   const TorsBody* tb, const double* theta, double dt, double delta, double* S);
+// y = S x for the dense nf x nf Schur matrix (column major)
+void gemv(hipStream_t st, int nf, const double* S, const double* x, double* y);
 // redistribute (src/operators_struct.F90:735-834) in CSR form: writes
 // finter = sum(w f)/sum(w) (if sum(w) > 1e-10, else 0) into touched faces.
 void redist_faces(hipStream_t st, const Geom& g, const RegCSR& c, const double* w, const double* f,

@@ -124,30 +124,41 @@ struct ib_handle {
   int* hflag = nullptr;     // pinned
   int itime = 0, ksp_iter = 0, fsi_iter = 0;
   double err_fsi = 0;
-  KTimer tim[2];
+  KTimer tim[4];
   // captured GMRES batches, keyed by (first basis column, batch length)
   std::map<std::pair<int, int>, hipGraphExec_t> gm_graphs;
+  // dense Schur matrix and the column-batch buffers that assemble it
+  bool assembled = false;
+  static constexpr int kAutoAssembleMax = 8192;  // schur_mode 0 assembles up to this many force entries
+  int ncol = 0;  // columns per assembly batch
+  DBuf<double> Sd, Vb, T0b, T1b, Sb, BCa, BCb;
 
   // ------------------------------------------------------------------ util
   void sync() {
     IB_HIP(hipStreamSynchronize(st));
-    tim[0].harvest();
-    tim[1].harvest();
+    for (auto& t : tim) t.harvest();
   }
 
   // algorithmic bytes of one DST pass launch: arrays read + tables + arrays written
-  double pass_bytes(int nread, int nwrite, int batch) const {
-    return 8.0 * (double)g.P * (double)(nread + nwrite) * (double)batch;
+  // algorithmic HBM bytes of one DST pass: every field array of the batch
+  // read and written once, spectral tables read once per launch
+  double pass_bytes(int nread, int ntab, int nwrite, int batch) const {
+    return 8.0 * (double)g.P * ((double)(nread + nwrite) * (double)batch + (double)ntab);
   }
+  // timing classes: 0/1 row/column passes of the time step, 2/3 the batched
+  // passes that assemble the Schur matrix
+  static int tclass(int base, const DstLaunch& a) { return base + (a.strided && a.batch > kMaxBatch ? 2 : 0); }
   void xpass(int mode, const DstLaunch& a) {
-    tim[0].begin(st, pass_bytes(mode == X_SUBFROM ? 2 : 1, 1, a.batch));
+    KTimer& t = tim[tclass(0, a)];
+    t.begin(st, pass_bytes(mode == X_SUBFROM ? 2 : (mode == X_POINTROT ? 0 : 1), 0, 1, a.batch));
     dst_x(st, px.dev, mode, a, g, (px.dev.L <= 512 ? 2 : 1), max_lds);
-    tim[0].end(st);
+    t.end(st);
   }
   void ypass(int mode, const DstLaunch& a) {
-    tim[1].begin(st, pass_bytes(mode == Y_PRED ? 4 : (mode == Y_SCALE2 ? 2 : 1), 1, a.batch));
+    KTimer& t = tim[tclass(1, a)];
+    t.begin(st, pass_bytes(mode == Y_PRED ? 2 : 1, mode == Y_PRED ? 2 : (mode == Y_SCALE2 ? 1 : 0), 1, a.batch));
     dst_y(st, py.dev, mode, a, g, 1, max_lds);
-    tim[1].end(st);
+    t.end(st);
   }
 
   // s = ctci(w [+ apply_bc(bc)]): src/myfft.F90:177-188
@@ -205,6 +216,70 @@ struct ib_handle {
     body_btimes(st, ntors, tb, th, dt, delta, x, y);
   }
 
+  // Dense Schur matrix S(:, j) = ab_times(e_j) for every force entry j, in
+  // batches of ncol columns: the first row pass builds rot(E^T e_j) from the
+  // entry's 49 weights, ainv and the multi-level vort2flux run on all columns
+  // of the batch at once, and the level-1 flux is evaluated at the taps
+  // (regT) together with b_times2.  Every launch covers ncol arrays, so the
+  // kernels are throughput-bound instead of latency-bound.
+  // Sd and the column-batch buffers; each column of a batch needs L + 3 node
+  // arrays and two bc vectors, and a batch is held to about 4 GiB of HBM.
+  void alloc_assembly() {
+    if (Sd.p || nf == 0) return;
+    const size_t per_col = sizeof(double) * ((size_t)(L + 3) * g.P + 2 * (size_t)g.nbc);
+    const size_t batch_budget = (size_t)4 << 30;
+    ncol = (int)std::max<size_t>(1, std::min<size_t>((size_t)nf, batch_budget / per_col));
+    ncol = std::min(ncol, 65535);  // grid y limit of the batched launches
+    Sd.alloc((size_t)nf * nf);
+    Vb.alloc((size_t)ncol * L * g.P);
+    T0b.alloc((size_t)ncol * g.P);
+    T1b.alloc((size_t)ncol * g.P);
+    Sb.alloc((size_t)ncol * g.P);
+    BCa.alloc((size_t)ncol * g.nbc);
+    BCb.alloc((size_t)ncol * g.nbc);
+    fill(st, Vb, (long)Vb.n, 0.0);  // coarse-level margins stay zero (coarsify rewrites only centres)
+  }
+
+  void assemble_schur() {
+    const long P = g.P, nbc = g.nbc, LP = (long)L * P;
+    for (int c0 = 0; c0 < nf; c0 += ncol) {
+      const int nc = std::min(ncol, nf - c0);
+      DstLaunch a;
+      a.batch = nc; a.strided = true; a.out[0] = T0b; a.s_out = P;
+      a.c0 = c0; a.w = w; a.key = csr.key;  // the input is formed from the weights
+      xpass(X_POINTROT, a);
+      DstLaunch b;
+      b.batch = nc; b.strided = true; b.in[0] = T0b; b.s_in = P; b.out[0] = T1b; b.s_out = P; b.tab = lam1i; b.div = normalize;
+      ypass(Y_SCALE2, b);
+      DstLaunch c;
+      c.batch = nc; c.strided = true; c.in[0] = T1b; c.s_in = P; c.out[0] = Vb; c.s_out = LP;
+      xpass(X_PLAIN, c);
+      for (int k = 1; k < L; ++k) coarsify(st, g, Vb + (k - 1) * P, Vb + k * P, nc, LP);
+      double* bcur = nullptr;
+      for (int k = L - 1; k >= 0; --k) {
+        DstLaunch d;
+        d.batch = nc; d.strided = true; d.in[0] = Vb + k * P; d.s_in = LP; d.out[0] = T0b; d.s_out = P;
+        if (bcur) { d.bc = bcur; d.s_bc = nbc; }
+        xpass(bcur ? X_APPLYBC : X_PLAIN, d);
+        DstLaunch e;
+        e.batch = nc; e.strided = true; e.in[0] = T0b; e.s_in = P; e.out[0] = T1b; e.s_out = P; e.tab = laminv; e.div = 1.0;
+        ypass(Y_SCALE2, e);
+        DstLaunch f2;
+        f2.batch = nc; f2.strided = true; f2.in[0] = T1b; f2.s_in = P; f2.out[0] = Sb; f2.s_out = P;
+        xpass(X_PLAIN, f2);
+        if (k > 0) {
+          double* bnext = (bcur == BCa.p) ? BCb.p : BCa.p;
+          get_bc(st, g, Sb, P, bnext, nbc, nc, 1.0);
+          bcur = bnext;
+        } else {
+          // the coarsest level used a zero boundary: reuse BCb cleared for it
+          schur_columns(st, g, nf, c0, nc, w, csr.key, Sb, P, bcur ? bcur : zbc.p, bcur ? nbc : 0, tors_of, tb, th,
+                        dt, delta, Sd);
+        }
+      }
+    }
+  }
+
   // setup_reg + the sorted tap list; the old touched faces of hx/hy are
   // cleared first (the invariant schur_dev relies on)
   void rebuild_weights() {
@@ -237,14 +312,18 @@ struct ib_handle {
   // pointers, so each (it0, nb) is captured once as a HIP graph and replayed.
   // Kernel timing records events per launch and therefore uses plain
   // launches.
+  void apply_schur(const double* x, double* y) {
+    if (assembled) gemv(st, nf, Sd, x, y);
+    else schur_dev(x, y);
+  }
   void gmres_batch_direct(int it0, int nb) {
     for (int q = 0; q < nb; ++q) {
-      schur_dev(V + (size_t)(it0 + q) * nf, V + (size_t)(it0 + q + 1) * nf);
+      apply_schur(V + (size_t)(it0 + q) * nf, V + (size_t)(it0 + q + 1) * nf);
       gm_arnoldi(st, nf, kmax, V, gst, H, cc, ss, grs);
     }
   }
   void gmres_batch(int it0, int nb) {
-    if (prm.launch_mode != 0 || tim[0].on || tim[1].on) {
+    if (prm.launch_mode != 0 || tim[0].on || tim[1].on || tim[2].on || tim[3].on) {
       gmres_batch_direct(it0, nb);
       return;
     }
@@ -276,7 +355,7 @@ struct ib_handle {
     int reason = 0, its = 0;
     const int maxit = prm.gmres_max_it;
     while (true) {
-      schur_dev(x, gw);
+      apply_schur(x, gw);
       gm_start(st, nf, kmax, b, gw, V, gst, H, grs, prm.rtol, prm.atol);
       read_gmres();
       reason = hst->reason;
@@ -353,6 +432,7 @@ struct ib_handle {
       if (ntors > 0) {
         body_dx(st, ntors, tb, th, dt, delta, f, xb, yb, dabs);
         rebuild_weights();
+        if (assembled) assemble_schur();
         IB_HIP(hipMemcpyAsync(hbuf, dabs.p, sizeof(double) * ntors, hipMemcpyDeviceToHost, st));
         if (read_err_flag()) {
           err = "body point regularisation stencil leaves the grid";
@@ -526,6 +606,14 @@ struct ib_handle {
     csr.alloc(nf * kTaps, g.Fx + g.Fy);
     rebuild_weights();
     if (read_err_flag()) return fail(IB_E_INVAL, "body point regularisation stencil leaves the grid");
+
+    // dense Schur matrix: auto picks it whenever it fits comfortably
+    if (prm.schur_mode < 0 || prm.schur_mode > 2) return fail(IB_E_INVAL, "schur_mode must be 0, 1 or 2");
+    if (prm.schur_mode == 2 || (prm.schur_mode == 0 && nf <= kAutoAssembleMax)) assembled = nf > 0;
+    if (assembled) {
+      alloc_assembly();
+      assemble_schur();
+    }
     itime = prm.istart;
     sync();
     return IB_OK;
@@ -534,8 +622,7 @@ struct ib_handle {
   ~ib_handle() {
     if (st) (void)hipStreamSynchronize(st);
     for (auto& kv : gm_graphs) (void)hipGraphExecDestroy(kv.second);
-    tim[0].destroy();
-    tim[1].destroy();
+    for (auto& t : tim) t.destroy();
     csr.release();
     px.release();
     py.release();
@@ -785,6 +872,19 @@ int ib_op_schur_matvec(ib_handle* h, const double* x, double* y) {
   API_CATCH(h)
 }
 
+int ib_op_schur_matrix(ib_handle* h, double* S) {
+  if (!h || (!S && h->nf)) return IB_E_INVAL;
+  API_TRY
+  if (h->nf) {
+    h->alloc_assembly();
+    h->assemble_schur();
+    IB_HIP(hipMemcpyAsync(S, h->Sd.p, sizeof(double) * (size_t)h->nf * h->nf, hipMemcpyDeviceToHost, h->st));
+  }
+  h->sync();
+  return IB_OK;
+  API_CATCH(h)
+}
+
 int ib_op_gmres(ib_handle* h, const double* b, double* x, int32_t* its) {
   if (!h) return IB_E_INVAL;
   API_TRY
@@ -819,7 +919,7 @@ int ib_op_nonlinear_rot(ib_handle* h, int32_t k, const double* omega, const doub
 }
 
 int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable) {
-  if (!h || kclass < 0 || kclass > 1) return IB_E_INVAL;
+  if (!h || kclass < 0 || kclass > 3) return IB_E_INVAL;
   API_TRY
   h->sync();
   h->tim[kclass].on = enable != 0;
@@ -831,7 +931,7 @@ int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable) {
 }
 
 int ib_kernel_timing_read(ib_handle* h, int32_t kclass, int64_t* launches, double* total_ms, double* total_bytes) {
-  if (!h || kclass < 0 || kclass > 1) return IB_E_INVAL;
+  if (!h || kclass < 0 || kclass > 3) return IB_E_INVAL;
   API_TRY
   h->sync();
   if (launches) *launches = h->tim[kclass].count;

@@ -139,6 +139,12 @@ class Simulation:
         self._check(self._f["op_schur_matvec"](self.h, abi.dptr(x), abi.dptr(y)))
         return y
 
+    def schur_matrix(self):
+        """The Schur matrix by columns, S[:, j] = ab_times(e_j) (nf x nf)."""
+        S = np.empty((self.nf, self.nf))  # row j of this buffer receives column j
+        self._check(self._f["op_schur_matrix"](self.h, abi.dptr(S)))
+        return S.T.copy()
+
     def gmres(self, b, x0=None, allow_noconv=False):
         """Solve the Schur system (ab_times) by GMRES from x0; returns (x, its)."""
         b = np.ascontiguousarray(b, dtype=np.float64)

@@ -66,7 +66,12 @@ typedef struct ib_params {
   int32_t sub_domain;      /* 1 would select the sub-domain path: not built */
   int32_t launch_mode;     /* 0: GMRES iterations replayed as captured HIP
                               graphs (default); 1: plain stream launches */
-  int32_t reserved[5];
+  int32_t schur_mode;      /* 0 auto, 1 matrix-free ab_times per Krylov
+                              iteration, 2 dense Schur matrix assembled from
+                              batched ab_times columns whenever the weights
+                              change (the column construction of
+                              src/timestepper.F90:575-590), GMRES on it */
+  int32_t reserved[4];
 } ib_params;
 
 /* One body deck input/body.NNN.inp (src/grid.F90:252-288), angles in radians. */
@@ -135,6 +140,13 @@ int ib_op_reg(ib_handle* h, const double* f, double* hx, double* hy);
 int ib_op_regT(ib_handle* h, const double* hx, const double* hy, const double* f_in, double* f_out);
 /* ab_times: y = a_times2(x) + b_times2(x), src/timestepper.F90:774-782 */
 int ib_op_schur_matvec(ib_handle* h, const double* x, double* y);
+/* The Schur matrix column by column, S(:, j) = ab_times(e_j), written
+ * column-major (nf x nf): the construction the reference's preprocess
+ * performs with a_times(e_j) before its Cholesky factorisation
+ * (src/timestepper.F90:560-598), here for torsional entries too and with all
+ * columns of a batch transformed at once.  With schur_mode 2 (or 0 and a
+ * small enough body) the time step's GMRES runs on this matrix. */
+int ib_op_schur_matrix(ib_handle* h, double* S);
 /* KSPSolve(ksp_fsi, AB, b, x) (src/timestepper.F90:363-366): GMRES on the
  * Schur operator with x as the initial guess (in) and solution (out); *its
  * gets the iteration count.  Returns IB_E_NOCONV_GMRES if not converged. */
@@ -146,9 +158,11 @@ int ib_op_nonlinear_rot(ib_handle* h, int32_t k, const double* omega, const doub
 
 /* --- instrumentation (no reference counterpart): per-kernel-class timing
  * with HIP events recorded around each launch on the handle's stream.
- * class 0 = DST row pass, 1 = DST column pass.  total_bytes accumulates the
- * algorithmic HBM bytes of the timed launches (each FP64 input array, table
- * and output array read or written once). --- */
+ * class 0 = DST row pass, 1 = DST column pass of the time step; 2 and 3 the
+ * batched row and column passes that assemble the Schur matrix.  total_bytes
+ * accumulates the algorithmic HBM bytes of the timed launches (each FP64
+ * field array read or written once, spectral tables read once per launch).
+ * --- */
 int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable);
 int ib_kernel_timing_read(ib_handle* h, int32_t kclass, int64_t* launches, double* total_ms,
                           double* total_bytes);

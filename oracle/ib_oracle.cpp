@@ -1007,6 +1007,21 @@ int orc_op_schur_matvec(orc_handle* h, const double* x, double* y) {
   return IB_OK;
 }
 
+// Column-by-column construction of the Schur matrix as the reference's
+// preprocess does it (src/timestepper.F90:575-590, a_times(e_j) for rigid
+// bodies); here column j = ab_times(e_j), which adds b_times2 for torsional
+// entries.  Column-major, nf x nf.
+int orc_op_schur_matrix(orc_handle* h, double* S) {
+  const int nf = h->nf;
+  std::vector<double> e(nf, 0.0);
+  for (int j = 0; j < nf; ++j) {
+    e[j] = 1.0;
+    h->schur(e.data(), S + (size_t)j * nf);
+    e[j] = 0.0;
+  }
+  return IB_OK;
+}
+
 int orc_op_nonlinear_rot(orc_handle* h, int32_t k, const double* omega, const double* qx, const double* qy,
                          const double* bc, double* rhs) {
   if (k < 1 || k > h->L) return IB_E_INVAL;

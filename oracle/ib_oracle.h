@@ -37,6 +37,7 @@ int orc_op_vort2flux(orc_handle* h, double* omega, double* s, double* qx, double
 int orc_op_reg(orc_handle* h, const double* f, double* hx, double* hy);
 int orc_op_regT(orc_handle* h, const double* hx, const double* hy, const double* f_in, double* f_out);
 int orc_op_schur_matvec(orc_handle* h, const double* x, double* y);
+int orc_op_schur_matrix(orc_handle* h, double* S);
 int orc_op_gmres(orc_handle* h, const double* b, double* x, int32_t* its);
 int orc_op_nonlinear_rot(orc_handle* h, int32_t k, const double* omega, const double* qx,
                          const double* qy, const double* bc, double* rhs);

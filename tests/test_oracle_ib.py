@@ -7,7 +7,11 @@ weights summing to one per force entry.  b_times2: rank one, symmetric, equal
 to c BR BR^T / den assembled densely.  Newmark body update with zero fluid
 force: the average-acceleration recurrence of a free torsional spring.
 GMRES: the solution of the dense Schur matrix assembled column by column from
-the matvec (the reference's num_stat preprocess) solved by LAPACK.
+the matvec (the reference's num_stat preprocess) solved by LAPACK.  Schur
+matrix: on one grid level C (Delta^-1 ainv) C^T is symmetric positive definite
+in exact arithmetic (Delta^-1 and ainv are diagonal in the same DST basis), so
+a transposed or dropped factor breaks symmetry; with more levels the boundary
+coupling makes it slightly non-symmetric.
 """
 import numpy as np
 import pytest
@@ -127,6 +131,27 @@ def test_gmres_matches_dense_solve():
     assert its <= n + 1
     assert rel(x, np.linalg.solve(A, rhs)) < 1e-8
     assert rel(A @ x, rhs) < 1e-12
+
+
+@pytest.mark.parametrize("torsional", (False, True))
+def test_schur_matrix_single_level_spd(torsional):
+    p, b = decks.tiny_case(m=40, n=32, mgridlev=1, torsional=torsional)
+    sim = ob.OracleSim(p, b)
+    S = sim.schur_matrix()
+    x = np.random.default_rng(9).standard_normal(sim.nf)
+    assert rel(S @ x, sim.schur_matvec(x)) < 1e-14
+    assert np.abs(S - S.T).max() < 1e-13 * np.abs(S).max()
+    assert np.linalg.eigvalsh(0.5 * (S + S.T)).min() > 0
+
+
+def test_schur_matrix_multilevel_columns():
+    p, b = decks.tiny_case(m=40, n=32, mgridlev=3)
+    sim = ob.OracleSim(p, b)
+    S = sim.schur_matrix()
+    for j in (0, sim.nf // 2, sim.nf - 1):
+        assert np.array_equal(S[:, j], sim.schur_matvec(np.eye(sim.nf)[j]))
+    ev = np.linalg.eigvals(S)
+    assert np.abs(ev.imag).max() < 1e-12 and ev.real.min() > 0
 
 
 def test_gmres_restart_is_petsc_gmres30():
