@@ -107,7 +107,10 @@ def run_gpu(args):
     roof["other_classes"] = [summary(kc) for kc in classes if kc != dom]
     roof["timing_region"] = (f"{nprof} further steps with per-launch HIP events (plain launches), "
                              f"{secs_prof / nprof * 1e3:.1f} ms/step")
-    res = {"sim": sim, "desc": desc, "secs": secs, "its": its, "fsi": fsi, "roofline": roof}
+    form, nfaces, builds = sim.schur_info()
+    res = {"sim": sim, "desc": desc, "secs": secs, "its": its, "fsi": fsi, "roofline": roof,
+           "schur": {"form": {1: "matrix-free", 2: "assembled", 3: "face-space"}[form], "faces": nfaces,
+                     "green_builds": builds}}
     return res
 
 
@@ -169,7 +172,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": g["desc"], "m": sim.m, "n": sim.n, "mgridlev": sim.L, "nb": sim.nf // 2,
                    "gmres_its_per_step": round(g["its"] / args.steps, 2),
-                   "fsi_its_per_step": round(g["fsi"] / args.steps, 2), "form": "plain"},
+                   "fsi_its_per_step": round(g["fsi"] / args.steps, 2), "form": "plain",
+                   "schur_operator": g["schur"]},
         "roofline": g["roofline"],
     }
     if not args.no_cpu:

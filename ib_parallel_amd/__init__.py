@@ -41,6 +41,12 @@ class GpuSim(Simulation):
     def __init__(self, params, bodies):
         super().__init__(load(), params, bodies, prefix="ib_")
 
+    def schur_info(self):
+        """(form, face-set size, G computations): form 1 matrix-free, 2 assembled, 3 face-space."""
+        mode, nfaces, builds = C.c_int32(), C.c_int32(), C.c_int64()
+        self._check(self._f["schur_info"](self.h, C.byref(mode), C.byref(nfaces), C.byref(builds)))
+        return mode.value, nfaces.value, builds.value
+
     def kernel_timing(self, kclass, enable=True):
         self._check(self._f["kernel_timing"](self.h, kclass, 1 if enable else 0))
 

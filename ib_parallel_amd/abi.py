@@ -33,7 +33,8 @@ class IBParams(C.Structure):
         ("fsi_tol", C.c_double), ("atol", C.c_double), ("rtol", C.c_double),
         ("gmres_restart", C.c_int32), ("gmres_max_it", C.c_int32), ("fsi_max_count", C.c_int32),
         ("deltafnc_f32_literals", C.c_int32), ("num_stat", C.c_int32), ("sub_domain", C.c_int32),
-        ("launch_mode", C.c_int32), ("schur_mode", C.c_int32), ("reserved", C.c_int32 * 4),
+        ("launch_mode", C.c_int32), ("schur_mode", C.c_int32),
+        ("face_margin", C.c_int32), ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -89,6 +90,7 @@ SIGNATURES = {
 }
 
 GPU_ONLY = {
+    "schur_info": (C.c_int, [_P, _I, _I, C.POINTER(C.c_int64)]),
     "kernel_timing": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "kernel_timing_read": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),
@@ -119,7 +121,7 @@ def dptr(a):
 
 def make_params(m, n, mgridlev, dt, re, len_, offsetx, offsety, fsi_tol=1e-7, atol=1e-7, rtol=1e-7,
                 gmres_restart=30, gmres_max_it=10000, fsi_max_count=1000, deltafnc_f32_literals=1,
-                istart=0, num_stat=0, sub_domain=0, launch_mode=0, schur_mode=0):
+                istart=0, num_stat=0, sub_domain=0, launch_mode=0, schur_mode=0, face_margin=0):
     p = IBParams()
     p.m, p.n, p.mgridlev, p.istart = m, n, mgridlev, istart
     p.dt, p.re, p.len, p.offsetx, p.offsety = dt, re, len_, offsetx, offsety
@@ -128,6 +130,7 @@ def make_params(m, n, mgridlev, dt, re, len_, offsetx, offsety, fsi_tol=1e-7, at
     p.deltafnc_f32_literals, p.num_stat, p.sub_domain = deltafnc_f32_literals, num_stat, sub_domain
     p.launch_mode = launch_mode
     p.schur_mode = schur_mode
+    p.face_margin = face_margin
     return p
 
 

@@ -218,6 +218,19 @@ __device__ __forceinline__ double BRv(const TorsBody& b, int j, double si, doubl
   return j <= b.npts ? (j - 1) * b.deltas * si : -(j - b.npts - 1) * b.deltas * ci;
 }
 
+// b_times2(e_j)(r): the rank-one torsional block (src/operators_struct.F90:14-127)
+__device__ __forceinline__ double btimes_entry(int r, int j, const int* __restrict__ tors_of,
+                                               const TorsBody* __restrict__ tbs, const double* __restrict__ theta,
+                                               double dt, double delta) {
+  const int tr = tors_of[r];
+  if (tr < 0 || tr != tors_of[j]) return 0.0;
+  const TorsBody b = tbs[tr];
+  const double si = sin(b.gamma0 - theta[tr]), ci = cos(b.gamma0 - theta[tr]);
+  const double den = 4.0 / (dt * dt) * b.M + 2.0 / dt * b.C + b.K;
+  const double sc = (delta * delta / b.deltas / dt) * (2 * b.deltas / dt);
+  return BRv(b, r - b.off + 1, si, ci) * BRv(b, j - b.off + 1, si, ci) / den * sc;
+}
+
 __global__ void k_schur_cols(Geom g, int nf, int c0, const double* __restrict__ w, const int* __restrict__ key,
                              const double* __restrict__ s1, long s_stride, const double* __restrict__ bc1,
                              long bc_stride, const int* __restrict__ tors_of, const TorsBody* __restrict__ tbs,
@@ -232,16 +245,7 @@ __global__ void k_schur_cols(Geom g, int nf, int c0, const double* __restrict__ 
   if (lane < kTaps) prod = w[r * kTaps + lane] * curl_at(g, s, bc, key[r * kTaps + lane]);
   double y1 = 0.0;
   for (int t = 0; t < kTaps; ++t) y1 = y1 + __shfl(prod, t);
-  double y2 = 0.0;
-  const int tr = tors_of[r];
-  if (tr >= 0 && tr == tors_of[j]) {
-    const TorsBody b = tbs[tr];
-    const double si = sin(b.gamma0 - theta[tr]), ci = cos(b.gamma0 - theta[tr]);
-    const double den = 4.0 / (dt * dt) * b.M + 2.0 / dt * b.C + b.K;
-    const double sc = (delta * delta / b.deltas / dt) * (2 * b.deltas / dt);
-    y2 = BRv(b, r - b.off + 1, si, ci) * BRv(b, j - b.off + 1, si, ci) / den * sc;
-  }
-  if (lane == 0) S[(long)j * nf + r] = 1.0 * y1 + y2;
+  if (lane == 0) S[(long)j * nf + r] = 1.0 * y1 + btimes_entry(r, j, tors_of, tbs, theta, dt, delta);
 }
 
 void schur_columns(hipStream_t st, const Geom& g, int nf, int c0, int nc, const double* w, const int* key,
@@ -250,6 +254,76 @@ void schur_columns(hipStream_t st, const Geom& g, int nf, int c0, int nc, const 
   if (nf <= 0 || nc <= 0) return;
   hipLaunchKernelGGL(k_schur_cols, dim3(ceil_div(nf, 4), nc), dim3(256), 0, st, g, nf, c0, w, key, s1, s_stride, bc1,
                      bc_stride, tors_of, tb, theta, dt, delta, S);
+  IB_LAUNCH_CHECK();
+}
+
+__global__ void k_green_cols(Geom g, int nF, int c0, const int* __restrict__ fkeys, const double* __restrict__ s1,
+                             long s_stride, const double* __restrict__ bc1, long bc_stride, double* __restrict__ G) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nF) return;
+  const int jl = blockIdx.y;
+  G[(long)(c0 + jl) * nF + b] = curl_at(g, s1 + (long)jl * s_stride, bc1 + (long)jl * bc_stride, fkeys[b]);
+}
+
+void green_columns(hipStream_t st, const Geom& g, int nF, int c0, int nc, const int* fkeys, const double* s1,
+                   long s_stride, const double* bc1, long bc_stride, double* G) {
+  if (nF <= 0 || nc <= 0) return;
+  hipLaunchKernelGGL(k_green_cols, dim3(ceil_div(nF, 256), nc), dim3(256), 0, st, g, nF, c0, fkeys, s1, s_stride,
+                     bc1, bc_stride, G);
+  IB_LAUNCH_CHECK();
+}
+
+__global__ void k_map_taps(int ntaps, const int* __restrict__ key, const int* __restrict__ fmap,
+                           int* __restrict__ tapidx, int* __restrict__ flag) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntaps) return;
+  const int idx = fmap[key[t]];
+  tapidx[t] = idx;
+  if (idx < 0) flag[0] = 1;
+}
+
+void map_taps(hipStream_t st, int nf, const int* key, const int* fmap, int* tapidx, int* flag) {
+  const int ntaps = nf * kTaps;
+  if (ntaps <= 0) return;
+  hipLaunchKernelGGL(k_map_taps, dim3(ceil_div(ntaps, 256)), dim3(256), 0, st, ntaps, key, fmap, tapidx, flag);
+  IB_LAUNCH_CHECK();
+}
+
+// T(:, j) = sum_q w[j,q] G(:, tapidx[j,q]): one thread per face row b
+__global__ void __launch_bounds__(256) k_green_right(int nF, const double* __restrict__ w,
+                                                     const int* __restrict__ tapidx, const double* __restrict__ G,
+                                                     double* __restrict__ T) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (b >= nF) return;
+  double acc = 0.0;
+  for (int q = 0; q < kTaps; ++q) acc += w[j * kTaps + q] * G[(long)tapidx[j * kTaps + q] * nF + b];
+  T[(long)j * nF + b] = acc;
+}
+
+// S(i, j) = sum_p w[i,p] T(tapidx[i,p], j) + b_times2(e_j)(i): one thread per row i
+__global__ void __launch_bounds__(256) k_green_left(int nf, int nF, const double* __restrict__ w,
+                                                    const int* __restrict__ tapidx, const double* __restrict__ T,
+                                                    const int* __restrict__ tors_of, const TorsBody* __restrict__ tbs,
+                                                    const double* __restrict__ theta, double dt, double delta,
+                                                    double* __restrict__ S) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= nf) return;
+  const double* Tj = T + (long)j * nF;
+  double acc = 0.0;
+  for (int p = 0; p < kTaps; ++p) acc += w[i * kTaps + p] * Tj[tapidx[i * kTaps + p]];
+  S[(long)j * nf + i] = 1.0 * acc + btimes_entry(i, j, tors_of, tbs, theta, dt, delta);
+}
+
+void schur_from_green(hipStream_t st, int nf, int nF, const double* w, const int* tapidx, const double* G,
+                      double* T, const int* tors_of, const TorsBody* tb, const double* theta, double dt,
+                      double delta, double* S) {
+  if (nf <= 0 || nF <= 0) return;
+  hipLaunchKernelGGL(k_green_right, dim3(ceil_div(nF, 256), nf), dim3(256), 0, st, nF, w, tapidx, G, T);
+  IB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_green_left, dim3(ceil_div(nf, 256), nf), dim3(256), 0, st, nf, nF, w, tapidx, T, tors_of, tb,
+                     theta, dt, delta, S);
   IB_LAUNCH_CHECK();
 }
 

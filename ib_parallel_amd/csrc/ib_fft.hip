@@ -175,21 +175,23 @@ struct XArgs {
   const double* hy;
   long s_in, s_out, s_bc;  // strided: batch b at base[0] + b * stride
   int strided;              // 0: batch b at in[b], out[b] (b < kMaxBatch)
-  int c0;                  // X_POINTROT: force entry of batch element 0
-  const double* w;         // X_POINTROT: weights, 49 per entry
-  const int* key;          // X_POINTROT: face keys, 49 per entry
+  int c0;                  // X_POINTROT: force entry (or face) of batch element 0
+  const double* w;         // X_POINTROT: weights, 49 per entry; null: unit face sources
+  const int* key;          // X_POINTROT: face keys, 49 per entry (or one per face)
   Geom g;
   int rp;
 };
 
 // X_POINTROT: rot(E^T e_k) at node (i, j), formed from the 49 taps of force
-// entry k (the x-face or y-face patch centred on (ix, iy)).
+// entry k (the x-face or y-face patch centred on (ix, iy)); with no weights,
+// rot of the unit face source at (ix, iy).
 struct PointPatch {
   int comp, ix, iy;
   const double* w;
   __device__ double face(int c, int i, int j) const {
     if (c != comp) return 0.0;
     const int p = i - ix, l = j - iy;
+    if (!w) return (p == 0 && l == 0) ? 1.0 : 0.0;
     if (p < -3 || p > 3 || l < -3 || l > 3) return 0.0;
     return w[(l + 3) * 7 + (p + 3)];
   }
@@ -234,11 +236,12 @@ __global__ void __launch_bounds__(256) k_dst_x(XArgs a, FFTPlan pl) {
   PointPatch pt{};
   if (MODE == X_POINTROT) {
     const int k = a.c0 + blockIdx.y;
-    const int ck = a.key[k * kTaps + kTaps / 2];  // centre tap (p = l = 0)
+    // centre tap (p = l = 0) of force entry k, or face k of a face list
+    const int ck = a.w ? a.key[k * kTaps + kTaps / 2] : a.key[k];
     const Geom& g = a.g;
     if (ck < g.Fx) { pt.comp = 1; pt.ix = ck % (g.m + 1) + 1; pt.iy = ck / (g.m + 1) + 1; }
     else { pt.comp = 2; pt.ix = (ck - g.Fx) % g.m + 1; pt.iy = (ck - g.Fx) / g.m + 1; }
-    pt.w = a.w + (long)k * kTaps;
+    pt.w = a.w ? a.w + (long)k * kTaps : nullptr;
     // rows whose nodes see no tap transform to zero: write zeros and leave
     const int j0 = 2 * pair0 + 2, j1 = 2 * (pair0 + rp) + 1;  // node rows of this workgroup
     if (j1 < pt.iy - 3 || j0 > pt.iy + 4) {

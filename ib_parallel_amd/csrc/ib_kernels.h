@@ -107,6 +107,20 @@ void regT(hipStream_t st, const Geom& g, int nf, const double* w, const int* key
 void schur_columns(hipStream_t st, const Geom& g, int nf, int c0, int nc, const double* w, const int* key,
                    const double* s1, long s_stride, const double* bc1, long bc_stride, const int* tors_of,
                    const TorsBody* tb, const double* theta, double dt, double delta, double* S);
+// Level-1 flux operator on a face set F (the faces near the bodies):
+// G(b, a) = curl(s1_a, bc1_a)(F[b]) for columns a = c0..c0+nc-1, where s1_a
+// is the level-1 streamfunction of the unit face source F[a]; G is nF x nF,
+// column major.
+void green_columns(hipStream_t st, const Geom& g, int nF, int c0, int nc, const int* fkeys, const double* s1,
+                   long s_stride, const double* bc1, long bc_stride, double* G);
+// tapidx[t] = fmap[key[t]] for the nf*49 taps; flag[0] = 1 if any tap's face
+// is outside the face set (fmap < 0)
+void map_taps(hipStream_t st, int nf, const int* key, const int* fmap, int* tapidx, int* flag);
+// The Schur matrix from G: T = G W^T (nF x nf), then
+// S(i, j) = sum_p w[i,p] T(tapidx[i,p], j) + b_times2(e_j)(i).
+void schur_from_green(hipStream_t st, int nf, int nF, const double* w, const int* tapidx, const double* G,
+                      double* T, const int* tors_of, const TorsBody* tb, const double* theta, double dt,
+                      double delta, double* S);
 // y = S x for the dense nf x nf Schur matrix (column major)
 void gemv(hipStream_t st, int nf, const double* S, const double* x, double* y);
 // redistribute (src/operators_struct.F90:735-834) in CSR form: writes

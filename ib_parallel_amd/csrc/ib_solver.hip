@@ -132,6 +132,15 @@ struct ib_handle {
   static constexpr int kAutoAssembleMax = 8192;  // schur_mode 0 assembles up to this many force entries
   int ncol = 0;  // columns per assembly batch
   DBuf<double> Sd, Vb, T0b, T1b, Sb, BCa, BCb;
+  // face-space form: face set, its index map, G = Q on the set, T = G W^T
+  bool facemode = false;
+  static constexpr int kFaceMarginDefault = 2;
+  int face_margin = kFaceMarginDefault;
+  static constexpr size_t kAutoGreenMax = (size_t)1 << 28;  // schur_mode 0: G up to 2 GiB
+  int nF = 0;
+  int64_t green_builds = 0;
+  DBuf<int> Fkeys, fmap, tapidx, face_flag;
+  DBuf<double> Gm, Tm;
 
   // ------------------------------------------------------------------ util
   void sync() {
@@ -216,19 +225,14 @@ struct ib_handle {
     body_btimes(st, ntors, tb, th, dt, delta, x, y);
   }
 
-  // Dense Schur matrix S(:, j) = ab_times(e_j) for every force entry j, in
-  // batches of ncol columns: the first row pass builds rot(E^T e_j) from the
-  // entry's 49 weights, ainv and the multi-level vort2flux run on all columns
-  // of the batch at once, and the level-1 flux is evaluated at the taps
-  // (regT) together with b_times2.  Every launch covers ncol arrays, so the
-  // kernels are throughput-bound instead of latency-bound.
-  // Sd and the column-batch buffers; each column of a batch needs L + 3 node
-  // arrays and two bc vectors, and a batch is held to about 4 GiB of HBM.
-  void alloc_assembly() {
+  // Sd and the column-batch buffers for batches of up to `need` columns;
+  // each column needs L + 3 node arrays and two bc vectors, and a batch is
+  // held to about 4 GiB of HBM (larger column counts run in several batches).
+  void alloc_assembly(int need) {
     if (Sd.p || nf == 0) return;
     const size_t per_col = sizeof(double) * ((size_t)(L + 3) * g.P + 2 * (size_t)g.nbc);
     const size_t batch_budget = (size_t)4 << 30;
-    ncol = (int)std::max<size_t>(1, std::min<size_t>((size_t)nf, batch_budget / per_col));
+    ncol = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(need, 1), batch_budget / per_col));
     ncol = std::min(ncol, 65535);  // grid y limit of the batched launches
     Sd.alloc((size_t)nf * nf);
     Vb.alloc((size_t)ncol * L * g.P);
@@ -240,44 +244,132 @@ struct ib_handle {
     fill(st, Vb, (long)Vb.n, 0.0);  // coarse-level margins stay zero (coarsify rewrites only centres)
   }
 
-  void assemble_schur() {
+  // One batch of nc operator columns through ainv and the multi-level
+  // vort2flux.  Column c of the batch starts from rot(E^T e_(c0+c)) (w given:
+  // the 49 taps of force entry c0+c) or from rot of the unit face source
+  // keys[c0+c] (w null).  Leaves the level-1 streamfunctions in Sb (stride P)
+  // and returns their boundary vectors (stride nbc, or zbc with stride 0 when
+  // there is a single level).
+  const double* column_batch(int c0, int nc, const double* wts, const int* keys, long* bc_stride) {
     const long P = g.P, nbc = g.nbc, LP = (long)L * P;
-    for (int c0 = 0; c0 < nf; c0 += ncol) {
-      const int nc = std::min(ncol, nf - c0);
-      DstLaunch a;
-      a.batch = nc; a.strided = true; a.out[0] = T0b; a.s_out = P;
-      a.c0 = c0; a.w = w; a.key = csr.key;  // the input is formed from the weights
-      xpass(X_POINTROT, a);
-      DstLaunch b;
-      b.batch = nc; b.strided = true; b.in[0] = T0b; b.s_in = P; b.out[0] = T1b; b.s_out = P; b.tab = lam1i; b.div = normalize;
-      ypass(Y_SCALE2, b);
-      DstLaunch c;
-      c.batch = nc; c.strided = true; c.in[0] = T1b; c.s_in = P; c.out[0] = Vb; c.s_out = LP;
-      xpass(X_PLAIN, c);
-      for (int k = 1; k < L; ++k) coarsify(st, g, Vb + (k - 1) * P, Vb + k * P, nc, LP);
-      double* bcur = nullptr;
-      for (int k = L - 1; k >= 0; --k) {
-        DstLaunch d;
-        d.batch = nc; d.strided = true; d.in[0] = Vb + k * P; d.s_in = LP; d.out[0] = T0b; d.s_out = P;
-        if (bcur) { d.bc = bcur; d.s_bc = nbc; }
-        xpass(bcur ? X_APPLYBC : X_PLAIN, d);
-        DstLaunch e;
-        e.batch = nc; e.strided = true; e.in[0] = T0b; e.s_in = P; e.out[0] = T1b; e.s_out = P; e.tab = laminv; e.div = 1.0;
-        ypass(Y_SCALE2, e);
-        DstLaunch f2;
-        f2.batch = nc; f2.strided = true; f2.in[0] = T1b; f2.s_in = P; f2.out[0] = Sb; f2.s_out = P;
-        xpass(X_PLAIN, f2);
-        if (k > 0) {
-          double* bnext = (bcur == BCa.p) ? BCb.p : BCa.p;
-          get_bc(st, g, Sb, P, bnext, nbc, nc, 1.0);
-          bcur = bnext;
-        } else {
-          // the coarsest level used a zero boundary: reuse BCb cleared for it
-          schur_columns(st, g, nf, c0, nc, w, csr.key, Sb, P, bcur ? bcur : zbc.p, bcur ? nbc : 0, tors_of, tb, th,
-                        dt, delta, Sd);
-        }
+    DstLaunch a;
+    a.batch = nc; a.strided = true; a.out[0] = T0b; a.s_out = P;
+    a.c0 = c0; a.w = wts; a.key = keys;  // the input is formed from the weights or the face list
+    xpass(X_POINTROT, a);
+    DstLaunch b;
+    b.batch = nc; b.strided = true; b.in[0] = T0b; b.s_in = P; b.out[0] = T1b; b.s_out = P; b.tab = lam1i;
+    b.div = normalize;
+    ypass(Y_SCALE2, b);
+    DstLaunch c;
+    c.batch = nc; c.strided = true; c.in[0] = T1b; c.s_in = P; c.out[0] = Vb; c.s_out = LP;
+    xpass(X_PLAIN, c);
+    for (int k = 1; k < L; ++k) coarsify(st, g, Vb + (k - 1) * P, Vb + k * P, nc, LP);
+    double* bcur = nullptr;  // boundary from the next coarser level (none on the coarsest)
+    for (int k = L - 1; k >= 0; --k) {
+      DstLaunch d;
+      d.batch = nc; d.strided = true; d.in[0] = Vb + k * P; d.s_in = LP; d.out[0] = T0b; d.s_out = P;
+      if (bcur) { d.bc = bcur; d.s_bc = nbc; }
+      xpass(bcur ? X_APPLYBC : X_PLAIN, d);
+      DstLaunch e;
+      e.batch = nc; e.strided = true; e.in[0] = T0b; e.s_in = P; e.out[0] = T1b; e.s_out = P; e.tab = laminv;
+      e.div = 1.0;
+      ypass(Y_SCALE2, e);
+      DstLaunch f2;
+      f2.batch = nc; f2.strided = true; f2.in[0] = T1b; f2.s_in = P; f2.out[0] = Sb; f2.s_out = P;
+      xpass(X_PLAIN, f2);
+      if (k > 0) {
+        double* bnext = (bcur == BCa.p) ? BCb.p : BCa.p;
+        get_bc(st, g, Sb, P, bnext, nbc, nc, 1.0);
+        bcur = bnext;
       }
     }
+    *bc_stride = bcur ? nbc : 0;
+    return bcur ? bcur : zbc.p;
+  }
+
+  // Dense Schur matrix S(:, j) = ab_times(e_j) for every force entry j, in
+  // batches of ncol columns: ainv and the multi-level vort2flux run on all
+  // columns of a batch at once, and the level-1 flux is evaluated at the taps
+  // (regT) together with b_times2.  Every launch covers ncol arrays, so the
+  // kernels are throughput-bound instead of latency-bound.
+  void assemble_schur() {
+    for (int c0 = 0; c0 < nf; c0 += ncol) {
+      const int nc = std::min(ncol, nf - c0);
+      long sbc = 0;
+      const double* bc = column_batch(c0, nc, w, csr.key, &sbc);
+      schur_columns(st, g, nf, c0, nc, w, csr.key, Sb, g.P, bc, sbc, tors_of, tb, th, dt, delta, Sd);
+    }
+  }
+
+  // ---- face-space form (moving bodies): the level-1 flux operator Q =
+  // C (multi-level CtC)^-1 ainv C^T does not depend on the body, only E does.
+  // G = Q restricted to the faces within face_margin cells of the current
+  // taps is computed once by batched unit-source columns; every weight
+  // rebuild then forms S = W G W^T + B with two small contractions.  G is
+  // recomputed (for a new face set) only when a tap leaves the set.
+  void build_face_set() {
+    // touched faces = unique tap keys of the current weights
+    int nr = 0;
+    IB_HIP(hipMemcpyAsync(hflag, csr.nruns, sizeof(int), hipMemcpyDeviceToHost, st));
+    sync();
+    nr = *hflag;
+    std::vector<int> uniq(nr);
+    if (nr) IB_HIP(hipMemcpy(uniq.data(), csr.uniq, sizeof(int) * nr, hipMemcpyDeviceToHost));
+    std::vector<int> fm((size_t)g.Fx + g.Fy, -1);
+    const int r = face_margin;
+    for (int key : uniq) {
+      const bool xf = key < g.Fx;
+      const int ni = xf ? g.m + 1 : g.m, nj = xf ? g.n : g.n + 1, base = xf ? 0 : g.Fx;
+      const int i = (key - base) % ni, j = (key - base) / ni;
+      for (int dj = -r; dj <= r; ++dj)
+        for (int di = -r; di <= r; ++di) {
+          const int ii = i + di, jj = j + dj;
+          if (ii >= 0 && ii < ni && jj >= 0 && jj < nj) fm[base + (size_t)jj * ni + ii] = 0;
+        }
+    }
+    std::vector<int> keys;
+    for (size_t q = 0; q < fm.size(); ++q)
+      if (fm[q] == 0) {
+        fm[q] = (int)keys.size();
+        keys.push_back((int)q);
+      }
+    nF = (int)keys.size();
+    Fkeys.alloc(std::max(1, nF));
+    fmap.alloc(fm.size());
+    IB_HIP(hipMemcpy(Fkeys.p, keys.data(), sizeof(int) * nF, hipMemcpyHostToDevice));
+    IB_HIP(hipMemcpy(fmap.p, fm.data(), sizeof(int) * fm.size(), hipMemcpyHostToDevice));
+  }
+
+  void compute_green() {
+    if (Gm.n < (size_t)nF * nF) Gm.alloc((size_t)nF * nF);
+    if (Tm.n < (size_t)nF * nf) Tm.alloc((size_t)nF * nf);
+    for (int c0 = 0; c0 < nF; c0 += ncol) {
+      const int nc = std::min(ncol, nF - c0);
+      long sbc = 0;
+      const double* bc = column_batch(c0, nc, nullptr, Fkeys, &sbc);
+      green_columns(st, g, nF, c0, nc, Fkeys, Sb, g.P, bc, sbc, Gm);
+    }
+    ++green_builds;
+  }
+
+  // the Schur matrix for the current weights (and theta): called after every
+  // weight rebuild once the stencil check has passed
+  void refresh_schur() {
+    if (!assembled) return;
+    if (!facemode) {
+      assemble_schur();
+      return;
+    }
+    IB_HIP(hipMemsetAsync(face_flag.p, 0, sizeof(int), st));
+    map_taps(st, nf, csr.key, fmap, tapidx, face_flag);
+    IB_HIP(hipMemcpyAsync(hflag, face_flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    sync();
+    if (*hflag) {
+      build_face_set();
+      compute_green();
+      map_taps(st, nf, csr.key, fmap, tapidx, face_flag);
+    }
+    schur_from_green(st, nf, nF, w, tapidx, Gm, Tm, tors_of, tb, th, dt, delta, Sd);
   }
 
   // setup_reg + the sorted tap list; the old touched faces of hx/hy are
@@ -432,12 +524,12 @@ struct ib_handle {
       if (ntors > 0) {
         body_dx(st, ntors, tb, th, dt, delta, f, xb, yb, dabs);
         rebuild_weights();
-        if (assembled) assemble_schur();
         IB_HIP(hipMemcpyAsync(hbuf, dabs.p, sizeof(double) * ntors, hipMemcpyDeviceToHost, st));
         if (read_err_flag()) {
           err = "body point regularisation stencil leaves the grid";
           return IB_E_INVAL;
         }
+        refresh_schur();
         err_glob = 0.0;
         for (int t = 0; t < ntors; ++t) err_glob = std::max(err_glob, hbuf[t]);
       } else {
@@ -608,11 +700,36 @@ struct ib_handle {
     if (read_err_flag()) return fail(IB_E_INVAL, "body point regularisation stencil leaves the grid");
 
     // dense Schur matrix: auto picks it whenever it fits comfortably
-    if (prm.schur_mode < 0 || prm.schur_mode > 2) return fail(IB_E_INVAL, "schur_mode must be 0, 1 or 2");
-    if (prm.schur_mode == 2 || (prm.schur_mode == 0 && nf <= kAutoAssembleMax)) assembled = nf > 0;
+    if (prm.schur_mode < 0 || prm.schur_mode > 3) return fail(IB_E_INVAL, "schur_mode must be 0, 1, 2 or 3");
+    if (prm.face_margin < -1) return fail(IB_E_INVAL, "face_margin must be >= -1");
+    face_margin = prm.face_margin == 0 ? kFaceMarginDefault : std::max(0, prm.face_margin);
+    if (nf > 0) {
+      if (prm.schur_mode == 3 || (prm.schur_mode == 0 && ntors > 0)) {
+        assembled = facemode = true;
+      } else if (prm.schur_mode == 2 || (prm.schur_mode == 0 && nf <= kAutoAssembleMax)) {
+        assembled = true;
+      }
+    }
+    if (facemode) {
+      face_flag.alloc(1);
+      tapidx.alloc((size_t)nf * kTaps);
+      build_face_set();
+      if (prm.schur_mode == 0 && (size_t)nF * nF > kAutoGreenMax) {
+        // too many faces near the bodies for G: assemble S directly instead
+        facemode = false;
+        assembled = nf <= kAutoAssembleMax;
+        if (assembled) alloc_assembly(nf);
+      } else {
+        alloc_assembly(nF + nF / 2);  // room for the face set to grow as bodies move
+        compute_green();
+        map_taps(st, nf, csr.key, fmap, tapidx, face_flag);
+      }
+    } else if (assembled) {
+      alloc_assembly(nf);
+    }
     if (assembled) {
-      alloc_assembly();
-      assemble_schur();
+      if (facemode) schur_from_green(st, nf, nF, w, tapidx, Gm, Tm, tors_of, tb, th, dt, delta, Sd);
+      else assemble_schur();
     }
     itime = prm.istart;
     sync();
@@ -876,7 +993,7 @@ int ib_op_schur_matrix(ib_handle* h, double* S) {
   if (!h || (!S && h->nf)) return IB_E_INVAL;
   API_TRY
   if (h->nf) {
-    h->alloc_assembly();
+    h->alloc_assembly(h->nf);
     h->assemble_schur();
     IB_HIP(hipMemcpyAsync(S, h->Sd.p, sizeof(double) * (size_t)h->nf * h->nf, hipMemcpyDeviceToHost, h->st));
   }
@@ -916,6 +1033,14 @@ int ib_op_nonlinear_rot(ib_handle* h, int32_t k, const double* omega, const doub
   h->sync();
   return IB_OK;
   API_CATCH(h)
+}
+
+int ib_schur_info(ib_handle* h, int32_t* mode, int32_t* nfaces, int64_t* green_builds) {
+  if (!h) return IB_E_INVAL;
+  if (mode) *mode = h->facemode ? 3 : (h->assembled ? 2 : 1);
+  if (nfaces) *nfaces = h->facemode ? h->nF : 0;
+  if (green_builds) *green_builds = h->green_builds;
+  return IB_OK;
 }
 
 int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable) {

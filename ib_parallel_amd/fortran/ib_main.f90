@@ -90,6 +90,7 @@ program ib_main
   p%num_stat = merge(1, 0, num_stat); p%sub_domain = merge(1, 0, sub_domain)
   p%launch_mode = 0
   p%schur_mode = 0
+  p%face_margin = 0
   p%reserved = 0
   call getenv_int('IB_GMRES_RESTART', p%gmres_restart)
 

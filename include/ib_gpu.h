@@ -66,12 +66,18 @@ typedef struct ib_params {
   int32_t sub_domain;      /* 1 would select the sub-domain path: not built */
   int32_t launch_mode;     /* 0: GMRES iterations replayed as captured HIP
                               graphs (default); 1: plain stream launches */
-  int32_t schur_mode;      /* 0 auto, 1 matrix-free ab_times per Krylov
-                              iteration, 2 dense Schur matrix assembled from
-                              batched ab_times columns whenever the weights
-                              change (the column construction of
-                              src/timestepper.F90:575-590), GMRES on it */
-  int32_t reserved[4];
+  int32_t schur_mode;      /* GMRES operator: 1 matrix-free ab_times per
+                              Krylov iteration; 2 dense Schur matrix assembled
+                              from batched ab_times columns whenever the
+                              weights change (the column construction of
+                              src/timestepper.F90:575-590); 3 dense Schur
+                              matrix formed as W G W^T + B from the level-1
+                              flux operator G on the faces near the bodies
+                              (G recomputed only when a tap leaves that set);
+                              0 auto: 3 with torsional bodies, else 2 */
+  int32_t face_margin;     /* schur_mode 3: cells of margin around the taps'
+                              faces in the face set (0: 2 cells, -1: none) */
+  int32_t reserved[3];
 } ib_params;
 
 /* One body deck input/body.NNN.inp (src/grid.F90:252-288), angles in radians. */
@@ -164,6 +170,9 @@ int ib_op_nonlinear_rot(ib_handle* h, int32_t k, const double* omega, const doub
  * field array read or written once, spectral tables read once per launch).
  * --- */
 int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable);
+/* the Schur form in use (1 matrix-free, 2 assembled by columns, 3 face-space
+ * G), the size of the face set and how many times G has been computed */
+int ib_schur_info(ib_handle* h, int32_t* mode, int32_t* nfaces, int64_t* green_builds);
 int ib_kernel_timing_read(ib_handle* h, int32_t kclass, int64_t* launches, double* total_ms,
                           double* total_bytes);
 

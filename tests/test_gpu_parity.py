@@ -109,7 +109,8 @@ def _step_parity(p, b, steps, tol=1e-10, its_slack=0.0, ftol=None):
 # schur_mode 1: ab_times applied matrix-free in every Krylov iteration;
 # 2: GMRES on the assembled matrix, whose products round differently, so the
 # iteration that first crosses the tolerance may move by one.
-MODES = pytest.mark.parametrize("mode,slack", ((1, 0.0), (2, 0.01)))
+# 3: the same matrix formed from the flux operator on the faces near the body.
+MODES = pytest.mark.parametrize("mode,slack", ((1, 0.0), (2, 0.01), (3, 0.01)))
 
 
 @MODES
@@ -141,6 +142,24 @@ def test_step_parity_restart30_deck_tolerance():
     """
     _step_parity(*decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-7, gmres_restart=30), steps=3, tol=1e-5,
                  its_slack=0.02, ftol=1e-3)
+
+
+def test_face_space_recompute_when_taps_leave_the_set():
+    """With no margin the face set goes stale as soon as a body point crosses
+    a cell boundary; G is then recomputed for the new set inside the FSI loop
+    and the steps still match the oracle."""
+    p, b = decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-12, gmres_restart=80, dt=0.02, schur_mode=3,
+                           face_margin=-1)
+    g, o = _step_parity(p, b, steps=8, its_slack=0.01)
+    form, nfaces, builds = g.schur_info()
+    assert form == 3 and nfaces > 0
+    assert builds > 1, builds
+    p2, b2 = decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-12, gmres_restart=80, dt=0.02, schur_mode=3)
+    g2 = gpu(p2, b2)
+    for _ in range(8):
+        g2.advance()
+    form2, nfaces2, builds2 = g2.schur_info()
+    assert nfaces2 > nfaces and builds2 == 1  # the 2-cell margin covers this motion
 
 
 def test_full_size_plate_invariants():

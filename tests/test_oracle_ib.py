@@ -154,6 +154,36 @@ def test_schur_matrix_multilevel_columns():
     assert np.abs(ev.imag).max() < 1e-12 and ev.real.min() > 0
 
 
+def test_schur_matrix_face_space_factorisation():
+    """S = E_F G E_F^T + B, with E_F the regT matrix restricted to the faces the
+    weights touch and G the level-1 flux operator C (multi-level CtC)^-1 ainv
+    C^T between those faces (unit face sources) -- the factorisation the
+    moving-body Schur form is built on."""
+    p, b = decks.tiny_case(m=40, n=32, mgridlev=3)
+    sim = ob.OracleSim(p, b)
+    nF_all = sim.Fx + sim.Fy
+    zx, zy = np.zeros(sim.Fx), np.zeros(sim.Fy)
+    E = np.empty((sim.nf, nF_all))
+    for a in range(nF_all):
+        hx, hy = zx.copy(), zy.copy()
+        (hx if a < sim.Fx else hy)[a if a < sim.Fx else a - sim.Fx] = 1.0
+        E[:, a] = sim.regT(hx, hy)
+    F = np.flatnonzero(np.abs(E).max(0) > 0)
+    G = np.empty((F.size, F.size))
+    for c, a in enumerate(F):
+        hx, hy = zx.copy(), zy.copy()
+        (hx if a < sim.Fx else hy)[a if a < sim.Fx else a - sim.Fx] = 1.0
+        om = np.zeros(sim.L * sim.P)
+        om[: sim.P] = sim.ainv(sim.rot(hx, hy))
+        _, _, qx, qy = sim.vort2flux(om)
+        G[:, c] = np.concatenate([qx[: sim.Fx], qy[: sim.Fy]])[F]
+    B = np.stack([ob.btimes(sim, np.eye(sim.nf)[j]) for j in range(sim.nf)], 1)
+    EF = E[:, F]
+    S = sim.schur_matrix()
+    assert rel((EF @ G @ EF.T + B).ravel(), S.ravel()) < 1e-13
+    assert np.abs(B).max() > 0  # the torsional block is present
+
+
 def test_gmres_restart_is_petsc_gmres30():
     """Restarted GMRES(30) on the same system needs more iterations than full GMRES."""
     p, b = decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-9, gmres_restart=30)
