@@ -261,6 +261,483 @@ void gemv(hipStream_t st, int nf, const double* S, const double* x, double* y) {
   IB_LAUNCH_CHECK();
 }
 
+// ------------------------------------------------------------------------
+// Whole solve in one workgroup for a dense Schur matrix.  The same PETSc
+// 3.11 GMRES as the kernels above (CGS, Givens, restart, stopping tests),
+// but every iteration stays inside the kernel: S is held in registers
+// (MAXC columns of one row per thread, the rest read from L2), w, the new
+// Hessenberg column, the rotations and the residual vector in LDS, and the
+// Krylov basis in HBM/L2.  Iterations cost a few microseconds instead of
+// three dependent launches.
+// ------------------------------------------------------------------------
+struct GmDenseArgs {
+  int nf, kmax, maxit;
+  int nvl;     // basis vectors held in LDS (the rest are read from HBM/L2)
+  int bsc;     // Hessenberg columns staged per back-substitution chunk
+  const double* S;
+  const double* b;
+  double* x;
+  double* V;
+  double* H;
+  GmresDev* s;
+  double rtol, atol;
+};
+
+constexpr int kGmRem = 12;  // S entries per thread fetched from L2 in one unrolled batch
+
+// acc += -sum_{j in [q0, q1)} h[j] * v_j[r] in j order, v_j from the LDS cache
+// for j < nvl and from HBM/L2 otherwise; loads are issued in batches of 8.
+__device__ __forceinline__ double gm_axpy_col(double acc, const double* __restrict__ h, const double* cache,
+                                              const double* __restrict__ V, int nf, int nvl, int q0, int q1,
+                                              int r) {
+  const int qa = min(q1, nvl);
+  for (int j = q0; j < qa; j += 8) {
+    double v[8], hv[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int jj = min(j + c, qa - 1);
+      v[c] = cache[(long)jj * nf + r];
+      hv[c] = h[jj];
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc += -(j + c < qa ? hv[c] : 0.0) * v[c];
+  }
+  for (int j = max(q0, nvl); j < q1; j += 8) {
+    double v[8], hv[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int jj = min(j + c, q1 - 1);
+      v[c] = V[(long)jj * nf + r];
+      hv[c] = h[jj];
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc += -(j + c < q1 ? hv[c] : 0.0) * v[c];
+  }
+  return acc;
+}
+
+// The thread index as a value the compiler cannot see through: per-thread
+// addresses derived from it are recomputed in each phase instead of being
+// hoisted out of the solve loop and kept live in registers for the whole
+// kernel (which would crowd out the register-resident part of S).
+__device__ __forceinline__ int gm_opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+// Cross-lane data movement through DPP (register-level, no LDS crossbar).
+// CTRL is the gfx9 dpp_ctrl code; doubles move as two 32-bit halves.
+template <int CTRL>
+__device__ __forceinline__ double gm_dpp(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// every lane gets the sum over its row of 16 lanes: quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_ror:4, row_ror:8
+__device__ __forceinline__ double gm_row_sum16(double v) {
+  v += gm_dpp<0xB1>(v);
+  v += gm_dpp<0x4E>(v);
+  v += gm_dpp<0x124>(v);
+  v += gm_dpp<0x128>(v);
+  return v;
+}
+
+__device__ __forceinline__ double gm_readlane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// sum over the 64 lanes of a wavefront (uniform result, fixed order)
+__device__ __forceinline__ double gm_wave_sum(double v) {
+  const double rs = gm_row_sum16(v);
+  return (gm_readlane(rs, 0) + gm_readlane(rs, 16)) + (gm_readlane(rs, 32) + gm_readlane(rs, 48));
+}
+
+// sum over i = sub, sub+16, ... < nf of w[i] v[i], in batches of 8 loads
+// per operand with clamped indices (no branches between the loads)
+template <bool LDS>
+__device__ __forceinline__ double gm_dot16(const double* w, const double* v, int nf, int sub) {
+  double p = 0.0;
+  for (int i0 = sub; i0 < nf; i0 += 16 * 8) {
+    double wv[8], vv[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int i = min(i0 + 16 * c, nf - 1);
+      wv[c] = w[i];
+      vv[c] = v[i];
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) p += (i0 + 16 * c < nf ? wv[c] : 0.0) * vv[c];
+  }
+  return p;
+}
+
+// row-segment layout of the matrix-vector product: thread (seg, r) owns row
+// r and columns [j0, j1); nseg segments of nf threads
+struct GmLayout {
+  int nseg, seg, r, j0, j1;
+  bool act;
+  __device__ GmLayout(int t, int nf, int T) {
+    nseg = nf <= T ? min(16, T / nf) : 1;
+    seg = t / nf;
+    r = t - seg * nf;
+    act = nf <= T && seg < nseg;
+    const int per = (nf + nseg - 1) / nseg;
+    j0 = act ? seg * per : 0;
+    j1 = act ? min(nf, j0 + per) : 0;
+  }
+};
+
+// part[0*nf+i] + part[1*nf+i] + ... + part[(nseg-1)*nf+i] in that order (nseg <= 16)
+__device__ __forceinline__ double gm_sum_parts(const double* part, int nf, int i, int nseg) {
+  double v[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) v[q] = part[min(q, nseg - 1) * nf + i];
+  double acc = v[0];
+#pragma unroll
+  for (int q = 1; q < 16; ++q) acc += (q < nseg ? v[q] : 0.0);
+  return acc;
+}
+
+__device__ __forceinline__ double gm_block_sum2(double v, double* red, int T) {
+  const int t = gm_opaque(threadIdx.x);
+  v = gm_wave_sum(v);
+  if ((t & 63) == 0) red[t >> 6] = v;
+  __syncthreads();
+  double tot = 0.0;
+  for (int q = 0; q < T / 64; ++q) tot += red[q];
+  __syncthreads();
+  return tot;
+}
+
+// out = S vin (vin, out in LDS); S entries [j0, j0+MAXC) of the thread's row
+// segment come from registers, the rest from L2 in unrolled batches.  Ends
+// with a barrier.
+template <int MAXC>
+__device__ __forceinline__ void gm_gemv(const double (&sreg)[MAXC], const double* __restrict__ S, int nf, int T,
+                                        const double* vin, double* out, double* part) {
+  const GmLayout g(gm_opaque(threadIdx.x), nf, T);
+  if (nf <= T) {
+    // branch-free (clamped indices, zero weights past the segment) so that
+    // all loads of a batch are in flight together
+    if (g.act) {
+      double acc = 0.0;
+#pragma unroll
+      for (int c0 = 0; c0 < MAXC; c0 += 8) {
+        double vv[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) vv[c] = c0 + c < MAXC ? vin[min(g.j0 + c0 + c, nf - 1)] : 0.0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+          if (c0 + c < MAXC) acc += sreg[c0 + c] * vv[c];  // sreg is 0 past the segment
+      }
+      for (int jb = g.j0 + MAXC; jb < g.j1; jb += kGmRem) {
+        double sv[kGmRem], xv[kGmRem];
+#pragma unroll
+        for (int c = 0; c < kGmRem; ++c) {
+          const int j = min(jb + c, g.j1 - 1);
+          sv[c] = S[(long)j * nf + g.r];
+          xv[c] = vin[j];
+        }
+#pragma unroll
+        for (int c = 0; c < kGmRem; ++c) acc += (jb + c < g.j1 ? sv[c] : 0.0) * xv[c];
+      }
+      part[g.seg * nf + g.r] = acc;
+    }
+    __syncthreads();
+    for (int i = gm_opaque(threadIdx.x); i < nf; i += T) out[i] = gm_sum_parts(part, nf, i, g.nseg);
+  } else {
+    for (int i = gm_opaque(threadIdx.x); i < nf; i += T) {
+      double acc = 0.0;
+      for (int j = 0; j < nf; ++j) acc += S[(long)j * nf + i] * vin[j];
+      out[i] = acc;
+    }
+  }
+  __syncthreads();
+}
+
+template <int MAXC>
+__global__ void __launch_bounds__(kGmT) k_gmres_dense(GmDenseArgs a) {
+  extern __shared__ double dyn[];
+  const int nf = a.nf, kmax = a.kmax, ld = kmax + 1, T = kGmT;
+  double* vl = dyn;               // nf: the basis vector being multiplied
+  double* wl = vl + nf;           // nf: w
+  double* hcol = wl + nf;         // kmax + 2
+  double* lcc = hcol + kmax + 2;  // kmax
+  double* lss = lcc + kmax;       // kmax
+  double* grs = lss + kmax;       // kmax + 1
+  double* yv = grs + kmax + 1;    // kmax + 1
+  double* part = yv + kmax + 1;   // T
+  double* cache = part + T;       // basis vectors 0..nvl-1; Hessenberg chunks in the back substitution
+  const int nvl = a.nvl;
+  __shared__ double red[kGmT / 64];
+  __shared__ double s_val[2];
+  __shared__ int s_flag[2];
+
+  double sreg[MAXC];  // the first MAXC columns of the thread's row segment, for the whole solve
+  {
+    const GmLayout g(threadIdx.x, nf, T);
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) sreg[c] = (g.j0 + c < g.j1) ? a.S[(long)(g.j0 + c) * nf + g.r] : 0.0;
+  }
+
+  int its = 0, reason = 0, first = 1;
+  double rn0 = 0.0, ttol = 0.0;
+  while (true) {
+    // ---- cycle start: r = b - S x
+    for (int i = gm_opaque(threadIdx.x); i < nf; i += T) vl[i] = a.x[i];
+    __syncthreads();
+    gm_gemv<MAXC>(sreg, a.S, nf, T, vl, wl, part);
+    double pr = 0.0, pb = 0.0;
+    for (int i = gm_opaque(threadIdx.x); i < nf; i += T) {
+      const double rv = a.b[i] - wl[i];
+      wl[i] = rv;
+      pr += rv * rv;
+      pb += a.b[i] * a.b[i];
+    }
+    const double res0 = sqrt(gm_block_sum2(pr, red, T));
+    const double bn = sqrt(gm_block_sum2(pb, red, T));
+    if (first) {
+      rn0 = bn > 0.0 ? bn : res0;
+      ttol = fmax(a.rtol * rn0, a.atol);
+      first = 0;
+    }
+    if (res0 == 0.0 || res0 <= ttol) reason = 1;
+    else if (res0 >= 1e4 * rn0) reason = -1;
+    if (reason) break;
+    if (threadIdx.x == 0) grs[0] = res0;
+    for (int i = gm_opaque(threadIdx.x); i < nf; i += T) {
+      const double v0 = wl[i] / res0;
+      vl[i] = v0;
+      a.V[i] = v0;
+      if (nvl > 0) cache[i] = v0;
+    }
+    __syncthreads();
+    int it = 0;
+    while (true) {
+      gm_gemv<MAXC>(sreg, a.S, nf, T, vl, wl, part);  // w = S v_it
+      // classical Gram-Schmidt coefficients h_j = <w, v_j>, j = 0..it: one
+      // row of 16 lanes per coefficient, reduced with DPP
+      {
+        const int t = gm_opaque(threadIdx.x);
+        const int sub = t & 15;
+        const int ndots = it + 1;
+        for (int j = t >> 4; j < ndots; j += T / 16) {
+          double p = 0.0;
+          if (j < nvl) p = gm_dot16<true>(wl, cache + (long)j * nf, nf, sub);
+          else p = gm_dot16<false>(wl, a.V + (long)j * nf, nf, sub);
+          p = gm_row_sum16(p);
+          if (sub == 0) hcol[j] = p;
+        }
+      }
+      __syncthreads();
+      // w -= sum_j h_j v_j (j range split over the row segments, partials in fixed order), ||w||
+      double pn = 0.0;
+      {
+        const GmLayout g(gm_opaque(threadIdx.x), nf, T);
+        const int jn = it + 1;
+        if (nf <= T && g.nseg > 1) {
+          const int per2 = (jn + g.nseg - 1) / g.nseg;
+          if (g.act) {
+            const int q0 = min(jn, g.seg * per2), q1 = min(jn, q0 + per2);
+            part[g.seg * nf + g.r] = gm_axpy_col(0.0, hcol, cache, a.V, nf, nvl, q0, q1, g.r);
+          }
+          __syncthreads();
+          for (int i = gm_opaque(threadIdx.x); i < nf; i += T) {
+            const double acc = wl[i] + gm_sum_parts(part, nf, i, g.nseg);
+            wl[i] = acc;
+            pn += acc * acc;
+          }
+        } else {
+          for (int i = gm_opaque(threadIdx.x); i < nf; i += T) {
+            const double acc = gm_axpy_col(wl[i], hcol, cache, a.V, nf, nvl, 0, jn, i);
+            wl[i] = acc;
+            pn += acc * acc;
+          }
+        }
+      }
+      const double tt = sqrt(gm_block_sum2(pn, red, T));
+      if (gm_opaque(threadIdx.x) < 64) {
+        const int lane = gm_opaque(threadIdx.x);
+        // previous rotations on the new column (KSPGMRESUpdateHessenberg): the
+        // running entry obeys carry_(j+1) = -s_j carry_j + c_j h_(j+1), an
+        // affine recurrence evaluated 64 rotations at a time by a scan
+        double carry = hcol[0];
+        for (int base = 0; base < it; base += 64) {
+          const int j = base + lane;
+          const bool act = j < it;
+          const double c = act ? lcc[j] : 1.0, sn = act ? lss[j] : 0.0, hn = act ? hcol[j + 1] : 0.0;
+          double A = -sn, B = c * hn;
+#pragma unroll
+          for (int d = 1; d < 64; d <<= 1) {
+            const double Ap = __shfl_up(A, d), Bp = __shfl_up(B, d);
+            if (lane >= d) {
+              B = A * Bp + B;
+              A = A * Ap;
+            }
+          }
+          const double cnext = A * carry + B;  // carry_(j+1)
+          double cj = __shfl_up(cnext, 1);     // carry_j
+          if (lane == 0) cj = carry;
+          if (act) hcol[j] = c * cj + sn * hn;
+          carry = __shfl(cnext, min(63, it - 1 - base));
+        }
+        if (lane == 0) {
+          hcol[it] = carry;
+          hcol[it + 1] = tt;
+          double hapbnd = fabs(tt / grs[it]);
+          if (hapbnd > 1e-30) hapbnd = 1e-30;
+          const int hap = tt < hapbnd;
+          double res;
+          if (!hap) {
+            const double t1 = sqrt(hcol[it] * hcol[it] + hcol[it + 1] * hcol[it + 1]);
+            const double c = hcol[it] / t1, sn = hcol[it + 1] / t1;
+            lcc[it] = c;
+            lss[it] = sn;
+            grs[it + 1] = -sn * grs[it];
+            grs[it] = c * grs[it];
+            hcol[it] = c * hcol[it] + sn * hcol[it + 1];
+            res = fabs(grs[it + 1]);
+          } else {
+            res = 0.0;
+          }
+          int rs = 0;
+          if (res <= ttol) rs = 1;
+          else if (res >= 1e4 * rn0) rs = -1;
+          if (hap && !rs) rs = 1;
+          s_flag[0] = rs;
+          s_flag[1] = hap;
+          s_val[0] = tt;
+          s_val[1] = res;
+        }
+      }
+      __syncthreads();
+      reason = s_flag[0];
+      const int hap = s_flag[1];
+      for (int j = gm_opaque(threadIdx.x); j <= it + 1; j += T) a.H[(long)it * ld + j] = hcol[j];
+      if (!hap) {
+        const double inv = 1.0 / s_val[0];
+        const bool keep = it + 1 < nvl;
+        for (int i = gm_opaque(threadIdx.x); i < nf; i += T) {
+          const double v1 = wl[i] * inv;
+          vl[i] = v1;
+          a.V[(long)(it + 1) * nf + i] = v1;
+          if (keep) cache[(long)(it + 1) * nf + i] = v1;
+        }
+      }
+      ++it;
+      ++its;
+      __syncthreads();
+      if (reason || it >= kmax || its >= a.maxit) break;
+    }
+    // ---- KSPGMRESBuildSoln: back substitution on the rotated Hessenberg
+    // (columns staged through the LDS cache, the serial sweep by one
+    // wavefront), then x += V y
+    const int top = it - 1;
+    for (int j = gm_opaque(threadIdx.x); j <= top; j += T) yv[j] = grs[j];  // right-hand side, overwritten by y
+    for (int k1 = top; k1 >= 0; k1 -= a.bsc) {
+      const int k0 = k1 - a.bsc + 1 < 0 ? 0 : k1 - a.bsc + 1;
+      const int nc = k1 - k0 + 1;
+      __syncthreads();
+      for (int q = gm_opaque(threadIdx.x); q < nc * (k1 + 1); q += T) {
+        const int c = q / (k1 + 1), rr = q - c * (k1 + 1);
+        cache[(long)c * ld + rr] = a.H[(long)(k0 + c) * ld + rr];
+      }
+      __syncthreads();
+      if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        for (int k = k1; k >= k0; --k) {
+          const double* col = cache + (long)(k - k0) * ld;
+          const double yk = yv[k] / col[k];
+          for (int j = lane; j < k; j += 64) yv[j] = yv[j] - col[j] * yk;
+          if (lane == 0) yv[k] = yk;
+          // LDS operations of one wavefront complete in order: the next
+          // step's read of y(k-1) sees this step's update
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+    }
+    __syncthreads();
+    {
+      const GmLayout g(gm_opaque(threadIdx.x), nf, T);
+      if (nf <= T && g.nseg > 1) {
+        const int per2 = (top + 1 + g.nseg - 1) / g.nseg;
+        if (g.act) {
+          const int q0 = min(top + 1, g.seg * per2), q1 = min(top + 1, q0 + per2);
+          double acc = 0.0;  // sum_j y_j v_j[r] over this segment's j range
+          for (int j = q0; j < q1; j += 8) {
+            double v[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) v[c] = a.V[(long)min(j + c, q1 - 1) * nf + g.r];
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+              if (j + c < q1) acc += yv[j + c] * v[c];
+          }
+          part[g.seg * nf + g.r] = acc;
+        }
+        __syncthreads();
+        for (int i = gm_opaque(threadIdx.x); i < nf; i += T) a.x[i] = a.x[i] + gm_sum_parts(part, nf, i, g.nseg);
+      } else {
+        for (int i = gm_opaque(threadIdx.x); i < nf; i += T) {
+          double acc = 0.0;
+          for (int j = 0; j <= top; ++j) acc += yv[j] * a.V[(long)j * nf + i];
+          a.x[i] = a.x[i] + acc;
+        }
+      }
+    }
+    __syncthreads();
+    if (reason) break;
+    if (its >= a.maxit) {
+      reason = -2;
+      break;
+    }
+  }
+  if (threadIdx.x == 0) {
+    a.s->its = its;
+    a.s->reason = reason;
+    a.s->rn0 = rn0;
+    a.s->ttol = ttol;
+  }
+}
+
+// dynamic LDS of k_gmres_dense: vl, wl (nf each), hcol (kmax+2), lcc, lss
+// (kmax each), grs, yv (kmax+1 each), part (kGmT), then the cache region
+size_t gmres_dense_lds(int nf, int kmax, size_t cache_doubles) {
+  return sizeof(double) * ((size_t)2 * nf + (size_t)5 * kmax + 4 + kGmT + cache_doubles);
+}
+
+bool gmres_dense_fits(int nf, int kmax, size_t max_lds) {
+  return gmres_dense_lds(nf, kmax, (size_t)kmax + 1) + 1024 <= max_lds;
+}
+
+void gmres_dense(hipStream_t st, int nf, int kmax, int maxit, const double* S, const double* b, double* x, double* V,
+                 double* H, GmresDev* s, double rtol, double atol, size_t max_lds) {
+  // the cache takes what is left of the LDS: basis vectors during the
+  // iterations, Hessenberg columns during the back substitution
+  const size_t avail = max_lds > 1024 ? max_lds - 1024 : 0;  // room for the static shared arrays
+  const size_t base = gmres_dense_lds(nf, kmax, 0);
+  if (base + sizeof(double) * (kmax + 1) > avail) throw HipError("dense GMRES: restart too long for LDS");
+  const size_t cache = (avail - base) / sizeof(double);
+  const int nvl = (int)std::min<size_t>((size_t)kmax + 1, cache / (size_t)std::max(nf, 1));
+  const int bsc = (int)std::max<size_t>(1, std::min<size_t>((size_t)kmax, cache / (size_t)(kmax + 1)));
+  GmDenseArgs a{nf, kmax, maxit, nvl, bsc, S, b, x, V, H, s, rtol, atol};
+  const size_t lds = gmres_dense_lds(nf, kmax, cache);
+  hipLaunchKernelGGL(k_gmres_dense<24>, dim3(1), dim3(kGmT), lds, st, a);
+  IB_LAUNCH_CHECK();
+}
+
+// the kernel also has a few hundred bytes of static shared memory, so its
+// dynamic share stops 1 KiB short of the device limit (as gmres_dense assumes)
+void gmres_dense_set_max_lds(size_t bytes) {
+  if (bytes <= 1024) return;
+  IB_HIP(hipFuncSetAttribute((const void*)k_gmres_dense<24>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(bytes - 1024)));
+}
+
 void gm_start(hipStream_t st, int nf, int kmax, const double* b, const double* Ax, double* V, GmresDev* s, double* H,
               double* grs, double rtol, double atol) {
   (void)kmax;

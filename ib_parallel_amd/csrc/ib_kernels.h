@@ -155,5 +155,11 @@ void gm_arnoldi(hipStream_t st, int nf, int kmax, double* V, GmresDev* s, double
                 double* grs);
 void gm_buildsol(hipStream_t st, int nf, int kmax, const double* V, GmresDev* s, const double* H,
                  const double* grs, double* x, double* scratch);
+// The whole GMRES solve with a dense nf x nf Schur matrix in one workgroup;
+// writes its, reason (1 converged, -1 diverged, -2 iteration limit) into *s.
+void gmres_dense(hipStream_t st, int nf, int kmax, int maxit, const double* S, const double* b, double* x, double* V,
+                 double* H, GmresDev* s, double rtol, double atol, size_t max_lds);
+void gmres_dense_set_max_lds(size_t bytes);
+bool gmres_dense_fits(int nf, int kmax, size_t max_lds);
 
 }  // namespace ibg

@@ -440,6 +440,13 @@ struct ib_handle {
     IB_HIP(hipGraphLaunch(itr->second, st));
   }
   int gmres_dev(const double* b, double* x, int* its_out) {
+    if (assembled && gmres_dense_fits(nf, kmax, max_lds)) {
+      // dense Schur matrix: the whole solve is one single-workgroup launch
+      gmres_dense(st, nf, kmax, prm.gmres_max_it, Sd, b, x, V, H, gst, prm.rtol, prm.atol, max_lds);
+      read_gmres();
+      *its_out = hst->its;
+      return hst->reason > 0 ? IB_OK : IB_E_NOCONV_GMRES;
+    }
     GmresDev init{};
     init.first = 1;
     *hst = init;
@@ -619,7 +626,10 @@ struct ib_handle {
     if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess) optin = 0;
     (void)hipGetLastError();
     max_lds = (size_t)std::max(lds, optin);
-    if (max_lds > 65536) dst_set_max_lds(max_lds);
+    if (max_lds > 65536) {
+      dst_set_max_lds(max_lds);
+      gmres_dense_set_max_lds(max_lds);
+    }
 
     px.init(m - 1);
     py.init(n - 1);
