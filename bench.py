@@ -11,16 +11,22 @@ in ib_parallel_amd.decks.plate_case.  State is resident in HBM before timing.
 Metric: Mgridpt-steps/s = m*n*mgridlev*steps / seconds / 1e6 (FP64).
 
 Also reported:
-  roofline      dominant DST pass kernel class: algorithmic bytes per launch
-                (each FP64 array/table read once, output written once) over
-                the launch duration measured with HIP events recorded around
-                every launch on the library's stream, during a second timed
-                region of the same workload right after the first (per-launch
-                events need plain stream launches, so that region does not
-                replay the captured GMRES graphs); peak = 8 TB/s HBM.
+  roofline      the kernel class with the largest share of the instrumented
+                steps (the single-workgroup dense GMRES solve on this
+                workload; the DST pass classes follow in other_classes):
+                algorithmic bytes per launch over the launch duration measured
+                with HIP events recorded around every launch on the library's
+                stream, during a second timed region of the same workload
+                right after the first (per-launch events need plain stream
+                launches); peak = 8 TB/s HBM.  traffic = calibrated
+                FETCH_SIZE + WRITE_SIZE bytes per launch from the committed
+                counter summary profiles/pmc_traffic.json (made by
+                ib_parallel_amd.pmc), null if it has no entry for the kernel.
   cpu_baseline  the CPU oracle (oracle/, a C++ restatement of the same
                 algorithm, OpenMP over DST rows) on the host cores, on a
                 bounded sample of the same workload from the same cold start.
+Workloads: plate (default, the headline), cylinder (rigid, 200x200x3) and
+airfoil_flap (rigid airfoil + torsional flap, 1024x512x5, full GMRES).
 """
 import argparse
 import json
@@ -33,7 +39,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 KCLASS = {0: "k_dst_x (row pass, time step)", 1: "k_dst_y (column pass, time step)",
-          2: "k_dst_x (batched row pass, Schur assembly)", 3: "k_dst_y (batched column pass, Schur assembly)"}
+          2: "k_dst_x (batched row pass, Schur assembly)", 3: "k_dst_y (batched column pass, Schur assembly)",
+          4: "k_gmres_dense (one GMRES solve in one workgroup)"}
 
 
 def gridpts(sim):
@@ -45,6 +52,10 @@ def make_case(name):
     if name == "plate":
         p, b = decks.plate_case()
         return p, b, "hinged plate FSI 400x400x4, 101 pts, Re=200, dt=1e-3, tol=1e-10"
+    if name == "airfoil_flap":
+        p, b = decks.airfoil_flap_case()
+        return p, b, ("elliptic airfoil (300 pts, rigid) + torsional flap (100 pts) FSI 1024x512x5, Re=1000, "
+                      "dt=5e-4, tol=1e-10")
     if name == "cylinder":
         p, b = decks.cylinder_case()
         return p, b, "rigid cylinder 200x200x3, 157 pts, Re=100, dt=4e-3, tol=1e-7"
@@ -78,7 +89,7 @@ def run_gpu(args):
     t1 = time.perf_counter()
     secs = t1 - t0
     # timed region 2: HIP events around every DST pass launch
-    classes = (0, 1, 2, 3)
+    classes = (0, 1, 2, 3, 4)
     for kc in classes:
         sim.kernel_timing(kc, True)
     nprof = max(1, args.steps // 2)
@@ -104,6 +115,10 @@ def run_gpu(args):
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
     roof.update(summary(dom))
+    tr, tk = traffic_for(KCLASS[dom])
+    roof["traffic"] = tr
+    if tr is not None:
+        roof["traffic_source"] = f"profiles/pmc_traffic.json ({tk}, FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
     roof["other_classes"] = [summary(kc) for kc in classes if kc != dom]
     roof["timing_region"] = (f"{nprof} further steps with per-launch HIP events (plain launches), "
                              f"{secs_prof / nprof * 1e3:.1f} ms/step")
@@ -112,6 +127,36 @@ def run_gpu(args):
            "schur": {"form": {1: "matrix-free", 2: "assembled", 3: "face-space"}[form], "faces": nfaces,
                      "green_builds": builds}}
     return res
+
+
+def copy_calibration(mb):
+    """A device-to-device copy of exactly mb MiB (a plain copy kernel of known
+    size for calibrating FETCH_SIZE / WRITE_SIZE in counter runs)."""
+    import torch
+    n = int(mb * 1024 * 1024) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    torch.cuda.synchronize()
+
+
+def traffic_for(kernel_label):
+    """Calibrated HBM bytes per launch of the kernel from the committed
+    counter summary (profiles/pmc_traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        data = json.load(fh)
+    base = kernel_label.split(" ")[0]
+    best = None
+    for k, v in data.get("kernels", {}).items():
+        if k.startswith(base) and (best is None or v["dispatches"] > best[1]["dispatches"]):
+            best = (k, v)
+    if best is None:
+        return None, None
+    return round(best[1]["bytes_per_launch"]), best[0]
 
 
 def run_cpu(args, gpu_its_per_step):
@@ -144,15 +189,19 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="plate", choices=["plate", "cylinder"])
+    ap.add_argument("--workload", default="plate", choices=["plate", "cylinder", "airfoil_flap"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-copy-mb", type=float, default=0.0,
+                    help="run a device-to-device copy of this size first (calibrates counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
 
+    if args.pmc_copy_mb > 0:
+        copy_calibration(args.pmc_copy_mb)
     g = run_gpu(args)
     sim = g["sim"]
     secs = g["secs"]

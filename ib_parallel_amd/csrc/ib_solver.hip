@@ -124,7 +124,7 @@ struct ib_handle {
   int* hflag = nullptr;     // pinned
   int itime = 0, ksp_iter = 0, fsi_iter = 0;
   double err_fsi = 0;
-  KTimer tim[4];
+  KTimer tim[5];
   // captured GMRES batches, keyed by (first basis column, batch length)
   std::map<std::pair<int, int>, hipGraphExec_t> gm_graphs;
   // dense Schur matrix and the column-batch buffers that assemble it
@@ -398,6 +398,10 @@ struct ib_handle {
   // kernels of the batch return at once, and the surplus matvecs only write
   // basis columns past the converged one, which the solution never reads.
   static constexpr int kGmBatch = 8;
+  // the single-workgroup solve streams the part of S that does not fit in
+  // its registers from L2 every iteration; past this size the multi-workgroup
+  // matrix-vector product of the graph-replayed iterations is faster
+  static constexpr int kFusedGmresMax = 400;
 
   // nb Arnoldi iterations starting at basis column it0.  Every argument of
   // these launches is fixed for the handle's lifetime except the column
@@ -415,7 +419,7 @@ struct ib_handle {
     }
   }
   void gmres_batch(int it0, int nb) {
-    if (prm.launch_mode != 0 || tim[0].on || tim[1].on || tim[2].on || tim[3].on) {
+    if (prm.launch_mode != 0 || tim[0].on || tim[1].on || tim[2].on || tim[3].on || tim[4].on) {
       gmres_batch_direct(it0, nb);
       return;
     }
@@ -440,10 +444,15 @@ struct ib_handle {
     IB_HIP(hipGraphLaunch(itr->second, st));
   }
   int gmres_dev(const double* b, double* x, int* its_out) {
-    if (assembled && gmres_dense_fits(nf, kmax, max_lds)) {
-      // dense Schur matrix: the whole solve is one single-workgroup launch
+    if (assembled && nf <= kFusedGmresMax && gmres_dense_fits(nf, kmax, max_lds)) {
+      // dense Schur matrix: the whole solve is one single-workgroup launch.
+      // Algorithmic bytes: S, b read once, x read and written, and one
+      // Krylov vector written per iteration (added once the count is known)
+      tim[4].begin(st, 8.0 * ((double)nf * nf + 3.0 * nf));
       gmres_dense(st, nf, kmax, prm.gmres_max_it, Sd, b, x, V, H, gst, prm.rtol, prm.atol, max_lds);
+      tim[4].end(st);
       read_gmres();
+      if (tim[4].on) tim[4].total_bytes += 8.0 * (double)nf * (hst->its + 1);
       *its_out = hst->its;
       return hst->reason > 0 ? IB_OK : IB_E_NOCONV_GMRES;
     }
@@ -1054,7 +1063,7 @@ int ib_schur_info(ib_handle* h, int32_t* mode, int32_t* nfaces, int64_t* green_b
 }
 
 int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable) {
-  if (!h || kclass < 0 || kclass > 3) return IB_E_INVAL;
+  if (!h || kclass < 0 || kclass > 4) return IB_E_INVAL;
   API_TRY
   h->sync();
   h->tim[kclass].on = enable != 0;
@@ -1066,7 +1075,7 @@ int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable) {
 }
 
 int ib_kernel_timing_read(ib_handle* h, int32_t kclass, int64_t* launches, double* total_ms, double* total_bytes) {
-  if (!h || kclass < 0 || kclass > 3) return IB_E_INVAL;
+  if (!h || kclass < 0 || kclass > 4) return IB_E_INVAL;
   API_TRY
   h->sync();
   if (launches) *launches = h->tim[kclass].count;

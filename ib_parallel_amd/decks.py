@@ -144,6 +144,41 @@ def plate_case(m=400, n=400, mgridlev=4, dt=0.001, re=200.0, npts=101, tol=1e-10
     return p, [plate_body(npts)]
 
 
+def ellipse_arclength(npts, chord=1.0, thickness=0.12, x0=0.0, y0=0.0):
+    """npts points equispaced in arc length on an ellipse with the given chord
+    (x extent) and thickness (y extent), leading edge at (x0, y0)."""
+    a, b = 0.5 * chord, 0.5 * thickness
+    t = np.linspace(0.0, 2.0 * np.pi, 20001)
+    x, y = a * np.cos(t), b * np.sin(t)
+    s = np.concatenate([[0.0], np.cumsum(np.hypot(np.diff(x), np.diff(y)))])
+    tt = np.interp(np.linspace(0.0, s[-1], npts, endpoint=False), s, t)
+    return x0 + a + a * np.cos(tt), y0 + b * np.sin(tt)
+
+
+def airfoil_flap_case(m=1024, n=512, mgridlev=5, dt=0.0005, re=1000.0, n_airfoil=300, n_flap=100,
+                      k_theta=0.05, mass_ratio=0.5, flap_deg=10.0, tol=1e-10, gmres_restart=1000, **kw):
+    """Two-body system: rigid elliptic airfoil (chord 1, thickness 0.12,
+    leading edge at the origin) and a flat flap (chord 0.3) hinged 0.01
+    behind its trailing edge on a torsional spring, deflected flap_deg
+    downward.  dx = 0.004 at 1024x512 (finest domain [-1, 3.096] x
+    [-1.024, 1.024]); the flap's inertia about the hinge is
+    mass_ratio * c^3 / 3.  The flap's points are 0.76 dx apart, which makes
+    the Schur system ill conditioned (cond ~1e8 at a quarter of this size):
+    GMRES(250) stagnates above 1e-10 at nf = 800, so the restart length
+    defaults to 1000 (full GMRES)."""
+    length = 0.004 * m
+    p = make_params(m, n, mgridlev, dt, re, length, 1.0, 0.004 * n / 2, fsi_tol=tol, atol=tol, rtol=tol,
+                    gmres_restart=gmres_restart, **kw)
+    xa, ya = ellipse_arclength(n_airfoil)
+    chord = 0.3
+    g = -math.radians(flap_deg)
+    r = chord / (n_flap - 1) * np.arange(n_flap)
+    hx, hy = 1.01, 0.0
+    flap = BodySpec(hx + r * math.cos(g), hy + r * math.sin(g), fam=2, gamma0=g, alpha0=0.0,
+                    itheta=mass_ratio * chord ** 3 / 3.0, ktheta=k_theta, ctheta=0.0)
+    return p, [BodySpec(xa, ya, fam=1), flap]
+
+
 def tiny_case(m=32, n=24, mgridlev=2, torsional=True, tol=1e-12, dt=0.01, re=50.0, **kw):
     """Small grid for oracle-vs-GPU parity: a coarse cylinder plus a hinged plate."""
     p = make_params(m, n, mgridlev, dt, re, 4.0, 1.6, 1.5, fsi_tol=tol, atol=tol, rtol=tol, **kw)
