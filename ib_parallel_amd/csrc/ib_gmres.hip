@@ -738,6 +738,75 @@ void gmres_dense_set_max_lds(size_t bytes) {
                              (int)(bytes - 1024)));
 }
 
+// ------------------------------------------------------------------------
+// Dense Cholesky for the numerically stationary path (num_stat): A = U^T U
+// from the upper triangle of a column-major n x n matrix, in place, and the
+// explicit inverse A^-1 = U^-1 U^-T, so that each step's solve is one
+// matrix-vector product.
+// ------------------------------------------------------------------------
+
+// row k of U: U(k,k) = sqrt(A(k,k)), U(k,j) = A(k,j) / U(k,k) for j > k
+__global__ void __launch_bounds__(256) k_chol_pivot(int n, double* __restrict__ A, int k, int* __restrict__ err) {
+  const double d = A[(long)k * n + k];
+  const double dk = d > 0.0 ? sqrt(d) : 1.0;
+  if (threadIdx.x == 0 && !(d > 0.0)) err[0] = 1;
+  for (int j = k + 1 + threadIdx.x; j < n; j += blockDim.x) A[(long)j * n + k] = A[(long)j * n + k] / dk;
+  __syncthreads();
+  if (threadIdx.x == 0) A[(long)k * n + k] = dk;
+}
+
+// trailing update A(i,j) -= U(k,i) U(k,j) for k < i <= j (16 x 16 tiles)
+__global__ void __launch_bounds__(256) k_chol_update(int n, double* __restrict__ A, int k) {
+  const int i = k + 1 + blockIdx.x * 16 + threadIdx.x % 16;
+  const int j = k + 1 + blockIdx.y * 16 + threadIdx.x / 16;
+  if (i >= n || j >= n || i > j) return;
+  A[(long)j * n + i] = A[(long)j * n + i] - A[(long)i * n + k] * A[(long)j * n + k];
+}
+
+// column c of A^-1: U^T y = e_c (forward), U x = y (backward); one workgroup per column
+__global__ void __launch_bounds__(256) k_chol_inverse(int n, const double* __restrict__ U, double* __restrict__ Ainv) {
+  extern __shared__ double y[];
+  const int c = blockIdx.x;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) y[j] = j == c ? 1.0 : 0.0;
+  __syncthreads();
+  for (int i = c; i < n; ++i) {
+    const double yi = y[i] / U[(long)i * n + i];
+    __syncthreads();
+    for (int j = i + 1 + threadIdx.x; j < n; j += blockDim.x) y[j] = y[j] - U[(long)j * n + i] * yi;
+    if (threadIdx.x == 0) y[i] = yi;
+    __syncthreads();
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    const double xi = y[i] / U[(long)i * n + i];
+    __syncthreads();
+    for (int j = threadIdx.x; j < i; j += blockDim.x) y[j] = y[j] - U[(long)i * n + j] * xi;
+    if (threadIdx.x == 0) y[i] = xi;
+    __syncthreads();
+  }
+  for (int j = threadIdx.x; j < n; j += blockDim.x) Ainv[(long)c * n + j] = y[j];
+}
+
+void chol_factor(hipStream_t st, int n, double* A, int* err) {
+  for (int k = 0; k < n; ++k) {
+    hipLaunchKernelGGL(k_chol_pivot, dim3(1), dim3(256), 0, st, n, A, k, err);
+    IB_LAUNCH_CHECK();
+    const int m = n - k - 1;
+    if (m > 0) {
+      hipLaunchKernelGGL(k_chol_update, dim3(ceil_div(m, 16), ceil_div(m, 16)), dim3(256), 0, st, n, A, k);
+      IB_LAUNCH_CHECK();
+    }
+  }
+}
+
+void chol_inverse(hipStream_t st, int n, const double* U, double* Ainv, size_t max_lds) {
+  const size_t lds = sizeof(double) * (size_t)n;
+  if (lds > max_lds) throw HipError("num_stat: Schur matrix too large for the LDS inverse");
+  if (lds > 65536) IB_HIP(hipFuncSetAttribute((const void*)k_chol_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)lds));
+  hipLaunchKernelGGL(k_chol_inverse, dim3(n), dim3(256), lds, st, n, U, Ainv);
+  IB_LAUNCH_CHECK();
+}
+
 void gm_start(hipStream_t st, int nf, int kmax, const double* b, const double* Ax, double* V, GmresDev* s, double* H,
               double* grs, double rtol, double atol) {
   (void)kmax;

@@ -161,5 +161,9 @@ void gmres_dense(hipStream_t st, int nf, int kmax, int maxit, const double* S, c
                  double* H, GmresDev* s, double rtol, double atol, size_t max_lds);
 void gmres_dense_set_max_lds(size_t bytes);
 bool gmres_dense_fits(int nf, int kmax, size_t max_lds);
+// In-place Cholesky A = U^T U of the upper triangle of a column-major n x n
+// matrix (err[0] = 1 if a pivot is not positive), and A^-1 from the factor.
+void chol_factor(hipStream_t st, int n, double* A, int* err);
+void chol_inverse(hipStream_t st, int n, const double* U, double* Ainv, size_t max_lds);
 
 }  // namespace ibg

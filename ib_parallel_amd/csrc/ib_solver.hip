@@ -141,6 +141,10 @@ struct ib_handle {
   int64_t green_builds = 0;
   DBuf<int> Fkeys, fmap, tapidx, face_flag;
   DBuf<double> Gm, Tm;
+  // numerically stationary path (num_stat): the inverse of the Cholesky-
+  // factored a_times matrix, and a torsional map with no torsional entries
+  DBuf<double> Ainv;
+  DBuf<int> no_tors;
 
   // ------------------------------------------------------------------ util
   void sync() {
@@ -489,36 +493,33 @@ struct ib_handle {
     return reason > 0 ? IB_OK : IB_E_NOCONV_GMRES;
   }
 
-  // ---------------------------------------------------------------- advance
-  int advance() {
-    itime += 1;
-    const long P = g.P, nbc = g.nbc;
-    // step 1: lastbc(k) = 0.25 get_bc(omega_{k+1}^n) (src/timestepper.F90:103-117)
-    if (L > 1) get_bc(st, g, omega + P, P, lastbc, nbc, L - 1, 0.25);
-    fill(st, lastbc + (L - 1) * nbc, nbc, 0.0);
-    for (int k = L - 1; k >= 0; --k) {
-      const double* ob = zbc;
-      if (k < L - 1) {
-        get_bc(st, g, omega + (k + 1) * P, P, obc, nbc, 1, 0.25);
-        ob = obc;
-      }
-      nl_rhs(st, g, omega + k * P, qx + (long)k * g.Fx, qy + (long)k * g.Fy, q0x[k], lastbc + k * nbc, ob, vfac[k],
-             con1[k], con2[k], itime == 1, rhs_old + k * P, t2, nullptr);
-      DstLaunch a;
-      a.batch = 2;
-      a.in[0] = t2; a.out[0] = t0;
-      a.in[1] = omega + k * P; a.out[1] = t1;
-      xpass(X_PLAIN, a);
-      DstLaunch b;
-      b.in[0] = t0; b.in2[0] = t1; b.out[0] = t2; b.tab = lam1 + k * P; b.tab2 = lam1i + k * P;
-      ypass(Y_PRED, b);
-      DstLaunch c;
-      c.in[0] = t2; c.out[0] = omega + k * P;
-      xpass(X_PLAIN, c);
+  // num_stat preprocess (src/timestepper.F90:560-598) and the Cholesky
+  // factorisation of KSPPREONLY + PCCHOLESKY (src/variables.F90:555-561):
+  // column j = a_times(e_j) (no b_times2: every entry is treated as rigid),
+  // A = U^T U from its upper triangle, then A^-1 once so each step's solve is
+  // a matrix-vector product.
+  int num_stat_setup() {
+    no_tors.alloc(std::max(nf, 1));
+    IB_HIP(hipMemsetAsync(no_tors.p, 0xff, sizeof(int) * std::max(nf, 1), st));  // every entry -1
+    alloc_assembly(nf);
+    for (int c0 = 0; c0 < nf; c0 += ncol) {
+      const int nc = std::min(ncol, nf - c0);
+      long sbc = 0;
+      const double* bc = column_batch(c0, nc, w, csr.key, &sbc);
+      schur_columns(st, g, nf, c0, nc, w, csr.key, Sb, g.P, bc, sbc, no_tors, tb, th, dt, delta, Sd);
     }
-    vort2flux_dev(omega, s, qx, qy);
+    IB_HIP(hipMemsetAsync(err_flag.p, 0, sizeof(int), st));
+    chol_factor(st, nf, Sd, err_flag);
+    if (read_err_flag()) return fail(IB_E_INVAL, "num_stat: the Schur matrix is not positive definite");
+    Ainv.alloc((size_t)nf * nf);
+    chol_inverse(st, nf, Sd, Ainv, max_lds);
+    sync();
+    return IB_OK;
+  }
 
-    // step 2: FSI fixed point (src/timestepper.F90:249-456)
+  // step 2: FSI fixed point (src/timestepper.F90:249-456).  Returns the
+  // step status; IB_E_INVAL when a body stencil leaves the grid.
+  int fsi_fixed_point() {
     int status = IB_OK;
     int fsi_count = 0;
     double err_glob = 10.0;
@@ -555,6 +556,54 @@ struct ib_handle {
     if (err_glob >= prm.fsi_tol && status == IB_OK) status = IB_E_NOCONV_FSI;
     fsi_iter = fsi_count;
     err_fsi = err_glob;
+    return status;
+  }
+
+  // ---------------------------------------------------------------- advance
+  int advance() {
+    itime += 1;
+    const long P = g.P, nbc = g.nbc;
+    // step 1: lastbc(k) = 0.25 get_bc(omega_{k+1}^n) (src/timestepper.F90:103-117)
+    if (L > 1) get_bc(st, g, omega + P, P, lastbc, nbc, L - 1, 0.25);
+    fill(st, lastbc + (L - 1) * nbc, nbc, 0.0);
+    for (int k = L - 1; k >= 0; --k) {
+      const double* ob = zbc;
+      if (k < L - 1) {
+        get_bc(st, g, omega + (k + 1) * P, P, obc, nbc, 1, 0.25);
+        ob = obc;
+      }
+      nl_rhs(st, g, omega + k * P, qx + (long)k * g.Fx, qy + (long)k * g.Fy, q0x[k], lastbc + k * nbc, ob, vfac[k],
+             con1[k], con2[k], itime == 1, rhs_old + k * P, t2, nullptr);
+      DstLaunch a;
+      a.batch = 2;
+      a.in[0] = t2; a.out[0] = t0;
+      a.in[1] = omega + k * P; a.out[1] = t1;
+      xpass(X_PLAIN, a);
+      DstLaunch b;
+      b.in[0] = t0; b.in2[0] = t1; b.out[0] = t2; b.tab = lam1 + k * P; b.tab2 = lam1i + k * P;
+      ypass(Y_PRED, b);
+      DstLaunch c;
+      c.in[0] = t2; c.out[0] = omega + k * P;
+      xpass(X_PLAIN, c);
+    }
+    vort2flux_dev(omega, s, qx, qy);
+
+    int status = IB_OK;
+    if (prm.num_stat) {
+      // step 2 for a numerically stationary body (src/timestepper.F90:457-484):
+      // rhsf = regT(q + q0), f = A^-1 rhsf, redistribute; nothing moves and
+      // no iteration counts are kept
+      fsi_rhs(st, g, nf, no_tors, tb, 0, th, dt, delta, w, csr.key, qx, qy, q0x[0], rhsf);
+      gemv(st, nf, Ainv, rhsf, f);
+      redist_faces(st, g, csr, w, f, hx, hy);
+      regT(st, g, nf, w, csr.key, hx, hy, 0.0, nullptr, f_rdst);
+      ksp_iter = 0;
+      fsi_iter = 0;
+      err_fsi = 0.0;
+    } else {
+      status = fsi_fixed_point();
+      if (status == IB_E_INVAL) return status;
+    }
 
     // step 3: corrector (src/timestepper.F90:488-553)
     reg(st, g, csr, w, f, hx, hy);
@@ -579,7 +628,7 @@ struct ib_handle {
     L = prm.mgridlev;
     if (m < 8 || n < 8 || m % 4 || n % 4) return fail(IB_E_INVAL, "grid must be divisible by 4 (m, n >= 8)");
     if (L < 1) return fail(IB_E_INVAL, "mgridlev must be >= 1");
-    if (prm.num_stat || prm.sub_domain) return fail(IB_E_UNSUPPORTED, "num_stat and sub_domain paths are not built");
+    if (prm.sub_domain) return fail(IB_E_UNSUPPORTED, "the sub_domain path is not built");
     if (prm.istart != 0) return fail(IB_E_UNSUPPORTED, "restart from .var is not built");
     if (prm.gmres_restart > 1000) return fail(IB_E_INVAL, "gmres_restart must be <= 1000");
     kmax = prm.gmres_restart;
@@ -722,7 +771,10 @@ struct ib_handle {
     if (prm.schur_mode < 0 || prm.schur_mode > 3) return fail(IB_E_INVAL, "schur_mode must be 0, 1, 2 or 3");
     if (prm.face_margin < -1) return fail(IB_E_INVAL, "face_margin must be >= -1");
     face_margin = prm.face_margin == 0 ? kFaceMarginDefault : std::max(0, prm.face_margin);
-    if (nf > 0) {
+    if (nf > 0 && prm.num_stat) {
+      const int rc = num_stat_setup();
+      if (rc != IB_OK) return rc;
+    } else if (nf > 0) {
       if (prm.schur_mode == 3 || (prm.schur_mode == 0 && ntors > 0)) {
         assembled = facemode = true;
       } else if (prm.schur_mode == 2 || (prm.schur_mode == 0 && nf <= kAutoAssembleMax)) {

@@ -534,11 +534,13 @@ struct orc_handle {
 
   /* ---------------- Schur matvec and GMRES ---------------- */
 
-  /* a_times2 + ab_times: src/operators_fluid.F90:14-120, src/timestepper.F90:774-782 */
-  void schur(const double* x, double* y) const {
+  /* a_times2 + ab_times: src/operators_fluid.F90:14-120, src/timestepper.F90:774-782;
+   * with_b = false gives a_times (src/operators_fluid.F90:1020-1105), the
+   * operator the num_stat preprocess assembles */
+  void schur(const double* x, double* y, bool with_b = true) const {
     std::vector<double> hx(Fx, 0.0), hy(Fy, 0.0), y2(nf, 0.0), y1(nf, 0.0);
     reg(x, hx.data(), hy.data());
-    b_times2(x, y2.data());
+    if (with_b) b_times2(x, y2.data());
     std::vector<std::vector<double>> vv(L, std::vector<double>(P, 0.0)), sv(L, std::vector<double>(P)),
         vx(L, std::vector<double>(Fx)), vy(L, std::vector<double>(Fy));
     rot(hx.data(), hy.data(), vv[0].data());
@@ -649,6 +651,52 @@ struct orc_handle {
     return reason > 0 ? IB_OK : IB_E_NOCONV_GMRES;
   }
 
+  /* ---------------- num_stat: precomputed matrix + Cholesky ----------------
+   * preprocess (src/timestepper.F90:560-598) assembles column j as
+   * a_times(e_j); KSPPREONLY + PCCHOLESKY (src/variables.F90:555-561) then
+   * factors it.  The Cholesky solver (Elemental, third party, not in the
+   * reference tree) reads one triangle; this restatement uses the upper
+   * one: A(i, j) = S(min(i, j), max(i, j)), A = U^T U. */
+  std::vector<double> cholU;  // n x n column major, upper triangle used
+  int preprocess() {
+    const int N = nf;
+    std::vector<double> S((size_t)N * N), e(N, 0.0);
+    for (int j = 0; j < N; ++j) {
+      e[j] = 1.0;
+      schur(e.data(), S.data() + (size_t)j * N, false);
+      e[j] = 0.0;
+    }
+    cholU.assign((size_t)N * N, 0.0);
+    for (int j = 0; j < N; ++j) {
+      for (int i = 0; i <= j; ++i) {
+        double acc = S[(size_t)j * N + i];
+        for (int k = 0; k < i; ++k) acc -= cholU[(size_t)i * N + k] * cholU[(size_t)j * N + k];
+        if (i < j) {
+          cholU[(size_t)j * N + i] = acc / cholU[(size_t)i * N + i];
+        } else {
+          if (!(acc > 0.0)) return fail(IB_E_INVAL, "num_stat: the Schur matrix is not positive definite");
+          cholU[(size_t)j * N + j] = std::sqrt(acc);
+        }
+      }
+    }
+    return IB_OK;
+  }
+  // x = A^-1 b: U^T y = b, then U x = y
+  void chol_solve(const double* b, double* x) const {
+    const int N = nf;
+    std::vector<double> y(N);
+    for (int i = 0; i < N; ++i) {
+      double acc = b[i];
+      for (int k = 0; k < i; ++k) acc -= cholU[(size_t)i * N + k] * y[k];
+      y[i] = acc / cholU[(size_t)i * N + i];
+    }
+    for (int i = N - 1; i >= 0; --i) {
+      double acc = y[i];
+      for (int k = i + 1; k < N; ++k) acc -= cholU[(size_t)k * N + i] * x[k];
+      x[i] = acc / cholU[(size_t)i * N + i];
+    }
+  }
+
   /* redistribute + redistribute2: src/operators_struct.F90:735-834 */
   void redistribute() {
     std::vector<double> ones(nf, 1.0), wx(Fx, 0.0), wy(Fy, 0.0), frx(Fx, 0.0), fry(Fy, 0.0), fix(Fx, 0.0),
@@ -695,6 +743,18 @@ struct orc_handle {
     for (int q = 0; q < Fx; ++q) qhx[q] = qx[0][q] + q0x[0][q];
     for (int q = 0; q < Fy; ++q) qhy[q] = qy[0][q] + q0y[0][q];
 
+    if (prm.num_stat) {
+      /* step 2 for a numerically stationary body (src/timestepper.F90:457-484):
+       * rhsf = regT(q + q0), f from the Cholesky factors, redistribute; the
+       * bodies do not move and no iteration counts are kept */
+      std::vector<double> rhsf(nf, 0.0);
+      regT(qhx.data(), qhy.data(), rhsf.data());
+      chol_solve(rhsf.data(), f.data());
+      redistribute();
+      ksp_iter = 0;
+      fsi_iter = 0;
+      err_fsi = 0.0;
+    } else {
     /* step 2: FSI fixed point with GMRES (num_stat = F, sub_domain = F) */
     int status = IB_OK;
     int fsi_count = 0;
@@ -734,6 +794,8 @@ struct orc_handle {
     if (err_glob >= prm.fsi_tol && status == IB_OK) status = IB_E_NOCONV_FSI;
     fsi_iter = fsi_count;
     err_fsi = err_glob;
+    step_status = status;
+    }
 
     /* step 3: corrector on level 1 */
     std::vector<double> hx(Fx, 0.0), hy(Fy, 0.0), vort(P);
@@ -742,8 +804,9 @@ struct orc_handle {
     ainv(vort.data());
     for (int q = 0; q < P; ++q) omega[0][q] = omega[0][q] - vort[q];
     vort2flux(omega, s, qx, qy);
-    return status;
+    return prm.num_stat ? IB_OK : step_status;
   }
+  int step_status = IB_OK;
 
   /* ---------------- setup: src/grid.F90:117-434, src/myfft.F90:237-298 --- */
   int init(const ib_params* p, int nbodies, const ib_body* bodies) {
@@ -754,7 +817,7 @@ struct orc_handle {
     m = prm.m; n = prm.n; L = prm.mgridlev;
     if (m < 8 || n < 8 || m % 4 || n % 4) return fail(IB_E_INVAL, "grid must be divisible by 4 (m, n >= 8)");
     if (L < 1) return fail(IB_E_INVAL, "mgridlev must be >= 1");
-    if (prm.num_stat || prm.sub_domain) return fail(IB_E_UNSUPPORTED, "num_stat and sub_domain paths are not built");
+    if (prm.sub_domain) return fail(IB_E_UNSUPPORTED, "the sub_domain path is not built");
     if (prm.istart != 0) return fail(IB_E_UNSUPPORTED, "restart from .var is not built");
     pi = 4.0 * std::atan(1.0);
     dt = prm.dt; Re = prm.re; len = prm.len; offx = prm.offsetx; offy = prm.offsety;
@@ -845,6 +908,12 @@ struct orc_handle {
     q0x.assign(L, std::vector<double>(Fx, 0.0));
     q0y.assign(L, std::vector<double>(Fy, 0.0));
     itime = prm.istart;
+    /* the reference factors at the first advance (src/timestepper.F90:76-78);
+     * the weights are the same then, so the factors are the same */
+    if (prm.num_stat) {
+      rc = preprocess();
+      if (rc != IB_OK) return rc;
+    }
     return IB_OK;
   }
 };

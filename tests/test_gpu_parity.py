@@ -144,6 +144,16 @@ def test_step_parity_restart30_deck_tolerance():
                  its_slack=0.02, ftol=1e-3)
 
 
+def test_step_parity_num_stat_cylinder():
+    _step_parity(*decks.cylinder_case(m=64, n=64, mgridlev=3, dt=0.01, npts=50, tol=1e-12, num_stat=1), steps=3)
+
+
+def test_step_parity_num_stat_treats_torsional_as_rigid():
+    p, b = decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-12, num_stat=1)
+    g, o = _step_parity(p, b, steps=3)
+    assert np.array_equal(g.body()["theta"], np.zeros_like(g.body()["theta"]))
+
+
 def test_face_space_recompute_when_taps_leave_the_set():
     """With no margin the face set goes stale as soon as a body point crosses
     a cell boundary; G is then recomputed for the new set inside the FSI loop

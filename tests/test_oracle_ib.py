@@ -11,7 +11,9 @@ the matvec (the reference's num_stat preprocess) solved by LAPACK.  Schur
 matrix: on one grid level C (Delta^-1 ainv) C^T is symmetric positive definite
 in exact arithmetic (Delta^-1 and ainv are diagonal in the same DST basis), so
 a transposed or dropped factor breaks symmetry; with more levels the boundary
-coupling makes it slightly non-symmetric.
+coupling makes it slightly non-symmetric.  num_stat (precomputed matrix +
+Cholesky): equal to tight GMRES on one level; on several levels it solves the
+matrix built from the upper triangle.
 """
 import numpy as np
 import pytest
@@ -182,6 +184,40 @@ def test_schur_matrix_face_space_factorisation():
     S = sim.schur_matrix()
     assert rel((EF @ G @ EF.T + B).ravel(), S.ravel()) < 1e-13
     assert np.abs(B).max() > 0  # the torsional block is present
+
+
+def _upper_sym(S):
+    """A(i, j) = S(min(i, j), max(i, j)): the matrix a Cholesky of the upper triangle factors."""
+    U = np.triu(S)
+    return U + np.triu(S, 1).T
+
+
+def test_num_stat_single_level_matches_gmres():
+    """On one level the a_times matrix is symmetric, so the Cholesky path
+    (num_stat) and tight GMRES give the same steps."""
+    kw = dict(m=64, n=64, mgridlev=1, dt=0.01, npts=50, tol=1e-13, gmres_restart=120)
+    g = ob.OracleSim(*decks.cylinder_case(**kw))
+    c = ob.OracleSim(*decks.cylinder_case(num_stat=1, **kw))
+    for _ in range(3):
+        ig, ic = g.advance(), c.advance()
+        assert ic.ksp_iter == 0 and ic.fsi_iter == 0 and ig.ksp_iter > 0
+        # GMRES stops at a residual of 1e-13 ||b||: f is fixed to cond(S) * 1e-13 ~ 1e-9
+        assert rel(c.body()["f"], g.body()["f"]) < 1e-8
+        assert rel(c.level(1)["omega"], g.level(1)["omega"]) < 1e-10
+
+
+def test_num_stat_solves_the_upper_triangle():
+    """On three levels the matrix is slightly non-symmetric; the num_stat
+    force solves A f = rhs with A built from the upper triangle (the one
+    triangle a Cholesky factorisation reads), rhs = regT(q + q0) = S f_gmres."""
+    kw = dict(m=64, n=64, mgridlev=3, dt=0.01, npts=50, tol=1e-14, gmres_restart=200)
+    g = ob.OracleSim(*decks.cylinder_case(**kw))
+    c = ob.OracleSim(*decks.cylinder_case(num_stat=1, **kw))
+    S = g.schur_matrix()
+    assert np.abs(S - S.T).max() > 1e-8 * np.abs(S).max()  # the case really is non-symmetric
+    g.advance(); c.advance()
+    rhs = S @ g.body()["f"]
+    assert rel(_upper_sym(S) @ c.body()["f"], rhs) < 1e-9
 
 
 def test_gmres_restart_is_petsc_gmres30():
