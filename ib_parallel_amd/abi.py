@@ -76,6 +76,7 @@ SIGNATURES = {
     "get_level_fields": (C.c_int, [_P, C.c_int32, _D, _D, _D, _D, _D]),
     "set_level_fields": (C.c_int, [_P, C.c_int32, _D, _D, _D, _D, _D]),
     "get_body": (C.c_int, [_P, _D, _D, _D, _D, _D, _D, _D]),
+    "set_body": (C.c_int, [_P, _D, _D, _D, _D, _D, _D]),
     "get_forces": (C.c_int, [_P, C.c_int32, _D, _D]),
     "op_dst2d": (C.c_int, [_P, C.c_int32, _D, _D]),
     "op_ctci": (C.c_int, [_P, _D, _D]),

@@ -629,7 +629,7 @@ struct ib_handle {
     if (m < 8 || n < 8 || m % 4 || n % 4) return fail(IB_E_INVAL, "grid must be divisible by 4 (m, n >= 8)");
     if (L < 1) return fail(IB_E_INVAL, "mgridlev must be >= 1");
     if (prm.sub_domain) return fail(IB_E_UNSUPPORTED, "the sub_domain path is not built");
-    if (prm.istart != 0) return fail(IB_E_UNSUPPORTED, "restart from .var is not built");
+    if (prm.istart < 0) return fail(IB_E_INVAL, "istart must be >= 0");
     if (prm.gmres_restart > 1000) return fail(IB_E_INVAL, "gmres_restart must be <= 1000");
     kmax = prm.gmres_restart;
     dt = prm.dt; Re = prm.re; offx = prm.offsetx; offy = prm.offsety;
@@ -939,6 +939,37 @@ int ib_get_body(ib_handle* h, double* f, double* f_rdst, double* xb, double* yb,
   if (theta && nt) h->d2h(theta, h->th, nt);
   if (thetad && nt) h->d2h(thetad, h->th + nt, nt);
   if (thetadd && nt) h->d2h(thetadd, h->th + 2 * nt, nt);
+  h->sync();
+  return IB_OK;
+  API_CATCH(h)
+}
+
+// read_variables' body part (src/variables.F90:871-895) followed by reg_prep
+// (:364-373): f, positions and torsional state, weights rebuilt from the
+// positions, and the Schur operator of the active form refreshed.
+int ib_set_body(ib_handle* h, const double* f, const double* xb, const double* yb, const double* theta,
+                const double* thetad, const double* thetadd) {
+  if (!h) return IB_E_INVAL;
+  API_TRY
+  const size_t nf = h->nf, nt = h->ntors;
+  if (f && nf) h->h2d(h->f, f, nf);
+  if (xb && nf) h->h2d(h->xb, xb, nf);
+  if (yb && nf) h->h2d(h->yb, yb, nf);
+  if (theta && nt) h->h2d(h->th, theta, nt);
+  if (thetad && nt) h->h2d(h->th + nt, thetad, nt);
+  if (thetadd && nt) h->h2d(h->th + 2 * nt, thetadd, nt);
+  if (nf) {
+    h->rebuild_weights();
+    if (h->read_err_flag()) return h->fail(IB_E_INVAL, "body point regularisation stencil leaves the grid");
+    if (h->prm.num_stat) {
+      const int rc = h->num_stat_setup();
+      if (rc != IB_OK) return rc;
+    } else if (h->facemode) {
+      h->refresh_schur();
+    } else if (h->assembled) {
+      h->assemble_schur();
+    }
+  }
   h->sync();
   return IB_OK;
   API_CATCH(h)

@@ -98,7 +98,8 @@ def write_ib_inp(path, p, istop=10, isave=10):
         fh.write(f"M = {p.m},\nN = {p.n},\nDT = {p.dt!r},\nRe = {p.re!r},\n")
         fh.write(f"FSI_TOL = {p.fsi_tol!r},\nATOL = {p.atol!r},\nRTOL = {p.rtol!r},\n")
         fh.write(f"LEN = {p.len!r},\nOFFSETX = {p.offsetx!r},\nOFFSETY = {p.offsety!r},\nMGRIDLEV = {p.mgridlev},\n")
-        fh.write("COMPUTE_PRESSURE = F,\ndimen = 2,\nnum_stat=F,\nmotion_prescribed=F,\nsub_domain = F,\n/\n")
+        ns = "T" if getattr(p, "num_stat", 0) else "F"
+        fh.write(f"COMPUTE_PRESSURE = F,\ndimen = 2,\nnum_stat={ns},\nmotion_prescribed=F,\nsub_domain = F,\n/\n")
 
 
 # --------------------------------------------------------------------------

@@ -80,6 +80,20 @@ module ib_gpu_mod
       real(c_double), intent(out) :: omega(*), s(*), qx(*), qy(*), rhs_old(*)
     end function ib_get_level_fields
 
+    integer(c_int) function ib_set_level_fields(h, k, omega, s, qx, qy, rhs_old) &
+        bind(C, name="ib_set_level_fields")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: k
+      real(c_double), intent(in) :: omega(*), s(*), qx(*), qy(*), rhs_old(*)
+    end function ib_set_level_fields
+
+    integer(c_int) function ib_set_body(h, f, xb, yb, theta, thetad, thetadd) bind(C, name="ib_set_body")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: f(*), xb(*), yb(*), theta(*), thetad(*), thetadd(*)
+    end function ib_set_body
+
     integer(c_int) function ib_get_body(h, f, f_rdst, xb, yb, theta, thetad, thetadd) &
         bind(C, name="ib_get_body")
       import :: c_int, c_ptr, c_double

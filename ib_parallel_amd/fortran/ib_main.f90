@@ -5,7 +5,10 @@
 ! body decks input/body.NNN.inp (src/grid.F90:252-288), then
 !   do while (it < istop): advance; save-step .var file; force, redistributed
 !   force, theta and iteration files
-! with advance(it) replaced by ib_advance through iso_c_binding.  The output
+! with advance(it) replaced by ib_advance through iso_c_binding.  A run with
+! istart > 0 resumes from output/data00_<istart>.var as setup_variables does
+! (src/variables.F90:237-245): read_variables, then the weights rebuilt from
+! the saved body positions (ib_set_body).  The output
 ! files keep the reference's names and record layouts for nproc = 1, rank 0:
 !   output/dataRR_IIIIIII.var   write_variables (src/variables.F90:709-803)
 !   output/force01_00.dat       write_force     (src/operators_struct.F90:655-679)
@@ -102,6 +105,7 @@ program ib_main
   rc = ib_get_sizes(h, sz)
 
   call execute_command_line('mkdir -p output')
+  if (istart > 0) call read_variables(int(istart, c_int32_t))
   it = istart
   do while (it < istop)
     rc = ib_advance(h, it, info)
@@ -210,6 +214,71 @@ contains
     write(100) fr
     close(100)
   end subroutine write_variables
+
+  ! read_variables (src/variables.F90:807-913), nproc = 1: the records that
+  ! write_variables writes, in the same order
+  subroutine read_variables(itn)
+    integer(c_int32_t), intent(in) :: itn
+    character(len=7) :: charit
+    real(c_double), allocatable :: w(:, :), s(:, :), qx(:, :), qy(:, :), rhs_old(:), q0(:)
+    real(c_double), allocatable :: f(:), xb(:), yb(:), th(:), thd(:), thdd(:)
+    integer(c_int32_t) :: hm, hn, hl, hnb, i6(6), pn, pi0, pe, nt
+    real(c_double) :: hre, hdt, hlen, hox, hoy, rot(3)
+    integer :: k, rcl, np
+    logical :: there
+    write(charit, '(I7.7)') itn
+    inquire(file='output/data00_'//charit//'.var', exist=there)
+    if (.not. there) then
+      write(*,*) 'restart file output/data00_'//charit//'.var not found'
+      stop 1
+    end if
+    write(*,*) 'reading variables at it=', itn
+    np = sz%n_nodes
+    allocate(w(np, mgridlev), s(np, mgridlev), qx(sz%n_xfaces, mgridlev), qy(sz%n_yfaces, mgridlev))
+    allocate(rhs_old(np * mgridlev), q0(max(sz%n_xfaces, sz%n_yfaces)))
+    allocate(f(max(sz%nf, 1)), xb(max(sz%nf, 1)), yb(max(sz%nf, 1)))
+    allocate(th(max(sz%ntors, 1)), thd(max(sz%ntors, 1)), thdd(max(sz%ntors, 1)))
+    open(unit=100, file='output/data00_'//charit//'.var', form='unformatted', status='old')
+    read(100) hm, hn, hl, hnb
+    if (hm /= m .or. hn /= n .or. hl /= mgridlev .or. hnb /= sz%nb) then
+      write(*,*) 'restart file does not match the deck (m, n, mgridlev, nb):', hm, hn, hl, hnb
+      stop 1
+    end if
+    read(100) hre, hdt, hlen, hox, hoy
+    read(100) i6
+    read(100) i6
+    read(100) i6
+    ! per level: omega, s, qx, q0px, q0rx, qy, q0py, q0ry (the uniform
+    ! freestream q0 is recomputed every step, so its records are skipped)
+    do k = 1, mgridlev
+      read(100) w(:, k)
+      read(100) s(:, k)
+      read(100) qx(:, k)
+      read(100) q0(1:sz%n_xfaces)
+      read(100) q0(1:sz%n_xfaces)
+      read(100) qy(:, k)
+      read(100) q0(1:sz%n_yfaces)
+      read(100) q0(1:sz%n_yfaces)
+    end do
+    read(100) rhs_old
+    do k = 1, mgridlev
+      rcl = ib_set_level_fields(h, int(k, c_int32_t), w(:, k), s(:, k), qx(:, k), qy(:, k), &
+                                rhs_old((k - 1) * np + 1:k * np))
+    end do
+    read(100) rot
+    read(100) pn, pi0, pe
+    read(100) f(1:sz%nf)
+    read(100) xb(1:sz%nf)
+    read(100) yb(1:sz%nf)
+    read(100) nt
+    if (nt > 0) read(100) th(1:nt), thd(1:nt), thdd(1:nt)
+    close(100)
+    rcl = ib_set_body(h, f, xb, yb, th, thd, thdd)
+    if (rcl /= IB_OK) then
+      write(*,*) 'ib_set_body failed (', rcl, '): ', trim(ib_error_message(h))
+      stop 1
+    end if
+  end subroutine read_variables
 
   ! write_force / write_force_rdst with write_force2 (src/operators_struct.F90:655-731)
   subroutine write_force(itn, which, stem)

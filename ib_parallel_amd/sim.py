@@ -82,6 +82,12 @@ class Simulation:
         self._check(self._f["set_level_fields"](self.h, k, abi.dptr(omega), abi.dptr(s), abi.dptr(qx),
                                                 abi.dptr(qy), abi.dptr(rhs_old)))
 
+    def set_body(self, f=None, xb=None, yb=None, theta=None, thetad=None, thetadd=None):
+        """Restore body state (restart); the weights are rebuilt from xb, yb."""
+        c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        args = [c(a) for a in (f, xb, yb, theta, thetad, thetadd)]
+        self._check(self._f["set_body"](self.h, *[abi.dptr(a) for a in args]))
+
     def body(self):
         nf, nt = self.nf, max(1, self.ntors)
         out = {k: np.empty(nf) for k in ("f", "f_rdst", "xb", "yb")}

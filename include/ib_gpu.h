@@ -123,6 +123,14 @@ int ib_get_level_fields(ib_handle* h, int32_t k, double* omega, double* s,
 int ib_set_level_fields(ib_handle* h, int32_t k, const double* omega, const double* s,
                         const double* qx, const double* qy, const double* rhs_old);
 /* theta* have one entry per torsional body, in deck order */
+/* restart: read_variables' body part (src/variables.F90:871-895) followed by
+ * reg_prep (:364-373): sets f (the next GMRES initial guess), the body point
+ * positions and the torsional state (nt values each; any pointer may be
+ * NULL to keep the current values), then rebuilds the regularisation weights
+ * from the positions.  With ib_set_level_fields and params.istart this
+ * resumes a run from a .var file. */
+int ib_set_body(ib_handle* h, const double* f, const double* xb, const double* yb, const double* theta,
+                const double* thetad, const double* thetadd);
 int ib_get_body(ib_handle* h, double* f, double* f_rdst, double* xb, double* yb,
                 double* theta, double* thetad, double* thetadd);
 /* write_force2: per body fx = 2*delta*sum(f_x)/dt, fy likewise (drag, lift).

@@ -818,7 +818,7 @@ struct orc_handle {
     if (m < 8 || n < 8 || m % 4 || n % 4) return fail(IB_E_INVAL, "grid must be divisible by 4 (m, n >= 8)");
     if (L < 1) return fail(IB_E_INVAL, "mgridlev must be >= 1");
     if (prm.sub_domain) return fail(IB_E_UNSUPPORTED, "the sub_domain path is not built");
-    if (prm.istart != 0) return fail(IB_E_UNSUPPORTED, "restart from .var is not built");
+    if (prm.istart < 0) return fail(IB_E_INVAL, "istart must be >= 0");
     pi = 4.0 * std::atan(1.0);
     dt = prm.dt; Re = prm.re; len = prm.len; offx = prm.offsetx; offy = prm.offsety;
     delta = len / (double)m;
@@ -1006,6 +1006,22 @@ int orc_get_body(orc_handle* h, double* f, double* f_rdst, double* xb, double* y
   if (thetad && nt) std::memcpy(thetad, h->thetad.data(), sizeof(double) * nt);
   if (thetadd && nt) std::memcpy(thetadd, h->thetadd.data(), sizeof(double) * nt);
   return IB_OK;
+}
+
+/* read_variables' body part (src/variables.F90:871-895) + reg_prep (:364-373) */
+int orc_set_body(orc_handle* h, const double* f, const double* xb, const double* yb, const double* theta,
+                 const double* thetad, const double* thetadd) {
+  size_t nf = h->nf, nt = h->tors.size();
+  if (f) std::memcpy(h->f.data(), f, sizeof(double) * nf);
+  if (xb) std::memcpy(h->xb.data(), xb, sizeof(double) * nf);
+  if (yb) std::memcpy(h->yb.data(), yb, sizeof(double) * nf);
+  if (theta && nt) std::memcpy(h->theta.data(), theta, sizeof(double) * nt);
+  if (thetad && nt) std::memcpy(h->thetad.data(), thetad, sizeof(double) * nt);
+  if (thetadd && nt) std::memcpy(h->thetadd.data(), thetadd, sizeof(double) * nt);
+  int rc = h->setup_reg();
+  if (rc != IB_OK) return rc;
+  if (h->prm.num_stat) rc = h->preprocess();
+  return rc;
 }
 
 /* write_force2: src/operators_struct.F90:711-731 */

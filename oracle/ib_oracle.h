@@ -27,6 +27,8 @@ int orc_get_level_fields(orc_handle* h, int32_t k, double* omega, double* s,
                          double* qx, double* qy, double* rhs_old);
 int orc_set_level_fields(orc_handle* h, int32_t k, const double* omega, const double* s,
                          const double* qx, const double* qy, const double* rhs_old);
+int orc_set_body(orc_handle* h, const double* f, const double* xb, const double* yb, const double* theta,
+                 const double* thetad, const double* thetadd);
 int orc_get_body(orc_handle* h, double* f, double* f_rdst, double* xb, double* yb,
                  double* theta, double* thetad, double* thetadd);
 int orc_get_forces(orc_handle* h, int32_t which, double* fx, double* fy);
