@@ -86,6 +86,7 @@ SIGNATURES = {
     "op_regT": (C.c_int, [_P, _D, _D, _D, _D]),
     "op_schur_matvec": (C.c_int, [_P, _D, _D]),
     "op_schur_matrix": (C.c_int, [_P, _D]),
+    "pressure": (C.c_int, [_P, _D]),
     "op_nonlinear_rot": (C.c_int, [_P, C.c_int32, _D, _D, _D, _D, _D]),
     "op_gmres": (C.c_int, [_P, _D, _D, _I]),
 }

@@ -161,6 +161,22 @@ void gmres_dense(hipStream_t st, int nf, int kmax, int maxit, const double* S, c
                  double* H, GmresDev* s, double rtol, double atol, size_t max_lds);
 void gmres_dense_set_max_lds(size_t bytes);
 bool gmres_dense_fits(int nf, int kmax, size_t max_lds);
+// pressure post-processing (src/operators_pressure.F90): cell arrays are
+// m x n, i fastest; pressure boundary vectors [left n | right n | bottom m |
+// top m]
+void pressure_corner(hipStream_t st, const Geom& g, const double* w, const double* qx, const double* qy, double q0x,
+                     double* oqx, double* oqy);
+void pressure_edges(hipStream_t st, const Geom& g, const double* oqx, const double* oqy, const double* ubc,
+                    const double* vbc, double del, double* nlx, double* nly);
+void pressure_div(hipStream_t st, const Geom& g, const double* hx, const double* hy, double* out, bool accumulate);
+void pressure_coarsify(hipStream_t st, const Geom& g, const double* fine, double* coarse);
+void pressure_bc1(hipStream_t st, const Geom& g, const double* qx, const double* qy, double q0x, double del,
+                  double* pbc);
+void pressure_bc2(hipStream_t st, const Geom& g, const double* pc, double* pbc);
+void pressure_applybc(hipStream_t st, const Geom& g, double* r, const double* pbc);
+void pressure_final(hipStream_t st, const Geom& g, const double* p, const double* qx, const double* qy, double q0x,
+                    double del, double* out);
+void scale_div(hipStream_t st, int n, const double* x, double d, double* y);
 // In-place Cholesky A = U^T U of the upper triangle of a column-major n x n
 // matrix (err[0] = 1 if a pivot is not positive), and A^-1 from the factor.
 void chol_factor(hipStream_t st, int n, double* A, int* err);

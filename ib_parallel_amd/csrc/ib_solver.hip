@@ -145,6 +145,12 @@ struct ib_handle {
   // factored a_times matrix, and a torsional map with no torsional entries
   DBuf<double> Ainv;
   DBuf<int> no_tors;
+  // pressure post-processing (allocated on first use): cell-grid DST plans
+  // and table, corner products (two levels), rhs, solution, scratch
+  bool pres_ready = false;
+  Geom gp{};  // the m x n cells seen by the DST kernels as an (m+1) x (n+1) node grid
+  HostFFTPlan pxp, pyp;
+  DBuf<double> laminv_ddti, poq, prhs, ppl, pout, ptmp, ptmp2, ppbc, pubc, pvbc, pfdt;
 
   // ------------------------------------------------------------------ util
   void sync() {
@@ -559,6 +565,100 @@ struct ib_handle {
     return status;
   }
 
+  // ddti (src/myfft.F90:192-203): out = S(laminv_ddti S(in)) on the cells
+  void ddti_dev(const double* in, double* out) {
+    DstLaunch a;
+    a.in[0] = in; a.out[0] = ptmp;
+    dst_x(st, pxp.dev, X_PLAIN, a, gp, (pxp.dev.L <= 512 ? 2 : 1), max_lds);
+    DstLaunch b;
+    b.in[0] = ptmp; b.out[0] = ptmp2; b.tab = laminv_ddti; b.div = 1.0;
+    dst_y(st, pyp.dev, Y_SCALE2, b, gp, 1, max_lds);
+    DstLaunch c;
+    c.in[0] = ptmp2; c.out[0] = out;
+    dst_x(st, pxp.dev, X_PLAIN, c, gp, (pxp.dev.L <= 512 ? 2 : 1), max_lds);
+  }
+
+  void pressure_setup() {
+    if (pres_ready) return;
+    const int m = g.m, n = g.n;
+    gp = g;
+    gp.m = m + 1;
+    gp.n = n + 1;
+    gp.P = m * n;
+    pxp.init(m);
+    pyp.init(n);
+    if (dst_lds_bytes(pxp.dev, 1) > max_lds || dst_lds_bytes(pyp.dev, 2) > max_lds)
+      throw HipError("grid too large for the single-workgroup pressure DST (LDS)");
+    // setup_fft's pressure table (src/myfft.F90:123,150-155)
+    const double pi = 4.0 * std::atan(1.0);
+    const double norm_ddti = 4.0 * (double)(float)((m + 1) * (n + 1));
+    std::vector<double> h((size_t)m * n);
+    for (int j = 1; j <= n; ++j)
+      for (int i = 1; i <= m; ++i) {
+        const double lam =
+            2.0 * (std::cos(pi * (double)i / (double)(m + 1)) + std::cos(pi * (double)j / (double)(n + 1)) - 2.0);
+        h[(size_t)(i - 1) + (size_t)m * (j - 1)] = 1.0 / lam / norm_ddti;
+      }
+    const size_t C = (size_t)m * n;
+    laminv_ddti.alloc(C);
+    IB_HIP(hipMemcpy(laminv_ddti.p, h.data(), sizeof(double) * C, hipMemcpyHostToDevice));
+    poq.alloc(4 * (size_t)g.P);
+    prhs.alloc(C * L); ppl.alloc(C * L); pout.alloc(C * L); ptmp.alloc(C); ptmp2.alloc(C);
+    ppbc.alloc(2 * (size_t)(m + n)); pubc.alloc(g.nbc); pvbc.alloc(g.nbc); pfdt.alloc(std::max(nf, 1));
+    pres_ready = true;
+  }
+
+  // calculate_pressure (src/operators_pressure.F90:110-437) at the current
+  // state into pout (mgridlev blocks of m*n pressure coefficients)
+  void pressure_dev() {
+    pressure_setup();
+    const long P = g.P, C = (long)g.m * g.n;
+    // nonlinear terms on the faces of every level, coarse to fine; the bc of
+    // level k comes from the corner products of level k+1
+    double* oq[2][2] = {{poq.p, poq.p + P}, {poq.p + 2 * P, poq.p + 3 * P}};
+    int cur = 0;
+    for (int k = L - 1; k >= 0; --k) {
+      const double* ubc = zbc.p;
+      const double* vbc = zbc.p;
+      if (k < L - 1) {
+        get_bc(st, g, oq[1 - cur][0], P, pubc, g.nbc, 1, 1.0);
+        get_bc(st, g, oq[1 - cur][1], P, pvbc, g.nbc, 1, 1.0);
+        ubc = pubc;
+        vbc = pvbc;
+      }
+      pressure_corner(st, g, omega + k * P, qx + (long)k * g.Fx, qy + (long)k * g.Fy, q0x[k], oq[cur][0],
+                      oq[cur][1]);
+      pressure_edges(st, g, oq[cur][0], oq[cur][1], ubc, vbc, delta * std::pow(2.0, (double)k),
+                     vx + (long)k * g.Fx, vy + (long)k * g.Fy);
+      cur = 1 - cur;
+    }
+    // rhs: -div(E^T f/dt) on level 1, coarsified outward, plus div N(q)
+    scale_div(st, nf, f, dt, pfdt);
+    reg(st, g, csr, w, pfdt, hx, hy);  // hx, hy are zero outside the touched faces
+    pressure_div(st, g, hx, hy, prhs, false);
+    for (int k = 1; k < L; ++k) {
+      fill(st, prhs + k * C, C, 0.0);
+      pressure_coarsify(st, g, prhs + (k - 1) * C, prhs + k * C);
+    }
+    for (int k = 0; k < L; ++k) pressure_div(st, g, vx + (long)k * g.Fx, vy + (long)k * g.Fy, prhs + k * C, true);
+    // D D^T p = rhs, coarse to fine
+    for (int k = L - 1; k >= 0; --k) {
+      if (k == L - 1) {
+        const double dk = delta * std::pow(2.0, (double)k);
+        pressure_bc1(st, g, qx + (long)k * g.Fx, qy + (long)k * g.Fy, q0x[k], 0.125 / (dk * dk), ppbc);
+      } else {
+        pressure_bc2(st, g, ppl + (k + 1) * C, ppbc);
+      }
+      pressure_applybc(st, g, prhs + k * C, ppbc);
+      ddti_dev(prhs + k * C, ppl + k * C);
+    }
+    for (int k = 0; k < L; ++k) {
+      const double dk = delta * std::pow(2.0, (double)k);
+      pressure_final(st, g, ppl + k * C, qx + (long)k * g.Fx, qy + (long)k * g.Fy, q0x[k], 0.125 / (dk * dk),
+                     pout + k * C);
+    }
+  }
+
   // ---------------------------------------------------------------- advance
   int advance() {
     itime += 1;
@@ -814,6 +914,8 @@ struct ib_handle {
     csr.release();
     px.release();
     py.release();
+    pxp.release();
+    pyp.release();
     if (gst) (void)hipFree(gst);
     if (hst) (void)hipHostFree(hst);
     if (hbuf) (void)hipHostFree(hbuf);
@@ -939,6 +1041,18 @@ int ib_get_body(ib_handle* h, double* f, double* f_rdst, double* xb, double* yb,
   if (theta && nt) h->d2h(theta, h->th, nt);
   if (thetad && nt) h->d2h(thetad, h->th + nt, nt);
   if (thetadd && nt) h->d2h(thetadd, h->th + 2 * nt, nt);
+  h->sync();
+  return IB_OK;
+  API_CATCH(h)
+}
+
+// calculate_pressure (src/operators_pressure.F90:110-437) at the current
+// state; p receives mgridlev blocks of m*n values (write_pressure layout)
+int ib_pressure(ib_handle* h, double* p) {
+  if (!h || !p) return IB_E_INVAL;
+  API_TRY
+  h->pressure_dev();
+  h->d2h(p, h->pout, (size_t)h->g.m * h->g.n * h->L);
   h->sync();
   return IB_OK;
   API_CATCH(h)

@@ -91,7 +91,7 @@ def write_body(path, b):
             fh.write(f"{x:.16f}   {y:.16f}\n")
 
 
-def write_ib_inp(path, p, istop=10, isave=10):
+def write_ib_inp(path, p, istop=10, isave=10, compute_pressure=False):
     with open(path, "w") as fh:
         fh.write("&READ_PARAMETERS\n")
         fh.write(f"ISTART = {p.istart},\nISTOP = {istop},\nISAVE = {isave},\nIRESTART = 0,\n")
@@ -99,7 +99,8 @@ def write_ib_inp(path, p, istop=10, isave=10):
         fh.write(f"FSI_TOL = {p.fsi_tol!r},\nATOL = {p.atol!r},\nRTOL = {p.rtol!r},\n")
         fh.write(f"LEN = {p.len!r},\nOFFSETX = {p.offsetx!r},\nOFFSETY = {p.offsety!r},\nMGRIDLEV = {p.mgridlev},\n")
         ns = "T" if getattr(p, "num_stat", 0) else "F"
-        fh.write(f"COMPUTE_PRESSURE = F,\ndimen = 2,\nnum_stat={ns},\nmotion_prescribed=F,\nsub_domain = F,\n/\n")
+        cpf = "T" if compute_pressure else "F"
+        fh.write(f"COMPUTE_PRESSURE = {cpf},\ndimen = 2,\nnum_stat={ns},\nmotion_prescribed=F,\nsub_domain = F,\n/\n")
 
 
 # --------------------------------------------------------------------------

@@ -88,6 +88,12 @@ module ib_gpu_mod
       real(c_double), intent(in) :: omega(*), s(*), qx(*), qy(*), rhs_old(*)
     end function ib_set_level_fields
 
+    integer(c_int) function ib_pressure(h, p) bind(C, name="ib_pressure")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: p(*)
+    end function ib_pressure
+
     integer(c_int) function ib_set_body(h, f, xb, yb, theta, thetad, thetadd) bind(C, name="ib_set_body")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h

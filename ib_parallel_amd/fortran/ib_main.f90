@@ -15,7 +15,8 @@
 !   output/forcerdst01_00.dat   write_force_rdst (src/operators_struct.F90:683-707)
 !   output/theta01_BB.dat       write_theta     (src/operators_struct.F90:624-651)
 !   output/iter01.dat           write_iterations (src/timestepper.F90:799-815)
-! Pressure (compute_pressure) is post-processing and is not computed here.
+!   output/pressure00_IIIIIII.var  write_pressure (src/operators_pressure.F90:857-880)
+!                                  on save steps when compute_pressure = T
 ! Text outputs are opened with a long record length so a step never wraps
 ! onto two lines (the reference is built with ifort -no-wrap-margin,
 ! Makefile:58, for the same reason).
@@ -115,7 +116,10 @@ program ib_main
     end if
     if (mod(it, 10) == 0) write(*,*) '...Advancing to itime =', it
     if (it >= irestart) then
-      if ((mod(it - irestart, isave) == 0) .or. (it == istop)) call write_variables(it)
+      if ((mod(it - irestart, isave) == 0) .or. (it == istop)) then
+        call write_variables(it)
+        if (compute_pressure) call write_pressure(it)
+      end if
     end if
     call write_force(it, 0, 'force')
     call write_force(it, 1, 'forcerdst')
@@ -279,6 +283,29 @@ contains
       stop 1
     end if
   end subroutine read_variables
+
+  ! calculate_pressure + write_pressure (src/main.F90:40-43,
+  ! src/operators_pressure.F90:857-880), nproc = 1
+  subroutine write_pressure(itn)
+    integer(c_int32_t), intent(in) :: itn
+    character(len=7) :: charit
+    real(c_double), allocatable :: pr(:)
+    integer :: k, rcl, nc
+    nc = m * n
+    allocate(pr(nc * mgridlev))
+    rcl = ib_pressure(h, pr)
+    if (rcl /= IB_OK) then
+      write(*,*) 'ib_pressure failed (', rcl, '): ', trim(ib_error_message(h))
+      stop 1
+    end if
+    write(charit, '(I7.7)') itn
+    open(unit=100, file='output/pressure00_'//charit//'.var', form='unformatted', status='unknown')
+    write(100) 1_c_int32_t, int(m, c_int32_t), int(m, c_int32_t), 1_c_int32_t, int(n, c_int32_t), int(n, c_int32_t)
+    do k = 1, mgridlev
+      write(100) pr((k - 1) * nc + 1:k * nc)
+    end do
+    close(100)
+  end subroutine write_pressure
 
   ! write_force / write_force_rdst with write_force2 (src/operators_struct.F90:655-731)
   subroutine write_force(itn, which, stem)

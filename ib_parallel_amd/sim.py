@@ -145,6 +145,12 @@ class Simulation:
         self._check(self._f["op_schur_matvec"](self.h, abi.dptr(x), abi.dptr(y)))
         return y
 
+    def pressure(self):
+        """Pressure coefficient on the cells of every level, shape (mgridlev, n, m)."""
+        p = np.empty(self.L * self.m * self.n)
+        self._check(self._f["pressure"](self.h, abi.dptr(p)))
+        return p.reshape(self.L, self.n, self.m)
+
     def schur_matrix(self):
         """The Schur matrix by columns, S[:, j] = ab_times(e_j) (nf x nf)."""
         S = np.empty((self.nf, self.nf))  # row j of this buffer receives column j

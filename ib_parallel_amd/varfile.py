@@ -1,5 +1,7 @@
-"""Reader for the dataRR_IIIIIII.var snapshot files (write_variables,
-src/variables.F90:709-803; the reference's own reader is output/getdata.m).
+"""Readers for the dataRR_IIIIIII.var snapshot files (write_variables,
+src/variables.F90:709-803; the reference's own reader is output/getdata.m)
+and the pressureRR_IIIIIII.var files (write_pressure,
+src/operators_pressure.F90:857-880).
 
 Fortran unformatted sequential: every record is framed by 4-byte lengths;
 integers are int32 (PetscInt), reals float64.
@@ -41,3 +43,11 @@ def read_var(path):
         r += 1
     out["f_rdst"] = f8(rec[r]).copy()
     return out
+
+
+def read_pressure(path):
+    """{"m", "n", "levels": [array (n, m) per level]} of a pressure .var file."""
+    rec = list(_records(path))
+    xpi, xpe, xpm, ypi, ype, ypm = (int(v) for v in np.frombuffer(rec[0], "<i4"))
+    lv = [np.frombuffer(r, "<f8").reshape(ypm, xpm).copy() for r in rec[1:]]
+    return {"m": xpm, "n": ypm, "levels": lv}

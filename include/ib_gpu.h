@@ -123,6 +123,10 @@ int ib_get_level_fields(ib_handle* h, int32_t k, double* omega, double* s,
 int ib_set_level_fields(ib_handle* h, int32_t k, const double* omega, const double* s,
                         const double* qx, const double* qy, const double* rhs_old);
 /* theta* have one entry per torsional body, in deck order */
+/* calculate_pressure (src/operators_pressure.F90:110-437) at the current
+ * state: the pressure coefficient on the m x n cells of every level, p =
+ * mgridlev blocks of m*n values, i fastest (the write_pressure layout) */
+int ib_pressure(ib_handle* h, double* p);
 /* restart: read_variables' body part (src/variables.F90:871-895) followed by
  * reg_prep (:364-373): sets f (the next GMRES initial guess), the body point
  * positions and the torsional state (nt values each; any pointer may be

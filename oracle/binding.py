@@ -23,6 +23,8 @@ EXTRA = {
     "op_get_bc": (C.c_int, [C.c_void_p, abi._D, C.c_double, abi._D]),
     "op_apply_bc": (C.c_int, [C.c_void_p, abi._D, abi._D, C.c_double]),
     "op_weights": (C.c_int, [C.c_void_p, abi._D, abi._I, abi._I]),
+    "op_ddti": (C.c_int, [C.c_void_p, abi._D, abi._D]),
+    "op_coarsify_pressure": (C.c_int, [C.c_void_p, abi._D, abi._D]),
     "deltafnc": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_int32]),
     "dst1d": (C.c_int, [C.c_int32, C.c_int32, abi._D, abi._D]),
     "op_btimes": (C.c_int, [C.c_void_p, abi._D, abi._D]),
@@ -85,6 +87,19 @@ class OracleSim(Simulation):
         out = np.array(vort, dtype=np.float64, copy=True)
         self._check(self._f["op_apply_bc"](self.h, abi.dptr(out), abi.dptr(np.ascontiguousarray(bc, float)), fac))
         return out
+
+    def ddti(self, rhs):
+        import numpy as np
+        out = np.empty(self.m * self.n)
+        self._check(self._f["op_ddti"](self.h, abi.dptr(np.ascontiguousarray(rhs, float).ravel()), abi.dptr(out)))
+        return out.reshape(self.n, self.m)
+
+    def coarsify_pressure(self, fine, coarse=None):
+        import numpy as np
+        out = np.zeros(self.m * self.n) if coarse is None else np.array(coarse, dtype=np.float64).ravel()
+        self._check(self._f["op_coarsify_pressure"](self.h, abi.dptr(np.ascontiguousarray(fine, float).ravel()),
+                                                    abi.dptr(out)))
+        return out.reshape(self.n, self.m)
 
     def weights(self):
         import numpy as np

@@ -147,6 +147,8 @@ struct orc_handle {
   std::vector<double> laminv, lam1, lam1i; /* lam1, lam1i: L consecutive P-blocks */
   std::vector<double> vfac, con1, con2;
   DST1 dx, dy;
+  DST1 dxp, dyp;                   /* pressure: cells 1..m x 1..n */
+  std::vector<double> laminv_ddti; /* m*n */
   /* state, level index 0..L-1 (= reference k-1) */
   std::vector<std::vector<double>> omega, s, qx, qy, rhs_old, q0x, q0y;
   /* bodies */
@@ -651,6 +653,174 @@ struct orc_handle {
     return reason > 0 ? IB_OK : IB_E_NOCONV_GMRES;
   }
 
+  /* ---------------- pressure: src/operators_pressure.F90 ---------------- */
+  inline long cp(int i, int j) const { return (long)(i - 1) + (long)m * (j - 1); }
+
+  /* 2-D RODFT00 on the m x n cell array (forward_ddti, src/myfft.F90:120) */
+  void dst2d_p(double* a) const {
+    std::vector<cplx> z(2 * std::max(dxp.M, dyp.M)), t(2 * std::max(dxp.M, dyp.M));
+    for (int jp = 0; jp < (n + 1) / 2; ++jp) {
+      int ja = 2 * jp, jb = ja + 1;
+      double* ra = a + (long)ja * m;
+      double* rb = jb < n ? a + (long)jb * m : nullptr;
+      dxp.pair(ra, rb, 1, ra, rb, 1, z.data(), t.data());
+    }
+    for (int ip = 0; ip < (m + 1) / 2; ++ip) {
+      int ia = 2 * ip, ib = ia + 1;
+      double* ca = a + ia;
+      double* cb = ib < m ? a + ib : nullptr;
+      dyp.pair(ca, cb, m, ca, cb, m, z.data(), t.data());
+    }
+  }
+  /* ddti: src/myfft.F90:192-203 */
+  void ddti(const double* rhs, double* p) const {
+    std::vector<double> a(rhs, rhs + (long)m * n);
+    dst2d_p(a.data());
+    for (long q = 0; q < (long)m * n; ++q) a[q] = laminv_ddti[q] * a[q];
+    dst2d_p(a.data());
+    std::memcpy(p, a.data(), sizeof(double) * m * n);
+  }
+  /* divergence over cells (:508-518) */
+  void divergence(const double* hx, const double* hy, double* out) const {
+    for (int j = 1; j <= n; ++j)
+      for (int i = 1; i <= m; ++i)
+        out[cp(i, j)] = hx[fx(i + 1, j)] - hx[fx(i, j)] + hy[fy(i, j + 1)] - hy[fy(i, j)];
+  }
+  /* coarsify_pressure (:530-563) with the index ranges of
+   * setup_coarsify_pressure (:567-605) for one process */
+  void coarsify_pressure(const double* p1, double* p2) const {
+    std::vector<double> temp((long)m * n, 0.0);
+    for (int j = 2; j <= n - 2; j += 2)
+      for (int i = 2; i <= m - 2; i += 2)
+        temp[cp(i, j)] = p1[cp(i, j)] +
+                         0.5 * (p1[cp(i + 1, j)] + p1[cp(i, j + 1)] + p1[cp(i - 1, j)] + p1[cp(i, j - 1)]) +
+                         0.25 * (p1[cp(i + 1, j + 1)] + p1[cp(i + 1, j - 1)] + p1[cp(i - 1, j - 1)] +
+                                 p1[cp(i - 1, j + 1)]);
+    for (int j = 2; j <= n - 4; j += 2)
+      for (int i = 2; i <= m - 4; i += 2) {
+        const double yy = (temp[cp(i, j)] + 3.0 * temp[cp(i + 2, j)] + 3.0 * temp[cp(i, j + 2)] +
+                           9.0 * temp[cp(i + 2, j + 2)]) /
+                          16.0;
+        const int id = m / 4 + i / 2 + 1, jd = n / 4 + j / 2 + 1;
+        p2[cp(id, jd)] = yy;
+      }
+  }
+  /* calculate_pressure (:110-437): pressure coefficient 2 (p + |u|^2/2 ...)
+   * on every level, out = L blocks of m*n (write_pressure layout) */
+  void pressure(double* out) const {
+    const long C = (long)m * n;
+    const int nbcp = 2 * m + 2 * n, lphi = 0, rphi = n, bphi = 2 * n, tphi = 2 * n + m;
+    std::vector<std::vector<double>> qhx(L, std::vector<double>(Fx)), qhy(L, std::vector<double>(Fy));
+    for (int k = 0; k < L; ++k) {
+      const double fac = delta * std::pow(2.0, (double)k);
+      for (int q = 0; q < Fx; ++q) qhx[k][q] = qx[k][q] + (-fac * (-1.0) + 0.0);
+      for (int q = 0; q < Fy; ++q) qhy[k][q] = qy[k][q] + (-fac * 0.0 + 0.0);
+    }
+    /* nonlinear terms N(q) = q x omega on faces, coarse to fine */
+    std::vector<std::vector<double>> nlx(L, std::vector<double>(Fx)), nly(L, std::vector<double>(Fy));
+    std::vector<double> oqx(P), oqy(P), oqx_c(P), oqy_c(P), ubc(nbc, 0.0), vbc(nbc, 0.0);
+    for (int k = L - 1; k >= 0; --k) {
+      if (k == L - 1) {
+        std::fill(ubc.begin(), ubc.end(), 0.0);
+        std::fill(vbc.begin(), vbc.end(), 0.0);
+      } else {
+        get_bc(oqx_c.data(), ubc.data(), 1.0);
+        get_bc(oqy_c.data(), vbc.data(), 1.0);
+      }
+      /* nonlinear_corner (:441-459) */
+      for (int j = 2; j <= n; ++j)
+        for (int i = 2; i <= m; ++i) {
+          oqx[nd(i, j)] = 0.5 * (qhy[k][fy(i, j)] + qhy[k][fy(i - 1, j)]) * omega[k][nd(i, j)];
+          oqy[nd(i, j)] = -0.5 * (qhx[k][fx(i, j)] + qhx[k][fx(i, j - 1)]) * omega[k][nd(i, j)];
+        }
+      /* nonlinear_corner2edge (:463-504) and the 1/del^2 scaling */
+      const double del = delta * std::pow(2.0, (double)k);
+      for (int j = 1; j <= n; ++j)
+        for (int i = 1; i <= m + 1; ++i)
+          nlx[k][fx(i, j)] = (0.5 * (ext(oqx.data(), ubc.data(), i, j + 1) + ext(oqx.data(), ubc.data(), i, j))) /
+                             (del * del);
+      for (int j = 1; j <= n + 1; ++j)
+        for (int i = 1; i <= m; ++i)
+          nly[k][fy(i, j)] = (0.5 * (ext(oqy.data(), vbc.data(), i + 1, j) + ext(oqy.data(), vbc.data(), i, j))) /
+                             (del * del);
+      oqx_c = oqx;
+      oqy_c = oqy;
+    }
+    /* rhs: -div(E^T f/dt) on level 1 (rhs_forcing adds the default zero body force) */
+    std::vector<std::vector<double>> rhs(L, std::vector<double>(C, 0.0));
+    std::vector<double> fdt(nf), hx(Fx, 0.0), hy(Fy, 0.0), dv(C);
+    for (int k = 0; k < nf; ++k) fdt[k] = f[k] / dt;
+    reg(fdt.data(), hx.data(), hy.data());
+    divergence(hx.data(), hy.data(), rhs[0].data());
+    for (long q = 0; q < C; ++q) rhs[0][q] = -rhs[0][q];
+    for (int k = 1; k < L; ++k) coarsify_pressure(rhs[k - 1].data(), rhs[k].data());
+    for (int k = 0; k < L; ++k) {
+      divergence(nlx[k].data(), nly[k].data(), dv.data());
+      for (long q = 0; q < C; ++q) rhs[k][q] = rhs[k][q] + dv[q];
+    }
+    /* solve D D^T p = rhs, coarse to fine */
+    std::vector<std::vector<double>> pl(L, std::vector<double>(C, 0.0));
+    std::vector<double> pbc(nbcp);
+    for (int k = L - 1; k >= 0; --k) {
+      std::fill(pbc.begin(), pbc.end(), 0.0);
+      if (k == L - 1) {
+        /* get_bc_phi1 (:707-746) */
+        const double del = 0.125 / ((delta * std::pow(2.0, (double)k)) * (delta * std::pow(2.0, (double)k)));
+        auto val = [&](int i, int j) {
+          const double a = qhx[k][fx(i + 1, j)] + qhx[k][fx(i, j)], b = qhy[k][fy(i, j + 1)] + qhy[k][fy(i, j)];
+          return del * (a * a + b * b);
+        };
+        for (int j = 1; j <= n; ++j) {
+          pbc[lphi + j - 1] = val(1, j);
+          pbc[rphi + j - 1] = val(m, j);
+        }
+        for (int i = 1; i <= m; ++i) {
+          pbc[bphi + i - 1] = val(i, 1);
+          pbc[tphi + i - 1] = val(i, n);
+        }
+      } else {
+        /* get_bc_phi2 (:750-795) from the coarser level's pressure */
+        const std::vector<double>& pc = pl[k + 1];
+        auto P_ = [&](int i, int j) { return pc[cp(i, j)]; };
+        for (int i = m / 4 + 1; i <= 3 * m / 4; ++i) {
+          int j = n / 4, sl = bphi + 2 * (i - 1 - m / 4);
+          pbc[sl] = 0.25 * (2 * P_(i, j) + P_(i, j + 1) + P_(i - 1, j));
+          pbc[sl + 1] = 0.25 * (2 * P_(i, j) + P_(i, j + 1) + P_(i + 1, j));
+          j = 3 * n / 4 + 1;
+          sl = tphi + 2 * (i - 1 - m / 4);
+          pbc[sl] = 0.25 * (2 * P_(i, j) + P_(i, j - 1) + P_(i - 1, j));
+          pbc[sl + 1] = 0.25 * (2 * P_(i, j) + P_(i, j - 1) + P_(i + 1, j));
+        }
+        for (int j = n / 4 + 1; j <= 3 * n / 4; ++j) {
+          int i = m / 4, sl = lphi + 2 * (j - 1 - n / 4);
+          pbc[sl] = 0.25 * (2 * P_(i, j) + P_(i + 1, j) + P_(i, j - 1));
+          pbc[sl + 1] = 0.25 * (2 * P_(i, j) + P_(i + 1, j) + P_(i, j + 1));
+          i = 3 * m / 4 + 1;
+          sl = rphi + 2 * (j - 1 - n / 4);
+          pbc[sl] = 0.25 * (2 * P_(i, j) + P_(i - 1, j) + P_(i, j - 1));
+          pbc[sl + 1] = 0.25 * (2 * P_(i, j) + P_(i - 1, j) + P_(i, j + 1));
+        }
+      }
+      /* apply_bc_phi (:799-815) */
+      std::vector<double>& r = rhs[k];
+      for (int j = 1; j <= n; ++j) r[cp(1, j)] = r[cp(1, j)] - pbc[lphi + j - 1];
+      for (int j = 1; j <= n; ++j) r[cp(m, j)] = r[cp(m, j)] - pbc[rphi + j - 1];
+      for (int i = 1; i <= m; ++i) r[cp(i, 1)] = r[cp(i, 1)] - pbc[bphi + i - 1];
+      for (int i = 1; i <= m; ++i) r[cp(i, n)] = r[cp(i, n)] - pbc[tphi + i - 1];
+      ddti(r.data(), pl[k].data());
+    }
+    /* static_plus_dynamic (:819-831) and the scaling to the pressure coefficient */
+    for (int k = 0; k < L; ++k) {
+      const double del = 0.125 / ((delta * std::pow(2.0, (double)k)) * (delta * std::pow(2.0, (double)k)));
+      for (int j = 1; j <= n; ++j)
+        for (int i = 1; i <= m; ++i) {
+          const double a = qhx[k][fx(i + 1, j)] + qhx[k][fx(i, j)], b = qhy[k][fy(i, j + 1)] + qhy[k][fy(i, j)];
+          const double v = pl[k][cp(i, j)] - del * (a * a + b * b);
+          out[(long)k * C + cp(i, j)] = 2.0 * v;
+        }
+    }
+  }
+
   /* ---------------- num_stat: precomputed matrix + Cholesky ----------------
    * preprocess (src/timestepper.F90:560-598) assembles column j as
    * a_times(e_j); KSPPREONLY + PCCHOLESKY (src/variables.F90:555-561) then
@@ -849,6 +1019,19 @@ struct orc_handle {
           lam1i[(size_t)k * P + q] = 1.0 / (1.0 + 0.5 * dt * lam / del22 / Re);
         }
       }
+    /* pressure tables (src/myfft.F90:123,150-155) */
+    dxp.init(m);
+    dyp.init(n);
+    {
+      const double norm_ddti = 4.0 * (double)(float)((m + 1) * (n + 1));
+      laminv_ddti.resize((size_t)m * n);
+      for (int j = 1; j <= n; ++j)
+        for (int i = 1; i <= m; ++i) {
+          const double lam = 2.0 * (std::cos(pi * (double)i / (double)(m + 1)) +
+                                    std::cos(pi * (double)j / (double)(n + 1)) - 2.0);
+          laminv_ddti[cp(i, j)] = 1.0 / lam / norm_ddti;
+        }
+    }
     /* bodies: setup_geometry (src/grid.F90:239-398) */
     if (nbodies < 0) return fail(IB_E_INVAL, "negative body count");
     bool seen_tors = false;
@@ -1005,6 +1188,26 @@ int orc_get_body(orc_handle* h, double* f, double* f_rdst, double* xb, double* y
   if (theta && nt) std::memcpy(theta, h->theta.data(), sizeof(double) * nt);
   if (thetad && nt) std::memcpy(thetad, h->thetad.data(), sizeof(double) * nt);
   if (thetadd && nt) std::memcpy(thetadd, h->thetadd.data(), sizeof(double) * nt);
+  return IB_OK;
+}
+
+/* calculate_pressure (src/operators_pressure.F90:110-437) at the current
+ * state: mgridlev blocks of m*n pressure coefficients (write_pressure) */
+int orc_pressure(orc_handle* h, double* p) {
+  if (!p) return IB_E_INVAL;
+  h->pressure(p);
+  return IB_OK;
+}
+
+/* oracle-only: ddti (src/myfft.F90:192-203) of one m x n cell array */
+int orc_op_ddti(orc_handle* h, const double* rhs, double* p) {
+  h->ddti(rhs, p);
+  return IB_OK;
+}
+
+/* oracle-only: coarsify_pressure (src/operators_pressure.F90:530-563) */
+int orc_op_coarsify_pressure(orc_handle* h, const double* fine, double* coarse) {
+  h->coarsify_pressure(fine, coarse);
   return IB_OK;
 }
 

@@ -29,6 +29,7 @@ int orc_set_level_fields(orc_handle* h, int32_t k, const double* omega, const do
                          const double* qx, const double* qy, const double* rhs_old);
 int orc_set_body(orc_handle* h, const double* f, const double* xb, const double* yb, const double* theta,
                  const double* thetad, const double* thetadd);
+int orc_pressure(orc_handle* h, double* p);
 int orc_get_body(orc_handle* h, double* f, double* f_rdst, double* xb, double* yb,
                  double* theta, double* thetad, double* thetadd);
 int orc_get_forces(orc_handle* h, int32_t which, double* fx, double* fy);
@@ -45,6 +46,8 @@ int orc_op_nonlinear_rot(orc_handle* h, int32_t k, const double* omega, const do
                          const double* qy, const double* bc, double* rhs);
 
 /* oracle-only helpers used by the tests */
+int orc_op_ddti(orc_handle* h, const double* rhs, double* p);
+int orc_op_coarsify_pressure(orc_handle* h, const double* fine, double* coarse);
 /* y = b_times2(x) on torsional entries, 0 elsewhere */
 int orc_op_btimes(orc_handle* h, const double* x, double* y);
 /* overwrite the torsional state (one entry per torsional body) */

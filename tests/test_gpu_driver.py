@@ -92,3 +92,26 @@ def test_driver_restart_matches_continuous_run(tmp_path):
     ta = np.loadtxt(a / "output" / "theta01_02.dat")
     tb = np.loadtxt(b / "output" / "theta01_02.dat")
     assert rel(tb[:, 1], ta[2:, 1]) < 1e-11
+
+
+def test_driver_writes_pressure(tmp_path):
+    """compute_pressure = T: output/pressure00_<it>.var on save steps
+    (write_pressure layout) equals the oracle's calculate_pressure."""
+    from oracle import binding as ob
+    p, bodies = decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-12)
+    (tmp_path / "input").mkdir()
+    decks.write_ib_inp(tmp_path / "input" / "ib.inp", p, istop=2, isave=2, compute_pressure=True)
+    for i, b in enumerate(bodies):
+        decks.write_body(tmp_path / "input" / f"body.{i + 1:03d}.inp", b)
+    env = dict(os.environ, IB_GMRES_RESTART="80")
+    r = subprocess.run([DRIVER], cwd=tmp_path, env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    pv = varfile.read_pressure(tmp_path / "output" / "pressure00_0000002.var")
+    p2, b2, _ = decks.read_case(str(tmp_path / "input"))
+    p2.gmres_restart = 80
+    o = ob.OracleSim(p2, b2)
+    o.advance(); o.advance()
+    po = o.pressure()
+    assert (pv["m"], pv["n"], len(pv["levels"])) == (40, 32, 2)
+    for k in range(2):
+        assert rel(pv["levels"][k].ravel(), po[k].ravel()) < 1e-10

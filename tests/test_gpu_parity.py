@@ -154,6 +154,28 @@ def test_step_parity_num_stat_treats_torsional_as_rigid():
     assert np.array_equal(g.body()["theta"], np.zeros_like(g.body()["theta"]))
 
 
+@pytest.mark.parametrize("case", ("tiny", "cylinder"))
+def test_pressure_parity(case):
+    if case == "tiny":
+        p, b = decks.tiny_case(m=40, n=32, mgridlev=3, tol=1e-12, gmres_restart=80)
+    else:
+        p, b = decks.cylinder_case(m=64, n=64, mgridlev=3, dt=0.01, npts=50, tol=1e-12, gmres_restart=120)
+    g, o = gpu(p, b), orc(p, b)
+    for _ in range(3):
+        g.advance(); o.advance()
+    pg, po = g.pressure(), o.pressure()
+    assert pg.shape == po.shape == (g.L, g.n, g.m)
+    for k in range(g.L):
+        assert rel(pg[k].ravel(), po[k].ravel()) < 1e-10, k
+
+
+def test_pressure_of_uniform_flow_is_zero():
+    g = gpu(abi.make_params(48, 40, 3, 0.01, 100.0, 4.0, 1.6, 1.5), [])
+    assert np.abs(g.pressure()).max() < 1e-12
+    g.advance(); g.advance()
+    assert np.abs(g.pressure()).max() < 1e-12
+
+
 def test_face_space_recompute_when_taps_leave_the_set():
     """With no margin the face set goes stale as soon as a body point crosses
     a cell boundary; G is then recomputed for the new set inside the FSI loop
