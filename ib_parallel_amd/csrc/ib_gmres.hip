@@ -91,104 +91,6 @@ __global__ void __launch_bounds__(256) k_gm_dots(int nf, int kmax, const double*
   if (lane == 0) H[(long)it * (kmax + 1) + j] = p;
 }
 
-// w -= sum_j h_j v_j, ||w||, normalisation and the Givens update of the new
-// Hessenberg column (KSPGMRESUpdateHessenberg) on LDS copies.
-__global__ void __launch_bounds__(kGmT) k_gm_update(int nf, int kmax, double* __restrict__ V, GmresDev* s,
-                                                    double* __restrict__ H, double* __restrict__ cc,
-                                                    double* __restrict__ ss, double* __restrict__ grs) {
-  __shared__ double sh[kGmT];
-  extern __shared__ double dyn[];  // hcol[kmax+1] | cc[kmax] | ss[kmax]
-  __shared__ int s_hap, s_skip;
-  __shared__ double s_tt;
-  if (threadIdx.x == 0) s_skip = s->reason != 0;
-  __syncthreads();
-  if (s_skip) return;
-  const int it = s->it;
-  const int ld = kmax + 1;
-  double* hcol = dyn;
-  double* lcc = dyn + ld;
-  double* lss = lcc + kmax;
-  double* w = V + (long)(it + 1) * nf;
-  double* hg = H + (long)it * ld;
-  for (int j = threadIdx.x; j <= it; j += blockDim.x) {
-    hcol[j] = hg[j];
-    if (j < it) {
-      lcc[j] = cc[j];
-      lss[j] = ss[j];
-    }
-  }
-  __syncthreads();
-  // w_i -= sum_j h_j v_j[i]: the j range is split over `nseg` thread groups
-  // (partials in LDS, summed in segment order) to keep more loads in flight
-  double* part = sh;  // reuses the reduction buffer: nseg * nf <= kGmT doubles
-  const int nseg = nf <= kGmT ? (kGmT / nf < 8 ? kGmT / nf : 8) : 1;
-  const int jn = it + 1;
-  const int per = (jn + nseg - 1) / nseg;
-  if (nseg > 1) {
-    const int seg = threadIdx.x / nf, i = threadIdx.x - seg * nf;
-    if (seg < nseg) {
-      const int j0 = seg * per, j1 = j0 + per < jn ? j0 + per : jn;
-      double acc = 0.0;
-      for (int j = j0; j < j1; ++j) acc += -hcol[j] * V[(long)j * nf + i];
-      part[seg * nf + i] = acc;
-    }
-    __syncthreads();
-  }
-  double pn = 0.0;
-  for (int i = threadIdx.x; i < nf; i += blockDim.x) {
-    double acc = w[i];
-    if (nseg > 1) {
-      for (int sg = 0; sg < nseg; ++sg) acc += part[sg * nf + i];
-    } else {
-      for (int j = 0; j < jn; ++j) acc += -hcol[j] * V[(long)j * nf + i];
-    }
-    w[i] = acc;
-    pn += acc * acc;
-  }
-  __syncthreads();
-  const double tt = sqrt(gm_block_sum(pn, sh));
-  if (threadIdx.x == 0) {
-    hcol[it + 1] = tt;
-    double hapbnd = fabs(tt / grs[it]);
-    if (hapbnd > 1e-30) hapbnd = 1e-30;
-    const int hap = tt < hapbnd;
-    s_hap = hap;
-    s_tt = tt;
-    for (int j = 0; j < it; ++j) {
-      const double t0 = hcol[j];
-      hcol[j] = lcc[j] * t0 + lss[j] * hcol[j + 1];
-      hcol[j + 1] = -lss[j] * t0 + lcc[j] * hcol[j + 1];
-    }
-    double res;
-    if (!hap) {
-      const double t1 = sqrt(hcol[it] * hcol[it] + hcol[it + 1] * hcol[it + 1]);
-      const double c = hcol[it] / t1, sn = hcol[it + 1] / t1;
-      cc[it] = c;
-      ss[it] = sn;
-      grs[it + 1] = -sn * grs[it];
-      grs[it] = c * grs[it];
-      hcol[it] = c * hcol[it] + sn * hcol[it + 1];
-      res = fabs(grs[it + 1]);
-    } else {
-      res = 0.0;
-    }
-    s->it = it + 1;
-    s->its = s->its + 1;
-    s->res = res;
-    int reason = 0;
-    if (res <= s->ttol) reason = 1;
-    else if (res >= 1e4 * s->rn0) reason = -1;
-    if (hap && !reason) reason = 1;
-    s->reason = reason;
-  }
-  __syncthreads();
-  for (int j = threadIdx.x; j <= it + 1; j += blockDim.x) hg[j] = hcol[j];
-  if (!s_hap) {
-    const double inv = 1.0 / s_tt;
-    for (int i = threadIdx.x; i < nf; i += blockDim.x) w[i] = w[i] * inv;
-  }
-}
-
 // KSPGMRESBuildSoln: back substitution on the rotated Hessenberg, x += V y.
 // Columns of H are staged in LDS in chunks of `bs` columns so the dependent
 // chain of the substitution runs on LDS values.
@@ -237,27 +139,44 @@ __global__ void __launch_bounds__(kGmT) k_gm_buildsol(int nf, int kmax, int kBsC
   }
 }
 
-// y = S x, S dense nf x nf column major.  A workgroup owns 64 rows (one
-// per lane) and splits the columns over its 4 wavefronts; the 4 partial sums
-// are added in wavefront order.
+// y = S x, S dense nf x nf column major.  A workgroup owns 16 rows and
+// splits the columns into 16 contiguous groups (4 per wavefront: lane & 15
+// is the row, lane >> 4 the group), so the matrix streams through many CUs;
+// the 16 partial sums of a row are added in group order.
 __global__ void __launch_bounds__(256) k_gemv(int nf, const double* __restrict__ S, const double* __restrict__ x,
                                               double* __restrict__ y) {
-  __shared__ double part[4][64];
-  const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
-  const int r = blockIdx.x * 64 + lane;
-  const int per = (nf + 3) / 4;
-  const int j0 = wv * per, j1 = j0 + per < nf ? j0 + per : nf;
+  __shared__ double part[16][16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int rl = lane & 15, grp = wv * 4 + (lane >> 4);
+  const int r = blockIdx.x * 16 + rl;
+  const int per = (nf + 15) / 16;
+  const int j0 = min(nf, grp * per), j1 = min(nf, j0 + per);
   double acc = 0.0;
-  if (r < nf)
-    for (int j = j0; j < j1; ++j) acc += S[(long)j * nf + r] * x[j];
-  part[wv][lane] = acc;
+  if (r < nf) {
+    for (int j = j0; j < j1; j += 8) {
+      double sv[8], xv[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int jj = min(j + c, j1 - 1);
+        sv[c] = S[(long)jj * nf + r];
+        xv[c] = x[jj];
+      }
+#pragma unroll
+      for (int c = 0; c < 8; ++c) acc += (j + c < j1 ? sv[c] : 0.0) * xv[c];
+    }
+  }
+  part[grp][rl] = acc;
   __syncthreads();
-  if (wv == 0 && r < nf) y[r] = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+  if (threadIdx.x < 16 && r < nf) {
+    double a = part[0][rl];
+    for (int q = 1; q < 16; ++q) a += part[q][rl];
+    y[r] = a;
+  }
 }
 
 void gemv(hipStream_t st, int nf, const double* S, const double* x, double* y) {
   if (nf <= 0) return;
-  hipLaunchKernelGGL(k_gemv, dim3(ceil_div(nf, 64)), dim3(256), 0, st, nf, S, x, y);
+  hipLaunchKernelGGL(k_gemv, dim3(ceil_div(nf, 16)), dim3(256), 0, st, nf, S, x, y);
   IB_LAUNCH_CHECK();
 }
 
@@ -375,6 +294,34 @@ __device__ __forceinline__ double gm_dot16(const double* w, const double* v, int
   }
   return p;
 }
+
+template <int CTRL, int RM>
+__device__ __forceinline__ double gm_dpp_old(double v, double old) {
+  const long long b = __double_as_longlong(v), o = __double_as_longlong(old);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)b, CTRL, RM, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), CTRL, RM, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ void gm_scan_step(double& A, double& B) {
+  const double Ap = gm_dpp_old<CTRL, RM>(A, 1.0), Bp = gm_dpp_old<CTRL, RM>(B, 0.0);
+  B = A * Bp + B;
+  A = A * Ap;
+}
+// inclusive scan of the affine maps x -> A x + B over the 64 lanes (lane l
+// ends with f_l o ... o f_0): row_shr 1/2/4/8 within rows of 16, then
+// row_bcast 15 and 31 across rows; lanes a shift leaves without a source
+// compose with the identity (1, 0)
+__device__ __forceinline__ void gm_affine_scan(double& A, double& B) {
+  gm_scan_step<0x111, 0xF>(A, B);
+  gm_scan_step<0x112, 0xF>(A, B);
+  gm_scan_step<0x114, 0xF>(A, B);
+  gm_scan_step<0x118, 0xF>(A, B);
+  gm_scan_step<0x142, 0xA>(A, B);
+  gm_scan_step<0x143, 0xC>(A, B);
+}
+// the value of lane l-1 (wave_shr 1); lane 0 gets `first`
+__device__ __forceinline__ double gm_shift_up1(double v, double first) { return gm_dpp_old<0x138, 0xF>(v, first); }
 
 // row-segment layout of the matrix-vector product: thread (seg, r) owns row
 // r and columns [j0, j1); nseg segments of nf threads
@@ -565,26 +512,18 @@ __global__ void __launch_bounds__(kGmT) k_gmres_dense(GmDenseArgs a) {
         const int lane = gm_opaque(threadIdx.x);
         // previous rotations on the new column (KSPGMRESUpdateHessenberg): the
         // running entry obeys carry_(j+1) = -s_j carry_j + c_j h_(j+1), an
-        // affine recurrence evaluated 64 rotations at a time by a scan
+        // affine recurrence evaluated 64 rotations at a time by a DPP scan
         double carry = hcol[0];
         for (int base = 0; base < it; base += 64) {
           const int j = base + lane;
           const bool act = j < it;
           const double c = act ? lcc[j] : 1.0, sn = act ? lss[j] : 0.0, hn = act ? hcol[j + 1] : 0.0;
           double A = -sn, B = c * hn;
-#pragma unroll
-          for (int d = 1; d < 64; d <<= 1) {
-            const double Ap = __shfl_up(A, d), Bp = __shfl_up(B, d);
-            if (lane >= d) {
-              B = A * Bp + B;
-              A = A * Ap;
-            }
-          }
-          const double cnext = A * carry + B;  // carry_(j+1)
-          double cj = __shfl_up(cnext, 1);     // carry_j
-          if (lane == 0) cj = carry;
+          gm_affine_scan(A, B);
+          const double cnext = A * carry + B;            // carry_(j+1)
+          const double cj = gm_shift_up1(cnext, carry);  // carry_j (lane 0: the carry in)
           if (act) hcol[j] = c * cj + sn * hn;
-          carry = __shfl(cnext, min(63, it - 1 - base));
+          carry = gm_readlane(cnext, min(63, it - 1 - base));
         }
         if (lane == 0) {
           hcol[it] = carry;
@@ -807,6 +746,125 @@ void chol_inverse(hipStream_t st, int n, const double* U, double* Ainv, size_t m
   IB_LAUNCH_CHECK();
 }
 
+// ------------------------------------------------------------------------
+// Multi-kernel Arnoldi step for large systems (the matrix-free form and dense
+// matrices past the one-workgroup size): orthogonalisation over many
+// workgroups, Givens in one wavefront (affine scan), normalisation.
+// ------------------------------------------------------------------------
+
+// w -= sum_j h_j v_j on 64 rows per workgroup (16 wavefronts split the j
+// range, partials added in wavefront order); partial ||w||^2 per workgroup
+__global__ void __launch_bounds__(1024) k_gm_orth(int nf, int kmax, double* __restrict__ V,
+                                                  const GmresDev* __restrict__ s, const double* __restrict__ H,
+                                                  double* __restrict__ pnorm) {
+  __shared__ double part[16][64];
+  if (s->reason) return;
+  const int it = s->it, ld = kmax + 1;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r = blockIdx.x * 64 + lane;
+  const double* h = H + (long)it * ld;
+  double* w = V + (long)(it + 1) * nf;
+  const int jn = it + 1, per = (jn + 15) / 16;
+  const int j0 = min(jn, wv * per), j1 = min(jn, j0 + per);
+  double acc = 0.0;
+  if (r < nf) {
+    for (int j = j0; j < j1; j += 8) {
+      double v[8], hv[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int jj = min(j + c, j1 - 1);
+        v[c] = V[(long)jj * nf + r];
+        hv[c] = h[jj];
+      }
+#pragma unroll
+      for (int c = 0; c < 8; ++c) acc += -(j + c < j1 ? hv[c] : 0.0) * v[c];
+    }
+  }
+  part[wv][lane] = acc;
+  __syncthreads();
+  if (wv == 0) {
+    double pn = 0.0;
+    if (r < nf) {
+      double a = w[r];
+      for (int q = 0; q < 16; ++q) a += part[q][lane];
+      w[r] = a;
+      pn = a * a;
+    }
+    pn = gm_wave_sum(pn);
+    if (lane == 0) pnorm[blockIdx.x] = pn;
+  }
+}
+
+// Givens update of the new Hessenberg column (KSPGMRESUpdateHessenberg) by
+// one wavefront: tt from the partial norms (block order), the previous
+// rotations as an affine scan, the new rotation, residual estimate and
+// stopping tests; advances the status to the next iteration.
+__global__ void __launch_bounds__(64) k_gm_givens(int kmax, GmresDev* s, double* __restrict__ H,
+                                                  double* __restrict__ cc, double* __restrict__ ss,
+                                                  double* __restrict__ grs, const double* __restrict__ pnorm,
+                                                  int nblk) {
+  if (s->reason) return;
+  const int lane = threadIdx.x;
+  const int it = s->it, ld = kmax + 1;
+  double* hcol = H + (long)it * ld;
+  double a = 0.0;
+  for (int b = 0; b < nblk; ++b) a += pnorm[b];
+  const double tt = sqrt(a);
+  double carry = hcol[0];
+  for (int base = 0; base < it; base += 64) {
+    const int j = base + lane;
+    const bool act = j < it;
+    const double c = act ? cc[j] : 1.0, sn = act ? ss[j] : 0.0, hn = act ? hcol[j + 1] : 0.0;
+    double A = -sn, B = c * hn;
+    gm_affine_scan(A, B);
+    const double cnext = A * carry + B;            // carry_(j+1)
+    const double cj = gm_shift_up1(cnext, carry);  // carry_j (lane 0: the carry in)
+    if (act) hcol[j] = c * cj + sn * hn;
+    carry = gm_readlane(cnext, min(63, it - 1 - base));
+  }
+  if (lane == 0) {
+    hcol[it] = carry;
+    hcol[it + 1] = tt;
+    double hapbnd = fabs(tt / grs[it]);
+    if (hapbnd > 1e-30) hapbnd = 1e-30;
+    const int hap = tt < hapbnd;
+    double res;
+    if (!hap) {
+      const double t1 = sqrt(hcol[it] * hcol[it] + hcol[it + 1] * hcol[it + 1]);
+      const double c = hcol[it] / t1, sn = hcol[it + 1] / t1;
+      cc[it] = c;
+      ss[it] = sn;
+      grs[it + 1] = -sn * grs[it];
+      grs[it] = c * grs[it];
+      hcol[it] = c * hcol[it] + sn * hcol[it + 1];
+      res = fabs(grs[it + 1]);
+    } else {
+      res = 0.0;
+    }
+    s->tt = tt;
+    s->hap = hap;
+    s->it = it + 1;
+    s->its = s->its + 1;
+    s->res = res;
+    int reason = 0;
+    if (res <= s->ttol) reason = 1;
+    else if (res >= 1e4 * s->rn0) reason = -1;
+    if (hap && !reason) reason = 1;
+    s->reason = reason;
+  }
+}
+
+// v_it = w / tt for the basis vector k_gm_givens has just closed (skipped
+// after a happy breakdown; a vector closed at convergence is never used)
+__global__ void __launch_bounds__(256) k_gm_scale(int nf, double* __restrict__ V, const GmresDev* __restrict__ s) {
+  if (s->reason || s->hap) return;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nf) return;
+  const double inv = 1.0 / s->tt;
+  double* v = V + (long)s->it * nf;
+  v[r] = v[r] * inv;
+}
+
 void gm_start(hipStream_t st, int nf, int kmax, const double* b, const double* Ax, double* V, GmresDev* s, double* H,
               double* grs, double rtol, double atol) {
   (void)kmax;
@@ -816,12 +874,16 @@ void gm_start(hipStream_t st, int nf, int kmax, const double* b, const double* A
 }
 
 void gm_arnoldi(hipStream_t st, int nf, int kmax, double* V, GmresDev* s, double* H, double* cc, double* ss,
-                double* grs) {
+                double* grs, double* pnorm) {
   // kmax+1 dot products at most: one wavefront each, 4 per workgroup
   hipLaunchKernelGGL(k_gm_dots, dim3(ceil_div(kmax + 1, 4)), dim3(256), 0, st, nf, kmax, V, s, H);
   IB_LAUNCH_CHECK();
-  const size_t lds = sizeof(double) * (size_t)(3 * kmax + 1);
-  hipLaunchKernelGGL(k_gm_update, dim3(1), dim3(kGmT), lds, st, nf, kmax, V, s, H, cc, ss, grs);
+  const int nblk = ceil_div(nf, 64);
+  hipLaunchKernelGGL(k_gm_orth, dim3(nblk), dim3(1024), 0, st, nf, kmax, V, s, H, pnorm);
+  IB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(64), 0, st, kmax, s, H, cc, ss, grs, pnorm, nblk);
+  IB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_gm_scale, dim3(ceil_div(nf, 256)), dim3(256), 0, st, nf, V, s);
   IB_LAUNCH_CHECK();
 }
 

@@ -146,13 +146,19 @@ void body_initial_acc(hipStream_t st, int nt, const TorsBody* tb, double* th, do
 struct GmresDev {
   double ttol, rn0, bnorm, res;
   int its, it, reason, first;
+  double tt;  // norm of the last orthogonalised vector (multi-kernel iterations)
+  int hap, pad;
 };
 void gm_start(hipStream_t st, int nf, int kmax, const double* b, const double* Ax, double* V, GmresDev* s,
               double* H, double* grs, double rtol, double atol);
 // one Arnoldi step: wavefront-parallel CGS dot products over several
 // workgroups, then orthogonalisation, norm and Givens update in one workgroup
+// one Arnoldi step after the product w = A v_it: wavefront-parallel CGS dot
+// products, multi-workgroup orthogonalisation with partial norms (pnorm
+// holds ceil(nf/64) doubles), the Givens update in one wavefront and the
+// normalisation of the new basis vector
 void gm_arnoldi(hipStream_t st, int nf, int kmax, double* V, GmresDev* s, double* H, double* cc, double* ss,
-                double* grs);
+                double* grs, double* pnorm);
 void gm_buildsol(hipStream_t st, int nf, int kmax, const double* V, GmresDev* s, const double* H,
                  const double* grs, double* x, double* scratch);
 // The whole GMRES solve with a dense nf x nf Schur matrix in one workgroup;

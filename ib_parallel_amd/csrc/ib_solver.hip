@@ -117,7 +117,7 @@ struct ib_handle {
   DBuf<double> th, dabs;
   // GMRES
   int kmax = 30;
-  DBuf<double> V, H, cc, ss, grs, nrs;
+  DBuf<double> V, H, cc, ss, grs, nrs, pnorm;
   GmresDev* gst = nullptr;
   GmresDev* hst = nullptr;  // pinned
   double* hbuf = nullptr;   // pinned readback
@@ -425,7 +425,7 @@ struct ib_handle {
   void gmres_batch_direct(int it0, int nb) {
     for (int q = 0; q < nb; ++q) {
       apply_schur(V + (size_t)(it0 + q) * nf, V + (size_t)(it0 + q + 1) * nf);
-      gm_arnoldi(st, nf, kmax, V, gst, H, cc, ss, grs);
+      gm_arnoldi(st, nf, kmax, V, gst, H, cc, ss, grs, pnorm);
     }
   }
   void gmres_batch(int it0, int nb) {
@@ -858,6 +858,7 @@ struct ib_handle {
     V.alloc((size_t)(kmax + 1) * std::max(1, nf));
     H.alloc((size_t)(kmax + 1) * kmax);
     cc.alloc(kmax); ss.alloc(kmax); grs.alloc(kmax + 1); nrs.alloc(kmax + 1);
+    pnorm.alloc(std::max(1, ceil_div(std::max(nf, 1), 64)));
     IB_HIP(hipMalloc(&gst, sizeof(GmresDev)));
     IB_HIP(hipHostMalloc(&hst, sizeof(GmresDev), hipHostMallocDefault));
     IB_HIP(hipHostMalloc(&hbuf, sizeof(double) * std::max(64, ntors + 1), hipHostMallocDefault));
