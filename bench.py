@@ -25,6 +25,9 @@ Also reported:
   cpu_baseline  the CPU oracle (oracle/, a C++ restatement of the same
                 algorithm, OpenMP over DST rows) on the host cores, on a
                 bounded sample of the same workload from the same cold start.
+  batched       the batched form (ib_batch_*) on 64 uneven rigid-ellipse
+                simulations against the same simulations stepped one after
+                another with the plain step (run_batched).
 Workloads: plate (default, the headline), cylinder (rigid, 200x200x3) and
 airfoil_flap (rigid airfoil + torsional flap, 1024x512x5, full GMRES).
 """
@@ -129,6 +132,53 @@ def run_gpu(args):
     return res
 
 
+def run_batched(args):
+    """Batched form against the same simulations stepped one after another.
+
+    Workload: ellipse_batch_cases(count) (rigid ellipses, num_stat, grids of
+    256..512 cells, 3..5 levels, 100..400 points, uneven).  Batched: one
+    ib_batch_advance per step (one graph replay covering every simulation).
+    Serial plain: ib_advance of each simulation in turn, the plain GPU step run
+    serially.  Both are whole-batch rates: sum of m*n*mgridlev over the
+    simulations per step."""
+    from ib_parallel_amd import GpuBatch, decks
+    cases = decks.ellipse_batch_cases(count=args.batch)
+    t0 = time.perf_counter()
+    batch = GpuBatch(cases)
+    t_create = time.perf_counter() - t0
+    pts = sum(gridpts(s) for s in batch.sims)
+    batch.advance()
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        batch.advance()
+    device_sync()
+    tb = time.perf_counter() - t0
+    for s in batch.sims:
+        s.advance()
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        for s in batch.sims:
+            s.advance()
+    device_sync()
+    ts = time.perf_counter() - t0
+    vb = pts * args.batch_steps / tb / 1e6
+    vs = pts * args.batch_steps / ts / 1e6
+    res = {
+        "workload": (f"{args.batch} rigid ellipses (AR U[1,4], AoA U[0,20] deg, Re U[100,400]), grids 256..512 "
+                     "per side, 3..5 levels, 100..400 points, num_stat, PCG64 seed 20240611"),
+        "instances": args.batch, "mode": {1: "fused", 2: "concurrent"}[batch.mode],
+        "gridpts_per_step": pts, "steps": args.batch_steps,
+        "value": round(vb, 3), "unit": "Mgridpt-steps/s", "ms_per_step": round(tb / args.batch_steps * 1e3, 3),
+        "serial_plain_value": round(vs, 3),
+        "serial_plain_ms_per_step": round(ts / args.batch_steps * 1e3, 3),
+        "speedup_vs_serial_plain": round(vb / vs, 2), "create_s": round(t_create, 2),
+    }
+    batch.close()
+    return res
+
+
 def copy_calibration(mb):
     """A device-to-device copy of exactly mb MiB (a plain copy kernel of known
     size for calibrating FETCH_SIZE / WRITE_SIZE in counter runs)."""
@@ -192,6 +242,8 @@ def main():
     ap.add_argument("--workload", default="plate", choices=["plate", "cylinder", "airfoil_flap"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=64, help="simulations in the batched measurement (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--pmc-copy-mb", type=float, default=0.0,
                     help="run a device-to-device copy of this size first (calibrates counter runs)")
     args = ap.parse_args()
@@ -225,6 +277,8 @@ def main():
                    "schur_operator": g["schur"]},
         "roofline": g["roofline"],
     }
+    if args.batch > 0:
+        out["batched"] = run_batched(args)
     if not args.no_cpu:
         cpu = run_cpu(args, g["its"] / args.steps)
         out["cpu_baseline"] = cpu

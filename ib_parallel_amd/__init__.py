@@ -59,4 +59,73 @@ class GpuSim(Simulation):
         return n.value, ms.value, by.value
 
 
-__all__ = ["GpuSim", "IBError", "Simulation", "abi", "load", "library"]
+class GpuBatch:
+    """Independent simulations advanced one step together (ib_batch_*).
+
+    ``sims[i]`` is simulation i as a ``GpuSim`` view for state access; it is
+    owned by the batch and valid while the batch is open."""
+
+    def __init__(self, cases):
+        """cases: list of (params, bodies) pairs."""
+        fns = load()
+        self._b = abi.bind(_lib, "ib_", abi.BATCH)
+        n = len(cases)
+        self._params = (abi.IBParams * n)(*[p for p, _ in cases])
+        self._nb = (C.c_int32 * n)(*[len(b) for _, b in cases])
+        flat = [b for _, bs in cases for b in bs]
+        self._bodies_keep = abi.make_bodies(flat)
+        h = C.c_void_p()
+        rc = self._b["batch_create"](n, self._params, self._nb, self._bodies_keep[0], C.byref(h))
+        if rc != abi.IB_OK:
+            raise IBError(rc, self._err(None))
+        self.h = h
+        self.sims = []
+        for i in range(n):
+            hi = C.c_void_p()
+            self._check(self._b["batch_handle"](h, i, C.byref(hi)))
+            s = GpuSim.__new__(GpuSim)
+            Simulation.__init__(s, fns, cases[i][0], cases[i][1], prefix="ib_", handle=hi)
+            self.sims.append(s)
+        self.itimes = (C.c_int32 * n)()
+        self.infos = (abi.IBStepInfo * n)()
+
+    def _err(self, h):
+        buf = C.create_string_buffer(1024)
+        self._b["batch_last_error"](h, buf, 1024)
+        return buf.value.decode(errors="replace")
+
+    def _check(self, rc, allow=()):
+        if rc != abi.IB_OK and rc not in allow:
+            raise IBError(rc, self._err(self.h))
+        return rc
+
+    @property
+    def mode(self):
+        """1: fused (one graph launch per step), 2: concurrent (a host thread per simulation)."""
+        n, m = C.c_int32(), C.c_int32()
+        self._check(self._b["batch_info"](self.h, C.byref(n), C.byref(m)))
+        return m.value
+
+    def advance(self, allow_noconv=False):
+        rc = self._b["batch_advance"](self.h, self.itimes, self.infos)
+        allow = (abi.IB_E_NOCONV_GMRES, abi.IB_E_NOCONV_FSI) if allow_noconv else ()
+        self._check(rc, allow)
+        for s, t in zip(self.sims, self.itimes):
+            s.itime.value = t
+        return list(self.infos)
+
+    def close(self):
+        if getattr(self, "h", None):
+            for s in self.sims:
+                s.h = None
+            self._b["batch_destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+__all__ = ["GpuBatch", "GpuSim", "IBError", "Simulation", "abi", "load", "library"]

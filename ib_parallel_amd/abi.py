@@ -99,6 +99,17 @@ GPU_ONLY = {
 }
 
 
+# the batched form: GPU library only
+BATCH = {
+    "batch_create": (C.c_int, [C.c_int32, C.POINTER(IBParams), _I, C.POINTER(IBBody), C.POINTER(_P)]),
+    "batch_destroy": (C.c_int, [_P]),
+    "batch_advance": (C.c_int, [_P, _I, C.POINTER(IBStepInfo)]),
+    "batch_handle": (C.c_int, [_P, C.c_int32, C.POINTER(_P)]),
+    "batch_info": (C.c_int, [_P, _I, _I]),
+    "batch_last_error": (C.c_int, [_P, C.c_char_p, C.c_int32]),
+}
+
+
 def bind(lib, prefix, extra=None):
     """Attach restype/argtypes for every SIGNATURES entry present in `lib`."""
     table = dict(SIGNATURES)

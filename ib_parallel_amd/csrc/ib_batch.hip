@@ -1,0 +1,413 @@
+// ib_batch.hip -- many independent simulations advanced one step together.
+//
+// Fused form: when every simulation's step has no data-dependent control
+// flow (rigid bodies with the num_stat solve), one batched step is a fixed
+// sequence of table launches, each covering all simulations that take part
+// in it (kernel arguments of simulation e in tab[e], blockIdx.z = e), and
+// the whole sequence is captured once as a HIP graph (launch_mode 1 of the
+// first simulation: the same launches issued directly each step).  Simulations of
+// uneven grid size and level count share each launch: levels are aligned
+// from the finest (level index 0) and a simulation joins the launches of
+// the levels it has.  Every simulation runs exactly the arithmetic of its
+// own plain step (the same kernels, the same operands in the same order),
+// so batched and plain results are identical.
+//
+// Concurrent form (every other mix: torsional bodies, GMRES solves with
+// host-side convergence control): one host thread per simulation drives its
+// plain step on the simulation's own stream.
+#include <thread>
+
+#include "ib_handle.h"
+
+namespace {
+
+// One launch of the fused step: a table kernel, its grid and its table.
+struct BatchOp {
+  const void* fn = nullptr;
+  dim3 grid;
+  size_t lds = 0;
+  size_t off = 0;  // table offset in the arena
+  int count = 0;
+};
+
+// Host staging of the launch tables; all tables go to one device arena.
+struct OpList {
+  std::vector<BatchOp> ops;
+  std::vector<char> bytes;
+
+  template <class A>
+  void add(const void* fn, const std::vector<TabEntry<A>>& es, size_t lds = 0) {
+    if (es.empty()) return;
+    BatchOp op;
+    op.fn = fn;
+    op.lds = lds;
+    op.count = (int)es.size();
+    int gx = 1, gy = 1;
+    for (const auto& e : es) {
+      gx = std::max(gx, e.nbx);
+      gy = std::max(gy, e.nby);
+    }
+    op.grid = dim3(gx, gy, op.count);
+    const size_t align = 64;
+    op.off = (bytes.size() + align - 1) / align * align;
+    bytes.resize(op.off + sizeof(TabEntry<A>) * es.size());
+    std::memcpy(bytes.data() + op.off, es.data(), sizeof(TabEntry<A>) * es.size());
+    ops.push_back(op);
+  }
+};
+
+std::string g_batch_create_err;
+
+}  // namespace
+
+struct ib_batch {
+  std::vector<ib_handle*> sims;
+  std::string err;
+  int mode = 0;  // 1 fused, 2 concurrent
+  hipStream_t st = nullptr;
+  OpList plan;
+  DBuf<char> arena;
+  DBuf<int> step;          // itime of the step being taken, per simulation
+  int* hstep = nullptr;    // pinned staging of step
+  hipGraphExec_t ge = nullptr;
+  bool direct = false;     // launch_mode 1 of simulation 0: plain launches, no graph
+
+  ~ib_batch() {
+    if (st) (void)hipStreamSynchronize(st);
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (hstep) (void)hipHostFree(hstep);
+    arena.release();
+    step.release();
+    if (st) (void)hipStreamDestroy(st);
+    for (ib_handle* h : sims) delete h;
+  }
+
+  // rigid bodies and the precomputed num_stat inverse: nothing in the step
+  // depends on values read back to the host
+  static bool fusable(const ib_handle* h) { return h->ntors == 0 && h->nf > 0 && h->prm.num_stat; }
+
+  // ---------------------------------------------------------- fused plan
+  // Row / column pass entries with the pair counts the plain step uses.
+  static TabEntry<XArgs> xe(const ib_handle* h, const DstLaunch& l, size_t* lds) {
+    return dst_x_entry(h->px.dev, l, h->g, (h->px.dev.L <= 512 ? 2 : 1), h->max_lds, lds);
+  }
+  static TabEntry<YArgs> ye(const ib_handle* h, int mode, const DstLaunch& l, size_t* lds) {
+    return dst_y_entry(h->py.dev, mode, l, h->g, 1, h->max_lds, lds);
+  }
+
+  template <class A>
+  using Tab = std::vector<TabEntry<A>>;
+
+  // vort2flux (src/operators_fluid.F90:237-377) of every simulation: the
+  // launch sequence of ib_handle::vort2flux_dev, level by level.  A
+  // simulation's coarsest ctci takes its zero boundary vector through the
+  // apply_bc form, which adds exact zeros.
+  void plan_vort2flux(int Lmax) {
+    for (int k = 1; k < Lmax; ++k) {
+      Tab<CoarsenArgs> c;
+      for (ib_handle* h : sims)
+        if (k < h->L) c.push_back(coarsify_entry(h->g, h->omega + (k - 1) * (long)h->g.P, h->omega + k * (long)h->g.P, 1, 0));
+      plan.add(fluid_tab_kernel(TK_COARSIFY), c);
+    }
+    std::vector<const double*> sbc(sims.size());
+    for (size_t i = 0; i < sims.size(); ++i) sbc[i] = sims[i]->zbc;
+    for (int k = Lmax - 1; k >= 0; --k) {
+      Tab<XArgs> x1, x2;
+      Tab<YArgs> y1;
+      Tab<CurlArgs> cu;
+      size_t l1 = 0, l2 = 0, l3 = 0, t;
+      for (size_t i = 0; i < sims.size(); ++i) {
+        ib_handle* h = sims[i];
+        if (k >= h->L) continue;
+        const long P = h->g.P;
+        DstLaunch a;
+        a.in[0] = h->omega + k * P; a.out[0] = h->t0; a.bc = sbc[i];
+        x1.push_back(xe(h, a, &t)); l1 = std::max(l1, t);
+        DstLaunch b;
+        b.in[0] = h->t0; b.out[0] = h->t1; b.tab = h->laminv; b.div = 1.0;
+        y1.push_back(ye(h, Y_SCALE2, b, &t)); l2 = std::max(l2, t);
+        DstLaunch c;
+        c.in[0] = h->t1; c.out[0] = h->s + k * P;
+        x2.push_back(xe(h, c, &t)); l3 = std::max(l3, t);
+        double* bout = nullptr;
+        if (k > 0) bout = (sbc[i] == h->sbcA.p) ? h->sbcB.p : h->sbcA.p;
+        cu.push_back(curl_getbc_entry(h->g, h->s + k * P, sbc[i], h->qx + (long)k * h->g.Fx,
+                                      h->qy + (long)k * h->g.Fy, bout));
+        sbc[i] = bout;
+      }
+      plan.add(dst_x_tab_kernel(X_APPLYBC), x1, l1);
+      plan.add(dst_y_tab_kernel(Y_SCALE2), y1, l2);
+      plan.add(dst_x_tab_kernel(X_PLAIN), x2, l3);
+      plan.add(fluid_tab_kernel(TK_CURL_GETBC), cu);
+    }
+  }
+
+  // advance(itime) for num_stat = T, rigid bodies: the launch sequence of
+  // ib_handle::advance (src/timestepper.F90:18-556) over all simulations
+  void plan_step() {
+    int Lmax = 0;
+    for (ib_handle* h : sims) Lmax = std::max(Lmax, h->L);
+    size_t t;
+    // step 1: lastbc (src/timestepper.F90:103-117)
+    {
+      Tab<GetBcArgs> gb;
+      Tab<FillArgs> fl;
+      for (ib_handle* h : sims) {
+        const long P = h->g.P, nbc = h->g.nbc;
+        if (h->L > 1) gb.push_back(get_bc_entry(h->g, h->omega + P, P, h->lastbc, nbc, h->L - 1, 0.25));
+        fl.push_back(fill_entry(h->lastbc + (h->L - 1) * nbc, nbc, 0.0));
+      }
+      plan.add(fluid_tab_kernel(TK_GET_BC), gb);
+      plan.add(fluid_tab_kernel(TK_FILL), fl);
+    }
+    // predictor, coarse to fine
+    for (int k = Lmax - 1; k >= 0; --k) {
+      Tab<GetBcArgs> gb;
+      Tab<NLArgs> nl;
+      Tab<XArgs> x1, x2;
+      Tab<YArgs> y1;
+      size_t l1 = 0, l2 = 0, l3 = 0;
+      for (size_t i = 0; i < sims.size(); ++i) {
+        ib_handle* h = sims[i];
+        if (k >= h->L) continue;
+        const long P = h->g.P, nbc = h->g.nbc;
+        const double* ob = h->zbc;
+        if (k < h->L - 1) {
+          gb.push_back(get_bc_entry(h->g, h->omega + (k + 1) * P, P, h->obc, nbc, 1, 0.25));
+          ob = h->obc;
+        }
+        nl.push_back(nl_rhs_entry(h->g, h->omega + k * P, h->qx + (long)k * h->g.Fx, h->qy + (long)k * h->g.Fy,
+                                  h->q0x[k], h->lastbc + k * nbc, ob, h->vfac[k], h->con1[k], h->con2[k],
+                                  step.p + i, h->rhs_old + k * P, h->t2));
+        DstLaunch a;
+        a.batch = 2;
+        a.in[0] = h->t2; a.out[0] = h->t0;
+        a.in[1] = h->omega + k * P; a.out[1] = h->t1;
+        x1.push_back(xe(h, a, &t)); l1 = std::max(l1, t);
+        DstLaunch b;
+        b.in[0] = h->t0; b.in2[0] = h->t1; b.out[0] = h->t2; b.tab = h->lam1 + k * P; b.tab2 = h->lam1i + k * P;
+        y1.push_back(ye(h, Y_PRED, b, &t)); l2 = std::max(l2, t);
+        DstLaunch c;
+        c.in[0] = h->t2; c.out[0] = h->omega + k * P;
+        x2.push_back(xe(h, c, &t)); l3 = std::max(l3, t);
+      }
+      plan.add(fluid_tab_kernel(TK_GET_BC), gb);
+      plan.add(fluid_tab_kernel(TK_NL_RHS), nl);
+      plan.add(dst_x_tab_kernel(X_PLAIN), x1, l1);
+      plan.add(dst_y_tab_kernel(Y_PRED), y1, l2);
+      plan.add(dst_x_tab_kernel(X_PLAIN), x2, l3);
+    }
+    plan_vort2flux(Lmax);
+    // step 2, num_stat (src/timestepper.F90:457-484): rhsf = regT(q + q0),
+    // f = A^-1 rhsf, redistribute
+    {
+      Tab<FsiRhsArgs> fr;
+      Tab<GemvArgs> gv;
+      Tab<RegArgs> rd, rg;
+      Tab<RegTArgs> rt;
+      for (ib_handle* h : sims) {
+        fr.push_back(fsi_rhs_entry(h->g, h->nf, h->no_tors, h->tb, 0, h->th, h->dt, h->delta, h->w, h->csr.key,
+                                   h->qx, h->qy, h->q0x[0], h->rhsf));
+        gv.push_back(gemv_entry(h->nf, h->Ainv, h->rhsf, h->f));
+        rd.push_back(redist_entry(h->g, h->csr, h->w, h->f, h->hx, h->hy));
+        rt.push_back(regT_entry(h->g, h->nf, h->w, h->csr.key, h->hx, h->hy, 0.0, nullptr, h->f_rdst));
+        rg.push_back(reg_entry(h->g, h->csr, h->w, h->f, h->hx, h->hy));
+      }
+      plan.add(body_tab_kernel(TK_FSI_RHS), fr);
+      plan.add(gemv_tab_kernel(), gv);
+      plan.add(body_tab_kernel(TK_REDIST), rd);
+      plan.add(body_tab_kernel(TK_REGT), rt);
+      // step 3: corrector (src/timestepper.F90:488-553)
+      plan.add(body_tab_kernel(TK_REG), rg);
+    }
+    {
+      Tab<XArgs> x1, x2;
+      Tab<YArgs> y1;
+      size_t l1 = 0, l2 = 0, l3 = 0;
+      for (ib_handle* h : sims) {
+        DstLaunch a;
+        a.in[0] = nullptr; a.out[0] = h->t0; a.hx = h->hx; a.hy = h->hy;
+        x1.push_back(xe(h, a, &t)); l1 = std::max(l1, t);
+        DstLaunch b;
+        b.in[0] = h->t0; b.out[0] = h->t1; b.tab = h->lam1i; b.div = h->normalize;
+        y1.push_back(ye(h, Y_SCALE2, b, &t)); l2 = std::max(l2, t);
+        DstLaunch c;
+        c.in[0] = h->t1; c.out[0] = h->omega;
+        x2.push_back(xe(h, c, &t)); l3 = std::max(l3, t);
+      }
+      plan.add(dst_x_tab_kernel(X_ROT), x1, l1);
+      plan.add(dst_y_tab_kernel(Y_SCALE2), y1, l2);
+      plan.add(dst_x_tab_kernel(X_SUBFROM), x2, l3);
+    }
+    plan_vort2flux(Lmax);
+  }
+
+  void launch_plan() {
+    for (const BatchOp& op : plan.ops) {
+      const void* tab = arena.p + op.off;
+      void* args[] = {(void*)&tab};
+      IB_HIP(hipLaunchKernel(op.fn, op.grid, dim3(256), args, op.lds, st));
+    }
+  }
+
+  void build_fused() {
+    plan_step();
+    arena.alloc(plan.bytes.size());
+    IB_HIP(hipMemcpy(arena.p, plan.bytes.data(), plan.bytes.size(), hipMemcpyHostToDevice));
+    if (direct) return;
+    hipGraph_t gr = nullptr;
+    IB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      launch_plan();
+    } catch (...) {
+      (void)hipStreamEndCapture(st, &gr);
+      if (gr) (void)hipGraphDestroy(gr);
+      throw;
+    }
+    IB_HIP(hipStreamEndCapture(st, &gr));
+    IB_HIP(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
+    IB_HIP(hipGraphDestroy(gr));
+  }
+
+  int init(int32_t nsim, const ib_params* params, const int32_t* nbodies, const ib_body* bodies) {
+    if (nsim < 1 || nsim > 65535) {
+      err = "nsim must be in 1..65535";
+      return IB_E_INVAL;
+    }
+    long b0 = 0;
+    for (int i = 0; i < nsim; ++i) {
+      if (nbodies[i] < 0) {
+        err = "negative body count";
+        return IB_E_INVAL;
+      }
+      ib_handle* h = new ib_handle();
+      sims.push_back(h);
+      const int rc = h->init(&params[i], nbodies[i], bodies + b0);
+      if (rc != IB_OK) {
+        err = "simulation " + std::to_string(i) + ": " + h->err;
+        return rc;
+      }
+      b0 += nbodies[i];
+    }
+    IB_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    step.alloc(nsim);
+    IB_HIP(hipHostMalloc(&hstep, sizeof(int) * nsim, hipHostMallocDefault));
+    bool all = true;
+    for (ib_handle* h : sims) all = all && fusable(h);
+    mode = all ? 1 : 2;
+    direct = params[0].launch_mode == 1;
+    if (mode == 1) build_fused();
+    return IB_OK;
+  }
+
+  int advance(int32_t* itimes, ib_step_info* infos) {
+    const int n = (int)sims.size();
+    std::vector<int> rc(n, IB_OK);
+    if (mode == 1) {
+      for (int i = 0; i < n; ++i) hstep[i] = sims[i]->itime + 1;
+      IB_HIP(hipMemcpyAsync(step.p, hstep, sizeof(int) * n, hipMemcpyHostToDevice, st));
+      if (direct) launch_plan();
+      else IB_HIP(hipGraphLaunch(ge, st));
+      IB_HIP(hipStreamSynchronize(st));
+      for (ib_handle* h : sims) {
+        h->itime += 1;
+        h->ksp_iter = 0;
+        h->fsi_iter = 0;
+        h->err_fsi = 0.0;
+      }
+    } else {
+      std::vector<std::string> ex(n);
+      std::vector<std::thread> pool;
+      pool.reserve(n);
+      for (int i = 0; i < n; ++i)
+        pool.emplace_back([&, i]() {
+          try {
+            rc[i] = sims[i]->advance();
+          } catch (const std::exception& e) {
+            ex[i] = e.what();
+            rc[i] = IB_E_HIP;
+          }
+        });
+      for (auto& t : pool) t.join();
+      for (int i = 0; i < n; ++i)
+        if (!ex[i].empty()) sims[i]->err = ex[i];
+    }
+    int first = IB_OK;
+    for (int i = 0; i < n; ++i) {
+      ib_handle* h = sims[i];
+      if (itimes) itimes[i] = h->itime;
+      if (infos) {
+        infos[i].itime = h->itime;
+        infos[i].ksp_iter = h->ksp_iter;
+        infos[i].fsi_iter = h->fsi_iter;
+        infos[i].status = rc[i];
+        infos[i].err_fsi = h->err_fsi;
+      }
+      if (rc[i] != IB_OK && first == IB_OK) {
+        first = rc[i];
+        err = "simulation " + std::to_string(i) + ": " + h->err;
+      }
+    }
+    return first;
+  }
+};
+
+extern "C" {
+
+int ib_batch_create(int32_t nsim, const ib_params* params, const int32_t* nbodies, const ib_body* bodies,
+                    ib_batch** out) {
+  if (!params || !nbodies || !out) return IB_E_INVAL;
+  *out = nullptr;
+  ib_batch* b = nullptr;
+  try {
+    b = new ib_batch();
+    const int rc = b->init(nsim, params, nbodies, bodies);
+    if (rc != IB_OK) {
+      g_batch_create_err = b->err;
+      delete b;
+      return rc;
+    }
+  } catch (const std::exception& e) {
+    g_batch_create_err = e.what();
+    delete b;
+    return IB_E_HIP;
+  }
+  *out = b;
+  return IB_OK;
+}
+
+int ib_batch_destroy(ib_batch* b) {
+  delete b;
+  return IB_OK;
+}
+
+int ib_batch_advance(ib_batch* b, int32_t* itimes, ib_step_info* infos) {
+  if (!b) return IB_E_INVAL;
+  try {
+    return b->advance(itimes, infos);
+  } catch (const std::exception& e) {
+    b->err = e.what();
+    return IB_E_HIP;
+  }
+}
+
+int ib_batch_handle(ib_batch* b, int32_t i, ib_handle** h) {
+  if (!b || !h || i < 0 || i >= (int)b->sims.size()) return IB_E_INVAL;
+  *h = b->sims[i];
+  return IB_OK;
+}
+
+int ib_batch_info(const ib_batch* b, int32_t* nsim, int32_t* mode) {
+  if (!b) return IB_E_INVAL;
+  if (nsim) *nsim = (int32_t)b->sims.size();
+  if (mode) *mode = b->mode;
+  return IB_OK;
+}
+
+int ib_batch_last_error(const ib_batch* b, char* buf, int32_t len) {
+  const std::string& e = b ? b->err : g_batch_create_err;
+  if (buf && len > 0) std::snprintf(buf, (size_t)len, "%s", e.c_str());
+  return (int)e.size();
+}
+
+}  // extern "C"

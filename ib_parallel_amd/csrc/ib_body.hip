@@ -156,26 +156,39 @@ void RegCSR::build(hipStream_t st) {
 }
 
 // reg: one thread per touched face, contributions summed in tap order
-__global__ void k_reg(Geom g, int ntaps, const int* __restrict__ nruns, const int* __restrict__ uniq,
-                      const int* __restrict__ off, const int* __restrict__ cnt, const int* __restrict__ taps,
-                      const double* __restrict__ w, const double* __restrict__ f, double* __restrict__ hx,
-                      double* __restrict__ hy) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= ntaps || r >= *nruns) return;
-  const int o = off[r], c = cnt[r], key = uniq[r];
-  double acc = 0.0;
-  for (int q = o; q < o + c; ++q) {
-    const int tp = taps[q];
-    acc = acc + w[tp] * f[tp / kTaps];
+struct RegOp {
+  using Args = RegArgs;
+  static __device__ __forceinline__ void tile(const RegArgs& a, int bx, int, int) {
+    const int r = bx * 256 + threadIdx.x;
+    if (r >= a.ntaps || r >= *a.nruns) return;
+    const int o = a.off[r], c = a.cnt[r], key = a.uniq[r];
+    const int* __restrict__ taps = a.taps;
+    const double* __restrict__ w = a.w;
+    const double* __restrict__ f = a.f;
+    double acc = 0.0;
+    for (int q = o; q < o + c; ++q) {
+      const int tp = taps[q];
+      acc = acc + w[tp] * f[tp / kTaps];
+    }
+    if (key < a.g.Fx) a.hx[key] = acc;
+    else a.hy[key - a.g.Fx] = acc;
   }
-  if (key < g.Fx) hx[key] = acc;
-  else hy[key - g.Fx] = acc;
+};
+
+__global__ void __launch_bounds__(256) k_reg(RegArgs a) { RegOp::tile(a, blockIdx.x, 0, 0); }
+
+TabEntry<RegArgs> reg_entry(const Geom& g, const RegCSR& c, const double* w, const double* f, double* hx, double* hy) {
+  TabEntry<RegArgs> e{};
+  e.a = RegArgs{g, c.ntaps, c.nruns, c.uniq, c.off, c.cnt, c.tap_sorted, w, f, hx, hy};
+  e.nbx = ceil_div(std::max(c.ntaps, 1), 256);
+  e.nby = 1;
+  return e;
 }
 
 void reg(hipStream_t st, const Geom& g, const RegCSR& c, const double* w, const double* f, double* hx, double* hy) {
   if (c.ntaps <= 0) return;
-  hipLaunchKernelGGL(k_reg, dim3(ceil_div(c.ntaps, 256)), dim3(256), 0, st, g, c.ntaps, c.nruns, c.uniq, c.off, c.cnt,
-                     c.tap_sorted, w, f, hx, hy);
+  const TabEntry<RegArgs> e = reg_entry(g, c, w, f, hx, hy);
+  hipLaunchKernelGGL(k_reg, dim3(e.nbx), dim3(256), 0, st, e.a);
   IB_LAUNCH_CHECK();
 }
 
@@ -187,26 +200,39 @@ __device__ __forceinline__ double taps_in_order(double acc, double prod) {
   return acc;
 }
 
-__global__ void k_regT(Geom g, int nf, const double* __restrict__ w, const int* __restrict__ key,
-                       const double* __restrict__ hx, const double* __restrict__ hy, double shift_x,
-                       const double* __restrict__ base, double* __restrict__ out) {
-  const int k = (blockIdx.x * blockDim.x + threadIdx.x) / warpSize;
-  const int lane = threadIdx.x % warpSize;
-  if (k >= nf) return;
-  double prod = 0.0;
-  if (lane < kTaps) {
-    const int kk = key[k * kTaps + lane];
-    const double v = kk < g.Fx ? hx[kk] + shift_x : hy[kk - g.Fx];
-    prod = w[k * kTaps + lane] * v;
+struct RegTOp {
+  using Args = RegTArgs;
+  static __device__ __forceinline__ void tile(const RegTArgs& a, int bx, int, int) {
+    const int k = (bx * 256 + threadIdx.x) / warpSize;
+    const int lane = threadIdx.x % warpSize;
+    if (k >= a.nf) return;
+    double prod = 0.0;
+    if (lane < kTaps) {
+      const int kk = a.key[k * kTaps + lane];
+      const double v = kk < a.g.Fx ? a.hx[kk] + a.shift_x : a.hy[kk - a.g.Fx];
+      prod = a.w[k * kTaps + lane] * v;
+    }
+    const double acc = taps_in_order(a.base ? a.base[k] : 0.0, prod);
+    if (lane == 0) a.out[k] = acc;
   }
-  const double acc = taps_in_order(base ? base[k] : 0.0, prod);
-  if (lane == 0) out[k] = acc;
+};
+
+__global__ void __launch_bounds__(256) k_regT(RegTArgs a) { RegTOp::tile(a, blockIdx.x, 0, 0); }
+
+TabEntry<RegTArgs> regT_entry(const Geom& g, int nf, const double* w, const int* key, const double* hx,
+                              const double* hy, double shift_x, const double* base, double* out) {
+  TabEntry<RegTArgs> e{};
+  e.a = RegTArgs{g, nf, w, key, hx, hy, shift_x, base, out};
+  e.nbx = ceil_div(std::max(nf, 1), 4);
+  e.nby = 1;
+  return e;
 }
 
 void regT(hipStream_t st, const Geom& g, int nf, const double* w, const int* key, const double* hx, const double* hy,
           double shift_x, const double* base, double* out) {
   if (nf <= 0) return;
-  hipLaunchKernelGGL(k_regT, dim3(ceil_div(nf, 4)), dim3(256), 0, st, g, nf, w, key, hx, hy, shift_x, base, out);
+  const TabEntry<RegTArgs> e = regT_entry(g, nf, w, key, hx, hy, shift_x, base, out);
+  hipLaunchKernelGGL(k_regT, dim3(e.nbx), dim3(256), 0, st, e.a);
   IB_LAUNCH_CHECK();
 }
 
@@ -328,29 +354,39 @@ void schur_from_green(hipStream_t st, int nf, int nF, const double* w, const int
 }
 
 // redistribute2 on the touched faces: finter = 1/wght * frc where wght > 1e-10
-__global__ void k_redist(Geom g, int ntaps, const int* __restrict__ nruns, const int* __restrict__ uniq,
-                         const int* __restrict__ off, const int* __restrict__ cnt, const int* __restrict__ taps,
-                         const double* __restrict__ w, const double* __restrict__ f, double* __restrict__ fx,
-                         double* __restrict__ fy) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= ntaps || r >= *nruns) return;
-  const int o = off[r], c = cnt[r], key = uniq[r];
-  double ws = 0.0, fs = 0.0;
-  for (int q = o; q < o + c; ++q) {
-    const int tp = taps[q];
-    ws = ws + w[tp] * 1.0;
-    fs = fs + w[tp] * f[tp / kTaps];
+struct RedistOp {
+  using Args = RegArgs;
+  static __device__ __forceinline__ void tile(const RegArgs& a, int bx, int, int) {
+    const int r = bx * 256 + threadIdx.x;
+    if (r >= a.ntaps || r >= *a.nruns) return;
+    const int o = a.off[r], c = a.cnt[r], key = a.uniq[r];
+    const int* __restrict__ taps = a.taps;
+    const double* __restrict__ w = a.w;
+    const double* __restrict__ f = a.f;
+    double ws = 0.0, fs = 0.0;
+    for (int q = o; q < o + c; ++q) {
+      const int tp = taps[q];
+      ws = ws + w[tp] * 1.0;
+      fs = fs + w[tp] * f[tp / kTaps];
+    }
+    const double v = ws > 1e-10 ? 1.0 / ws * fs : 0.0;
+    if (key < a.g.Fx) a.hx[key] = v;
+    else a.hy[key - a.g.Fx] = v;
   }
-  const double v = ws > 1e-10 ? 1.0 / ws * fs : 0.0;
-  if (key < g.Fx) fx[key] = v;
-  else fy[key - g.Fx] = v;
+};
+
+__global__ void __launch_bounds__(256) k_redist(RegArgs a) { RedistOp::tile(a, blockIdx.x, 0, 0); }
+
+TabEntry<RegArgs> redist_entry(const Geom& g, const RegCSR& c, const double* w, const double* f, double* fx_out,
+                               double* fy_out) {
+  return reg_entry(g, c, w, f, fx_out, fy_out);
 }
 
 void redist_faces(hipStream_t st, const Geom& g, const RegCSR& c, const double* w, const double* f, double* fx_out,
                   double* fy_out) {
   if (c.ntaps <= 0) return;
-  hipLaunchKernelGGL(k_redist, dim3(ceil_div(c.ntaps, 256)), dim3(256), 0, st, g, c.ntaps, c.nruns, c.uniq, c.off,
-                     c.cnt, c.tap_sorted, w, f, fx_out, fy_out);
+  const TabEntry<RegArgs> e = redist_entry(g, c, w, f, fx_out, fy_out);
+  hipLaunchKernelGGL(k_redist, dim3(e.nbx), dim3(256), 0, st, e.a);
   IB_LAUNCH_CHECK();
 }
 
@@ -402,11 +438,25 @@ void body_btimes(hipStream_t st, int nt, const TorsBody* tb, const double* theta
 
 // compute_rhs (:342-398) for torsional entries + regT of q + q0 (rhsf of the
 // FSI loop, src/timestepper.F90:322-358); one wavefront per force entry
-__global__ void k_fsi_rhs(Geom g, int nf, const int* __restrict__ tors_of, const TorsBody* __restrict__ tbs, int nt,
-                          const double* __restrict__ th, double dt, double delta, const double* __restrict__ w,
-                          const int* __restrict__ key, const double* __restrict__ qx, const double* __restrict__ qy,
-                          double q0x, double* __restrict__ rhsf) {
-  const int k = (blockIdx.x * blockDim.x + threadIdx.x) / warpSize;
+struct FsiRhsOp {
+  using Args = FsiRhsArgs;
+  static __device__ __forceinline__ void tile(const FsiRhsArgs& a, int bx, int by, int);
+};
+
+__device__ __forceinline__ void FsiRhsOp::tile(const FsiRhsArgs& a, int bx, int, int) {
+  const Geom& g = a.g;
+  const int nf = a.nf, nt = a.nt;
+  const int* __restrict__ tors_of = a.tors_of;
+  const TorsBody* __restrict__ tbs = a.tbs;
+  const double* __restrict__ th = a.th;
+  const double dt = a.dt, delta = a.delta;
+  const double* __restrict__ w = a.w;
+  const int* __restrict__ key = a.key;
+  const double* __restrict__ qx = a.qx;
+  const double* __restrict__ qy = a.qy;
+  const double q0x = a.q0x;
+  double* __restrict__ rhsf = a.rhsf;
+  const int k = (bx * 256 + threadIdx.x) / warpSize;
   const int lane = threadIdx.x % warpSize;
   if (k >= nf) return;
   double v = 0.0;
@@ -435,13 +485,36 @@ __global__ void k_fsi_rhs(Geom g, int nf, const int* __restrict__ tors_of, const
   if (lane == 0) rhsf[k] = v;
 }
 
+__global__ void __launch_bounds__(256) k_fsi_rhs(FsiRhsArgs a) { FsiRhsOp::tile(a, blockIdx.x, 0, 0); }
+
+TabEntry<FsiRhsArgs> fsi_rhs_entry(const Geom& g, int nf, const int* tors_of_entry, const TorsBody* tb, int nt,
+                                   const double* th, double dt, double delta, const double* w, const int* key,
+                                   const double* qx, const double* qy, double q0x, double* rhsf) {
+  TabEntry<FsiRhsArgs> e{};
+  e.a = FsiRhsArgs{g, nf, tors_of_entry, tb, nt, th, dt, delta, w, key, qx, qy, q0x, rhsf};
+  e.nbx = ceil_div(std::max(nf, 1), 4);
+  e.nby = 1;
+  return e;
+}
+
 void fsi_rhs(hipStream_t st, const Geom& g, int nf, const int* tors_of_entry, const TorsBody* tb, int nt,
              const double* th, double dt, double delta, const double* w, const int* key, const double* qx,
              const double* qy, double q0x, double* rhsf) {
   if (nf <= 0) return;
-  hipLaunchKernelGGL(k_fsi_rhs, dim3(ceil_div(nf, 4)), dim3(256), 0, st, g, nf, tors_of_entry, tb, nt, th, dt, delta,
-                     w, key, qx, qy, q0x, rhsf);
+  const TabEntry<FsiRhsArgs> e =
+      fsi_rhs_entry(g, nf, tors_of_entry, tb, nt, th, dt, delta, w, key, qx, qy, q0x, rhsf);
+  hipLaunchKernelGGL(k_fsi_rhs, dim3(e.nbx), dim3(256), 0, st, e.a);
   IB_LAUNCH_CHECK();
+}
+
+const void* body_tab_kernel(int kind) {
+  switch (kind) {
+    case TK_REG: return (const void*)k_tab<RegOp>;
+    case TK_REDIST: return (const void*)k_tab<RedistOp>;
+    case TK_REGT: return (const void*)k_tab<RegTOp>;
+    case TK_FSI_RHS: return (const void*)k_tab<FsiRhsOp>;
+    default: throw HipError("no body table kernel of this kind");
+  }
 }
 
 // compute_dx (:402-505) + update_xb (:600-620); one workgroup per body

@@ -60,4 +60,14 @@ struct TorsBody {
 
 __host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
+// One simulation's entry in a launch over several independent simulations
+// (the batched step): its launch arguments and the extent of its own grid.
+// Such a launch runs a grid of (max nbx, max nby, entries) workgroups; the
+// workgroups past their entry's extent return at once.
+template <class A>
+struct TabEntry {
+  A a;
+  int nbx, nby;
+};
+
 }  // namespace ibg

@@ -34,4 +34,17 @@ __device__ __forceinline__ double curl_at(const Geom& g, const double* s, const 
   return ext(g, s, bc, i, j) - ext(g, s, bc, i + 1, j);
 }
 
+// A kernel family OP (OP::Args, OP::tile(args, bx, by, nbx)) launched over
+// a device table of simulations: blockIdx.z selects the entry.
+// The table is read through the constant address space (it does not change
+// while a launch runs), so argument loads stay scalar like kernel arguments.
+template <class OP>
+__global__ void __launch_bounds__(256) k_tab(const TabEntry<typename OP::Args>* __restrict__ t) {
+  using E = TabEntry<typename OP::Args>;
+  const __attribute__((address_space(4))) E* ct = (const __attribute__((address_space(4))) E*)t;
+  const E& e = *(const E*)(ct + blockIdx.z);
+  if ((int)blockIdx.x >= e.nbx || (int)blockIdx.y >= e.nby) return;
+  OP::tile(e.a, blockIdx.x, blockIdx.y, e.nbx);
+}
+
 }  // namespace ibg

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <vector>
 
+#include "ib_device.h"
 #include "ib_kernels.h"
 
 namespace ibg {
@@ -121,32 +122,71 @@ __device__ __forceinline__ void stage_fixed(const double2* __restrict__ in, doub
   }
 }
 
-__device__ void stage_generic(const double2* __restrict__ in, double2* __restrict__ out, int nf, int L, int Ns,
-                              int R, const double2* __restrict__ W) {
+// A stage of any other radix R (a prime > 5): the inputs of sub-transform j
+// first take their inter-stage twiddles in place, then each thread forms a
+// tile of 2 sub-transforms x 4 outputs as a direct length-R DFT, so every
+// input and every DFT coefficient read from LDS feeds several products.
+__device__ void stage_generic(double2* __restrict__ in, double2* __restrict__ out, int nf, int L, int Ns, int R,
+                              const double2* __restrict__ W) {
   const int LR = L / R;
   const int tws = LR / Ns;
-  const int total = nf * L;
-  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-    const int s = idx / L;
-    const int rem = idx - s * L;
-    const int q = rem / LR;
-    const int j = rem - q * LR;
-    const int k = j % Ns;
-    const double2* ip = in + s * L;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int r = 0; r < R; ++r) {
-      int tw = k * r * tws + ((r * q) % R) * LR;
-      if (tw >= L) tw -= L;
-      acc = cadd(acc, cmul(ip[j + r * LR], W[tw]));
+  if (Ns > 1) {
+    for (int idx = threadIdx.x; idx < nf * L; idx += blockDim.x) {
+      const int rem = idx % L;
+      const int r = rem / LR, j = rem - r * LR;
+      const int kr = (j % Ns) * r;
+      if (kr) in[idx] = cmul(in[idx], W[kr * tws]);
     }
-    out[s * L + (j - k) * R + k + q * Ns] = acc;
+    __syncthreads();
+  }
+  const int nj = (LR + 1) / 2, nq = (R + 3) / 4;
+  const int ntile = nf * nj * nq;
+  for (int t = threadIdx.x; t < ntile; t += blockDim.x) {
+    const int s = t / (nj * nq);
+    const int u = t - s * (nj * nq);
+    const int qt = u / nj;
+    const int j0 = 2 * (u - qt * nj), q0 = 4 * qt;
+    const int j1 = min(j0 + 1, LR - 1);  // an odd LR repeats the last column (not stored)
+    const double2* __restrict__ x = in + s * L;
+    // coefficient of term r, output q: w_R^(rq) = W[((r q) mod R) LR], kept as a
+    // running index below L per output
+    int bq[4], dq[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      dq[c] = min(q0 + c, R - 1) * LR;
+      bq[c] = 0;
+    }
+    double2 acc0[4], acc1[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc0[c] = acc1[c] = make_double2(0.0, 0.0);
+    for (int r = 0; r < R; ++r) {
+      const double2 xa = x[j0 + r * LR], xb = x[j1 + r * LR];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double2 w = W[bq[c]];
+        acc0[c] = cadd(acc0[c], cmul(xa, w));
+        acc1[c] = cadd(acc1[c], cmul(xb, w));
+        bq[c] += dq[c];
+        if (bq[c] >= L) bq[c] -= L;
+      }
+    }
+    double2* __restrict__ o = out + s * L;
+    const int k0 = j0 % Ns, k1 = (j0 + 1) % Ns;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int q = q0 + c;
+      if (q < R) {
+        o[(j0 - k0) * R + k0 + q * Ns] = acc0[c];
+        if (j0 + 1 < LR) o[(j0 + 1 - k1) * R + k1 + q * Ns] = acc1[c];
+      }
+    }
   }
 }
 
 // Runs the plan on nf transforms held in a (scratch b) with the twiddle table
 // W staged in LDS; returns the buffer that holds the result.  Caller
 // synchronises before the call.
-__device__ double2* fft_run(double2* a, double2* b, int nf, const FFTPlan& pl, const double2* W) {
+__device__ __forceinline__ double2* fft_run(double2* a, double2* b, int nf, const FFTPlan& pl, const double2* W) {
   int Ns = 1;
   for (int st = 0; st < pl.nst; ++st) {
     const int R = pl.rad[st];
@@ -167,20 +207,6 @@ __device__ double2* fft_run(double2* a, double2* b, int nf, const FFTPlan& pl, c
   return a;
 }
 
-struct XArgs {
-  const double* in[kMaxBatch];
-  double* out[kMaxBatch];
-  const double* bc;
-  const double* hx;  // X_ROT: the node value is rot(hx, hy)
-  const double* hy;
-  long s_in, s_out, s_bc;  // strided: batch b at base[0] + b * stride
-  int strided;              // 0: batch b at in[b], out[b] (b < kMaxBatch)
-  int c0;                  // X_POINTROT: force entry (or face) of batch element 0
-  const double* w;         // X_POINTROT: weights, 49 per entry; null: unit face sources
-  const int* key;          // X_POINTROT: face keys, 49 per entry (or one per face)
-  Geom g;
-  int rp;
-};
 
 // X_POINTROT: rot(E^T e_k) at node (i, j), formed from the 49 taps of force
 // entry k (the x-face or y-face patch centred on (ix, iy)); with no weights,
@@ -200,19 +226,6 @@ struct PointPatch {
   }
 };
 
-struct YArgs {
-  const double* in[kMaxBatch];
-  const double* in2[kMaxBatch];
-  double* out[kMaxBatch];
-  long s_in, s_in2, s_out;  // strided: batch b at base[0] + b * stride
-  int strided;              // 0: batch b at in[b], out[b] (b < kMaxBatch)
-  const double* tab;
-  const double* tab2;
-  double div;
-  Geom g;
-  int cp;
-  int chunk;  // column-pair tiles per XCD (XCD-aware tile order)
-};
 
 // Stage the twiddle table in LDS (read by every stage of every transform).
 __device__ __forceinline__ void load_twiddles(double2* Wl, const FFTPlan& pl) {
@@ -221,21 +234,22 @@ __device__ __forceinline__ void load_twiddles(double2* Wl, const FFTPlan& pl) {
 
 // Row pass.  Rows j0 = 0..n-2 (node j = j0+2) of length N = m-1.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_dst_x(XArgs a, FFTPlan pl) {
+__device__ __forceinline__ void dst_x_tile(const XArgs& a, int bx, int by) {
   extern __shared__ double2 sm[];
+  const FFTPlan& pl = a.pl;
   const int L = pl.L, M = pl.M, N = pl.N, rp = a.rp;
   const int nrows = a.g.n - 1;
   double2* A = sm;
   double2* B = sm + rp * L;
   double2* Wl = sm + 2 * rp * L;
-  const int bsel = a.strided ? 0 : blockIdx.y;
-  const double* __restrict__ in = a.in[bsel] ? a.in[bsel] + (long)blockIdx.y * a.s_in : nullptr;
-  double* out = a.out[bsel] + (long)blockIdx.y * a.s_out;
-  const double* bcb = a.bc ? a.bc + (long)blockIdx.y * a.s_bc : nullptr;
-  const int pair0 = blockIdx.x * rp;
+  const int bsel = a.strided ? 0 : by;
+  const double* __restrict__ in = a.in[bsel] ? a.in[bsel] + (long)by * a.s_in : nullptr;
+  double* out = a.out[bsel] + (long)by * a.s_out;
+  const double* bcb = a.bc ? a.bc + (long)by * a.s_bc : nullptr;
+  const int pair0 = bx * rp;
   PointPatch pt{};
   if (MODE == X_POINTROT) {
-    const int k = a.c0 + blockIdx.y;
+    const int k = a.c0 + by;
     // centre tap (p = l = 0) of force entry k, or face k of a face list
     const int ck = a.w ? a.key[k * kTaps + kTaps / 2] : a.key[k];
     const Geom& g = a.g;
@@ -328,21 +342,22 @@ __global__ void __launch_bounds__(256) k_dst_x(XArgs a, FFTPlan pl) {
 
 // Column pass.  Columns i0 = 0..m-2 (stride m-1) of length N = n-1.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_dst_y(YArgs a, FFTPlan pl) {
+__device__ __forceinline__ void dst_y_tile(const YArgs& a, int bx, int by) {
   extern __shared__ double2 sm[];
+  const FFTPlan& pl = a.pl;
   const int L = pl.L, M = pl.M, N = pl.N, cp = a.cp;
   const int ncols = a.g.m - 1;
   const int nfft = (MODE == Y_PRED) ? 2 * cp : cp;
   double2* A = sm;
   double2* B = sm + nfft * L;
   double2* Wl = sm + 2 * nfft * L;
-  const int bsel = a.strided ? 0 : blockIdx.y;
-  const double* __restrict__ in = a.in[bsel] + (long)blockIdx.y * a.s_in;
-  const double* __restrict__ in2 = a.in2[bsel] ? a.in2[bsel] + (long)blockIdx.y * a.s_in2 : nullptr;
-  double* __restrict__ out = a.out[bsel] + (long)blockIdx.y * a.s_out;
+  const int bsel = a.strided ? 0 : by;
+  const double* __restrict__ in = a.in[bsel] + (long)by * a.s_in;
+  const double* __restrict__ in2 = a.in2[bsel] ? a.in2[bsel] + (long)by * a.s_in2 : nullptr;
+  double* __restrict__ out = a.out[bsel] + (long)by * a.s_out;
   // workgroups are dealt to the 8 XCDs round-robin: give each XCD a
   // contiguous band of column tiles so neighbouring tiles share its L2
-  const int tile = (blockIdx.x % 8) * a.chunk + blockIdx.x / 8;
+  const int tile = (bx % 8) * a.chunk + bx / 8;
   const int col0 = 2 * tile * cp;
   if (col0 >= ncols) return;
   const int w2 = 2 * cp;  // columns handled by this workgroup
@@ -416,6 +431,25 @@ __global__ void __launch_bounds__(256) k_dst_y(YArgs a, FFTPlan pl) {
   }
 }
 
+template <int MODE>
+__global__ void __launch_bounds__(256) k_dst_x(XArgs a) {
+  dst_x_tile<MODE>(a, blockIdx.x, blockIdx.y);
+}
+template <int MODE>
+__global__ void __launch_bounds__(256) k_dst_y(YArgs a) {
+  dst_y_tile<MODE>(a, blockIdx.x, blockIdx.y);
+}
+template <int MODE>
+struct DstXOp {
+  using Args = XArgs;
+  static __device__ __forceinline__ void tile(const XArgs& a, int bx, int by, int) { dst_x_tile<MODE>(a, bx, by); }
+};
+template <int MODE>
+struct DstYOp {
+  using Args = YArgs;
+  static __device__ __forceinline__ void tile(const YArgs& a, int bx, int by, int) { dst_y_tile<MODE>(a, bx, by); }
+};
+
 // ------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------
@@ -466,13 +500,34 @@ void dst_set_max_lds(size_t bytes) {
                        (const void*)k_dst_x<X_SUBFROM>, (const void*)k_dst_x<X_ROT>,
                        (const void*)k_dst_x<X_POINTROT>,
                        (const void*)k_dst_y<Y_PLAIN>,
-                       (const void*)k_dst_y<Y_SCALE2>, (const void*)k_dst_y<Y_PRED>};
+                       (const void*)k_dst_y<Y_SCALE2>, (const void*)k_dst_y<Y_PRED>,
+                       dst_x_tab_kernel(X_PLAIN), dst_x_tab_kernel(X_APPLYBC), dst_x_tab_kernel(X_SUBFROM),
+                       dst_x_tab_kernel(X_ROT), dst_y_tab_kernel(Y_SCALE2), dst_y_tab_kernel(Y_PRED)};
   for (const void* f : fns) IB_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
-void dst_x(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, const Geom& g, int rp,
-           size_t max_lds) {
-  XArgs a{};
+const void* dst_x_tab_kernel(int mode) {
+  switch (mode) {
+    case X_PLAIN: return (const void*)k_tab<DstXOp<X_PLAIN>>;
+    case X_APPLYBC: return (const void*)k_tab<DstXOp<X_APPLYBC>>;
+    case X_SUBFROM: return (const void*)k_tab<DstXOp<X_SUBFROM>>;
+    case X_ROT: return (const void*)k_tab<DstXOp<X_ROT>>;
+    default: throw HipError("DST row pass: no table kernel for this mode");
+  }
+}
+
+const void* dst_y_tab_kernel(int mode) {
+  switch (mode) {
+    case Y_SCALE2: return (const void*)k_tab<DstYOp<Y_SCALE2>>;
+    case Y_PRED: return (const void*)k_tab<DstYOp<Y_PRED>>;
+    default: throw HipError("DST column pass: no table kernel for this mode");
+  }
+}
+
+TabEntry<XArgs> dst_x_entry(const FFTPlan& pl, const DstLaunch& l, const Geom& g, int rp, size_t max_lds,
+                            size_t* lds) {
+  TabEntry<XArgs> e{};
+  XArgs& a = e.a;
   const bool strided = l.strided;
   if (!strided && l.batch > kMaxBatch) throw HipError("DST row pass: batch > 4 needs strides");
   if (!strided && (l.s_in || l.s_out || l.s_bc)) throw HipError("DST row pass: strides on an unstrided batch");
@@ -491,26 +546,21 @@ void dst_x(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, cons
   a.hx = l.hx;
   a.hy = l.hy;
   a.g = g;
+  a.pl = pl;
   while (rp > 1 && dst_lds_bytes(pl, rp) > max_lds) rp /= 2;
   a.rp = rp;
   const int npairs = ceil_div(g.n - 1, 2);
-  dim3 grid(ceil_div(npairs, rp), l.batch);
-  const size_t lds = dst_lds_bytes(pl, rp);
-  if (lds > max_lds) throw HipError("DST row pass needs more LDS than the device allows");
-  switch (mode) {
-    case X_PLAIN: hipLaunchKernelGGL(k_dst_x<X_PLAIN>, grid, dim3(256), lds, st, a, pl); break;
-    case X_APPLYBC: hipLaunchKernelGGL(k_dst_x<X_APPLYBC>, grid, dim3(256), lds, st, a, pl); break;
-    case X_SUBFROM: hipLaunchKernelGGL(k_dst_x<X_SUBFROM>, grid, dim3(256), lds, st, a, pl); break;
-    case X_ROT: hipLaunchKernelGGL(k_dst_x<X_ROT>, grid, dim3(256), lds, st, a, pl); break;
-    case X_POINTROT: hipLaunchKernelGGL(k_dst_x<X_POINTROT>, grid, dim3(256), lds, st, a, pl); break;
-    default: throw HipError("bad DST row mode");
-  }
-  IB_LAUNCH_CHECK();
+  e.nbx = ceil_div(npairs, rp);
+  e.nby = l.batch;
+  *lds = dst_lds_bytes(pl, rp);
+  if (*lds > max_lds) throw HipError("DST row pass needs more LDS than the device allows");
+  return e;
 }
 
-void dst_y(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, const Geom& g, int cp,
-           size_t max_lds) {
-  YArgs a{};
+TabEntry<YArgs> dst_y_entry(const FFTPlan& pl, int mode, const DstLaunch& l, const Geom& g, int cp,
+                            size_t max_lds, size_t* lds) {
+  TabEntry<YArgs> e{};
+  YArgs& a = e.a;
   const bool strided = l.strided;
   if (!strided && l.batch > kMaxBatch) throw HipError("DST column pass: batch > 4 needs strides");
   if (!strided && (l.s_in || l.s_in2 || l.s_out)) throw HipError("DST column pass: strides on an unstrided batch");
@@ -527,19 +577,45 @@ void dst_y(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, cons
   a.tab2 = l.tab2;
   a.div = l.div;
   a.g = g;
+  a.pl = pl;
   const int mult = (mode == Y_PRED) ? 2 : 1;
   while (cp > 1 && dst_lds_bytes(pl, mult * cp) > max_lds) cp /= 2;
   a.cp = cp;
   const int npairs = ceil_div(g.m - 1, 2);
   const int ntiles = ceil_div(npairs, cp);
   a.chunk = ceil_div(ntiles, 8);
-  dim3 grid(8 * a.chunk, l.batch);
-  const size_t lds = dst_lds_bytes(pl, mult * cp);
-  if (lds > max_lds) throw HipError("DST column pass needs more LDS than the device allows");
+  e.nbx = 8 * a.chunk;
+  e.nby = l.batch;
+  *lds = dst_lds_bytes(pl, mult * cp);
+  if (*lds > max_lds) throw HipError("DST column pass needs more LDS than the device allows");
+  return e;
+}
+
+void dst_x(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, const Geom& g, int rp,
+           size_t max_lds) {
+  size_t lds = 0;
+  const TabEntry<XArgs> e = dst_x_entry(pl, l, g, rp, max_lds, &lds);
+  const dim3 grid(e.nbx, e.nby);
   switch (mode) {
-    case Y_PLAIN: hipLaunchKernelGGL(k_dst_y<Y_PLAIN>, grid, dim3(256), lds, st, a, pl); break;
-    case Y_SCALE2: hipLaunchKernelGGL(k_dst_y<Y_SCALE2>, grid, dim3(256), lds, st, a, pl); break;
-    case Y_PRED: hipLaunchKernelGGL(k_dst_y<Y_PRED>, grid, dim3(256), lds, st, a, pl); break;
+    case X_PLAIN: hipLaunchKernelGGL(k_dst_x<X_PLAIN>, grid, dim3(256), lds, st, e.a); break;
+    case X_APPLYBC: hipLaunchKernelGGL(k_dst_x<X_APPLYBC>, grid, dim3(256), lds, st, e.a); break;
+    case X_SUBFROM: hipLaunchKernelGGL(k_dst_x<X_SUBFROM>, grid, dim3(256), lds, st, e.a); break;
+    case X_ROT: hipLaunchKernelGGL(k_dst_x<X_ROT>, grid, dim3(256), lds, st, e.a); break;
+    case X_POINTROT: hipLaunchKernelGGL(k_dst_x<X_POINTROT>, grid, dim3(256), lds, st, e.a); break;
+    default: throw HipError("bad DST row mode");
+  }
+  IB_LAUNCH_CHECK();
+}
+
+void dst_y(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, const Geom& g, int cp,
+           size_t max_lds) {
+  size_t lds = 0;
+  const TabEntry<YArgs> e = dst_y_entry(pl, mode, l, g, cp, max_lds, &lds);
+  const dim3 grid(e.nbx, e.nby);
+  switch (mode) {
+    case Y_PLAIN: hipLaunchKernelGGL(k_dst_y<Y_PLAIN>, grid, dim3(256), lds, st, e.a); break;
+    case Y_SCALE2: hipLaunchKernelGGL(k_dst_y<Y_SCALE2>, grid, dim3(256), lds, st, e.a); break;
+    case Y_PRED: hipLaunchKernelGGL(k_dst_y<Y_PRED>, grid, dim3(256), lds, st, e.a); break;
     default: throw HipError("bad DST column mode");
   }
   IB_LAUNCH_CHECK();

@@ -12,11 +12,16 @@
 
 namespace ibg {
 
-__global__ void k_fill(double* p, long n, double v) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long stride = (long)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) p[i] = v;
-}
+struct FillOp {
+  using Args = FillArgs;
+  static __device__ __forceinline__ void tile(const FillArgs& a, int bx, int, int nbx) {
+    long i = (long)bx * 256 + threadIdx.x;
+    const long stride = (long)nbx * 256;
+    for (; i < a.n; i += stride) a.p[i] = a.v;
+  }
+};
+
+__global__ void __launch_bounds__(256) k_fill(FillArgs a) { FillOp::tile(a, blockIdx.x, 0, gridDim.x); }
 
 __global__ void k_copy(double* d, const double* s, long n) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -36,9 +41,18 @@ static int grid_for(long n) {
   return (int)(b < 1 ? 1 : b);
 }
 
+TabEntry<FillArgs> fill_entry(double* p, long n, double v) {
+  TabEntry<FillArgs> e{};
+  e.a = FillArgs{p, n, v};
+  e.nbx = grid_for(n);
+  e.nby = 1;
+  return e;
+}
+
 void fill(hipStream_t st, double* p, long n, double v) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, st, p, n, v);
+  const TabEntry<FillArgs> e = fill_entry(p, n, v);
+  hipLaunchKernelGGL(k_fill, dim3(e.nbx), dim3(256), 0, st, e.a);
   IB_LAUNCH_CHECK();
 }
 
@@ -68,11 +82,17 @@ void rot(hipStream_t st, const Geom& g, const double* hx, const double* hy, doub
 }
 
 // coarsify: src/operators_fluid.F90:381-419 (fine odd nodes I,J = 3..m-1)
-__global__ void k_coarsify(Geom g, const double* __restrict__ wf, double* __restrict__ wc, long stride) {
+struct CoarsenOp {
+  using Args = CoarsenArgs;
+  static __device__ __forceinline__ void tile(const CoarsenArgs& a, int bx, int by, int);
+};
+
+__device__ __forceinline__ void CoarsenOp::tile(const CoarsenArgs& a, int bx, int by, int) {
+  const Geom& g = a.g;
   const int nx = (g.m - 2) / 2, ny = (g.n - 2) / 2;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  wf += blockIdx.y * stride;
-  wc += blockIdx.y * stride;
+  const int q = bx * 256 + threadIdx.x;
+  const double* __restrict__ wf = a.wf + by * a.stride;
+  double* __restrict__ wc = a.wc + by * a.stride;
   if (q >= nx * ny) return;
   const int I = 3 + 2 * (q % nx), J = 3 + 2 * (q / nx);
   const int ic = g.m / 4 + (I + 1) / 2, jc = g.n / 4 + (J + 1) / 2;
@@ -83,9 +103,19 @@ __global__ void k_coarsify(Geom g, const double* __restrict__ wf, double* __rest
   wc[ND(g, ic, jc)] = v;
 }
 
+__global__ void __launch_bounds__(256) k_coarsify(CoarsenArgs a) { CoarsenOp::tile(a, blockIdx.x, blockIdx.y, 0); }
+
+TabEntry<CoarsenArgs> coarsify_entry(const Geom& g, const double* fine, double* coarse, int batch, long stride) {
+  TabEntry<CoarsenArgs> e{};
+  e.a = CoarsenArgs{g, fine, coarse, stride};
+  e.nbx = ceil_div(((g.m - 2) / 2) * ((g.n - 2) / 2), 256);
+  e.nby = batch;
+  return e;
+}
+
 void coarsify(hipStream_t st, const Geom& g, const double* fine, double* coarse, int batch, long stride) {
-  const int nt = ((g.m - 2) / 2) * ((g.n - 2) / 2);
-  hipLaunchKernelGGL(k_coarsify, dim3(ceil_div(nt, 256), batch), dim3(256), 0, st, g, fine, coarse, stride);
+  const TabEntry<CoarsenArgs> e = coarsify_entry(g, fine, coarse, batch, stride);
+  hipLaunchKernelGGL(k_coarsify, dim3(e.nbx, e.nby), dim3(256), 0, st, e.a);
   IB_LAUNCH_CHECK();
 }
 
@@ -112,27 +142,50 @@ __device__ __forceinline__ double getbc_slot(const Geom& g, const double* sc, in
   return v * fac;
 }
 
-__global__ void k_get_bc(Geom g, const double* __restrict__ src, long sstride, double* __restrict__ bc, long bstride,
-                         double fac) {
-  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= g.nbc) return;
-  const int l = blockIdx.y;
-  bc[l * bstride + slot] = getbc_slot(g, src + l * sstride, slot, fac);
+struct GetBcOp {
+  using Args = GetBcArgs;
+  static __device__ __forceinline__ void tile(const GetBcArgs& a, int bx, int by, int) {
+    const int slot = bx * 256 + threadIdx.x;
+    if (slot >= a.g.nbc) return;
+    const int l = by;
+    a.bc[l * a.bstride + slot] = getbc_slot(a.g, a.src + l * a.sstride, slot, a.fac);
+  }
+};
+
+__global__ void __launch_bounds__(256) k_get_bc(GetBcArgs a) { GetBcOp::tile(a, blockIdx.x, blockIdx.y, 0); }
+
+TabEntry<GetBcArgs> get_bc_entry(const Geom& g, const double* src, long src_stride, double* bc, long bc_stride,
+                                 int nlev, double fac) {
+  TabEntry<GetBcArgs> e{};
+  e.a = GetBcArgs{g, src, src_stride, bc, bc_stride, fac};
+  e.nbx = ceil_div(g.nbc, 256);
+  e.nby = nlev;
+  return e;
 }
 
 void get_bc(hipStream_t st, const Geom& g, const double* src, long src_stride, double* bc, long bc_stride, int nlev,
             double fac) {
   if (nlev <= 0) return;
-  hipLaunchKernelGGL(k_get_bc, dim3(ceil_div(g.nbc, 256), nlev), dim3(256), 0, st, g, src, src_stride, bc, bc_stride,
-                     fac);
+  const TabEntry<GetBcArgs> e = get_bc_entry(g, src, src_stride, bc, bc_stride, nlev, fac);
+  hipLaunchKernelGGL(k_get_bc, dim3(e.nbx, e.nby), dim3(256), 0, st, e.a);
   IB_LAUNCH_CHECK();
 }
 
 // curl_int + curl_bc (src/operators_fluid.F90:423-471) and, in the same
 // launch, get_bc(s, 1) for the next finer level.
-__global__ void k_curl_getbc(Geom g, const double* __restrict__ s, const double* __restrict__ sbc,
-                             double* __restrict__ qx, double* __restrict__ qy, double* __restrict__ bc_out) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+struct CurlOp {
+  using Args = CurlArgs;
+  static __device__ __forceinline__ void tile(const CurlArgs& a, int bx, int by, int);
+};
+
+__device__ __forceinline__ void CurlOp::tile(const CurlArgs& a, int bx, int, int) {
+  const Geom& g = a.g;
+  const double* __restrict__ s = a.s;
+  const double* __restrict__ sbc = a.sbc;
+  double* __restrict__ qx = a.qx;
+  double* __restrict__ qy = a.qy;
+  double* __restrict__ bc_out = a.bc_out;
+  const int q = bx * 256 + threadIdx.x;
   if (q < g.Fx) {
     const int i = q % (g.m + 1) + 1, j = q / (g.m + 1) + 1;
     qx[q] = ext(g, s, sbc, i, j + 1) - ext(g, s, sbc, i, j);
@@ -146,30 +199,27 @@ __global__ void k_curl_getbc(Geom g, const double* __restrict__ s, const double*
   }
 }
 
+__global__ void __launch_bounds__(256) k_curl_getbc(CurlArgs a) { CurlOp::tile(a, blockIdx.x, 0, 0); }
+
+TabEntry<CurlArgs> curl_getbc_entry(const Geom& g, const double* s, const double* sbc, double* qx, double* qy,
+                                    double* bc_out) {
+  TabEntry<CurlArgs> e{};
+  e.a = CurlArgs{g, s, sbc, qx, qy, bc_out};
+  e.nbx = ceil_div(g.Fx + g.Fy + (bc_out ? g.nbc : 0), 256);
+  e.nby = 1;
+  return e;
+}
+
 void curl_getbc(hipStream_t st, const Geom& g, const double* s, const double* sbc, double* qx, double* qy,
                 double* bc_out) {
-  const int nt = g.Fx + g.Fy + (bc_out ? g.nbc : 0);
-  hipLaunchKernelGGL(k_curl_getbc, dim3(ceil_div(nt, 256)), dim3(256), 0, st, g, s, sbc, qx, qy, bc_out);
+  const TabEntry<CurlArgs> e = curl_getbc_entry(g, s, sbc, qx, qy, bc_out);
+  hipLaunchKernelGGL(k_curl_getbc, dim3(e.nbx), dim3(256), 0, st, e.a);
   IB_LAUNCH_CHECK();
 }
 
 // nonlinear (src/operators_fluid.F90:553-587) + rot (:203-233) + apply_bc of
 // lastbc and omegabc with vfac (src/timestepper.F90:184-189) + the AB2/Euler
 // right-hand side of fluid_predictor (src/operators_fluid.F90:657-660).
-struct NLArgs {
-  Geom g;
-  const double* w;
-  const double* qx;
-  const double* qy;
-  double q0x;
-  const double* lastbc;
-  const double* omegabc;
-  double vfac, con1, con2;
-  int first;
-  double* rhs_old;
-  double* r;
-  double* raw;
-};
 
 __device__ __forceinline__ double QX(const NLArgs& a, int i, int j) {
   return (j < 1 || j > a.g.n) ? 0.0 : a.qx[FXI(a.g, i, j)] + a.q0x;
@@ -187,9 +237,14 @@ __device__ __forceinline__ double nly(const NLArgs& a, int i, int j) {
   return -0.5 * (uavg(a, i + 1, j) * W_(a, i + 1, j) + uavg(a, i, j) * W_(a, i, j));
 }
 
-__global__ void k_nl_rhs(NLArgs a) {
+struct NLOp {
+  using Args = NLArgs;
+  static __device__ __forceinline__ void tile(const NLArgs& a, int bx, int by, int);
+};
+
+__device__ __forceinline__ void NLOp::tile(const NLArgs& a, int bx, int, int) {
   const Geom& g = a.g;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = bx * 256 + threadIdx.x;
   if (q >= g.P) return;
   const int i = q % (g.m - 1) + 2, j = q / (g.m - 1) + 2;
   const double rhs = nly(a, i, j) - nly(a, i - 1, j) - nlx(a, i, j) + nlx(a, i, j - 1);
@@ -205,17 +260,41 @@ __global__ void k_nl_rhs(NLArgs a) {
     if (j == 2) bc = bc + a.vfac * v[g.bottom + i - 1];
     if (j == g.n) bc = bc + a.vfac * v[g.top + i - 1];
   }
-  const double ro = a.first ? rhs : a.rhs_old[q];
+  const int first = a.step ? (*a.step == 1) : a.first;
+  const double ro = first ? rhs : a.rhs_old[q];
   a.r[q] = a.con1 * rhs + a.con2 * ro + bc;
   a.rhs_old[q] = rhs;
 }
 
+__global__ void __launch_bounds__(256) k_nl_rhs(NLArgs a) { NLOp::tile(a, blockIdx.x, 0, 0); }
+
 void nl_rhs(hipStream_t st, const Geom& g, const double* omega, const double* qx, const double* qy, double q0x,
             const double* lastbc, const double* omegabc, double vfac, double con1, double con2, int first_step,
             double* rhs_old, double* r, double* raw_rhs) {
-  NLArgs a{g, omega, qx, qy, q0x, lastbc, omegabc, vfac, con1, con2, first_step, rhs_old, r, raw_rhs};
+  NLArgs a{g, omega, qx, qy, q0x, lastbc, omegabc, vfac, con1, con2, first_step, nullptr, rhs_old, r, raw_rhs};
   hipLaunchKernelGGL(k_nl_rhs, dim3(ceil_div(g.P, 256)), dim3(256), 0, st, a);
   IB_LAUNCH_CHECK();
+}
+
+TabEntry<NLArgs> nl_rhs_entry(const Geom& g, const double* omega, const double* qx, const double* qy, double q0x,
+                              const double* lastbc, const double* omegabc, double vfac, double con1, double con2,
+                              const int* step, double* rhs_old, double* r) {
+  TabEntry<NLArgs> e{};
+  e.a = NLArgs{g, omega, qx, qy, q0x, lastbc, omegabc, vfac, con1, con2, 0, step, rhs_old, r, nullptr};
+  e.nbx = ceil_div(g.P, 256);
+  e.nby = 1;
+  return e;
+}
+
+const void* fluid_tab_kernel(int kind) {
+  switch (kind) {
+    case TK_FILL: return (const void*)k_tab<FillOp>;
+    case TK_GET_BC: return (const void*)k_tab<GetBcOp>;
+    case TK_COARSIFY: return (const void*)k_tab<CoarsenOp>;
+    case TK_CURL_GETBC: return (const void*)k_tab<CurlOp>;
+    case TK_NL_RHS: return (const void*)k_tab<NLOp>;
+    default: throw HipError("no stencil table kernel of this kind");
+  }
 }
 
 }  // namespace ibg

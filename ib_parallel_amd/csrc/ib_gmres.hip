@@ -12,6 +12,7 @@
 // convergence reason, so the host may enqueue several iterations ahead of
 // the status it has read (the extra matvecs then write only into basis
 // columns the solution never uses).
+#include "ib_device.h"
 #include "ib_kernels.h"
 
 namespace ibg {
@@ -143,12 +144,20 @@ __global__ void __launch_bounds__(kGmT) k_gm_buildsol(int nf, int kmax, int kBsC
 // splits the columns into 16 contiguous groups (4 per wavefront: lane & 15
 // is the row, lane >> 4 the group), so the matrix streams through many CUs;
 // the 16 partial sums of a row are added in group order.
-__global__ void __launch_bounds__(256) k_gemv(int nf, const double* __restrict__ S, const double* __restrict__ x,
-                                              double* __restrict__ y) {
+struct GemvOp {
+  using Args = GemvArgs;
+  static __device__ __forceinline__ void tile(const GemvArgs& a, int bx, int by, int);
+};
+
+__device__ __forceinline__ void GemvOp::tile(const GemvArgs& a, int bx, int, int) {
   __shared__ double part[16][16];
+  const int nf = a.nf;
+  const double* __restrict__ S = a.S;
+  const double* __restrict__ x = a.x;
+  double* __restrict__ y = a.y;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int rl = lane & 15, grp = wv * 4 + (lane >> 4);
-  const int r = blockIdx.x * 16 + rl;
+  const int r = bx * 16 + rl;
   const int per = (nf + 15) / 16;
   const int j0 = min(nf, grp * per), j1 = min(nf, j0 + per);
   double acc = 0.0;
@@ -174,9 +183,22 @@ __global__ void __launch_bounds__(256) k_gemv(int nf, const double* __restrict__
   }
 }
 
+__global__ void __launch_bounds__(256) k_gemv(GemvArgs a) { GemvOp::tile(a, blockIdx.x, 0, 0); }
+
+TabEntry<GemvArgs> gemv_entry(int nf, const double* S, const double* x, double* y) {
+  TabEntry<GemvArgs> e{};
+  e.a = GemvArgs{nf, S, x, y};
+  e.nbx = ceil_div(std::max(nf, 1), 16);
+  e.nby = 1;
+  return e;
+}
+
+const void* gemv_tab_kernel() { return (const void*)k_tab<GemvOp>; }
+
 void gemv(hipStream_t st, int nf, const double* S, const double* x, double* y) {
   if (nf <= 0) return;
-  hipLaunchKernelGGL(k_gemv, dim3(ceil_div(nf, 16)), dim3(256), 0, st, nf, S, x, y);
+  const TabEntry<GemvArgs> e = gemv_entry(nf, S, x, y);
+  hipLaunchKernelGGL(k_gemv, dim3(e.nbx), dim3(256), 0, st, e.a);
   IB_LAUNCH_CHECK();
 }
 

@@ -1,0 +1,955 @@
+// ib_handle.h -- one simulation's device-resident state and its time step.
+//
+// The step mirrors src/timestepper.F90:18-556 for sub_domain = F:
+//   1. predictor on every level, coarse to fine (:100-236): fused
+//      nonlinear+rot+apply_bc+AB2 kernel, then x pass (2 arrays), fused
+//      y-DST / lam1 / lam1i / y-DST column pass, x pass;
+//   2. vort2flux (:239-245) and the FSI fixed point (:249-456) with
+//      device-resident GMRES on the Schur operator a_times2 + b_times2, or
+//      the num_stat solve with the precomputed inverse (:457-484);
+//   3. corrector (:488-553): reg, rot, ainv with the subtraction fused into its
+//      last row pass, vort2flux.
+// All state lives in HBM for the handle's lifetime; the host reads back only
+// GMRES/FSI status words.  Included by the C ABI (ib_solver.hip) and by the
+// batched form (ib_batch.hip).
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ib_gpu.h"
+#include "ib_kernels.h"
+
+namespace ibg {
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t n_) {
+    release();
+    n = n_;
+    IB_HIP(hipMalloc(&p, sizeof(T) * (n > 0 ? n : 1)));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { release(); }
+  operator T*() const { return p; }
+};
+
+struct KTimer {
+  bool on = false;
+  int64_t count = 0;
+  double total_ms = 0;
+  double total_bytes = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, spare;
+  hipEvent_t start_ev = nullptr;
+  void begin(hipStream_t st, double bytes) {
+    if (!on) return;
+    total_bytes += bytes;
+    std::pair<hipEvent_t, hipEvent_t> e;
+    if (!spare.empty()) {
+      e = spare.back();
+      spare.pop_back();
+    } else {
+      IB_HIP(hipEventCreate(&e.first));
+      IB_HIP(hipEventCreate(&e.second));
+    }
+    IB_HIP(hipEventRecord(e.first, st));
+    pending.push_back(e);
+  }
+  void end(hipStream_t st) {
+    if (!on) return;
+    IB_HIP(hipEventRecord(pending.back().second, st));
+  }
+  void harvest() {
+    for (auto& e : pending) {
+      float ms = 0;
+      IB_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+      total_ms += ms;
+      ++count;
+      spare.push_back(e);
+    }
+    pending.clear();
+  }
+  void destroy() {
+    for (auto& e : pending) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    for (auto& e : spare) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    pending.clear();
+    spare.clear();
+  }
+};
+
+}  // namespace ibg
+
+using namespace ibg;
+
+struct ib_handle {
+  ib_params prm{};
+  std::string err;
+  int dev = 0;
+  hipStream_t st = nullptr;
+  size_t max_lds = 65536;
+  Geom g{};
+  int L = 0;
+  double delta = 0, dt = 0, Re = 0, offx = 0, offy = 0, normalize = 0;
+  HostFFTPlan px, py;
+  DBuf<double> laminv, lam1, lam1i;
+  std::vector<double> vfac, con1, con2, q0x;
+  DBuf<double> omega, s, qx, qy, rhs_old;
+  DBuf<double> t0, t1, t2;
+  DBuf<double> vv, sn, vx, vy, hx, hy;
+  DBuf<double> lastbc, obc, sbcA, sbcB, zbc;
+  // bodies
+  int nb = 0, nf = 0, nbodies = 0, ntors = 0;
+  std::vector<int> h_cbody, h_ccomp, h_tors_of;
+  std::vector<TorsBody> h_tb;
+  DBuf<double> xb, yb, f, f_rdst, w, rhsf, gw, opx;
+  DBuf<int> comp, tors_of, err_flag;
+  RegCSR csr;
+  DBuf<TorsBody> tb;
+  DBuf<double> th, dabs;
+  // GMRES
+  int kmax = 30;
+  DBuf<double> V, H, cc, ss, grs, nrs, pnorm;
+  GmresDev* gst = nullptr;
+  GmresDev* hst = nullptr;  // pinned
+  double* hbuf = nullptr;   // pinned readback
+  int* hflag = nullptr;     // pinned
+  int itime = 0, ksp_iter = 0, fsi_iter = 0;
+  double err_fsi = 0;
+  KTimer tim[5];
+  // captured GMRES batches, keyed by (first basis column, batch length)
+  std::map<std::pair<int, int>, hipGraphExec_t> gm_graphs;
+  // dense Schur matrix and the column-batch buffers that assemble it
+  bool assembled = false;
+  static constexpr int kAutoAssembleMax = 8192;  // schur_mode 0 assembles up to this many force entries
+  int ncol = 0;  // columns per assembly batch
+  DBuf<double> Sd, Vb, T0b, T1b, Sb, BCa, BCb;
+  // face-space form: face set, its index map, G = Q on the set, T = G W^T
+  bool facemode = false;
+  static constexpr int kFaceMarginDefault = 2;
+  int face_margin = kFaceMarginDefault;
+  static constexpr size_t kAutoGreenMax = (size_t)1 << 28;  // schur_mode 0: G up to 2 GiB
+  int nF = 0;
+  int64_t green_builds = 0;
+  DBuf<int> Fkeys, fmap, tapidx, face_flag;
+  DBuf<double> Gm, Tm;
+  // numerically stationary path (num_stat): the inverse of the Cholesky-
+  // factored a_times matrix, and a torsional map with no torsional entries
+  DBuf<double> Ainv;
+  DBuf<int> no_tors;
+  // pressure post-processing (allocated on first use): cell-grid DST plans
+  // and table, corner products (two levels), rhs, solution, scratch
+  bool pres_ready = false;
+  Geom gp{};  // the m x n cells seen by the DST kernels as an (m+1) x (n+1) node grid
+  HostFFTPlan pxp, pyp;
+  DBuf<double> laminv_ddti, poq, prhs, ppl, pout, ptmp, ptmp2, ppbc, pubc, pvbc, pfdt;
+
+  // ------------------------------------------------------------------ util
+  void sync() {
+    IB_HIP(hipStreamSynchronize(st));
+    for (auto& t : tim) t.harvest();
+  }
+
+  // algorithmic bytes of one DST pass launch: arrays read + tables + arrays written
+  // algorithmic HBM bytes of one DST pass: every field array of the batch
+  // read and written once, spectral tables read once per launch
+  double pass_bytes(int nread, int ntab, int nwrite, int batch) const {
+    return 8.0 * (double)g.P * ((double)(nread + nwrite) * (double)batch + (double)ntab);
+  }
+  // timing classes: 0/1 row/column passes of the time step, 2/3 the batched
+  // passes that assemble the Schur matrix
+  static int tclass(int base, const DstLaunch& a) { return base + (a.strided && a.batch > kMaxBatch ? 2 : 0); }
+  void xpass(int mode, const DstLaunch& a) {
+    KTimer& t = tim[tclass(0, a)];
+    t.begin(st, pass_bytes(mode == X_SUBFROM ? 2 : (mode == X_POINTROT ? 0 : 1), 0, 1, a.batch));
+    dst_x(st, px.dev, mode, a, g, (px.dev.L <= 512 ? 2 : 1), max_lds);
+    t.end(st);
+  }
+  void ypass(int mode, const DstLaunch& a) {
+    KTimer& t = tim[tclass(1, a)];
+    t.begin(st, pass_bytes(mode == Y_PRED ? 2 : 1, mode == Y_PRED ? 2 : (mode == Y_SCALE2 ? 1 : 0), 1, a.batch));
+    dst_y(st, py.dev, mode, a, g, 1, max_lds);
+    t.end(st);
+  }
+
+  // s = ctci(w [+ apply_bc(bc)]): src/myfft.F90:177-188
+  void ctci_dev(const double* win, double* out, const double* bc) {
+    DstLaunch a;
+    a.in[0] = win; a.out[0] = t0; a.bc = bc;
+    xpass(bc ? X_APPLYBC : X_PLAIN, a);
+    DstLaunch b;
+    b.in[0] = t0; b.out[0] = t1; b.tab = laminv; b.div = 1.0;
+    ypass(Y_SCALE2, b);
+    DstLaunch c;
+    c.in[0] = t1; c.out[0] = out;
+    xpass(X_PLAIN, c);
+  }
+
+  // ainv: src/myfft.F90:161-173; with subfrom, target -= ainv(v) (corrector).
+  // v == nullptr: the input is rot(hx, hy) (rot_int + rot_bc), formed in the
+  // load of the first row pass.
+  void ainv_dev(const double* v, double* out, bool subfrom) {
+    DstLaunch a;
+    a.in[0] = v; a.out[0] = t0;
+    a.hx = hx; a.hy = hy;
+    xpass(v ? X_PLAIN : X_ROT, a);
+    DstLaunch b;
+    b.in[0] = t0; b.out[0] = t1; b.tab = lam1i; b.div = normalize;
+    ypass(Y_SCALE2, b);
+    DstLaunch c;
+    c.in[0] = t1; c.out[0] = out;
+    xpass(subfrom ? X_SUBFROM : X_PLAIN, c);
+  }
+
+  // vort2flux: src/operators_fluid.F90:237-377
+  void vort2flux_dev(double* wv, double* sv, double* qxv, double* qyv) {
+    const long P = g.P;
+    for (int k = 1; k < L; ++k) coarsify(st, g, wv + (k - 1) * P, wv + k * P);
+    const double* sbc = zbc;
+    for (int k = L - 1; k >= 0; --k) {
+      ctci_dev(wv + k * P, sv + k * P, k == L - 1 ? nullptr : sbc);
+      double* bout = nullptr;
+      if (k > 0) bout = (sbc == sbcA) ? sbcB.p : sbcA.p;
+      curl_getbc(st, g, sv + k * P, sbc, qxv + (long)k * g.Fx, qyv + (long)k * g.Fy, bout);
+      sbc = bout;
+    }
+  }
+
+  // ab_times = a_times2 + b_times2: src/operators_fluid.F90:14-120.
+  // hx/hy are zero outside the faces the current weights touch and reg
+  // rewrites every touched face, so they need no clearing per product; the
+  // coarse levels of vv are zero outside the centres coarsify rewrites.
+  void schur_dev(const double* x, double* y) {
+    reg(st, g, csr, w, x, hx, hy);
+    ainv_dev(nullptr, vv, false);
+    vort2flux_dev(vv, sn, vx, vy);
+    regT(st, g, nf, w, csr.key, vx, vy, 0.0, nullptr, y);
+    body_btimes(st, ntors, tb, th, dt, delta, x, y);
+  }
+
+  // Sd and the column-batch buffers for batches of up to `need` columns;
+  // each column needs L + 3 node arrays and two bc vectors, and a batch is
+  // held to about 4 GiB of HBM (larger column counts run in several batches).
+  void alloc_assembly(int need) {
+    if (nf == 0) return;
+    if (!Sd.p) Sd.alloc((size_t)nf * nf);
+    if (Vb.p) return;
+    const size_t per_col = sizeof(double) * ((size_t)(L + 3) * g.P + 2 * (size_t)g.nbc);
+    const size_t batch_budget = (size_t)4 << 30;
+    ncol = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(need, 1), batch_budget / per_col));
+    ncol = std::min(ncol, 65535);  // grid y limit of the batched launches
+    Vb.alloc((size_t)ncol * L * g.P);
+    T0b.alloc((size_t)ncol * g.P);
+    T1b.alloc((size_t)ncol * g.P);
+    Sb.alloc((size_t)ncol * g.P);
+    BCa.alloc((size_t)ncol * g.nbc);
+    BCb.alloc((size_t)ncol * g.nbc);
+    fill(st, Vb, (long)Vb.n, 0.0);  // coarse-level margins stay zero (coarsify rewrites only centres)
+  }
+
+  // The column-batch buffers are only needed while columns are formed;
+  // forms whose matrix never changes give them back after set-up.
+  bool keeps_columns() const { return facemode || (assembled && ntors > 0); }
+  void trim_columns() {
+    if (keeps_columns() || !Vb.p) return;
+    sync();
+    Vb.release(); T0b.release(); T1b.release(); Sb.release(); BCa.release(); BCb.release();
+    ncol = 0;
+  }
+
+  // One batch of nc operator columns through ainv and the multi-level
+  // vort2flux.  Column c of the batch starts from rot(E^T e_(c0+c)) (w given:
+  // the 49 taps of force entry c0+c) or from rot of the unit face source
+  // keys[c0+c] (w null).  Leaves the level-1 streamfunctions in Sb (stride P)
+  // and returns their boundary vectors (stride nbc, or zbc with stride 0 when
+  // there is a single level).
+  const double* column_batch(int c0, int nc, const double* wts, const int* keys, long* bc_stride) {
+    const long P = g.P, nbc = g.nbc, LP = (long)L * P;
+    DstLaunch a;
+    a.batch = nc; a.strided = true; a.out[0] = T0b; a.s_out = P;
+    a.c0 = c0; a.w = wts; a.key = keys;  // the input is formed from the weights or the face list
+    xpass(X_POINTROT, a);
+    DstLaunch b;
+    b.batch = nc; b.strided = true; b.in[0] = T0b; b.s_in = P; b.out[0] = T1b; b.s_out = P; b.tab = lam1i;
+    b.div = normalize;
+    ypass(Y_SCALE2, b);
+    DstLaunch c;
+    c.batch = nc; c.strided = true; c.in[0] = T1b; c.s_in = P; c.out[0] = Vb; c.s_out = LP;
+    xpass(X_PLAIN, c);
+    for (int k = 1; k < L; ++k) coarsify(st, g, Vb + (k - 1) * P, Vb + k * P, nc, LP);
+    double* bcur = nullptr;  // boundary from the next coarser level (none on the coarsest)
+    for (int k = L - 1; k >= 0; --k) {
+      DstLaunch d;
+      d.batch = nc; d.strided = true; d.in[0] = Vb + k * P; d.s_in = LP; d.out[0] = T0b; d.s_out = P;
+      if (bcur) { d.bc = bcur; d.s_bc = nbc; }
+      xpass(bcur ? X_APPLYBC : X_PLAIN, d);
+      DstLaunch e;
+      e.batch = nc; e.strided = true; e.in[0] = T0b; e.s_in = P; e.out[0] = T1b; e.s_out = P; e.tab = laminv;
+      e.div = 1.0;
+      ypass(Y_SCALE2, e);
+      DstLaunch f2;
+      f2.batch = nc; f2.strided = true; f2.in[0] = T1b; f2.s_in = P; f2.out[0] = Sb; f2.s_out = P;
+      xpass(X_PLAIN, f2);
+      if (k > 0) {
+        double* bnext = (bcur == BCa.p) ? BCb.p : BCa.p;
+        get_bc(st, g, Sb, P, bnext, nbc, nc, 1.0);
+        bcur = bnext;
+      }
+    }
+    *bc_stride = bcur ? nbc : 0;
+    return bcur ? bcur : zbc.p;
+  }
+
+  // Dense Schur matrix S(:, j) = ab_times(e_j) for every force entry j, in
+  // batches of ncol columns: ainv and the multi-level vort2flux run on all
+  // columns of a batch at once, and the level-1 flux is evaluated at the taps
+  // (regT) together with b_times2.  Every launch covers ncol arrays, so the
+  // kernels are throughput-bound instead of latency-bound.
+  void assemble_schur() {
+    alloc_assembly(nf);
+    for (int c0 = 0; c0 < nf; c0 += ncol) {
+      const int nc = std::min(ncol, nf - c0);
+      long sbc = 0;
+      const double* bc = column_batch(c0, nc, w, csr.key, &sbc);
+      schur_columns(st, g, nf, c0, nc, w, csr.key, Sb, g.P, bc, sbc, tors_of, tb, th, dt, delta, Sd);
+    }
+  }
+
+  // ---- face-space form (moving bodies): the level-1 flux operator Q =
+  // C (multi-level CtC)^-1 ainv C^T does not depend on the body, only E does.
+  // G = Q restricted to the faces within face_margin cells of the current
+  // taps is computed once by batched unit-source columns; every weight
+  // rebuild then forms S = W G W^T + B with two small contractions.  G is
+  // recomputed (for a new face set) only when a tap leaves the set.
+  void build_face_set() {
+    // touched faces = unique tap keys of the current weights
+    int nr = 0;
+    IB_HIP(hipMemcpyAsync(hflag, csr.nruns, sizeof(int), hipMemcpyDeviceToHost, st));
+    sync();
+    nr = *hflag;
+    std::vector<int> uniq(nr);
+    if (nr) IB_HIP(hipMemcpy(uniq.data(), csr.uniq, sizeof(int) * nr, hipMemcpyDeviceToHost));
+    std::vector<int> fm((size_t)g.Fx + g.Fy, -1);
+    const int r = face_margin;
+    for (int key : uniq) {
+      const bool xf = key < g.Fx;
+      const int ni = xf ? g.m + 1 : g.m, nj = xf ? g.n : g.n + 1, base = xf ? 0 : g.Fx;
+      const int i = (key - base) % ni, j = (key - base) / ni;
+      for (int dj = -r; dj <= r; ++dj)
+        for (int di = -r; di <= r; ++di) {
+          const int ii = i + di, jj = j + dj;
+          if (ii >= 0 && ii < ni && jj >= 0 && jj < nj) fm[base + (size_t)jj * ni + ii] = 0;
+        }
+    }
+    std::vector<int> keys;
+    for (size_t q = 0; q < fm.size(); ++q)
+      if (fm[q] == 0) {
+        fm[q] = (int)keys.size();
+        keys.push_back((int)q);
+      }
+    nF = (int)keys.size();
+    Fkeys.alloc(std::max(1, nF));
+    fmap.alloc(fm.size());
+    IB_HIP(hipMemcpy(Fkeys.p, keys.data(), sizeof(int) * nF, hipMemcpyHostToDevice));
+    IB_HIP(hipMemcpy(fmap.p, fm.data(), sizeof(int) * fm.size(), hipMemcpyHostToDevice));
+  }
+
+  void compute_green() {
+    if (Gm.n < (size_t)nF * nF) Gm.alloc((size_t)nF * nF);
+    if (Tm.n < (size_t)nF * nf) Tm.alloc((size_t)nF * nf);
+    for (int c0 = 0; c0 < nF; c0 += ncol) {
+      const int nc = std::min(ncol, nF - c0);
+      long sbc = 0;
+      const double* bc = column_batch(c0, nc, nullptr, Fkeys, &sbc);
+      green_columns(st, g, nF, c0, nc, Fkeys, Sb, g.P, bc, sbc, Gm);
+    }
+    ++green_builds;
+  }
+
+  // the Schur matrix for the current weights (and theta): called after every
+  // weight rebuild once the stencil check has passed
+  void refresh_schur() {
+    if (!assembled) return;
+    if (!facemode) {
+      assemble_schur();
+      return;
+    }
+    IB_HIP(hipMemsetAsync(face_flag.p, 0, sizeof(int), st));
+    map_taps(st, nf, csr.key, fmap, tapidx, face_flag);
+    IB_HIP(hipMemcpyAsync(hflag, face_flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    sync();
+    if (*hflag) {
+      build_face_set();
+      compute_green();
+      map_taps(st, nf, csr.key, fmap, tapidx, face_flag);
+    }
+    schur_from_green(st, nf, nF, w, tapidx, Gm, Tm, tors_of, tb, th, dt, delta, Sd);
+  }
+
+  // setup_reg + the sorted tap list; the old touched faces of hx/hy are
+  // cleared first (the invariant schur_dev relies on)
+  void rebuild_weights() {
+    fill(st, hx, g.Fx, 0.0);
+    fill(st, hy, g.Fy, 0.0);
+    setup_reg(st, g, nf, xb, yb, comp, delta, offx, offy, prm.deltafnc_f32_literals, w, csr.key, err_flag);
+    csr.build(st);
+  }
+
+  int read_err_flag() {
+    IB_HIP(hipMemcpyAsync(hflag, err_flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    sync();
+    return *hflag;
+  }
+
+  void read_gmres() {
+    IB_HIP(hipMemcpyAsync(hst, gst, sizeof(GmresDev), hipMemcpyDeviceToHost, st));
+    sync();
+  }
+
+  // PETSc-style GMRES(kmax) on the Schur operator, x = initial guess / result.
+  // Iterations are enqueued in batches of up to kGmBatch between status
+  // reads; once the device status holds a reason the remaining Arnoldi
+  // kernels of the batch return at once, and the surplus matvecs only write
+  // basis columns past the converged one, which the solution never reads.
+  static constexpr int kGmBatch = 8;
+  // the single-workgroup solve streams the part of S that does not fit in
+  // its registers from L2 every iteration; past this size the multi-workgroup
+  // matrix-vector product of the graph-replayed iterations is faster
+  static constexpr int kFusedGmresMax = 400;
+
+  // nb Arnoldi iterations starting at basis column it0.  Every argument of
+  // these launches is fixed for the handle's lifetime except the column
+  // pointers, so each (it0, nb) is captured once as a HIP graph and replayed.
+  // Kernel timing records events per launch and therefore uses plain
+  // launches.
+  void apply_schur(const double* x, double* y) {
+    if (assembled) gemv(st, nf, Sd, x, y);
+    else schur_dev(x, y);
+  }
+  void gmres_batch_direct(int it0, int nb) {
+    for (int q = 0; q < nb; ++q) {
+      apply_schur(V + (size_t)(it0 + q) * nf, V + (size_t)(it0 + q + 1) * nf);
+      gm_arnoldi(st, nf, kmax, V, gst, H, cc, ss, grs, pnorm);
+    }
+  }
+  void gmres_batch(int it0, int nb) {
+    if (prm.launch_mode != 0 || tim[0].on || tim[1].on || tim[2].on || tim[3].on || tim[4].on) {
+      gmres_batch_direct(it0, nb);
+      return;
+    }
+    auto key = std::make_pair(it0, nb);
+    auto itr = gm_graphs.find(key);
+    if (itr == gm_graphs.end()) {
+      hipGraph_t gr = nullptr;
+      hipGraphExec_t ge = nullptr;
+      IB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        gmres_batch_direct(it0, nb);
+      } catch (...) {
+        (void)hipStreamEndCapture(st, &gr);
+        if (gr) (void)hipGraphDestroy(gr);
+        throw;
+      }
+      IB_HIP(hipStreamEndCapture(st, &gr));
+      IB_HIP(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
+      IB_HIP(hipGraphDestroy(gr));
+      itr = gm_graphs.emplace(key, ge).first;
+    }
+    IB_HIP(hipGraphLaunch(itr->second, st));
+  }
+  int gmres_dev(const double* b, double* x, int* its_out) {
+    if (assembled && nf <= kFusedGmresMax && gmres_dense_fits(nf, kmax, max_lds)) {
+      // dense Schur matrix: the whole solve is one single-workgroup launch.
+      // Algorithmic bytes: S, b read once, x read and written, and one
+      // Krylov vector written per iteration (added once the count is known)
+      tim[4].begin(st, 8.0 * ((double)nf * nf + 3.0 * nf));
+      gmres_dense(st, nf, kmax, prm.gmres_max_it, Sd, b, x, V, H, gst, prm.rtol, prm.atol, max_lds);
+      tim[4].end(st);
+      read_gmres();
+      if (tim[4].on) tim[4].total_bytes += 8.0 * (double)nf * (hst->its + 1);
+      *its_out = hst->its;
+      return hst->reason > 0 ? IB_OK : IB_E_NOCONV_GMRES;
+    }
+    GmresDev init{};
+    init.first = 1;
+    *hst = init;
+    IB_HIP(hipMemcpyAsync(gst, hst, sizeof(GmresDev), hipMemcpyHostToDevice, st));
+    int reason = 0, its = 0;
+    const int maxit = prm.gmres_max_it;
+    while (true) {
+      apply_schur(x, gw);
+      gm_start(st, nf, kmax, b, gw, V, gst, H, grs, prm.rtol, prm.atol);
+      read_gmres();
+      reason = hst->reason;
+      if (reason) break;
+      int it = 0;
+      while (true) {
+        const int nb = std::min(kGmBatch, std::min(kmax - it, maxit - its));
+        gmres_batch(it, nb);
+        read_gmres();
+        reason = hst->reason;
+        its = hst->its;
+        it = hst->it;
+        if (reason || it >= kmax || its >= maxit) break;
+      }
+      gm_buildsol(st, nf, kmax, V, gst, H, grs, x, nrs);
+      if (reason) break;
+      if (its >= maxit) {
+        reason = -2;
+        break;
+      }
+    }
+    *its_out = its;
+    return reason > 0 ? IB_OK : IB_E_NOCONV_GMRES;
+  }
+
+  // num_stat preprocess (src/timestepper.F90:560-598) and the Cholesky
+  // factorisation of KSPPREONLY + PCCHOLESKY (src/variables.F90:555-561):
+  // column j = a_times(e_j) (no b_times2: every entry is treated as rigid),
+  // A = U^T U from its upper triangle, then A^-1 once so each step's solve is
+  // a matrix-vector product.
+  // Buffers a captured batch graph holds pointers to keep their addresses
+  // when the set-up runs again (restart).
+  int num_stat_setup() {
+    if (!no_tors.p) no_tors.alloc(std::max(nf, 1));
+    IB_HIP(hipMemsetAsync(no_tors.p, 0xff, sizeof(int) * std::max(nf, 1), st));  // every entry -1
+    alloc_assembly(nf);
+    for (int c0 = 0; c0 < nf; c0 += ncol) {
+      const int nc = std::min(ncol, nf - c0);
+      long sbc = 0;
+      const double* bc = column_batch(c0, nc, w, csr.key, &sbc);
+      schur_columns(st, g, nf, c0, nc, w, csr.key, Sb, g.P, bc, sbc, no_tors, tb, th, dt, delta, Sd);
+    }
+    IB_HIP(hipMemsetAsync(err_flag.p, 0, sizeof(int), st));
+    chol_factor(st, nf, Sd, err_flag);
+    if (read_err_flag()) return fail(IB_E_INVAL, "num_stat: the Schur matrix is not positive definite");
+    if (!Ainv.p) Ainv.alloc((size_t)nf * nf);
+    chol_inverse(st, nf, Sd, Ainv, max_lds);
+    sync();
+    trim_columns();
+    return IB_OK;
+  }
+
+  // step 2: FSI fixed point (src/timestepper.F90:249-456).  Returns the
+  // step status; IB_E_INVAL when a body stencil leaves the grid.
+  int fsi_fixed_point() {
+    int status = IB_OK;
+    int fsi_count = 0;
+    double err_glob = 10.0;
+    ksp_iter = 0;
+    if (ntors > 0) {
+      if (itime == 1) body_initial_acc(st, ntors, tb, th, dt, delta, f);
+      IB_HIP(hipMemcpyAsync(th + 3 * ntors, th.p, sizeof(double) * 3 * ntors, hipMemcpyDeviceToDevice, st));
+    }
+    while (err_glob >= prm.fsi_tol && fsi_count <= prm.fsi_max_count) {
+      ++fsi_count;
+      fsi_rhs(st, g, nf, tors_of, tb, ntors, th, dt, delta, w, csr.key, qx, qy, q0x[0], rhsf);
+      int its = 0;
+      int rc = gmres_dev(rhsf, f, &its);
+      ksp_iter += its;
+      if (rc != IB_OK) status = rc;
+      // redistribute (output only)
+      redist_faces(st, g, csr, w, f, hx, hy);
+      regT(st, g, nf, w, csr.key, hx, hy, 0.0, nullptr, f_rdst);
+      if (ntors > 0) {
+        body_dx(st, ntors, tb, th, dt, delta, f, xb, yb, dabs);
+        rebuild_weights();
+        IB_HIP(hipMemcpyAsync(hbuf, dabs.p, sizeof(double) * ntors, hipMemcpyDeviceToHost, st));
+        if (read_err_flag()) {
+          err = "body point regularisation stencil leaves the grid";
+          return IB_E_INVAL;
+        }
+        refresh_schur();
+        err_glob = 0.0;
+        for (int t = 0; t < ntors; ++t) err_glob = std::max(err_glob, hbuf[t]);
+      } else {
+        err_glob = 0.0;  // rigid bodies only: positions and weights unchanged
+      }
+    }
+    if (err_glob >= prm.fsi_tol && status == IB_OK) status = IB_E_NOCONV_FSI;
+    fsi_iter = fsi_count;
+    err_fsi = err_glob;
+    return status;
+  }
+
+  // ddti (src/myfft.F90:192-203): out = S(laminv_ddti S(in)) on the cells
+  void ddti_dev(const double* in, double* out) {
+    DstLaunch a;
+    a.in[0] = in; a.out[0] = ptmp;
+    dst_x(st, pxp.dev, X_PLAIN, a, gp, (pxp.dev.L <= 512 ? 2 : 1), max_lds);
+    DstLaunch b;
+    b.in[0] = ptmp; b.out[0] = ptmp2; b.tab = laminv_ddti; b.div = 1.0;
+    dst_y(st, pyp.dev, Y_SCALE2, b, gp, 1, max_lds);
+    DstLaunch c;
+    c.in[0] = ptmp2; c.out[0] = out;
+    dst_x(st, pxp.dev, X_PLAIN, c, gp, (pxp.dev.L <= 512 ? 2 : 1), max_lds);
+  }
+
+  void pressure_setup() {
+    if (pres_ready) return;
+    const int m = g.m, n = g.n;
+    gp = g;
+    gp.m = m + 1;
+    gp.n = n + 1;
+    gp.P = m * n;
+    pxp.init(m);
+    pyp.init(n);
+    if (dst_lds_bytes(pxp.dev, 1) > max_lds || dst_lds_bytes(pyp.dev, 2) > max_lds)
+      throw HipError("grid too large for the single-workgroup pressure DST (LDS)");
+    // setup_fft's pressure table (src/myfft.F90:123,150-155)
+    const double pi = 4.0 * std::atan(1.0);
+    const double norm_ddti = 4.0 * (double)(float)((m + 1) * (n + 1));
+    std::vector<double> h((size_t)m * n);
+    for (int j = 1; j <= n; ++j)
+      for (int i = 1; i <= m; ++i) {
+        const double lam =
+            2.0 * (std::cos(pi * (double)i / (double)(m + 1)) + std::cos(pi * (double)j / (double)(n + 1)) - 2.0);
+        h[(size_t)(i - 1) + (size_t)m * (j - 1)] = 1.0 / lam / norm_ddti;
+      }
+    const size_t C = (size_t)m * n;
+    laminv_ddti.alloc(C);
+    IB_HIP(hipMemcpy(laminv_ddti.p, h.data(), sizeof(double) * C, hipMemcpyHostToDevice));
+    poq.alloc(4 * (size_t)g.P);
+    prhs.alloc(C * L); ppl.alloc(C * L); pout.alloc(C * L); ptmp.alloc(C); ptmp2.alloc(C);
+    ppbc.alloc(2 * (size_t)(m + n)); pubc.alloc(g.nbc); pvbc.alloc(g.nbc); pfdt.alloc(std::max(nf, 1));
+    pres_ready = true;
+  }
+
+  // calculate_pressure (src/operators_pressure.F90:110-437) at the current
+  // state into pout (mgridlev blocks of m*n pressure coefficients)
+  void pressure_dev() {
+    pressure_setup();
+    const long P = g.P, C = (long)g.m * g.n;
+    // nonlinear terms on the faces of every level, coarse to fine; the bc of
+    // level k comes from the corner products of level k+1
+    double* oq[2][2] = {{poq.p, poq.p + P}, {poq.p + 2 * P, poq.p + 3 * P}};
+    int cur = 0;
+    for (int k = L - 1; k >= 0; --k) {
+      const double* ubc = zbc.p;
+      const double* vbc = zbc.p;
+      if (k < L - 1) {
+        get_bc(st, g, oq[1 - cur][0], P, pubc, g.nbc, 1, 1.0);
+        get_bc(st, g, oq[1 - cur][1], P, pvbc, g.nbc, 1, 1.0);
+        ubc = pubc;
+        vbc = pvbc;
+      }
+      pressure_corner(st, g, omega + k * P, qx + (long)k * g.Fx, qy + (long)k * g.Fy, q0x[k], oq[cur][0],
+                      oq[cur][1]);
+      pressure_edges(st, g, oq[cur][0], oq[cur][1], ubc, vbc, delta * std::pow(2.0, (double)k),
+                     vx + (long)k * g.Fx, vy + (long)k * g.Fy);
+      cur = 1 - cur;
+    }
+    // rhs: -div(E^T f/dt) on level 1, coarsified outward, plus div N(q)
+    scale_div(st, nf, f, dt, pfdt);
+    reg(st, g, csr, w, pfdt, hx, hy);  // hx, hy are zero outside the touched faces
+    pressure_div(st, g, hx, hy, prhs, false);
+    for (int k = 1; k < L; ++k) {
+      fill(st, prhs + k * C, C, 0.0);
+      pressure_coarsify(st, g, prhs + (k - 1) * C, prhs + k * C);
+    }
+    for (int k = 0; k < L; ++k) pressure_div(st, g, vx + (long)k * g.Fx, vy + (long)k * g.Fy, prhs + k * C, true);
+    // D D^T p = rhs, coarse to fine
+    for (int k = L - 1; k >= 0; --k) {
+      if (k == L - 1) {
+        const double dk = delta * std::pow(2.0, (double)k);
+        pressure_bc1(st, g, qx + (long)k * g.Fx, qy + (long)k * g.Fy, q0x[k], 0.125 / (dk * dk), ppbc);
+      } else {
+        pressure_bc2(st, g, ppl + (k + 1) * C, ppbc);
+      }
+      pressure_applybc(st, g, prhs + k * C, ppbc);
+      ddti_dev(prhs + k * C, ppl + k * C);
+    }
+    for (int k = 0; k < L; ++k) {
+      const double dk = delta * std::pow(2.0, (double)k);
+      pressure_final(st, g, ppl + k * C, qx + (long)k * g.Fx, qy + (long)k * g.Fy, q0x[k], 0.125 / (dk * dk),
+                     pout + k * C);
+    }
+  }
+
+  // ---------------------------------------------------------------- advance
+  int advance() {
+    itime += 1;
+    const long P = g.P, nbc = g.nbc;
+    // step 1: lastbc(k) = 0.25 get_bc(omega_{k+1}^n) (src/timestepper.F90:103-117)
+    if (L > 1) get_bc(st, g, omega + P, P, lastbc, nbc, L - 1, 0.25);
+    fill(st, lastbc + (L - 1) * nbc, nbc, 0.0);
+    for (int k = L - 1; k >= 0; --k) {
+      const double* ob = zbc;
+      if (k < L - 1) {
+        get_bc(st, g, omega + (k + 1) * P, P, obc, nbc, 1, 0.25);
+        ob = obc;
+      }
+      nl_rhs(st, g, omega + k * P, qx + (long)k * g.Fx, qy + (long)k * g.Fy, q0x[k], lastbc + k * nbc, ob, vfac[k],
+             con1[k], con2[k], itime == 1, rhs_old + k * P, t2, nullptr);
+      DstLaunch a;
+      a.batch = 2;
+      a.in[0] = t2; a.out[0] = t0;
+      a.in[1] = omega + k * P; a.out[1] = t1;
+      xpass(X_PLAIN, a);
+      DstLaunch b;
+      b.in[0] = t0; b.in2[0] = t1; b.out[0] = t2; b.tab = lam1 + k * P; b.tab2 = lam1i + k * P;
+      ypass(Y_PRED, b);
+      DstLaunch c;
+      c.in[0] = t2; c.out[0] = omega + k * P;
+      xpass(X_PLAIN, c);
+    }
+    vort2flux_dev(omega, s, qx, qy);
+
+    int status = IB_OK;
+    if (prm.num_stat) {
+      // step 2 for a numerically stationary body (src/timestepper.F90:457-484):
+      // rhsf = regT(q + q0), f = A^-1 rhsf, redistribute; nothing moves and
+      // no iteration counts are kept
+      fsi_rhs(st, g, nf, no_tors, tb, 0, th, dt, delta, w, csr.key, qx, qy, q0x[0], rhsf);
+      gemv(st, nf, Ainv, rhsf, f);
+      redist_faces(st, g, csr, w, f, hx, hy);
+      regT(st, g, nf, w, csr.key, hx, hy, 0.0, nullptr, f_rdst);
+      ksp_iter = 0;
+      fsi_iter = 0;
+      err_fsi = 0.0;
+    } else {
+      status = fsi_fixed_point();
+      if (status == IB_E_INVAL) return status;
+    }
+
+    // step 3: corrector (src/timestepper.F90:488-553)
+    reg(st, g, csr, w, f, hx, hy);
+    ainv_dev(nullptr, omega, true);
+    vort2flux_dev(omega, s, qx, qy);
+    sync();
+    return status;
+  }
+
+  // ------------------------------------------------------------------ setup
+  int fail(int code, const std::string& m) {
+    err = m;
+    return code;
+  }
+
+  int init(const ib_params* p, int nbod, const ib_body* bodies) {
+    prm = *p;
+    if (prm.gmres_restart <= 0) prm.gmres_restart = 30;
+    if (prm.gmres_max_it <= 0) prm.gmres_max_it = 10000;
+    if (prm.fsi_max_count <= 0) prm.fsi_max_count = 1000;
+    const int m = prm.m, n = prm.n;
+    L = prm.mgridlev;
+    if (m < 8 || n < 8 || m % 4 || n % 4) return fail(IB_E_INVAL, "grid must be divisible by 4 (m, n >= 8)");
+    if (L < 1) return fail(IB_E_INVAL, "mgridlev must be >= 1");
+    if (prm.sub_domain) return fail(IB_E_UNSUPPORTED, "the sub_domain path is not built");
+    if (prm.istart < 0) return fail(IB_E_INVAL, "istart must be >= 0");
+    if (prm.gmres_restart > 1000) return fail(IB_E_INVAL, "gmres_restart must be <= 1000");
+    kmax = prm.gmres_restart;
+    dt = prm.dt; Re = prm.re; offx = prm.offsetx; offy = prm.offsety;
+    delta = prm.len / (double)m;
+    g.m = m; g.n = n; g.L = L;
+    g.P = (m - 1) * (n - 1); g.Fx = (m + 1) * n; g.Fy = m * (n + 1);
+    g.nbc = 2 * (m + 1) + 2 * (n + 1);
+    g.left = 0; g.right = n + 1; g.bottom = 2 * (n + 1); g.top = 2 * (n + 1) + m + 1;
+
+    // bodies (src/grid.F90:239-433)
+    if (nbod < 0) return fail(IB_E_INVAL, "negative body count");
+    bool seen_tors = false;
+    std::vector<int> nbb(1, 0);
+    std::vector<double> hxb, hyb;
+    for (int b = 0; b < nbod; ++b) {
+      const ib_body& B = bodies[b];
+      if (B.npts < 2) return fail(IB_E_INVAL, "a body needs at least 2 points");
+      if (B.fam != 1 && B.fam != 2)
+        return fail(IB_E_UNSUPPORTED, "only body families 1 (rigid) and 2 (torsional) exist in the reference");
+      if (B.fam == 1 && seen_tors) return fail(IB_E_INVAL, "rigid bodies must precede torsional bodies (src/grid.F90:1226)");
+      if (B.fam == 2) {
+        seen_tors = true;
+        if (B.itheta <= 0) return fail(IB_E_INVAL, "torsional body needs itheta > 0");
+        TorsBody t{};
+        t.off = nbb.back();
+        t.npts = B.npts;
+        t.gamma0 = B.gamma0;
+        t.deltas = std::sqrt((B.x[1] - B.x[0]) * (B.x[1] - B.x[0]) + (B.y[1] - B.y[0]) * (B.y[1] - B.y[0]));
+        t.xbf0 = B.x[0]; t.ybf0 = B.y[0];
+        t.M = B.itheta; t.K = B.ktheta; t.C = B.ctheta;
+        h_tb.push_back(t);
+      }
+      for (int j = 0; j < 2 * B.npts; ++j) {
+        const int pnt = j < B.npts ? j : j - B.npts;
+        h_cbody.push_back(b);
+        h_ccomp.push_back(j < B.npts ? 1 : 2);
+        h_tors_of.push_back(B.fam == 2 ? (int)h_tb.size() - 1 : -1);
+        hxb.push_back(B.x[pnt]);
+        hyb.push_back(B.y[pnt]);
+      }
+      nbb.push_back(nbb.back() + 2 * B.npts);
+      nb += B.npts;
+    }
+    nbodies = nbod;
+    nf = 2 * nb;
+    ntors = (int)h_tb.size();
+
+    IB_HIP(hipGetDevice(&dev));
+    IB_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    int lds = 0, optin = 0;
+    IB_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess) optin = 0;
+    (void)hipGetLastError();
+    max_lds = (size_t)std::max(lds, optin);
+    if (max_lds > 65536) {
+      dst_set_max_lds(max_lds);
+      gmres_dense_set_max_lds(max_lds);
+    }
+
+    px.init(m - 1);
+    py.init(n - 1);
+    if (dst_lds_bytes(px.dev, 1) > max_lds || dst_lds_bytes(py.dev, 2) > max_lds)
+      return fail(IB_E_UNSUPPORTED, "grid too large for the single-workgroup DST (LDS)");
+
+    // setup_fft tables: src/myfft.F90:262-289
+    const double pi = 4.0 * std::atan(1.0);
+    normalize = 4.0 * (double)(float)(m * n);
+    const double del2 = delta * delta;
+    vfac.resize(L); con1.resize(L); con2.resize(L); q0x.resize(L);
+    for (int k = 0; k < L; ++k) {
+      const double del22 = del2 * std::pow(4.0, (double)k);
+      vfac[k] = 0.5 * dt / Re / del22 / normalize;
+      con1[k] = 1.5 * dt / del22 / normalize;
+      con2[k] = -0.5 * dt / del22 / normalize;
+      // motion_potential with motion_grid = (-1, 0, ...): q0x = -fac * (-1) (src/variables.F90:299-314)
+      const double fac = delta * std::pow(2.0, (double)k);
+      q0x[k] = -fac * (-1.0) + 0.0;
+    }
+    const long P = g.P;
+    std::vector<double> hl(P), hl1((size_t)P * L), hl1i((size_t)P * L);
+    for (int j = 2; j <= n; ++j)
+      for (int i = 2; i <= m; ++i) {
+        const double lam = -2.0 * (std::cos(pi * (double)(i - 1) / (double)m) + std::cos(pi * (double)(j - 1) / (double)n) - 2.0);
+        const long q = (long)(i - 2) + (long)(m - 1) * (j - 2);
+        hl[q] = 1.0 / lam / normalize;
+        for (int k = 0; k < L; ++k) {
+          const double del22 = del2 * std::pow(4.0, (double)k);
+          hl1[(size_t)k * P + q] = (1.0 - 0.5 * dt * lam / del22 / Re) / normalize;
+          hl1i[(size_t)k * P + q] = 1.0 / (1.0 + 0.5 * dt * lam / del22 / Re);
+        }
+      }
+    laminv.alloc(P); lam1.alloc((size_t)P * L); lam1i.alloc((size_t)P * L);
+    IB_HIP(hipMemcpy(laminv.p, hl.data(), sizeof(double) * P, hipMemcpyHostToDevice));
+    IB_HIP(hipMemcpy(lam1.p, hl1.data(), sizeof(double) * P * L, hipMemcpyHostToDevice));
+    IB_HIP(hipMemcpy(lam1i.p, hl1i.data(), sizeof(double) * P * L, hipMemcpyHostToDevice));
+
+    // state (initial_condition: all zero, src/variables.F90:262-295)
+    omega.alloc((size_t)P * L); s.alloc((size_t)P * L); rhs_old.alloc((size_t)P * L);
+    qx.alloc((size_t)g.Fx * L); qy.alloc((size_t)g.Fy * L);
+    t0.alloc(P); t1.alloc(P); t2.alloc(P);
+    vv.alloc((size_t)P * L); sn.alloc((size_t)P * L); vx.alloc((size_t)g.Fx * L); vy.alloc((size_t)g.Fy * L);
+    hx.alloc(g.Fx); hy.alloc(g.Fy);
+    lastbc.alloc((size_t)g.nbc * L); obc.alloc(g.nbc); sbcA.alloc(g.nbc); sbcB.alloc(g.nbc); zbc.alloc(g.nbc);
+    double* zero_these[] = {omega, s, rhs_old, qx, qy, zbc, vv, hx, hy};
+    size_t zero_n[] = {omega.n, s.n, rhs_old.n, qx.n, qy.n, zbc.n, vv.n, hx.n, hy.n};
+    for (int i = 0; i < 9; ++i) fill(st, zero_these[i], (long)zero_n[i], 0.0);
+
+    xb.alloc(nf); yb.alloc(nf); f.alloc(nf); f_rdst.alloc(nf); rhsf.alloc(nf); gw.alloc(nf); opx.alloc(nf);
+    w.alloc((size_t)nf * kTaps);
+    comp.alloc(nf); tors_of.alloc(nf); err_flag.alloc(1);
+    if (nf > 0) {
+      IB_HIP(hipMemcpy(xb.p, hxb.data(), sizeof(double) * nf, hipMemcpyHostToDevice));
+      IB_HIP(hipMemcpy(yb.p, hyb.data(), sizeof(double) * nf, hipMemcpyHostToDevice));
+      IB_HIP(hipMemcpy(comp.p, h_ccomp.data(), sizeof(int) * nf, hipMemcpyHostToDevice));
+      IB_HIP(hipMemcpy(tors_of.p, h_tors_of.data(), sizeof(int) * nf, hipMemcpyHostToDevice));
+    }
+    fill(st, f, nf, 0.0);
+    fill(st, f_rdst, nf, 0.0);
+    IB_HIP(hipMemsetAsync(err_flag.p, 0, sizeof(int), st));
+    tb.alloc(std::max(1, ntors));
+    if (ntors > 0) IB_HIP(hipMemcpy(tb.p, h_tb.data(), sizeof(TorsBody) * ntors, hipMemcpyHostToDevice));
+    th.alloc((size_t)6 * std::max(1, ntors));
+    dabs.alloc(std::max(1, ntors));
+    fill(st, th, (long)th.n, 0.0);
+
+    V.alloc((size_t)(kmax + 1) * std::max(1, nf));
+    H.alloc((size_t)(kmax + 1) * kmax);
+    cc.alloc(kmax); ss.alloc(kmax); grs.alloc(kmax + 1); nrs.alloc(kmax + 1);
+    pnorm.alloc(std::max(1, ceil_div(std::max(nf, 1), 64)));
+    IB_HIP(hipMalloc(&gst, sizeof(GmresDev)));
+    IB_HIP(hipHostMalloc(&hst, sizeof(GmresDev), hipHostMallocDefault));
+    IB_HIP(hipHostMalloc(&hbuf, sizeof(double) * std::max(64, ntors + 1), hipHostMallocDefault));
+    IB_HIP(hipHostMalloc(&hflag, sizeof(int), hipHostMallocDefault));
+
+    csr.alloc(nf * kTaps, g.Fx + g.Fy);
+    rebuild_weights();
+    if (read_err_flag()) return fail(IB_E_INVAL, "body point regularisation stencil leaves the grid");
+
+    // dense Schur matrix: auto picks it whenever it fits comfortably
+    if (prm.schur_mode < 0 || prm.schur_mode > 3) return fail(IB_E_INVAL, "schur_mode must be 0, 1, 2 or 3");
+    if (prm.face_margin < -1) return fail(IB_E_INVAL, "face_margin must be >= -1");
+    face_margin = prm.face_margin == 0 ? kFaceMarginDefault : std::max(0, prm.face_margin);
+    if (nf > 0 && prm.num_stat) {
+      const int rc = num_stat_setup();
+      if (rc != IB_OK) return rc;
+    } else if (nf > 0) {
+      if (prm.schur_mode == 3 || (prm.schur_mode == 0 && ntors > 0)) {
+        assembled = facemode = true;
+      } else if (prm.schur_mode == 2 || (prm.schur_mode == 0 && nf <= kAutoAssembleMax)) {
+        assembled = true;
+      }
+    }
+    if (facemode) {
+      face_flag.alloc(1);
+      tapidx.alloc((size_t)nf * kTaps);
+      build_face_set();
+      if (prm.schur_mode == 0 && (size_t)nF * nF > kAutoGreenMax) {
+        // too many faces near the bodies for G: assemble S directly instead
+        facemode = false;
+        assembled = nf <= kAutoAssembleMax;
+        if (assembled) alloc_assembly(nf);
+      } else {
+        alloc_assembly(nF + nF / 2);  // room for the face set to grow as bodies move
+        compute_green();
+        map_taps(st, nf, csr.key, fmap, tapidx, face_flag);
+      }
+    } else if (assembled) {
+      alloc_assembly(nf);
+    }
+    if (assembled) {
+      if (facemode) {
+        schur_from_green(st, nf, nF, w, tapidx, Gm, Tm, tors_of, tb, th, dt, delta, Sd);
+      } else {
+        assemble_schur();
+      }
+    }
+    trim_columns();
+    itime = prm.istart;
+    sync();
+    return IB_OK;
+  }
+
+  ~ib_handle() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (auto& kv : gm_graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto& t : tim) t.destroy();
+    csr.release();
+    px.release();
+    py.release();
+    pxp.release();
+    pyp.release();
+    if (gst) (void)hipFree(gst);
+    if (hst) (void)hipHostFree(hst);
+    if (hbuf) (void)hipHostFree(hbuf);
+    if (hflag) (void)hipHostFree(hflag);
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  // -------------------------------------------------------------- host I/O
+  void h2d(double* d, const double* h, size_t n) {
+    IB_HIP(hipMemcpyAsync(d, h, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  }
+  void d2h(double* h, const double* d, size_t n) {
+    IB_HIP(hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  }
+};

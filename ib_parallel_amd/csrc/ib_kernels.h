@@ -44,6 +44,41 @@ struct HostFFTPlan {
   void release();
 };
 
+// Device arguments of a row pass: batch b reads in[b] (or in[0] + b*s_in
+// when strided) and writes out[b]; the plan of the row length travels with
+// them.
+struct XArgs {
+  const double* in[kMaxBatch];
+  double* out[kMaxBatch];
+  const double* bc;
+  const double* hx;  // X_ROT: the node value is rot(hx, hy)
+  const double* hy;
+  long s_in, s_out, s_bc;  // strided: batch b at base[0] + b * stride
+  int strided;              // 0: batch b at in[b], out[b] (b < kMaxBatch)
+  int c0;                  // X_POINTROT: force entry (or face) of batch element 0
+  const double* w;         // X_POINTROT: weights, 49 per entry; null: unit face sources
+  const int* key;          // X_POINTROT: face keys, 49 per entry (or one per face)
+  Geom g;
+  int rp;
+  FFTPlan pl;
+};
+
+// Device arguments of a column pass.
+struct YArgs {
+  const double* in[kMaxBatch];
+  const double* in2[kMaxBatch];
+  double* out[kMaxBatch];
+  long s_in, s_in2, s_out;  // strided: batch b at base[0] + b * stride
+  int strided;              // 0: batch b at in[b], out[b] (b < kMaxBatch)
+  const double* tab;
+  const double* tab2;
+  double div;
+  Geom g;
+  int cp;
+  int chunk;  // column-pair tiles per XCD (XCD-aware tile order)
+  FFTPlan pl;
+};
+
 // x pass over rows of length m-1 (n-1 rows); rp = row pairs per workgroup
 void dst_x(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& a, const Geom& g, int rp,
            size_t max_lds);
@@ -52,6 +87,14 @@ void dst_y(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& a, cons
            size_t max_lds);
 // LDS bytes a launch needs
 size_t dst_lds_bytes(const FFTPlan& pl, int nfft);
+// The arguments and grid dst_x / dst_y would launch with, as a table entry,
+// and the LDS bytes of that launch; the kernel of a mode for table launches.
+TabEntry<XArgs> dst_x_entry(const FFTPlan& pl, const DstLaunch& l, const Geom& g, int rp, size_t max_lds,
+                            size_t* lds);
+TabEntry<YArgs> dst_y_entry(const FFTPlan& pl, int mode, const DstLaunch& l, const Geom& g, int cp,
+                            size_t max_lds, size_t* lds);
+const void* dst_x_tab_kernel(int mode);
+const void* dst_y_tab_kernel(int mode);
 // Raises the dynamic-LDS limit of the DST kernels to `bytes` (call once).
 void dst_set_max_lds(size_t bytes);
 
@@ -74,6 +117,62 @@ void nl_rhs(hipStream_t st, const Geom& g, const double* omega, const double* qx
             double q0x, const double* lastbc, const double* omegabc, double vfac, double con1, double con2,
             int first_step, double* rhs_old, double* r, double* raw_rhs);
 void sub_inplace(hipStream_t st, double* a, const double* b, long n);
+
+// Device arguments of the stencil kernels, for table launches.
+struct FillArgs {
+  double* p;
+  long n;
+  double v;
+};
+struct GetBcArgs {
+  Geom g;
+  const double* src;
+  long sstride;
+  double* bc;
+  long bstride;
+  double fac;
+};
+struct CoarsenArgs {
+  Geom g;
+  const double* wf;
+  double* wc;
+  long stride;
+};
+struct CurlArgs {
+  Geom g;
+  const double* s;
+  const double* sbc;
+  double* qx;
+  double* qy;
+  double* bc_out;
+};
+struct NLArgs {
+  Geom g;
+  const double* w;
+  const double* qx;
+  const double* qy;
+  double q0x;
+  const double* lastbc;
+  const double* omegabc;
+  double vfac, con1, con2;
+  int first;
+  const int* step;  // if set, the first step is *step == 1 (first is ignored)
+  double* rhs_old;
+  double* r;
+  double* raw;
+};
+TabEntry<FillArgs> fill_entry(double* p, long n, double v);
+TabEntry<GetBcArgs> get_bc_entry(const Geom& g, const double* src, long src_stride, double* bc, long bc_stride,
+                                 int nlev, double fac);
+TabEntry<CoarsenArgs> coarsify_entry(const Geom& g, const double* fine, double* coarse, int batch, long stride);
+TabEntry<CurlArgs> curl_getbc_entry(const Geom& g, const double* s, const double* sbc, double* qx, double* qy,
+                                    double* bc_out);
+TabEntry<NLArgs> nl_rhs_entry(const Geom& g, const double* omega, const double* qx, const double* qy, double q0x,
+                              const double* lastbc, const double* omegabc, double vfac, double con1, double con2,
+                              const int* step, double* rhs_old, double* r);
+// kernels for table launches of the stencil and body families
+enum { TK_FILL, TK_GET_BC, TK_COARSIFY, TK_CURL_GETBC, TK_NL_RHS, TK_REG, TK_REDIST, TK_REGT, TK_FSI_RHS, TK_GEMV };
+const void* fluid_tab_kernel(int kind);
 
 // ---------------------------------------------------- immersed boundary ---
 // weights of setup_reg / deltafnc for every (entry, tap); face key per tap:
@@ -123,6 +222,64 @@ void schur_from_green(hipStream_t st, int nf, int nF, const double* w, const int
                       double delta, double* S);
 // y = S x for the dense nf x nf Schur matrix (column major)
 void gemv(hipStream_t st, int nf, const double* S, const double* x, double* y);
+// Device arguments of reg / redistribute (the CSR of the taps), regT,
+// fsi_rhs and gemv, for table launches.
+struct RegArgs {
+  Geom g;
+  int ntaps;
+  const int* nruns;
+  const int* uniq;
+  const int* off;
+  const int* cnt;
+  const int* taps;
+  const double* w;
+  const double* f;
+  double* hx;
+  double* hy;
+};
+struct RegTArgs {
+  Geom g;
+  int nf;
+  const double* w;
+  const int* key;
+  const double* hx;
+  const double* hy;
+  double shift_x;
+  const double* base;
+  double* out;
+};
+struct FsiRhsArgs {
+  Geom g;
+  int nf;
+  const int* tors_of;
+  const TorsBody* tbs;
+  int nt;
+  const double* th;
+  double dt, delta;
+  const double* w;
+  const int* key;
+  const double* qx;
+  const double* qy;
+  double q0x;
+  double* rhsf;
+};
+struct GemvArgs {
+  int nf;
+  const double* S;
+  const double* x;
+  double* y;
+};
+TabEntry<RegArgs> reg_entry(const Geom& g, const RegCSR& c, const double* w, const double* f, double* hx, double* hy);
+TabEntry<RegArgs> redist_entry(const Geom& g, const RegCSR& c, const double* w, const double* f, double* fx_out,
+                               double* fy_out);
+TabEntry<RegTArgs> regT_entry(const Geom& g, int nf, const double* w, const int* key, const double* hx,
+                              const double* hy, double shift_x, const double* base, double* out);
+TabEntry<FsiRhsArgs> fsi_rhs_entry(const Geom& g, int nf, const int* tors_of_entry, const TorsBody* tb, int nt,
+                                   const double* th, double dt, double delta, const double* w, const int* key,
+                                   const double* qx, const double* qy, double q0x, double* rhsf);
+TabEntry<GemvArgs> gemv_entry(int nf, const double* S, const double* x, double* y);
+const void* body_tab_kernel(int kind);
+const void* gemv_tab_kernel();
 // redistribute (src/operators_struct.F90:735-834) in CSR form: writes
 // finter = sum(w f)/sum(w) (if sum(w) > 1e-10, else 0) into touched faces.
 void redist_faces(hipStream_t st, const Geom& g, const RegCSR& c, const double* w, const double* f,

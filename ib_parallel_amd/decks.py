@@ -195,3 +195,36 @@ def tiny_case(m=32, n=24, mgridlev=2, torsional=True, tol=1e-12, dt=0.01, re=50.
         bodies.append(BodySpec(0.2 + r * math.cos(g), 0.05 + r * math.sin(g), fam=2, gamma0=g,
                                itheta=0.05, ktheta=0.2, ctheta=0.0))
     return p, bodies
+
+
+def ellipse_batch_cases(count=64, seed=20240611, m_range=(256, 512), levels=(3, 5), npts_range=(100, 400),
+                        aspect=(1.0, 4.0), aoa_deg=(0.0, 20.0), re_range=(100.0, 400.0), cfl=0.2, num_stat=1,
+                        tol=1e-10, **kw):
+    """Independent rigid ellipses for the batched form, drawn from a PCG64
+    stream: aspect ratio, angle of attack and Re uniform in their ranges; m
+    and n uniform over the multiples of 4 in m_range; mgridlev uniform in
+    levels; the body point count uniform in npts_range.  Each ellipse has
+    chord 1 and thickness 1/aspect, centred in its finest domain and pitched
+    nose-up by the angle of attack; its points are equispaced in arc length,
+    so the grid spacing is set to the point spacing (dx = perimeter / npts)
+    and dt = cfl * dx.  Returns a list of (params, [body]) pairs."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cases = []
+    for _ in range(count):
+        ar = rng.uniform(*aspect)
+        aoa = math.radians(rng.uniform(*aoa_deg))
+        re = rng.uniform(*re_range)
+        m = 4 * int(rng.integers(m_range[0] // 4, m_range[1] // 4 + 1))
+        n = 4 * int(rng.integers(m_range[0] // 4, m_range[1] // 4 + 1))
+        lev = int(rng.integers(levels[0], levels[1] + 1))
+        npts = int(rng.integers(npts_range[0], npts_range[1] + 1))
+        x, y = ellipse_arclength(npts, chord=1.0, thickness=1.0 / ar, x0=-0.5)
+        perim = float(np.sum(np.hypot(np.diff(np.append(x, x[0])), np.diff(np.append(y, y[0])))))
+        dx = perim / npts
+        c, s = math.cos(-aoa), math.sin(-aoa)
+        xb, yb = c * x - s * y, s * x + c * y
+        length = m * dx
+        p = make_params(m, n, lev, cfl * dx, re, length, 0.5 * length, 0.5 * n * dx, fsi_tol=tol, atol=tol,
+                        rtol=tol, num_stat=num_stat, **kw)
+        cases.append((p, [BodySpec(xb, yb, fam=1)]))
+    return cases

@@ -20,15 +20,20 @@ class IBError(RuntimeError):
 
 
 class Simulation:
-    def __init__(self, fns, params, bodies, prefix="ib_"):
+    def __init__(self, fns, params, bodies, prefix="ib_", handle=None):
+        """Create a simulation; with `handle`, wrap one that a batch owns."""
         self._f = fns
         self.prefix = prefix
-        self._bodies_keep = abi.make_bodies(bodies)
         self.params = params
-        h = C.c_void_p()
-        rc = fns["create"](C.byref(params), len(bodies), self._bodies_keep[0], C.byref(h))
-        if rc != abi.IB_OK:
-            raise IBError(rc, self._err(None))
+        self._owned = handle is None
+        if handle is None:
+            self._bodies_keep = abi.make_bodies(bodies)
+            h = C.c_void_p()
+            rc = fns["create"](C.byref(params), len(bodies), self._bodies_keep[0], C.byref(h))
+            if rc != abi.IB_OK:
+                raise IBError(rc, self._err(None))
+        else:
+            h = handle
         self.h = h
         sz = abi.IBSizes()
         fns["get_sizes"](h, C.byref(sz))
@@ -41,7 +46,8 @@ class Simulation:
     # -- lifetime -----------------------------------------------------------
     def close(self):
         if getattr(self, "h", None):
-            self._f["destroy"](self.h)
+            if self._owned:
+                self._f["destroy"](self.h)
             self.h = None
 
     def __del__(self):

@@ -62,7 +62,9 @@ typedef struct ib_params {
   int32_t fsi_max_count;   /* loop runs while count <= this (reference: 1000) */
   int32_t deltafnc_f32_literals; /* 1: exponent-less literals of deltafnc are
                                     REAL(4) as written (src/grid.F90:972-993) */
-  int32_t num_stat;        /* 1 would select the Cholesky path: not built */
+  int32_t num_stat;        /* 1: rigid bodies held fixed, the Schur system
+                              solved with the precomputed inverse of its
+                              Cholesky factor (src/timestepper.F90:457-484) */
   int32_t sub_domain;      /* 1 would select the sub-domain path: not built */
   int32_t launch_mode;     /* 0: GMRES iterations replayed as captured HIP
                               graphs (default); 1: plain stream launches */
@@ -189,6 +191,33 @@ int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable);
 int ib_schur_info(ib_handle* h, int32_t* mode, int32_t* nfaces, int64_t* green_builds);
 int ib_kernel_timing_read(ib_handle* h, int32_t kclass, int64_t* launches, double* total_ms,
                           double* total_bytes);
+
+/* --- batched form (no reference counterpart): many independent simulations
+ * of uneven grid size, level count and body size advanced one step
+ * together.  Simulation i is created from params[i] and the nbodies[i] body
+ * decks that follow those of simulations 0..i-1 in `bodies`.  When every
+ * simulation has rigid bodies and num_stat = 1, a step is one replay of a HIP
+ * graph whose launches each cover all simulations (mode 1, fused; with
+ * launch_mode 1 in params[0] the same launches go to the stream directly,
+ * without a graph); any other
+ * mix advances each simulation's plain step from its own host thread on its
+ * own stream (mode 2, concurrent).  Each simulation's results are those of
+ * its plain ib_advance. --- */
+typedef struct ib_batch ib_batch;
+int ib_batch_create(int32_t nsim, const ib_params* params, const int32_t* nbodies, const ib_body* bodies,
+                    ib_batch** out);
+int ib_batch_destroy(ib_batch* b);
+/* one step of every simulation; itimes[i] and infos[i] (either may be NULL)
+ * get simulation i's time index and step record.  Returns IB_OK or the
+ * first failing simulation's status. */
+int ib_batch_advance(ib_batch* b, int32_t* itimes, ib_step_info* infos);
+/* simulation i as a handle for every single-simulation entry point (state
+ * access, forces, pressure, operators); owned by the batch, not destroyed by
+ * the caller */
+int ib_batch_handle(ib_batch* b, int32_t i, ib_handle** h);
+int ib_batch_info(const ib_batch* b, int32_t* nsim, int32_t* mode);
+/* message of the last failing call on b (or of ib_batch_create when b is NULL) */
+int ib_batch_last_error(const ib_batch* b, char* buf, int32_t len);
 
 #ifdef __cplusplus
 }

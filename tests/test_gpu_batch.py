@@ -1,0 +1,108 @@
+"""Batched form (ib_batch_*): every simulation of a batch gives exactly the
+results of its own plain step, and the plain step is pinned to the oracle."""
+import numpy as np
+import pytest
+
+from conftest import rel
+from ib_parallel_amd import decks
+
+pytestmark = pytest.mark.gpu
+
+# small uneven instances: grids 32..72 cells, 2..4 levels, 20..60 points
+SMALL = dict(m_range=(32, 72), levels=(2, 4), npts_range=(20, 60))
+
+
+def state(sim):
+    out = []
+    for k in range(1, sim.L + 1):
+        lv = sim.level(k)
+        out += [lv["omega"], lv["s"], lv["qx"], lv["qy"], lv["rhs_old"]]
+    b = sim.body()
+    out += [b["f"], b["f_rdst"], b["xb"], b["yb"], b["theta"], b["thetad"], b["thetadd"]]
+    return out
+
+
+def same(a_list, b_list):
+    # identical values (signed zeros compare equal)
+    return all(a.shape == b.shape and np.array_equal(a, b) for a, b in zip(a_list, b_list))
+
+
+def test_fused_batch_equals_plain_steps():
+    from ib_parallel_amd import GpuBatch, GpuSim
+    cases = decks.ellipse_batch_cases(count=5, seed=7, **SMALL)
+    assert len({(p.m, p.n, p.mgridlev) for p, _ in cases}) == 5     # uneven sizes
+    batch = GpuBatch(cases)
+    assert batch.mode == 1
+    plain = [GpuSim(p, b) for p, b in cases]
+    for step in range(1, 4):
+        infos = batch.advance()
+        for s in plain:
+            s.advance()
+        assert [i.itime for i in infos] == [step] * 5 and all(i.status == 0 for i in infos)
+        for i in range(5):
+            assert same(state(batch.sims[i]), state(plain[i])), (step, i)
+    batch.close()
+
+
+def test_fused_batch_matches_oracle():
+    from ib_parallel_amd import GpuBatch
+    from oracle import binding as ob
+    cases = decks.ellipse_batch_cases(count=3, seed=11, **SMALL)
+    batch = GpuBatch(cases)
+    orcs = [ob.OracleSim(p, b) for p, b in cases]
+    for _ in range(3):
+        batch.advance()
+        for o in orcs:
+            o.advance()
+    for g, o in zip(batch.sims, orcs):
+        for k in range(1, g.L + 1):
+            assert rel(g.level(k)["omega"], o.level(k)["omega"]) < 1e-10
+        assert rel(g.body()["f"], o.body()["f"]) < 1e-8
+    batch.close()
+
+
+def test_fused_batch_restart_first_step_flag():
+    # a simulation resumed at istart > 0 takes the AB2 branch on its first
+    # batched step, one at istart = 0 the Euler branch, in the same launch
+    from ib_parallel_amd import GpuBatch, GpuSim
+    cases = decks.ellipse_batch_cases(count=2, seed=3, **SMALL)
+    ref = GpuSim(*cases[1])
+    ref.advance()
+    snap = [ref.level(k) for k in range(1, ref.L + 1)]
+    bref = ref.body()
+    p1 = cases[1][0]
+    p1.istart = 1
+    batch = GpuBatch(cases)
+    for k, lv in enumerate(snap, start=1):
+        batch.sims[1].set_level(k, **lv)
+    batch.sims[1].set_body(f=bref["f"])
+    infos = batch.advance()
+    assert infos[0].itime == 1 and infos[1].itime == 2
+    ref.advance()
+    assert same(state(batch.sims[1]), state(ref))
+    batch.close()
+
+
+def test_concurrent_batch_equals_plain_steps():
+    # torsional bodies: data-dependent FSI iterations, one host thread each
+    from ib_parallel_amd import GpuBatch, GpuSim
+    cases = [decks.tiny_case(m=32 + 8 * i, n=24 + 4 * i, mgridlev=2 + i % 2, tol=1e-12) for i in range(3)]
+    batch = GpuBatch(cases)
+    assert batch.mode == 2
+    plain = [GpuSim(p, b) for p, b in cases]
+    for _ in range(2):
+        infos = batch.advance()
+        refs = [s.advance() for s in plain]
+        for i in range(3):
+            assert (infos[i].ksp_iter, infos[i].fsi_iter) == (refs[i].ksp_iter, refs[i].fsi_iter)
+            assert same(state(batch.sims[i]), state(plain[i])), i
+    batch.close()
+
+
+def test_batch_create_errors():
+    from ib_parallel_amd import GpuBatch, IBError, abi
+    good = decks.ellipse_batch_cases(count=1, seed=1, **SMALL)[0]
+    bad = (abi.make_params(30, 32, 2, 0.01, 100, 1.0, 0.5, 0.5), [])
+    with pytest.raises(IBError) as e:
+        GpuBatch([good, bad])
+    assert e.value.code == abi.IB_E_INVAL and "simulation 1" in str(e.value)
