@@ -142,7 +142,7 @@ def run_batched(args):
     serially.  Both are whole-batch rates: sum of m*n*mgridlev over the
     simulations per step."""
     from ib_parallel_amd import GpuBatch, decks
-    cases = decks.ellipse_batch_cases(count=args.batch)
+    cases = decks.ellipse_batch_cases(count=args.batch, launch_mode=args.batch_launch_mode)
     t0 = time.perf_counter()
     batch = GpuBatch(cases)
     t_create = time.perf_counter() - t0
@@ -244,6 +244,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=64, help="simulations in the batched measurement (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--batch-only", action="store_true", help="only the batched measurement (one JSON object)")
+    ap.add_argument("--batch-launch-mode", type=int, default=0,
+                    help="1: the batched step's launches go to the stream directly instead of a graph replay")
     ap.add_argument("--pmc-copy-mb", type=float, default=0.0,
                     help="run a device-to-device copy of this size first (calibrates counter runs)")
     args = ap.parse_args()
@@ -254,6 +257,9 @@ def main():
 
     if args.pmc_copy_mb > 0:
         copy_calibration(args.pmc_copy_mb)
+    if args.batch_only:
+        print(json.dumps({"batched": run_batched(args)}), flush=True)
+        return
     g = run_gpu(args)
     sim = g["sim"]
     secs = g["secs"]
