@@ -122,62 +122,101 @@ __device__ __forceinline__ void stage_fixed(const double2* __restrict__ in, doub
   }
 }
 
-// A stage of any other radix R (a prime > 5): the inputs of sub-transform j
-// first take their inter-stage twiddles in place, then each thread forms a
-// tile of 2 sub-transforms x 4 outputs as a direct length-R DFT, so every
-// input and every DFT coefficient read from LDS feeds several products.
+// A stage of any other radix R (an odd prime >= 7) as a direct length-R DFT
+// using its conjugate symmetry.  Pass 1 applies the inter-stage twiddles and
+// folds each input pair in place: p_r = x_r + x_(R-r) at position r and
+// m_r = x_r - x_(R-r) at position R-r (r = 1..H, H = (R-1)/2).  Pass 2 forms
+// U_q = sum p_r cos(2 pi rq/R) and V_q = sum m_r sin(2 pi rq/R), which give
+// both X_q = x_0 + U_q - i V_q and X_(R-q) = x_0 + U_q + i V_q: a quarter of
+// the real products of the plain sum.  Each thread holds a tile of 2
+// sub-transforms x 4 values of q, so every LDS read feeds several products.
 __device__ void stage_generic(double2* __restrict__ in, double2* __restrict__ out, int nf, int L, int Ns, int R,
                               const double2* __restrict__ W) {
   const int LR = L / R;
   const int tws = LR / Ns;
-  if (Ns > 1) {
-    for (int idx = threadIdx.x; idx < nf * L; idx += blockDim.x) {
-      const int rem = idx % L;
-      const int r = rem / LR, j = rem - r * LR;
-      const int kr = (j % Ns) * r;
-      if (kr) in[idx] = cmul(in[idx], W[kr * tws]);
+  const int H = (R - 1) / 2;
+  for (int idx = threadIdx.x; idx < nf * LR * H; idx += blockDim.x) {
+    const int s = idx / (LR * H);
+    const int u = idx - s * (LR * H);
+    const int r = 1 + u / LR, j = u % LR;
+    double2* x = in + s * L + j;
+    double2 xa = x[r * LR], xb = x[(R - r) * LR];
+    const int k = j % Ns;
+    if (k) {
+      xa = cmul(xa, W[k * r * tws]);
+      xb = cmul(xb, W[k * (R - r) * tws]);
     }
-    __syncthreads();
+    x[r * LR] = cadd(xa, xb);
+    x[(R - r) * LR] = csub(xa, xb);
   }
-  const int nj = (LR + 1) / 2, nq = (R + 3) / 4;
+  __syncthreads();
+  const int nj = (LR + 1) / 2, nq = (H + 4) / 4;  // q = 0..H in groups of 4
   const int ntile = nf * nj * nq;
-  for (int t = threadIdx.x; t < ntile; t += blockDim.x) {
+  // with few tiles, 2 or 4 adjacent lanes share a tile's sum over r (terms
+  // r = 1 + part, 1 + part + split, ...) and combine it by lane shuffles
+  int lsplit = 0;
+  while (lsplit < 2 && (ntile << (lsplit + 1)) <= (int)blockDim.x) ++lsplit;
+  const int split = 1 << lsplit;
+  for (int tt = threadIdx.x; tt < (ntile << lsplit); tt += blockDim.x) {
+    const int t = tt >> lsplit, part = tt & (split - 1);
     const int s = t / (nj * nq);
     const int u = t - s * (nj * nq);
     const int qt = u / nj;
     const int j0 = 2 * (u - qt * nj), q0 = 4 * qt;
     const int j1 = min(j0 + 1, LR - 1);  // an odd LR repeats the last column (not stored)
     const double2* __restrict__ x = in + s * L;
-    // coefficient of term r, output q: w_R^(rq) = W[((r q) mod R) LR], kept as a
-    // running index below L per output
+    // cos/sin of 2 pi rq/R from W[((r q) mod R) LR] = (cos, -sin), the index
+    // kept running below L per value of q
     int bq[4], dq[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      dq[c] = min(q0 + c, R - 1) * LR;
-      bq[c] = 0;
+      const int qL = min(q0 + c, H) * LR;  // below L
+      bq[c] = (int)(((long)(1 + part) * qL) % L);
+      dq[c] = (int)(((long)split * qL) % L);
     }
-    double2 acc0[4], acc1[4];
+    double2 U0[4], V0[4], U1[4], V1[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc0[c] = acc1[c] = make_double2(0.0, 0.0);
-    for (int r = 0; r < R; ++r) {
-      const double2 xa = x[j0 + r * LR], xb = x[j1 + r * LR];
+    for (int c = 0; c < 4; ++c) U0[c] = V0[c] = U1[c] = V1[c] = make_double2(0.0, 0.0);
+    for (int r = 1 + part; r <= H; r += split) {
+      const double2 pa = x[j0 + r * LR], ma = x[j0 + (R - r) * LR];
+      const double2 pb = x[j1 + r * LR], mb = x[j1 + (R - r) * LR];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const double2 w = W[bq[c]];
-        acc0[c] = cadd(acc0[c], cmul(xa, w));
-        acc1[c] = cadd(acc1[c], cmul(xb, w));
+        const double cs = w.x, sn = -w.y;
+        U0[c] = make_double2(U0[c].x + pa.x * cs, U0[c].y + pa.y * cs);
+        V0[c] = make_double2(V0[c].x + ma.x * sn, V0[c].y + ma.y * sn);
+        U1[c] = make_double2(U1[c].x + pb.x * cs, U1[c].y + pb.y * cs);
+        V1[c] = make_double2(V1[c].x + mb.x * sn, V1[c].y + mb.y * sn);
         bq[c] += dq[c];
         if (bq[c] >= L) bq[c] -= L;
       }
     }
+    for (int off = 1; off < split; off <<= 1) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        U0[c].x += __shfl_xor(U0[c].x, off); U0[c].y += __shfl_xor(U0[c].y, off);
+        V0[c].x += __shfl_xor(V0[c].x, off); V0[c].y += __shfl_xor(V0[c].y, off);
+        U1[c].x += __shfl_xor(U1[c].x, off); U1[c].y += __shfl_xor(U1[c].y, off);
+        V1[c].x += __shfl_xor(V1[c].x, off); V1[c].y += __shfl_xor(V1[c].y, off);
+      }
+    }
+    if (part) continue;
+    const double2 x0a = x[j0], x0b = x[j1];
     double2* __restrict__ o = out + s * L;
     const int k0 = j0 % Ns, k1 = (j0 + 1) % Ns;
+    const bool second = j0 + 1 < LR;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const int q = q0 + c;
-      if (q < R) {
-        o[(j0 - k0) * R + k0 + q * Ns] = acc0[c];
-        if (j0 + 1 < LR) o[(j0 + 1 - k1) * R + k1 + q * Ns] = acc1[c];
+      if (q <= H) {
+        const int ba = (j0 - k0) * R + k0, bb = (j0 + 1 - k1) * R + k1;
+        o[ba + q * Ns] = make_double2(x0a.x + U0[c].x + V0[c].y, x0a.y + U0[c].y - V0[c].x);
+        if (second) o[bb + q * Ns] = make_double2(x0b.x + U1[c].x + V1[c].y, x0b.y + U1[c].y - V1[c].x);
+        if (q > 0) {
+          o[ba + (R - q) * Ns] = make_double2(x0a.x + U0[c].x - V0[c].y, x0a.y + U0[c].y + V0[c].x);
+          if (second) o[bb + (R - q) * Ns] = make_double2(x0b.x + U1[c].x - V1[c].y, x0b.y + U1[c].y + V1[c].x);
+        }
       }
     }
   }
