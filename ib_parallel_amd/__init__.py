@@ -38,8 +38,8 @@ def library():
 class GpuSim(Simulation):
     """Simulation whose state lives in HBM of the current HIP device."""
 
-    def __init__(self, params, bodies):
-        super().__init__(load(), params, bodies, prefix="ib_")
+    def __init__(self, params, bodies, subdomain=None):
+        super().__init__(load(), params, bodies, prefix="ib_", subdomain=subdomain)
 
     def schur_info(self):
         """(form, face-set size, G computations): form 1 matrix-free, 2 assembled, 3 face-space."""

@@ -47,6 +47,24 @@ class IBBody(C.Structure):
     ]
 
 
+class IBSubdomain(C.Structure):
+    _fields_ = [
+        ("sdxi", C.c_double), ("sdxe", C.c_double), ("sdyi", C.c_double), ("sdye", C.c_double),
+        ("delta_sd", C.c_double), ("full_rectangular", C.c_int32), ("precomputed", C.c_int32),
+        ("droptol", C.c_double), ("interp_points", C.c_int32), ("support_nz", C.c_int32),
+    ]
+
+
+def make_subdomain(sdxi, sdxe, sdyi, sdye, delta_sd, full_rectangular=0, precomputed=0, droptol=0.0,
+                   interp_points=0, support_nz=0):
+    """Sub-domain parameters of ib.inp (0 in the last three: the reference's values)."""
+    sd = IBSubdomain()
+    sd.sdxi, sd.sdxe, sd.sdyi, sd.sdye, sd.delta_sd = sdxi, sdxe, sdyi, sdye, delta_sd
+    sd.full_rectangular, sd.precomputed = int(full_rectangular), int(precomputed)
+    sd.droptol, sd.interp_points, sd.support_nz = droptol, interp_points, support_nz
+    return sd
+
+
 class IBStepInfo(C.Structure):
     _fields_ = [
         ("itime", C.c_int32), ("ksp_iter", C.c_int32), ("fsi_iter", C.c_int32),
@@ -89,6 +107,10 @@ SIGNATURES = {
     "pressure": (C.c_int, [_P, _D]),
     "op_nonlinear_rot": (C.c_int, [_P, C.c_int32, _D, _D, _D, _D, _D]),
     "op_gmres": (C.c_int, [_P, _D, _D, _I]),
+    "create_subdomain": (C.c_int, [C.POINTER(IBParams), C.POINTER(IBSubdomain), C.c_int32, C.POINTER(IBBody),
+                                   C.POINTER(_P)]),
+    "subdomain_info": (C.c_int, [_P, _I, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "op_subdomain_matrix": (C.c_int, [_P, _D]),
 }
 
 GPU_ONLY = {

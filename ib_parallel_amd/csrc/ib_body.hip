@@ -476,7 +476,7 @@ __device__ __forceinline__ void FsiRhsOp::tile(const FsiRhsArgs& a, int bx, int,
     v = BR(b, k - b.off + 1, si, ci) * sc;
   }
   double prod = 0.0;
-  if (lane < kTaps) {
+  if (key && lane < kTaps) {  // no key list: the structural part only
     const int kk = key[k * kTaps + lane];
     const double u = kk < g.Fx ? qx[kk] + q0x : qy[kk - g.Fx] + 0.0;
     prod = w[k * kTaps + lane] * u;

@@ -147,6 +147,21 @@ struct ib_handle {
   // factored a_times matrix, and a torsional map with no torsional entries
   DBuf<double> Ainv;
   DBuf<int> no_tors;
+  // sub-domain approach (sub_domain = T): points, their weights and tap
+  // list, B by columns after the drop tolerance, and P with its transpose
+  bool sd_on = false;
+  ib_subdomain sdc{};
+  int sd_np = 4, sd_nx = 0, sd_ny = 0, nr_sd = 0, nb_sd = 0, nf_sd = 0, sd_used = 0;
+  std::vector<double> sd_x, sd_y;
+  std::vector<int> h_fam, h_npts;  // per body
+  DBuf<double> w_sd, rsd, h0sd;
+  RegCSR csr_sd;  // csr_sd.key: the face of each sub-domain tap
+  DBuf<long> bptr;
+  DBuf<int> brow;
+  DBuf<double> bval;
+  int64_t sd_nnz_eval = 0, sd_nnz_kept = 0;
+  DBuf<int> d_pidx, d_used, d_uptr, d_uj;
+  DBuf<double> d_pw, d_uw;
   // pressure post-processing (allocated on first use): cell-grid DST plans
   // and table, corner products (two levels), rhs, solution, scratch
   bool pres_ready = false;
@@ -259,7 +274,7 @@ struct ib_handle {
 
   // The column-batch buffers are only needed while columns are formed;
   // forms whose matrix never changes give them back after set-up.
-  bool keeps_columns() const { return facemode || (assembled && ntors > 0); }
+  bool keeps_columns() const { return !sd_on && (facemode || (assembled && ntors > 0)); }
   void trim_columns() {
     if (keeps_columns() || !Vb.p) return;
     sync();
@@ -540,6 +555,255 @@ struct ib_handle {
     return IB_OK;
   }
 
+  // ------------------------------------------------ sub-domain approach
+  // Set-up of src/grid.F90:320-348 (points), :628-692 (E on the sub-domain),
+  // src/variables.F90:425-514 with src/grid.F90:696-751 (which rows of each
+  // column are evaluated) and src/timestepper.F90:602-770 (the columns,
+  // a_times_sd of a unit entry, and the drop tolerance).
+  int sd_setup(const ib_body* bodies) {
+    const ib_subdomain& c = sdc;
+    if (!(c.delta_sd > 0.0) || !(c.sdxe >= c.sdxi) || !(c.sdye >= c.sdyi))
+      return fail(IB_E_INVAL, "sub-domain: need delta_sd > 0 and a non-empty rectangle");
+    const int s2n = (int)std::lround(std::sqrt((double)sd_np));
+    if (sd_np < 1 || s2n * s2n != sd_np) return fail(IB_E_INVAL, "sub-domain: interp_points must be a square");
+    sd_x.clear();
+    sd_y.clear();
+    nr_sd = 0;
+    if (!c.full_rectangular) {
+      int nrig = 0;
+      for (int b = 0; b < nbodies; ++b)
+        if (h_fam[b] == 1) {
+          ++nrig;
+          sd_x.insert(sd_x.end(), bodies[b].x, bodies[b].x + bodies[b].npts);
+          sd_y.insert(sd_y.end(), bodies[b].y, bodies[b].y + bodies[b].npts);
+        }
+      if (nrig > 1)
+        return fail(IB_E_UNSUPPORTED, "sub-domain with body points: the reference includes one rigid body only");
+      nr_sd = (int)sd_x.size();
+    }
+    sd_nx = (int)std::lround((c.sdxe - c.sdxi) / c.delta_sd + 1.0);
+    sd_ny = (int)std::lround((c.sdye - c.sdyi) / c.delta_sd + 1.0);
+    for (int j = 1; j <= sd_ny; ++j)
+      for (int i = 1; i <= sd_nx; ++i) {
+        sd_x.push_back(c.sdxi + (i - 1) * c.delta_sd);
+        sd_y.push_back(c.sdyi + (j - 1) * c.delta_sd);
+      }
+    nb_sd = (int)sd_x.size();
+    nf_sd = 2 * nb_sd;
+    // E on the sub-domain: the body weight kernel over the sub-domain entries
+    {
+      std::vector<double> ex(nf_sd), ey(nf_sd);
+      std::vector<int> ec(nf_sd);
+      for (int k = 0; k < nf_sd; ++k) {
+        ex[k] = sd_x[k % nb_sd];
+        ey[k] = sd_y[k % nb_sd];
+        ec[k] = k < nb_sd ? 1 : 2;
+      }
+      DBuf<double> dx_, dy_;
+      DBuf<int> dc_;
+      dx_.alloc(nf_sd); dy_.alloc(nf_sd); dc_.alloc(nf_sd);
+      IB_HIP(hipMemcpy(dx_.p, ex.data(), sizeof(double) * nf_sd, hipMemcpyHostToDevice));
+      IB_HIP(hipMemcpy(dy_.p, ey.data(), sizeof(double) * nf_sd, hipMemcpyHostToDevice));
+      IB_HIP(hipMemcpy(dc_.p, ec.data(), sizeof(int) * nf_sd, hipMemcpyHostToDevice));
+      w_sd.alloc((size_t)nf_sd * kTaps);
+      csr_sd.alloc(nf_sd * kTaps, g.Fx + g.Fy);
+      IB_HIP(hipMemsetAsync(err_flag.p, 0, sizeof(int), st));
+      setup_reg(st, g, nf_sd, dx_, dy_, dc_, delta, offx, offy, prm.deltafnc_f32_literals, w_sd, csr_sd.key, err_flag);
+      if (read_err_flag()) return fail(IB_E_INVAL, "sub-domain point regularisation stencil leaves the grid");
+      csr_sd.build(st);
+    }
+    // rows of each column: the E E^T windows (half-width sn) of two entries
+    // share a column of the window matrix exactly when both entries have the
+    // same component and their stencil cells differ by at most 2 sn in x and
+    // in y, as long as no window reaches the edge of the grid (checked)
+    const int sn = c.support_nz;
+    std::vector<int> bix(nb_sd), biy(nb_sd);
+    for (int q = 0; q < nb_sd; ++q) {
+      bix[q] = (int)((sd_x[q] + offx) / delta) + 1;
+      biy[q] = (int)((sd_y[q] + offy) / delta) + 1;
+      if (bix[q] - sn - 1 < 0 || bix[q] + sn - 1 > g.m - 1 || biy[q] - sn - 1 < 0 || biy[q] + sn - 1 > g.n - 1)
+        return fail(IB_E_UNSUPPORTED, "sub-domain: a sparsity window reaches the edge of the grid");
+    }
+    std::vector<std::vector<int>> cell((size_t)(g.m + 2) * (g.n + 2));
+    for (int q = 0; q < nb_sd; ++q) cell[(size_t)biy[q] * (g.m + 2) + bix[q]].push_back(q);
+    std::vector<long> colptr(nf_sd + 1, 0);
+    std::vector<std::vector<int>> nbr(nb_sd);
+    for (int q = 0; q < nb_sd; ++q) {
+      std::vector<int>& N = nbr[q];
+      for (int jj = std::max(0, biy[q] - 2 * sn); jj <= std::min(g.n + 1, biy[q] + 2 * sn); ++jj)
+        for (int ii = std::max(0, bix[q] - 2 * sn); ii <= std::min(g.m + 1, bix[q] + 2 * sn); ++ii) {
+          const std::vector<int>& C = cell[(size_t)jj * (g.m + 2) + ii];
+          N.insert(N.end(), C.begin(), C.end());
+        }
+      std::sort(N.begin(), N.end());
+    }
+    for (int k = 0; k < nf_sd; ++k) colptr[k + 1] = colptr[k] + 2 * (long)nbr[k % nb_sd].size();
+    const long total = colptr[nf_sd];
+    std::vector<int> rows(total);
+    for (int k = 0; k < nf_sd; ++k) {
+      const std::vector<int>& N = nbr[k % nb_sd];
+      long o = colptr[k];
+      for (int q : N) rows[o++] = q;
+      for (int q : N) rows[o++] = q + nb_sd;
+    }
+    nbr.clear();
+    cell.clear();
+    // the columns: batched unit sources through ainv + vort2flux, the level-1
+    // flux regularised at the listed rows
+    DBuf<long> dptr;
+    DBuf<int> drows;
+    DBuf<double> dvals;
+    dptr.alloc(nf_sd + 1);
+    drows.alloc(std::max<long>(total, 1));
+    dvals.alloc(std::max<long>(total, 1));
+    IB_HIP(hipMemcpy(dptr.p, colptr.data(), sizeof(long) * (nf_sd + 1), hipMemcpyHostToDevice));
+    IB_HIP(hipMemcpy(drows.p, rows.data(), sizeof(int) * total, hipMemcpyHostToDevice));
+    alloc_assembly(nf_sd);
+    for (int c0 = 0; c0 < nf_sd; c0 += ncol) {
+      const int nc = std::min(ncol, nf_sd - c0);
+      long sbc = 0;
+      const double* bc = column_batch(c0, nc, w_sd, csr_sd.key, &sbc);
+      sd_columns(st, g, c0, nc, colptr[c0 + nc] - colptr[c0], dptr, drows, w_sd, csr_sd.key, Sb, g.P, bc, sbc, dvals);
+    }
+    std::vector<double> vals(total);
+    IB_HIP(hipMemcpyAsync(vals.data(), dvals.p, sizeof(double) * total, hipMemcpyDeviceToHost, st));
+    sync();
+    // drop tolerance: keep |B(r, j)| > droptol * B(j, j), in row order
+    std::vector<long> kptr(nf_sd + 1, 0);
+    std::vector<int> krow;
+    std::vector<double> kval;
+    krow.reserve(total / 2);
+    kval.reserve(total / 2);
+    for (int k = 0; k < nf_sd; ++k) {
+      double diag = 0.0;
+      for (long a = colptr[k]; a < colptr[k + 1]; ++a)
+        if (rows[a] == k) diag = vals[a];
+      for (long a = colptr[k]; a < colptr[k + 1]; ++a)
+        if (std::fabs(vals[a]) > c.droptol * diag) {
+          krow.push_back(rows[a]);
+          kval.push_back(vals[a]);
+        }
+      kptr[k + 1] = (long)krow.size();
+    }
+    sd_nnz_eval = total;
+    sd_nnz_kept = (int64_t)krow.size();
+    bptr.alloc(nf_sd + 1);
+    brow.alloc(std::max<size_t>(krow.size(), 1));
+    bval.alloc(std::max<size_t>(kval.size(), 1));
+    IB_HIP(hipMemcpy(bptr.p, kptr.data(), sizeof(long) * (nf_sd + 1), hipMemcpyHostToDevice));
+    if (!krow.empty()) {
+      IB_HIP(hipMemcpy(brow.p, krow.data(), sizeof(int) * krow.size(), hipMemcpyHostToDevice));
+      IB_HIP(hipMemcpy(bval.p, kval.data(), sizeof(double) * kval.size(), hipMemcpyHostToDevice));
+    }
+    rsd.alloc(nf_sd);
+    h0sd.alloc(nf_sd);
+    d_pidx.alloc((size_t)sd_np * std::max(nf, 1));
+    d_pw.alloc((size_t)sd_np * std::max(nf, 1));
+    return sd_build_interp();
+  }
+
+  // setup_reg_subdomain_indices (src/grid.F90:777-951) for the current body
+  // positions: P (np sub-domain entries and weights per force entry,
+  // bilinear hat weights, or the entry itself for the rigid body's points
+  // when they are sub-domain points) and its transpose over the used
+  // entries, for P^T f in ascending force order
+  int sd_build_interp() {
+    const ib_subdomain& c = sdc;
+    std::vector<double> hxb(nf), hyb(nf);
+    IB_HIP(hipMemcpyAsync(hxb.data(), xb.p, sizeof(double) * nf, hipMemcpyDeviceToHost, st));
+    IB_HIP(hipMemcpyAsync(hyb.data(), yb.p, sizeof(double) * nf, hipMemcpyDeviceToHost, st));
+    sync();
+    const int s = (int)std::lround(std::sqrt((double)sd_np));
+    const int i1 = (sd_np % 2 == 0) ? s / 2 + 1 - s : (s - 1) / 2 + 1 - s;
+    std::vector<int> pidx((size_t)sd_np * nf), nn(sd_np), ll(s), pp(s);
+    std::vector<double> pw((size_t)sd_np * nf), dist(sd_np), wv(sd_np);
+    for (int j = 0; j < nf; ++j) {
+      const int comp = h_ccomp[j];
+      const double indx = (hxb[j] - c.sdxi) / c.delta_sd + 1.0, indy = (hyb[j] - c.sdyi) / c.delta_sd + 1.0;
+      for (int a = 0; a < s; ++a) {
+        ll[a] = (sd_np % 2 == 0 ? (int)indx : (int)std::lround(indx)) + i1 + a;
+        pp[a] = (sd_np % 2 == 0 ? (int)indy : (int)std::lround(indy)) + i1 + a;
+      }
+      if (!c.full_rectangular && h_fam[h_cbody[j]] == 1) {
+        const int j1 = j + 1, nr = h_npts[h_cbody[j]];
+        for (int i = 0; i < sd_np; ++i) nn[i] = comp == 1 ? j1 + 1 : j1 + 1 - nr;
+        nn[0] = comp == 1 ? j1 : j1 - nr;
+        for (int i = 0; i < sd_np; ++i) dist[i] = 1.0;
+        dist[0] = 0.0;
+      } else {
+        int i = 0;
+        for (int a = 0; a < s; ++a)
+          for (int b = 0; b < s; ++b) {
+            if (ll[a] < 1) return fail(IB_E_INVAL, "Body has exited the subdomain from left");
+            if (ll[a] > sd_nx) return fail(IB_E_INVAL, "Body has exited the subdomain from right");
+            if (pp[b] < 1) return fail(IB_E_INVAL, "Body has exited the subdomain from bottom");
+            if (pp[b] > sd_ny) return fail(IB_E_INVAL, "Body has exited the subdomain from top");
+            nn[i] = nr_sd + (pp[b] - 1) * sd_nx + ll[a];
+            const double dx = hxb[j] - sd_x[nn[i] - 1], dy = hyb[j] - sd_y[nn[i] - 1];
+            dist[i] = std::sqrt(dx * dx + dy * dy);
+            ++i;
+          }
+      }
+      bool exact = false;
+      for (int i = 0; i < sd_np; ++i) {
+        const int q = nn[i] - 1;
+        if (q < 0 || q >= nb_sd) return fail(IB_E_INVAL, "sub-domain: interpolation point out of range");
+        const double hx_ = std::fabs(sd_x[q] - hxb[j]) / c.delta_sd, hy_ = std::fabs(sd_y[q] - hyb[j]) / c.delta_sd;
+        const double fxv = (hx_ <= 1.0 ? 1.0 - hx_ : 0.0) / c.delta_sd, fyv = (hy_ <= 1.0 ? 1.0 - hy_ : 0.0) / c.delta_sd;
+        wv[i] = c.delta_sd * c.delta_sd * fxv * fyv;
+        if (dist[i] == 0.0) {
+          wv[i] = 1.0;
+          exact = true;
+        }
+      }
+      for (int i = 0; i < sd_np; ++i) {
+        if (exact && dist[i] != 0.0) wv[i] = 0.0;
+        pidx[(size_t)j * sd_np + i] = nn[i] - 1 + (comp == 1 ? 0 : nb_sd);
+        pw[(size_t)j * sd_np + i] = wv[i];
+      }
+    }
+    // transpose over the used entries (the last of equal indices of an entry wins)
+    std::vector<int> used(pidx);
+    std::sort(used.begin(), used.end());
+    used.erase(std::unique(used.begin(), used.end()), used.end());
+    std::vector<std::vector<std::pair<int, double>>> tr(used.size());
+    for (int j = 0; j < nf; ++j) {
+      for (int i = 0; i < sd_np; ++i) {
+        const int q = (int)(std::lower_bound(used.begin(), used.end(), pidx[(size_t)j * sd_np + i]) - used.begin());
+        auto& T = tr[q];
+        if (!T.empty() && T.back().first == j) T.back().second = pw[(size_t)j * sd_np + i];
+        else T.push_back({j, pw[(size_t)j * sd_np + i]});
+      }
+    }
+    std::vector<int> uptr(used.size() + 1, 0), uj;
+    std::vector<double> uw;
+    for (size_t q = 0; q < used.size(); ++q) {
+      for (auto& e : tr[q]) {
+        uj.push_back(e.first);
+        uw.push_back(e.second);
+      }
+      uptr[q + 1] = (int)uj.size();
+    }
+    sd_used = (int)used.size();
+    d_used.alloc(std::max<size_t>(used.size(), 1));
+    d_uptr.alloc(used.size() + 1);
+    d_uj.alloc(std::max<size_t>(uj.size(), 1));
+    d_uw.alloc(std::max<size_t>(uw.size(), 1));
+    IB_HIP(hipMemcpyAsync(d_pidx.p, pidx.data(), sizeof(int) * pidx.size(), hipMemcpyHostToDevice, st));
+    IB_HIP(hipMemcpyAsync(d_pw.p, pw.data(), sizeof(double) * pw.size(), hipMemcpyHostToDevice, st));
+    IB_HIP(hipMemcpyAsync(d_used.p, used.data(), sizeof(int) * used.size(), hipMemcpyHostToDevice, st));
+    IB_HIP(hipMemcpyAsync(d_uptr.p, uptr.data(), sizeof(int) * uptr.size(), hipMemcpyHostToDevice, st));
+    if (!uj.empty()) {
+      IB_HIP(hipMemcpyAsync(d_uj.p, uj.data(), sizeof(int) * uj.size(), hipMemcpyHostToDevice, st));
+      IB_HIP(hipMemcpyAsync(d_uw.p, uw.data(), sizeof(double) * uw.size(), hipMemcpyHostToDevice, st));
+    }
+    sync();  // the host vectors go out of scope
+    return IB_OK;
+  }
+
+  // lumat for the current P and theta into Sd (the matrix GMRES runs on)
+  void sd_matrix() { sd_lumat(st, nf, sd_np, d_pidx, d_pw, bptr, brow, bval, tors_of, tb, th, dt, delta, Sd); }
+
   // step 2: FSI fixed point (src/timestepper.F90:249-456).  Returns the
   // step status; IB_E_INVAL when a body stencil leaves the grid.
   int fsi_fixed_point() {
@@ -553,15 +817,36 @@ struct ib_handle {
     }
     while (err_glob >= prm.fsi_tol && fsi_count <= prm.fsi_max_count) {
       ++fsi_count;
-      fsi_rhs(st, g, nf, tors_of, tb, ntors, th, dt, delta, w, csr.key, qx, qy, q0x[0], rhsf);
       int its = 0;
-      int rc = gmres_dev(rhsf, f, &its);
-      ksp_iter += its;
+      int rc;
+      if (!sd_on) {
+        fsi_rhs(st, g, nf, tors_of, tb, ntors, th, dt, delta, w, csr.key, qx, qy, q0x[0], rhsf);
+        rc = gmres_dev(rhsf, f, &its);
+        ksp_iter += its;
+      } else {
+        // src/timestepper.F90:368-405: rhsf = compute_rhs + P E_sd (q + q0),
+        // lumat for the current P, GMRES on it (its iterations are not
+        // added to ksp_iter in the reference)
+        fsi_rhs(st, g, nf, tors_of, tb, ntors, th, dt, delta, w, nullptr, qx, qy, q0x[0], rhsf);
+        regT(st, g, nf_sd, w_sd, csr_sd.key, qx, qy, q0x[0], nullptr, rsd);
+        sd_interp(st, nf, sd_np, d_pidx, d_pw, rsd, rhsf);
+        sd_matrix();
+        rc = gmres_dev(rhsf, f, &its);
+      }
       if (rc != IB_OK) status = rc;
       // redistribute (output only)
       redist_faces(st, g, csr, w, f, hx, hy);
       regT(st, g, nf, w, csr.key, hx, hy, 0.0, nullptr, f_rdst);
-      if (ntors > 0) {
+      if (ntors > 0 && sd_on) {
+        // the body's E stays as set up (the reference updates only P on the
+        // sub-domain, src/timestepper.F90:449-450)
+        body_dx(st, ntors, tb, th, dt, delta, f, xb, yb, dabs);
+        IB_HIP(hipMemcpyAsync(hbuf, dabs.p, sizeof(double) * ntors, hipMemcpyDeviceToHost, st));
+        const int ri = sd_build_interp();
+        if (ri != IB_OK) return ri;
+        err_glob = 0.0;
+        for (int t = 0; t < ntors; ++t) err_glob = std::max(err_glob, hbuf[t]);
+      } else if (ntors > 0) {
         body_dx(st, ntors, tb, th, dt, delta, f, xb, yb, dabs);
         rebuild_weights();
         IB_HIP(hipMemcpyAsync(hbuf, dabs.p, sizeof(double) * ntors, hipMemcpyDeviceToHost, st));
@@ -722,9 +1007,22 @@ struct ib_handle {
       if (status == IB_E_INVAL) return status;
     }
 
-    // step 3: corrector (src/timestepper.F90:488-553)
-    reg(st, g, csr, w, f, hx, hy);
-    ainv_dev(nullptr, omega, true);
+    // step 3: corrector (src/timestepper.F90:488-553); on the sub-domain the
+    // source is E_sd^T P^T f (:500-510)
+    if (sd_on) {
+      fill(st, hx, g.Fx, 0.0);
+      fill(st, hy, g.Fy, 0.0);
+      fill(st, h0sd, nf_sd, 0.0);
+      sd_ptf(st, sd_used, d_used, d_uptr, d_uj, d_uw, f, h0sd);
+      reg(st, g, csr_sd, w_sd, h0sd, hx, hy);
+      ainv_dev(nullptr, omega, true);
+      // leave hx, hy zero outside the body's faces, as the other forms expect
+      fill(st, hx, g.Fx, 0.0);
+      fill(st, hy, g.Fy, 0.0);
+    } else {
+      reg(st, g, csr, w, f, hx, hy);
+      ainv_dev(nullptr, omega, true);
+    }
     vort2flux_dev(omega, s, qx, qy);
     sync();
     return status;
@@ -736,7 +1034,7 @@ struct ib_handle {
     return code;
   }
 
-  int init(const ib_params* p, int nbod, const ib_body* bodies) {
+  int init(const ib_params* p, int nbod, const ib_body* bodies, const ib_subdomain* sd = nullptr) {
     prm = *p;
     if (prm.gmres_restart <= 0) prm.gmres_restart = 30;
     if (prm.gmres_max_it <= 0) prm.gmres_max_it = 10000;
@@ -745,7 +1043,9 @@ struct ib_handle {
     L = prm.mgridlev;
     if (m < 8 || n < 8 || m % 4 || n % 4) return fail(IB_E_INVAL, "grid must be divisible by 4 (m, n >= 8)");
     if (L < 1) return fail(IB_E_INVAL, "mgridlev must be >= 1");
-    if (prm.sub_domain) return fail(IB_E_UNSUPPORTED, "the sub_domain path is not built");
+    if (prm.sub_domain && !sd) return fail(IB_E_UNSUPPORTED, "sub_domain = T needs ib_create_subdomain");
+    if (sd && !prm.sub_domain) return fail(IB_E_INVAL, "sub-domain parameters given with sub_domain = F");
+    if (sd && prm.num_stat) return fail(IB_E_INVAL, "num_stat and sub_domain exclude each other");
     if (prm.istart < 0) return fail(IB_E_INVAL, "istart must be >= 0");
     if (prm.gmres_restart > 1000) return fail(IB_E_INVAL, "gmres_restart must be <= 1000");
     kmax = prm.gmres_restart;
@@ -789,6 +1089,8 @@ struct ib_handle {
       }
       nbb.push_back(nbb.back() + 2 * B.npts);
       nb += B.npts;
+      h_fam.push_back(B.fam);
+      h_npts.push_back(B.npts);
     }
     nbodies = nbod;
     nf = 2 * nb;
@@ -889,7 +1191,21 @@ struct ib_handle {
     if (prm.schur_mode < 0 || prm.schur_mode > 3) return fail(IB_E_INVAL, "schur_mode must be 0, 1, 2 or 3");
     if (prm.face_margin < -1) return fail(IB_E_INVAL, "face_margin must be >= -1");
     face_margin = prm.face_margin == 0 ? kFaceMarginDefault : std::max(0, prm.face_margin);
-    if (nf > 0 && prm.num_stat) {
+    if (sd) {
+      // the reference builds B and P at the first advance (src/timestepper.F90:
+      // 79-97); neither depends on the flow state
+      sd_on = true;
+      sdc = *sd;
+      if (sdc.droptol == 0.0) sdc.droptol = 7.0e-3;
+      if (sdc.interp_points == 0) sdc.interp_points = 4;
+      if (sdc.support_nz == 0) sdc.support_nz = 10;
+      sd_np = sdc.interp_points;
+      if (nf == 0) return fail(IB_E_INVAL, "the sub-domain approach needs bodies");
+      assembled = true;  // GMRES runs on lumat, rebuilt every FSI iteration
+      Sd.alloc((size_t)nf * nf);
+      const int rc = sd_setup(bodies);
+      if (rc != IB_OK) return rc;
+    } else if (nf > 0 && prm.num_stat) {
       const int rc = num_stat_setup();
       if (rc != IB_OK) return rc;
     } else if (nf > 0) {
@@ -913,10 +1229,10 @@ struct ib_handle {
         compute_green();
         map_taps(st, nf, csr.key, fmap, tapidx, face_flag);
       }
-    } else if (assembled) {
+    } else if (assembled && !sd_on) {
       alloc_assembly(nf);
     }
-    if (assembled) {
+    if (assembled && !sd_on) {
       if (facemode) {
         schur_from_green(st, nf, nF, w, tapidx, Gm, Tm, tors_of, tb, th, dt, delta, Sd);
       } else {

@@ -285,6 +285,24 @@ const void* gemv_tab_kernel();
 void redist_faces(hipStream_t st, const Geom& g, const RegCSR& c, const double* w, const double* f,
                   double* fx_out, double* fy_out);
 
+// ------------------------------------------------ sub-domain approach ---
+// B(r, c) at the listed rows of columns c0..c0+nc-1 (rows[colptr[c] ..
+// colptr[c+1])), from the batch's level-1 streamfunctions; nentries =
+// colptr[c0+nc] - colptr[c0]
+void sd_columns(hipStream_t st, const Geom& g, int c0, int nc, long nentries, const long* colptr, const int* rows,
+                const double* w, const int* key, const double* s1, long s_stride, const double* bc1, long bc_stride,
+                double* vals);
+// lumat = P B^T P^T + body_mat (nf x nf, column major), B by columns with
+// rows ascending
+void sd_lumat(hipStream_t st, int nf, int np, const int* pidx, const double* pw, const long* colptr, const int* rows,
+              const double* vals, const int* tors_of, const TorsBody* tb, const double* theta, double dt,
+              double delta, double* lumat);
+// rhsf(j) = sum_p pw(j,p) r(pidx(j,p)) + rhsf(j)
+void sd_interp(hipStream_t st, int nf, int np, const int* pidx, const double* pw, const double* r, double* rhsf);
+// h0(used[u]) = sum over the (j, w) of used entry u of w f(j)
+void sd_ptf(hipStream_t st, int nused, const int* used, const int* uptr, const int* uj, const double* uw,
+            const double* f, double* h0);
+
 // ---------------------------------------------------- torsional bodies ---
 // y[body entries] += b_times2(x); one workgroup per body
 void body_btimes(hipStream_t st, int nt, const TorsBody* tb, const double* theta, double dt, double delta,

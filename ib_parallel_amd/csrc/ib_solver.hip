@@ -45,6 +45,46 @@ int ib_create(const ib_params* p, int32_t nbodies, const ib_body* bodies, ib_han
   return IB_OK;
 }
 
+int ib_create_subdomain(const ib_params* p, const ib_subdomain* sd, int32_t nbodies, const ib_body* bodies,
+                        ib_handle** out) {
+  if (!p || !sd || !out) return IB_E_INVAL;
+  *out = nullptr;
+  ib_handle* h = nullptr;
+  try {
+    h = new ib_handle();
+    int rc = h->init(p, nbodies, bodies, sd);
+    if (rc != IB_OK) {
+      g_create_err = h->err;
+      delete h;
+      return rc;
+    }
+  } catch (const std::exception& e) {
+    g_create_err = e.what();
+    delete h;
+    return IB_E_HIP;
+  }
+  *out = h;
+  return IB_OK;
+}
+
+int ib_subdomain_info(ib_handle* h, int32_t* nb_sd, int64_t* nnz_evaluated, int64_t* nnz_kept) {
+  if (!h) return IB_E_INVAL;
+  if (nb_sd) *nb_sd = h->sd_on ? h->nb_sd : 0;
+  if (nnz_evaluated) *nnz_evaluated = h->sd_nnz_eval;
+  if (nnz_kept) *nnz_kept = h->sd_nnz_kept;
+  return IB_OK;
+}
+
+int ib_op_subdomain_matrix(ib_handle* h, double* lumat) {
+  if (!h || !lumat || !h->sd_on) return IB_E_INVAL;
+  API_TRY
+  h->sd_matrix();
+  IB_HIP(hipMemcpyAsync(lumat, h->Sd.p, sizeof(double) * (size_t)h->nf * h->nf, hipMemcpyDeviceToHost, h->st));
+  h->sync();
+  return IB_OK;
+  API_CATCH(h)
+}
+
 int ib_destroy(ib_handle* h) {
   delete h;
   return IB_OK;
@@ -152,7 +192,10 @@ int ib_set_body(ib_handle* h, const double* f, const double* xb, const double* y
   if (nf) {
     h->rebuild_weights();
     if (h->read_err_flag()) return h->fail(IB_E_INVAL, "body point regularisation stencil leaves the grid");
-    if (h->prm.num_stat) {
+    if (h->sd_on) {
+      const int rc = h->sd_build_interp();
+      if (rc != IB_OK) return rc;
+    } else if (h->prm.num_stat) {
       const int rc = h->num_stat_setup();
       if (rc != IB_OK) return rc;
     } else if (h->facemode) {

@@ -20,16 +20,22 @@ class IBError(RuntimeError):
 
 
 class Simulation:
-    def __init__(self, fns, params, bodies, prefix="ib_", handle=None):
-        """Create a simulation; with `handle`, wrap one that a batch owns."""
+    def __init__(self, fns, params, bodies, prefix="ib_", handle=None, subdomain=None):
+        """Create a simulation; with `handle`, wrap one that a batch owns; with
+        `subdomain` (abi.make_subdomain), the sub-domain approach."""
         self._f = fns
         self.prefix = prefix
         self.params = params
+        self.subdomain = subdomain
         self._owned = handle is None
         if handle is None:
             self._bodies_keep = abi.make_bodies(bodies)
             h = C.c_void_p()
-            rc = fns["create"](C.byref(params), len(bodies), self._bodies_keep[0], C.byref(h))
+            if subdomain is not None:
+                rc = fns["create_subdomain"](C.byref(params), C.byref(subdomain), len(bodies), self._bodies_keep[0],
+                                             C.byref(h))
+            else:
+                rc = fns["create"](C.byref(params), len(bodies), self._bodies_keep[0], C.byref(h))
             if rc != abi.IB_OK:
                 raise IBError(rc, self._err(None))
         else:
@@ -162,6 +168,18 @@ class Simulation:
         S = np.empty((self.nf, self.nf))  # row j of this buffer receives column j
         self._check(self._f["op_schur_matrix"](self.h, abi.dptr(S)))
         return S.T.copy()
+
+    def subdomain_info(self):
+        """(sub-domain points, B entries evaluated, B entries kept)."""
+        nb, ne, nk = C.c_int32(), C.c_int64(), C.c_int64()
+        self._check(self._f["subdomain_info"](self.h, C.byref(nb), C.byref(ne), C.byref(nk)))
+        return nb.value, ne.value, nk.value
+
+    def subdomain_matrix(self):
+        """lumat = P B^T P^T + body_mat at the current positions (nf x nf)."""
+        A = np.empty((self.nf, self.nf))  # row j of this buffer receives column j
+        self._check(self._f["op_subdomain_matrix"](self.h, abi.dptr(A)))
+        return A.T.copy()
 
     def gmres(self, b, x0=None, allow_noconv=False):
         """Solve the Schur system (ab_times) by GMRES from x0; returns (x, its)."""

@@ -65,7 +65,7 @@ typedef struct ib_params {
   int32_t num_stat;        /* 1: rigid bodies held fixed, the Schur system
                               solved with the precomputed inverse of its
                               Cholesky factor (src/timestepper.F90:457-484) */
-  int32_t sub_domain;      /* 1 would select the sub-domain path: not built */
+  int32_t sub_domain;      /* 1: the sub-domain approach (ib_create_subdomain) */
   int32_t launch_mode;     /* 0: GMRES iterations replayed as captured HIP
                               graphs (default); 1: plain stream launches */
   int32_t schur_mode;      /* GMRES operator: 1 matrix-free ab_times per
@@ -191,6 +191,33 @@ int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable);
 int ib_schur_info(ib_handle* h, int32_t* mode, int32_t* nfaces, int64_t* green_builds);
 int ib_kernel_timing_read(ib_handle* h, int32_t kclass, int64_t* launches, double* total_ms,
                           double* total_bytes);
+
+/* --- sub-domain approach: ib.inp with sub_domain = T (src/timestepper.F90:
+ * 79-97,368-405,602-770; src/grid.F90:320-348,572-951; src/variables.F90:
+ * 425-514; src/operators_struct.F90:838-1004; src/fsinterface.F90:84-161).
+ * The Schur operator on a rectangle of sub-domain points (plus the points of
+ * the one stationary rigid body unless full_rectangular) is precomputed once,
+ * column j at the rows whose E E^T windows (half-width support_nz) overlap,
+ * entries at or below droptol * diagonal dropped; the body points
+ * interpolate from the sub-domain points (P, bilinear), and each FSI
+ * iteration solves lumat f = rhsf by GMRES with lumat = P B^T P^T + body_mat.
+ * params->sub_domain must be 1.  Fields left 0 take the reference's values. --- */
+typedef struct ib_subdomain {
+  double sdxi, sdxe, sdyi, sdye; /* the rectangle of sub-domain points */
+  double delta_sd;               /* their spacing */
+  int32_t full_rectangular;      /* 0: the rigid body's points are sub-domain points too */
+  int32_t precomputed;           /* sub_domain_precomputed of the deck (B is computed here either way) */
+  double droptol;                /* drop tolerance of B (0: 7e-3, src/grid.F90:14) */
+  int32_t interp_points;         /* points of P per body entry, a square (0: 4, src/grid.F90:13) */
+  int32_t support_nz;            /* half-width of the sparsity window (0: 10, src/variables.F90:440) */
+} ib_subdomain;
+int ib_create_subdomain(const ib_params* p, const ib_subdomain* sd, int32_t nbodies, const ib_body* bodies,
+                        ib_handle** out);
+/* sub-domain points, B entries evaluated, B entries kept after droptol */
+int ib_subdomain_info(ib_handle* h, int32_t* nb_sd, int64_t* nnz_evaluated, int64_t* nnz_kept);
+/* lumat (compute_operator, src/operators_struct.F90:838-913) at the current
+ * body positions, nf x nf column-major */
+int ib_op_subdomain_matrix(ib_handle* h, double* lumat);
 
 /* --- batched form (no reference counterpart): many independent simulations
  * of uneven grid size, level count and body size advanced one step

@@ -32,6 +32,7 @@ EXTRA = {
     "op_body_update": (C.c_int, [C.c_void_p, abi._D, abi._D]),
     "set_threads": (C.c_int, [C.c_int32]),
     "get_threads": (C.c_int, []),
+    "op_subdomain_interp": (C.c_int, [C.c_void_p, abi._I, abi._D, abi._D, abi._D]),
 }
 
 
@@ -53,8 +54,17 @@ def load():
 class OracleSim(Simulation):
     """Simulation on the CPU restatement, plus the oracle-only operators."""
 
-    def __init__(self, params, bodies):
-        super().__init__(load(), params, bodies, prefix="orc_")
+    def __init__(self, params, bodies, subdomain=None):
+        super().__init__(load(), params, bodies, prefix="orc_", subdomain=subdomain)
+
+    def subdomain_interp(self, npts_sd, np_=4):
+        """(indices (nf, np), weights (nf, np), sub-domain x, y) of P."""
+        import numpy as np
+        idx = np.empty(self.nf * np_, dtype=np.int32); w = np.empty(self.nf * np_)
+        sx = np.empty(npts_sd); sy = np.empty(npts_sd)
+        self._check(self._f["op_subdomain_interp"](self.h, idx.ctypes.data_as(abi._I), abi.dptr(w), abi.dptr(sx),
+                                                   abi.dptr(sy)))
+        return idx.reshape(self.nf, np_), w.reshape(self.nf, np_), sx, sy
 
     def rot(self, hx, hy):
         import numpy as np

@@ -27,6 +27,7 @@
 #include "ib_oracle.h"
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -563,7 +564,11 @@ struct orc_handle {
    * Gram-Schmidt without refinement, restart gmres_restart, convergence
    * ||r|| <= max(rtol*||b||, atol) on the Givens residual estimate. */
   int gmres(const double* b, double* x, int* its_out) const {
-    const int N = nf, kmax = prm.gmres_restart, maxit = prm.gmres_max_it;
+    return gmres_op(nf, [this](const double* v, double* w) { schur(v, w); }, b, x, its_out);
+  }
+  int gmres_op(int N, const std::function<void(const double*, double*)>& A, const double* b, double* x,
+               int* its_out) const {
+    const int kmax = prm.gmres_restart, maxit = prm.gmres_max_it;
     std::vector<std::vector<double>> V(kmax + 1, std::vector<double>(N));
     std::vector<double> H((size_t)(kmax + 1) * kmax, 0.0), cc(kmax), ss(kmax), grs(kmax + 1), w(N);
     auto HH = [&](int r, int c) -> double& { return H[(size_t)c * (kmax + 1) + r]; };
@@ -573,7 +578,7 @@ struct orc_handle {
     int reason = 0; /* 0 iterating, 1 converged, -1 diverged */
     bool first = true;
     while (!reason) {
-      schur(x, w.data());
+      A(x, w.data());
       for (int i = 0; i < N; ++i) V[0][i] = b[i] - w[i];
       double res = std::sqrt(dot(V[0].data(), V[0].data(), N));
       grs[0] = res;
@@ -589,7 +594,7 @@ struct orc_handle {
       int it = 0;
       bool hapend = false;
       while (!reason && it < kmax && its < maxit) {
-        schur(V[it].data(), V[it + 1].data());
+        A(V[it].data(), V[it + 1].data());
         double lhh[1024];
         std::vector<double> lbig;
         double* lh = lhh;
@@ -881,6 +886,345 @@ struct orc_handle {
     regT(fix.data(), fiy.data(), f_rdst.data());
   }
 
+  /* ---------------- sub-domain approach ----------------------------------
+   * Sub-domain points (src/grid.F90:320-348): the points of the one
+   * stationary rigid body unless full_rectangular, then the rectangle, x
+   * fastest.  Entry k < nb_sd is the x force of point k, the rest the y
+   * forces (collect_subdomain, :437-455). */
+  bool sd_on = false;
+  ib_subdomain sdc{};
+  int sd_nx = 0, sd_ny = 0, nr_sd = 0, nb_sd = 0, nf_sd = 0, sd_np = 4;
+  std::vector<double> sd_x, sd_y;
+  std::vector<double> wsd;  /* weight_sd: 49 per sub-domain entry */
+  std::vector<int> ksd;     /* face of each tap: x-face index, or Fx + y-face index */
+  std::vector<std::vector<int>> bcol_row;    /* kept rows of column j of B, in list order */
+  std::vector<std::vector<double>> bcol_val;
+  int64_t sd_nnz_eval = 0, sd_nnz_kept = 0;
+  std::vector<int> pidx;    /* interp_indices: sd_np per force entry (0-based sub-domain entries) */
+  std::vector<double> pw;   /* interp_weights */
+  std::vector<double> lumat;
+
+  static int fnint(double v) { return (int)std::lround(v); } /* Fortran NINT */
+
+  /* deltafnc2 (src/grid.F90:1007-1029): the 2-point hat */
+  static double deltafnc2(double x1, double x2, double dr) {
+    const double r1 = std::fabs(x1 - x2) / dr;
+    const double d = r1 <= 1.0 ? 1.0 - r1 : 0.0;
+    return d / dr;
+  }
+
+  int setup_subdomain() {
+    const ib_subdomain& c = sdc;
+    if (!(c.delta_sd > 0.0) || !(c.sdxe >= c.sdxi) || !(c.sdye >= c.sdyi))
+      return fail(IB_E_INVAL, "sub-domain: need delta_sd > 0 and a non-empty rectangle");
+    const int s2n = (int)std::lround(std::sqrt((double)sd_np));
+    if (s2n * s2n != sd_np) return fail(IB_E_INVAL, "sub-domain: interp_points must be a square");
+    nr_sd = 0;
+    sd_x.clear();
+    sd_y.clear();
+    if (!c.full_rectangular) {
+      int nrig = 0;
+      for (const Body& b : bdy)
+        if (b.fam == 1) {
+          ++nrig;
+          sd_x.insert(sd_x.end(), b.x.begin(), b.x.end());
+          sd_y.insert(sd_y.end(), b.y.begin(), b.y.end());
+        }
+      if (nrig > 1)
+        return fail(IB_E_UNSUPPORTED, "sub-domain with body points: the reference includes one rigid body only");
+      nr_sd = (int)sd_x.size();
+    }
+    sd_nx = fnint((c.sdxe - c.sdxi) / c.delta_sd + 1.0);
+    sd_ny = fnint((c.sdye - c.sdyi) / c.delta_sd + 1.0);
+    for (int j = 1; j <= sd_ny; ++j)
+      for (int i = 1; i <= sd_nx; ++i) {
+        sd_x.push_back(c.sdxi + (i - 1) * c.delta_sd);
+        sd_y.push_back(c.sdyi + (j - 1) * c.delta_sd);
+      }
+    nb_sd = (int)sd_x.size();
+    nf_sd = 2 * nb_sd;
+    /* setup_pre_reg_subdomain (src/grid.F90:628-692): E on the sub-domain */
+    const int s2 = support2, nt = (2 * s2 + 1) * (2 * s2 + 1);
+    const double d2 = 0.5 * delta;
+    const bool f32 = prm.deltafnc_f32_literals != 0;
+    wsd.assign((size_t)nt * nf_sd, 0.0);
+    ksd.assign((size_t)nt * nf_sd, 0);
+    std::vector<int> bix(nb_sd), biy(nb_sd);
+    for (int k = 0; k < nf_sd; ++k) {
+      const int pt = k % nb_sd, comp = k < nb_sd ? 1 : 2;
+      const int ix = (int)((sd_x[pt] + offx) / delta) + 1;
+      const int iy = (int)((sd_y[pt] + offy) / delta) + 1;
+      bix[pt] = ix;
+      biy[pt] = iy;
+      int next = 0;
+      for (int l = -s2; l <= s2; ++l)
+        for (int p = -s2; p <= s2; ++p) {
+          double x, y;
+          const int ii = ix + p, jj = iy + l;
+          if (comp == 1) {
+            x = delta * (ix - 1 + p) - offx;
+            y = delta * (iy - 1 + l) - offy + d2;
+            if (ii < 1 || ii > m + 1 || jj < 1 || jj > n)
+              return fail(IB_E_INVAL, "sub-domain point regularisation stencil leaves the grid");
+            ksd[(size_t)k * nt + next] = (m + 1) * (iy + l - 1) + (ix + p - 1);
+          } else {
+            x = delta * (ix - 1 + p) - offx + d2;
+            y = delta * (iy - 1 + l) - offy;
+            if (ii < 1 || ii > m || jj < 1 || jj > n + 1)
+              return fail(IB_E_INVAL, "sub-domain point regularisation stencil leaves the grid");
+            ksd[(size_t)k * nt + next] = Fx + m * (iy + l - 1) + (ix + p - 1);
+          }
+          wsd[(size_t)k * nt + next] =
+              delta * delta * deltafnc(x, sd_x[pt], delta, f32) * deltafnc(y, sd_y[pt], delta, f32);
+          ++next;
+        }
+    }
+    /* get_sparsity_EEt (src/variables.F90:425-514) with the window matrix of
+     * setup_subdomain_sparsity (src/grid.F90:696-751): rows i and j of B are
+     * evaluated together when their windows of half-width support_nz share
+     * a column of that matrix (x entries: (m+1)(iy+l-1)+(ix+p-1); y entries:
+     * (m+1)n-1 + m(iy+l-1)+(ix+p-1)) */
+    const int sn = c.support_nz, nvel = (m + 1) * n + m * (n + 1);
+    std::vector<std::pair<int, int>> colrow;
+    colrow.reserve((size_t)nf_sd * (2 * sn + 1) * (2 * sn + 1));
+    for (int k = 0; k < nf_sd; ++k) {
+      const int pt = k % nb_sd;
+      for (int l = -sn; l <= sn; ++l)
+        for (int p = -sn; p <= sn; ++p) {
+          const int col = k < nb_sd ? (m + 1) * (biy[pt] + l - 1) + (bix[pt] + p - 1)
+                                    : (m + 1) * n - 1 + m * (biy[pt] + l - 1) + (bix[pt] + p - 1);
+          if (col < 0 || col >= nvel) return fail(IB_E_INVAL, "sub-domain: sparsity window leaves the grid");
+          colrow.push_back({col, k});
+        }
+    }
+    std::sort(colrow.begin(), colrow.end());
+    std::vector<std::vector<int>> ja(nf_sd);
+    for (size_t a = 0; a < colrow.size();) {
+      size_t b = a;
+      while (b < colrow.size() && colrow[b].first == colrow[a].first) ++b;
+      for (size_t u = a; u < b; ++u)
+        for (size_t v = a; v < b; ++v) ja[colrow[u].second].push_back(colrow[v].second);
+      a = b;
+    }
+    std::vector<std::vector<int>> rows(nf_sd);
+    for (int k = 0; k < nf_sd; ++k) {
+      std::vector<int>& J = ja[k];
+      std::sort(J.begin(), J.end());
+      J.erase(std::unique(J.begin(), J.end()), J.end());
+      for (int j : J)
+        if ((k < nb_sd) != (j < nb_sd))
+          return fail(IB_E_INVAL, "sub-domain: an x window meets a y window (sub-domain at the grid corner)");
+      std::vector<int>& R = rows[k];
+      if (k < nb_sd) {
+        R = J;
+        for (int j : J) R.push_back(j + nb_sd);
+      } else {
+        for (int j : J) R.push_back(j - nb_sd);
+        R.insert(R.end(), J.begin(), J.end());
+      }
+    }
+    /* preprocess_sd (src/timestepper.F90:602-770): column j of B is
+     * a_times_sd(e_j) (src/operators_fluid.F90:124-199) at its rows; an
+     * entry is kept when |B(r, j)| > droptol * B(j, j) */
+    const double dtol = c.droptol;
+    bcol_row.assign(nf_sd, {});
+    bcol_val.assign(nf_sd, {});
+    sd_nnz_eval = sd_nnz_kept = 0;
+    std::vector<double> hx(Fx), hy(Fy), vals;
+    std::vector<std::vector<double>> vv(L, std::vector<double>(P)), sv(L, std::vector<double>(P)),
+        vx(L, std::vector<double>(Fx)), vy(L, std::vector<double>(Fy));
+    for (int j = 0; j < nf_sd; ++j) {
+      std::fill(hx.begin(), hx.end(), 0.0);
+      std::fill(hy.begin(), hy.end(), 0.0);
+      for (int t = 0; t < nt; ++t) {
+        const int key = ksd[(size_t)j * nt + t];
+        const double v = wsd[(size_t)j * nt + t] * 1.0;
+        if (key < Fx) hx[key] += v;
+        else hy[key - Fx] += v;
+      }
+      for (auto& a : vv) std::fill(a.begin(), a.end(), 0.0);
+      rot(hx.data(), hy.data(), vv[0].data());
+      ainv(vv[0].data());
+      vort2flux(vv, sv, vx, vy);
+      const std::vector<int>& R = rows[j];
+      vals.assign(R.size(), 0.0);
+      double diag = 0.0;
+      for (size_t a = 0; a < R.size(); ++a) {
+        const int r = R[a];
+        double acc = 0.0;
+        for (int t = 0; t < nt; ++t) {
+          const int key = ksd[(size_t)r * nt + t];
+          acc = acc + wsd[(size_t)r * nt + t] * (key < Fx ? vx[0][key] : vy[0][key - Fx]);
+        }
+        vals[a] = acc;
+        if (r == j) diag = acc;
+      }
+      sd_nnz_eval += (int64_t)R.size();
+      for (size_t a = 0; a < R.size(); ++a)
+        if (std::fabs(vals[a]) > dtol * diag) {
+          bcol_row[j].push_back(R[a]);
+          bcol_val[j].push_back(vals[a]);
+        }
+      sd_nnz_kept += (int64_t)bcol_row[j].size();
+    }
+    return IB_OK;
+  }
+
+  /* setup_reg_subdomain_indices (src/grid.F90:777-951): P, the
+   * interpolation of each force entry from sub-domain points (bilinear hat
+   * weights), or the entry itself for the points of the rigid body when
+   * they are sub-domain points */
+  int subdomain_interp() {
+    const ib_subdomain& c = sdc;
+    const int s = (int)std::lround(std::sqrt((double)sd_np));
+    int i1, i2;
+    if (sd_np % 2 == 0) i1 = s / 2 + 1 - s;
+    else i1 = (s - 1) / 2 + 1 - s;
+    i2 = i1 + s - 1;
+    (void)i2;
+    pidx.assign((size_t)sd_np * nf, 0);
+    pw.assign((size_t)sd_np * nf, 0.0);
+    std::vector<int> nn(sd_np), ll(s), pp(s);
+    std::vector<double> dist(sd_np), w(sd_np);
+    for (int j = 0; j < nf; ++j) {
+      const int comp = ccomp[j];
+      const double indx = (xb[j] - c.sdxi) / c.delta_sd + 1.0, indy = (yb[j] - c.sdyi) / c.delta_sd + 1.0;
+      for (int a = 0; a < s; ++a) {
+        ll[a] = (sd_np % 2 == 0 ? (int)indx : fnint(indx)) + i1 + a;
+        pp[a] = (sd_np % 2 == 0 ? (int)indy : fnint(indy)) + i1 + a;
+      }
+      if (!c.full_rectangular && bdy[cbody[j]].fam == 1) {
+        /* the rigid body is the first body; its points are sub-domain
+         * points 1..nr in order (1-based j: nearest_neighbor = j, or j - nr
+         * for y entries; the rest are placeholders of weight 0) */
+        const int j1 = j + 1, nr = nb_ind[cbody[j]];
+        for (int i = 0; i < sd_np; ++i) nn[i] = comp == 1 ? j1 + 1 : j1 + 1 - nr;
+        nn[0] = comp == 1 ? j1 : j1 - nr;
+        for (int i = 0; i < sd_np; ++i) dist[i] = 1.0;
+        dist[0] = 0.0;
+      } else {
+        int i = 0;
+        for (int a = 0; a < s; ++a)
+          for (int b = 0; b < s; ++b) {
+            if (ll[a] < 1) return fail(IB_E_INVAL, "Body has exited the subdomain from left");
+            if (ll[a] > sd_nx) return fail(IB_E_INVAL, "Body has exited the subdomain from right");
+            if (pp[b] < 1) return fail(IB_E_INVAL, "Body has exited the subdomain from bottom");
+            if (pp[b] > sd_ny) return fail(IB_E_INVAL, "Body has exited the subdomain from top");
+            nn[i] = nr_sd + (pp[b] - 1) * sd_nx + ll[a];
+            const double dx = xb[j] - sd_x[nn[i] - 1], dy = yb[j] - sd_y[nn[i] - 1];
+            dist[i] = std::sqrt(dx * dx + dy * dy);
+            ++i;
+          }
+      }
+      bool anyexact = false;
+      for (int i = 0; i < sd_np; ++i) {
+        const int q = nn[i] - 1;
+        if (q < 0 || q >= nb_sd) return fail(IB_E_INVAL, "sub-domain: interpolation point out of range");
+        w[i] = c.delta_sd * c.delta_sd * deltafnc2(sd_x[q], xb[j], c.delta_sd) * deltafnc2(sd_y[q], yb[j], c.delta_sd);
+        if (dist[i] == 0.0) {
+          w[i] = 1.0;
+          anyexact = true;
+        }
+      }
+      for (int i = 0; i < sd_np; ++i) {
+        if (anyexact && dist[i] != 0.0) w[i] = 0.0;
+        pidx[(size_t)j * sd_np + i] = nn[i] - 1 + (comp == 1 ? 0 : nb_sd);
+        pw[(size_t)j * sd_np + i] = w[i];
+      }
+    }
+    return IB_OK;
+  }
+
+  /* B(r, c) as compute_operator reads it: the kept entry, else 0 */
+  double bget(int r, int col) const {
+    const std::vector<int>& R = bcol_row[col];
+    for (size_t a = 0; a < R.size(); ++a)
+      if (R[a] == r) return bcol_val[col][a];
+    return 0.0;
+  }
+
+  /* compute_operator + body_mat (src/operators_struct.F90:838-1004):
+   * lumat(k, i) = body_mat(k, i) + sum_p w_p(k) I(i, idx_p(k)),
+   * I(i, c) = sum_q w_q(i) B(idx_q(i), c) */
+  void compute_operator() {
+    lumat.assign((size_t)nf * nf, 0.0);
+    for (size_t t = 0; t < tors.size(); ++t) {
+      const Body& b = bdy[tors[t]];
+      const int off = nbb_ind[tors[t]], nbi = b.npts;
+      const double si = std::sin(b.gamma0 - theta[t]), ci = std::cos(b.gamma0 - theta[t]);
+      const double den = 4.0 / (dt * dt) * Mt[t] + 2.0 / dt * Ct[t] + Jt[t];
+      for (int jj = 1; jj <= 2 * nbi; ++jj)
+        for (int kk = 1; kk <= 2 * nbi; ++kk) {
+          const double im = jj - 1, in = kk - 1, ds2 = b.deltas * b.deltas;
+          double v;
+          /* the reference's expressions, evaluated left to right */
+          if (jj <= nbi && kk <= nbi) v = (im * in) * (si * si) * ds2;
+          else if (jj <= nbi) v = -((im * (in - nbi)) * si * ci * ds2);
+          else if (kk <= nbi) v = -(((im - nbi) * in) * si * ci * ds2);
+          else v = ((im - nbi) * (in - nbi)) * (ci * ci) * ds2;
+          v = v / den * (delta * delta / b.deltas / dt) * (2 * b.deltas / dt);
+          lumat[(size_t)(off + kk - 1) * nf + (off + jj - 1)] = v;
+        }
+    }
+    std::vector<double> I(nf);
+    for (int i = 0; i < nf; ++i)
+      for (int k = 0; k < nf; ++k) {
+        double acc = 0.0;
+        for (int p = 0; p < sd_np; ++p) {
+          const int cidx = pidx[(size_t)k * sd_np + p];
+          double ic = 0.0;
+          for (int q = 0; q < sd_np; ++q) ic += bget(pidx[(size_t)i * sd_np + q], cidx) * pw[(size_t)i * sd_np + q];
+          acc += pw[(size_t)k * sd_np + p] * ic;
+        }
+        lumat[(size_t)i * nf + k] = lumat[(size_t)i * nf + k] + acc;
+      }
+  }
+
+  /* rhsf += P (E_sd (q + q0)) (regT_sd_sparse, src/fsinterface.F90:130-161,
+   * and the interpolation of src/timestepper.F90:389-393) */
+  void subdomain_rhs(const double* qhx, const double* qhy, double* rhsf) const {
+    const int nt = (2 * support2 + 1) * (2 * support2 + 1);
+    std::vector<double> r(nf_sd, 0.0);
+    for (int k = 0; k < nf_sd; ++k)
+      for (int t = 0; t < nt; ++t) {
+        const int key = ksd[(size_t)k * nt + t];
+        r[k] = r[k] + wsd[(size_t)k * nt + t] * (key < Fx ? qhx[key] : qhy[key - Fx]);
+      }
+    for (int j = 0; j < nf; ++j) {
+      double acc = 0.0;
+      for (int p = 0; p < sd_np; ++p) acc += pw[(size_t)j * sd_np + p] * r[pidx[(size_t)j * sd_np + p]];
+      rhsf[j] = acc + rhsf[j];
+    }
+  }
+
+  /* corrector source: hx, hy = E_sd^T (P^T f) over the used sub-domain
+   * entries in ascending order (src/timestepper.F90:500-510,
+   * reg_sd_sparse src/fsinterface.F90:84-126) */
+  void subdomain_reg(double* hx, double* hy) const {
+    const int nt = (2 * support2 + 1) * (2 * support2 + 1);
+    std::vector<int> used(pidx.begin(), pidx.end());
+    std::sort(used.begin(), used.end());
+    used.erase(std::unique(used.begin(), used.end()), used.end());
+    std::vector<double> h0(nf_sd, 0.0);
+    for (int cidx : used) {
+      double acc = 0.0;
+      for (int j = 0; j < nf; ++j) {
+        double wj = 0.0;
+        for (int p = 0; p < sd_np; ++p)
+          if (pidx[(size_t)j * sd_np + p] == cidx) wj = pw[(size_t)j * sd_np + p];
+        acc += wj * f[j];
+      }
+      h0[cidx] = acc;
+    }
+    for (int k : used)
+      for (int t = 0; t < nt; ++t) {
+        const int key = ksd[(size_t)k * nt + t];
+        const double v = wsd[(size_t)k * nt + t] * h0[k];
+        if (key < Fx) hx[key] += v;
+        else hy[key - Fx] += v;
+      }
+  }
+
   /* ---------------- one time step: src/timestepper.F90:18-556 ----------- */
   int advance() {
     itime += 1;
@@ -948,17 +1292,37 @@ struct orc_handle {
       std::fill(yb_.begin(), yb_.end(), 0.0);
       compute_rhs(yb_.data());
       for (int k = 0; k < nf; ++k) rhsf[k] += yb_[k];
-      regT(qhx.data(), qhy.data(), rhsf.data());
       int its = 0;
-      int rc = gmres(rhsf.data(), f.data(), &its);
-      ksp_iter += its;
+      int rc;
+      if (!sd_on) {
+        regT(qhx.data(), qhy.data(), rhsf.data());
+        rc = gmres(rhsf.data(), f.data(), &its);
+        ksp_iter += its;
+      } else {
+        /* src/timestepper.F90:368-405: lumat for the current P, GMRES on
+         * it; the reference does not add these iterations to ksp_iter */
+        subdomain_rhs(qhx.data(), qhy.data(), rhsf.data());
+        compute_operator();
+        rc = gmres_op(
+            nf,
+            [this](const double* v, double* w) {
+              for (int k = 0; k < nf; ++k) {
+                double acc = 0.0;
+                for (int i = 0; i < nf; ++i) acc += lumat[(size_t)i * nf + k] * v[i];
+                w[k] = acc;
+              }
+            },
+            rhsf.data(), f.data(), &its);
+      }
       if (rc != IB_OK) status = rc;
       redistribute();
       double e = compute_dx();
       err_glob = e;
       if (err_glob >= prm.fsi_tol)
         for (size_t t = 0; t < tors.size(); ++t) Ftint[t] = Jt[t] * theta[t];
-      int r2 = setup_reg();
+      /* E follows the bodies; on the sub-domain P does instead
+       * (reg_prep_sd_sparse, src/variables.F90:377-396) */
+      int r2 = sd_on ? subdomain_interp() : setup_reg();
       if (r2 != IB_OK) return r2;
     }
     if (err_glob >= prm.fsi_tol && status == IB_OK) status = IB_E_NOCONV_FSI;
@@ -969,7 +1333,8 @@ struct orc_handle {
 
     /* step 3: corrector on level 1 */
     std::vector<double> hx(Fx, 0.0), hy(Fy, 0.0), vort(P);
-    reg(f.data(), hx.data(), hy.data());
+    if (sd_on) subdomain_reg(hx.data(), hy.data());
+    else reg(f.data(), hx.data(), hy.data());
     rot(hx.data(), hy.data(), vort.data());
     ainv(vort.data());
     for (int q = 0; q < P; ++q) omega[0][q] = omega[0][q] - vort[q];
@@ -979,7 +1344,7 @@ struct orc_handle {
   int step_status = IB_OK;
 
   /* ---------------- setup: src/grid.F90:117-434, src/myfft.F90:237-298 --- */
-  int init(const ib_params* p, int nbodies, const ib_body* bodies) {
+  int init(const ib_params* p, int nbodies, const ib_body* bodies, const ib_subdomain* sd = nullptr) {
     prm = *p;
     if (prm.gmres_restart <= 0) prm.gmres_restart = 30;
     if (prm.gmres_max_it <= 0) prm.gmres_max_it = 10000;
@@ -987,7 +1352,9 @@ struct orc_handle {
     m = prm.m; n = prm.n; L = prm.mgridlev;
     if (m < 8 || n < 8 || m % 4 || n % 4) return fail(IB_E_INVAL, "grid must be divisible by 4 (m, n >= 8)");
     if (L < 1) return fail(IB_E_INVAL, "mgridlev must be >= 1");
-    if (prm.sub_domain) return fail(IB_E_UNSUPPORTED, "the sub_domain path is not built");
+    if (prm.sub_domain && !sd) return fail(IB_E_UNSUPPORTED, "sub_domain = T needs the sub-domain parameters (create_subdomain)");
+    if (sd && !prm.sub_domain) return fail(IB_E_INVAL, "sub-domain parameters given with sub_domain = F");
+    if (sd && prm.num_stat) return fail(IB_E_INVAL, "num_stat and sub_domain exclude each other");
     if (prm.istart < 0) return fail(IB_E_INVAL, "istart must be >= 0");
     pi = 4.0 * std::atan(1.0);
     dt = prm.dt; Re = prm.re; len = prm.len; offx = prm.offsetx; offy = prm.offsety;
@@ -1097,6 +1464,20 @@ struct orc_handle {
       rc = preprocess();
       if (rc != IB_OK) return rc;
     }
+    /* the reference precomputes B and P at the first advance
+     * (src/timestepper.F90:79-97); neither depends on the flow state */
+    if (sd) {
+      sd_on = true;
+      sdc = *sd;
+      if (sdc.droptol == 0.0) sdc.droptol = 7.0e-3;
+      if (sdc.interp_points == 0) sdc.interp_points = 4;
+      if (sdc.support_nz == 0) sdc.support_nz = 10;
+      sd_np = sdc.interp_points;
+      rc = setup_subdomain();
+      if (rc != IB_OK) return rc;
+      rc = subdomain_interp();
+      if (rc != IB_OK) return rc;
+    }
     return IB_OK;
   }
 };
@@ -1121,6 +1502,48 @@ int orc_create(const ib_params* p, int32_t nbodies, const ib_body* bodies, orc_h
     return rc;
   }
   *out = h;
+  return IB_OK;
+}
+
+int orc_create_subdomain(const ib_params* p, const ib_subdomain* sd, int32_t nbodies, const ib_body* bodies,
+                         orc_handle** out) {
+  if (!p || !sd || !out) return IB_E_INVAL;
+  orc_handle* h = new (std::nothrow) orc_handle();
+  if (!h) return IB_E_INVAL;
+  int rc = h->init(p, nbodies, bodies, sd);
+  if (rc != IB_OK) {
+    g_create_err = h->err;
+    delete h;
+    *out = nullptr;
+    return rc;
+  }
+  *out = h;
+  return IB_OK;
+}
+
+int orc_subdomain_info(orc_handle* h, int32_t* nb_sd, int64_t* nnz_evaluated, int64_t* nnz_kept) {
+  if (!h) return IB_E_INVAL;
+  if (nb_sd) *nb_sd = h->sd_on ? h->nb_sd : 0;
+  if (nnz_evaluated) *nnz_evaluated = h->sd_nnz_eval;
+  if (nnz_kept) *nnz_kept = h->sd_nnz_kept;
+  return IB_OK;
+}
+
+int orc_op_subdomain_matrix(orc_handle* h, double* lumat) {
+  if (!h || !lumat || !h->sd_on) return IB_E_INVAL;
+  h->compute_operator();
+  std::copy(h->lumat.begin(), h->lumat.end(), lumat);
+  return IB_OK;
+}
+
+/* the interpolation P: np sub-domain entries and weights per force entry,
+ * and the sub-domain point coordinates (oracle-only, for tests) */
+int orc_op_subdomain_interp(orc_handle* h, int32_t* idx, double* w, double* sdx, double* sdy) {
+  if (!h || !h->sd_on) return IB_E_INVAL;
+  if (idx) for (size_t a = 0; a < h->pidx.size(); ++a) idx[a] = h->pidx[a];
+  if (w) std::copy(h->pw.begin(), h->pw.end(), w);
+  if (sdx) std::copy(h->sd_x.begin(), h->sd_x.end(), sdx);
+  if (sdy) std::copy(h->sd_y.begin(), h->sd_y.end(), sdy);
   return IB_OK;
 }
 
@@ -1224,6 +1647,7 @@ int orc_set_body(orc_handle* h, const double* f, const double* xb, const double*
   int rc = h->setup_reg();
   if (rc != IB_OK) return rc;
   if (h->prm.num_stat) rc = h->preprocess();
+  if (rc == IB_OK && h->sd_on) rc = h->subdomain_interp();
   return rc;
 }
 

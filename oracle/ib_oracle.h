@@ -19,6 +19,11 @@ extern "C" {
 typedef struct orc_handle orc_handle;
 
 int orc_create(const ib_params* p, int32_t nbodies, const ib_body* bodies, orc_handle** out);
+int orc_create_subdomain(const ib_params* p, const ib_subdomain* sd, int32_t nbodies, const ib_body* bodies,
+                         orc_handle** out);
+int orc_subdomain_info(orc_handle* h, int32_t* nb_sd, int64_t* nnz_evaluated, int64_t* nnz_kept);
+int orc_op_subdomain_matrix(orc_handle* h, double* lumat);
+int orc_op_subdomain_interp(orc_handle* h, int32_t* idx, double* w, double* sdx, double* sdy);
 int orc_destroy(orc_handle* h);
 int orc_get_sizes(const orc_handle* h, ib_sizes* out);
 int orc_last_error(const orc_handle* h, char* buf, int32_t len);

@@ -1,0 +1,88 @@
+"""Sub-domain approach on the GPU against the oracle (same sparsified B,
+same P, same lumat, same steps)."""
+import numpy as np
+import pytest
+
+from conftest import rel
+from ib_parallel_amd import abi, decks
+
+pytestmark = pytest.mark.gpu
+
+TINY = 1e-300
+
+
+def sd_params(p):
+    q = abi.IBParams.from_buffer_copy(p)
+    q.sub_domain = 1
+    return q
+
+
+CASES = {
+    # torsional plate interpolated, cylinder points included exactly (no
+    # entry dropped: cond(lumat) ~ 2e4, so GMRES runs to 1e-12 for 1e-10 parity)
+    "rigid_included": (dict(m=64, n=48, mgridlev=3, tol=1e-12), (0.0, 1.0, -0.4, 0.3, 4.0 / 64, 0, TINY)),
+    # every body interpolated from the rectangle, reference drop tolerance
+    "full_rectangle": (dict(m=64, n=48, mgridlev=3, tol=1e-10), (-0.8, 1.0, -0.45, 0.4, 4.0 / 64, 1, 0.0)),
+}
+
+
+def make(case):
+    from ib_parallel_amd import GpuSim
+    from oracle import binding as ob
+    kw, (x0, x1, y0, y1, d, full, dtol) = CASES[case]
+    p, b = decks.tiny_case(**kw)
+    sd = abi.make_subdomain(x0, x1, y0, y1, d, full_rectangular=full, droptol=dtol)
+    return GpuSim(sd_params(p), b, subdomain=sd), ob.OracleSim(sd_params(p), b, subdomain=sd)
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_setup_and_lumat_match_oracle(case):
+    g, o = make(case)
+    ng, no = g.subdomain_info(), o.subdomain_info()
+    assert ng[:2] == no[:2]
+    assert abs(ng[2] - no[2]) <= 2  # entries at the drop threshold may round either way
+    assert rel(g.subdomain_matrix(), o.subdomain_matrix()) < 1e-12
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_steps_match_oracle(case):
+    g, o = make(case)
+    for _ in range(3):
+        ig, io = g.advance(), o.advance()
+        assert (ig.ksp_iter, ig.fsi_iter) == (io.ksp_iter, io.fsi_iter)
+    for k in range(1, g.L + 1):
+        assert rel(g.level(k)["omega"], o.level(k)["omega"]) < 1e-10
+    bg, bo = g.body(), o.body()
+    assert rel(bg["f"], bo["f"]) < 1e-8
+    assert rel(bg["theta"], bo["theta"]) < 1e-10
+    assert rel(bg["xb"], bo["xb"]) < 1e-13
+    assert rel(g.subdomain_matrix(), o.subdomain_matrix()) < 1e-12  # P follows the moved plate
+
+
+def test_one_level_rigid_included_equals_plain_step():
+    from ib_parallel_amd import GpuSim
+    # (the sparsity windows of this build must stay inside the grid)
+    p, b = decks.tiny_case(m=64, n=48, mgridlev=1, torsional=False, tol=1e-12)
+    sd = abi.make_subdomain(-0.8, 0.0, -0.4, 0.4, 4.0 / 64, full_rectangular=0, droptol=TINY)
+    g, pl = GpuSim(sd_params(p), b, subdomain=sd), GpuSim(p, b)
+    for _ in range(3):
+        g.advance()
+        pl.advance()
+    assert rel(g.level(1)["omega"], pl.level(1)["omega"]) < 1e-8
+    assert rel(g.body()["f"], pl.body()["f"]) < 1e-6
+
+
+def test_window_at_the_grid_edge_is_rejected():
+    from ib_parallel_amd import GpuSim, IBError
+    p, b = decks.tiny_case(m=48, n=40, mgridlev=1, torsional=False, tol=1e-12)
+    with pytest.raises(IBError) as e:
+        GpuSim(sd_params(p), b, subdomain=abi.make_subdomain(-0.8, 0.0, -0.4, 0.4, 4.0 / 48, full_rectangular=0))
+    assert e.value.code == abi.IB_E_UNSUPPORTED and "edge of the grid" in str(e.value)
+
+
+def test_plain_create_rejects_subdomain_flag():
+    from ib_parallel_amd import GpuSim, IBError
+    p, b = decks.tiny_case(m=32, n=24, mgridlev=2)
+    with pytest.raises(IBError) as e:
+        GpuSim(sd_params(p), b)
+    assert e.value.code == abi.IB_E_UNSUPPORTED
