@@ -81,6 +81,18 @@ def read_case(input_dir, **overrides):
     return p, bodies, nml
 
 
+def subdomain_of(nml):
+    """The sub-domain parameters of a deck (abi.IBSubdomain), or None when
+    sub_domain = F; interp_points, droptol and the sparsity window take the
+    reference's values."""
+    from .abi import make_subdomain
+    if not nml.get("sub_domain", False):
+        return None
+    return make_subdomain(nml["sdxi"], nml["sdxe"], nml["sdyi"], nml["sdye"], nml["delta_sd"],
+                          full_rectangular=int(bool(nml.get("sub_domain_full_rectangular", False))),
+                          precomputed=int(bool(nml.get("sub_domain_precomputed", False))))
+
+
 def write_body(path, b):
     with open(path, "w") as fh:
         fh.write(f"{b.npts}\n{b.fam}\n")
@@ -91,7 +103,7 @@ def write_body(path, b):
             fh.write(f"{x:.16f}   {y:.16f}\n")
 
 
-def write_ib_inp(path, p, istop=10, isave=10, compute_pressure=False):
+def write_ib_inp(path, p, istop=10, isave=10, compute_pressure=False, subdomain=None):
     with open(path, "w") as fh:
         fh.write("&READ_PARAMETERS\n")
         fh.write(f"ISTART = {p.istart},\nISTOP = {istop},\nISAVE = {isave},\nIRESTART = 0,\n")
@@ -100,7 +112,16 @@ def write_ib_inp(path, p, istop=10, isave=10, compute_pressure=False):
         fh.write(f"LEN = {p.len!r},\nOFFSETX = {p.offsetx!r},\nOFFSETY = {p.offsety!r},\nMGRIDLEV = {p.mgridlev},\n")
         ns = "T" if getattr(p, "num_stat", 0) else "F"
         cpf = "T" if compute_pressure else "F"
-        fh.write(f"COMPUTE_PRESSURE = {cpf},\ndimen = 2,\nnum_stat={ns},\nmotion_prescribed=F,\nsub_domain = F,\n/\n")
+        fh.write(f"COMPUTE_PRESSURE = {cpf},\ndimen = 2,\nnum_stat={ns},\nmotion_prescribed=F,\n")
+        if subdomain is None:
+            fh.write("sub_domain = F,\n/\n")
+        else:
+            sd = subdomain
+            fr = "T" if sd.full_rectangular else "F"
+            pc = "T" if sd.precomputed else "F"
+            fh.write(f"sub_domain = T,\nsub_domain_full_rectangular = {fr},\nsub_domain_precomputed = {pc},\n")
+            fh.write(f"sdxi = {sd.sdxi!r},\nsdxe = {sd.sdxe!r},\nsdyi = {sd.sdyi!r},\nsdye = {sd.sdye!r},\n")
+            fh.write(f"delta_sd = {sd.delta_sd!r}\n/\n")
 
 
 # --------------------------------------------------------------------------

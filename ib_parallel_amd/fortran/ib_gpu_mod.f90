@@ -21,6 +21,13 @@ module ib_gpu_mod
     integer(c_int32_t) :: reserved(3)
   end type ib_params
 
+  type, bind(C) :: ib_subdomain
+    real(c_double) :: sdxi, sdxe, sdyi, sdye, delta_sd
+    integer(c_int32_t) :: full_rectangular, precomputed
+    real(c_double) :: droptol
+    integer(c_int32_t) :: interp_points, support_nz
+  end type ib_subdomain
+
   type, bind(C) :: ib_body
     integer(c_int32_t) :: npts, fam
     real(c_double) :: gamma0, alpha0, itheta, ktheta, ctheta
@@ -46,6 +53,15 @@ module ib_gpu_mod
       type(ib_body), intent(in) :: bodies(*)
       type(c_ptr), intent(out) :: h
     end function ib_create
+
+    integer(c_int) function ib_create_subdomain(p, sd, nbodies, bodies, h) bind(C, name="ib_create_subdomain")
+      import :: c_int, c_int32_t, c_ptr, ib_params, ib_subdomain, ib_body
+      type(ib_params), intent(in) :: p
+      type(ib_subdomain), intent(in) :: sd
+      integer(c_int32_t), value :: nbodies
+      type(ib_body), intent(in) :: bodies(*)
+      type(c_ptr), intent(out) :: h
+    end function ib_create_subdomain
 
     integer(c_int) function ib_destroy(h) bind(C, name="ib_destroy")
       import :: c_int, c_ptr

@@ -46,6 +46,7 @@ program ib_main
   type(body_deck), allocatable, target :: decks(:)
   type(ib_body), allocatable :: cbodies(:)
   type(ib_params) :: p
+  type(ib_subdomain) :: sdp
   type(ib_step_info) :: info
   type(ib_sizes) :: sz
   type(c_ptr) :: h
@@ -98,7 +99,17 @@ program ib_main
   p%reserved = 0
   call getenv_int('IB_GMRES_RESTART', p%gmres_restart)
 
-  rc = ib_create(p, int(nbod, c_int32_t), cbodies, h)
+  if (sub_domain) then
+    ! sub-domain approach (src/timestepper.F90:79-97); interp_points,
+    ! droptol and the sparsity window take the reference's values (0)
+    sdp%sdxi = sdxi; sdp%sdxe = sdxe; sdp%sdyi = sdyi; sdp%sdye = sdye; sdp%delta_sd = delta_sd
+    sdp%full_rectangular = merge(1, 0, sub_domain_full_rectangular)
+    sdp%precomputed = merge(1, 0, sub_domain_precomputed)
+    sdp%droptol = 0.d0; sdp%interp_points = 0; sdp%support_nz = 0
+    rc = ib_create_subdomain(p, sdp, int(nbod, c_int32_t), cbodies, h)
+  else
+    rc = ib_create(p, int(nbod, c_int32_t), cbodies, h)
+  end if
   if (rc /= IB_OK) then
     write(*,*) 'ib_create failed (', rc, '): ', trim(ib_error_message(c_null_ptr))
     stop 1

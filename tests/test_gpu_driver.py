@@ -115,3 +115,34 @@ def test_driver_writes_pressure(tmp_path):
     assert (pv["m"], pv["n"], len(pv["levels"])) == (40, 32, 2)
     for k in range(2):
         assert rel(pv["levels"][k].ravel(), po[k].ravel()) < 1e-10
+
+
+def test_driver_runs_a_subdomain_deck(tmp_path):
+    # sub_domain = T in ib.inp (the switch of the reference's example deck):
+    # the driver creates the handle with the deck's sub-domain parameters
+    from ib_parallel_amd import abi
+    from oracle import binding as ob
+    p, bodies = decks.tiny_case(m=64, n=48, mgridlev=3, tol=1e-10)
+    p.sub_domain = 1
+    sd = abi.make_subdomain(-0.8, 1.0, -0.45, 0.4, 4.0 / 64, full_rectangular=1)
+    (tmp_path / "input").mkdir()
+    decks.write_ib_inp(tmp_path / "input" / "ib.inp", p, istop=3, isave=3, subdomain=sd)
+    for i, b in enumerate(bodies):
+        decks.write_body(tmp_path / "input" / f"body.{i + 1:03d}.inp", b)
+    r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    p2, b2, nml = decks.read_case(str(tmp_path / "input"))
+    sd2 = decks.subdomain_of(nml)
+    assert p2.sub_domain == 1 and sd2.full_rectangular == 1 and sd2.delta_sd == sd.delta_sd
+    o = ob.OracleSim(p2, b2, subdomain=sd2)
+    forces, iters = [], []
+    for _ in range(3):
+        info = o.advance()
+        fx, fy = o.forces()
+        forces.append(np.concatenate([fx, fy]))
+        iters.append((info.ksp_iter, info.fsi_iter))
+    got = np.loadtxt(tmp_path / "output" / "force01_00.dat")
+    for k in range(3):
+        assert rel(got[k, 1:], forces[k]) < 1e-8
+    it = np.loadtxt(tmp_path / "output" / "iter01.dat")
+    assert [tuple(int(v) for v in row[1:]) for row in it] == iters  # ksp_iter 0, as the reference writes it

@@ -86,3 +86,29 @@ def test_plain_create_rejects_subdomain_flag():
     with pytest.raises(IBError) as e:
         GpuSim(sd_params(p), b)
     assert e.value.code == abi.IB_E_UNSUPPORTED
+
+
+def test_reference_example_deck():
+    """The reference's shipped deck (input/ib.inp with body.001/002: rigid
+    airfoil + torsional flap, 860x860x5, sub_domain = T) at full size: the
+    sub-domain has the airfoil's 287 points and the 136 x 95 rectangle; one
+    step converges and leaves q = C s with div q = 0 and C^T q = omega."""
+    import os
+    from conftest import GOLDEN
+    from ib_parallel_amd import GpuSim
+    p, b, nml = decks.read_case(os.path.join(GOLDEN, "decks", "example"))
+    g = GpuSim(p, b, subdomain=decks.subdomain_of(nml))
+    nb_sd, ne, nk = g.subdomain_info()
+    assert nb_sd == 287 + 136 * 95 and 0 < nk < ne
+    info = g.advance()
+    assert info.status == abi.IB_OK and info.err_fsi < p.fsi_tol and info.ksp_iter == 0
+    m, n = g.m, g.n
+    for k in range(1, g.L + 1):
+        lv = g.level(k)
+        QX = lv["qx"].reshape(n, m + 1); QY = lv["qy"].reshape(n + 1, m)
+        div = QX[:, 1:] - QX[:, :-1] + QY[1:, :] - QY[:-1, :]
+        assert np.abs(div).max() < 1e-12 * np.abs(lv["s"]).max()
+        rot = QY[1:-1, 1:] - QY[1:-1, :-1] - QX[1:, 1:-1] + QX[:-1, 1:-1]
+        assert rel(rot.ravel(), lv["omega"]) < 1e-10
+    fx, fy = g.forces()
+    assert np.all(np.isfinite(fx)) and np.all(np.isfinite(fy)) and fx[0] > 0
