@@ -28,6 +28,8 @@ Also reported:
   batched       the batched form (ib_batch_*) on 64 uneven rigid-ellipse
                 simulations against the same simulations stepped one after
                 another with the plain step (run_batched).
+  adjoint       the discrete adjoint (ib_adjoint_run) against the plain
+                forward steps of the same rigid cylinder (run_adjoint).
 Workloads: plate (default, the headline), cylinder (rigid, 200x200x3) and
 airfoil_flap (rigid airfoil + torsional flap, 1024x512x5, full GMRES).
 """
@@ -179,6 +181,47 @@ def run_batched(args):
     return res
 
 
+def run_adjoint(args):
+    """Discrete adjoint (ib_adjoint_run) against the plain forward steps.
+
+    Workload: the rigid cylinder (200x200x3, 157 points, Re = 100, dt =
+    4e-3) on the num_stat path, started from the flow after a few steps.
+    Forward: ib_advance of N steps.  Adjoint: ib_adjoint_run of N steps (its
+    forward pass, the recomputation from checkpoints and the reverse sweep;
+    gradients of the time-mean drag + 0.5 lift with respect to Re and the
+    initial vorticity), once with the whole run taped in HBM and once with a
+    checkpoint every 10 steps.  cost_ratio = adjoint wall / forward wall."""
+    from ib_parallel_amd import GpuSim, decks
+    p, b = decks.cylinder_case(num_stat=1)
+    sim = GpuSim(p, b)
+    for _ in range(3):
+        sim.advance()
+    n = args.adjoint_steps
+    sim.adjoint(n, [1.0], [0.5], ckpt_every=None)  # set-up (dA/dRe, buffers) outside the timed regions
+    sim.adjoint(n, [1.0], [0.5], ckpt_every=10)
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        sim.advance()
+    device_sync()
+    tf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sim.adjoint(n, [1.0], [0.5], ckpt_every=None)
+    device_sync()
+    ta = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sim.adjoint(n, [1.0], [0.5], ckpt_every=10)
+    device_sync()
+    tc = time.perf_counter() - t0
+    res = {"workload": "rigid cylinder 200x200x3, 157 pts, Re=100, dt=4e-3, num_stat; d/dRe and d/domega0 of "
+                       "mean(drag + 0.5 lift)",
+           "steps": n, "forward_ms_per_step": round(tf / n * 1e3, 3),
+           "adjoint_ms_per_step": round(ta / n * 1e3, 3), "cost_ratio": round(ta / tf, 3),
+           "adjoint_ckpt10_ms_per_step": round(tc / n * 1e3, 3), "cost_ratio_ckpt10": round(tc / tf, 3)}
+    sim.close()
+    return res
+
+
 def copy_calibration(mb):
     """A device-to-device copy of exactly mb MiB (a plain copy kernel of known
     size for calibrating FETCH_SIZE / WRITE_SIZE in counter runs)."""
@@ -249,6 +292,8 @@ def main():
                     help="1: the batched step's launches go to the stream directly instead of a graph replay")
     ap.add_argument("--pmc-copy-mb", type=float, default=0.0,
                     help="run a device-to-device copy of this size first (calibrates counter runs)")
+    ap.add_argument("--adjoint-steps", type=int, default=40, help="steps of the adjoint measurement (0: skip)")
+    ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint measurement (one JSON object)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path",
@@ -257,6 +302,9 @@ def main():
 
     if args.pmc_copy_mb > 0:
         copy_calibration(args.pmc_copy_mb)
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": run_adjoint(args)}), flush=True)
+        return
     if args.batch_only:
         print(json.dumps({"batched": run_batched(args)}), flush=True)
         return
@@ -285,6 +333,8 @@ def main():
     }
     if args.batch > 0:
         out["batched"] = run_batched(args)
+    if args.adjoint_steps > 0:
+        out["adjoint"] = run_adjoint(args)
     if not args.no_cpu:
         cpu = run_cpu(args, g["its"] / args.steps)
         out["cpu_baseline"] = cpu

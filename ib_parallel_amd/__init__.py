@@ -9,6 +9,8 @@ interface.  There is no CPU fallback: if the library or a GPU is missing,
 import ctypes as C
 import os
 
+import numpy as np
+
 from . import abi
 from .sim import IBError, Simulation
 
@@ -46,6 +48,23 @@ class GpuSim(Simulation):
         mode, nfaces, builds = C.c_int32(), C.c_int32(), C.c_int64()
         self._check(self._f["schur_info"](self.h, C.byref(mode), C.byref(nfaces), C.byref(builds)))
         return mode.value, nfaces.value, builds.value
+
+    def adjoint(self, nsteps, cx, cy, ckpt_every=None, omega_grad=True):
+        """Advance nsteps steps and return (J, dJ/dRe, dJ/domega0) for the
+        time-mean objective J = mean over the steps of sum_b cx[b] fx_b +
+        cy[b] fy_b; dJ/domega0 has the layout of vort2flux's omega (None
+        unless omega_grad).  ckpt_every None: the whole run is taped in HBM
+        when it fits, else checkpoints every ceil(sqrt(nsteps)) steps.  Rigid
+        bodies with num_stat = 1."""
+        cx = np.ascontiguousarray(np.broadcast_to(np.asarray(cx, dtype=np.float64), (self.nbodies,)))
+        cy = np.ascontiguousarray(np.broadcast_to(np.asarray(cy, dtype=np.float64), (self.nbodies,)))
+        K = int(ckpt_every) if ckpt_every else 0
+        J, dRe = C.c_double(), C.c_double()
+        g = np.empty(self.L * self.P) if omega_grad else None
+        self._check(self._f["adjoint_run"](self.h, int(nsteps), K, abi.dptr(cx), abi.dptr(cy), C.byref(J),
+                                           C.byref(dRe), abi.dptr(g)))
+        self.itime.value += int(nsteps)
+        return J.value, dRe.value, g
 
     def kernel_timing(self, kclass, enable=True):
         self._check(self._f["kernel_timing"](self.h, kclass, 1 if enable else 0))

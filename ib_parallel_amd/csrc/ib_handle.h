@@ -162,6 +162,17 @@ struct ib_handle {
   int64_t sd_nnz_eval = 0, sd_nnz_kept = 0;
   DBuf<int> d_pidx, d_used, d_uptr, d_uj;
   DBuf<double> d_pw, d_uw;
+  // discrete adjoint (allocated on first use): d/dRe of the spectral tables
+  // and of the num_stat matrix, checkpoints, one segment's tape, adjoint
+  // state and scratch
+  bool adj_ready = false;
+  DBuf<double> dlam1i, ll, dA, cw;  // d lam1i/dRe, lam1 lam1i, dA/dRe, objective weights
+  DBuf<double> aw, aro, aqx, aqy, aws, af, arf, af2, racc;
+  DBuf<double> aP1, aP2, aD1, aD2, aD3, asb, aFx, aFy, anlx, anly, aub, avb;
+  DBuf<double> albb, albk, aobk, asbcb, abct, adot, adJ;
+  DBuf<double> c_w, c_ro, c_qx, c_qy, t_w, t_qx, t_qy, t_ws, t_f, fin_w, fin_ro, fin_qx, fin_qy;
+  DBuf<double> fin_s, fin_f, fin_fr;
+  double* tape_ws = nullptr;  // set: advance copies the predicted omega (all levels) here
   // pressure post-processing (allocated on first use): cell-grid DST plans
   // and table, corner products (two levels), rhs, solution, scratch
   bool pres_ready = false;
@@ -288,14 +299,16 @@ struct ib_handle {
   // keys[c0+c] (w null).  Leaves the level-1 streamfunctions in Sb (stride P)
   // and returns their boundary vectors (stride nbc, or zbc with stride 0 when
   // there is a single level).
-  const double* column_batch(int c0, int nc, const double* wts, const int* keys, long* bc_stride) {
+  const double* column_batch(int c0, int nc, const double* wts, const int* keys, long* bc_stride,
+                             const double* ainv_tab = nullptr) {
     const long P = g.P, nbc = g.nbc, LP = (long)L * P;
     DstLaunch a;
     a.batch = nc; a.strided = true; a.out[0] = T0b; a.s_out = P;
     a.c0 = c0; a.w = wts; a.key = keys;  // the input is formed from the weights or the face list
     xpass(X_POINTROT, a);
     DstLaunch b;
-    b.batch = nc; b.strided = true; b.in[0] = T0b; b.s_in = P; b.out[0] = T1b; b.s_out = P; b.tab = lam1i;
+    b.batch = nc; b.strided = true; b.in[0] = T0b; b.s_in = P; b.out[0] = T1b; b.s_out = P;
+    b.tab = ainv_tab ? ainv_tab : lam1i;
     b.div = normalize;
     ypass(Y_SCALE2, b);
     DstLaunch c;
@@ -988,6 +1001,7 @@ struct ib_handle {
       c.in[0] = t2; c.out[0] = omega + k * P;
       xpass(X_PLAIN, c);
     }
+    if (tape_ws) copy(st, tape_ws, omega, (long)L * P);
     vort2flux_dev(omega, s, qx, qy);
 
     int status = IB_OK;
@@ -1026,6 +1040,293 @@ struct ib_handle {
     vort2flux_dev(omega, s, qx, qy);
     sync();
     return status;
+  }
+
+  // ------------------------------------------------------ discrete adjoint
+  // (no reference counterpart).  J = sum over the steps of sum over bodies of
+  // cx_b drag_b + cy_b lift_b (write_force2's fx, fy), divided by the number
+  // of steps.  Rigid bodies with num_stat: a step is predictor, vort2flux,
+  // f = A^-1 E(q* + q0) with the symmetric num_stat matrix A, corrector,
+  // vort2flux; the adjoint runs these backwards with the transposed
+  // operators (the 2-D DST D is symmetric with D D = normalize I, the
+  // spectral tables are diagonal, E^T <-> E, rot^T = curl with zero boundary
+  // values, the stencil transposes of ib_adjoint.hip).  Re enters through
+  // lam1, lam1i, vfac and A.  With d lam1i/dRe = (lam1i - lam1i^2)/Re and
+  // d lam1/dRe = (1/normalize - lam1)/Re every spectral Re term reduces to
+  // node products of arrays the sweep already holds; dA/dRe is assembled
+  // once by the batched columns with d lam1i/dRe in place of lam1i.
+  // The steps of the last segment are taped during the forward pass; the
+  // other segments restart from checkpoints (every K steps) and are
+  // recomputed once, with a tape, before their adjoint.
+  void adjoint_setup(int nsteps, int K) {
+    const long P = g.P, LP = (long)L * P, nbc = g.nbc;
+    if (!adj_ready) {
+      const double pi = 4.0 * std::atan(1.0), del2 = delta * delta;
+      std::vector<double> h1i((size_t)LP), hll((size_t)LP);
+      for (int j = 2; j <= g.n; ++j)
+        for (int i = 2; i <= g.m; ++i) {
+          const double lam = -2.0 * (std::cos(pi * (double)(i - 1) / (double)g.m) +
+                                      std::cos(pi * (double)(j - 1) / (double)g.n) - 2.0);
+          const long q = (long)(i - 2) + (long)(g.m - 1) * (j - 2);
+          for (int k = 0; k < L; ++k) {
+            const double a = 0.5 * dt * lam / (del2 * std::pow(4.0, (double)k));
+            const double li = 1.0 / (1.0 + a / Re), l1 = (1.0 - a / Re) / normalize;
+            h1i[(size_t)k * P + q] = li * li * a / (Re * Re);
+            hll[(size_t)k * P + q] = l1 * li;
+          }
+        }
+      dlam1i.alloc(LP); ll.alloc(LP);
+      IB_HIP(hipMemcpy(dlam1i.p, h1i.data(), sizeof(double) * LP, hipMemcpyHostToDevice));
+      IB_HIP(hipMemcpy(ll.p, hll.data(), sizeof(double) * LP, hipMemcpyHostToDevice));
+      // dA/dRe: the num_stat columns with d lam1i / dRe in the ainv stage
+      dA.alloc((size_t)nf * nf);
+      alloc_assembly(nf);
+      for (int c0 = 0; c0 < nf; c0 += ncol) {
+        const int nc = std::min(ncol, nf - c0);
+        long sbc = 0;
+        const double* bc = column_batch(c0, nc, w, csr.key, &sbc, dlam1i);
+        schur_columns(st, g, nf, c0, nc, w, csr.key, Sb, g.P, bc, sbc, no_tors, tb, th, dt, delta, dA);
+      }
+      sync();
+      trim_columns();
+      aw.alloc(LP); aro.alloc(LP); aws.alloc(LP); asb.alloc(LP); racc.alloc(P);
+      aqx.alloc((size_t)L * g.Fx); aqy.alloc((size_t)L * g.Fy);
+      af.alloc(nf); arf.alloc(nf); af2.alloc(nf); cw.alloc(nf);
+      aP1.alloc(P); aP2.alloc(P); aD1.alloc(P); aD2.alloc(P); aD3.alloc(P);
+      aFx.alloc(g.Fx); aFy.alloc(g.Fy);
+      const long NA = (long)(g.m + 1) * (g.n + 1);
+      anlx.alloc(NA); anly.alloc(NA); aub.alloc(NA); avb.alloc(NA);
+      albb.alloc((size_t)L * nbc); albk.alloc(nbc); aobk.alloc(nbc); asbcb.alloc(nbc); abct.alloc(nbc);
+      adot.alloc(1); adJ.alloc(1);
+      fin_w.alloc(LP); fin_ro.alloc(LP); fin_qx.alloc((size_t)L * g.Fx); fin_qy.alloc((size_t)L * g.Fy);
+      fin_s.alloc(LP); fin_f.alloc(nf); fin_fr.alloc(nf);
+      adj_ready = true;
+    }
+    const int nseg = (nsteps + K - 1) / K;
+    const size_t need_c = (size_t)std::max(nseg - 1, 1), need_t = (size_t)std::min(K, nsteps);
+    if (c_w.n < need_c * LP) {
+      c_w.alloc(need_c * LP); c_ro.alloc(need_c * LP);
+      c_qx.alloc(need_c * L * g.Fx); c_qy.alloc(need_c * L * g.Fy);
+    }
+    if (t_w.n < need_t * LP) {
+      t_w.alloc(need_t * LP); t_ws.alloc(need_t * LP);
+      t_qx.alloc(need_t * L * g.Fx); t_qy.alloc(need_t * L * g.Fy); t_f.alloc(need_t * nf);
+    }
+  }
+
+  // HBM bytes of one taped step and of one checkpoint
+  size_t tape_bytes() const { return sizeof(double) * ((size_t)2 * L * g.P + (size_t)L * (g.Fx + g.Fy) + nf); }
+  size_t ckpt_bytes() const { return sizeof(double) * ((size_t)2 * L * g.P + (size_t)L * (g.Fx + g.Fy)); }
+  // ckpt_every <= 0: tape the whole run when it fits in half of the free
+  // HBM, else checkpoints every ceil(sqrt(nsteps)) steps
+  int adjoint_segment(int nsteps, int K) {
+    if (K > 0) return std::min(K, nsteps);
+    size_t fr = 0, tot = 0;
+    IB_HIP(hipMemGetInfo(&fr, &tot));
+    const size_t have = c_w.n * sizeof(double) + t_w.n * sizeof(double) * 3;  // reusable buffers
+    const size_t budget = (fr + have) / 2;
+    if ((size_t)nsteps * tape_bytes() <= budget) return nsteps;
+    return std::max(1, (int)std::ceil(std::sqrt((double)nsteps)));
+  }
+
+  void save_state(double* w_, double* ro_, double* qx_, double* qy_) {
+    const long LP = (long)L * g.P;
+    copy(st, w_, omega, LP);
+    if (ro_) copy(st, ro_, rhs_old, LP);
+    copy(st, qx_, qx, (long)L * g.Fx); copy(st, qy_, qy, (long)L * g.Fy);
+  }
+  void load_state(const double* w_, const double* ro_, const double* qx_, const double* qy_) {
+    const long LP = (long)L * g.P;
+    copy(st, omega, w_, LP); copy(st, rhs_old, ro_, LP);
+    copy(st, qx, qx_, (long)L * g.Fx); copy(st, qy, qy_, (long)L * g.Fy);
+  }
+  // one taped step: the step-start omega and q, the predicted omega, f
+  int taped_advance(int sl) {
+    const long LP = (long)L * g.P;
+    save_state(t_w + sl * LP, nullptr, t_qx + (long)sl * L * g.Fx, t_qy + (long)sl * L * g.Fy);
+    tape_ws = t_ws + sl * LP;
+    const int rc = advance();
+    tape_ws = nullptr;
+    copy(st, t_f + (long)sl * nf, f, nf);
+    return rc;
+  }
+
+  // transpose of vort2flux: wb (all levels) and qxb, qyb (all levels) are the
+  // adjoints of its outputs; on return wb is the adjoint of its input
+  // omega, and qxb, qyb are zero
+  void v2f_adj(double* wb, double* qxb, double* qyb) {
+    const long P = g.P, nbc = g.nbc;
+    fill(st, asb, (long)L * P, 0.0);
+    for (int k = 0; k < L; ++k) {
+      double* qxk = qxb + (long)k * g.Fx;
+      double* qyk = qyb + (long)k * g.Fy;
+      rot_acc(st, g, qxk, qyk, asb + k * P);
+      if (k < L - 1) curl_adj_bc(st, g, qxk, qyk, asbcb);
+      ctci_dev(asb + k * P, aP2, nullptr);
+      axpy(st, P, 1.0, aP2, wb + k * P);
+      if (k < L - 1) {
+        applybc_adj(st, g, aP2, 1.0, abct);
+        axpy(st, nbc, 1.0, abct, asbcb);
+        getbc_adj(st, g, asbcb, 1.0, asb + (k + 1) * P);
+      }
+    }
+    fill(st, qxb, (long)L * g.Fx, 0.0);
+    fill(st, qyb, (long)L * g.Fy, 0.0);
+    for (int k = L - 1; k >= 1; --k) coarsify_adj(st, g, wb + k * P, wb + (k - 1) * P);
+  }
+
+  // one step backwards; tape slot sl holds the step-start omega and q, the
+  // predicted omega and f; (aw, aro, aqx, aqy) enter as the adjoint of the
+  // step-end state and leave as that of the step-start state
+  void adjoint_step(int sl, bool first) {
+    const long P = g.P, LP = (long)L * P, nbc = g.nbc;
+    const double* w0 = t_w + sl * LP;
+    const double* qx0 = t_qx + (long)sl * L * g.Fx;
+    const double* qy0 = t_qy + (long)sl * L * g.Fy;
+    const double* ws0 = t_ws + sl * LP;
+    const double* f0 = t_f + (long)sl * nf;
+    const double invRe = 1.0 / Re;
+    // the end-of-step vort2flux
+    v2f_adj(aw, aqx, aqy);
+    // corrector omega_0 -= ainv(rot(E^T f)): the adjoint of f (J depends on f
+    // through cw), and its Re term -<aw_0 - ainv(aw_0), ainv(v)>/Re
+    copy(st, af, cw, nf);
+    ainv_dev(aw, aP1, false);
+    curl_getbc(st, g, aP1, zbc, aFx, aFy, nullptr);
+    regT(st, g, nf, w, csr.key, aFx, aFy, 0.0, nullptr, arf);
+    axpy(st, nf, -1.0, arf, af);
+    reg(st, g, csr, w, f0, hx, hy);
+    ainv_dev(nullptr, aP2, false);
+    re_corr(st, P, aw, aP1, aP2, invRe, racc);
+    // f = A^-1 rhsf (A symmetric): the adjoint of rhsf, and the Re term
+    // -<rhsf bar, dA/dRe f>
+    gemv(st, nf, Ainv, af, arf);
+    gemv_symup(st, nf, dA, f0, af2);
+    dot_acc(st, nf, arf, af2, -1.0, adot);
+    // rhsf = E(q*_0 + q0): the adjoint of q*_0
+    fill(st, aFx, g.Fx, 0.0);
+    fill(st, aFy, g.Fy, 0.0);
+    reg(st, g, csr, w, arf, aFx, aFy);
+    axpy(st, g.Fx, 1.0, aFx, aqx);
+    axpy(st, g.Fy, 1.0, aFy, aqy);
+    // the vort2flux after the predictor
+    v2f_adj(aw, aqx, aqy);
+    copy(st, aws, aw, LP);
+    fill(st, aw, LP, 0.0);
+    fill(st, albb, (long)L * nbc, 0.0);
+    // predictor omega*_k = D(lam1i o (D r_k + lam1 o D omega_k)), fine to
+    // coarse (the reverse of its coarse-to-fine order)
+    for (int k = 0; k < L; ++k) {
+      const double* wk = w0 + k * P;
+      if (k < L - 1) {
+        get_bc(st, g, w0 + (k + 1) * P, P, albk, nbc, 1, 0.25);
+        get_bc(st, g, ws0 + (k + 1) * P, P, aobk, nbc, 1, 0.25);
+      } else {
+        fill(st, albk, nbc, 0.0);
+        fill(st, aobk, nbc, 0.0);
+      }
+      // rbar = D(lam1i o D wsb_k), D(lam1 lam1i o D wsb_k): one x pass, two
+      // column passes, one x pass over both
+      DstLaunch a;
+      a.in[0] = aws + k * P; a.out[0] = aD1;
+      xpass(X_PLAIN, a);
+      DstLaunch b1;
+      b1.in[0] = aD1; b1.out[0] = aD2; b1.tab = lam1i + k * P; b1.div = 1.0;
+      ypass(Y_SCALE2, b1);
+      DstLaunch b2;
+      b2.in[0] = aD1; b2.out[0] = aD3; b2.tab = ll + k * P; b2.div = 1.0;
+      ypass(Y_SCALE2, b2);
+      DstLaunch c;
+      c.batch = 2;
+      c.in[0] = aD2; c.out[0] = aP1;
+      c.in[1] = aD3; c.out[1] = aP2;
+      xpass(X_PLAIN, c);
+      re_pred(st, P, ws0 + k * P, wk, aws + k * P, aP1, aP2, 1.0 / normalize, invRe, racc, aw + k * P);
+      // the boundary terms vfac (apply_bc(lastbc) + apply_bc(omegabc))
+      applybc_adj(st, g, aP1, 1.0, asbcb);
+      dot_acc(st, nbc, asbcb, albk, -vfac[k] * invRe, adot);
+      dot_acc(st, nbc, asbcb, aobk, -vfac[k] * invRe, adot);
+      axpy(st, nbc, vfac[k], asbcb, albb + k * nbc);
+      if (k < L - 1) getbc_adj(st, g, asbcb, 0.25 * vfac[k], aws + (k + 1) * P);
+      // r = con1 rhs + con2 rold + ..., rhs_old <- rhs
+      rhs_adj(st, P, aP1, first ? con1[k] + con2[k] : con1[k], first ? 0.0 : con2[k], aro + k * P, aD2);
+      nl_adj(st, g, wk, qx0 + (long)k * g.Fx, qy0 + (long)k * g.Fy, q0x[k], albk, aD2, anlx, anly, aub, avb,
+             aw + k * P, albb + k * nbc, aqx + (long)k * g.Fx, aqy + (long)k * g.Fy);
+    }
+    // lastbc_k = 0.25 get_bc(omega_(k+1))
+    for (int k = 0; k < L - 1; ++k) getbc_adj(st, g, albb + k * nbc, 0.25, aw + (k + 1) * P);
+  }
+
+  int adjoint_run(int nsteps, int Kin, const double* cx, const double* cy, double* J, double* dRe, double* dw0) {
+    if (!prm.num_stat || ntors > 0 || sd_on || nf == 0)
+      return fail(IB_E_UNSUPPORTED, "the adjoint covers rigid bodies with num_stat = 1");
+    if (nsteps < 1) return fail(IB_E_INVAL, "nsteps must be >= 1");
+    const int K = adjoint_segment(nsteps, Kin);
+    adjoint_setup(nsteps, K);
+    const long LP = (long)L * g.P;
+    const long CQX = (long)L * g.Fx, CQY = (long)L * g.Fy;
+    // objective weights per force entry: write_force2's 2 delta / dt, time mean
+    std::vector<double> hw(nf);
+    for (int k = 0; k < nf; ++k)
+      hw[k] = (h_ccomp[k] == 1 ? cx[h_cbody[k]] : cy[h_cbody[k]]) * 2.0 * delta / dt / (double)nsteps;
+    IB_HIP(hipMemcpy(cw.p, hw.data(), sizeof(double) * nf, hipMemcpyHostToDevice));
+    fill(st, adJ, 1, 0.0);
+    fill(st, adot, 1, 0.0);
+    fill(st, racc, g.P, 0.0);
+    const int it0 = itime;
+    const int nseg = (nsteps + K - 1) / K, last0 = (nseg - 1) * K;
+    // forward: checkpoints at the starts of the segments before the last,
+    // the last segment taped
+    for (int n = 0; n < nsteps; ++n) {
+      int rc;
+      if (n >= last0) {
+        rc = taped_advance(n - last0);
+      } else {
+        if (n % K == 0) {
+          const long c = n / K;
+          save_state(c_w + c * LP, c_ro + c * LP, c_qx + c * CQX, c_qy + c * CQY);
+        }
+        rc = advance();
+      }
+      if (rc != IB_OK) return rc;
+      dot_acc(st, nf, cw, f, 1.0, adJ);
+    }
+    if (nseg > 1) {
+      save_state(fin_w, fin_ro, fin_qx, fin_qy);
+      copy(st, fin_s, s, LP); copy(st, fin_f, f, nf); copy(st, fin_fr, f_rdst, nf);
+    }
+    // backward, segment by segment
+    fill(st, aw, LP, 0.0); fill(st, aro, LP, 0.0);
+    fill(st, aqx, CQX, 0.0); fill(st, aqy, CQY, 0.0);
+    for (int sgi = nseg - 1; sgi >= 0; --sgi) {
+      const int n0 = sgi * K, n1 = std::min(nsteps, n0 + K);
+      if (sgi < nseg - 1) {
+        load_state(c_w + (long)sgi * LP, c_ro + (long)sgi * LP, c_qx + (long)sgi * CQX, c_qy + (long)sgi * CQY);
+        itime = it0 + n0;
+        for (int n = n0; n < n1; ++n) {
+          const int rc = taped_advance(n - n0);
+          if (rc != IB_OK) return rc;
+        }
+      }
+      for (int n = n1 - 1; n >= n0; --n) adjoint_step(n - n0, it0 + n + 1 == 1);
+    }
+    // the initial q is vort2flux of the initial omega
+    if (dw0) v2f_adj(aw, aqx, aqy);
+    sum_acc(st, g.P, racc, 1.0, adot);
+    if (nseg > 1) {
+      // leave the handle in the forward end state
+      load_state(fin_w, fin_ro, fin_qx, fin_qy);
+      copy(st, s, fin_s, LP); copy(st, f, fin_f, nf); copy(st, f_rdst, fin_fr, nf);
+    }
+    itime = it0 + nsteps;
+    double hv[2] = {0.0, 0.0};
+    IB_HIP(hipMemcpyAsync(hv, adJ.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    IB_HIP(hipMemcpyAsync(hv + 1, adot.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    if (dw0) IB_HIP(hipMemcpyAsync(dw0, aw.p, sizeof(double) * LP, hipMemcpyDeviceToHost, st));
+    sync();
+    if (J) *J = hv[0];
+    if (dRe) *dRe = hv[1];
+    return IB_OK;
   }
 
   // ------------------------------------------------------------------ setup

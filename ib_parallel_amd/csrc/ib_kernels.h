@@ -303,6 +303,28 @@ void sd_interp(hipStream_t st, int nf, int np, const int* pidx, const double* pw
 void sd_ptf(hipStream_t st, int nused, const int* used, const int* uptr, const int* uj, const double* uw,
             const double* f, double* h0);
 
+// ------------------------------------------------------ discrete adjoint ---
+// transposes of curl's boundary reads, get_bc, apply_bc, coarsify and of the
+// nonlinear term's Jacobian (ib_adjoint.hip)
+void curl_adj_bc(hipStream_t st, const Geom& g, const double* qx, const double* qy, double* bcb);
+void getbc_adj(hipStream_t st, const Geom& g, const double* bcb, double fac, double* sc);
+void applybc_adj(hipStream_t st, const Geom& g, const double* vb, double fac, double* bcb);
+void coarsify_adj(hipStream_t st, const Geom& g, double* wc, double* wf);
+void nl_adj(hipStream_t st, const Geom& g, const double* w, const double* qx, const double* qy, double q0x,
+            const double* lastbc, const double* rb, double* nlxb, double* nlyb, double* ub, double* vb, double* wb,
+            double* lbb, double* qxb, double* qyb);
+void rot_acc(hipStream_t st, const Geom& g, const double* hx, const double* hy, double* v);
+void rhs_adj(hipStream_t st, long n, const double* r, double c1, double c2, double* ro, double* rb);
+void re_pred(hipStream_t st, long n, const double* ws, const double* w, const double* wsb, const double* rb,
+             const double* dlz, double invN, double invRe, double* acc, double* wb);
+void re_corr(hipStream_t st, long n, const double* wb, const double* aiwb, const double* aiv, double invRe,
+             double* acc);
+void sum_acc(hipStream_t st, long n, const double* x, double scale, double* out);
+void axpy(hipStream_t st, long n, double a, const double* x, double* y);
+void mul(hipStream_t st, long n, const double* x, const double* t, double* y);
+void dot_acc(hipStream_t st, long n, const double* a, const double* b, double scale, double* out);
+void gemv_symup(hipStream_t st, int n, const double* A, const double* x, double* y);
+
 // ---------------------------------------------------- torsional bodies ---
 // y[body entries] += b_times2(x); one workgroup per body
 void body_btimes(hipStream_t st, int nt, const TorsBody* tb, const double* theta, double dt, double delta,

@@ -392,6 +392,14 @@ int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable) {
   API_CATCH(h)
 }
 
+int ib_adjoint_run(ib_handle* h, int32_t nsteps, int32_t ckpt_every, const double* cx, const double* cy, double* J,
+                   double* dJ_dRe, double* dJ_domega0) {
+  if (!h || !cx || !cy) return IB_E_INVAL;
+  API_TRY
+  return h->adjoint_run(nsteps, ckpt_every, cx, cy, J, dJ_dRe, dJ_domega0);
+  API_CATCH(h)
+}
+
 int ib_kernel_timing_read(ib_handle* h, int32_t kclass, int64_t* launches, double* total_ms, double* total_bytes) {
   if (!h || kclass < 0 || kclass > 4) return IB_E_INVAL;
   API_TRY

@@ -246,6 +246,24 @@ int ib_batch_info(const ib_batch* b, int32_t* nsim, int32_t* mode);
 /* message of the last failing call on b (or of ib_batch_create when b is NULL) */
 int ib_batch_last_error(const ib_batch* b, char* buf, int32_t len);
 
+/* --- discrete adjoint (no reference counterpart): advances h by nsteps
+ * steps and returns the time-mean weighted force objective
+ *   J = (1/nsteps) sum over the steps of sum_b (cx[b] fx_b + cy[b] fy_b)
+ * (fx, fy as ib_get_forces(which = 0) reports them after each step), its
+ * derivative with respect to Re, and, when dJ_domega0 is not NULL, with
+ * respect to the vorticity before the first step (mgridlev blocks of
+ * (m-1)(n-1), the layout of ib_op_vort2flux's omega: the state is taken as
+ * q = vort2flux(omega) of that vorticity).  The gradients are those of the
+ * discrete step (the transposed operators run backwards).  The steps are
+ * taped in HBM in segments of ckpt_every steps: the last segment during the
+ * forward pass, the others recomputed once from checkpoints; ckpt_every <= 0
+ * tapes the whole run when that fits in half of the free HBM, else uses
+ * segments of ceil(sqrt(nsteps)).  Rigid bodies with num_stat = 1 only (else
+ * IB_E_UNSUPPORTED).  h is left in the state
+ * after the nsteps steps, as ib_advance would leave it. --- */
+int ib_adjoint_run(ib_handle* h, int32_t nsteps, int32_t ckpt_every, const double* cx, const double* cy, double* J,
+                   double* dJ_dRe, double* dJ_domega0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,169 @@
+"""Discrete adjoint (ib_adjoint_run) against central finite differences of the
+GPU step, whose parity with the oracle the other GPU tests establish.
+
+J = mean over the steps of sum_b cx[b] fx_b + cy[b] fy_b.  dJ/dRe is checked
+against (J(Re + h) - J(Re - h)) / 2h from fresh handles started from the same
+state; dJ/domega0 along random directions against the same difference in the
+initial vorticity.  Bar: relative error 1e-5 (the FD truncation error at these
+steps is below 1e-8)."""
+import numpy as np
+import pytest
+
+from ib_parallel_amd import abi, decks
+
+pytestmark = pytest.mark.gpu
+
+CX = np.array([1.0])
+CY = np.array([0.5])
+
+
+def case(re=100.0, istart=0, m=64, n=64, mgridlev=3):
+    p, b = decks.cylinder_case(m=m, n=n, mgridlev=mgridlev, dt=0.01, npts=50, tol=1e-12, num_stat=1, re=re)
+    p.istart = istart
+    return p, b
+
+
+def sim_from(p, b, omega0, rhs_old):
+    from ib_parallel_amd import GpuSim
+    from ib_parallel_amd.adjoint import start_state
+    s = GpuSim(p, b)
+    start_state(s, omega0, rhs_old)
+    return s
+
+
+def J_forward(p, b, omega0, rhs_old, nsteps, cx=CX, cy=CY):
+    s = sim_from(p, b, omega0, rhs_old)
+    J = 0.0
+    for _ in range(nsteps):
+        s.advance()
+        fx, fy = s.forces(0)
+        J += float(np.dot(cx, fx) + np.dot(cy, fy))
+    return J / nsteps, s
+
+
+def state_of(s):
+    lv = [s.level(k + 1) for k in range(s.L)]
+    return np.concatenate([d["omega"] for d in lv]), np.concatenate([d["rhs_old"] for d in lv])
+
+
+@pytest.fixture(scope="module")
+def warm():
+    """The flow after 15 steps from rest at Re = 100 (the AB2 branch, a shed wake forming)."""
+    from ib_parallel_amd import GpuSim
+    p, b = case()
+    s = GpuSim(p, b)
+    for _ in range(15):
+        s.advance()
+    w, ro = state_of(s)
+    s.close()
+    return 15, w, ro
+
+
+def test_objective_and_end_state_match_plain_steps(warm):
+    it0, w0, ro0 = warm
+    p, b = case(istart=it0)
+    Jf, sf = J_forward(p, b, w0, ro0, 7)
+    sa = sim_from(p, b, w0, ro0)
+    Ja, dre, dw = sa.adjoint(7, CX, CY, ckpt_every=3)
+    assert abs(Ja - Jf) <= 1e-13 * abs(Jf)
+    assert sa.itime.value == sf.itime.value == it0 + 7
+    for k in range(1, p.mgridlev + 1):
+        a, f = sa.level(k), sf.level(k)
+        for key in ("omega", "s", "qx", "qy", "rhs_old"):
+            assert np.array_equal(a[key], f[key]), (k, key)
+    assert all(np.array_equal(x, y) for x, y in zip(sa.forces(0), sf.forces(0)))
+    # the handle steps on from there exactly like the plain one
+    sa.advance(); sf.advance()
+    assert np.array_equal(sa.forces(0)[0], sf.forces(0)[0])
+
+
+@pytest.mark.parametrize("nsteps,ckpt", ((1, 1), (6, 4), (9, 3), (7, None)))
+def test_dre_matches_finite_differences(warm, nsteps, ckpt):
+    it0, w0, ro0 = warm
+    re, h = 100.0, 0.01
+    p, b = case(re, istart=it0)
+    _, dre, _ = sim_from(p, b, w0, ro0).adjoint(nsteps, CX, CY, ckpt_every=ckpt, omega_grad=False)
+    Jp, _ = J_forward(*case(re + h, istart=it0), w0, ro0, nsteps)
+    Jm, _ = J_forward(*case(re - h, istart=it0), w0, ro0, nsteps)
+    fd = (Jp - Jm) / (2 * h)
+    print(f"nsteps {nsteps}: dJ/dRe adjoint {dre:.12e} fd {fd:.12e}")
+    assert abs(dre - fd) <= 1e-5 * abs(fd)
+
+
+@pytest.mark.parametrize("nsteps,ckpt,seed", ((1, 1, 0), (5, 2, 1)))
+def test_domega0_matches_finite_differences(warm, nsteps, ckpt, seed):
+    it0, w0, ro0 = warm
+    p, b = case(istart=it0)
+    _, _, dw = sim_from(p, b, w0, ro0).adjoint(nsteps, CX, CY, ckpt_every=ckpt)
+    rng = np.random.default_rng(seed)
+    for trial in range(2):
+        d = rng.standard_normal(w0.size)
+        eps = 1e-4 * np.abs(w0).max() / np.abs(d).max()
+        Jp, _ = J_forward(p, b, w0 + eps * d, ro0, nsteps)
+        Jm, _ = J_forward(p, b, w0 - eps * d, ro0, nsteps)
+        fd = (Jp - Jm) / (2 * eps)
+        ad = float(np.dot(dw, d))
+        print(f"nsteps {nsteps} trial {trial}: <dJ/dw0, d> adjoint {ad:.12e} fd {fd:.12e}")
+        assert abs(ad - fd) <= 1e-5 * abs(fd)
+
+
+def test_gradients_from_rest_include_the_euler_step():
+    """From istart = 0 the first step is Euler; omega0 is a small random
+    vorticity so that the nonlinear term is exercised from the first step."""
+    p, b = case()
+    rng = np.random.default_rng(7)
+    L, P = p.mgridlev, (p.m - 1) * (p.n - 1)
+    w0 = 0.1 * rng.standard_normal(L * P)
+    ro0 = np.zeros(L * P)
+    _, dre, dw = sim_from(p, b, w0, ro0).adjoint(4, CX, CY, ckpt_every=2)
+    h = 0.01
+    Jp, _ = J_forward(*case(100.0 + h), w0, ro0, 4)
+    Jm, _ = J_forward(*case(100.0 - h), w0, ro0, 4)
+    assert abs(dre - (Jp - Jm) / (2 * h)) <= 1e-5 * abs((Jp - Jm) / (2 * h))
+    d = rng.standard_normal(L * P)
+    eps = 1e-5
+    Jp, _ = J_forward(p, b, w0 + eps * d, ro0, 4)
+    Jm, _ = J_forward(p, b, w0 - eps * d, ro0, 4)
+    fd = (Jp - Jm) / (2 * eps)
+    assert abs(np.dot(dw, d) - fd) <= 1e-5 * abs(fd)
+
+
+def test_torch_function_gives_the_adjoint_gradients(warm):
+    import torch
+    from ib_parallel_amd.adjoint import ForceObjective
+    it0, w0, ro0 = warm
+    p, b = case(istart=it0)
+    re = torch.tensor(100.0, dtype=torch.float64, requires_grad=True)
+    w = torch.tensor(w0, dtype=torch.float64, requires_grad=True)
+    J = ForceObjective.apply(re, w, p, b, 3, CX, CY, 2, ro0)
+    gre, gw = torch.autograd.grad(3.0 * J, (re, w))
+    Ja, dre, dw = sim_from(p, b, w0, ro0).adjoint(3, CX, CY, ckpt_every=2)
+    assert float(J.detach()) == Ja
+    assert float(gre) == pytest.approx(3.0 * dre, rel=1e-14)
+    assert np.allclose(gw.numpy(), 3.0 * dw, rtol=1e-14, atol=0)
+
+
+def test_unsupported_configurations_are_rejected():
+    from ib_parallel_amd import GpuSim, IBError
+    p, b = decks.cylinder_case(m=64, n=64, mgridlev=2, dt=0.01, npts=50, tol=1e-10, num_stat=0)
+    s = GpuSim(p, b)
+    with pytest.raises(IBError) as e:
+        s.adjoint(2, CX, CY)
+    assert e.value.code == abi.IB_E_UNSUPPORTED
+    pt, bt = decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-12)
+    st = GpuSim(pt, bt)
+    with pytest.raises(IBError) as e:
+        st.adjoint(2, [0.0] * st.nbodies, [1.0] * st.nbodies)
+    assert e.value.code == abi.IB_E_UNSUPPORTED
+
+
+def test_segmenting_does_not_change_the_gradients(warm):
+    """Recomputation from checkpoints reproduces the taped forward exactly, so
+    every segment length gives the same bits."""
+    it0, w0, ro0 = warm
+    p, b = case(istart=it0)
+    ref = sim_from(p, b, w0, ro0).adjoint(7, CX, CY, ckpt_every=None)
+    for K in (1, 2, 3, 7):
+        got = sim_from(p, b, w0, ro0).adjoint(7, CX, CY, ckpt_every=K)
+        assert got[0] == ref[0] and got[1] == ref[1], K
+        assert np.array_equal(got[2], ref[2]), K
