@@ -49,11 +49,12 @@ class GpuSim(Simulation):
         self._check(self._f["schur_info"](self.h, C.byref(mode), C.byref(nfaces), C.byref(builds)))
         return mode.value, nfaces.value, builds.value
 
-    def adjoint(self, nsteps, cx, cy, ckpt_every=None, omega_grad=True):
+    def adjoint(self, nsteps, cx, cy, ckpt_every=None, omega_grad=True, position_grad=False):
         """Advance nsteps steps and return (J, dJ/dRe, dJ/domega0) for the
         time-mean objective J = mean over the steps of sum_b cx[b] fx_b +
         cy[b] fy_b; dJ/domega0 has the layout of vort2flux's omega (None
-        unless omega_grad).  ckpt_every None: the whole run is taped in HBM
+        unless omega_grad); with position_grad also dJ/dxb and dJ/dyb per
+        body point (deck order).  ckpt_every None: the whole run is taped in HBM
         when it fits, else checkpoints every ceil(sqrt(nsteps)) steps.  Rigid
         bodies with num_stat = 1."""
         cx = np.ascontiguousarray(np.broadcast_to(np.asarray(cx, dtype=np.float64), (self.nbodies,)))
@@ -61,9 +62,14 @@ class GpuSim(Simulation):
         K = int(ckpt_every) if ckpt_every else 0
         J, dRe = C.c_double(), C.c_double()
         g = np.empty(self.L * self.P) if omega_grad else None
+        npts = self.nf // 2
+        gx = np.empty(npts) if position_grad else None
+        gy = np.empty(npts) if position_grad else None
         self._check(self._f["adjoint_run"](self.h, int(nsteps), K, abi.dptr(cx), abi.dptr(cy), C.byref(J),
-                                           C.byref(dRe), abi.dptr(g)))
+                                           C.byref(dRe), abi.dptr(g), abi.dptr(gx), abi.dptr(gy)))
         self.itime.value += int(nsteps)
+        if position_grad:
+            return J.value, dRe.value, g, gx, gy
         return J.value, dRe.value, g
 
     def kernel_timing(self, kclass, enable=True):

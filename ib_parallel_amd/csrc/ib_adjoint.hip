@@ -294,6 +294,121 @@ void re_corr(hipStream_t st, long n, const double* wb, const double* aiwb, const
   IB_LAUNCH_CHECK();
 }
 
+// ---- body-point positions (rigid bodies) ---------------------------------
+// With A = W G W^T on the faces the taps touch (G: the level-1 flux of a
+// unit face source, W: the weights), T(b, j) = sum_q w[j,q] G(b, tap[j,q])
+// and U(i, b) = sum_p w[i,p] G(tap[i,p], b), both stored nf x nF with the
+// entry index fastest: TT[j + nf b] = T(b, j), U[i + nf b].
+__global__ void k_green_tt(int nf, int nF, const double* __restrict__ w, const int* __restrict__ tapidx,
+                           const double* __restrict__ G, double* __restrict__ TT) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)nf * nF) return;
+  const int j = (int)(q % nf);
+  const long b = q / nf;
+  double acc = 0.0;
+  for (int t = 0; t < kTaps; ++t) acc += w[j * kTaps + t] * G[(long)tapidx[j * kTaps + t] * nF + b];
+  TT[q] = acc;
+}
+
+__global__ void k_green_u(int nf, int nF, const double* __restrict__ w, const int* __restrict__ tapidx,
+                          const double* __restrict__ G, double* __restrict__ U) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)nf * nF) return;
+  const int i = (int)(q % nf);
+  const long b = q / nf;
+  double acc = 0.0;
+  for (int t = 0; t < kTaps; ++t) acc += w[i * kTaps + t] * G[b * nF + tapidx[i * kTaps + t]];
+  U[q] = acc;
+}
+
+void green_tt_u(hipStream_t st, int nf, int nF, const double* w, const int* tapidx, const double* G, double* TT,
+                double* U) {
+  const long n = (long)nf * nF;
+  if (n <= 0) return;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_green_tt, dim3(blocks), dim3(256), 0, st, nf, nF, w, tapidx, G, TT);
+  IB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_green_u, dim3(blocks), dim3(256), 0, st, nf, nF, w, tapidx, G, U);
+  IB_LAUNCH_CHECK();
+}
+
+// out[k*49 + t] = (q + q0)(face of tap t of entry k): the level-1 flux the
+// regularisation reads (q0 on x faces only)
+__global__ void k_gather_taps(Geom g, int ntaps, const int* __restrict__ key, const double* __restrict__ qx,
+                              const double* __restrict__ qy, double q0x, double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntaps) return;
+  const int kk = key[t];
+  out[t] = kk < g.Fx ? qx[kk] + q0x : qy[kk - g.Fx] + 0.0;
+}
+
+void gather_taps(hipStream_t st, const Geom& g, int ntaps, const int* key, const double* qx, const double* qy,
+                 double q0x, double* out) {
+  if (ntaps <= 0) return;
+  hipLaunchKernelGGL(k_gather_taps, dim3(ceil_div(ntaps, 256)), dim3(256), 0, st, g, ntaps, key, qx, qy, q0x, out);
+  IB_LAUNCH_CHECK();
+}
+
+// Position terms of one step, per force entry i (one workgroup, a wavefront
+// per tap): with u the adjoint of rhsf, v = f, hb = curl(ainv(omega_0 bar))
+// and qt the taped (q* + q0) at the taps,
+//   X(i,t) = -v_i hb(tap) + u_i qt(i,t) - Y(i,t)
+//   Y(i,t) = sum_{j>i} s_j T(b, j) + sum_{j<i} s_j U(j, b) + u_i v_i (T(b, i) + U(i, b)),
+//   s_j = u_i v_j + v_i u_j,  b = tap (i,t)
+// (Y: the tap-weight derivative of <u, A_sym v> with A_sym the upper
+// triangle of A mirrored, the matrix num_stat factors), and
+//   gx[i] += sum_t dw/dx(i,t) X(i,t),  gy[i] += sum_t dw/dy(i,t) X(i,t).
+__global__ void __launch_bounds__(256) k_pos_terms(Geom g, int nf, const int* __restrict__ key,
+                                                   const int* __restrict__ tapidx, const double* __restrict__ dwx,
+                                                   const double* __restrict__ dwy, const double* __restrict__ TT,
+                                                   const double* __restrict__ U, const double* __restrict__ u,
+                                                   const double* __restrict__ v, const double* __restrict__ qt,
+                                                   const double* __restrict__ hbx, const double* __restrict__ hby,
+                                                   double* __restrict__ gx, double* __restrict__ gy) {
+  __shared__ double sx[4], sy[4];
+  const int i = blockIdx.x;
+  const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const double ui = u[i], vi = v[i];
+  double ax = 0.0, ay = 0.0;
+  for (int t = wv; t < kTaps; t += 4) {
+    const int q = i * kTaps + t;
+    const long b = tapidx[q];
+    const double* __restrict__ Tb = TT + b * nf;
+    const double* __restrict__ Ub = U + b * nf;
+    double y = 0.0;
+    for (int j = lane; j < nf; j += 64) {
+      const double sj = ui * v[j] + vi * u[j];
+      if (j > i) y += sj * Tb[j];
+      else if (j < i) y += sj * Ub[j];
+      else y += ui * vi * (Tb[j] + Ub[j]);
+    }
+    for (int off = 32; off > 0; off >>= 1) y += __shfl_xor(y, off);
+    const int kk = key[q];
+    const double hb = kk < g.Fx ? hbx[kk] : hby[kk - g.Fx];
+    const double X = -vi * hb + ui * qt[q] - y;
+    ax += dwx[q] * X;
+    ay += dwy[q] * X;
+  }
+  if (lane == 0) {
+    sx[wv] = ax;
+    sy[wv] = ay;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    gx[i] += ((sx[0] + sx[1]) + sx[2]) + sx[3];
+    gy[i] += ((sy[0] + sy[1]) + sy[2]) + sy[3];
+  }
+}
+
+void pos_terms(hipStream_t st, const Geom& g, int nf, const int* key, const int* tapidx, const double* dwx,
+               const double* dwy, const double* TT, const double* U, const double* u, const double* v,
+               const double* qt, const double* hbx, const double* hby, double* gx, double* gy) {
+  if (nf <= 0) return;
+  hipLaunchKernelGGL(k_pos_terms, dim3(nf), dim3(256), 0, st, g, nf, key, tapidx, dwx, dwy, TT, U, u, v, qt, hbx, hby,
+                     gx, gy);
+  IB_LAUNCH_CHECK();
+}
+
 // Elementwise helpers of the adjoint sweep
 __global__ void k_axpy(long n, double a, const double* __restrict__ x, double* __restrict__ y) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -98,6 +98,88 @@ void setup_reg(hipStream_t st, const Geom& g, int nf, const double* xb, const do
   IB_LAUNCH_CHECK();
 }
 
+// d deltafnc(x1, x2, dr) / d x2 (for the adjoint): the derivative in r1 of
+// each branch of deltafnc above, with the same literals, times d r1 / d x2 =
+// -sign(x1 - x2) / dr, over dr.
+__device__ double deltafnc_dx2(double x1, double x2, double dr, int f32) {
+  const double r = fabs(x1 - x2);
+  const double r1 = r / dr, r2 = r1 * r1, r3 = r2 * r1;
+  const double s3 = sqrt(3.0);
+  double dd;
+#define F(lit) (f32 ? (double)(lit##f) : (lit))
+  if (r1 <= 1.0) {
+    const double u = 0.5 * s3 * (2.0 * r1 - 1.0);
+    const double a5 = asin(u), da5 = s3 / sqrt(1.0 - u * u);
+    const double a8 = sqrt(1.0 - 12.0 * r2 + 12.0 * r1), da8 = (6.0 - 12.0 * r1) / a8;
+    const double c3 = F(-.1388888889) + 0.3472222222e-1 * a8;
+    const double c2 = -0.7121664902e-1 - 0.5208333333e-1 * a8 + F(0.2405626122) * a5;
+    const double c1 = F(-.2405626122) * a5 + F(-.3792313933) + F(.1012731481) * a8;
+    dd = 4.0 * 0.4166666667e-1 * r3 + 3.0 * c3 * r2 + 0.3472222222e-1 * da8 * r3 + 2.0 * c2 * r1 +
+         (-0.5208333333e-1 * da8 + F(0.2405626122) * da5) * r2 + c1 +
+         (F(-.2405626122) * da5 + F(.1012731481) * da8) * r1 + 0.8018753741e-1 * da5 - 0.4195601852e-1 * da8;
+  } else if (r1 <= 2.0) {
+    const double u = 0.5 * s3 * (-3.0 + 2.0 * r1);
+    const double a6 = asin(u), da6 = s3 / sqrt(1.0 - u * u);
+    const double a9 = sqrt(-23.0 + 36.0 * r1 - 12.0 * r2), da9 = (18.0 - 12.0 * r1) / a9;
+    const double c3 = F(.4861111111) - 0.1736111111e-1 * a9;
+    const double c2 = F(-1.143175026) + 0.7812500000e-1 * a9 + F(-.1202813061) * a6;
+    const double c1 = F(.8751991178) + F(.3608439183) * a6 + F(-.1548032407) * a9;
+    dd = 4.0 * -0.6250000000e-1 * r3 + 3.0 * c3 * r2 - 0.1736111111e-1 * da9 * r3 + 2.0 * c2 * r1 +
+         (0.7812500000e-1 * da9 + F(-.1202813061) * da6) * r2 + c1 +
+         (F(.3608439183) * da6 + F(-.1548032407) * da9) * r1 + F(-.2806563809) * da6 + F(.1150173611) * da9;
+  } else if (r1 <= 3.0) {
+    const double u = 0.5 * (2.0 * r1 - 5.0) * s3;
+    const double a1 = asin(u), da1 = s3 / sqrt(1.0 - u * u);
+    const double a7 = sqrt(-71.0 - 12.0 * r2 + 60.0 * r1), da7 = (30.0 - 12.0 * r1) / a7;
+    const double c3 = 0.3472222222e-2 * a7 + F(-.2638888889);
+    const double c2 = F(1.214391675) - 0.2604166667e-1 * a7 + 0.2405626122e-1 * a1;
+    const double c1 = F(-.1202813061) * a1 + F(-2.449273192) + 0.7262731481e-1 * a7;
+    dd = 4.0 * 0.2083333333e-1 * r3 + 3.0 * c3 * r2 + 0.3472222222e-2 * da7 * r3 + 2.0 * c2 * r1 +
+         (-0.2604166667e-1 * da7 + 0.2405626122e-1 * da1) * r2 + c1 +
+         (F(-.1202813061) * da1 + 0.7262731481e-1 * da7) * r1 + F(.1523563211) * da1 - 0.7306134259e-1 * da7;
+  } else {
+    dd = 0.0;
+  }
+#undef F
+  const double sg = (x1 - x2) > 0.0 ? 1.0 : ((x1 - x2) < 0.0 ? -1.0 : 0.0);
+  return -sg * dd / (dr * dr);
+}
+
+// d w / d xb and d w / d yb of every (entry, tap) of setup_reg (weights
+// w = delta^2 deltafnc(x, xb) deltafnc(y, yb) at the taps setup_reg chose)
+__global__ void k_setup_reg_d(Geom g, int nf, const double* __restrict__ xb, const double* __restrict__ yb,
+                              const int* __restrict__ comp, double delta, double offx, double offy, int f32,
+                              double* __restrict__ dwx, double* __restrict__ dwy) {
+#pragma clang fp contract(off)
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nf * kTaps) return;
+  const int k = idx / kTaps, t = idx - k * kTaps;
+  const int l = t / 7 - 3, p = t % 7 - 3;
+  const double d2 = 0.5 * delta;
+  const int ix = (int)((xb[k] + offx) / delta) + 1;
+  const int iy = (int)((yb[k] + offy) / delta) + 1;
+  double x, y;
+  if (comp[k] == 1) {
+    x = delta * (ix - 1 + p) - offx;
+    y = delta * (iy - 1 + l) - offy + d2;
+  } else {
+    x = delta * (ix - 1 + p) - offx + d2;
+    y = delta * (iy - 1 + l) - offy;
+  }
+  const double fx = deltafnc(x, xb[k], delta, f32), fy = deltafnc(y, yb[k], delta, f32);
+  dwx[idx] = delta * delta * deltafnc_dx2(x, xb[k], delta, f32) * fy;
+  dwy[idx] = delta * delta * fx * deltafnc_dx2(y, yb[k], delta, f32);
+}
+
+void setup_reg_d(hipStream_t st, const Geom& g, int nf, const double* xb, const double* yb, const int* comp,
+                 double delta, double offx, double offy, int f32lit, double* dwx, double* dwy) {
+  const int nt = nf * kTaps;
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_setup_reg_d, dim3(ceil_div(nt, 256)), dim3(256), 0, st, g, nf, xb, yb, comp, delta, offx,
+                     offy, f32lit, dwx, dwy);
+  IB_LAUNCH_CHECK();
+}
+
 __global__ void k_iota(int* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = i;

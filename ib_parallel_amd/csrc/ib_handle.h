@@ -173,6 +173,11 @@ struct ib_handle {
   DBuf<double> c_w, c_ro, c_qx, c_qy, t_w, t_qx, t_qy, t_ws, t_f, fin_w, fin_ro, fin_qx, fin_qy;
   DBuf<double> fin_s, fin_f, fin_fr;
   double* tape_ws = nullptr;  // set: advance copies the predicted omega (all levels) here
+  double* tape_qt = nullptr;  // set: advance gathers q* + q0 at the taps here
+  // body-point positions (rigid): d w/d xb, d w/d yb, G W^T and W G on the
+  // faces the taps touch, per-entry gradient accumulators, (q* + q0) tape
+  bool adj_pos = false, want_pos = false;
+  DBuf<double> dwx, dwy, TTm, Um, gpx, gpy, t_qt;
   // pressure post-processing (allocated on first use): cell-grid DST plans
   // and table, corner products (two levels), rhs, solution, scratch
   bool pres_ready = false;
@@ -1003,6 +1008,7 @@ struct ib_handle {
     }
     if (tape_ws) copy(st, tape_ws, omega, (long)L * P);
     vort2flux_dev(omega, s, qx, qy);
+    if (tape_qt) gather_taps(st, g, nf * kTaps, csr.key, qx, qy, q0x[0], tape_qt);
 
     int status = IB_OK;
     if (prm.num_stat) {
@@ -1112,6 +1118,32 @@ struct ib_handle {
       t_w.alloc(need_t * LP); t_ws.alloc(need_t * LP);
       t_qx.alloc(need_t * L * g.Fx); t_qy.alloc(need_t * L * g.Fy); t_f.alloc(need_t * nf);
     }
+    if (want_pos && t_qt.n < need_t * nf * kTaps) t_qt.alloc(need_t * nf * kTaps);
+  }
+
+  // set-up of the position gradient (once per handle): the weight
+  // derivatives, G on the faces the taps touch (the face set with margin 0),
+  // and the two contractions T = G W^T, U = W G
+  void adjoint_pos_setup() {
+    if (adj_pos) return;
+    dwx.alloc((size_t)nf * kTaps); dwy.alloc((size_t)nf * kTaps);
+    setup_reg_d(st, g, nf, xb, yb, comp, delta, offx, offy, prm.deltafnc_f32_literals, dwx, dwy);
+    const int fm_save = face_margin;
+    face_margin = 0;
+    build_face_set();
+    face_margin = fm_save;
+    if (!tapidx.p) tapidx.alloc((size_t)nf * kTaps);
+    if (!face_flag.p) face_flag.alloc(1);
+    alloc_assembly(nF);
+    compute_green();
+    map_taps(st, nf, csr.key, fmap, tapidx, face_flag);
+    TTm.alloc((size_t)nf * nF); Um.alloc((size_t)nf * nF);
+    green_tt_u(st, nf, nF, w, tapidx, Gm, TTm, Um);
+    sync();
+    trim_columns();
+    Gm.release(); Tm.release();
+    gpx.alloc(nf); gpy.alloc(nf);
+    adj_pos = true;
   }
 
   // HBM bytes of one taped step and of one checkpoint
@@ -1145,8 +1177,10 @@ struct ib_handle {
     const long LP = (long)L * g.P;
     save_state(t_w + sl * LP, nullptr, t_qx + (long)sl * L * g.Fx, t_qy + (long)sl * L * g.Fy);
     tape_ws = t_ws + sl * LP;
+    tape_qt = want_pos ? t_qt + (long)sl * nf * kTaps : nullptr;
     const int rc = advance();
     tape_ws = nullptr;
+    tape_qt = nullptr;
     copy(st, t_f + (long)sl * nf, f, nf);
     return rc;
   }
@@ -1203,6 +1237,10 @@ struct ib_handle {
     gemv(st, nf, Ainv, af, arf);
     gemv_symup(st, nf, dA, f0, af2);
     dot_acc(st, nf, arf, af2, -1.0, adot);
+    // body-point positions: E in rhsf and in the corrector, and A
+    if (want_pos)
+      pos_terms(st, g, nf, csr.key, tapidx, dwx, dwy, TTm, Um, arf, f0, t_qt + (long)sl * nf * kTaps, aFx, aFy, gpx,
+                gpy);
     // rhsf = E(q*_0 + q0): the adjoint of q*_0
     fill(st, aFx, g.Fx, 0.0);
     fill(st, aFy, g.Fy, 0.0);
@@ -1257,12 +1295,19 @@ struct ib_handle {
     for (int k = 0; k < L - 1; ++k) getbc_adj(st, g, albb + k * nbc, 0.25, aw + (k + 1) * P);
   }
 
-  int adjoint_run(int nsteps, int Kin, const double* cx, const double* cy, double* J, double* dRe, double* dw0) {
+  int adjoint_run(int nsteps, int Kin, const double* cx, const double* cy, double* J, double* dRe, double* dw0,
+                  double* dxb, double* dyb) {
     if (!prm.num_stat || ntors > 0 || sd_on || nf == 0)
       return fail(IB_E_UNSUPPORTED, "the adjoint covers rigid bodies with num_stat = 1");
     if (nsteps < 1) return fail(IB_E_INVAL, "nsteps must be >= 1");
     const int K = adjoint_segment(nsteps, Kin);
+    want_pos = dxb || dyb;
+    if (want_pos) adjoint_pos_setup();
     adjoint_setup(nsteps, K);
+    if (want_pos) {
+      fill(st, gpx, nf, 0.0);
+      fill(st, gpy, nf, 0.0);
+    }
     const long LP = (long)L * g.P;
     const long CQX = (long)L * g.Fx, CQY = (long)L * g.Fy;
     // objective weights per force entry: write_force2's 2 delta / dt, time mean
@@ -1326,6 +1371,26 @@ struct ib_handle {
     sync();
     if (J) *J = hv[0];
     if (dRe) *dRe = hv[1];
+    if (want_pos) {
+      // per point: the sum over its x-force and y-force entries (both read
+      // the point's coordinates); points in deck order
+      std::vector<double> gx(nf), gy(nf);
+      d2h(gx.data(), gpx, nf);
+      d2h(gy.data(), gpy, nf);
+      sync();
+      int pt = 0;
+      for (int k = 0; k < nf;) {
+        int n = 0;
+        while (k + n < nf && h_cbody[k + n] == h_cbody[k] && h_ccomp[k + n] == 1) ++n;
+        for (int j = 0; j < n; ++j) {
+          if (dxb) dxb[pt + j] = gx[k + j] + gx[k + n + j];
+          if (dyb) dyb[pt + j] = gy[k + j] + gy[k + n + j];
+        }
+        pt += n;
+        k += 2 * n;
+      }
+    }
+    want_pos = false;
     return IB_OK;
   }
 

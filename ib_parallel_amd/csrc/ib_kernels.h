@@ -313,6 +313,17 @@ void coarsify_adj(hipStream_t st, const Geom& g, double* wc, double* wf);
 void nl_adj(hipStream_t st, const Geom& g, const double* w, const double* qx, const double* qy, double q0x,
             const double* lastbc, const double* rb, double* nlxb, double* nlyb, double* ub, double* vb, double* wb,
             double* lbb, double* qxb, double* qyb);
+// d w / d xb, d w / d yb of every (entry, tap) of setup_reg (ib_body.hip)
+void setup_reg_d(hipStream_t st, const Geom& g, int nf, const double* xb, const double* yb, const int* comp,
+                 double delta, double offx, double offy, int f32lit, double* dwx, double* dwy);
+// TT[j + nf b] = (G W^T)(b, j), U[i + nf b] = (W G)(i, b)
+void green_tt_u(hipStream_t st, int nf, int nF, const double* w, const int* tapidx, const double* G, double* TT,
+                double* U);
+void gather_taps(hipStream_t st, const Geom& g, int ntaps, const int* key, const double* qx, const double* qy,
+                 double q0x, double* out);
+void pos_terms(hipStream_t st, const Geom& g, int nf, const int* key, const int* tapidx, const double* dwx,
+               const double* dwy, const double* TT, const double* U, const double* u, const double* v,
+               const double* qt, const double* hbx, const double* hby, double* gx, double* gy);
 void rot_acc(hipStream_t st, const Geom& g, const double* hx, const double* hy, double* v);
 void rhs_adj(hipStream_t st, long n, const double* r, double c1, double c2, double* ro, double* rb);
 void re_pred(hipStream_t st, long n, const double* ws, const double* w, const double* wsb, const double* rb,

@@ -253,7 +253,10 @@ int ib_batch_last_error(const ib_batch* b, char* buf, int32_t len);
  * derivative with respect to Re, and, when dJ_domega0 is not NULL, with
  * respect to the vorticity before the first step (mgridlev blocks of
  * (m-1)(n-1), the layout of ib_op_vort2flux's omega: the state is taken as
- * q = vort2flux(omega) of that vorticity).  The gradients are those of the
+ * q = vort2flux(omega) of that vorticity), and, when dJ_dxb / dJ_dyb are not
+ * NULL, with respect to the coordinates of every body point (deck order,
+ * body after body; the Schur matrix is taken as rebuilt for the moved
+ * points).  The gradients are those of the
  * discrete step (the transposed operators run backwards).  The steps are
  * taped in HBM in segments of ckpt_every steps: the last segment during the
  * forward pass, the others recomputed once from checkpoints; ckpt_every <= 0
@@ -262,7 +265,7 @@ int ib_batch_last_error(const ib_batch* b, char* buf, int32_t len);
  * IB_E_UNSUPPORTED).  h is left in the state
  * after the nsteps steps, as ib_advance would leave it. --- */
 int ib_adjoint_run(ib_handle* h, int32_t nsteps, int32_t ckpt_every, const double* cx, const double* cy, double* J,
-                   double* dJ_dRe, double* dJ_domega0);
+                   double* dJ_dRe, double* dJ_domega0, double* dJ_dxb, double* dJ_dyb);
 
 #ifdef __cplusplus
 }

@@ -5,7 +5,9 @@ J = mean over the steps of sum_b cx[b] fx_b + cy[b] fy_b.  dJ/dRe is checked
 against (J(Re + h) - J(Re - h)) / 2h from fresh handles started from the same
 state; dJ/domega0 along random directions against the same difference in the
 initial vorticity.  Bar: relative error 1e-5 (the FD truncation error at these
-steps is below 1e-8)."""
+steps is below 1e-8).  dJ/d(body-point coordinates) along random directions
+against differences of fresh handles whose bodies (and Schur matrix) are
+built at the moved points."""
 import numpy as np
 import pytest
 
@@ -167,3 +169,45 @@ def test_segmenting_does_not_change_the_gradients(warm):
         got = sim_from(p, b, w0, ro0).adjoint(7, CX, CY, ckpt_every=K)
         assert got[0] == ref[0] and got[1] == ref[1], K
         assert np.array_equal(got[2], ref[2]), K
+
+
+def two_body_case(xs=None, ys=None, f32=0):
+    """A cylinder (40 points, off the grid lines) and a small second cylinder
+    (16 points) at 64x64x3; xs, ys replace the point coordinates (deck order)."""
+    t1 = 2 * np.pi * (np.arange(40) + 0.37) / 40
+    t2 = 2 * np.pi * (np.arange(16) + 0.11) / 16
+    x = np.concatenate([0.5 * np.cos(t1) + 0.0123, 0.2 * np.cos(t2) + 1.5071])
+    y = np.concatenate([0.5 * np.sin(t1) - 0.0071, 0.2 * np.sin(t2) + 0.6033])
+    if xs is not None:
+        x, y = xs, ys
+    p = abi.make_params(64, 64, 3, 0.01, 100.0, 4.0, 1.0, 2.0, fsi_tol=1e-12, atol=1e-12, rtol=1e-12,
+                        num_stat=1, deltafnc_f32_literals=f32)
+    return p, [abi.BodySpec(x[:40], y[:40]), abi.BodySpec(x[40:], y[40:])], x, y
+
+
+@pytest.mark.parametrize("f32", (0, 1))
+def test_dpositions_match_finite_differences(f32):
+    from ib_parallel_amd import GpuSim
+    p, b, x, y = two_body_case(f32=f32)
+    s = GpuSim(p, b)
+    for _ in range(10):
+        s.advance()
+    w0, ro0 = state_of(s)
+    s.close()
+    p.istart = 10
+    cx, cy = np.array([1.0, 0.3]), np.array([0.5, -0.2])
+    J, _, _, gx, gy = sim_from(p, b, w0, ro0).adjoint(4, cx, cy, ckpt_every=2, position_grad=True)
+    assert gx.shape == (56,) and gy.shape == (56,)
+    rng = np.random.default_rng(11)
+    h = 1e-6
+    for trial in range(2):
+        dx, dy = rng.standard_normal(56), rng.standard_normal(56)
+        Js = []
+        for sg in (1.0, -1.0):
+            pp, bb, _, _ = two_body_case(x + sg * h * dx, y + sg * h * dy, f32=f32)
+            pp.istart = 10
+            Js.append(J_forward(pp, bb, w0, ro0, 4, cx, cy)[0])
+        fd = (Js[0] - Js[1]) / (2 * h)
+        ad = float(np.dot(gx, dx) + np.dot(gy, dy))
+        print(f"f32 {f32} trial {trial}: <dJ/dx, d> adjoint {ad:.12e} fd {fd:.12e}")
+        assert abs(ad - fd) <= 1e-5 * abs(fd)
