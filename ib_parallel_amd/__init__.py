@@ -49,14 +49,16 @@ class GpuSim(Simulation):
         self._check(self._f["schur_info"](self.h, C.byref(mode), C.byref(nfaces), C.byref(builds)))
         return mode.value, nfaces.value, builds.value
 
-    def adjoint(self, nsteps, cx, cy, ckpt_every=None, omega_grad=True, position_grad=False):
+    def adjoint(self, nsteps, cx, cy, ckpt_every=None, omega_grad=True, position_grad=False, body_grad=False):
         """Advance nsteps steps and return (J, dJ/dRe, dJ/domega0) for the
         time-mean objective J = mean over the steps of sum_b cx[b] fx_b +
         cy[b] fy_b; dJ/domega0 has the layout of vort2flux's omega (None
-        unless omega_grad); with position_grad also dJ/dxb and dJ/dyb per
-        body point (deck order).  ckpt_every None: the whole run is taped in HBM
+        unless omega_grad).  position_grad appends dJ/dxb and dJ/dyb per body
+        point (deck order; rigid bodies), body_grad appends dJ/dK and
+        dJ/dgamma0 per body (torsional stiffness and pitch angle in radians;
+        0 for rigid bodies).  ckpt_every None: the whole run is taped in HBM
         when it fits, else checkpoints every ceil(sqrt(nsteps)) steps.  Rigid
-        bodies with num_stat = 1."""
+        bodies with num_stat = 1, or the face-space form (moving bodies)."""
         cx = np.ascontiguousarray(np.broadcast_to(np.asarray(cx, dtype=np.float64), (self.nbodies,)))
         cy = np.ascontiguousarray(np.broadcast_to(np.asarray(cy, dtype=np.float64), (self.nbodies,)))
         K = int(ckpt_every) if ckpt_every else 0
@@ -65,12 +67,18 @@ class GpuSim(Simulation):
         npts = self.nf // 2
         gx = np.empty(npts) if position_grad else None
         gy = np.empty(npts) if position_grad else None
+        dk = np.empty(self.nbodies) if body_grad else None
+        dg = np.empty(self.nbodies) if body_grad else None
         self._check(self._f["adjoint_run"](self.h, int(nsteps), K, abi.dptr(cx), abi.dptr(cy), C.byref(J),
-                                           C.byref(dRe), abi.dptr(g), abi.dptr(gx), abi.dptr(gy)))
+                                           C.byref(dRe), abi.dptr(g), abi.dptr(gx), abi.dptr(gy), abi.dptr(dk),
+                                           abi.dptr(dg)))
         self.itime.value += int(nsteps)
+        out = [J.value, dRe.value, g]
         if position_grad:
-            return J.value, dRe.value, g, gx, gy
-        return J.value, dRe.value, g
+            out += [gx, gy]
+        if body_grad:
+            out += [dk, dg]
+        return tuple(out)
 
     def kernel_timing(self, kclass, enable=True):
         self._check(self._f["kernel_timing"](self.h, kclass, 1 if enable else 0))

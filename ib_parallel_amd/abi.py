@@ -118,7 +118,7 @@ GPU_ONLY = {
     "kernel_timing": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "kernel_timing_read": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),
-    "adjoint_run": (C.c_int, [_P, C.c_int32, C.c_int32, _D, _D, _D, _D, _D, _D, _D]),
+    "adjoint_run": (C.c_int, [_P, C.c_int32, C.c_int32, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
 }
 
 

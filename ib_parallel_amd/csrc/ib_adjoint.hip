@@ -409,6 +409,259 @@ void pos_terms(hipStream_t st, const Geom& g, int nf, const int* key, const int*
   IB_LAUNCH_CHECK();
 }
 
+// ---- torsional bodies (family 2) in the face-space form -----------------
+// d w / d theta of every (entry, tap) of torsional body entries: the points
+// rotate about the hinge, x = xbf0 + (j-1) ds cos(gamma0 - theta), y = ybf0 +
+// (j-1) ds sin(gamma0 - theta) (update_xb, src/operators_struct.F90:600-620),
+// so d x/d theta = (j-1) ds sin(gamma0 - theta), d y/d theta = -(j-1) ds
+// cos(gamma0 - theta).  Zero for the entries of rigid bodies.
+__global__ void k_dthw(int nf, int nt, const int* __restrict__ tors_of, const TorsBody* __restrict__ tbs,
+                       const double* __restrict__ th, const double* __restrict__ dwx, const double* __restrict__ dwy,
+                       double* __restrict__ out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nf * kTaps) return;
+  const int i = q / kTaps;
+  const int t = tors_of[i];
+  double v = 0.0;
+  if (t >= 0) {
+    const TorsBody b = tbs[t];
+    const int jl = i - b.off;  // 0 .. 2 npts - 1
+    const int jj = jl < b.npts ? jl : jl - b.npts;
+    const double ph = b.gamma0 - th[t];
+    v = dwx[q] * ((double)jj * b.deltas * sin(ph)) - dwy[q] * ((double)jj * b.deltas * cos(ph));
+  }
+  out[q] = v;
+}
+
+void dthw(hipStream_t st, int nf, int nt, const int* tors_of, const TorsBody* tb, const double* th, const double* dwx,
+          const double* dwy, double* out) {
+  if (nf <= 0) return;
+  hipLaunchKernelGGL(k_dthw, dim3(ceil_div(nf * kTaps, 256)), dim3(256), 0, st, nf, nt, tors_of, tb, th, dwx, dwy,
+                     out);
+  IB_LAUNCH_CHECK();
+}
+
+// out[b] = face field at face Fkeys[b] (x faces, then Fx + y faces)
+__global__ void k_face_gather(Geom g, int nF, const int* __restrict__ fkeys, const double* __restrict__ hx,
+                              const double* __restrict__ hy, double* __restrict__ out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nF) return;
+  const int k = fkeys[b];
+  out[b] = k < g.Fx ? hx[k] : hy[k - g.Fx];
+}
+
+void face_gather(hipStream_t st, const Geom& g, int nF, const int* fkeys, const double* hx, const double* hy,
+                 double* out) {
+  if (nF <= 0) return;
+  hipLaunchKernelGGL(k_face_gather, dim3(ceil_div(nF, 256)), dim3(256), 0, st, g, nF, fkeys, hx, hy, out);
+  IB_LAUNCH_CHECK();
+}
+
+// out[i] = sum_t w[i,t] zf[tapidx[i,t]] (E applied to a face-set vector)
+__global__ void k_face_regT(int nf, const double* __restrict__ w, const int* __restrict__ tapidx,
+                            const double* __restrict__ zf, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  double acc = 0.0;
+  for (int t = 0; t < kTaps; ++t) acc += w[i * kTaps + t] * zf[tapidx[i * kTaps + t]];
+  out[i] = acc;
+}
+
+void face_regT(hipStream_t st, int nf, const double* w, const int* tapidx, const double* zf, double* out) {
+  if (nf <= 0) return;
+  hipLaunchKernelGGL(k_face_regT, dim3(ceil_div(nf, 256)), dim3(256), 0, st, nf, w, tapidx, zf, out);
+  IB_LAUNCH_CHECK();
+}
+
+// out[i] += coef[i] * sum_t wt[i,t] fld[i,t] for entries i0 <= i < i1, with
+// fld either a per-tap array (ftap) or a face-set vector read through tapidx
+// minus a per-tap array (fset[tapidx] - ftap)
+__global__ void k_tap_contract(int i0, int i1, const double* __restrict__ wt, const double* __restrict__ coef,
+                               const double* __restrict__ ftap, const double* __restrict__ fset,
+                               const int* __restrict__ tapidx, double* __restrict__ out) {
+  const int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= i1) return;
+  double acc = 0.0;
+  for (int t = 0; t < kTaps; ++t) {
+    const int q = i * kTaps + t;
+    const double v = fset ? fset[tapidx[q]] - (ftap ? ftap[q] : 0.0) : ftap[q];
+    acc += wt[q] * v;
+  }
+  out[i] += (coef ? coef[i] : 1.0) * acc;
+}
+
+void tap_contract(hipStream_t st, int i0, int i1, const double* wt, const double* coef, const double* ftap,
+                  const double* fset, const int* tapidx, double* out) {
+  if (i1 <= i0) return;
+  hipLaunchKernelGGL(k_tap_contract, dim3(ceil_div(i1 - i0, 256)), dim3(256), 0, st, i0, i1, wt, coef, ftap, fset,
+                     tapidx, out);
+  IB_LAUNCH_CHECK();
+}
+
+// y = G^T x for a square column-major n x n matrix (a wavefront per column)
+__global__ void __launch_bounds__(256) k_gemv_t(int n, const double* __restrict__ G, const double* __restrict__ x,
+                                                double* __restrict__ y) {
+  const int j = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x % 64;
+  if (j >= n) return;
+  const double* __restrict__ c = G + (long)j * n;
+  double acc = 0.0;
+  for (int b = lane; b < n; b += 64) acc += c[b] * x[b];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if (lane == 0) y[j] = acc;
+}
+
+void gemv_t(hipStream_t st, int n, const double* G, const double* x, double* y) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gemv_t, dim3(ceil_div(n, 4)), dim3(256), 0, st, n, G, x, y);
+  IB_LAUNCH_CHECK();
+}
+
+// AT = A^T (n x n, column major)
+__global__ void k_transpose(int n, const double* __restrict__ A, double* __restrict__ AT) {
+  __shared__ double tile[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 256 threads: 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int i = bx + tx, j = by + r;
+    if (i < n && j < n) tile[r][tx] = A[(long)j * n + i];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int i = by + tx, j = bx + r;
+    if (i < n && j < n) AT[(long)j * n + i] = tile[tx][r];
+  }
+}
+
+void transpose(hipStream_t st, int n, const double* A, double* AT) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_transpose, dim3(ceil_div(n, 32), ceil_div(n, 32)), dim3(256), 0, st, n, A, AT);
+  IB_LAUNCH_CHECK();
+}
+
+// One torsional body t, one workgroup (theta = th[t], or theta0 with at0):
+// the scalars of its block,
+//   s[0] = BR^T f, s[1] = BR'^T f, s[2] = BR^T a (a given) ,
+// and, if r1 is given, r1 += (sc_b/den)(BR' (BR^T f) + BR (BR'^T f)) - sc BR'
+// (the theta derivative of b_times2 applied to f and of the structural
+// right-hand side BR sc), and, if zc is given, zc = c2 BR on the body's
+// entries (zero elsewhere is left to the caller).  BR (BR_build,
+// src/operators_struct.F90:319-338) and BR' = d BR / d theta at theta = th[t];
+// sc = -delta ((2/dt) Q - thetad0 - 2 theta0/dt) with Q the Newmark
+// predictor of compute_rhs (src/operators_struct.F90:342-398) at the
+// consistent rates; c2 = delta / (dt (4M/dt^2 + K)) (compute_dx's
+// denominator, :448).
+__device__ __forceinline__ double brv(const TorsBody& b, int j, double si, double ci) {
+  return j <= b.npts ? (double)(j - 1) * b.deltas * si : -(double)(j - b.npts - 1) * b.deltas * ci;
+}
+__device__ __forceinline__ double brp(const TorsBody& b, int j, double si, double ci) {
+  return j <= b.npts ? -(double)(j - 1) * b.deltas * ci : -(double)(j - b.npts - 1) * b.deltas * si;
+}
+
+__global__ void __launch_bounds__(256) k_tors_block(const TorsBody* __restrict__ tbs, const double* __restrict__ th,
+                                                    int nt, int t, int at0, double dt, double delta,
+                                                    const double* __restrict__ f, const double* __restrict__ a,
+                                                    double* __restrict__ r1, double* __restrict__ zc,
+                                                    double* __restrict__ s) {
+  __shared__ double sh[3][256];
+  const TorsBody b = tbs[t];
+  const double theta = at0 ? th[3 * nt + t] : th[t];
+  const double si = sin(b.gamma0 - theta), ci = cos(b.gamma0 - theta);
+  double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+  for (int j = threadIdx.x + 1; j <= 2 * b.npts; j += blockDim.x) {
+    const double fv = f[b.off + j - 1];
+    p0 += brv(b, j, si, ci) * fv;
+    p1 += brp(b, j, si, ci) * fv;
+    if (a) p2 += brv(b, j, si, ci) * a[b.off + j - 1];
+  }
+  sh[0][threadIdx.x] = p0;
+  sh[1][threadIdx.x] = p1;
+  sh[2][threadIdx.x] = p2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  const double brf = sh[0][0], brpf = sh[1][0];
+  if (threadIdx.x == 0 && s) {
+    s[0] = brf;
+    s[1] = brpf;
+    s[2] = sh[2][0];
+  }
+  const double den = 4.0 / (dt * dt) * b.M + 2.0 / dt * b.C + b.K;
+  const double scb = (delta * delta / b.deltas / dt) * (2 * b.deltas / dt);
+  const double th0 = th[3 * nt + t], thd0 = th[4 * nt + t], thdd0 = th[5 * nt + t];
+  const double Q = (b.M * (4.0 / (dt * dt) * th0 + 4.0 / dt * thd0 + thdd0) + b.C * (thd0 + 2.0 / dt * th0)) / den;
+  const double sc = -delta * (2.0 / dt * Q - thd0 - 2.0 / dt * th0);
+  const double c2 = delta / (dt * (4.0 / (dt * dt) * b.M + b.K));
+  for (int j = threadIdx.x + 1; j <= 2 * b.npts; j += blockDim.x) {
+    const double B = brv(b, j, si, ci), Bp = brp(b, j, si, ci);
+    if (r1) r1[b.off + j - 1] += scb / den * (Bp * brf + B * brpf) - sc * Bp;
+    if (zc) zc[b.off + j - 1] = c2 * B;
+  }
+}
+
+void tors_block(hipStream_t st, const TorsBody* tb, const double* th, int nt, int t, int at0, double dt, double delta,
+                const double* f, const double* a, double* r1, double* zc, double* s) {
+  hipLaunchKernelGGL(k_tors_block, dim3(1), dim3(256), 0, st, tb, th, nt, t, at0, dt, delta, f, a, r1, zc, s);
+  IB_LAUNCH_CHECK();
+}
+
+// Rigid-body point terms of one step on the FSI path (S = W G W^T + B
+// solved as it is): with l the adjoint of rhsf, f, hb = curl(ainv(omega_0
+// bar)) and qt = q* + q0 at the taps, vT = G W^T f and uT = G^T W^T l on the
+// face set,
+//   X(i,t) = -f_i (hb(i,t) + uT[tap]) - l_i (vT[tap] - qt(i,t)),
+//   gx[i] += sum_t dw/dx(i,t) X(i,t), gy[i] likewise (rigid entries only).
+__global__ void k_pos_fsi(int nf, const int* __restrict__ tors_of, const int* __restrict__ tapidx,
+                          const double* __restrict__ dwx, const double* __restrict__ dwy, const double* __restrict__ l,
+                          const double* __restrict__ f, const double* __restrict__ hb, const double* __restrict__ qt,
+                          const double* __restrict__ vT, const double* __restrict__ uT, double* __restrict__ gx,
+                          double* __restrict__ gy) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf || tors_of[i] >= 0) return;
+  double ax = 0.0, ay = 0.0;
+  for (int t = 0; t < kTaps; ++t) {
+    const int q = i * kTaps + t;
+    const int b = tapidx[q];
+    const double X = -f[i] * (hb[q] + uT[b]) - l[i] * (vT[b] - qt[q]);
+    ax += dwx[q] * X;
+    ay += dwy[q] * X;
+  }
+  gx[i] += ax;
+  gy[i] += ay;
+}
+
+void pos_fsi(hipStream_t st, int nf, const int* tors_of, const int* tapidx, const double* dwx, const double* dwy,
+             const double* l, const double* f, const double* hb, const double* qt, const double* vT, const double* uT,
+             double* gx, double* gy) {
+  if (nf <= 0) return;
+  hipLaunchKernelGGL(k_pos_fsi, dim3(ceil_div(nf, 256)), dim3(256), 0, st, nf, tors_of, tapidx, dwx, dwy, l, f, hb, qt,
+                     vT, uT, gx, gy);
+  IB_LAUNCH_CHECK();
+}
+
+// out[0] = sum_i a[i] b[i] over [i0, i1) (written, one workgroup)
+__global__ void __launch_bounds__(256) k_dot_set(int i0, int i1, const double* __restrict__ a,
+                                                 const double* __restrict__ b, double* __restrict__ out) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) acc += a[i] * b[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+void dot_set(hipStream_t st, int i0, int i1, const double* a, const double* b, double* out) {
+  hipLaunchKernelGGL(k_dot_set, dim3(1), dim3(256), 0, st, i0, i1, a, b, out);
+  IB_LAUNCH_CHECK();
+}
+
 // Elementwise helpers of the adjoint sweep
 __global__ void k_axpy(long n, double a, const double* __restrict__ x, double* __restrict__ y) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -178,6 +178,15 @@ struct ib_handle {
   // faces the taps touch, per-entry gradient accumulators, (q* + q0) tape
   bool adj_pos = false, want_pos = false;
   DBuf<double> dwx, dwy, TTm, Um, gpx, gpy, t_qt;
+  // moving (torsional) bodies in the face-space form: S^T, dG/dRe, face-set
+  // vectors, the per-body columns r1 and Z, the body-state tape and
+  // checkpoints, and the body adjoints (host; a few scalars per body)
+  bool adj_fsi = false;
+  double* tape_th = nullptr;  // set: advance copies th, xb, yb after the FSI loop here (and +6nt, +nf)
+  DBuf<double> SdT, dGm, xfv, vTv, yfv, zfv, lfv, dgv, uTv, r1v, Zv, aav, zcv, htv, dthv, cvv, hx2, hy2, fmask, scal;
+  DBuf<double> t_th, t_f0, c_th, c_f, fin_th;
+  int64_t dG_build = -1;
+  std::vector<double> bth, bthd, bthdd, gK, gG0, h_scal;
   // pressure post-processing (allocated on first use): cell-grid DST plans
   // and table, corner products (two levels), rhs, solution, scratch
   bool pres_ready = false;
@@ -1008,7 +1017,6 @@ struct ib_handle {
     }
     if (tape_ws) copy(st, tape_ws, omega, (long)L * P);
     vort2flux_dev(omega, s, qx, qy);
-    if (tape_qt) gather_taps(st, g, nf * kTaps, csr.key, qx, qy, q0x[0], tape_qt);
 
     int status = IB_OK;
     if (prm.num_stat) {
@@ -1025,6 +1033,14 @@ struct ib_handle {
     } else {
       status = fsi_fixed_point();
       if (status == IB_E_INVAL) return status;
+    }
+
+    // adjoint tape: q* + q0 at the final taps, the body state after the loop
+    if (tape_qt) gather_taps(st, g, nf * kTaps, csr.key, qx, qy, q0x[0], tape_qt);
+    if (tape_th) {
+      copy(st, tape_th, th, 6 * (long)ntors);
+      copy(st, tape_th + 6 * ntors, xb, nf);
+      copy(st, tape_th + 6 * ntors + nf, yb, nf);
     }
 
     // step 3: corrector (src/timestepper.F90:488-553); on the sub-domain the
@@ -1084,17 +1100,31 @@ struct ib_handle {
       dlam1i.alloc(LP); ll.alloc(LP);
       IB_HIP(hipMemcpy(dlam1i.p, h1i.data(), sizeof(double) * LP, hipMemcpyHostToDevice));
       IB_HIP(hipMemcpy(ll.p, hll.data(), sizeof(double) * LP, hipMemcpyHostToDevice));
-      // dA/dRe: the num_stat columns with d lam1i / dRe in the ainv stage
-      dA.alloc((size_t)nf * nf);
-      alloc_assembly(nf);
-      for (int c0 = 0; c0 < nf; c0 += ncol) {
-        const int nc = std::min(ncol, nf - c0);
-        long sbc = 0;
-        const double* bc = column_batch(c0, nc, w, csr.key, &sbc, dlam1i);
-        schur_columns(st, g, nf, c0, nc, w, csr.key, Sb, g.P, bc, sbc, no_tors, tb, th, dt, delta, dA);
+      if (!adj_fsi) {
+        // dA/dRe: the num_stat columns with d lam1i / dRe in the ainv stage
+        dA.alloc((size_t)nf * nf);
+        alloc_assembly(nf);
+        for (int c0 = 0; c0 < nf; c0 += ncol) {
+          const int nc = std::min(ncol, nf - c0);
+          long sbc = 0;
+          const double* bc = column_batch(c0, nc, w, csr.key, &sbc, dlam1i);
+          schur_columns(st, g, nf, c0, nc, w, csr.key, Sb, g.P, bc, sbc, no_tors, tb, th, dt, delta, dA);
+        }
+        sync();
+        trim_columns();
+      } else {
+        const int nt = std::max(ntors, 1);
+        SdT.alloc((size_t)nf * nf);
+        aav.alloc(nf); zcv.alloc(nf); fmask.alloc(nf); cvv.alloc(nf);
+        r1v.alloc((size_t)nt * nf); Zv.alloc((size_t)nt * nf);
+        htv.alloc((size_t)nf * kTaps); dthv.alloc((size_t)nf * kTaps);
+        dwx.alloc((size_t)nf * kTaps); dwy.alloc((size_t)nf * kTaps);
+        hx2.alloc(g.Fx); hy2.alloc(g.Fy);
+        scal.alloc((size_t)16 * nt + (size_t)8 * nt * nt);
+        h_scal.resize((size_t)16 * nt + (size_t)8 * nt * nt);
+        gpx.alloc(nf); gpy.alloc(nf);
+        fin_th.alloc((size_t)6 * nt + 2 * (size_t)nf);
       }
-      sync();
-      trim_columns();
       aw.alloc(LP); aro.alloc(LP); aws.alloc(LP); asb.alloc(LP); racc.alloc(P);
       aqx.alloc((size_t)L * g.Fx); aqy.alloc((size_t)L * g.Fy);
       af.alloc(nf); arf.alloc(nf); af2.alloc(nf); cw.alloc(nf);
@@ -1118,7 +1148,85 @@ struct ib_handle {
       t_w.alloc(need_t * LP); t_ws.alloc(need_t * LP);
       t_qx.alloc(need_t * L * g.Fx); t_qy.alloc(need_t * L * g.Fy); t_f.alloc(need_t * nf);
     }
-    if (want_pos && t_qt.n < need_t * nf * kTaps) t_qt.alloc(need_t * nf * kTaps);
+    if ((want_pos || adj_fsi) && t_qt.n < need_t * nf * kTaps) t_qt.alloc(need_t * nf * kTaps);
+    if (adj_fsi) {
+      const size_t bs = (size_t)6 * ntors + 2 * (size_t)nf;  // th, xb, yb
+      if (t_th.n < need_t * bs) { t_th.alloc(need_t * bs); t_f0.alloc(need_t * nf); }
+      if (c_th.n < need_c * bs) { c_th.alloc(need_c * bs); c_f.alloc(need_c * nf); }
+    }
+  }
+
+  // G's derivative in Re on the current face set (columns with d lam1i/dRe)
+  void compute_green_d() {
+    if (dGm.n < (size_t)nF * nF) dGm.alloc((size_t)nF * nF);
+    alloc_assembly(nF);
+    for (int c0 = 0; c0 < nF; c0 += ncol) {
+      const int nc = std::min(ncol, nF - c0);
+      long sbc = 0;
+      const double* bc = column_batch(c0, nc, nullptr, Fkeys, &sbc, dlam1i);
+      green_columns(st, g, nF, c0, nc, Fkeys, Sb, g.P, bc, sbc, dGm);
+    }
+    dG_build = green_builds;
+  }
+
+  // GMRES on S^T (SdT), zero initial guess, the deck's tolerances
+  int solve_T(const double* b, double* x) {
+    fill(st, x, nf, 0.0);
+    if (nf <= kFusedGmresMax && gmres_dense_fits(nf, kmax, max_lds)) {
+      gmres_dense(st, nf, kmax, prm.gmres_max_it, SdT, b, x, V, H, gst, prm.rtol, prm.atol, max_lds);
+      read_gmres();
+      return hst->reason > 0 ? IB_OK : IB_E_NOCONV_GMRES;
+    }
+    GmresDev init{};
+    init.first = 1;
+    *hst = init;
+    IB_HIP(hipMemcpyAsync(gst, hst, sizeof(GmresDev), hipMemcpyHostToDevice, st));
+    int reason = 0, its = 0;
+    const int maxit = prm.gmres_max_it;
+    while (true) {
+      gemv(st, nf, SdT, x, gw);
+      gm_start(st, nf, kmax, b, gw, V, gst, H, grs, prm.rtol, prm.atol);
+      read_gmres();
+      reason = hst->reason;
+      if (reason) break;
+      int it = 0;
+      while (true) {
+        const int nb = std::min(kGmBatch, std::min(kmax - it, maxit - its));
+        for (int q = 0; q < nb; ++q) {
+          gemv(st, nf, SdT, V + (size_t)(it + q) * nf, V + (size_t)(it + q + 1) * nf);
+          gm_arnoldi(st, nf, kmax, V, gst, H, cc, ss, grs, pnorm);
+        }
+        read_gmres();
+        reason = hst->reason;
+        its = hst->its;
+        it = hst->it;
+        if (reason || it >= kmax || its >= maxit) break;
+      }
+      gm_buildsol(st, nf, kmax, V, gst, H, grs, x, nrs);
+      if (reason) break;
+      if (its >= maxit) {
+        reason = -2;
+        break;
+      }
+    }
+    return reason > 0 ? IB_OK : IB_E_NOCONV_GMRES;
+  }
+
+  // the body at the end of a taped step's FSI loop: theta (and its history),
+  // positions, weights, S (and S^T), the weight derivatives
+  void body_restore(const double* bs) {
+    copy(st, th, bs, 6 * (long)ntors);
+    copy(st, xb, bs + 6 * ntors, nf);
+    copy(st, yb, bs + 6 * ntors + nf, nf);
+    rebuild_weights();
+    refresh_schur();
+    if (dG_build != green_builds) compute_green_d();
+    if (xfv.n < (size_t)nF) {
+      xfv.alloc(nF); vTv.alloc(nF); yfv.alloc(nF); zfv.alloc(nF); lfv.alloc(nF); dgv.alloc(nF); uTv.alloc(nF);
+    }
+    transpose(st, nf, Sd, SdT);
+    setup_reg_d(st, g, nf, xb, yb, comp, delta, offx, offy, prm.deltafnc_f32_literals, dwx, dwy);
+    dthw(st, nf, ntors, tors_of, tb, th, dwx, dwy, dthv);
   }
 
   // set-up of the position gradient (once per handle): the weight
@@ -1172,15 +1280,35 @@ struct ib_handle {
     copy(st, omega, w_, LP); copy(st, rhs_old, ro_, LP);
     copy(st, qx, qx_, (long)L * g.Fx); copy(st, qy, qy_, (long)L * g.Fy);
   }
+  // moving bodies: th, positions and f (GMRES's initial guess) with the flow
+  void save_body(double* bs, double* f_) {
+    copy(st, bs, th, 6 * (long)ntors);
+    copy(st, bs + 6 * ntors, xb, nf);
+    copy(st, bs + 6 * ntors + nf, yb, nf);
+    if (f_) copy(st, f_, f, nf);
+  }
+  void load_body(const double* bs, const double* f_) {
+    copy(st, th, bs, 6 * (long)ntors);
+    copy(st, xb, bs + 6 * ntors, nf);
+    copy(st, yb, bs + 6 * ntors + nf, nf);
+    if (f_) copy(st, f, f_, nf);
+    rebuild_weights();
+    refresh_schur();
+  }
   // one taped step: the step-start omega and q, the predicted omega, f
   int taped_advance(int sl) {
     const long LP = (long)L * g.P;
     save_state(t_w + sl * LP, nullptr, t_qx + (long)sl * L * g.Fx, t_qy + (long)sl * L * g.Fy);
     tape_ws = t_ws + sl * LP;
-    tape_qt = want_pos ? t_qt + (long)sl * nf * kTaps : nullptr;
+    tape_qt = (want_pos || adj_fsi) ? t_qt + (long)sl * nf * kTaps : nullptr;
+    if (adj_fsi) {
+      tape_th = t_th + (long)sl * (6 * ntors + 2 * nf);
+      copy(st, t_f0 + (long)sl * nf, f, nf);
+    }
     const int rc = advance();
     tape_ws = nullptr;
     tape_qt = nullptr;
+    tape_th = nullptr;
     copy(st, t_f + (long)sl * nf, f, nf);
     return rc;
   }
@@ -1212,14 +1340,16 @@ struct ib_handle {
   // one step backwards; tape slot sl holds the step-start omega and q, the
   // predicted omega and f; (aw, aro, aqx, aqy) enter as the adjoint of the
   // step-end state and leave as that of the step-start state
-  void adjoint_step(int sl, bool first) {
+  int adjoint_step(int sl, bool first) {
     const long P = g.P, LP = (long)L * P, nbc = g.nbc;
     const double* w0 = t_w + sl * LP;
     const double* qx0 = t_qx + (long)sl * L * g.Fx;
     const double* qy0 = t_qy + (long)sl * L * g.Fy;
     const double* ws0 = t_ws + sl * LP;
     const double* f0 = t_f + (long)sl * nf;
+    const double* qt = (want_pos || adj_fsi) ? t_qt + (long)sl * nf * kTaps : nullptr;
     const double invRe = 1.0 / Re;
+    if (adj_fsi) body_restore(t_th + (long)sl * (6 * ntors + 2 * nf));
     // the end-of-step vort2flux
     v2f_adj(aw, aqx, aqy);
     // corrector omega_0 -= ainv(rot(E^T f)): the adjoint of f (J depends on f
@@ -1227,26 +1357,40 @@ struct ib_handle {
     copy(st, af, cw, nf);
     ainv_dev(aw, aP1, false);
     curl_getbc(st, g, aP1, zbc, aFx, aFy, nullptr);
+    if (adj_fsi) gather_taps(st, g, nf * kTaps, csr.key, aFx, aFy, 0.0, htv);
     regT(st, g, nf, w, csr.key, aFx, aFy, 0.0, nullptr, arf);
     axpy(st, nf, -1.0, arf, af);
     reg(st, g, csr, w, f0, hx, hy);
     ainv_dev(nullptr, aP2, false);
     re_corr(st, P, aw, aP1, aP2, invRe, racc);
-    // f = A^-1 rhsf (A symmetric): the adjoint of rhsf, and the Re term
-    // -<rhsf bar, dA/dRe f>
-    gemv(st, nf, Ainv, af, arf);
-    gemv_symup(st, nf, dA, f0, af2);
-    dot_acc(st, nf, arf, af2, -1.0, adot);
-    // body-point positions: E in rhsf and in the corrector, and A
-    if (want_pos)
-      pos_terms(st, g, nf, csr.key, tapidx, dwx, dwy, TTm, Um, arf, f0, t_qt + (long)sl * nf * kTaps, aFx, aFy, gpx,
-                gpy);
+    if (!adj_fsi) {
+      // f = A^-1 rhsf (A symmetric): the adjoint of rhsf, and the Re term
+      // -<rhsf bar, dA/dRe f>
+      gemv(st, nf, Ainv, af, arf);
+      gemv_symup(st, nf, dA, f0, af2);
+      dot_acc(st, nf, arf, af2, -1.0, adot);
+      // body-point positions: E in rhsf and in the corrector, and A
+      if (want_pos) pos_terms(st, g, nf, csr.key, tapidx, dwx, dwy, TTm, Um, arf, f0, qt, aFx, aFy, gpx, gpy);
+    } else {
+      const int rc = fsi_adjoint(sl, first, f0, qt);
+      if (rc != IB_OK) return rc;
+    }
     // rhsf = E(q*_0 + q0): the adjoint of q*_0
     fill(st, aFx, g.Fx, 0.0);
     fill(st, aFy, g.Fy, 0.0);
     reg(st, g, csr, w, arf, aFx, aFy);
     axpy(st, g.Fx, 1.0, aFx, aqx);
     axpy(st, g.Fy, 1.0, aFy, aqy);
+    if (adj_fsi) {
+      // Re through G: -<W^T lambda1, dG/dRe W^T f>; rigid points: S's weights
+      face_gather(st, g, nF, Fkeys, aFx, aFy, lfv);
+      gemv(st, nF, dGm, xfv, dgv);
+      dot_acc(st, nF, lfv, dgv, -1.0, adot);
+      if (want_pos) {
+        gemv_t(st, nF, Gm, lfv, uTv);
+        pos_fsi(st, nf, tors_of, tapidx, dwx, dwy, arf, f0, htv, qt, vTv, uTv, gpx, gpy);
+      }
+    }
     // the vort2flux after the predictor
     v2f_adj(aw, aqx, aqy);
     copy(st, aws, aw, LP);
@@ -1293,23 +1437,164 @@ struct ib_handle {
     }
     // lastbc_k = 0.25 get_bc(omega_(k+1))
     for (int k = 0; k < L - 1; ++k) getbc_adj(st, g, albb + k * nbc, 0.25, aw + (k + 1) * P);
+    return IB_OK;
+  }
+
+  // The FSI fixed point of a step with moving (torsional) bodies, backwards.
+  // At convergence (f, theta) solve
+  //   R1 = S(theta) f - BR(theta) sc - E(theta)(q* + q0) = 0
+  //   R2 = Q - theta + c2 BR(theta)^T f = 0        (one per torsional body)
+  // (compute_rhs, GMRES, compute_dx; at consistent rates compute_rhs's
+  // structural factor sc depends on the history only).  Given the adjoints
+  // of f (af) and of theta (the body adjoints carried on the host plus the
+  // corrector's E(theta) term) the transposed system is solved through
+  // S^T: a = S^-T af, Z_t = S^-T c2 BR_t, then the small system for
+  // lambda2 and lambda1 = a - Z lambda2 (left in arf).  History, stiffness
+  // and pitch adjoints follow from R1, R2 and the Newmark updates of
+  // compute_dx; on the first step also from initial_acc.
+  int fsi_adjoint(int sl, bool first, const double* f0, const double* qt) {
+    const int nt = ntors;
+    const double* fstart = t_f0 + (long)sl * nf;
+    // f on the face set and G W^T f (hx, hy hold E^T f on the touched faces)
+    face_gather(st, g, nF, Fkeys, hx, hy, xfv);
+    gemv(st, nF, Gm, xfv, vTv);
+    int rc = solve_T(af, aav);
+    if (rc != IB_OK) return fail(rc, "adjoint: GMRES on S^T did not converge");
+    double* sc = scal;
+    const long zs = 16 * (long)nt;  // per (t, s): BR_t^T Z_s at [2], r1_t^T Z_s at [3]
+    for (int t = 0; t < nt; ++t) {
+      const TorsBody& hb = h_tb[t];
+      const int o = hb.off, n2 = 2 * hb.npts;
+      fill(st, zcv, nf, 0.0);
+      tors_block(st, tb, th, nt, t, 0, dt, delta, f0, aav, nullptr, zcv, sc + 16 * t);
+      rc = solve_T(zcv, Zv + (long)t * nf);
+      if (rc != IB_OK) return fail(rc, "adjoint: GMRES on S^T did not converge");
+      double* r1 = r1v + (long)t * nf;
+      // W G (E'_t^T f) + E'_t (G W^T f - (q* + q0)) + B'_t f - BR'_t sc
+      fill(st, fmask, nf, 0.0);
+      copy(st, fmask + o, f0 + o, n2);
+      fill(st, hx2, g.Fx, 0.0);
+      fill(st, hy2, g.Fy, 0.0);
+      reg(st, g, csr, dthv, fmask, hx2, hy2);
+      face_gather(st, g, nF, Fkeys, hx2, hy2, yfv);
+      gemv(st, nF, Gm, yfv, zfv);
+      face_regT(st, nf, w, tapidx, zfv, r1);
+      tap_contract(st, o, o + n2, dthv, nullptr, qt, vTv, tapidx, r1);
+      tors_block(st, tb, th, nt, t, 0, dt, delta, f0, nullptr, r1, nullptr, nullptr);
+      dot_set(st, 0, nf, r1, aav, sc + 16 * t + 3);
+      // the corrector's E(theta): sum over the body's entries of f dthw hb
+      fill(st, cvv, nf, 0.0);
+      tap_contract(st, o, o + n2, dthv, nullptr, htv, nullptr, nullptr, cvv);
+      dot_set(st, o, o + n2, f0, cvv, sc + 16 * t + 4);
+      if (first) tors_block(st, tb, th, nt, t, 1, dt, delta, fstart, nullptr, nullptr, nullptr, sc + 16 * t + 8);
+    }
+    for (int t = 0; t < nt; ++t)
+      for (int u = 0; u < nt; ++u) {
+        double* s2 = sc + zs + 8 * ((long)t * nt + u);
+        tors_block(st, tb, th, nt, t, 0, dt, delta, f0, Zv + (long)u * nf, nullptr, nullptr, s2);
+        dot_set(st, 0, nf, r1v + (long)t * nf, Zv + (long)u * nf, s2 + 3);
+      }
+    const long nsc = zs + 8 * (long)nt * nt;
+    std::vector<double> hth(6 * (size_t)nt);
+    IB_HIP(hipMemcpyAsync(h_scal.data(), scal.p, sizeof(double) * nsc, hipMemcpyDeviceToHost, st));
+    IB_HIP(hipMemcpyAsync(hth.data(), th.p, sizeof(double) * 6 * nt, hipMemcpyDeviceToHost, st));
+    sync();
+    auto S0 = [&](int t, int k) { return h_scal[16 * (size_t)t + k]; };
+    auto SZ = [&](int t, int u, int k) { return h_scal[zs + 8 * ((size_t)t * nt + u) + k]; };
+    // (D - r1^T Z) lambda2 = gtheta - r1^T a, D = diag(-1 + c2 BR'^T f)
+    std::vector<double> A((size_t)nt * nt), rhs(nt), lam2(nt);
+    for (int t = 0; t < nt; ++t) {
+      const TorsBody& b = h_tb[t];
+      const double c2 = delta / (dt * (4.0 / (dt * dt) * b.M + b.K));
+      const double gth = bth[t] + 2.0 / dt * bthd[t] + 4.0 / (dt * dt) * bthdd[t] - S0(t, 4);
+      rhs[t] = gth - S0(t, 3);
+      for (int u = 0; u < nt; ++u) A[(size_t)t * nt + u] = (t == u ? -1.0 + c2 * S0(t, 1) : 0.0) - SZ(t, u, 3);
+    }
+    // Gaussian elimination with partial pivoting (nt is the number of
+    // torsional bodies)
+    for (int c = 0; c < nt; ++c) {
+      int pr = c;
+      for (int r = c + 1; r < nt; ++r)
+        if (std::fabs(A[(size_t)r * nt + c]) > std::fabs(A[(size_t)pr * nt + c])) pr = r;
+      if (pr != c) {
+        for (int k = 0; k < nt; ++k) std::swap(A[(size_t)c * nt + k], A[(size_t)pr * nt + k]);
+        std::swap(rhs[c], rhs[pr]);
+      }
+      for (int r = c + 1; r < nt; ++r) {
+        const double m = A[(size_t)r * nt + c] / A[(size_t)c * nt + c];
+        for (int k = c; k < nt; ++k) A[(size_t)r * nt + k] -= m * A[(size_t)c * nt + k];
+        rhs[r] -= m * rhs[c];
+      }
+    }
+    for (int c = nt - 1; c >= 0; --c) {
+      double v = rhs[c];
+      for (int k = c + 1; k < nt; ++k) v -= A[(size_t)c * nt + k] * lam2[k];
+      lam2[c] = v / A[(size_t)c * nt + c];
+    }
+    // lambda1 = a - Z lambda2
+    copy(st, arf, aav, nf);
+    for (int u = 0; u < nt; ++u) axpy(st, nf, -lam2[u], Zv + (long)u * nf, arf);
+    // history, stiffness and pitch adjoints
+    for (int t = 0; t < nt; ++t) {
+      const TorsBody& b = h_tb[t];
+      const double M = b.M, Cd = b.C, K = b.K;
+      const double den = 4.0 / (dt * dt) * M + 2.0 / dt * Cd + K, kd = 4.0 / (dt * dt) * M + K;
+      const double c2 = delta / (dt * kd), scb = 2.0 * delta * delta / (dt * dt);
+      const double th0 = hth[3 * nt + t], thd0 = hth[4 * nt + t], thdd0 = hth[5 * nt + t];
+      const double Q = (M * (4.0 / (dt * dt) * th0 + 4.0 / dt * thd0 + thdd0) + Cd * (thd0 + 2.0 / dt * th0)) / den;
+      const double dQ0 = (4.0 / (dt * dt) * M + 2.0 / dt * Cd) / den, dQ1 = (4.0 / dt * M + Cd) / den, dQ2 = M / den;
+      const double ds0 = -delta * (2.0 / dt * dQ0 - 2.0 / dt), ds1 = -delta * (2.0 / dt * dQ1 - 1.0);
+      const double ds2 = -delta * 2.0 / dt * dQ2;
+      double brl1 = S0(t, 2), r1l1 = S0(t, 3);
+      for (int u = 0; u < nt; ++u) {
+        brl1 -= lam2[u] * SZ(t, u, 2);
+        r1l1 -= lam2[u] * SZ(t, u, 3);
+      }
+      const double l2 = lam2[t], brf = S0(t, 0), brpf = S0(t, 1);
+      double nb0 = -2.0 / dt * bthd[t] - 4.0 / (dt * dt) * bthdd[t] + brl1 * ds0 - l2 * dQ0;
+      double nb1 = -bthd[t] - 4.0 / dt * bthdd[t] + brl1 * ds1 - l2 * dQ1;
+      double nb2 = -bthdd[t] + brl1 * ds2 - l2 * dQ2;
+      gK[t] += scb * brl1 * brf / (den * den) + brl1 * (-delta * 2.0 / dt * (-Q / den)) - l2 * (-Q / den - c2 / kd * brf);
+      gG0[t] += r1l1 + c2 * brpf * l2 + S0(t, 4);
+      if (first) {
+        // initial_acc: thetadd0 = (-C thetad0 - K theta0 + delta/dt BR(theta0)^T f) / M
+        const double brp0 = S0(t, 9);
+        nb0 += nb2 * (-K / M + delta / dt * brp0 / M);
+        nb1 += nb2 * (-Cd / M);
+        gK[t] += nb2 * (-th0 / M);
+        gG0[t] += nb2 * (-delta / dt * brp0 / M);
+        nb2 = 0.0;
+      }
+      bth[t] = nb0;
+      bthd[t] = nb1;
+      bthdd[t] = nb2;
+    }
+    return IB_OK;
   }
 
   int adjoint_run(int nsteps, int Kin, const double* cx, const double* cy, double* J, double* dRe, double* dw0,
-                  double* dxb, double* dyb) {
-    if (!prm.num_stat || ntors > 0 || sd_on || nf == 0)
-      return fail(IB_E_UNSUPPORTED, "the adjoint covers rigid bodies with num_stat = 1");
+                  double* dxb, double* dyb, double* dK, double* dg0) {
+    const bool stat_ok = prm.num_stat && ntors == 0 && !sd_on && nf > 0;
+    const bool fsi_ok = !prm.num_stat && facemode && !sd_on && nf > 0;
+    if (!stat_ok && !fsi_ok)
+      return fail(IB_E_UNSUPPORTED,
+                  "the adjoint covers rigid bodies with num_stat = 1 and the face-space form (schur_mode 3, or 0 "
+                  "with torsional bodies)");
     if (nsteps < 1) return fail(IB_E_INVAL, "nsteps must be >= 1");
+    if (adj_ready && adj_fsi != fsi_ok) return fail(IB_E_INVAL, "adjoint set-up of another form");
+    adj_fsi = fsi_ok;
     const int K = adjoint_segment(nsteps, Kin);
     want_pos = dxb || dyb;
-    if (want_pos) adjoint_pos_setup();
+    if (want_pos && !adj_fsi) adjoint_pos_setup();
     adjoint_setup(nsteps, K);
-    if (want_pos) {
+    if (want_pos || adj_fsi) {
       fill(st, gpx, nf, 0.0);
       fill(st, gpy, nf, 0.0);
     }
+    const int nt = ntors;
+    bth.assign(nt, 0.0); bthd.assign(nt, 0.0); bthdd.assign(nt, 0.0); gK.assign(nt, 0.0); gG0.assign(nt, 0.0);
     const long LP = (long)L * g.P;
-    const long CQX = (long)L * g.Fx, CQY = (long)L * g.Fy;
+    const long CQX = (long)L * g.Fx, CQY = (long)L * g.Fy, BS = 6 * (long)nt + 2 * (long)nf;
     // objective weights per force entry: write_force2's 2 delta / dt, time mean
     std::vector<double> hw(nf);
     for (int k = 0; k < nf; ++k)
@@ -1320,6 +1605,8 @@ struct ib_handle {
     fill(st, racc, g.P, 0.0);
     const int it0 = itime;
     const int nseg = (nsteps + K - 1) / K, last0 = (nseg - 1) * K;
+    // the reverse sweep moves the bodies: keep the forward end state
+    const bool keep_end = nseg > 1 || adj_fsi;
     // forward: checkpoints at the starts of the segments before the last,
     // the last segment taped
     for (int n = 0; n < nsteps; ++n) {
@@ -1330,40 +1617,48 @@ struct ib_handle {
         if (n % K == 0) {
           const long c = n / K;
           save_state(c_w + c * LP, c_ro + c * LP, c_qx + c * CQX, c_qy + c * CQY);
+          if (adj_fsi) save_body(c_th + c * BS, c_f + c * nf);
         }
         rc = advance();
       }
       if (rc != IB_OK) return rc;
       dot_acc(st, nf, cw, f, 1.0, adJ);
     }
-    if (nseg > 1) {
+    if (keep_end) {
       save_state(fin_w, fin_ro, fin_qx, fin_qy);
       copy(st, fin_s, s, LP); copy(st, fin_f, f, nf); copy(st, fin_fr, f_rdst, nf);
+      if (adj_fsi) save_body(fin_th, nullptr);
     }
     // backward, segment by segment
     fill(st, aw, LP, 0.0); fill(st, aro, LP, 0.0);
     fill(st, aqx, CQX, 0.0); fill(st, aqy, CQY, 0.0);
-    for (int sgi = nseg - 1; sgi >= 0; --sgi) {
+    int status = IB_OK;
+    for (int sgi = nseg - 1; sgi >= 0 && status == IB_OK; --sgi) {
       const int n0 = sgi * K, n1 = std::min(nsteps, n0 + K);
       if (sgi < nseg - 1) {
         load_state(c_w + (long)sgi * LP, c_ro + (long)sgi * LP, c_qx + (long)sgi * CQX, c_qy + (long)sgi * CQY);
+        if (adj_fsi) load_body(c_th + (long)sgi * BS, c_f + (long)sgi * nf);
         itime = it0 + n0;
-        for (int n = n0; n < n1; ++n) {
-          const int rc = taped_advance(n - n0);
-          if (rc != IB_OK) return rc;
-        }
+        for (int n = n0; n < n1 && status == IB_OK; ++n) status = taped_advance(n - n0);
       }
-      for (int n = n1 - 1; n >= n0; --n) adjoint_step(n - n0, it0 + n + 1 == 1);
+      for (int n = n1 - 1; n >= n0 && status == IB_OK; --n) status = adjoint_step(n - n0, it0 + n + 1 == 1);
     }
     // the initial q is vort2flux of the initial omega
-    if (dw0) v2f_adj(aw, aqx, aqy);
+    if (dw0 && status == IB_OK) v2f_adj(aw, aqx, aqy);
     sum_acc(st, g.P, racc, 1.0, adot);
-    if (nseg > 1) {
+    if (keep_end) {
       // leave the handle in the forward end state
       load_state(fin_w, fin_ro, fin_qx, fin_qy);
+      if (adj_fsi) load_body(fin_th, nullptr);
       copy(st, s, fin_s, LP); copy(st, f, fin_f, nf); copy(st, f_rdst, fin_fr, nf);
     }
     itime = it0 + nsteps;
+    want_pos = want_pos && status == IB_OK;
+    if (status != IB_OK) {
+      sync();
+      want_pos = false;
+      return status;
+    }
     double hv[2] = {0.0, 0.0};
     IB_HIP(hipMemcpyAsync(hv, adJ.p, sizeof(double), hipMemcpyDeviceToHost, st));
     IB_HIP(hipMemcpyAsync(hv + 1, adot.p, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1373,7 +1668,8 @@ struct ib_handle {
     if (dRe) *dRe = hv[1];
     if (want_pos) {
       // per point: the sum over its x-force and y-force entries (both read
-      // the point's coordinates); points in deck order
+      // the point's coordinates); points in deck order; the points of
+      // torsional bodies (placed by their hinge and angle) get 0
       std::vector<double> gx(nf), gy(nf);
       d2h(gx.data(), gpx, nf);
       d2h(gy.data(), gpy, nf);
@@ -1389,6 +1685,16 @@ struct ib_handle {
         pt += n;
         k += 2 * n;
       }
+    }
+    // stiffness and pitch: per body in deck order (rigid bodies 0)
+    for (int b = 0; b < nbodies; ++b) {
+      if (dK) dK[b] = 0.0;
+      if (dg0) dg0[b] = 0.0;
+    }
+    for (int t = 0; t < nt; ++t) {
+      const int b = h_cbody[h_tb[t].off];
+      if (dK) dK[b] = gK[t];
+      if (dg0) dg0[b] = gG0[t];
     }
     want_pos = false;
     return IB_OK;

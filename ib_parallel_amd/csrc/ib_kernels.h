@@ -324,6 +324,22 @@ void gather_taps(hipStream_t st, const Geom& g, int ntaps, const int* key, const
 void pos_terms(hipStream_t st, const Geom& g, int nf, const int* key, const int* tapidx, const double* dwx,
                const double* dwy, const double* TT, const double* U, const double* u, const double* v,
                const double* qt, const double* hbx, const double* hby, double* gx, double* gy);
+// torsional bodies in the face-space form (ib_adjoint.hip)
+void dthw(hipStream_t st, int nf, int nt, const int* tors_of, const TorsBody* tb, const double* th, const double* dwx,
+          const double* dwy, double* out);
+void face_gather(hipStream_t st, const Geom& g, int nF, const int* fkeys, const double* hx, const double* hy,
+                 double* out);
+void face_regT(hipStream_t st, int nf, const double* w, const int* tapidx, const double* zf, double* out);
+void tap_contract(hipStream_t st, int i0, int i1, const double* wt, const double* coef, const double* ftap,
+                  const double* fset, const int* tapidx, double* out);
+void gemv_t(hipStream_t st, int n, const double* G, const double* x, double* y);
+void transpose(hipStream_t st, int n, const double* A, double* AT);
+void tors_block(hipStream_t st, const TorsBody* tb, const double* th, int nt, int t, int at0, double dt, double delta,
+                const double* f, const double* a, double* r1, double* zc, double* s);
+void pos_fsi(hipStream_t st, int nf, const int* tors_of, const int* tapidx, const double* dwx, const double* dwy,
+             const double* l, const double* f, const double* hb, const double* qt, const double* vT, const double* uT,
+             double* gx, double* gy);
+void dot_set(hipStream_t st, int i0, int i1, const double* a, const double* b, double* out);
 void rot_acc(hipStream_t st, const Geom& g, const double* hx, const double* hy, double* v);
 void rhs_adj(hipStream_t st, long n, const double* r, double c1, double c2, double* ro, double* rb);
 void re_pred(hipStream_t st, long n, const double* ws, const double* w, const double* wsb, const double* rb,

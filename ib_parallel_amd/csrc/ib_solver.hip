@@ -393,10 +393,11 @@ int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable) {
 }
 
 int ib_adjoint_run(ib_handle* h, int32_t nsteps, int32_t ckpt_every, const double* cx, const double* cy, double* J,
-                   double* dJ_dRe, double* dJ_domega0, double* dJ_dxb, double* dJ_dyb) {
+                   double* dJ_dRe, double* dJ_domega0, double* dJ_dxb, double* dJ_dyb, double* dJ_dK,
+                   double* dJ_dgamma0) {
   if (!h || !cx || !cy) return IB_E_INVAL;
   API_TRY
-  return h->adjoint_run(nsteps, ckpt_every, cx, cy, J, dJ_dRe, dJ_domega0, dJ_dxb, dJ_dyb);
+  return h->adjoint_run(nsteps, ckpt_every, cx, cy, J, dJ_dRe, dJ_domega0, dJ_dxb, dJ_dyb, dJ_dK, dJ_dgamma0);
   API_CATCH(h)
 }
 

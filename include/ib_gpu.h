@@ -265,7 +265,8 @@ int ib_batch_last_error(const ib_batch* b, char* buf, int32_t len);
  * IB_E_UNSUPPORTED).  h is left in the state
  * after the nsteps steps, as ib_advance would leave it. --- */
 int ib_adjoint_run(ib_handle* h, int32_t nsteps, int32_t ckpt_every, const double* cx, const double* cy, double* J,
-                   double* dJ_dRe, double* dJ_domega0, double* dJ_dxb, double* dJ_dyb);
+                   double* dJ_dRe, double* dJ_domega0, double* dJ_dxb, double* dJ_dyb, double* dJ_dK,
+                   double* dJ_dgamma0);
 
 #ifdef __cplusplus
 }

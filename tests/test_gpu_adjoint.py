@@ -146,13 +146,16 @@ def test_torch_function_gives_the_adjoint_gradients(warm):
 
 
 def test_unsupported_configurations_are_rejected():
+    """GMRES on an assembled (not face-space) or matrix-free operator has no
+    G to differentiate the weights with; num_stat treats torsional bodies as
+    rigid and is rejected with them."""
     from ib_parallel_amd import GpuSim, IBError
-    p, b = decks.cylinder_case(m=64, n=64, mgridlev=2, dt=0.01, npts=50, tol=1e-10, num_stat=0)
+    p, b = decks.cylinder_case(m=64, n=64, mgridlev=2, dt=0.01, npts=50, tol=1e-10, num_stat=0, schur_mode=2)
     s = GpuSim(p, b)
     with pytest.raises(IBError) as e:
         s.adjoint(2, CX, CY)
     assert e.value.code == abi.IB_E_UNSUPPORTED
-    pt, bt = decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-12)
+    pt, bt = decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-12, num_stat=1)
     st = GpuSim(pt, bt)
     with pytest.raises(IBError) as e:
         st.adjoint(2, [0.0] * st.nbodies, [1.0] * st.nbodies)
@@ -211,3 +214,100 @@ def test_dpositions_match_finite_differences(f32):
         ad = float(np.dot(gx, dx) + np.dot(gy, dy))
         print(f"f32 {f32} trial {trial}: <dJ/dx, d> adjoint {ad:.12e} fd {fd:.12e}")
         assert abs(ad - fd) <= 1e-5 * abs(fd)
+
+
+# ---- moving bodies: the FSI fixed point in the face-space form ------------
+
+def tors_case(re=50.0, k=0.2, g0=None, dx=None, dy=None):
+    """decks.tiny_case at 40x32x2: a rigid cylinder (moved off the grid lines,
+    where deltafnc's REAL(4) literals jump) and a hinged torsional plate
+    (face-space Schur form), tolerances 1e-12."""
+    p, b = decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-12, re=re)
+    nc = b[0].npts
+    t = 2 * np.pi * (np.arange(nc) + 0.29) / nc
+    x, y = 0.3 * np.cos(t) - 0.4 + 0.0123, 0.3 * np.sin(t) - 0.0071
+    if dx is not None:
+        x, y = x + dx, y + dy
+    b[0] = abi.BodySpec(x, y, fam=1)
+    b[1].ktheta = k
+    if g0 is not None:
+        b[1].gamma0 = g0
+    return p, b
+
+
+def tors_J(p, b, w0, nsteps, cx, cy):
+    return J_forward(p, b, w0, np.zeros_like(w0), nsteps, cx, cy)[0]
+
+
+def test_fsi_objective_and_end_state_match_plain_steps():
+    p, b = tors_case()
+    L, P = p.mgridlev, (p.m - 1) * (p.n - 1)
+    w0 = np.zeros(L * P)
+    cx, cy = np.array([0.4, 1.0]), np.array([0.2, -0.7])
+    Jf, sf = J_forward(p, b, w0, w0, 6, cx, cy)
+    sa = sim_from(p, b, w0, w0)
+    assert sa.schur_info()[0] == 3
+    Ja = sa.adjoint(6, cx, cy, ckpt_every=4)[0]
+    assert abs(Ja - Jf) <= 1e-13 * abs(Jf)
+    for k in range(1, p.mgridlev + 1):
+        a, f = sa.level(k), sf.level(k)
+        for key in ("omega", "s", "qx", "qy", "rhs_old"):
+            assert np.array_equal(a[key], f[key]), (k, key)
+    ba, bf = sa.body(), sf.body()
+    for key in ("f", "xb", "yb", "theta", "thetad", "thetadd"):
+        assert np.array_equal(ba[key], bf[key]), key
+    sa.advance(); sf.advance()
+    assert np.array_equal(sa.body()["f"], sf.body()["f"])
+
+
+@pytest.mark.parametrize("nsteps,ckpt", ((1, 1), (5, 2)))
+def test_fsi_stiffness_and_pitch_match_finite_differences(nsteps, ckpt):
+    p, b = tors_case()
+    L, P = p.mgridlev, (p.m - 1) * (p.n - 1)
+    w0 = np.zeros(L * P)
+    cx, cy = np.array([0.4, 1.0]), np.array([0.2, -0.7])
+    out = sim_from(p, b, w0, w0).adjoint(nsteps, cx, cy, ckpt_every=ckpt, body_grad=True)
+    dK, dg = out[3][1], out[4][1]
+    assert out[3][0] == 0.0 and out[4][0] == 0.0
+    # dJ/dK is small (~2e-3) against the FSI loop's 1e-12 convergence noise,
+    # so the difference in K takes a larger step with Richardson
+    # extrapolation (truncation O(h^4))
+    def fd_k(hk):
+        return (tors_J(*tors_case(k=0.2 + hk), w0, nsteps, cx, cy) -
+                tors_J(*tors_case(k=0.2 - hk), w0, nsteps, cx, cy)) / (2 * hk)
+    fdk = (4 * fd_k(1e-3) - fd_k(2e-3)) / 3
+    g0 = b[1].gamma0
+    hg = 1e-6
+    fdg = (tors_J(*tors_case(g0=g0 + hg), w0, nsteps, cx, cy) -
+           tors_J(*tors_case(g0=g0 - hg), w0, nsteps, cx, cy)) / (2 * hg)
+    print(f"nsteps {nsteps}: dJ/dK adjoint {dK:.12e} fd {fdk:.12e}; dJ/dgamma0 adjoint {dg:.12e} fd {fdg:.12e}")
+    assert abs(dK - fdk) <= 1e-5 * abs(fdk)
+    assert abs(dg - fdg) <= 1e-5 * abs(fdg)
+
+
+def test_fsi_re_omega_and_rigid_positions_match_finite_differences():
+    p, b = tors_case()
+    L, P = p.mgridlev, (p.m - 1) * (p.n - 1)
+    rng = np.random.default_rng(5)
+    w0 = 0.05 * rng.standard_normal(L * P)
+    cx, cy = np.array([0.4, 1.0]), np.array([0.2, -0.7])
+    n = 4
+    J, dre, dw, gx, gy = sim_from(p, b, w0, np.zeros_like(w0)).adjoint(n, cx, cy, ckpt_every=3, position_grad=True)
+    h = 1e-3
+    fd = (tors_J(*tors_case(re=50.0 + h), w0, n, cx, cy) - tors_J(*tors_case(re=50.0 - h), w0, n, cx, cy)) / (2 * h)
+    print(f"dJ/dRe adjoint {dre:.12e} fd {fd:.12e}")
+    assert abs(dre - fd) <= 1e-5 * abs(fd)
+    d = rng.standard_normal(L * P)
+    eps = 1e-5
+    fd = (tors_J(p, b, w0 + eps * d, n, cx, cy) - tors_J(p, b, w0 - eps * d, n, cx, cy)) / (2 * eps)
+    print(f"<dJ/dw0, d> adjoint {np.dot(dw, d):.12e} fd {fd:.12e}")
+    assert abs(np.dot(dw, d) - fd) <= 1e-5 * abs(fd)
+    nc = b[0].npts
+    assert np.all(gx[nc:] == 0.0) and np.all(gy[nc:] == 0.0)
+    ex, ey = rng.standard_normal(nc), rng.standard_normal(nc)
+    h = 1e-6
+    fd = (tors_J(*tors_case(dx=h * ex, dy=h * ey), w0, n, cx, cy) -
+          tors_J(*tors_case(dx=-h * ex, dy=-h * ey), w0, n, cx, cy)) / (2 * h)
+    ad = float(np.dot(gx[:nc], ex) + np.dot(gy[:nc], ey))
+    print(f"<dJ/dx, d> adjoint {ad:.12e} fd {fd:.12e}")
+    assert abs(ad - fd) <= 1e-5 * abs(fd)
