@@ -29,7 +29,8 @@ Also reported:
                 simulations against the same simulations stepped one after
                 another with the plain step (run_batched).
   adjoint       the discrete adjoint (ib_adjoint_run) against the plain
-                forward steps of the same rigid cylinder (run_adjoint).
+                forward steps of the same simulation: a rigid cylinder and
+                the hinged plate at 400x400x3 (run_adjoint).
 Workloads: plate (default, the headline), cylinder (rigid, 200x200x3) and
 airfoil_flap (rigid airfoil + torsional flap, 1024x512x5, full GMRES).
 """
@@ -181,44 +182,68 @@ def run_batched(args):
     return res
 
 
-def run_adjoint(args):
-    """Discrete adjoint (ib_adjoint_run) against the plain forward steps.
-
-    Workload: the rigid cylinder (200x200x3, 157 points, Re = 100, dt =
-    4e-3) on the num_stat path, started from the flow after a few steps.
-    Forward: ib_advance of N steps.  Adjoint: ib_adjoint_run of N steps (its
-    forward pass, the recomputation from checkpoints and the reverse sweep;
-    gradients of the time-mean drag + 0.5 lift with respect to Re and the
-    initial vorticity), once with the whole run taped in HBM and once with a
-    checkpoint every 10 steps.  cost_ratio = adjoint wall / forward wall."""
-    from ib_parallel_amd import GpuSim, decks
-    p, b = decks.cylinder_case(num_stat=1)
-    sim = GpuSim(p, b)
-    for _ in range(3):
-        sim.advance()
-    n = args.adjoint_steps
-    sim.adjoint(n, [1.0], [0.5], ckpt_every=None)  # set-up (dA/dRe, buffers) outside the timed regions
-    sim.adjoint(n, [1.0], [0.5], ckpt_every=10)
+def _time_adjoint(sim, n, cx, cy, ckpts, **grads):
+    """Forward wall time of n plain steps, then the wall time of n adjoint
+    steps per segment length in ckpts (None: the whole run taped)."""
+    sim.adjoint(n, cx, cy, ckpt_every=ckpts[0], **grads)  # set-up outside the timed regions
     device_sync()
     t0 = time.perf_counter()
     for _ in range(n):
         sim.advance()
     device_sync()
     tf = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    sim.adjoint(n, [1.0], [0.5], ckpt_every=None)
-    device_sync()
-    ta = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    sim.adjoint(n, [1.0], [0.5], ckpt_every=10)
-    device_sync()
-    tc = time.perf_counter() - t0
-    res = {"workload": "rigid cylinder 200x200x3, 157 pts, Re=100, dt=4e-3, num_stat; d/dRe and d/domega0 of "
-                       "mean(drag + 0.5 lift)",
-           "steps": n, "forward_ms_per_step": round(tf / n * 1e3, 3),
-           "adjoint_ms_per_step": round(ta / n * 1e3, 3), "cost_ratio": round(ta / tf, 3),
-           "adjoint_ckpt10_ms_per_step": round(tc / n * 1e3, 3), "cost_ratio_ckpt10": round(tc / tf, 3)}
+    ta = []
+    for K in ckpts:
+        t0 = time.perf_counter()
+        out = sim.adjoint(n, cx, cy, ckpt_every=K, **grads)
+        device_sync()
+        ta.append(time.perf_counter() - t0)
+    return tf, ta, out
+
+
+def run_adjoint(args):
+    """Discrete adjoint (ib_adjoint_run) against the plain forward steps.
+
+    rigid: the cylinder (200x200x3, 157 points, Re = 100, dt = 4e-3) on the
+    num_stat path, gradients of the time-mean drag + 0.5 lift with respect to
+    Re and the initial vorticity; the whole run taped, and a checkpoint every
+    10 steps.
+    fsi: the hinged plate at 400x400x3 (torsional body, K = 0.1, pitch 10 deg,
+    Re = 200, dt = 1e-3, GMRES and FSI tolerance 1e-10, face-space form),
+    gradients of the time-mean lift with respect to stiffness, pitch, Re and
+    the initial vorticity; a checkpoint every 20 steps, and the whole run
+    taped.
+    Forward: ib_advance of N steps.  Adjoint: ib_adjoint_run of N steps (its
+    forward pass, the recomputation from checkpoints and the reverse sweep).
+    cost_ratio = adjoint wall / forward wall."""
+    from ib_parallel_amd import GpuSim, decks
+    res = {}
+    p, b = decks.cylinder_case(num_stat=1)
+    sim = GpuSim(p, b)
+    for _ in range(3):
+        sim.advance()
+    n = args.adjoint_steps
+    tf, (ta, tc), _ = _time_adjoint(sim, n, [1.0], [0.5], [None, 10])
+    res["rigid"] = {"workload": "rigid cylinder 200x200x3, 157 pts, Re=100, dt=4e-3, num_stat; d/dRe and d/domega0 "
+                                "of mean(drag + 0.5 lift)",
+                    "steps": n, "forward_ms_per_step": round(tf / n * 1e3, 3),
+                    "adjoint_ms_per_step": round(ta / n * 1e3, 3), "cost_ratio": round(ta / tf, 3),
+                    "adjoint_ckpt10_ms_per_step": round(tc / n * 1e3, 3), "cost_ratio_ckpt10": round(tc / tf, 3)}
     sim.close()
+    if args.adjoint_fsi_steps > 0:
+        p, b = decks.plate_case(mgridlev=3)
+        sim = GpuSim(p, b)
+        for _ in range(2):
+            sim.advance()
+        n = args.adjoint_fsi_steps
+        tf, (tc, ta), out = _time_adjoint(sim, n, [0.0], [1.0], [20, None], body_grad=True)
+        res["fsi"] = {"workload": "hinged plate 400x400x3, 101 pts, torsional K=0.1, pitch 10 deg, Re=200, dt=1e-3, "
+                                  "tol 1e-10, face-space form; d/dK, d/dpitch, d/dRe, d/domega0 of mean(lift)",
+                      "steps": n, "forward_ms_per_step": round(tf / n * 1e3, 3),
+                      "adjoint_ckpt20_ms_per_step": round(tc / n * 1e3, 3), "cost_ratio_ckpt20": round(tc / tf, 3),
+                      "adjoint_ms_per_step": round(ta / n * 1e3, 3), "cost_ratio": round(ta / tf, 3),
+                      "dJ_dK": out[3][0], "dJ_dgamma0": out[4][0], "dJ_dRe": out[1]}
+        sim.close()
     return res
 
 
@@ -293,6 +318,8 @@ def main():
     ap.add_argument("--pmc-copy-mb", type=float, default=0.0,
                     help="run a device-to-device copy of this size first (calibrates counter runs)")
     ap.add_argument("--adjoint-steps", type=int, default=40, help="steps of the adjoint measurement (0: skip)")
+    ap.add_argument("--adjoint-fsi-steps", type=int, default=40,
+                    help="steps of the moving-body adjoint measurement (0: skip)")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint measurement (one JSON object)")
     args = ap.parse_args()
     if args.gpus != 1:
