@@ -247,6 +247,51 @@ def run_adjoint(args):
     return res
 
 
+def run_adjoint_batch(args):
+    """The two-body system as a batch with adjoints: elliptic airfoil (rigid)
+    + torsional flap at 1024x512x5 (decks.airfoil_flap_case), args.adjoint_batch
+    instances with the flap stiffness spread over [0.02, 0.1], each with the
+    gradient of its time-mean airfoil + flap drag with respect to the
+    stiffness (ib_batch_adjoint: one host thread and stream per instance).
+    Forward: ib_batch_advance of N steps; adjoint: ib_batch_adjoint of N
+    steps (forward pass, reverse sweep), the whole run taped."""
+    import numpy as np
+    from ib_parallel_amd import GpuBatch, decks
+    nb = args.adjoint_batch
+    ks = np.linspace(0.02, 0.1, nb)
+    cases = [decks.airfoil_flap_case(k_theta=float(k)) for k in ks]
+    t0 = time.perf_counter()
+    batch = GpuBatch(cases)
+    t_create = time.perf_counter() - t0
+    n = args.adjoint_batch_steps
+    batch.advance()
+    device_sync()
+    t0 = time.perf_counter()
+    batch.adjoint(1, [1.0] * (2 * nb), [0.0] * (2 * nb))  # adjoint set-up (dG/dRe per instance)
+    device_sync()
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(n):
+        batch.advance()
+    device_sync()
+    tf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J, dre, dk, dg = batch.adjoint(n, [1.0] * (2 * nb), [0.0] * (2 * nb))
+    device_sync()
+    ta = time.perf_counter() - t0
+    pts = sum(gridpts(s) for s in batch.sims)
+    res = {"workload": f"{nb} x (elliptic airfoil 300 pts rigid + torsional flap 100 pts) 1024x512x5, Re=1000, "
+                       "dt=5e-4, tol 1e-10, flap K_theta in [0.02, 0.1]; d/dK of mean(airfoil + flap drag)",
+           "instances": nb, "steps": n, "create_s": round(t_create, 1), "adjoint_setup_s": round(t_setup, 1),
+           "forward_ms_per_step": round(tf / n * 1e3, 2),
+           "forward_Mgridpt_steps_per_s": round(pts * n / tf / 1e6, 2),
+           "adjoint_ms_per_step": round(ta / n * 1e3, 2), "cost_ratio": round(ta / tf, 3),
+           "k_theta": [round(float(k), 4) for k in ks], "dJ_dK": [float(dk[2 * i + 1]) for i in range(nb)],
+           "mean_drag": [float(j) for j in J]}
+    batch.close()
+    return res
+
+
 def copy_calibration(mb):
     """A device-to-device copy of exactly mb MiB (a plain copy kernel of known
     size for calibrating FETCH_SIZE / WRITE_SIZE in counter runs)."""
@@ -321,6 +366,9 @@ def main():
     ap.add_argument("--adjoint-fsi-steps", type=int, default=40,
                     help="steps of the moving-body adjoint measurement (0: skip)")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint measurement (one JSON object)")
+    ap.add_argument("--adjoint-batch", type=int, default=0,
+                    help="instances of the airfoil + flap batch adjoint measurement (0: skip; 8 in the workload)")
+    ap.add_argument("--adjoint-batch-steps", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path",
@@ -330,7 +378,10 @@ def main():
     if args.pmc_copy_mb > 0:
         copy_calibration(args.pmc_copy_mb)
     if args.adjoint_only:
-        print(json.dumps({"adjoint": run_adjoint(args)}), flush=True)
+        out = {"adjoint": run_adjoint(args)} if args.adjoint_steps > 0 else {}
+        if args.adjoint_batch > 0:
+            out["adjoint_batch"] = run_adjoint_batch(args)
+        print(json.dumps(out), flush=True)
         return
     if args.batch_only:
         print(json.dumps({"batched": run_batched(args)}), flush=True)

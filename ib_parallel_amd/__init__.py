@@ -147,6 +147,23 @@ class GpuBatch:
             s.itime.value = t
         return list(self.infos)
 
+    def adjoint(self, nsteps, cx, cy, ckpt_every=None):
+        """ib_adjoint_run of every simulation, concurrently: cx, cy are per
+        body over all simulations (as the bodies of the cases); returns
+        (J per simulation, dJ/dRe per simulation, dJ/dK and dJ/dgamma0 per
+        body)."""
+        nb = sum(s.nbodies for s in self.sims)
+        cx = np.ascontiguousarray(np.broadcast_to(np.asarray(cx, dtype=np.float64), (nb,)))
+        cy = np.ascontiguousarray(np.broadcast_to(np.asarray(cy, dtype=np.float64), (nb,)))
+        n = len(self.sims)
+        J, dre, dk, dg = np.empty(n), np.empty(n), np.empty(nb), np.empty(nb)
+        K = int(ckpt_every) if ckpt_every else 0
+        self._check(self._b["batch_adjoint"](self.h, int(nsteps), K, abi.dptr(cx), abi.dptr(cy), abi.dptr(J),
+                                             abi.dptr(dre), abi.dptr(dk), abi.dptr(dg)))
+        for s in self.sims:
+            s.itime.value += int(nsteps)
+        return J, dre, dk, dg
+
     def close(self):
         if getattr(self, "h", None):
             for s in self.sims:

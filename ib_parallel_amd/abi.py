@@ -130,6 +130,7 @@ BATCH = {
     "batch_handle": (C.c_int, [_P, C.c_int32, C.POINTER(_P)]),
     "batch_info": (C.c_int, [_P, _I, _I]),
     "batch_last_error": (C.c_int, [_P, C.c_char_p, C.c_int32]),
+    "batch_adjoint": (C.c_int, [_P, C.c_int32, C.c_int32, _D, _D, _D, _D, _D, _D]),
 }
 
 

@@ -350,6 +350,41 @@ struct ib_batch {
     }
     return first;
   }
+
+  // every simulation's discrete adjoint (ib_adjoint_run) from its own host
+  // thread on its own stream; cx, cy, dK, dg0 run over the bodies of
+  // simulation 0, then 1, ...
+  int adjoint(int nsteps, int ckpt, const double* cx, const double* cy, double* J, double* dRe, double* dK,
+              double* dg0) {
+    const int n = (int)sims.size();
+    std::vector<int> boff(n + 1, 0);
+    for (int i = 0; i < n; ++i) boff[i + 1] = boff[i] + sims[i]->nbodies;
+    std::vector<int> rc(n, IB_OK);
+    std::vector<std::string> ex(n);
+    std::vector<std::thread> pool;
+    pool.reserve(n);
+    for (int i = 0; i < n; ++i)
+      pool.emplace_back([&, i]() {
+        try {
+          const int o = boff[i];
+          rc[i] = sims[i]->adjoint_run(nsteps, ckpt, cx + o, cy + o, J ? J + i : nullptr, dRe ? dRe + i : nullptr,
+                                       nullptr, nullptr, nullptr, dK ? dK + o : nullptr, dg0 ? dg0 + o : nullptr);
+        } catch (const std::exception& e) {
+          ex[i] = e.what();
+          rc[i] = IB_E_HIP;
+        }
+      });
+    for (auto& t : pool) t.join();
+    int first = IB_OK;
+    for (int i = 0; i < n; ++i) {
+      if (!ex[i].empty()) sims[i]->err = ex[i];
+      if (rc[i] != IB_OK && first == IB_OK) {
+        first = rc[i];
+        err = "simulation " + std::to_string(i) + ": " + sims[i]->err;
+      }
+    }
+    return first;
+  }
 };
 
 extern "C" {
@@ -385,6 +420,17 @@ int ib_batch_advance(ib_batch* b, int32_t* itimes, ib_step_info* infos) {
   if (!b) return IB_E_INVAL;
   try {
     return b->advance(itimes, infos);
+  } catch (const std::exception& e) {
+    b->err = e.what();
+    return IB_E_HIP;
+  }
+}
+
+int ib_batch_adjoint(ib_batch* b, int32_t nsteps, int32_t ckpt_every, const double* cx, const double* cy, double* J,
+                     double* dJ_dRe, double* dJ_dK, double* dJ_dgamma0) {
+  if (!b || !cx || !cy) return IB_E_INVAL;
+  try {
+    return b->adjoint(nsteps, ckpt_every, cx, cy, J, dJ_dRe, dJ_dK, dJ_dgamma0);
   } catch (const std::exception& e) {
     b->err = e.what();
     return IB_E_HIP;

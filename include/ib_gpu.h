@@ -245,6 +245,13 @@ int ib_batch_handle(ib_batch* b, int32_t i, ib_handle** h);
 int ib_batch_info(const ib_batch* b, int32_t* nsim, int32_t* mode);
 /* message of the last failing call on b (or of ib_batch_create when b is NULL) */
 int ib_batch_last_error(const ib_batch* b, char* buf, int32_t len);
+/* ib_adjoint_run of every simulation, each from its own host thread on its
+ * own stream (no reference counterpart): cx, cy, dJ_dK and dJ_dgamma0 run over
+ * the bodies of simulation 0, then 1, ... (as in ib_batch_create); J and
+ * dJ_dRe get one value per simulation.  Returns IB_OK or the first failing
+ * simulation's status. */
+int ib_batch_adjoint(ib_batch* b, int32_t nsteps, int32_t ckpt_every, const double* cx, const double* cy, double* J,
+                     double* dJ_dRe, double* dJ_dK, double* dJ_dgamma0);
 
 /* --- discrete adjoint (no reference counterpart): advances h by nsteps
  * steps and returns the time-mean weighted force objective

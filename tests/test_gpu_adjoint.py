@@ -311,3 +311,22 @@ def test_fsi_re_omega_and_rigid_positions_match_finite_differences():
     ad = float(np.dot(gx[:nc], ex) + np.dot(gy[:nc], ey))
     print(f"<dJ/dx, d> adjoint {ad:.12e} fd {fd:.12e}")
     assert abs(ad - fd) <= 1e-5 * abs(fd)
+
+
+def test_batch_adjoint_equals_each_simulations_adjoint():
+    """ib_batch_adjoint runs every simulation's adjoint from its own host
+    thread: the results are those of each handle's own ib_adjoint_run."""
+    from ib_parallel_amd import GpuBatch
+    cases = [tors_case(k=k) for k in (0.15, 0.2, 0.3)]
+    cx, cy = np.array([0.4, 1.0] * 3), np.array([0.2, -0.7] * 3)
+    batch = GpuBatch(cases)
+    J, dre, dk, dg = batch.adjoint(4, cx, cy, ckpt_every=3)
+    for i, (p, b) in enumerate(cases):
+        L, P = p.mgridlev, (p.m - 1) * (p.n - 1)
+        z = np.zeros(L * P)
+        out = sim_from(p, b, z, z).adjoint(4, cx[2 * i:2 * i + 2], cy[2 * i:2 * i + 2], ckpt_every=3,
+                                            omega_grad=False, body_grad=True)
+        assert J[i] == out[0] and dre[i] == out[1], i
+        assert np.array_equal(dk[2 * i:2 * i + 2], out[3]) and np.array_equal(dg[2 * i:2 * i + 2], out[4]), i
+    assert all(s.itime.value == 4 for s in batch.sims)
+    batch.close()
