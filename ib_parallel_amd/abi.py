@@ -34,7 +34,7 @@ class IBParams(C.Structure):
         ("gmres_restart", C.c_int32), ("gmres_max_it", C.c_int32), ("fsi_max_count", C.c_int32),
         ("deltafnc_f32_literals", C.c_int32), ("num_stat", C.c_int32), ("sub_domain", C.c_int32),
         ("launch_mode", C.c_int32), ("schur_mode", C.c_int32),
-        ("face_margin", C.c_int32), ("reserved", C.c_int32 * 3),
+        ("face_margin", C.c_int32), ("gmres_form", C.c_int32), ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -158,7 +158,7 @@ def dptr(a):
 
 def make_params(m, n, mgridlev, dt, re, len_, offsetx, offsety, fsi_tol=1e-7, atol=1e-7, rtol=1e-7,
                 gmres_restart=30, gmres_max_it=10000, fsi_max_count=1000, deltafnc_f32_literals=1,
-                istart=0, num_stat=0, sub_domain=0, launch_mode=0, schur_mode=0, face_margin=0):
+                istart=0, num_stat=0, sub_domain=0, launch_mode=0, schur_mode=0, face_margin=0, gmres_form=0):
     p = IBParams()
     p.m, p.n, p.mgridlev, p.istart = m, n, mgridlev, istart
     p.dt, p.re, p.len, p.offsetx, p.offsety = dt, re, len_, offsetx, offsety
@@ -168,6 +168,7 @@ def make_params(m, n, mgridlev, dt, re, len_, offsetx, offsety, fsi_tol=1e-7, at
     p.launch_mode = launch_mode
     p.schur_mode = schur_mode
     p.face_margin = face_margin
+    p.gmres_form = gmres_form
     return p
 
 

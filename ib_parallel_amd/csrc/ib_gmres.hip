@@ -691,11 +691,403 @@ void gmres_dense(hipStream_t st, int nf, int kmax, int maxit, const double* S, c
   IB_LAUNCH_CHECK();
 }
 
+// ------------------------------------------------------------------------
+// The same dense solve spread over P workgroups of one XCD.  Workgroup c
+// owns rows [c R, c R + R) of S (in registers: at most kCoopMaxC columns of
+// one row per thread) and those rows of every Krylov vector (in LDS), so
+// neither S nor the basis is re-read from L2.  Per Arnoldi step the
+// workgroups exchange, through L2 and two grid barriers, the partial CGS
+// coefficients of their rows, then the partial squared norm and their rows
+// of w.  Every workgroup sums the partials of all workgroups in the same
+// order and runs the rotations, tests and back substitution on identical
+// values, so all take the same branches; each updates its rows of x.  Only
+// workgroups 0, 8, 16, ... take part: the dispatcher deals workgroups to the
+// 8 XCDs in turn, so they share one L2.  The exchanged values are device-
+// scope atomic stores and loads (coherent without cache write-backs); the
+// barrier waits are bounded: on a timeout every workgroup leaves with
+// reason -3.
+// ------------------------------------------------------------------------
+constexpr int kCoopMaxC = 12;
+constexpr long kCoopSpin = 1L << 22;
+
+struct GmCoopArgs {
+  int nf, kmax, maxit, P, R;
+  int nseg, per;  // matvec layout: nseg column segments of `per` columns per local row
+  int bsc;        // Hessenberg columns per back-substitution chunk
+  const double* S;
+  const double* b;
+  double* x;
+  GmresDev* s;
+  double rtol, atol;
+  unsigned* sync;  // [0] arrivals, [1] generation, [2] error
+  double* dots;    // P x (kmax + 2) partial CGS coefficients
+  double* norms;   // P x 4 partial squared norms
+  double* wfull;   // nf: w (or r) rows of every workgroup
+  double* H;       // P x (kmax + 1) x kmax: one Hessenberg copy per workgroup
+};
+
+// A value another workgroup wrote before the last grid barrier.  An atomic
+// load is always a vector memory access (served by L2 once the barrier has
+// invalidated L1); a plain load of a uniform address could be served by the
+// scalar cache, which the barrier does not refresh.
+__device__ __forceinline__ double gc_ld(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// A value other workgroups read after the next grid barrier (coherent at
+// device scope without a cache write-back).
+__device__ __forceinline__ void gc_st(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid barrier over the P workgroups: every thread's global writes are made
+// visible at device scope, workgroup thread 0 arrives on the counter and
+// waits for the generation to move (bounded), then invalidates its L1 view.
+__device__ __forceinline__ bool gc_sync(const GmCoopArgs& a, int* s_err) {
+  // every exchanged value is a device-scope atomic store: waiting for their
+  // completion (workgroup-scope release) orders them before the arrival
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned* sy = a.sync;
+    const unsigned g0 = __hip_atomic_load(sy + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    const unsigned prev = __hip_atomic_fetch_add(sy, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev + 1u == (unsigned)a.P) {
+      __hip_atomic_store(sy, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __hip_atomic_fetch_add(sy + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      long n = 0;
+      while (__hip_atomic_load(sy + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g0) {
+        if (__hip_atomic_load(sy + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++n > kCoopSpin) {
+          __hip_atomic_store(sy + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          *s_err = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+  __syncthreads();
+  return *s_err == 0;
+}
+
+__global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
+  if (blockIdx.x % 8 != 0) return;
+  const int c = blockIdx.x / 8;
+  if (c >= a.P) return;
+  extern __shared__ double dyn[];
+  const int nf = a.nf, kmax = a.kmax, ld = kmax + 1, T = kGmT, R = a.R;
+  const int r0 = c * R, Rc = max(0, min(nf, r0 + R) - r0);
+  double* vfull = dyn;                          // nf: the vector being multiplied
+  double* Vl = vfull + nf;                      // (kmax + 1) R: local rows of the basis, vector j at Vl + j R
+  double* wl = Vl + (size_t)(kmax + 1) * R;     // R: local rows of w
+  double* part = wl + R;                        // T
+  double* hcol = part + T;                      // kmax + 2
+  double* lcc = hcol + kmax + 2;                // kmax
+  double* lss = lcc + kmax;                     // kmax
+  double* grs = lss + kmax;                     // kmax + 1
+  double* yv = grs + kmax + 1;                  // kmax + 1
+  __shared__ double red[kGmT / 64];
+  __shared__ double s_val[2];
+  __shared__ int s_flag[2];
+  __shared__ int s_err;
+  if (threadIdx.x == 0) s_err = 0;
+  double* Hc = a.H + (size_t)c * ld * kmax;
+  const int tq = gm_opaque(threadIdx.x) / R, tr = gm_opaque(threadIdx.x) - tq * R;
+  const bool mact = tq < a.nseg && tr < Rc;
+  const int jc0 = tq * a.per, jc1 = min(nf, jc0 + a.per);
+  double sreg[kCoopMaxC];
+#pragma unroll
+  for (int q = 0; q < kCoopMaxC; ++q) sreg[q] = (mact && jc0 + q < jc1) ? a.S[(long)(jc0 + q) * nf + r0 + tr] : 0.0;
+  __syncthreads();
+
+  // wl = S[local rows] vfull (ends with a barrier)
+  auto matvec = [&]() {
+    if (mact) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q0 = 0; q0 < kCoopMaxC; q0 += 8) {
+        double vv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) vv[q] = q0 + q < kCoopMaxC ? vfull[min(jc0 + q0 + q, nf - 1)] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (q0 + q < kCoopMaxC) acc += sreg[q0 + q] * vv[q];  // sreg is 0 past the segment
+      }
+      part[tq * R + tr] = acc;
+    }
+    __syncthreads();
+    for (int r = gm_opaque(threadIdx.x); r < Rc; r += T) {
+      double acc = 0.0;
+      for (int q = 0; q < a.nseg; ++q) acc += part[q * R + r];
+      wl[r] = acc;
+    }
+    __syncthreads();
+  };
+
+  int its = 0, reason = 0, first = 1;
+  double rn0 = 0.0, ttol = 0.0;
+  while (true) {
+    // ---- cycle start: r = b - S x (x gathered from every workgroup's rows)
+    for (int i = gm_opaque(threadIdx.x); i < nf; i += T) vfull[i] = gc_ld(a.x + i);
+    __syncthreads();
+    matvec();
+    double pr = 0.0, pb = 0.0;
+    for (int r = gm_opaque(threadIdx.x); r < Rc; r += T) {
+      const double bv = a.b[r0 + r];
+      const double rv = bv - wl[r];
+      gc_st(a.wfull + r0 + r, rv);
+      pr += rv * rv;
+      pb += bv * bv;
+    }
+    pr = gm_block_sum2(pr, red, T);
+    pb = gm_block_sum2(pb, red, T);
+    if (threadIdx.x == 0) {
+      gc_st(a.norms + c * 4 + 1, pr);
+      gc_st(a.norms + c * 4 + 2, pb);
+    }
+    if (!gc_sync(a, &s_err)) break;
+    double sr = 0.0, sb = 0.0;
+    for (int q = 0; q < a.P; ++q) {
+      sr += gc_ld(a.norms + q * 4 + 1);
+      sb += gc_ld(a.norms + q * 4 + 2);
+    }
+    const double res0 = sqrt(sr), bn = sqrt(sb);
+    if (first) {
+      rn0 = bn > 0.0 ? bn : res0;
+      ttol = fmax(a.rtol * rn0, a.atol);
+      first = 0;
+    }
+    if (res0 == 0.0 || res0 <= ttol) reason = 1;
+    else if (res0 >= 1e4 * rn0) reason = -1;
+    if (reason) break;
+    if (threadIdx.x == 0) grs[0] = res0;
+    for (int i = gm_opaque(threadIdx.x); i < nf; i += T) vfull[i] = gc_ld(a.wfull + i) / res0;
+    for (int r = gm_opaque(threadIdx.x); r < Rc; r += T) Vl[r] = gc_ld(a.wfull + r0 + r) / res0;
+    __syncthreads();
+    int it = 0;
+    bool ok = true;
+    while (true) {
+      matvec();  // w = S v_it, local rows
+      // partial CGS coefficients <w, v_j>, j = 0..it, over the local rows:
+      // 16 lanes per coefficient
+      {
+        const int t = gm_opaque(threadIdx.x), sub = t & 15;
+        for (int j = t >> 4; j <= it; j += T / 16) {
+          double p = 0.0;
+          for (int r = sub; r < Rc; r += 16) p += wl[r] * Vl[(long)j * R + r];
+          p = gm_row_sum16(p);
+          if (sub == 0) gc_st(a.dots + (long)c * (kmax + 2) + j, p);
+        }
+      }
+      if (!gc_sync(a, &s_err)) {
+        ok = false;
+        break;
+      }
+      for (int j = gm_opaque(threadIdx.x); j <= it; j += T) {
+        double h = 0.0;
+        for (int q = 0; q < a.P; ++q) h += gc_ld(a.dots + (long)q * (kmax + 2) + j);
+        hcol[j] = h;
+      }
+      __syncthreads();
+      // w -= sum_j h_j v_j on the local rows (j range split over segments,
+      // partials added in segment order); partial ||w||^2; w rows to L2
+      {
+        const int nj = it + 1;
+        const int nsg = max(1, min(a.nseg, nj));
+        const int pj = (nj + nsg - 1) / nsg;
+        if (tq < nsg && tr < Rc) {
+          const int q0 = min(nj, tq * pj), q1 = min(nj, q0 + pj);
+          double acc = 0.0;
+          for (int j = q0; j < q1; ++j) acc += -hcol[j] * Vl[(long)j * R + tr];
+          part[tq * R + tr] = acc;
+        }
+        __syncthreads();
+        double pn = 0.0;
+        for (int r = gm_opaque(threadIdx.x); r < Rc; r += T) {
+          double acc = wl[r];
+          for (int q = 0; q < nsg; ++q) acc += part[q * R + r];
+          wl[r] = acc;
+          gc_st(a.wfull + r0 + r, acc);
+          pn += acc * acc;
+        }
+        pn = gm_block_sum2(pn, red, T);
+        if (threadIdx.x == 0) gc_st(a.norms + c * 4, pn);
+      }
+      if (!gc_sync(a, &s_err)) {
+        ok = false;
+        break;
+      }
+      double sn2 = 0.0;
+      for (int q = 0; q < a.P; ++q) sn2 += gc_ld(a.norms + q * 4);
+      const double tt = sqrt(sn2);
+      if (gm_opaque(threadIdx.x) < 64) {
+        const int lane = gm_opaque(threadIdx.x);
+        // previous rotations on the new column (KSPGMRESUpdateHessenberg) as
+        // an affine-recurrence scan, 64 rotations at a time
+        double carry = hcol[0];
+        for (int base = 0; base < it; base += 64) {
+          const int j = base + lane;
+          const bool act = j < it;
+          const double cj0 = act ? lcc[j] : 1.0, sj = act ? lss[j] : 0.0, hn = act ? hcol[j + 1] : 0.0;
+          double A = -sj, B = cj0 * hn;
+          gm_affine_scan(A, B);
+          const double cnext = A * carry + B;
+          const double cj = gm_shift_up1(cnext, carry);
+          if (act) hcol[j] = cj0 * cj + sj * hn;
+          carry = gm_readlane(cnext, min(63, it - 1 - base));
+        }
+        if (lane == 0) {
+          hcol[it] = carry;
+          hcol[it + 1] = tt;
+          double hapbnd = fabs(tt / grs[it]);
+          if (hapbnd > 1e-30) hapbnd = 1e-30;
+          const int hap = tt < hapbnd;
+          double res;
+          if (!hap) {
+            const double t1 = sqrt(hcol[it] * hcol[it] + hcol[it + 1] * hcol[it + 1]);
+            const double cc = hcol[it] / t1, sn = hcol[it + 1] / t1;
+            lcc[it] = cc;
+            lss[it] = sn;
+            grs[it + 1] = -sn * grs[it];
+            grs[it] = cc * grs[it];
+            hcol[it] = cc * hcol[it] + sn * hcol[it + 1];
+            res = fabs(grs[it + 1]);
+          } else {
+            res = 0.0;
+          }
+          int rs = 0;
+          if (res <= ttol) rs = 1;
+          else if (res >= 1e4 * rn0) rs = -1;
+          if (hap && !rs) rs = 1;
+          s_flag[0] = rs;
+          s_flag[1] = hap;
+          s_val[0] = tt;
+        }
+      }
+      __syncthreads();
+      reason = s_flag[0];
+      const int hap = s_flag[1];
+      for (int j = gm_opaque(threadIdx.x); j <= it + 1; j += T) Hc[(long)it * ld + j] = hcol[j];
+      if (!hap) {
+        const double inv = 1.0 / s_val[0];
+        for (int i = gm_opaque(threadIdx.x); i < nf; i += T) vfull[i] = gc_ld(a.wfull + i) * inv;
+        for (int r = gm_opaque(threadIdx.x); r < Rc; r += T) Vl[(long)(it + 1) * R + r] = gc_ld(a.wfull + r0 + r) * inv;
+      }
+      ++it;
+      ++its;
+      __syncthreads();
+      if (reason || it >= kmax || its >= a.maxit) break;
+    }
+    if (!ok) break;
+    // ---- KSPGMRESBuildSoln: back substitution on this workgroup's copy of
+    // the rotated Hessenberg (chunks staged in LDS, the serial sweep by one
+    // wavefront), then x += V y on the local rows
+    const int top = it - 1;
+    for (int j = gm_opaque(threadIdx.x); j <= top; j += T) yv[j] = grs[j];
+    for (int k1 = top; k1 >= 0; k1 -= a.bsc) {
+      const int k0 = k1 - a.bsc + 1 < 0 ? 0 : k1 - a.bsc + 1;
+      const int nc = k1 - k0 + 1;
+      __syncthreads();
+      for (int q = gm_opaque(threadIdx.x); q < nc * (k1 + 1); q += T) {
+        const int cq = q / (k1 + 1), rr = q - cq * (k1 + 1);
+        part[(long)cq * ld + rr] = Hc[(long)(k0 + cq) * ld + rr];
+      }
+      __syncthreads();
+      if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        for (int k = k1; k >= k0; --k) {
+          const double* col = part + (long)(k - k0) * ld;
+          const double yk = yv[k] / col[k];
+          for (int j = lane; j < k; j += 64) yv[j] = yv[j] - col[j] * yk;
+          if (lane == 0) yv[k] = yk;
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+    }
+    __syncthreads();
+    for (int r = gm_opaque(threadIdx.x); r < Rc; r += T) {
+      double acc = 0.0;
+      for (int j = 0; j <= top; ++j) acc += yv[j] * Vl[(long)j * R + r];
+      gc_st(a.x + r0 + r, gc_ld(a.x + r0 + r) + acc);
+    }
+    if (reason) break;
+    if (its >= a.maxit) {
+      reason = -2;
+      break;
+    }
+    if (!gc_sync(a, &s_err)) break;  // every workgroup's rows of x before the next residual
+  }
+  if (threadIdx.x == 0) {
+    if (s_err) {
+      a.s->reason = -3;
+      a.s->its = its;
+    } else if (c == 0) {
+      a.s->its = its;
+      a.s->reason = reason;
+      a.s->rn0 = rn0;
+      a.s->ttol = ttol;
+    }
+  }
+}
+
+// dynamic LDS of k_gmres_coop (doubles): vfull nf, Vl (kmax+1) R, wl R,
+// part kGmT, hcol kmax+2, lcc/lss kmax each, grs/yv kmax+1 each
+static size_t coop_lds(int nf, int kmax, int R) {
+  return sizeof(double) * ((size_t)nf + (size_t)(kmax + 1) * R + R + kGmT + (size_t)5 * kmax + 4);
+}
+
+bool gmres_coop_plan(int nf, int kmax, size_t max_lds, int* P, int* R) {
+  static const int cand[] = {2, 4, 8, 16, 32};
+  for (int p : cand) {
+    const int r = (nf + p - 1) / p;
+    if (r < 1 || r > kGmT || (p - 1) * r >= nf) continue;
+    const int ns = kGmT / r;
+    const int pc = (nf + ns - 1) / ns;
+    if (pc > kCoopMaxC) continue;
+    if (kGmT / (kmax + 1) < 1) continue;  // back-substitution chunk in the part buffer
+    if (coop_lds(nf, kmax, r) + 1024 > max_lds) continue;
+    *P = p;
+    *R = r;
+    return true;
+  }
+  return false;
+}
+
+size_t gmres_coop_ws(int nf, int kmax, int P) {
+  return 8 + (size_t)P * (kmax + 2) + (size_t)4 * P + (size_t)nf + (size_t)P * (kmax + 1) * kmax;
+}
+
+void gmres_coop(hipStream_t st, int nf, int kmax, int maxit, int P, int R, const double* S, const double* b, double* x,
+                GmresDev* s, double rtol, double atol, double* ws) {
+  const int ns = kGmT / R;
+  GmCoopArgs a{};
+  a.nf = nf; a.kmax = kmax; a.maxit = maxit; a.P = P; a.R = R;
+  a.nseg = ns; a.per = (nf + ns - 1) / ns; a.bsc = kGmT / (kmax + 1);
+  a.S = S; a.b = b; a.x = x; a.s = s; a.rtol = rtol; a.atol = atol;
+  a.sync = reinterpret_cast<unsigned*>(ws);
+  a.dots = ws + 8;
+  a.norms = a.dots + (size_t)P * (kmax + 2);
+  a.wfull = a.norms + (size_t)4 * P;
+  a.H = a.wfull + nf;
+  IB_HIP(hipMemsetAsync(ws, 0, 8 * sizeof(double), st));
+  // an ordinary launch: the barrier is the kernel's own (no runtime grid
+  // synchronisation resources), 8P workgroups of which P stay, one per CU
+  // on a device of 256 CUs; a workgroup that is not resident in time makes
+  // the others leave with reason -3 instead of waiting for ever
+  hipLaunchKernelGGL(k_gmres_coop, dim3(8 * P), dim3(kGmT), coop_lds(nf, kmax, R), st, a);
+  IB_LAUNCH_CHECK();
+}
+
 // the kernel also has a few hundred bytes of static shared memory, so its
 // dynamic share stops 1 KiB short of the device limit (as gmres_dense assumes)
 void gmres_dense_set_max_lds(size_t bytes) {
   if (bytes <= 1024) return;
   IB_HIP(hipFuncSetAttribute((const void*)k_gmres_dense<24>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(bytes - 1024)));
+  IB_HIP(hipFuncSetAttribute((const void*)k_gmres_coop, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)(bytes - 1024)));
 }
 

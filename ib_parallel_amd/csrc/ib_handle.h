@@ -121,6 +121,10 @@ struct ib_handle {
   int kmax = 30;
   DBuf<double> V, H, cc, ss, grs, nrs, pnorm;
   GmresDev* gst = nullptr;
+  // the dense solve over several workgroups (gmres_coop): workgroups, rows
+  // per workgroup, workspace (barrier words, partials, w, Hessenberg copies)
+  int coopP = 0, coopR = 0;
+  DBuf<double> coopws;
   GmresDev* hst = nullptr;  // pinned
   double* hbuf = nullptr;   // pinned readback
   int* hflag = nullptr;     // pinned
@@ -509,18 +513,29 @@ struct ib_handle {
     }
     IB_HIP(hipGraphLaunch(itr->second, st));
   }
+  // the whole solve in one launch on a dense matrix M: several workgroups
+  // (gmres_coop) where the split fits, else one workgroup
+  bool dense_solve_ok() const {
+    return coopP > 0 || (nf <= kFusedGmresMax && gmres_dense_fits(nf, kmax, max_lds));
+  }
+  int dense_solve(const double* M, const double* b, double* x) {
+    if (coopP > 0) gmres_coop(st, nf, kmax, prm.gmres_max_it, coopP, coopR, M, b, x, gst, prm.rtol, prm.atol, coopws);
+    else gmres_dense(st, nf, kmax, prm.gmres_max_it, M, b, x, V, H, gst, prm.rtol, prm.atol, max_lds);
+    read_gmres();
+    if (hst->reason == -3) return fail(IB_E_HIP, "GMRES: a grid barrier of the cooperative solve timed out");
+    return hst->reason > 0 ? IB_OK : IB_E_NOCONV_GMRES;
+  }
   int gmres_dev(const double* b, double* x, int* its_out) {
-    if (assembled && nf <= kFusedGmresMax && gmres_dense_fits(nf, kmax, max_lds)) {
-      // dense Schur matrix: the whole solve is one single-workgroup launch.
-      // Algorithmic bytes: S, b read once, x read and written, and one
-      // Krylov vector written per iteration (added once the count is known)
+    if (assembled && dense_solve_ok()) {
+      // dense Schur matrix: the whole solve is one launch.  Algorithmic
+      // bytes: S, b read once, x read and written, and one Krylov vector
+      // written per iteration (added once the count is known)
       tim[4].begin(st, 8.0 * ((double)nf * nf + 3.0 * nf));
-      gmres_dense(st, nf, kmax, prm.gmres_max_it, Sd, b, x, V, H, gst, prm.rtol, prm.atol, max_lds);
+      const int rc = dense_solve(Sd, b, x);
       tim[4].end(st);
-      read_gmres();
       if (tim[4].on) tim[4].total_bytes += 8.0 * (double)nf * (hst->its + 1);
       *its_out = hst->its;
-      return hst->reason > 0 ? IB_OK : IB_E_NOCONV_GMRES;
+      return rc;
     }
     GmresDev init{};
     init.first = 1;
@@ -1172,11 +1187,7 @@ struct ib_handle {
   // GMRES on S^T (SdT), zero initial guess, the deck's tolerances
   int solve_T(const double* b, double* x) {
     fill(st, x, nf, 0.0);
-    if (nf <= kFusedGmresMax && gmres_dense_fits(nf, kmax, max_lds)) {
-      gmres_dense(st, nf, kmax, prm.gmres_max_it, SdT, b, x, V, H, gst, prm.rtol, prm.atol, max_lds);
-      read_gmres();
-      return hst->reason > 0 ? IB_OK : IB_E_NOCONV_GMRES;
-    }
+    if (dense_solve_ok()) return dense_solve(SdT, b, x);
     GmresDev init{};
     init.first = 1;
     *hst = init;
@@ -1779,6 +1790,15 @@ struct ib_handle {
       dst_set_max_lds(max_lds);
       gmres_dense_set_max_lds(max_lds);
     }
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) ncu = 0;
+    (void)hipGetLastError();
+    if (prm.gmres_form < 0 || prm.gmres_form > 2) return fail(IB_E_INVAL, "gmres_form must be 0, 1 or 2");
+    coopP = coopR = 0;
+    if (prm.gmres_form != 1 && nf > 0 && gmres_coop_plan(nf, kmax, max_lds, &coopP, &coopR) && 8 * coopP <= ncu)
+      coopws.alloc(gmres_coop_ws(nf, kmax, coopP));
+    else
+      coopP = coopR = 0;
 
     px.init(m - 1);
     py.init(n - 1);

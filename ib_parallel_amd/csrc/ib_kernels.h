@@ -391,6 +391,14 @@ void gmres_dense(hipStream_t st, int nf, int kmax, int maxit, const double* S, c
                  double* H, GmresDev* s, double rtol, double atol, size_t max_lds);
 void gmres_dense_set_max_lds(size_t bytes);
 bool gmres_dense_fits(int nf, int kmax, size_t max_lds);
+// The dense solve over P workgroups of one XCD (rows of S and of the basis
+// on chip, two grid barriers per step); plan: P and rows per workgroup R,
+// false if no split fits.  ws holds gmres_coop_ws doubles.  Status as
+// gmres_dense; reason -3: a grid barrier timed out.
+bool gmres_coop_plan(int nf, int kmax, size_t max_lds, int* P, int* R);
+size_t gmres_coop_ws(int nf, int kmax, int P);
+void gmres_coop(hipStream_t st, int nf, int kmax, int maxit, int P, int R, const double* S, const double* b, double* x,
+                GmresDev* s, double rtol, double atol, double* ws);
 // pressure post-processing (src/operators_pressure.F90): cell arrays are
 // m x n, i fastest; pressure boundary vectors [left n | right n | bottom m |
 // top m]

@@ -18,7 +18,8 @@ module ib_gpu_mod
     integer(c_int32_t) :: gmres_restart, gmres_max_it, fsi_max_count
     integer(c_int32_t) :: deltafnc_f32_literals, num_stat, sub_domain, launch_mode
     integer(c_int32_t) :: schur_mode, face_margin
-    integer(c_int32_t) :: reserved(3)
+    integer(c_int32_t) :: gmres_form
+    integer(c_int32_t) :: reserved(2)
   end type ib_params
 
   type, bind(C) :: ib_subdomain

@@ -96,6 +96,7 @@ program ib_main
   p%launch_mode = 0
   p%schur_mode = 0
   p%face_margin = 0
+  p%gmres_form = 0
   p%reserved = 0
   call getenv_int('IB_GMRES_RESTART', p%gmres_restart)
 

@@ -79,7 +79,11 @@ typedef struct ib_params {
                               0 auto: 3 with torsional bodies, else 2 */
   int32_t face_margin;     /* schur_mode 3: cells of margin around the taps'
                               faces in the face set (0: 2 cells, -1: none) */
-  int32_t reserved[3];
+  int32_t gmres_form;      /* GMRES on a dense Schur matrix: 0 auto (the
+                              cooperative multi-workgroup solve where it
+                              fits, else one workgroup), 1 one workgroup,
+                              2 cooperative where it fits */
+  int32_t reserved[2];
 } ib_params;
 
 /* One body deck input/body.NNN.inp (src/grid.F90:252-288), angles in radians. */

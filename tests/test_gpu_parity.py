@@ -263,3 +263,28 @@ def test_error_paths():
     p.fsi_tol = 1e-30
     info = gpu(p, b).advance(allow_noconv=True)
     assert info.status == abi.IB_E_NOCONV_FSI and info.fsi_iter == 2
+
+
+@pytest.mark.parametrize("case", ("tiny", "plate"))
+def test_cooperative_gmres_equals_single_workgroup_gmres(case):
+    """The dense GMRES over several workgroups (gmres_form 2) against the
+    single-workgroup solve (gmres_form 1): the same PETSc iteration (CGS,
+    Givens, stopping tests) with partial sums in another order, so the
+    same iteration count and solutions equal to rounding."""
+    if case == "tiny":
+        mk = lambda f: decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-12, gmres_form=f, gmres_restart=80)
+    else:
+        mk = lambda f: decks.plate_case(gmres_form=f)
+    out = []
+    for form in (1, 2):
+        g = gpu(*mk(form))
+        S = g.schur_matrix()
+        rng = np.random.default_rng(3)
+        xt = rng.standard_normal(g.nf)
+        rhs = S @ xt
+        x, its = g.gmres(rhs, x0=0.9 * xt)
+        out.append((x, its, np.linalg.norm(S @ x - rhs) / np.linalg.norm(rhs)))
+    (x1, i1, r1), (x2, i2, r2) = out
+    assert i1 == i2
+    assert r1 < 1e-10 and r2 < 1e-10
+    assert rel(x1, x2) < 1e-8

@@ -50,8 +50,14 @@ def test_steps_match_oracle(case):
     for _ in range(3):
         ig, io = g.advance(), o.advance()
         assert (ig.ksp_iter, ig.fsi_iter) == (io.ksp_iter, io.fsi_iter)
-    for k in range(1, g.L + 1):
-        assert rel(g.level(k)["omega"], o.level(k)["omega"]) < 1e-10
+    # per level against the largest vorticity of the run: a coarse level
+    # the flow has not reached after three steps holds only rounding noise
+    # (~1e-18), whose own relative difference means nothing
+    wg = [g.level(k)["omega"] for k in range(1, g.L + 1)]
+    wo = [o.level(k)["omega"] for k in range(1, g.L + 1)]
+    scale = max(np.abs(w).max() for w in wo)
+    for k in range(g.L):
+        assert np.abs(wg[k] - wo[k]).max() <= 1e-10 * scale, k
     bg, bo = g.body(), o.body()
     assert rel(bg["f"], bo["f"]) < 1e-8
     assert rel(bg["theta"], bo["theta"]) < 1e-10
