@@ -46,7 +46,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 KCLASS = {0: "k_dst_x (row pass, time step)", 1: "k_dst_y (column pass, time step)",
           2: "k_dst_x (batched row pass, Schur assembly)", 3: "k_dst_y (batched column pass, Schur assembly)",
-          4: "k_gmres_dense (one GMRES solve in one workgroup)"}
+          4: "k_gmres_coop (one dense GMRES solve over P workgroups of one XCD; k_gmres_dense in one workgroup "
+             "where no split fits)"}
 
 
 def gridpts(sim):
