@@ -719,7 +719,7 @@ struct GmCoopArgs {
   double* x;
   GmresDev* s;
   double rtol, atol;
-  unsigned* sync;  // [0] arrivals, [1] generation, [2] error
+  unsigned* sync;  // [0] error, [8 + c] barriers passed by workgroup c
   double* dots;    // P x (kmax + 2) partial CGS coefficients
   double* norms;   // P x 4 partial squared norms
   double* wfull;   // nf: w (or r) rows of every workgroup
@@ -739,33 +739,32 @@ __device__ __forceinline__ void gc_st(double* p, double v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Grid barrier over the P workgroups: every thread's global writes are made
-// visible at device scope, workgroup thread 0 arrives on the counter and
-// waits for the generation to move (bounded), then invalidates its L1 view.
-__device__ __forceinline__ bool gc_sync(const GmCoopArgs& a, int* s_err) {
-  // every exchanged value is a device-scope atomic store: waiting for their
-  // completion (workgroup-scope release) orders them before the arrival
+// Grid barrier number k (1, 2, ...) over the P workgroups.  Each
+// workgroup's values are device-scope atomic stores; once every wave has
+// waited for its stores (workgroup-scope release) workgroup c publishes
+// flag[c] = k, and wavefront 0 polls the P flags (one lane each) until all
+// have reached k.  Bounded: on a timeout (or another workgroup's) the error
+// word is set and every workgroup leaves.
+__device__ __forceinline__ bool gc_sync(const GmCoopArgs& a, int c, unsigned k, int* s_err) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned* sy = a.sync;
-    const unsigned g0 = __hip_atomic_load(sy + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    const unsigned prev = __hip_atomic_fetch_add(sy, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (prev + 1u == (unsigned)a.P) {
-      __hip_atomic_store(sy, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __hip_atomic_fetch_add(sy + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      long n = 0;
-      while (__hip_atomic_load(sy + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g0) {
-        if (__hip_atomic_load(sy + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++n > kCoopSpin) {
-          __hip_atomic_store(sy + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x < 64) {
+    unsigned* err = a.sync;
+    unsigned* fl = a.sync + 8;
+    const int lane = threadIdx.x;
+    if (lane == 0) __hip_atomic_store(fl + c, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    long n = 0;
+    while (true) {
+      const unsigned v = lane < a.P ? __hip_atomic_load(fl + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : k;
+      if (__all(v >= k)) break;
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++n > kCoopSpin) {
+        if (lane == 0) {
+          __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           *s_err = 1;
-          break;
         }
-        __builtin_amdgcn_s_sleep(1);
+        break;
       }
+      __builtin_amdgcn_s_sleep(1);
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   }
@@ -828,6 +827,7 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
   };
 
   int its = 0, reason = 0, first = 1;
+  unsigned bk = 0;  // grid barriers passed
   double rn0 = 0.0, ttol = 0.0;
   while (true) {
     // ---- cycle start: r = b - S x (x gathered from every workgroup's rows)
@@ -848,7 +848,7 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
       gc_st(a.norms + c * 4 + 1, pr);
       gc_st(a.norms + c * 4 + 2, pb);
     }
-    if (!gc_sync(a, &s_err)) break;
+    if (!gc_sync(a, c, ++bk, &s_err)) break;
     double sr = 0.0, sb = 0.0;
     for (int q = 0; q < a.P; ++q) {
       sr += gc_ld(a.norms + q * 4 + 1);
@@ -882,7 +882,7 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
           if (sub == 0) gc_st(a.dots + (long)c * (kmax + 2) + j, p);
         }
       }
-      if (!gc_sync(a, &s_err)) {
+      if (!gc_sync(a, c, ++bk, &s_err)) {
         ok = false;
         break;
       }
@@ -916,7 +916,7 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
         pn = gm_block_sum2(pn, red, T);
         if (threadIdx.x == 0) gc_st(a.norms + c * 4, pn);
       }
-      if (!gc_sync(a, &s_err)) {
+      if (!gc_sync(a, c, ++bk, &s_err)) {
         ok = false;
         break;
       }
@@ -1018,7 +1018,7 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
       reason = -2;
       break;
     }
-    if (!gc_sync(a, &s_err)) break;  // every workgroup's rows of x before the next residual
+    if (!gc_sync(a, c, ++bk, &s_err)) break;  // every workgroup's rows of x before the next residual
   }
   if (threadIdx.x == 0) {
     if (s_err) {
@@ -1056,8 +1056,10 @@ bool gmres_coop_plan(int nf, int kmax, size_t max_lds, int* P, int* R) {
   return false;
 }
 
+constexpr int kCoopSyncDoubles = 24;  // the error word and 32 workgroup flags (unsigned)
+
 size_t gmres_coop_ws(int nf, int kmax, int P) {
-  return 8 + (size_t)P * (kmax + 2) + (size_t)4 * P + (size_t)nf + (size_t)P * (kmax + 1) * kmax;
+  return kCoopSyncDoubles + (size_t)P * (kmax + 2) + (size_t)4 * P + (size_t)nf + (size_t)P * (kmax + 1) * kmax;
 }
 
 void gmres_coop(hipStream_t st, int nf, int kmax, int maxit, int P, int R, const double* S, const double* b, double* x,
@@ -1068,11 +1070,11 @@ void gmres_coop(hipStream_t st, int nf, int kmax, int maxit, int P, int R, const
   a.nseg = ns; a.per = (nf + ns - 1) / ns; a.bsc = kGmT / (kmax + 1);
   a.S = S; a.b = b; a.x = x; a.s = s; a.rtol = rtol; a.atol = atol;
   a.sync = reinterpret_cast<unsigned*>(ws);
-  a.dots = ws + 8;
+  a.dots = ws + kCoopSyncDoubles;
   a.norms = a.dots + (size_t)P * (kmax + 2);
   a.wfull = a.norms + (size_t)4 * P;
   a.H = a.wfull + nf;
-  IB_HIP(hipMemsetAsync(ws, 0, 8 * sizeof(double), st));
+  IB_HIP(hipMemsetAsync(ws, 0, kCoopSyncDoubles * sizeof(double), st));
   // an ordinary launch: the barrier is the kernel's own (no runtime grid
   // synchronisation resources), 8P workgroups of which P stay, one per CU
   // on a device of 256 CUs; a workgroup that is not resident in time makes
