@@ -194,11 +194,15 @@ def _time_adjoint(sim, n, cx, cy, ckpts, **grads):
     device_sync()
     tf = time.perf_counter() - t0
     ta = []
+    rebuilds = []
     for K in ckpts:
+        g0 = sim.schur_info()[2]
         t0 = time.perf_counter()
         out = sim.adjoint(n, cx, cy, ckpt_every=K, **grads)
         device_sync()
         ta.append(time.perf_counter() - t0)
+        rebuilds.append(sim.schur_info()[2] - g0)
+    _time_adjoint.rebuilds = rebuilds
     return tf, ta, out
 
 
@@ -243,7 +247,10 @@ def run_adjoint(args):
                       "steps": n, "forward_ms_per_step": round(tf / n * 1e3, 3),
                       "adjoint_ckpt20_ms_per_step": round(tc / n * 1e3, 3), "cost_ratio_ckpt20": round(tc / tf, 3),
                       "adjoint_ms_per_step": round(ta / n * 1e3, 3), "cost_ratio": round(ta / tf, 3),
-                      "dJ_dK": out[3][0], "dJ_dgamma0": out[4][0], "dJ_dRe": out[1]}
+                      "dJ_dK": out[3][0], "dJ_dgamma0": out[4][0], "dJ_dRe": out[1],
+                      "face_set_rebuilds": {"ckpt20": _time_adjoint.rebuilds[0], "taped": _time_adjoint.rebuilds[1]},
+                      "note": ("a face-set rebuild (a tap leaving the set of faces G covers) also rebuilds dG/dRe "
+                               "in the next adjoint call, inside its timed region")}
         sim.close()
     return res
 

@@ -377,6 +377,11 @@ struct ib_handle {
   // taps is computed once by batched unit-source columns; every weight
   // rebuild then forms S = W G W^T + B with two small contractions.  G is
   // recomputed (for a new face set) only when a tap leaves the set.
+  // face_grow (set while an adjoint runs, whose reverse sweep returns to
+  // earlier body positions): keep every face of the current set and add the
+  // new taps' neighbourhoods, so the set is rebuilt once rather than at
+  // every change of direction.  S's entries do not depend on the set.
+  bool face_grow = false;
   void build_face_set() {
     // touched faces = unique tap keys of the current weights
     int nr = 0;
@@ -386,6 +391,12 @@ struct ib_handle {
     std::vector<int> uniq(nr);
     if (nr) IB_HIP(hipMemcpy(uniq.data(), csr.uniq, sizeof(int) * nr, hipMemcpyDeviceToHost));
     std::vector<int> fm((size_t)g.Fx + g.Fy, -1);
+    std::vector<int> keys;
+    if (face_grow && nF > 0 && Fkeys.p) {
+      keys.resize(nF);
+      IB_HIP(hipMemcpy(keys.data(), Fkeys.p, sizeof(int) * nF, hipMemcpyDeviceToHost));
+      for (int q = 0; q < nF; ++q) fm[keys[q]] = q;
+    }
     const int r = face_margin;
     for (int key : uniq) {
       const bool xf = key < g.Fx;
@@ -394,12 +405,12 @@ struct ib_handle {
       for (int dj = -r; dj <= r; ++dj)
         for (int di = -r; di <= r; ++di) {
           const int ii = i + di, jj = j + dj;
-          if (ii >= 0 && ii < ni && jj >= 0 && jj < nj) fm[base + (size_t)jj * ni + ii] = 0;
+          if (ii >= 0 && ii < ni && jj >= 0 && jj < nj && fm[base + (size_t)jj * ni + ii] < 0)
+            fm[base + (size_t)jj * ni + ii] = -2;
         }
     }
-    std::vector<int> keys;
     for (size_t q = 0; q < fm.size(); ++q)
-      if (fm[q] == 0) {
+      if (fm[q] == -2) {
         fm[q] = (int)keys.size();
         keys.push_back((int)q);
       }
@@ -518,12 +529,18 @@ struct ib_handle {
   bool dense_solve_ok() const {
     return coopP > 0 || (nf <= kFusedGmresMax && gmres_dense_fits(nf, kmax, max_lds));
   }
-  int dense_solve(const double* M, const double* b, double* x) {
+  void dense_launch(const double* M, const double* b, double* x) {
     if (coopP > 0) gmres_coop(st, nf, kmax, prm.gmres_max_it, coopP, coopR, M, b, x, gst, prm.rtol, prm.atol, coopws);
     else gmres_dense(st, nf, kmax, prm.gmres_max_it, M, b, x, V, H, gst, prm.rtol, prm.atol, max_lds);
+  }
+  int dense_status() {
     read_gmres();
     if (hst->reason == -3) return fail(IB_E_HIP, "GMRES: a grid barrier of the cooperative solve timed out");
     return hst->reason > 0 ? IB_OK : IB_E_NOCONV_GMRES;
+  }
+  int dense_solve(const double* M, const double* b, double* x) {
+    dense_launch(M, b, x);
+    return dense_status();
   }
   int gmres_dev(const double* b, double* x, int* its_out) {
     if (assembled && dense_solve_ok()) {
@@ -531,8 +548,9 @@ struct ib_handle {
       // bytes: S, b read once, x read and written, and one Krylov vector
       // written per iteration (added once the count is known)
       tim[4].begin(st, 8.0 * ((double)nf * nf + 3.0 * nf));
-      const int rc = dense_solve(Sd, b, x);
+      dense_launch(Sd, b, x);
       tim[4].end(st);
+      const int rc = dense_status();
       if (tim[4].on) tim[4].total_bytes += 8.0 * (double)nf * (hst->its + 1);
       *its_out = hst->its;
       return rc;
@@ -1618,6 +1636,11 @@ struct ib_handle {
     const int nseg = (nsteps + K - 1) / K, last0 = (nseg - 1) * K;
     // the reverse sweep moves the bodies: keep the forward end state
     const bool keep_end = nseg > 1 || adj_fsi;
+    struct GrowGuard {
+      bool& f;
+      explicit GrowGuard(bool& x) : f(x) { f = true; }
+      ~GrowGuard() { f = false; }
+    } grow_guard(face_grow);
     // forward: checkpoints at the starts of the segments before the last,
     // the last segment taped
     for (int n = 0; n < nsteps; ++n) {
