@@ -186,13 +186,16 @@ def run_batched(args):
 def _time_adjoint(sim, n, cx, cy, ckpts, **grads):
     """Forward wall time of n plain steps, then the wall time of n adjoint
     steps per segment length in ckpts (None: the whole run taped)."""
-    sim.adjoint(n, cx, cy, ckpt_every=ckpts[0], **grads)  # set-up outside the timed regions
     device_sync()
     t0 = time.perf_counter()
     for _ in range(n):
         sim.advance()
     device_sync()
     tf = time.perf_counter() - t0
+    # set-up outside the timed regions (tables, dA or dG/dRe for the current
+    # face set, tape buffers)
+    sim.adjoint(n, cx, cy, ckpt_every=ckpts[0], **grads)
+    device_sync()
     ta = []
     rebuilds = []
     for K in ckpts:
@@ -250,7 +253,7 @@ def run_adjoint(args):
                       "dJ_dK": out[3][0], "dJ_dgamma0": out[4][0], "dJ_dRe": out[1],
                       "face_set_rebuilds": {"ckpt20": _time_adjoint.rebuilds[0], "taped": _time_adjoint.rebuilds[1]},
                       "note": ("a face-set rebuild (a tap leaving the set of faces G covers) also rebuilds dG/dRe "
-                               "in the next adjoint call, inside its timed region")}
+                               "within the same adjoint call")}
         sim.close()
     return res
 

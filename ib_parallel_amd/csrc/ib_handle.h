@@ -140,7 +140,7 @@ struct ib_handle {
   DBuf<double> Sd, Vb, T0b, T1b, Sb, BCa, BCb;
   // face-space form: face set, its index map, G = Q on the set, T = G W^T
   bool facemode = false;
-  static constexpr int kFaceMarginDefault = 2;
+  static constexpr int kFaceMarginDefault = 4;
   int face_margin = kFaceMarginDefault;
   static constexpr size_t kAutoGreenMax = (size_t)1 << 28;  // schur_mode 0: G up to 2 GiB
   int nF = 0;
