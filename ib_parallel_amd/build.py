@@ -16,7 +16,7 @@ FORTRAN = os.path.join(PKG, "fortran")
 DRIVER = os.path.join(PKG, "bin", "ib")
 
 SOURCES = ["ib_fft.hip", "ib_fluid.hip", "ib_body.hip", "ib_gmres.hip", "ib_pressure.hip", "ib_subdomain.hip",
-           "ib_adjoint.hip", "ib_solver.hip", "ib_batch.hip"]
+           "ib_adjoint.hip", "ib_adjoint_host.hip", "ib_solver.hip", "ib_batch.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIPFLAGS = [
