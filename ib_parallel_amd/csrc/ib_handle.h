@@ -533,10 +533,14 @@ struct ib_handle {
     if (coopP > 0) gmres_coop(st, nf, kmax, prm.gmres_max_it, coopP, coopR, M, b, x, gst, prm.rtol, prm.atol, coopws);
     else gmres_dense(st, nf, kmax, prm.gmres_max_it, M, b, x, V, H, gst, prm.rtol, prm.atol, max_lds);
   }
-  int dense_status() {
-    read_gmres();
+  // the status of a dense solve already copied to hst
+  int dense_check() {
     if (hst->reason == -3) return fail(IB_E_HIP, "GMRES: a grid barrier of the cooperative solve timed out");
     return hst->reason > 0 ? IB_OK : IB_E_NOCONV_GMRES;
+  }
+  int dense_status() {
+    read_gmres();
+    return dense_check();
   }
   int dense_solve(const double* M, const double* b, double* x) {
     dense_launch(M, b, x);
@@ -875,10 +879,50 @@ struct ib_handle {
       if (itime == 1) body_initial_acc(st, ntors, tb, th, dt, delta, f);
       IB_HIP(hipMemcpyAsync(th + 3 * ntors, th.p, sizeof(double) * 3 * ntors, hipMemcpyDeviceToDevice, st));
     }
+    // moving bodies in the face-space form with a dense solve: the GMRES
+    // status, the stencil check of the new weights, the face-set check of the
+    // new taps and |d theta| come back in one host synchronisation per
+    // iteration
+    const bool one_sync = ntors > 0 && !sd_on && facemode && dense_solve_ok();
     while (err_glob >= prm.fsi_tol && fsi_count <= prm.fsi_max_count) {
       ++fsi_count;
       int its = 0;
-      int rc;
+      int rc = IB_OK;
+      if (one_sync) {
+        fsi_rhs(st, g, nf, tors_of, tb, ntors, th, dt, delta, w, csr.key, qx, qy, q0x[0], rhsf);
+        tim[4].begin(st, 8.0 * ((double)nf * nf + 3.0 * nf));
+        dense_launch(Sd, rhsf, f);
+        tim[4].end(st);
+        redist_faces(st, g, csr, w, f, hx, hy);
+        regT(st, g, nf, w, csr.key, hx, hy, 0.0, nullptr, f_rdst);
+        body_dx(st, ntors, tb, th, dt, delta, f, xb, yb, dabs);
+        rebuild_weights();
+        IB_HIP(hipMemsetAsync(face_flag.p, 0, sizeof(int), st));
+        map_taps(st, nf, csr.key, fmap, tapidx, face_flag);
+        IB_HIP(hipMemcpyAsync(hbuf, dabs.p, sizeof(double) * ntors, hipMemcpyDeviceToHost, st));
+        IB_HIP(hipMemcpyAsync(hflag, err_flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        IB_HIP(hipMemcpyAsync(hflag + 1, face_flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        IB_HIP(hipMemcpyAsync(hst, gst, sizeof(GmresDev), hipMemcpyDeviceToHost, st));
+        sync();
+        rc = dense_check();
+        its = hst->its;
+        if (tim[4].on) tim[4].total_bytes += 8.0 * (double)nf * (its + 1);
+        ksp_iter += its;
+        if (rc != IB_OK) status = rc;
+        if (hflag[0]) {
+          err = "body point regularisation stencil leaves the grid";
+          return IB_E_INVAL;
+        }
+        if (hflag[1]) {
+          build_face_set();
+          compute_green();
+          map_taps(st, nf, csr.key, fmap, tapidx, face_flag);
+        }
+        schur_from_green(st, nf, nF, w, tapidx, Gm, Tm, tors_of, tb, th, dt, delta, Sd);
+        err_glob = 0.0;
+        for (int t = 0; t < ntors; ++t) err_glob = std::max(err_glob, hbuf[t]);
+        continue;
+      }
       if (!sd_on) {
         fsi_rhs(st, g, nf, tors_of, tb, ntors, th, dt, delta, w, csr.key, qx, qy, q0x[0], rhsf);
         rc = gmres_dev(rhsf, f, &its);
@@ -1310,7 +1354,7 @@ struct ib_handle {
     IB_HIP(hipMalloc(&gst, sizeof(GmresDev)));
     IB_HIP(hipHostMalloc(&hst, sizeof(GmresDev), hipHostMallocDefault));
     IB_HIP(hipHostMalloc(&hbuf, sizeof(double) * std::max(64, ntors + 1), hipHostMallocDefault));
-    IB_HIP(hipHostMalloc(&hflag, sizeof(int), hipHostMallocDefault));
+    IB_HIP(hipHostMalloc(&hflag, 4 * sizeof(int), hipHostMallocDefault));
 
     csr.alloc(nf * kTaps, g.Fx + g.Fy);
     rebuild_weights();
