@@ -102,6 +102,49 @@ void applybc_adj(hipStream_t st, const Geom& g, const double* vb, double fac, do
   IB_LAUNCH_CHECK();
 }
 
+// the node value apply_bc adds slot `slot` to (0 for slots no node reads)
+__device__ __forceinline__ double applybc_node(const Geom& g, const double* __restrict__ vb, int slot) {
+  if (slot < g.right) { const int j = slot + 1; return (j >= 2 && j <= g.n) ? vb[ND(g, 2, j)] : 0.0; }
+  if (slot < g.bottom) { const int j = slot - g.right + 1; return (j >= 2 && j <= g.n) ? vb[ND(g, g.m, j)] : 0.0; }
+  if (slot < g.top) { const int i = slot - g.bottom + 1; return (i >= 2 && i <= g.m) ? vb[ND(g, i, 2)] : 0.0; }
+  const int i = slot - g.top + 1;
+  return (i >= 2 && i <= g.m) ? vb[ND(g, i, g.n)] : 0.0;
+}
+
+// bcb += apply_bc^T(vb) (the transposed vort2flux's boundary adjoint)
+__global__ void k_applybc_adj_acc(Geom g, const double* __restrict__ vb, double* __restrict__ bcb) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= g.nbc) return;
+  bcb[slot] += applybc_node(g, vb, slot);
+}
+
+void applybc_adj_acc(hipStream_t st, const Geom& g, const double* vb, double* bcb) {
+  hipLaunchKernelGGL(k_applybc_adj_acc, dim3(ceil_div(g.nbc, 256)), dim3(256), 0, st, g, vb, bcb);
+  IB_LAUNCH_CHECK();
+}
+
+// The predictor's boundary terms of one level, backwards: with
+// v = apply_bc^T(rb) (rb the adjoint of r),
+//   bcb = v,  lbb += vfac v,  racc_bc += -vfac/Re v (lastbc + omegabc)
+// (the latter the Re term of vfac, summed once at the end of the sweep).
+__global__ void k_pred_bc_adj(Geom g, const double* __restrict__ rb, double vfac, double invRe,
+                              const double* __restrict__ lastbc, const double* __restrict__ omegabc,
+                              double* __restrict__ bcb, double* __restrict__ lbb, double* __restrict__ racc_bc) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= g.nbc) return;
+  const double v = applybc_node(g, rb, slot);
+  bcb[slot] = v;
+  lbb[slot] += vfac * v;
+  racc_bc[slot] += -vfac * invRe * (v * (lastbc[slot] + omegabc[slot]));
+}
+
+void pred_bc_adj(hipStream_t st, const Geom& g, const double* rb, double vfac, double invRe, const double* lastbc,
+                 const double* omegabc, double* bcb, double* lbb, double* racc_bc) {
+  hipLaunchKernelGGL(k_pred_bc_adj, dim3(ceil_div(g.nbc, 256)), dim3(256), 0, st, g, rb, vfac, invRe, lastbc, omegabc,
+                     bcb, lbb, racc_bc);
+  IB_LAUNCH_CHECK();
+}
+
 // Transpose of coarsify (src/operators_fluid.F90:381-419): wf += Co^T(wc
 // centre), then the coarse centre nodes (overwritten forward) are zeroed.
 // One thread per fine node gathers from the up to 4 coarse centres whose
@@ -745,6 +788,22 @@ __global__ void k_gemv_symup(int n, const double* __restrict__ A, const double* 
     acc += A[(long)c * n + r] * x[j];
   }
   y[i] = acc;
+}
+
+// S(i, j) = A(min(i, j), max(i, j)) (column major): the symmetric matrix
+// whose upper triangle is A's
+__global__ void k_symmetrize_upper(int n, const double* __restrict__ A, double* __restrict__ S) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)n * n) return;
+  const int i = (int)(q % n), j = (int)(q / n);
+  S[q] = A[(long)max(i, j) * n + min(i, j)];
+}
+
+void symmetrize_upper(hipStream_t st, int n, const double* A, double* S) {
+  if (n <= 0) return;
+  const long nn = (long)n * n;
+  hipLaunchKernelGGL(k_symmetrize_upper, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, n, A, S);
+  IB_LAUNCH_CHECK();
 }
 
 void gemv_symup(hipStream_t st, int n, const double* A, const double* x, double* y) {

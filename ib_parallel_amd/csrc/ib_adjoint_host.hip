@@ -53,6 +53,12 @@ void ib_handle::adjoint_setup(int nsteps, int K) {
         const double* bc = column_batch(c0, nc, w, csr.key, &sbc, dlam1i);
         schur_columns(st, g, nf, c0, nc, w, csr.key, Sb, g.P, bc, sbc, no_tors, tb, th, dt, delta, dA);
       }
+      // num_stat solves with A's upper triangle mirrored: store dA/dRe that way
+      // so each step's product is a plain gemv
+      DBuf<double> du;
+      du.alloc((size_t)nf * nf);
+      copy(st, du, dA, (long)nf * nf);
+      symmetrize_upper(st, nf, du, dA);
       sync();
       trim_columns();
     } else {
@@ -69,6 +75,9 @@ void ib_handle::adjoint_setup(int nsteps, int K) {
       fin_th.alloc((size_t)6 * nt + 2 * (size_t)nf);
     }
     aw.alloc(LP); aro.alloc(LP); aws.alloc(LP); asb.alloc(LP); racc.alloc(P);
+    cur_w.alloc(LP); cur_ws.alloc(LP); cur_qx.alloc((size_t)L * g.Fx); cur_qy.alloc((size_t)L * g.Fy);
+    cur_f.alloc(nf); cur_qt.alloc((size_t)nf * kTaps);
+    raccb.alloc(nbc);
     aqx.alloc((size_t)L * g.Fx); aqy.alloc((size_t)L * g.Fy);
     af.alloc(nf); arf.alloc(nf); af2.alloc(nf); cw.alloc(nf);
     aP1.alloc(P); aP2.alloc(P); aD1.alloc(P); aD2.alloc(P); aD3.alloc(P);
@@ -267,8 +276,7 @@ void ib_handle::v2f_adj(double* wb, double* qxb, double* qyb) {
     ctci_dev(asb + k * P, aP2, nullptr);
     axpy(st, P, 1.0, aP2, wb + k * P);
     if (k < L - 1) {
-      applybc_adj(st, g, aP2, 1.0, abct);
-      axpy(st, nbc, 1.0, abct, asbcb);
+      applybc_adj_acc(st, g, aP2, asbcb);
       getbc_adj(st, g, asbcb, 1.0, asb + (k + 1) * P);
     }
   }
@@ -289,94 +297,129 @@ int ib_handle::adjoint_step(int sl, bool first) {
   const double* f0 = t_f + (long)sl * nf;
   const double* qt = (want_pos || adj_fsi) ? t_qt + (long)sl * nf * kTaps : nullptr;
   const double invRe = 1.0 / Re;
+  // with graphs, the slot's tape is copied into fixed buffers first so that
+  // the captured launches never change
+  const bool graphs = step_graphs_ok();
+  if (graphs) {
+    copy(st, cur_w, w0, LP);
+    copy(st, cur_qx, qx0, (long)L * g.Fx);
+    copy(st, cur_qy, qy0, (long)L * g.Fy);
+    copy(st, cur_ws, ws0, LP);
+    copy(st, cur_f, f0, nf);
+    if (qt) copy(st, cur_qt, qt, (long)nf * kTaps);
+    w0 = cur_w; qx0 = cur_qx; qy0 = cur_qy; ws0 = cur_ws; f0 = cur_f;
+    if (qt) qt = cur_qt;
+  }
   if (adj_fsi) body_restore(t_th + (long)sl * (6 * ntors + 2 * nf));
-  // the end-of-step vort2flux
-  v2f_adj(aw, aqx, aqy);
-  // corrector omega_0 -= ainv(rot(E^T f)): the adjoint of f (J depends on f
-  // through cw), and its Re term -<aw_0 - ainv(aw_0), ainv(v)>/Re
-  copy(st, af, cw, nf);
-  ainv_dev(aw, aP1, false);
-  curl_getbc(st, g, aP1, zbc, aFx, aFy, nullptr);
-  if (adj_fsi) gather_taps(st, g, nf * kTaps, csr.key, aFx, aFy, 0.0, htv);
-  regT(st, g, nf, w, csr.key, aFx, aFy, 0.0, nullptr, arf);
-  axpy(st, nf, -1.0, arf, af);
-  reg(st, g, csr, w, f0, hx, hy);
-  ainv_dev(nullptr, aP2, false);
-  re_corr(st, P, aw, aP1, aP2, invRe, racc);
-  if (!adj_fsi) {
-    // f = A^-1 rhsf (A symmetric): the adjoint of rhsf, and the Re term
-    // -<rhsf bar, dA/dRe f>
+
+  // the end-of-step vort2flux; corrector omega_0 -= ainv(rot(E^T f)): the
+  // adjoint of f (J depends on f through cw), and its Re term
+  // -<aw_0 - ainv(aw_0), ainv(v)>/Re
+  auto corrector = [&] {
+    v2f_adj(aw, aqx, aqy);
+    copy(st, af, cw, nf);
+    ainv_dev(aw, aP1, false);
+    curl_getbc(st, g, aP1, zbc, aFx, aFy, nullptr);
+    if (adj_fsi) gather_taps(st, g, nf * kTaps, csr.key, aFx, aFy, 0.0, htv);
+    regT(st, g, nf, w, csr.key, aFx, aFy, 0.0, nullptr, arf);
+    axpy(st, nf, -1.0, arf, af);
+    reg(st, g, csr, w, f0, hx, hy);
+    ainv_dev(nullptr, aP2, false);
+    re_corr(st, P, aw, aP1, aP2, invRe, racc);
+  };
+  // f = A^-1 rhsf (A symmetric): the adjoint of rhsf, the Re term
+  // -<rhsf bar, dA/dRe f>, and the body-point position terms
+  auto stat_solve_adj = [&] {
     gemv(st, nf, Ainv, af, arf);
-    gemv_symup(st, nf, dA, f0, af2);
+    gemv(st, nf, dA, f0, af2);
     dot_acc(st, nf, arf, af2, -1.0, adot);
-    // body-point positions: E in rhsf and in the corrector, and A
     if (want_pos) pos_terms(st, g, nf, csr.key, tapidx, dwx, dwy, TTm, Um, arf, f0, qt, aFx, aFy, gpx, gpy);
-  } else {
-    const int rc = fsi_adjoint(sl, first, f0, qt);
-    if (rc != IB_OK) return rc;
-  }
+  };
   // rhsf = E(q*_0 + q0): the adjoint of q*_0
-  fill(st, aFx, g.Fx, 0.0);
-  fill(st, aFy, g.Fy, 0.0);
-  reg(st, g, csr, w, arf, aFx, aFy);
-  axpy(st, g.Fx, 1.0, aFx, aqx);
-  axpy(st, g.Fy, 1.0, aFy, aqy);
-  if (adj_fsi) {
-    // Re through G: -<W^T lambda1, dG/dRe W^T f>; rigid points: S's weights
-    face_gather(st, g, nF, Fkeys, aFx, aFy, lfv);
-    gemv(st, nF, dGm, xfv, dgv);
-    dot_acc(st, nF, lfv, dgv, -1.0, adot);
-    if (want_pos) {
-      gemv_t(st, nF, Gm, lfv, uTv);
-      pos_fsi(st, nf, tors_of, tapidx, dwx, dwy, arf, f0, htv, qt, vTv, uTv, gpx, gpy);
+  auto qstar_adj = [&] {
+    fill(st, aFx, g.Fx, 0.0);
+    fill(st, aFy, g.Fy, 0.0);
+    reg(st, g, csr, w, arf, aFx, aFy);
+    axpy(st, g.Fx, 1.0, aFx, aqx);
+    axpy(st, g.Fy, 1.0, aFy, aqy);
+  };
+  // the vort2flux after the predictor, then the predictor
+  // omega*_k = D(lam1i o (D r_k + lam1 o D omega_k)), fine to coarse (the
+  // reverse of its coarse-to-fine order)
+  auto predictor = [&] {
+    v2f_adj(aw, aqx, aqy);
+    copy(st, aws, aw, LP);
+    fill(st, aw, LP, 0.0);
+    fill(st, albb, (long)L * nbc, 0.0);
+    for (int k = 0; k < L; ++k) {
+      const double* wk = w0 + k * P;
+      if (k < L - 1) {
+        get_bc(st, g, w0 + (k + 1) * P, P, albk, nbc, 1, 0.25);
+        get_bc(st, g, ws0 + (k + 1) * P, P, aobk, nbc, 1, 0.25);
+      } else {
+        fill(st, albk, nbc, 0.0);
+        fill(st, aobk, nbc, 0.0);
+      }
+      // rbar = D(lam1i o D wsb_k), D(lam1 lam1i o D wsb_k): one x pass, two
+      // column passes, one x pass over both
+      DstLaunch a;
+      a.in[0] = aws + k * P; a.out[0] = aD1;
+      xpass(X_PLAIN, a);
+      DstLaunch b1;
+      b1.in[0] = aD1; b1.out[0] = aD2; b1.tab = lam1i + k * P; b1.div = 1.0;
+      ypass(Y_SCALE2, b1);
+      DstLaunch b2;
+      b2.in[0] = aD1; b2.out[0] = aD3; b2.tab = ll + k * P; b2.div = 1.0;
+      ypass(Y_SCALE2, b2);
+      DstLaunch c;
+      c.batch = 2;
+      c.in[0] = aD2; c.out[0] = aP1;
+      c.in[1] = aD3; c.out[1] = aP2;
+      xpass(X_PLAIN, c);
+      re_pred(st, P, ws0 + k * P, wk, aws + k * P, aP1, aP2, 1.0 / normalize, invRe, racc, aw + k * P);
+      // the boundary terms vfac (apply_bc(lastbc) + apply_bc(omegabc)) and
+      // their Re term
+      pred_bc_adj(st, g, aP1, vfac[k], invRe, albk, aobk, asbcb, albb + k * nbc, raccb);
+      if (k < L - 1) getbc_adj(st, g, asbcb, 0.25 * vfac[k], aws + (k + 1) * P);
+      // r = con1 rhs + con2 rold + ..., rhs_old <- rhs
+      rhs_adj(st, P, aP1, first ? con1[k] + con2[k] : con1[k], first ? 0.0 : con2[k], aro + k * P, aD2);
+      nl_adj(st, g, wk, qx0 + (long)k * g.Fx, qy0 + (long)k * g.Fy, q0x[k], albk, aD2, anlx, anly, aub, avb,
+             aw + k * P, albb + k * nbc, aqx + (long)k * g.Fx, aqy + (long)k * g.Fy);
     }
+    // lastbc_k = 0.25 get_bc(omega_(k+1))
+    for (int k = 0; k < L - 1; ++k) getbc_adj(st, g, albb + k * nbc, 0.25, aw + (k + 1) * P);
+  };
+
+  const int fk = first ? 1 : 0;
+  if (!adj_fsi) {
+    auto all = [&] {
+      corrector();
+      stat_solve_adj();
+      qstar_adj();
+      predictor();
+    };
+    if (graphs) replay(100 + 2 * fk + (want_pos ? 1 : 0), all);
+    else all();
+    return IB_OK;
   }
-  // the vort2flux after the predictor
-  v2f_adj(aw, aqx, aqy);
-  copy(st, aws, aw, LP);
-  fill(st, aw, LP, 0.0);
-  fill(st, albb, (long)L * nbc, 0.0);
-  // predictor omega*_k = D(lam1i o (D r_k + lam1 o D omega_k)), fine to
-  // coarse (the reverse of its coarse-to-fine order)
-  for (int k = 0; k < L; ++k) {
-    const double* wk = w0 + k * P;
-    if (k < L - 1) {
-      get_bc(st, g, w0 + (k + 1) * P, P, albk, nbc, 1, 0.25);
-      get_bc(st, g, ws0 + (k + 1) * P, P, aobk, nbc, 1, 0.25);
-    } else {
-      fill(st, albk, nbc, 0.0);
-      fill(st, aobk, nbc, 0.0);
-    }
-    // rbar = D(lam1i o D wsb_k), D(lam1 lam1i o D wsb_k): one x pass, two
-    // column passes, one x pass over both
-    DstLaunch a;
-    a.in[0] = aws + k * P; a.out[0] = aD1;
-    xpass(X_PLAIN, a);
-    DstLaunch b1;
-    b1.in[0] = aD1; b1.out[0] = aD2; b1.tab = lam1i + k * P; b1.div = 1.0;
-    ypass(Y_SCALE2, b1);
-    DstLaunch b2;
-    b2.in[0] = aD1; b2.out[0] = aD3; b2.tab = ll + k * P; b2.div = 1.0;
-    ypass(Y_SCALE2, b2);
-    DstLaunch c;
-    c.batch = 2;
-    c.in[0] = aD2; c.out[0] = aP1;
-    c.in[1] = aD3; c.out[1] = aP2;
-    xpass(X_PLAIN, c);
-    re_pred(st, P, ws0 + k * P, wk, aws + k * P, aP1, aP2, 1.0 / normalize, invRe, racc, aw + k * P);
-    // the boundary terms vfac (apply_bc(lastbc) + apply_bc(omegabc))
-    applybc_adj(st, g, aP1, 1.0, asbcb);
-    dot_acc(st, nbc, asbcb, albk, -vfac[k] * invRe, adot);
-    dot_acc(st, nbc, asbcb, aobk, -vfac[k] * invRe, adot);
-    axpy(st, nbc, vfac[k], asbcb, albb + k * nbc);
-    if (k < L - 1) getbc_adj(st, g, asbcb, 0.25 * vfac[k], aws + (k + 1) * P);
-    // r = con1 rhs + con2 rold + ..., rhs_old <- rhs
-    rhs_adj(st, P, aP1, first ? con1[k] + con2[k] : con1[k], first ? 0.0 : con2[k], aro + k * P, aD2);
-    nl_adj(st, g, wk, qx0 + (long)k * g.Fx, qy0 + (long)k * g.Fy, q0x[k], albk, aD2, anlx, anly, aub, avb,
-           aw + k * P, albb + k * nbc, aqx + (long)k * g.Fx, aqy + (long)k * g.Fy);
+  // moving bodies: the FSI part reads scalars back to the host and its
+  // face-set products change size with the set, so it runs between graphs
+  if (graphs) replay(110, corrector);
+  else corrector();
+  const int rc = fsi_adjoint(sl, first, f0, qt);
+  if (rc != IB_OK) return rc;
+  if (graphs) replay(111, qstar_adj);
+  else qstar_adj();
+  // Re through G: -<W^T lambda1, dG/dRe W^T f>; rigid points: S's weights
+  face_gather(st, g, nF, Fkeys, aFx, aFy, lfv);
+  gemv(st, nF, dGm, xfv, dgv);
+  dot_acc(st, nF, lfv, dgv, -1.0, adot);
+  if (want_pos) {
+    gemv_t(st, nF, Gm, lfv, uTv);
+    pos_fsi(st, nf, tors_of, tapidx, dwx, dwy, arf, f0, htv, qt, vTv, uTv, gpx, gpy);
   }
-  // lastbc_k = 0.25 get_bc(omega_(k+1))
-  for (int k = 0; k < L - 1; ++k) getbc_adj(st, g, albb + k * nbc, 0.25, aw + (k + 1) * P);
+  if (graphs) replay(112 + fk, predictor);
+  else predictor();
   return IB_OK;
 }
 
@@ -543,6 +586,7 @@ int ib_handle::adjoint_run(int nsteps, int Kin, const double* cx, const double* 
   fill(st, adJ, 1, 0.0);
   fill(st, adot, 1, 0.0);
   fill(st, racc, g.P, 0.0);
+  fill(st, raccb, g.nbc, 0.0);
   const int it0 = itime;
   const int nseg = (nsteps + K - 1) / K, last0 = (nseg - 1) * K;
   // the reverse sweep moves the bodies: keep the forward end state
@@ -591,6 +635,7 @@ int ib_handle::adjoint_run(int nsteps, int Kin, const double* cx, const double* 
   // the initial q is vort2flux of the initial omega
   if (dw0 && status == IB_OK) v2f_adj(aw, aqx, aqy);
   sum_acc(st, g.P, racc, 1.0, adot);
+  sum_acc(st, g.nbc, raccb, 1.0, adot);
   if (keep_end) {
     // leave the handle in the forward end state
     load_state(fin_w, fin_ro, fin_qx, fin_qy);

@@ -171,11 +171,12 @@ struct ib_handle {
   // state and scratch
   bool adj_ready = false;
   DBuf<double> dlam1i, ll, dA, cw;  // d lam1i/dRe, lam1 lam1i, dA/dRe, objective weights
-  DBuf<double> aw, aro, aqx, aqy, aws, af, arf, af2, racc;
+  DBuf<double> aw, aro, aqx, aqy, aws, af, arf, af2, racc, raccb;
   DBuf<double> aP1, aP2, aD1, aD2, aD3, asb, aFx, aFy, anlx, anly, aub, avb;
   DBuf<double> albb, albk, aobk, asbcb, abct, adot, adJ;
   DBuf<double> c_w, c_ro, c_qx, c_qy, t_w, t_qx, t_qy, t_ws, t_f, fin_w, fin_ro, fin_qx, fin_qy;
   DBuf<double> fin_s, fin_f, fin_fr;
+  DBuf<double> cur_w, cur_ws, cur_qx, cur_qy, cur_f, cur_qt;  // the tape slot being reversed (graph replays)
   double* tape_ws = nullptr;  // set: advance copies the predicted omega (all levels) here
   double* tape_qt = nullptr;  // set: advance gathers q* + q0 at the taps here
   // body-point positions (rigid): d w/d xb, d w/d yb, G W^T and W G on the
@@ -1066,10 +1067,10 @@ struct ib_handle {
   }
 
   // ---------------------------------------------------------------- advance
-  int advance() {
-    itime += 1;
+  // step 1: the predictor on every level, coarse to fine (vort2flux follows)
+  void predict() {
     const long P = g.P, nbc = g.nbc;
-    // step 1: lastbc(k) = 0.25 get_bc(omega_{k+1}^n) (src/timestepper.F90:103-117)
+    // lastbc(k) = 0.25 get_bc(omega_{k+1}^n) (src/timestepper.F90:103-117)
     if (L > 1) get_bc(st, g, omega + P, P, lastbc, nbc, L - 1, 0.25);
     fill(st, lastbc + (L - 1) * nbc, nbc, 0.0);
     for (int k = L - 1; k >= 0; --k) {
@@ -1092,18 +1093,100 @@ struct ib_handle {
       c.in[0] = t2; c.out[0] = omega + k * P;
       xpass(X_PLAIN, c);
     }
-    if (tape_ws) copy(st, tape_ws, omega, (long)L * P);
+  }
+
+  // step 2 for a numerically stationary body (src/timestepper.F90:457-484):
+  // rhsf = regT(q + q0), f = A^-1 rhsf, redistribute; nothing moves
+  void stat_solve() {
+    fsi_rhs(st, g, nf, no_tors, tb, 0, th, dt, delta, w, csr.key, qx, qy, q0x[0], rhsf);
+    gemv(st, nf, Ainv, rhsf, f);
+    redist_faces(st, g, csr, w, f, hx, hy);
+    regT(st, g, nf, w, csr.key, hx, hy, 0.0, nullptr, f_rdst);
+  }
+
+  // step 3: corrector (src/timestepper.F90:488-553), then vort2flux; on the
+  // sub-domain the source is E_sd^T P^T f (:500-510)
+  void correct() {
+    if (sd_on) {
+      fill(st, hx, g.Fx, 0.0);
+      fill(st, hy, g.Fy, 0.0);
+      fill(st, h0sd, nf_sd, 0.0);
+      sd_ptf(st, sd_used, d_used, d_uptr, d_uj, d_uw, f, h0sd);
+      reg(st, g, csr_sd, w_sd, h0sd, hx, hy);
+      ainv_dev(nullptr, omega, true);
+      // leave hx, hy zero outside the body's faces, as the other forms expect
+      fill(st, hx, g.Fx, 0.0);
+      fill(st, hy, g.Fy, 0.0);
+    } else {
+      reg(st, g, csr, w, f, hx, hy);
+      ainv_dev(nullptr, omega, true);
+    }
     vort2flux_dev(omega, s, qx, qy);
+  }
+
+  // The parts of a step whose launches never change for the handle (every
+  // pointer and scalar fixed; the Euler/AB2 switch gives two versions) are
+  // captured once as HIP graphs and replayed: the predictor, the vort2flux
+  // after it, the corrector with its vort2flux, and on the num_stat path
+  // without an adjoint tape the whole step.  Plain launches with
+  // launch_mode 1 and while kernel timing records per-launch events; the
+  // adjoint's tape copies go between the graphs.
+  std::map<int, hipGraphExec_t> step_graphs;
+  bool step_graphs_ok() const {
+    bool timing = false;
+    for (const auto& t : tim) timing = timing || t.on;
+    return prm.launch_mode == 0 && !timing && !sd_on;
+  }
+  template <class F>
+  void replay(int key, F&& body) {
+    auto itr = step_graphs.find(key);
+    if (itr == step_graphs.end()) {
+      hipGraph_t gr = nullptr;
+      hipGraphExec_t ge = nullptr;
+      IB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        body();
+      } catch (...) {
+        (void)hipStreamEndCapture(st, &gr);
+        if (gr) (void)hipGraphDestroy(gr);
+        throw;
+      }
+      IB_HIP(hipStreamEndCapture(st, &gr));
+      IB_HIP(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
+      IB_HIP(hipGraphDestroy(gr));
+      itr = step_graphs.emplace(key, ge).first;
+    }
+    IB_HIP(hipGraphLaunch(itr->second, st));
+  }
+
+  int advance() {
+    itime += 1;
+    const bool graphs = step_graphs_ok();
+    const int first = itime == 1 ? 1 : 0;
+    const bool taping = tape_ws || tape_qt || tape_th;
+    if (graphs && prm.num_stat && !taping) {
+      replay(4 + first, [&] {
+        predict();
+        vort2flux_dev(omega, s, qx, qy);
+        stat_solve();
+        correct();
+      });
+      ksp_iter = 0;
+      fsi_iter = 0;
+      err_fsi = 0.0;
+      sync();
+      return IB_OK;
+    }
+    if (graphs) replay(first, [&] { predict(); });
+    else predict();
+    if (tape_ws) copy(st, tape_ws, omega, (long)L * g.P);
+    if (graphs) replay(3, [&] { vort2flux_dev(omega, s, qx, qy); });
+    else vort2flux_dev(omega, s, qx, qy);
 
     int status = IB_OK;
     if (prm.num_stat) {
-      // step 2 for a numerically stationary body (src/timestepper.F90:457-484):
-      // rhsf = regT(q + q0), f = A^-1 rhsf, redistribute; nothing moves and
-      // no iteration counts are kept
-      fsi_rhs(st, g, nf, no_tors, tb, 0, th, dt, delta, w, csr.key, qx, qy, q0x[0], rhsf);
-      gemv(st, nf, Ainv, rhsf, f);
-      redist_faces(st, g, csr, w, f, hx, hy);
-      regT(st, g, nf, w, csr.key, hx, hy, 0.0, nullptr, f_rdst);
+      // no iteration counts are kept on this path
+      stat_solve();
       ksp_iter = 0;
       fsi_iter = 0;
       err_fsi = 0.0;
@@ -1120,23 +1203,8 @@ struct ib_handle {
       copy(st, tape_th + 6 * ntors + nf, yb, nf);
     }
 
-    // step 3: corrector (src/timestepper.F90:488-553); on the sub-domain the
-    // source is E_sd^T P^T f (:500-510)
-    if (sd_on) {
-      fill(st, hx, g.Fx, 0.0);
-      fill(st, hy, g.Fy, 0.0);
-      fill(st, h0sd, nf_sd, 0.0);
-      sd_ptf(st, sd_used, d_used, d_uptr, d_uj, d_uw, f, h0sd);
-      reg(st, g, csr_sd, w_sd, h0sd, hx, hy);
-      ainv_dev(nullptr, omega, true);
-      // leave hx, hy zero outside the body's faces, as the other forms expect
-      fill(st, hx, g.Fx, 0.0);
-      fill(st, hy, g.Fy, 0.0);
-    } else {
-      reg(st, g, csr, w, f, hx, hy);
-      ainv_dev(nullptr, omega, true);
-    }
-    vort2flux_dev(omega, s, qx, qy);
+    if (graphs) replay(2, [&] { correct(); });
+    else correct();
     sync();
     return status;
   }
@@ -1421,6 +1489,7 @@ struct ib_handle {
   ~ib_handle() {
     if (st) (void)hipStreamSynchronize(st);
     for (auto& kv : gm_graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto& kv : step_graphs) (void)hipGraphExecDestroy(kv.second);
     for (auto& t : tim) t.destroy();
     csr.release();
     px.release();

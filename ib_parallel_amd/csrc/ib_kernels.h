@@ -309,6 +309,9 @@ void sd_ptf(hipStream_t st, int nused, const int* used, const int* uptr, const i
 void curl_adj_bc(hipStream_t st, const Geom& g, const double* qx, const double* qy, double* bcb);
 void getbc_adj(hipStream_t st, const Geom& g, const double* bcb, double fac, double* sc);
 void applybc_adj(hipStream_t st, const Geom& g, const double* vb, double fac, double* bcb);
+void applybc_adj_acc(hipStream_t st, const Geom& g, const double* vb, double* bcb);
+void pred_bc_adj(hipStream_t st, const Geom& g, const double* rb, double vfac, double invRe, const double* lastbc,
+                 const double* omegabc, double* bcb, double* lbb, double* racc_bc);
 void coarsify_adj(hipStream_t st, const Geom& g, double* wc, double* wf);
 void nl_adj(hipStream_t st, const Geom& g, const double* w, const double* qx, const double* qy, double q0x,
             const double* lastbc, const double* rb, double* nlxb, double* nlyb, double* ub, double* vb, double* wb,
@@ -351,6 +354,7 @@ void axpy(hipStream_t st, long n, double a, const double* x, double* y);
 void mul(hipStream_t st, long n, const double* x, const double* t, double* y);
 void dot_acc(hipStream_t st, long n, const double* a, const double* b, double scale, double* out);
 void gemv_symup(hipStream_t st, int n, const double* A, const double* x, double* y);
+void symmetrize_upper(hipStream_t st, int n, const double* A, double* S);
 
 // ---------------------------------------------------- torsional bodies ---
 // y[body entries] += b_times2(x); one workgroup per body
