@@ -1,7 +1,7 @@
 """The discrete adjoint of the GPU time step as a torch.autograd.Function.
 
 ``ForceObjective.apply(re, omega0, params, bodies, nsteps, cx, cy)`` runs
-``nsteps`` steps of rigid bodies with num_stat = 1 from the vorticity
+``nsteps`` steps of the bodies and Schur form ``params``/``bodies`` give from the vorticity
 ``omega0`` (mgridlev blocks of (m-1)(n-1); the fluxes are vort2flux of it)
 at Reynolds number ``re`` and returns the time-mean weighted force
 

@@ -44,21 +44,29 @@ void ib_handle::adjoint_setup(int nsteps, int K) {
     IB_HIP(hipMemcpy(dlam1i.p, h1i.data(), sizeof(double) * LP, hipMemcpyHostToDevice));
     IB_HIP(hipMemcpy(ll.p, hll.data(), sizeof(double) * LP, hipMemcpyHostToDevice));
     if (!adj_fsi) {
-      // dA/dRe: the num_stat columns with d lam1i / dRe in the ainv stage
+      // dA/dRe: the Schur columns with d lam1i / dRe in the ainv stage (no
+      // body block: every body is rigid)
       dA.alloc((size_t)nf * nf);
       alloc_assembly(nf);
       for (int c0 = 0; c0 < nf; c0 += ncol) {
         const int nc = std::min(ncol, nf - c0);
         long sbc = 0;
         const double* bc = column_batch(c0, nc, w, csr.key, &sbc, dlam1i);
-        schur_columns(st, g, nf, c0, nc, w, csr.key, Sb, g.P, bc, sbc, no_tors, tb, th, dt, delta, dA);
+        schur_columns(st, g, nf, c0, nc, w, csr.key, Sb, g.P, bc, sbc, adj_gm ? tors_of.p : no_tors.p, tb, th, dt,
+                      delta, dA);
       }
-      // num_stat solves with A's upper triangle mirrored: store dA/dRe that way
-      // so each step's product is a plain gemv
-      DBuf<double> du;
-      du.alloc((size_t)nf * nf);
-      copy(st, du, dA, (long)nf * nf);
-      symmetrize_upper(st, nf, du, dA);
+      if (adj_gm) {
+        // GMRES on the fixed assembled S: its transpose once
+        SdT.alloc((size_t)nf * nf);
+        transpose(st, nf, Sd, SdT);
+      } else {
+        // num_stat solves with A's upper triangle mirrored: store dA/dRe that
+        // way so each step's product is a plain gemv
+        DBuf<double> du;
+        du.alloc((size_t)nf * nf);
+        copy(st, du, dA, (long)nf * nf);
+        symmetrize_upper(st, nf, du, dA);
+      }
       sync();
       trim_columns();
     } else {
@@ -391,7 +399,7 @@ int ib_handle::adjoint_step(int sl, bool first) {
   };
 
   const int fk = first ? 1 : 0;
-  if (!adj_fsi) {
+  if (!adj_fsi && !adj_gm) {
     auto all = [&] {
       corrector();
       stat_solve_adj();
@@ -400,6 +408,26 @@ int ib_handle::adjoint_step(int sl, bool first) {
     };
     if (graphs) replay(100 + 2 * fk + (want_pos ? 1 : 0), all);
     else all();
+    return IB_OK;
+  }
+  if (adj_gm) {
+    // rigid bodies, GMRES on the fixed assembled S: rhsf bar = S^-T f bar
+    // (a GMRES solve, read back between graphs) and the Re term
+    // -<rhsf bar, dS/dRe f>
+    if (graphs) replay(120, corrector);
+    else corrector();
+    const int rc = solve_T(af, arf);
+    if (rc != IB_OK) return fail(rc, "adjoint: GMRES on S^T did not converge");
+    gemv(st, nf, dA, f0, af2);
+    dot_acc(st, nf, arf, af2, -1.0, adot);
+    if (graphs) replay(122 + fk, [&] {
+      qstar_adj();
+      predictor();
+    });
+    else {
+      qstar_adj();
+      predictor();
+    }
     return IB_OK;
   }
   // moving bodies: the FSI part reads scalars back to the host and its
@@ -559,13 +587,17 @@ int ib_handle::adjoint_run(int nsteps, int Kin, const double* cx, const double* 
                 double* dxb, double* dyb, double* dK, double* dg0) {
   const bool stat_ok = prm.num_stat && ntors == 0 && !sd_on && nf > 0;
   const bool fsi_ok = !prm.num_stat && facemode && !sd_on && nf > 0;
-  if (!stat_ok && !fsi_ok)
+  const bool gm_ok = !prm.num_stat && assembled && !facemode && ntors == 0 && !sd_on && nf > 0;
+  if (!stat_ok && !fsi_ok && !gm_ok)
     return fail(IB_E_UNSUPPORTED,
-                "the adjoint covers rigid bodies with num_stat = 1 and the face-space form (schur_mode 3, or 0 "
-                "with torsional bodies)");
+                "the adjoint covers rigid bodies with num_stat = 1 or an assembled Schur matrix, and the "
+                "face-space form (schur_mode 3, or 0 with torsional bodies)");
+  if (gm_ok && (dxb || dyb))
+    return fail(IB_E_UNSUPPORTED, "position gradients need num_stat = 1 or the face-space form");
   if (nsteps < 1) return fail(IB_E_INVAL, "nsteps must be >= 1");
-  if (adj_ready && adj_fsi != fsi_ok) return fail(IB_E_INVAL, "adjoint set-up of another form");
+  if (adj_ready && (adj_fsi != fsi_ok || adj_gm != gm_ok)) return fail(IB_E_INVAL, "adjoint set-up of another form");
   adj_fsi = fsi_ok;
+  adj_gm = gm_ok;
   const int K = adjoint_segment(nsteps, Kin);
   want_pos = dxb || dyb;
   if (want_pos && !adj_fsi) adjoint_pos_setup();

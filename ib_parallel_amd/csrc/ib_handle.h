@@ -187,6 +187,7 @@ struct ib_handle {
   // vectors, the per-body columns r1 and Z, the body-state tape and
   // checkpoints, and the body adjoints (host; a few scalars per body)
   bool adj_fsi = false;
+  bool adj_gm = false;  // rigid bodies, GMRES on the fixed assembled S
   double* tape_th = nullptr;  // set: advance copies th, xb, yb after the FSI loop here (and +6nt, +nf)
   DBuf<double> SdT, dGm, xfv, vTv, yfv, zfv, lfv, dgv, uTv, r1v, Zv, aav, zcv, htv, dthv, cvv, hx2, hy2, fmask, scal;
   DBuf<double> t_th, t_f0, c_th, c_f, fin_th;

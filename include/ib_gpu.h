@@ -272,9 +272,12 @@ int ib_batch_adjoint(ib_batch* b, int32_t nsteps, int32_t ckpt_every, const doub
  * taped in HBM in segments of ckpt_every steps: the last segment during the
  * forward pass, the others recomputed once from checkpoints; ckpt_every <= 0
  * tapes the whole run when that fits in half of the free HBM, else uses
- * segments of ceil(sqrt(nsteps)).  Rigid bodies with num_stat = 1 only (else
- * IB_E_UNSUPPORTED).  h is left in the state
- * after the nsteps steps, as ib_advance would leave it. --- */
+ * segments of ceil(sqrt(nsteps)).  Covered: rigid bodies with num_stat = 1;
+ * rigid bodies on GMRES with the assembled Schur matrix (no position
+ * gradients); the face-space form (schur_mode = 3, the default with
+ * torsional bodies).  Matrix-free GMRES and sub-domains return
+ * IB_E_UNSUPPORTED.  h is left in the state after the nsteps steps, as
+ * ib_advance would leave it. --- */
 int ib_adjoint_run(ib_handle* h, int32_t nsteps, int32_t ckpt_every, const double* cx, const double* cy, double* J,
                    double* dJ_dRe, double* dJ_domega0, double* dJ_dxb, double* dJ_dyb, double* dJ_dK,
                    double* dJ_dgamma0);

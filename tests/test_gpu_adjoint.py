@@ -146,14 +146,19 @@ def test_torch_function_gives_the_adjoint_gradients(warm):
 
 
 def test_unsupported_configurations_are_rejected():
-    """GMRES on an assembled (not face-space) or matrix-free operator has no
-    G to differentiate the weights with; num_stat treats torsional bodies as
-    rigid and is rejected with them."""
+    """The matrix-free GMRES operator has no matrix to transpose; positions
+    on the assembled form need G; num_stat treats torsional bodies as rigid
+    and is rejected with them."""
     from ib_parallel_amd import GpuSim, IBError
-    p, b = decks.cylinder_case(m=64, n=64, mgridlev=2, dt=0.01, npts=50, tol=1e-10, num_stat=0, schur_mode=2)
+    p, b = decks.cylinder_case(m=64, n=64, mgridlev=2, dt=0.01, npts=50, tol=1e-10, num_stat=0, schur_mode=1)
     s = GpuSim(p, b)
     with pytest.raises(IBError) as e:
         s.adjoint(2, CX, CY)
+    assert e.value.code == abi.IB_E_UNSUPPORTED
+    p, b = decks.cylinder_case(m=64, n=64, mgridlev=2, dt=0.01, npts=50, tol=1e-10, num_stat=0, schur_mode=2)
+    s = GpuSim(p, b)
+    with pytest.raises(IBError) as e:
+        s.adjoint(2, CX, CY, position_grad=True)
     assert e.value.code == abi.IB_E_UNSUPPORTED
     pt, bt = decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-12, num_stat=1)
     st = GpuSim(pt, bt)
@@ -330,3 +335,34 @@ def test_batch_adjoint_equals_each_simulations_adjoint():
         assert np.array_equal(dk[2 * i:2 * i + 2], out[3]) and np.array_equal(dg[2 * i:2 * i + 2], out[4]), i
     assert all(s.itime.value == 4 for s in batch.sims)
     batch.close()
+
+
+def test_assembled_gmres_form_matches_finite_differences():
+    """Rigid bodies on the GMRES path with the assembled Schur matrix (the
+    default form without num_stat): the adjoint solves S^T by GMRES."""
+    def mk(re=100.0, istart=0):
+        p, b = decks.cylinder_case(m=64, n=64, mgridlev=3, dt=0.01, npts=50, tol=1e-13, num_stat=0, re=re,
+                                   schur_mode=2, gmres_restart=120)
+        p.istart = istart
+        return p, b
+    from ib_parallel_amd import GpuSim
+    p, b = mk()
+    s = GpuSim(p, b)
+    assert s.schur_info()[0] == 2
+    for _ in range(10):
+        s.advance()
+    w0, ro0 = state_of(s)
+    s.close()
+    n = 4
+    J, dre, dw = sim_from(*mk(istart=10), w0, ro0).adjoint(n, CX, CY, ckpt_every=3)
+    Jf, _ = J_forward(*mk(istart=10), w0, ro0, n)
+    assert abs(J - Jf) <= 1e-13 * abs(Jf)
+    h = 0.01
+    fd = (J_forward(*mk(100.0 + h, 10), w0, ro0, n)[0] - J_forward(*mk(100.0 - h, 10), w0, ro0, n)[0]) / (2 * h)
+    print(f"assembled: dJ/dRe adjoint {dre:.12e} fd {fd:.12e}")
+    assert abs(dre - fd) <= 1e-5 * abs(fd)
+    d = np.random.default_rng(4).standard_normal(w0.size)
+    eps = 1e-4 * np.abs(w0).max() / np.abs(d).max()
+    fd = (J_forward(*mk(istart=10), w0 + eps * d, ro0, n)[0] - J_forward(*mk(istart=10), w0 - eps * d, ro0, n)[0]) / (2 * eps)
+    print(f"assembled: <dJ/dw0, d> adjoint {np.dot(dw, d):.12e} fd {fd:.12e}")
+    assert abs(np.dot(dw, d) - fd) <= 1e-5 * abs(fd)
