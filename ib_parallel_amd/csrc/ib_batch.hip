@@ -282,6 +282,7 @@ struct ib_batch {
       }
       ib_handle* h = new ib_handle();
       sims.push_back(h);
+      h->no_capture = true;  // advance and adjoint may run the simulations from several host threads
       const int rc = h->init(&params[i], nbodies[i], bodies + b0);
       if (rc != IB_OK) {
         err = "simulation " + std::to_string(i) + ": " + h->err;

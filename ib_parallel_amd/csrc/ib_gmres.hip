@@ -693,8 +693,8 @@ void gmres_dense(hipStream_t st, int nf, int kmax, int maxit, const double* S, c
 
 // ------------------------------------------------------------------------
 // The same dense solve spread over P workgroups of one XCD.  Workgroup c
-// owns rows [c R, c R + R) of S (in registers: at most kCoopMaxC columns of
-// one row per thread) and those rows of every Krylov vector (in LDS), so
+// owns rows [c R, c R + R) of S (in registers: four rows by kCoopColQ
+// columns per thread) and those rows of every Krylov vector (in LDS), so
 // neither S nor the basis is re-read from L2.  Per Arnoldi step the
 // workgroups exchange, through L2 and two grid barriers, the partial CGS
 // coefficients of their rows, then the partial squared norm and their rows
@@ -707,12 +707,16 @@ void gmres_dense(hipStream_t st, int nf, int kmax, int maxit, const double* S, c
 // barrier waits are bounded: on a timeout every workgroup leaves with
 // reason -3.
 // ------------------------------------------------------------------------
-constexpr int kCoopMaxC = 12;
+// Thread layout of the matrix-vector product and of w -= V h: wavefront q
+// of the 16 owns local rows 4q .. 4q+3 (R <= 64), lane l the columns
+// l, l + 64, ... (nf <= 64 kCoopColQ); each lane's four row partials are
+// summed over the wavefront by gc_reduce4.
+constexpr int kCoopColQ = 4;
+constexpr int kCoopRowsPer16 = 4;  // local rows per 16-lane group in the CGS dots: R <= 64
 constexpr long kCoopSpin = 1L << 22;
 
 struct GmCoopArgs {
   int nf, kmax, maxit, P, R;
-  int nseg, per;  // matvec layout: nseg column segments of `per` columns per local row
   int bsc;        // Hessenberg columns per back-substitution chunk
   const double* S;
   const double* b;
@@ -738,6 +742,47 @@ __device__ __forceinline__ double gc_ld(const double* p) {
 __device__ __forceinline__ void gc_st(double* p, double v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// The sum over q = 0..P-1 (in that order) of p[q * stride], values other
+// workgroups wrote before the last grid barrier: the loads go out in
+// batches of 8 (clamped indices) before any of them is used, so a batch
+// costs one memory round trip instead of one per value.
+__device__ __forceinline__ double gc_sum(const double* p, long stride, int P) {
+  double h = 0.0;
+  for (int q0 = 0; q0 < P; q0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = gc_ld(p + (long)min(q0 + q, P - 1) * stride);
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (q0 + q < P) h += v[q];
+  }
+  return h;
+}
+
+// Every lane of a wavefront holds partial sums v0..v3 of four rows; returns
+// in lane l the sum over the 64 lanes of row ((l & 1) << 1) | ((l >> 1) & 1)
+// (lanes 0, 1, 2, 3: rows 0, 2, 1, 3).  Pairs exchange half their rows
+// (xor 1), then quads (xor 2), then the quads of a row of 16 are added
+// (row_ror 4, 8) and the four rows of 16 (xor 16, 32).  Fixed order; the
+// whole wavefront must be active.
+__device__ __forceinline__ double gc_reduce4(double v0, double v1, double v2, double v3) {
+  const int l = gm_opaque(threadIdx.x) & 63;
+  const bool odd = l & 1, hi = l & 2;
+  double k0 = odd ? v2 : v0, k1 = odd ? v3 : v1;
+  const double s0 = odd ? v0 : v2, s1 = odd ? v1 : v3;
+  k0 += gm_dpp<0xB1>(s0);
+  k1 += gm_dpp<0xB1>(s1);
+  double k = hi ? k1 : k0;
+  k += gm_dpp<0x4E>(hi ? k0 : k1);
+  k += gm_dpp<0x124>(k);
+  k += gm_dpp<0x128>(k);
+  k += __shfl_xor(k, 16);
+  k += __shfl_xor(k, 32);
+  return k;
+}
+// the local row whose sum gc_reduce4 leaves in lane l of wavefront q
+__device__ __forceinline__ int gc_reduce4_row(int q, int l) { return 4 * q + (((l & 1) << 1) | ((l >> 1) & 1)); }
 
 // Grid barrier number k (1, 2, ...) over the P workgroups.  Each
 // workgroup's values are device-scope atomic stores; once every wave has
@@ -794,34 +839,35 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
   __shared__ int s_err;
   if (threadIdx.x == 0) s_err = 0;
   double* Hc = a.H + (size_t)c * ld * kmax;
-  const int tq = gm_opaque(threadIdx.x) / R, tr = gm_opaque(threadIdx.x) - tq * R;
-  const bool mact = tq < a.nseg && tr < Rc;
-  const int jc0 = tq * a.per, jc1 = min(nf, jc0 + a.per);
-  double sreg[kCoopMaxC];
+  // S[r0 + 4 wq + k][ln + 64 q] in registers (0 outside the matrix)
+  const int wq = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  double sreg[4][kCoopColQ];
 #pragma unroll
-  for (int q = 0; q < kCoopMaxC; ++q) sreg[q] = (mact && jc0 + q < jc1) ? a.S[(long)(jc0 + q) * nf + r0 + tr] : 0.0;
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int q = 0; q < kCoopColQ; ++q) {
+      const int row = 4 * wq + k, col = ln + 64 * q;
+      sreg[k][q] = row < Rc && col < nf ? a.S[(long)col * nf + r0 + row] : 0.0;
+    }
   __syncthreads();
 
   // wl = S[local rows] vfull (ends with a barrier)
   auto matvec = [&]() {
-    if (mact) {
-      double acc = 0.0;
+    const int q4 = gm_opaque(threadIdx.x) >> 6, l = gm_opaque(threadIdx.x) & 63;
+    if (4 * q4 < Rc) {  // uniform over the wavefront
+      double vv[kCoopColQ];
 #pragma unroll
-      for (int q0 = 0; q0 < kCoopMaxC; q0 += 8) {
-        double vv[8];
+      for (int q = 0; q < kCoopColQ; ++q) vv[q] = vfull[min(l + 64 * q, nf - 1)];
+      double acc[4];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) vv[q] = q0 + q < kCoopMaxC ? vfull[min(jc0 + q0 + q, nf - 1)] : 0.0;
+      for (int k = 0; k < 4; ++k) {
+        acc[k] = 0.0;
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-          if (q0 + q < kCoopMaxC) acc += sreg[q0 + q] * vv[q];  // sreg is 0 past the segment
+        for (int q = 0; q < kCoopColQ; ++q) acc[k] += sreg[k][q] * vv[q];  // sreg is 0 past the matrix
       }
-      part[tq * R + tr] = acc;
-    }
-    __syncthreads();
-    for (int r = gm_opaque(threadIdx.x); r < Rc; r += T) {
-      double acc = 0.0;
-      for (int q = 0; q < a.nseg; ++q) acc += part[q * R + r];
-      wl[r] = acc;
+      const double sum = gc_reduce4(acc[0], acc[1], acc[2], acc[3]);
+      const int row = gc_reduce4_row(q4, l);
+      if (l < 4 && row < Rc) wl[row] = sum;
     }
     __syncthreads();
   };
@@ -849,11 +895,7 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
       gc_st(a.norms + c * 4 + 2, pb);
     }
     if (!gc_sync(a, c, ++bk, &s_err)) break;
-    double sr = 0.0, sb = 0.0;
-    for (int q = 0; q < a.P; ++q) {
-      sr += gc_ld(a.norms + q * 4 + 1);
-      sb += gc_ld(a.norms + q * 4 + 2);
-    }
+    const double sr = gc_sum(a.norms + 1, 4, a.P), sb = gc_sum(a.norms + 2, 4, a.P);
     const double res0 = sqrt(sr), bn = sqrt(sb);
     if (first) {
       rn0 = bn > 0.0 ? bn : res0;
@@ -875,9 +917,14 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
       // 16 lanes per coefficient
       {
         const int t = gm_opaque(threadIdx.x), sub = t & 15;
+        double wr[kCoopRowsPer16];  // rows sub, sub + 16, ... of w (R <= 16 kCoopRowsPer16)
+#pragma unroll
+        for (int q = 0; q < kCoopRowsPer16; ++q) wr[q] = sub + 16 * q < Rc ? wl[sub + 16 * q] : 0.0;
         for (int j = t >> 4; j <= it; j += T / 16) {
           double p = 0.0;
-          for (int r = sub; r < Rc; r += 16) p += wl[r] * Vl[(long)j * R + r];
+#pragma unroll
+          for (int q = 0; q < kCoopRowsPer16; ++q)
+            if (sub + 16 * q < Rc) p += wr[q] * Vl[(long)j * R + sub + 16 * q];
           p = gm_row_sum16(p);
           if (sub == 0) gc_st(a.dots + (long)c * (kmax + 2) + j, p);
         }
@@ -886,32 +933,33 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
         ok = false;
         break;
       }
-      for (int j = gm_opaque(threadIdx.x); j <= it; j += T) {
-        double h = 0.0;
-        for (int q = 0; q < a.P; ++q) h += gc_ld(a.dots + (long)q * (kmax + 2) + j);
-        hcol[j] = h;
-      }
+      for (int j = gm_opaque(threadIdx.x); j <= it; j += T) hcol[j] = gc_sum(a.dots + j, kmax + 2, a.P);
       __syncthreads();
-      // w -= sum_j h_j v_j on the local rows (j range split over segments,
-      // partials added in segment order); partial ||w||^2; w rows to L2
+      // w -= sum_j h_j v_j on the local rows (wavefront q4: rows 4 q4 ..
+      // 4 q4 + 3, lane l: j = l, l + 64, ...); partial ||w||^2; w rows out
       {
-        const int nj = it + 1;
-        const int nsg = max(1, min(a.nseg, nj));
-        const int pj = (nj + nsg - 1) / nsg;
-        if (tq < nsg && tr < Rc) {
-          const int q0 = min(nj, tq * pj), q1 = min(nj, q0 + pj);
-          double acc = 0.0;
-          for (int j = q0; j < q1; ++j) acc += -hcol[j] * Vl[(long)j * R + tr];
-          part[tq * R + tr] = acc;
-        }
-        __syncthreads();
+        const int q4 = gm_opaque(threadIdx.x) >> 6, l = gm_opaque(threadIdx.x) & 63;
         double pn = 0.0;
-        for (int r = gm_opaque(threadIdx.x); r < Rc; r += T) {
-          double acc = wl[r];
-          for (int q = 0; q < nsg; ++q) acc += part[q * R + r];
-          wl[r] = acc;
-          gc_st(a.wfull + r0 + r, acc);
-          pn += acc * acc;
+        if (4 * q4 < Rc) {  // uniform over the wavefront
+          const int rb = 4 * q4;
+          const bool ok1 = rb + 1 < Rc, ok2 = rb + 2 < Rc, ok3 = rb + 3 < Rc;
+          double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+          for (int j = l; j <= it; j += 64) {
+            const double hj = -hcol[j];
+            const double* vj = Vl + (long)j * R + rb;
+            acc0 += hj * vj[0];
+            acc1 += hj * (ok1 ? vj[1] : 0.0);
+            acc2 += hj * (ok2 ? vj[2] : 0.0);
+            acc3 += hj * (ok3 ? vj[3] : 0.0);
+          }
+          const double sum = gc_reduce4(acc0, acc1, acc2, acc3);
+          const int row = gc_reduce4_row(q4, l);
+          if (l < 4 && row < Rc) {
+            const double wn = wl[row] + sum;
+            wl[row] = wn;
+            gc_st(a.wfull + r0 + row, wn);
+            pn = wn * wn;
+          }
         }
         pn = gm_block_sum2(pn, red, T);
         if (threadIdx.x == 0) gc_st(a.norms + c * 4, pn);
@@ -920,9 +968,12 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
         ok = false;
         break;
       }
-      double sn2 = 0.0;
-      for (int q = 0; q < a.P; ++q) sn2 += gc_ld(a.norms + q * 4);
-      const double tt = sqrt(sn2);
+      // this thread's entries of the new vector (full and local rows), loaded
+      // with the norm partials so that both take one memory round trip
+      const int t0 = gm_opaque(threadIdx.x);
+      const double wpre = t0 < nf ? gc_ld(a.wfull + t0) : 0.0;
+      const double wpre_l = t0 < Rc ? gc_ld(a.wfull + r0 + t0) : 0.0;
+      const double tt = sqrt(gc_sum(a.norms, 4, a.P));
       if (gm_opaque(threadIdx.x) < 64) {
         const int lane = gm_opaque(threadIdx.x);
         // previous rotations on the new column (KSPGMRESUpdateHessenberg) as
@@ -942,9 +993,9 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
         if (lane == 0) {
           hcol[it] = carry;
           hcol[it + 1] = tt;
-          double hapbnd = fabs(tt / grs[it]);
-          if (hapbnd > 1e-30) hapbnd = 1e-30;
-          const int hap = tt < hapbnd;
+          // tt < min(|tt / grs[it]|, 1e-30) (KSPGMRESCycle's happy-breakdown
+          // test); the quotient is needed only when tt < 1e-30
+          const int hap = tt < 1e-30 && tt < fabs(tt / grs[it]);
           double res;
           if (!hap) {
             const double t1 = sqrt(hcol[it] * hcol[it] + hcol[it + 1] * hcol[it + 1]);
@@ -973,8 +1024,9 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
       for (int j = gm_opaque(threadIdx.x); j <= it + 1; j += T) Hc[(long)it * ld + j] = hcol[j];
       if (!hap) {
         const double inv = 1.0 / s_val[0];
-        for (int i = gm_opaque(threadIdx.x); i < nf; i += T) vfull[i] = gc_ld(a.wfull + i) * inv;
-        for (int r = gm_opaque(threadIdx.x); r < Rc; r += T) Vl[(long)(it + 1) * R + r] = gc_ld(a.wfull + r0 + r) * inv;
+        if (t0 < nf) vfull[t0] = wpre * inv;
+        for (int i = t0 + T; i < nf; i += T) vfull[i] = gc_ld(a.wfull + i) * inv;
+        if (t0 < Rc) Vl[(long)(it + 1) * R + t0] = wpre_l * inv;
       }
       ++it;
       ++its;
@@ -1041,12 +1093,10 @@ static size_t coop_lds(int nf, int kmax, int R) {
 
 bool gmres_coop_plan(int nf, int kmax, size_t max_lds, int* P, int* R) {
   static const int cand[] = {2, 4, 8, 16, 32};
+  if (nf > 64 * kCoopColQ) return false;
   for (int p : cand) {
     const int r = (nf + p - 1) / p;
-    if (r < 1 || r > kGmT || (p - 1) * r >= nf) continue;
-    const int ns = kGmT / r;
-    const int pc = (nf + ns - 1) / ns;
-    if (pc > kCoopMaxC) continue;
+    if (r < 1 || r > 16 * kCoopRowsPer16 || r > 4 * (kGmT / 64) || (p - 1) * r >= nf) continue;
     if (kGmT / (kmax + 1) < 1) continue;  // back-substitution chunk in the part buffer
     if (coop_lds(nf, kmax, r) + 1024 > max_lds) continue;
     *P = p;
@@ -1064,10 +1114,9 @@ size_t gmres_coop_ws(int nf, int kmax, int P) {
 
 void gmres_coop(hipStream_t st, int nf, int kmax, int maxit, int P, int R, const double* S, const double* b, double* x,
                 GmresDev* s, double rtol, double atol, double* ws) {
-  const int ns = kGmT / R;
   GmCoopArgs a{};
   a.nf = nf; a.kmax = kmax; a.maxit = maxit; a.P = P; a.R = R;
-  a.nseg = ns; a.per = (nf + ns - 1) / ns; a.bsc = kGmT / (kmax + 1);
+  a.bsc = kGmT / (kmax + 1);
   a.S = S; a.b = b; a.x = x; a.s = s; a.rtol = rtol; a.atol = atol;
   a.sync = reinterpret_cast<unsigned*>(ws);
   a.dots = ws + kCoopSyncDoubles;

@@ -502,7 +502,7 @@ struct ib_handle {
     }
   }
   void gmres_batch(int it0, int nb) {
-    if (prm.launch_mode != 0 || tim[0].on || tim[1].on || tim[2].on || tim[3].on || tim[4].on) {
+    if (prm.launch_mode != 0 || no_capture || tim[0].on || tim[1].on || tim[2].on || tim[3].on || tim[4].on) {
       gmres_batch_direct(it0, nb);
       return;
     }
@@ -1133,10 +1133,14 @@ struct ib_handle {
   // launch_mode 1 and while kernel timing records per-launch events; the
   // adjoint's tape copies go between the graphs.
   std::map<int, hipGraphExec_t> step_graphs;
+  // set for the simulations of a batch, which run on several host threads
+  // at once: no stream capture, so that none is open while another thread
+  // allocates or copies synchronously
+  bool no_capture = false;
   bool step_graphs_ok() const {
     bool timing = false;
     for (const auto& t : tim) timing = timing || t.on;
-    return prm.launch_mode == 0 && !timing && !sd_on;
+    return prm.launch_mode == 0 && !timing && !sd_on && !no_capture;
   }
   template <class F>
   void replay(int key, F&& body) {
