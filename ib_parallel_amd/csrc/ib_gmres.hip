@@ -833,6 +833,7 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
   double* lss = lcc + kmax;                     // kmax
   double* grs = lss + kmax;                     // kmax + 1
   double* yv = grs + kmax + 1;                  // kmax + 1
+  double* hrot = yv + kmax + 1;                 // kmax + 2: the new Hessenberg column after the rotations
   __shared__ double red[kGmT / 64];
   __shared__ double s_val[2];
   __shared__ int s_flag[2];
@@ -871,6 +872,29 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
     }
     __syncthreads();
   };
+
+  // the previous rotations applied to the new column (KSPGMRESUpdateHessenberg)
+  // as an affine-recurrence scan, 64 rotations at a time, by one whole
+  // wavefront: hrot[j] for j < it, and hrot[it] = the entry the new rotation
+  // acts on; hcol keeps the unrotated CGS coefficients
+  auto rotate_prev = [&](int it, int lane) {
+    double carry = hcol[0];
+    for (int base = 0; base < it; base += 64) {
+      const int j = base + lane;
+      const bool act = j < it;
+      const double cj0 = act ? lcc[j] : 1.0, sj = act ? lss[j] : 0.0, hn = act ? hcol[j + 1] : 0.0;
+      double A = -sj, B = cj0 * hn;
+      gm_affine_scan(A, B);
+      const double cnext = A * carry + B;
+      const double cj = gm_shift_up1(cnext, carry);
+      if (act) hrot[j] = cj0 * cj + sj * hn;
+      carry = gm_readlane(cnext, min(63, it - 1 - base));
+    }
+    if (lane == 0) hrot[it] = carry;
+  };
+  // the last wavefront owns no rows of the update w -= V h when R <= 60: it
+  // runs the scan while the others update w
+  const bool scan_early = 4 * (T / 64 - 1) >= Rc;
 
   int its = 0, reason = 0, first = 1;
   unsigned bk = 0;  // grid barriers passed
@@ -960,6 +984,8 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
             gc_st(a.wfull + r0 + row, wn);
             pn = wn * wn;
           }
+        } else if (scan_early && q4 == T / 64 - 1) {
+          rotate_prev(it, l);
         }
         pn = gm_block_sum2(pn, red, T);
         if (threadIdx.x == 0) gc_st(a.norms + c * 4, pn);
@@ -976,35 +1002,21 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
       const double tt = sqrt(gc_sum(a.norms, 4, a.P));
       if (gm_opaque(threadIdx.x) < 64) {
         const int lane = gm_opaque(threadIdx.x);
-        // previous rotations on the new column (KSPGMRESUpdateHessenberg) as
-        // an affine-recurrence scan, 64 rotations at a time
-        double carry = hcol[0];
-        for (int base = 0; base < it; base += 64) {
-          const int j = base + lane;
-          const bool act = j < it;
-          const double cj0 = act ? lcc[j] : 1.0, sj = act ? lss[j] : 0.0, hn = act ? hcol[j + 1] : 0.0;
-          double A = -sj, B = cj0 * hn;
-          gm_affine_scan(A, B);
-          const double cnext = A * carry + B;
-          const double cj = gm_shift_up1(cnext, carry);
-          if (act) hcol[j] = cj0 * cj + sj * hn;
-          carry = gm_readlane(cnext, min(63, it - 1 - base));
-        }
+        if (!scan_early) rotate_prev(it, lane);
         if (lane == 0) {
-          hcol[it] = carry;
-          hcol[it + 1] = tt;
+          hrot[it + 1] = tt;
           // tt < min(|tt / grs[it]|, 1e-30) (KSPGMRESCycle's happy-breakdown
           // test); the quotient is needed only when tt < 1e-30
           const int hap = tt < 1e-30 && tt < fabs(tt / grs[it]);
           double res;
           if (!hap) {
-            const double t1 = sqrt(hcol[it] * hcol[it] + hcol[it + 1] * hcol[it + 1]);
-            const double cc = hcol[it] / t1, sn = hcol[it + 1] / t1;
+            const double t1 = sqrt(hrot[it] * hrot[it] + hrot[it + 1] * hrot[it + 1]);
+            const double cc = hrot[it] / t1, sn = hrot[it + 1] / t1;
             lcc[it] = cc;
             lss[it] = sn;
             grs[it + 1] = -sn * grs[it];
             grs[it] = cc * grs[it];
-            hcol[it] = cc * hcol[it] + sn * hcol[it + 1];
+            hrot[it] = cc * hrot[it] + sn * hrot[it + 1];
             res = fabs(grs[it + 1]);
           } else {
             res = 0.0;
@@ -1021,7 +1033,7 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
       __syncthreads();
       reason = s_flag[0];
       const int hap = s_flag[1];
-      for (int j = gm_opaque(threadIdx.x); j <= it + 1; j += T) Hc[(long)it * ld + j] = hcol[j];
+      for (int j = gm_opaque(threadIdx.x); j <= it + 1; j += T) Hc[(long)it * ld + j] = hrot[j];
       if (!hap) {
         const double inv = 1.0 / s_val[0];
         if (t0 < nf) vfull[t0] = wpre * inv;
@@ -1086,9 +1098,9 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
 }
 
 // dynamic LDS of k_gmres_coop (doubles): vfull nf, Vl (kmax+1) R, wl R,
-// part kGmT, hcol kmax+2, lcc/lss kmax each, grs/yv kmax+1 each
+// part kGmT, hcol kmax+2, lcc/lss kmax each, grs/yv kmax+1 each, hrot kmax+2
 static size_t coop_lds(int nf, int kmax, int R) {
-  return sizeof(double) * ((size_t)nf + (size_t)(kmax + 1) * R + R + kGmT + (size_t)5 * kmax + 4);
+  return sizeof(double) * ((size_t)nf + (size_t)(kmax + 1) * R + R + kGmT + (size_t)6 * kmax + 6);
 }
 
 bool gmres_coop_plan(int nf, int kmax, size_t max_lds, int* P, int* R) {
