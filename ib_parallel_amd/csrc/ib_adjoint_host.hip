@@ -197,7 +197,7 @@ void ib_handle::adjoint_pos_setup() {
   build_face_set();
   face_margin = fm_save;
   if (!tapidx.p) tapidx.alloc((size_t)nf * kTaps);
-  if (!face_flag.p) face_flag.alloc(1);
+  if (!face_flag.p) face_flag.alloc(2);
   alloc_assembly(nF);
   compute_green();
   map_taps(st, nf, csr.key, fmap, tapidx, face_flag);

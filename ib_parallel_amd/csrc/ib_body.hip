@@ -55,7 +55,8 @@ __device__ double deltafnc(double x1, double x2, double dr, int f32) {
 // setup_reg: src/grid.F90:459-528, one thread per (entry, tap)
 __global__ void k_setup_reg(Geom g, int nf, const double* __restrict__ xb, const double* __restrict__ yb,
                             const int* __restrict__ comp, double delta, double offx, double offy, int f32,
-                            double* __restrict__ w, int* __restrict__ key, int* __restrict__ err) {
+                            double* __restrict__ w, int* __restrict__ key, int* __restrict__ err,
+                            int* __restrict__ changed) {
 #pragma clang fp contract(off)
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nf * kTaps) return;
@@ -65,6 +66,7 @@ __global__ void k_setup_reg(Geom g, int nf, const double* __restrict__ xb, const
   const int ix = (int)((xb[k] + offx) / delta) + 1;
   const int iy = (int)((yb[k] + offy) / delta) + 1;
   const int ii = ix + p, jj = iy + l;
+  const int old = changed ? key[idx] : 0;
   double x, y;
   int kk;
   bool ok;
@@ -83,18 +85,20 @@ __global__ void k_setup_reg(Geom g, int nf, const double* __restrict__ xb, const
     *err = 1;
     w[idx] = 0.0;
     key[idx] = 0;
+    if (changed && old != 0) *changed = 1;
     return;
   }
   w[idx] = delta * delta * deltafnc(x, xb[k], delta, f32) * deltafnc(y, yb[k], delta, f32);
   key[idx] = kk;
+  if (changed && old != kk) *changed = 1;
 }
 
 void setup_reg(hipStream_t st, const Geom& g, int nf, const double* xb, const double* yb, const int* comp,
-               double delta, double offx, double offy, int f32lit, double* w, int* key, int* err) {
+               double delta, double offx, double offy, int f32lit, double* w, int* key, int* err, int* changed) {
   const int nt = nf * kTaps;
   if (nt <= 0) return;
   hipLaunchKernelGGL(k_setup_reg, dim3(ceil_div(nt, 256)), dim3(256), 0, st, g, nf, xb, yb, comp, delta, offx, offy,
-                     f32lit, w, key, err);
+                     f32lit, w, key, err, changed);
   IB_LAUNCH_CHECK();
 }
 

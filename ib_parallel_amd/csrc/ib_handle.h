@@ -456,12 +456,16 @@ struct ib_handle {
   }
 
   // setup_reg + the sorted tap list; the old touched faces of hx/hy are
-  // cleared first (the invariant schur_dev relies on)
-  void rebuild_weights() {
+  // cleared first (the invariant schur_dev relies on).  With key_changed
+  // the sorted list is left to the caller: the kernel sets *key_changed when
+  // a tap key moved, and the caller runs csr.build() before the list is read
+  // again (the list depends on the keys only)
+  void rebuild_weights(int* key_changed = nullptr) {
     fill(st, hx, g.Fx, 0.0);
     fill(st, hy, g.Fy, 0.0);
-    setup_reg(st, g, nf, xb, yb, comp, delta, offx, offy, prm.deltafnc_f32_literals, w, csr.key, err_flag);
-    csr.build(st);
+    setup_reg(st, g, nf, xb, yb, comp, delta, offx, offy, prm.deltafnc_f32_literals, w, csr.key, err_flag,
+              key_changed);
+    if (!key_changed) csr.build(st);
   }
 
   int read_err_flag() {
@@ -898,14 +902,18 @@ struct ib_handle {
         redist_faces(st, g, csr, w, f, hx, hy);
         regT(st, g, nf, w, csr.key, hx, hy, 0.0, nullptr, f_rdst);
         body_dx(st, ntors, tb, th, dt, delta, f, xb, yb, dabs);
-        rebuild_weights();
-        IB_HIP(hipMemsetAsync(face_flag.p, 0, sizeof(int), st));
+        // face_flag[0]: a tap outside the face set; [1]: a tap key moved (the
+        // sorted tap list is rebuilt after the synchronisation, only then;
+        // the next GMRES solve does not read it)
+        IB_HIP(hipMemsetAsync(face_flag.p, 0, 2 * sizeof(int), st));
+        rebuild_weights(face_flag.p + 1);
         map_taps(st, nf, csr.key, fmap, tapidx, face_flag);
         IB_HIP(hipMemcpyAsync(hbuf, dabs.p, sizeof(double) * ntors, hipMemcpyDeviceToHost, st));
         IB_HIP(hipMemcpyAsync(hflag, err_flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
-        IB_HIP(hipMemcpyAsync(hflag + 1, face_flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        IB_HIP(hipMemcpyAsync(hflag + 1, face_flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
         IB_HIP(hipMemcpyAsync(hst, gst, sizeof(GmresDev), hipMemcpyDeviceToHost, st));
         sync();
+        if (hflag[2]) csr.build(st);
         rc = dense_check();
         its = hst->its;
         if (tim[4].on) tim[4].total_bytes += 8.0 * (double)nf * (its + 1);
@@ -1462,7 +1470,7 @@ struct ib_handle {
       }
     }
     if (facemode) {
-      face_flag.alloc(1);
+      face_flag.alloc(2);
       tapidx.alloc((size_t)nf * kTaps);
       build_face_set();
       if (prm.schur_mode == 0 && (size_t)nF * nF > kAutoGreenMax) {

@@ -178,8 +178,11 @@ const void* fluid_tab_kernel(int kind);
 // weights of setup_reg / deltafnc for every (entry, tap); face key per tap:
 // x-face index for x entries, Fx + y-face index for y entries; err flag set on
 // a stencil leaving the grid (key then clamped to 0, weight 0).
+// weights and tap keys of every body point; with `changed`, *changed is set
+// to 1 when some tap's key differs from the one key held before the call
 void setup_reg(hipStream_t st, const Geom& g, int nf, const double* xb, const double* yb, const int* comp,
-               double delta, double offx, double offy, int f32lit, double* w, int* key, int* err);
+               double delta, double offx, double offy, int f32lit, double* w, int* key, int* err,
+               int* changed = nullptr);
 // sort taps by key (stable) and run-length encode: uniq keys, offsets, count
 struct RegCSR {
   int ntaps = 0;
