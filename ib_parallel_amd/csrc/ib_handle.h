@@ -1473,6 +1473,13 @@ struct ib_handle {
       face_flag.alloc(2);
       tapidx.alloc((size_t)nf * kTaps);
       build_face_set();
+      // schur_mode 0 with the default margin: narrower margins (2, then 0
+      // cells) before the form gives up G; a narrower set is rebuilt more
+      // often as the bodies move
+      while (prm.schur_mode == 0 && prm.face_margin == 0 && face_margin > 0 && (size_t)nF * nF > kAutoGreenMax) {
+        face_margin = face_margin > 2 ? 2 : 0;
+        build_face_set();
+      }
       if (prm.schur_mode == 0 && (size_t)nF * nF > kAutoGreenMax) {
         // too many faces near the bodies for G: assemble S directly instead
         facemode = false;

@@ -78,7 +78,9 @@ typedef struct ib_params {
                               (G recomputed only when a tap leaves that set);
                               0 auto: 3 with torsional bodies, else 2 */
   int32_t face_margin;     /* schur_mode 3: cells of margin around the taps'
-                              faces in the face set (0: 4 cells, -1: none) */
+                              faces in the face set (0: 4 cells, with
+                              schur_mode 0 narrowed to 2 or 0 cells when G
+                              would pass 2 GiB; -1: none) */
   int32_t gmres_form;      /* GMRES on a dense Schur matrix: 0 auto (the
                               cooperative multi-workgroup solve where it
                               fits, else one workgroup), 1 one workgroup,
