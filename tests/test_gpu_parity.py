@@ -191,7 +191,22 @@ def test_face_space_recompute_when_taps_leave_the_set():
     for _ in range(8):
         g2.advance()
     form2, nfaces2, builds2 = g2.schur_info()
-    assert nfaces2 > nfaces and builds2 == 1  # the 2-cell margin covers this motion
+    assert nfaces2 > nfaces and builds2 == 1  # the default margin covers this motion
+
+
+def test_airfoil_flap_keeps_the_face_space_form():
+    """schur_mode 0 with a torsional flap picks the face-space form.  For the
+    airfoil + flap at 1024x512x5 the default 4-cell face set would pass G's
+    2 GiB limit (16384 faces), so the margin narrows before the form gives
+    up G (the assembled fallback has no adjoint for torsional bodies)."""
+    p, b = decks.airfoil_flap_case()
+    g = gpu(p, b)
+    form, nfaces, builds = g.schur_info()
+    assert form == 3, form
+    assert 0 < nfaces <= 16384 and builds == 1, (nfaces, builds)
+    info = g.advance()
+    assert info.status == abi.IB_OK and info.fsi_iter >= 2
+    g.close()
 
 
 def test_full_size_plate_invariants():
