@@ -1,6 +1,6 @@
 """HBM traffic per kernel launch from two rocprofv3 counter runs.
 
-Usage (on the GPU box, each counter in a run of its own):
+Usage (on a machine with the GPU, each counter in a run of its own):
 
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT/fetch -o run -- \\
         python bench.py --steps 2 --warmup 0 --no-cpu --pmc-copy-mb 256

@@ -67,7 +67,7 @@ def test_steps_match_oracle(case):
 
 def test_one_level_rigid_included_equals_plain_step():
     from ib_parallel_amd import GpuSim
-    # (the sparsity windows of this build must stay inside the grid)
+    # (the GPU sparsity windows must stay inside the grid)
     p, b = decks.tiny_case(m=64, n=48, mgridlev=1, torsional=False, tol=1e-12)
     sd = abi.make_subdomain(-0.8, 0.0, -0.4, 0.4, 4.0 / 64, full_rectangular=0, droptol=TINY)
     g, pl = GpuSim(sd_params(p), b, subdomain=sd), GpuSim(p, b)
