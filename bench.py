@@ -131,6 +131,7 @@ def run_gpu(args):
                              f"{secs_prof / nprof * 1e3:.1f} ms/step")
     form, nfaces, builds = sim.schur_info()
     res = {"sim": sim, "desc": desc, "secs": secs, "its": its, "fsi": fsi, "roofline": roof,
+           "tiles": sim.dst_tiles(),
            "schur": {"form": {1: "matrix-free", 2: "assembled", 3: "face-space"}[form], "faces": nfaces,
                      "green_builds": builds}}
     return res
@@ -178,6 +179,7 @@ def run_batched(args):
         "serial_plain_value": round(vs, 3),
         "serial_plain_ms_per_step": round(ts / args.batch_steps * 1e3, 3),
         "speedup_vs_serial_plain": round(vb / vs, 2), "create_s": round(t_create, 2),
+        "fused_dst_tile_rules": dict(zip(("row", "column"), batch.dst_tiles())),
     }
     batch.close()
     return res
@@ -417,7 +419,9 @@ def main():
         "config": {"workload": g["desc"], "m": sim.m, "n": sim.n, "mgridlev": sim.L, "nb": sim.nf // 2,
                    "gmres_its_per_step": round(g["its"] / args.steps, 2),
                    "fsi_its_per_step": round(g["fsi"] / args.steps, 2), "form": "plain",
-                   "schur_operator": g["schur"]},
+                   "schur_operator": g["schur"],
+                   "dst_tiles": {"row_pairs_per_workgroup": g["tiles"][0], "column_pairs_per_workgroup": g["tiles"][1],
+                                 "chosen_by": {0: "fixed rule", 1: "timed at create", 2: "IB_DST_TILES"}[g["tiles"][2]]}},
         "roofline": g["roofline"],
     }
     if args.batch > 0:

@@ -49,6 +49,13 @@ class GpuSim(Simulation):
         self._check(self._f["schur_info"](self.h, C.byref(mode), C.byref(nfaces), C.byref(builds)))
         return mode.value, nfaces.value, builds.value
 
+    def dst_tiles(self):
+        """(rp, cp, source): row pairs per workgroup of the DST row pass, column
+        pairs of the column pass, 0 fixed rule / 1 timed / 2 forced (IB_DST_TILES)."""
+        rp, cp, src = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self._f["dst_tiles"](self.h, C.byref(rp), C.byref(cp), C.byref(src)))
+        return rp.value, cp.value, src.value
+
     def adjoint(self, nsteps, cx, cy, ckpt_every=None, omega_grad=True, position_grad=False, body_grad=False):
         """Advance nsteps steps and return (J, dJ/dRe, dJ/domega0) for the
         time-mean objective J = mean over the steps of sum_b cx[b] fx_b +
@@ -138,6 +145,13 @@ class GpuBatch:
         n, m = C.c_int32(), C.c_int32()
         self._check(self._b["batch_info"](self.h, C.byref(n), C.byref(m)))
         return m.value
+
+    def dst_tiles(self):
+        """(row rule, column rule) of the fused form's DST launches: 0 each
+        simulation's own tiles, -1 the fixed rule, k > 0 k pairs per workgroup."""
+        x, y = C.c_int32(), C.c_int32()
+        self._check(self._b["batch_dst_tiles"](self.h, C.byref(x), C.byref(y)))
+        return x.value, y.value
 
     def advance(self, allow_noconv=False):
         rc = self._b["batch_advance"](self.h, self.itimes, self.infos)

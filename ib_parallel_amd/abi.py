@@ -115,6 +115,7 @@ SIGNATURES = {
 
 GPU_ONLY = {
     "schur_info": (C.c_int, [_P, _I, _I, C.POINTER(C.c_int64)]),
+    "dst_tiles": (C.c_int, [_P, _I, _I, _I]),
     "kernel_timing": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "kernel_timing_read": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),
@@ -129,6 +130,7 @@ BATCH = {
     "batch_advance": (C.c_int, [_P, _I, C.POINTER(IBStepInfo)]),
     "batch_handle": (C.c_int, [_P, C.c_int32, C.POINTER(_P)]),
     "batch_info": (C.c_int, [_P, _I, _I]),
+    "batch_dst_tiles": (C.c_int, [_P, _I, _I]),
     "batch_last_error": (C.c_int, [_P, C.c_char_p, C.c_int32]),
     "batch_adjoint": (C.c_int, [_P, C.c_int32, C.c_int32, _D, _D, _D, _D, _D, _D]),
 }

@@ -15,6 +15,8 @@
 // Concurrent form (every other mix: torsional bodies, GMRES solves with
 // host-side convergence control): one host thread per simulation drives its
 // plain step on the simulation's own stream.
+#include <cstdlib>
+#include <cstring>
 #include <thread>
 
 #include "ib_handle.h"
@@ -87,12 +89,108 @@ struct ib_batch {
   static bool fusable(const ib_handle* h) { return h->ntors == 0 && h->nf > 0 && h->prm.num_stat; }
 
   // ---------------------------------------------------------- fused plan
-  // Row / column pass entries with the pair counts the plain step uses.
-  static TabEntry<XArgs> xe(const ib_handle* h, const DstLaunch& l, size_t* lds) {
-    return dst_x_entry(h->px.dev, l, h->g, (h->px.dev.L <= 512 ? 2 : 1), h->max_lds, lds);
+  // Tile shapes of the fused DST table launches.  One launch covers every
+  // simulation, so it is throughput-bound where one simulation's launch is
+  // latency-bound, and the best tiles differ from each handle's own:
+  // tune_tiles times a row-pass and a column-pass table launch over all
+  // simulations for each rule.  Rules: 0 each handle's tiles, -1 the fixed
+  // rule (2 row pairs up to transform length 512, else 1; 1 column pair),
+  // k > 0 k pairs (halved where the LDS is short).  Tiles never change
+  // results.
+  int xrule = 0, yrule = 0;
+  int xtile(const ib_handle* h) const {
+    return xrule > 0 ? xrule : (xrule < 0 ? (h->px.dev.L <= 512 ? 2 : 1) : h->tiles.rp);
   }
-  static TabEntry<YArgs> ye(const ib_handle* h, int mode, const DstLaunch& l, size_t* lds) {
-    return dst_y_entry(h->py.dev, mode, l, h->g, 1, h->max_lds, lds);
+  int ytile(const ib_handle* h) const { return yrule > 0 ? yrule : (yrule < 0 ? 1 : h->tiles.cp); }
+  TabEntry<XArgs> xe(const ib_handle* h, const DstLaunch& l, size_t* lds) const {
+    return dst_x_entry(h->px.dev, l, h->g, xtile(h), h->max_lds, lds);
+  }
+  TabEntry<YArgs> ye(const ib_handle* h, int mode, const DstLaunch& l, size_t* lds) const {
+    return dst_y_entry(h->py.dev, mode, l, h->g, ytile(h), h->max_lds, lds);
+  }
+
+  // best of 3 timings of 2 back-to-back launches of one table op
+  template <class A>
+  float time_tab(const void* fn, const std::vector<TabEntry<A>>& es, size_t lds) {
+    OpList ol;
+    ol.add(fn, es, lds);
+    DBuf<char> ar;
+    ar.alloc(ol.bytes.size());
+    IB_HIP(hipMemcpy(ar.p, ol.bytes.data(), ol.bytes.size(), hipMemcpyHostToDevice));
+    const BatchOp& op = ol.ops[0];
+    const void* tab = ar.p + op.off;
+    void* args[] = {(void*)&tab};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    IB_HIP(hipEventCreate(&e0));
+    IB_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    try {
+      for (int rep = 0; rep < 4; ++rep) {
+        IB_HIP(hipEventRecord(e0, st));
+        for (int k = 0; k < 2; ++k) IB_HIP(hipLaunchKernel(op.fn, op.grid, dim3(256), args, op.lds, st));
+        IB_HIP(hipEventRecord(e1, st));
+        IB_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        IB_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (rep > 0) best = std::min(best, ms);  // the first round warms up
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+  }
+
+  // row pass t0 -> t1 and column pass (scaled by laminv) t1 -> t0 of every
+  // simulation (scratch fields of the step) under each rule; the environment's choice
+  // (IB_DST_TILES, IB_DST_TUNE=0) stays with the handles' tiles
+  void tune_tiles() {
+    xrule = yrule = 0;
+    if (std::getenv("IB_DST_TILES")) return;
+    if (const char* f = std::getenv("IB_DST_TUNE"))
+      if (std::strcmp(f, "0") == 0) return;
+    size_t t = 0;
+    float best = 1e30f;
+    int bx = 0;
+    for (int r : {0, -1, 1, 2, 4}) {
+      xrule = r;
+      std::vector<TabEntry<XArgs>> es;
+      size_t l = 0;
+      for (ib_handle* h : sims) {
+        DstLaunch a;
+        a.in[0] = h->t0; a.out[0] = h->t1;
+        es.push_back(xe(h, a, &t));
+        l = std::max(l, t);
+      }
+      const float ms = time_tab(dst_x_tab_kernel(X_PLAIN), es, l);
+      if (ms < best) {
+        best = ms;
+        bx = r;
+      }
+    }
+    xrule = bx;
+    best = 1e30f;
+    int by = 0;
+    for (int r : {0, -1, 2, 4}) {
+      yrule = r;
+      std::vector<TabEntry<YArgs>> es;
+      size_t l = 0;
+      for (ib_handle* h : sims) {
+        DstLaunch a;
+        a.in[0] = h->t1; a.out[0] = h->t0; a.tab = h->laminv; a.div = 1.0;
+        es.push_back(ye(h, Y_SCALE2, a, &t));
+        l = std::max(l, t);
+      }
+      const float ms = time_tab(dst_y_tab_kernel(Y_SCALE2), es, l);
+      if (ms < best) {
+        best = ms;
+        by = r;
+      }
+    }
+    yrule = by;
   }
 
   template <class A>
@@ -251,6 +349,7 @@ struct ib_batch {
   }
 
   void build_fused() {
+    tune_tiles();
     plan_step();
     arena.alloc(plan.bytes.size());
     IB_HIP(hipMemcpy(arena.p, plan.bytes.data(), plan.bytes.size(), hipMemcpyHostToDevice));
@@ -448,6 +547,13 @@ int ib_batch_info(const ib_batch* b, int32_t* nsim, int32_t* mode) {
   if (!b) return IB_E_INVAL;
   if (nsim) *nsim = (int32_t)b->sims.size();
   if (mode) *mode = b->mode;
+  return IB_OK;
+}
+
+int ib_batch_dst_tiles(const ib_batch* b, int32_t* row_rule, int32_t* col_rule) {
+  if (!b) return IB_E_INVAL;
+  if (row_rule) *row_rule = b->xrule;
+  if (col_rule) *col_rule = b->yrule;
   return IB_OK;
 }
 

@@ -12,7 +12,13 @@
 // pre/post operations keep the ctci/ainv/predictor spectral products inside
 // the column pass (Y_SCALE2, Y_PRED) and apply_bc / the corrector subtraction
 // inside the row pass (X_APPLYBC, X_SUBFROM).
+#include <array>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "ib_device.h"
@@ -658,6 +664,110 @@ void dst_y(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, cons
     default: throw HipError("bad DST column mode");
   }
   IB_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------------
+// Tile shapes of the DST passes, picked by timing.  The row pairs per
+// workgroup of the row pass (rp) and the column pairs of the column pass
+// (cp) decide only how many transforms share a workgroup's LDS, never a
+// transform's arithmetic, so every choice gives the same bits.  Each
+// candidate that fits the LDS runs a few times on scratch fields between
+// HIP events; the fastest is kept for the process, per grid shape.
+// IB_DST_TILES=rp,cp forces a choice; IB_DST_TUNE=0 keeps the fixed rule
+// (rp 2 up to transform length 512, else 1; cp 1).
+namespace {
+std::mutex g_tile_mu;
+std::map<std::array<long, 5>, DstTiles> g_tiles;
+
+template <class F>
+float time_launches(hipStream_t st, F&& launch) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  IB_HIP(hipEventCreate(&e0));
+  IB_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  try {
+    launch();  // warm-up (code objects, caches)
+    launch();
+    for (int rep = 0; rep < 3; ++rep) {
+      IB_HIP(hipEventRecord(e0, st));
+      for (int k = 0; k < 4; ++k) launch();
+      IB_HIP(hipEventRecord(e1, st));
+      IB_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      IB_HIP(hipEventElapsedTime(&ms, e0, e1));
+      best = std::min(best, ms);
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
+}
+}  // namespace
+
+DstTiles dst_tiles(hipStream_t st, const FFTPlan& px, const FFTPlan& py, const Geom& g, size_t max_lds) {
+  DstTiles t;
+  t.rp = px.L <= 512 ? 2 : 1;
+  t.cp = 1;
+  if (const char* f = std::getenv("IB_DST_TILES")) {
+    int rp = 0, cp = 0;
+    if (std::sscanf(f, "%d,%d", &rp, &cp) != 2 || rp < 1 || rp > 8 || cp < 1 || cp > 8)
+      throw HipError("IB_DST_TILES must be rp,cp with 1 <= rp, cp <= 8");
+    t.rp = rp;  // dst_x_entry / dst_y_entry halve a choice the LDS cannot hold
+    t.cp = cp;
+    t.source = 2;
+    return t;
+  }
+  if (const char* f = std::getenv("IB_DST_TUNE"))
+    if (std::strcmp(f, "0") == 0) return t;
+  const std::array<long, 5> key{g.m, g.n, px.L, py.L, (long)max_lds};
+  std::lock_guard<std::mutex> lock(g_tile_mu);
+  auto it = g_tiles.find(key);
+  if (it != g_tiles.end()) return it->second;
+  double* in = nullptr;
+  double* out = nullptr;
+  const size_t bytes = sizeof(double) * (size_t)g.P;
+  IB_HIP(hipMalloc(&in, bytes));
+  if (hipMalloc(&out, bytes) != hipSuccess) {
+    (void)hipFree(in);
+    throw HipError("DST tile tuning: out of device memory");
+  }
+  try {
+    IB_HIP(hipMemsetAsync(in, 0, bytes, st));
+    DstLaunch l;
+    l.in[0] = in;
+    l.out[0] = out;
+    float best = 1e30f;
+    for (int rp : {1, 2, 4}) {
+      if (dst_lds_bytes(px, rp) > max_lds) continue;
+      const float ms = time_launches(st, [&] { dst_x(st, px, X_PLAIN, l, g, rp, max_lds); });
+      if (ms < best) {
+        best = ms;
+        t.rp = rp;
+      }
+    }
+    best = 1e30f;
+    for (int cp : {1, 2, 4}) {
+      if (dst_lds_bytes(py, cp) > max_lds) continue;
+      const float ms = time_launches(st, [&] { dst_y(st, py, Y_PLAIN, l, g, cp, max_lds); });
+      if (ms < best) {
+        best = ms;
+        t.cp = cp;
+      }
+    }
+  } catch (...) {
+    (void)hipFree(in);
+    (void)hipFree(out);
+    throw;
+  }
+  (void)hipFree(in);
+  (void)hipFree(out);
+  t.source = 1;
+  g_tiles[key] = t;
+  return t;
 }
 
 }  // namespace ibg

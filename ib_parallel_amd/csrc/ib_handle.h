@@ -102,6 +102,7 @@ struct ib_handle {
   int L = 0;
   double delta = 0, dt = 0, Re = 0, offx = 0, offy = 0, normalize = 0;
   HostFFTPlan px, py;
+  DstTiles tiles;  // tile shapes of the step's DST passes (dst_tiles)
   DBuf<double> laminv, lam1, lam1i;
   std::vector<double> vfac, con1, con2, q0x;
   DBuf<double> omega, s, qx, qy, rhs_old;
@@ -218,13 +219,13 @@ struct ib_handle {
   void xpass(int mode, const DstLaunch& a) {
     KTimer& t = tim[tclass(0, a)];
     t.begin(st, pass_bytes(mode == X_SUBFROM ? 2 : (mode == X_POINTROT ? 0 : 1), 0, 1, a.batch));
-    dst_x(st, px.dev, mode, a, g, (px.dev.L <= 512 ? 2 : 1), max_lds);
+    dst_x(st, px.dev, mode, a, g, tiles.rp, max_lds);
     t.end(st);
   }
   void ypass(int mode, const DstLaunch& a) {
     KTimer& t = tim[tclass(1, a)];
     t.begin(st, pass_bytes(mode == Y_PRED ? 2 : 1, mode == Y_PRED ? 2 : (mode == Y_SCALE2 ? 1 : 0), 1, a.batch));
-    dst_y(st, py.dev, mode, a, g, 1, max_lds);
+    dst_y(st, py.dev, mode, a, g, tiles.cp, max_lds);
     t.end(st);
   }
 
@@ -1366,6 +1367,7 @@ struct ib_handle {
     py.init(n - 1);
     if (dst_lds_bytes(px.dev, 1) > max_lds || dst_lds_bytes(py.dev, 2) > max_lds)
       return fail(IB_E_UNSUPPORTED, "grid too large for the single-workgroup DST (LDS)");
+    tiles = dst_tiles(st, px.dev, py.dev, g, max_lds);
 
     // setup_fft tables: src/myfft.F90:262-289
     const double pi = 4.0 * std::atan(1.0);

@@ -85,6 +85,13 @@ void dst_x(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& a, cons
 // y pass over columns of length n-1 (m-1 columns); cp = column pairs per workgroup
 void dst_y(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& a, const Geom& g, int cp,
            size_t max_lds);
+// Tile shapes of the DST passes (row pairs per workgroup of the row pass,
+// column pairs of the column pass): timed per grid shape once per process.
+// source: 0 fixed rule, 1 timed, 2 forced by IB_DST_TILES.
+struct DstTiles {
+  int rp = 1, cp = 1, source = 0;
+};
+DstTiles dst_tiles(hipStream_t st, const FFTPlan& px, const FFTPlan& py, const Geom& g, size_t max_lds);
 // LDS bytes a launch needs
 size_t dst_lds_bytes(const FFTPlan& pl, int nfft);
 // The arguments and grid dst_x / dst_y would launch with, as a table entry,

@@ -380,6 +380,14 @@ int ib_schur_info(ib_handle* h, int32_t* mode, int32_t* nfaces, int64_t* green_b
   return IB_OK;
 }
 
+int ib_dst_tiles(const ib_handle* h, int32_t* rp, int32_t* cp, int32_t* source) {
+  if (!h) return IB_E_INVAL;
+  if (rp) *rp = h->tiles.rp;
+  if (cp) *cp = h->tiles.cp;
+  if (source) *source = h->tiles.source;
+  return IB_OK;
+}
+
 int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable) {
   if (!h || kclass < 0 || kclass > 4) return IB_E_INVAL;
   API_TRY

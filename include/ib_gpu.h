@@ -195,6 +195,14 @@ int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable);
 /* the Schur form in use (1 matrix-free, 2 assembled by columns, 3 face-space
  * G), the size of the face set and how many times G has been computed */
 int ib_schur_info(ib_handle* h, int32_t* mode, int32_t* nfaces, int64_t* green_builds);
+/* the tile shapes of the handle's DST passes (no reference counterpart): rp
+ * row pairs per workgroup of the row pass, cp column pairs per workgroup of
+ * the column pass, and how they were chosen (0 fixed rule, 1 timed at
+ * ib_create once per grid shape and process, 2 forced by the environment
+ * variable IB_DST_TILES=rp,cp; IB_DST_TUNE=0 selects the fixed rule).  The
+ * tiles never change results, only speed; a choice the LDS cannot hold is
+ * halved at launch. */
+int ib_dst_tiles(const ib_handle* h, int32_t* rp, int32_t* cp, int32_t* source);
 int ib_kernel_timing_read(ib_handle* h, int32_t kclass, int64_t* launches, double* total_ms,
                           double* total_bytes);
 
@@ -249,6 +257,11 @@ int ib_batch_advance(ib_batch* b, int32_t* itimes, ib_step_info* infos);
  * the caller */
 int ib_batch_handle(ib_batch* b, int32_t i, ib_handle** h);
 int ib_batch_info(const ib_batch* b, int32_t* nsim, int32_t* mode);
+/* the DST tile rule of the fused form (no reference counterpart), timed at
+ * ib_batch_create over table launches that cover every simulation: 0 each
+ * simulation's own tiles (ib_dst_tiles), -1 the fixed rule, k > 0 k pairs
+ * per workgroup (halved where the LDS is short); row pass and column pass */
+int ib_batch_dst_tiles(const ib_batch* b, int32_t* row_rule, int32_t* col_rule);
 /* message of the last failing call on b (or of ib_batch_create when b is NULL) */
 int ib_batch_last_error(const ib_batch* b, char* buf, int32_t len);
 /* ib_adjoint_run of every simulation, each from its own host thread on its

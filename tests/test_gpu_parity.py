@@ -303,3 +303,26 @@ def test_cooperative_gmres_equals_single_workgroup_gmres(case):
     assert i1 == i2
     assert r1 < 1e-10 and r2 < 1e-10
     assert rel(x1, x2) < 1e-8
+
+
+def test_dst_tile_shapes_leave_results_unchanged(monkeypatch):
+    """The DST tile shapes (timed at create, forced, or the fixed rule) only
+    decide how many transforms share a workgroup's LDS: three steps give the
+    same bits under each."""
+    p, b = decks.tiny_case(m=64, n=48, mgridlev=2, tol=1e-12, gmres_restart=80)
+    runs = []
+    for env in ({"IB_DST_TUNE": "0"}, {}, {"IB_DST_TILES": "4,4"}, {"IB_DST_TILES": "1,2"}):
+        monkeypatch.delenv("IB_DST_TUNE", raising=False)
+        monkeypatch.delenv("IB_DST_TILES", raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        g = gpu(p, b)
+        for _ in range(3):
+            assert g.advance().status == abi.IB_OK
+        w = np.concatenate([g.level(k)["omega"] for k in range(1, g.L + 1)])
+        runs.append((g.dst_tiles(), w, g.body()["f"].copy()))
+        g.close()
+    assert [r[0][2] for r in runs] == [0, 1, 2, 2]
+    assert runs[2][0][:2] == (4, 4) and runs[3][0][:2] == (1, 2)
+    for tiles, w, f in runs[1:]:
+        assert np.array_equal(w, runs[0][1]) and np.array_equal(f, runs[0][2]), tiles
