@@ -228,18 +228,19 @@ def run_adjoint(args):
     cost_ratio = adjoint wall / forward wall."""
     from ib_parallel_amd import GpuSim, decks
     res = {}
-    p, b = decks.cylinder_case(num_stat=1)
-    sim = GpuSim(p, b)
-    for _ in range(3):
-        sim.advance()
     n = args.adjoint_steps
-    tf, (ta, tc), _ = _time_adjoint(sim, n, [1.0], [0.5], [None, 10])
-    res["rigid"] = {"workload": "rigid cylinder 200x200x3, 157 pts, Re=100, dt=4e-3, num_stat; d/dRe and d/domega0 "
-                                "of mean(drag + 0.5 lift)",
-                    "steps": n, "forward_ms_per_step": round(tf / n * 1e3, 3),
-                    "adjoint_ms_per_step": round(ta / n * 1e3, 3), "cost_ratio": round(ta / tf, 3),
-                    "adjoint_ckpt10_ms_per_step": round(tc / n * 1e3, 3), "cost_ratio_ckpt10": round(tc / tf, 3)}
-    sim.close()
+    if n > 0:
+        p, b = decks.cylinder_case(num_stat=1)
+        sim = GpuSim(p, b)
+        for _ in range(3):
+            sim.advance()
+        tf, (ta, tc), _ = _time_adjoint(sim, n, [1.0], [0.5], [None, 10])
+        res["rigid"] = {"workload": "rigid cylinder 200x200x3, 157 pts, Re=100, dt=4e-3, num_stat; d/dRe and "
+                                    "d/domega0 of mean(drag + 0.5 lift)",
+                        "steps": n, "forward_ms_per_step": round(tf / n * 1e3, 3),
+                        "adjoint_ms_per_step": round(ta / n * 1e3, 3), "cost_ratio": round(ta / tf, 3),
+                        "adjoint_ckpt10_ms_per_step": round(tc / n * 1e3, 3), "cost_ratio_ckpt10": round(tc / tf, 3)}
+        sim.close()
     if args.adjoint_fsi_steps > 0:
         p, b = decks.plate_case(mgridlev=3)
         sim = GpuSim(p, b)
@@ -391,7 +392,7 @@ def main():
     if args.pmc_copy_mb > 0:
         copy_calibration(args.pmc_copy_mb)
     if args.adjoint_only:
-        out = {"adjoint": run_adjoint(args)} if args.adjoint_steps > 0 else {}
+        out = {"adjoint": run_adjoint(args)} if args.adjoint_steps > 0 or args.adjoint_fsi_steps > 0 else {}
         if args.adjoint_batch > 0:
             out["adjoint_batch"] = run_adjoint_batch(args)
         print(json.dumps(out), flush=True)
@@ -426,7 +427,7 @@ def main():
     }
     if args.batch > 0:
         out["batched"] = run_batched(args)
-    if args.adjoint_steps > 0:
+    if args.adjoint_steps > 0 or args.adjoint_fsi_steps > 0:
         out["adjoint"] = run_adjoint(args)
     if not args.no_cpu:
         cpu = run_cpu(args, g["its"] / args.steps)
