@@ -326,3 +326,27 @@ def test_dst_tile_shapes_leave_results_unchanged(monkeypatch):
     assert runs[2][0][:2] == (4, 4) and runs[3][0][:2] == (1, 2)
     for tiles, w, f in runs[1:]:
         assert np.array_equal(w, runs[0][1]) and np.array_equal(f, runs[0][2]), tiles
+
+
+def test_forces_over_500_steps_match_the_oracle():
+    """The C0 cylinder (200x200x3, 157 points, dt 0.004, 500 steps from rest)
+    with the num_stat solve, so both sides solve exactly: at every step the
+    drag agrees to 1e-8 relative and the lift to 1e-8 max|C_D| absolute (the
+    start is symmetric, so the lift is round-off and a relative measure is
+    ill-posed)."""
+    p, b = decks.cylinder_case(num_stat=1)
+    g, o = gpu(p, b), orc(p, b)
+    cd_g, cl_g, cd_o, cl_o = [], [], [], []
+    for _ in range(500):
+        assert g.advance().status == abi.IB_OK
+        o.advance()
+        fx, fy = g.forces()
+        cd_g.append(fx[0]); cl_g.append(fy[0])
+        fx, fy = o.forces()
+        cd_o.append(fx[0]); cl_o.append(fy[0])
+    cd_g, cl_g, cd_o, cl_o = map(np.array, (cd_g, cl_g, cd_o, cl_o))
+    err_cd = np.max(np.abs(cd_g - cd_o) / np.abs(cd_o))
+    err_cl = np.max(np.abs(cl_g - cl_o)) / np.max(np.abs(cd_o))
+    print(f"500 steps: drag rel {err_cd:.2e}, lift abs / max|C_D| {err_cl:.2e}")
+    assert err_cd <= 1e-8 and err_cl <= 1e-8
+    g.close()
