@@ -395,6 +395,19 @@ struct ib_batch {
     bool all = true;
     for (ib_handle* h : sims) all = all && fusable(h);
     mode = all ? 1 : 2;
+    if (mode == 2) {
+      // the simulations run at once, and each cooperative GMRES solve keeps
+      // its P workgroups on the CUs of one XCD: when together they need more
+      // than one XCD's CUs, a solve could wait (up to its bounded spin) for
+      // workgroups that cannot be placed, so every simulation uses the
+      // one-workgroup solve instead
+      int ncu = 0;
+      if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, sims[0]->dev) != hipSuccess) ncu = 0;
+      long need = 0;
+      for (ib_handle* h : sims) need += h->coopP;
+      if (need > ncu / 8)
+        for (ib_handle* h : sims) h->coopP = h->coopR = 0;
+    }
     direct = params[0].launch_mode == 1;
     if (mode == 1) build_fused();
     return IB_OK;

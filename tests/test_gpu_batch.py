@@ -99,6 +99,26 @@ def test_concurrent_batch_equals_plain_steps():
     batch.close()
 
 
+def test_crowded_concurrent_batch_uses_one_workgroup_solves():
+    """17 concurrent torsional simulations would keep 34 cooperative GMRES
+    workgroups on the 32 CUs of one XCD: the batch switches every simulation
+    to the one-workgroup solve, so each equals its plain step with
+    gmres_form = 1 bit for bit."""
+    from ib_parallel_amd import GpuBatch, GpuSim
+    cases = [decks.tiny_case(m=32, n=24, mgridlev=2, tol=1e-12, re=50.0 + i) for i in range(17)]
+    batch = GpuBatch(cases)
+    assert batch.mode == 2
+    plain = [GpuSim(*decks.tiny_case(m=32, n=24, mgridlev=2, tol=1e-12, re=50.0 + i, gmres_form=1))
+             for i in range(17)]
+    for _ in range(2):
+        infos = batch.advance()
+        refs = [s.advance() for s in plain]
+        for i in range(17):
+            assert (infos[i].ksp_iter, infos[i].fsi_iter) == (refs[i].ksp_iter, refs[i].fsi_iter), i
+            assert same(state(batch.sims[i]), state(plain[i])), i
+    batch.close()
+
+
 def test_batch_create_errors():
     from ib_parallel_amd import GpuBatch, IBError, abi
     good = decks.ellipse_batch_cases(count=1, seed=1, **SMALL)[0]
