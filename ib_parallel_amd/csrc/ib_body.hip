@@ -401,7 +401,10 @@ void map_taps(hipStream_t st, int nf, const int* key, const int* fmap, int* tapi
   IB_LAUNCH_CHECK();
 }
 
-// T(:, j) = sum_q w[j,q] G(:, tapidx[j,q]): one thread per face row b
+// T(:, j) = sum_q w[j,q] G(:, tapidx[j,q]): one thread per face row b.  A
+// tap outside the face set (tapidx -1, which k_map_taps flags) reads face 0:
+// the FSI loop forms S before it has read that flag and forms it again
+// after rebuilding the set.
 __global__ void __launch_bounds__(256) k_green_right(int nF, const double* __restrict__ w,
                                                      const int* __restrict__ tapidx, const double* __restrict__ G,
                                                      double* __restrict__ T) {
@@ -409,7 +412,7 @@ __global__ void __launch_bounds__(256) k_green_right(int nF, const double* __res
   const int j = blockIdx.y;
   if (b >= nF) return;
   double acc = 0.0;
-  for (int q = 0; q < kTaps; ++q) acc += w[j * kTaps + q] * G[(long)tapidx[j * kTaps + q] * nF + b];
+  for (int q = 0; q < kTaps; ++q) acc += w[j * kTaps + q] * G[(long)max(tapidx[j * kTaps + q], 0) * nF + b];
   T[(long)j * nF + b] = acc;
 }
 
@@ -424,7 +427,7 @@ __global__ void __launch_bounds__(256) k_green_left(int nf, int nF, const double
   if (i >= nf) return;
   const double* Tj = T + (long)j * nF;
   double acc = 0.0;
-  for (int p = 0; p < kTaps; ++p) acc += w[i * kTaps + p] * Tj[tapidx[i * kTaps + p]];
+  for (int p = 0; p < kTaps; ++p) acc += w[i * kTaps + p] * Tj[max(tapidx[i * kTaps + p], 0)];
   S[(long)j * nf + i] = 1.0 * acc + btimes_entry(i, j, tors_of, tbs, theta, dt, delta);
 }
 

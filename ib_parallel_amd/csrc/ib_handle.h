@@ -913,7 +913,13 @@ struct ib_handle {
         IB_HIP(hipMemcpyAsync(hflag, err_flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
         IB_HIP(hipMemcpyAsync(hflag + 1, face_flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
         IB_HIP(hipMemcpyAsync(hst, gst, sizeof(GmresDev), hipMemcpyDeviceToHost, st));
-        sync();
+        // S for the new weights goes on the stream before the host waits for
+        // the flags, so it runs while the host wakes up; it is formed again
+        // below if a tap has left the face set
+        IB_HIP(hipEventRecord(ev_flags, st));
+        schur_from_green(st, nf, nF, w, tapidx, Gm, Tm, tors_of, tb, th, dt, delta, Sd);
+        IB_HIP(hipEventSynchronize(ev_flags));
+        for (auto& t : tim) t.harvest();  // the timed launches all precede the event
         if (hflag[2]) csr.build(st);
         rc = dense_check();
         its = hst->its;
@@ -925,11 +931,11 @@ struct ib_handle {
           return IB_E_INVAL;
         }
         if (hflag[1]) {
-          build_face_set();
+          build_face_set();  // synchronises the stream before G is reallocated
           compute_green();
           map_taps(st, nf, csr.key, fmap, tapidx, face_flag);
+          schur_from_green(st, nf, nF, w, tapidx, Gm, Tm, tors_of, tb, th, dt, delta, Sd);
         }
-        schur_from_green(st, nf, nF, w, tapidx, Gm, Tm, tors_of, tb, th, dt, delta, Sd);
         err_glob = 0.0;
         for (int t = 0; t < ntors; ++t) err_glob = std::max(err_glob, hbuf[t]);
         continue;
@@ -1142,6 +1148,7 @@ struct ib_handle {
   // launch_mode 1 and while kernel timing records per-launch events; the
   // adjoint's tape copies go between the graphs.
   std::map<int, hipGraphExec_t> step_graphs;
+  hipEvent_t ev_flags = nullptr;  // the FSI loop's status copies (one_sync path)
   // set for the simulations of a batch, which run on several host threads
   // at once: no stream capture, so that none is open while another thread
   // allocates or copies synchronously
@@ -1344,6 +1351,7 @@ struct ib_handle {
 
     IB_HIP(hipGetDevice(&dev));
     IB_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    IB_HIP(hipEventCreateWithFlags(&ev_flags, hipEventDisableTiming));
     int lds = 0, optin = 0;
     IB_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess) optin = 0;
@@ -1512,6 +1520,7 @@ struct ib_handle {
     if (st) (void)hipStreamSynchronize(st);
     for (auto& kv : gm_graphs) (void)hipGraphExecDestroy(kv.second);
     for (auto& kv : step_graphs) (void)hipGraphExecDestroy(kv.second);
+    if (ev_flags) (void)hipEventDestroy(ev_flags);
     for (auto& t : tim) t.destroy();
     csr.release();
     px.release();
