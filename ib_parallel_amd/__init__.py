@@ -65,7 +65,8 @@ class GpuSim(Simulation):
         dJ/dgamma0 per body (torsional stiffness and pitch angle in radians;
         0 for rigid bodies).  ckpt_every None: the whole run is taped in HBM
         when it fits, else checkpoints every ceil(sqrt(nsteps)) steps.  Rigid
-        bodies with num_stat = 1, or the face-space form (moving bodies)."""
+        bodies with num_stat = 1 or GMRES on the assembled Schur matrix (no
+        position gradients), or the face-space form (moving bodies)."""
         cx = np.ascontiguousarray(np.broadcast_to(np.asarray(cx, dtype=np.float64), (self.nbodies,)))
         cy = np.ascontiguousarray(np.broadcast_to(np.asarray(cy, dtype=np.float64), (self.nbodies,)))
         K = int(ckpt_every) if ckpt_every else 0
