@@ -274,7 +274,7 @@ int ib_handle::taped_advance(int sl) {
 // adjoints of its outputs; on return wb is the adjoint of its input
 // omega, and qxb, qyb are zero
 void ib_handle::v2f_adj(double* wb, double* qxb, double* qyb) {
-  const long P = g.P, nbc = g.nbc;
+  const long P = g.P;
   fill(st, asb, (long)L * P, 0.0);
   for (int k = 0; k < L; ++k) {
     double* qxk = qxb + (long)k * g.Fx;
