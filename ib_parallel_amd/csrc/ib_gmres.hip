@@ -1273,73 +1273,81 @@ __global__ void __launch_bounds__(1024) k_gm_orth(int nf, int kmax, double* __re
 }
 
 // Givens update of the new Hessenberg column (KSPGMRESUpdateHessenberg) by
-// one wavefront: tt from the partial norms (block order), the previous
+// wavefront 0: tt from the partial norms (block order), the previous
 // rotations as an affine scan, the new rotation, residual estimate and
-// stopping tests; advances the status to the next iteration.
-__global__ void __launch_bounds__(64) k_gm_givens(int kmax, GmresDev* s, double* __restrict__ H,
-                                                  double* __restrict__ cc, double* __restrict__ ss,
-                                                  double* __restrict__ grs, const double* __restrict__ pnorm,
-                                                  int nblk) {
-  if (s->reason) return;
-  const int lane = threadIdx.x;
-  const int it = s->it, ld = kmax + 1;
-  double* hcol = H + (long)it * ld;
-  double a = 0.0;
-  for (int b = 0; b < nblk; ++b) a += pnorm[b];
-  const double tt = sqrt(a);
-  double carry = hcol[0];
-  for (int base = 0; base < it; base += 64) {
-    const int j = base + lane;
-    const bool act = j < it;
-    const double c = act ? cc[j] : 1.0, sn = act ? ss[j] : 0.0, hn = act ? hcol[j + 1] : 0.0;
-    double A = -sn, B = c * hn;
-    gm_affine_scan(A, B);
-    const double cnext = A * carry + B;            // carry_(j+1)
-    const double cj = gm_shift_up1(cnext, carry);  // carry_j (lane 0: the carry in)
-    if (act) hcol[j] = c * cj + sn * hn;
-    carry = gm_readlane(cnext, min(63, it - 1 - base));
-  }
-  if (lane == 0) {
-    hcol[it] = carry;
-    hcol[it + 1] = tt;
-    double hapbnd = fabs(tt / grs[it]);
-    if (hapbnd > 1e-30) hapbnd = 1e-30;
-    const int hap = tt < hapbnd;
-    double res;
-    if (!hap) {
-      const double t1 = sqrt(hcol[it] * hcol[it] + hcol[it + 1] * hcol[it + 1]);
-      const double c = hcol[it] / t1, sn = hcol[it + 1] / t1;
-      cc[it] = c;
-      ss[it] = sn;
-      grs[it + 1] = -sn * grs[it];
-      grs[it] = c * grs[it];
-      hcol[it] = c * hcol[it] + sn * hcol[it + 1];
-      res = fabs(grs[it + 1]);
+// stopping tests; advances the status to the next iteration.  Then the
+// whole workgroup forms v_(it+1) = w / tt (skipped after a happy breakdown;
+// a vector closed at convergence is never used).  Only wavefront 0 reads the
+// status: the others take what they need from LDS after the barrier, since
+// wavefront 0 rewrites the status while they may not have started.
+__global__ void __launch_bounds__(256) k_gm_givens(int kmax, int nf, double* __restrict__ V, GmresDev* s,
+                                                   double* __restrict__ H, double* __restrict__ cc,
+                                                   double* __restrict__ ss, double* __restrict__ grs,
+                                                   const double* __restrict__ pnorm, int nblk) {
+  __shared__ double sh_tt;
+  __shared__ int sh_go, sh_col;
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    if (s->reason) {
+      if (lane == 0) sh_go = 0;
     } else {
-      res = 0.0;
+      const int it = s->it, ld = kmax + 1;
+      double* hcol = H + (long)it * ld;
+      double a = 0.0;
+      for (int b = 0; b < nblk; ++b) a += pnorm[b];
+      const double tt = sqrt(a);
+      double carry = hcol[0];
+      for (int base = 0; base < it; base += 64) {
+        const int j = base + lane;
+        const bool act = j < it;
+        const double c = act ? cc[j] : 1.0, sn = act ? ss[j] : 0.0, hn = act ? hcol[j + 1] : 0.0;
+        double A = -sn, B = c * hn;
+        gm_affine_scan(A, B);
+        const double cnext = A * carry + B;            // carry_(j+1)
+        const double cj = gm_shift_up1(cnext, carry);  // carry_j (lane 0: the carry in)
+        if (act) hcol[j] = c * cj + sn * hn;
+        carry = gm_readlane(cnext, min(63, it - 1 - base));
+      }
+      if (lane == 0) {
+        hcol[it] = carry;
+        hcol[it + 1] = tt;
+        double hapbnd = fabs(tt / grs[it]);
+        if (hapbnd > 1e-30) hapbnd = 1e-30;
+        const int hap = tt < hapbnd;
+        double res;
+        if (!hap) {
+          const double t1 = sqrt(hcol[it] * hcol[it] + hcol[it + 1] * hcol[it + 1]);
+          const double c = hcol[it] / t1, sn = hcol[it + 1] / t1;
+          cc[it] = c;
+          ss[it] = sn;
+          grs[it + 1] = -sn * grs[it];
+          grs[it] = c * grs[it];
+          hcol[it] = c * hcol[it] + sn * hcol[it + 1];
+          res = fabs(grs[it + 1]);
+        } else {
+          res = 0.0;
+        }
+        s->tt = tt;
+        s->hap = hap;
+        s->it = it + 1;
+        s->its = s->its + 1;
+        s->res = res;
+        int reason = 0;
+        if (res <= s->ttol) reason = 1;
+        else if (res >= 1e4 * s->rn0) reason = -1;
+        if (hap && !reason) reason = 1;
+        s->reason = reason;
+        sh_go = !reason && !hap;
+        sh_col = it + 1;
+        sh_tt = tt;
+      }
     }
-    s->tt = tt;
-    s->hap = hap;
-    s->it = it + 1;
-    s->its = s->its + 1;
-    s->res = res;
-    int reason = 0;
-    if (res <= s->ttol) reason = 1;
-    else if (res >= 1e4 * s->rn0) reason = -1;
-    if (hap && !reason) reason = 1;
-    s->reason = reason;
   }
-}
-
-// v_it = w / tt for the basis vector k_gm_givens has just closed (skipped
-// after a happy breakdown; a vector closed at convergence is never used)
-__global__ void __launch_bounds__(256) k_gm_scale(int nf, double* __restrict__ V, const GmresDev* __restrict__ s) {
-  if (s->reason || s->hap) return;
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nf) return;
-  const double inv = 1.0 / s->tt;
-  double* v = V + (long)s->it * nf;
-  v[r] = v[r] * inv;
+  __syncthreads();
+  if (!sh_go) return;
+  const double inv = 1.0 / sh_tt;
+  double* v = V + (long)sh_col * nf;
+  for (int r = threadIdx.x; r < nf; r += blockDim.x) v[r] = v[r] * inv;
 }
 
 void gm_start(hipStream_t st, int nf, int kmax, const double* b, const double* Ax, double* V, GmresDev* s, double* H,
@@ -1358,9 +1366,7 @@ void gm_arnoldi(hipStream_t st, int nf, int kmax, double* V, GmresDev* s, double
   const int nblk = ceil_div(nf, 64);
   hipLaunchKernelGGL(k_gm_orth, dim3(nblk), dim3(1024), 0, st, nf, kmax, V, s, H, pnorm);
   IB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(64), 0, st, kmax, s, H, cc, ss, grs, pnorm, nblk);
-  IB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_gm_scale, dim3(ceil_div(nf, 256)), dim3(256), 0, st, nf, V, s);
+  hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, st, kmax, nf, V, s, H, cc, ss, grs, pnorm, nblk);
   IB_LAUNCH_CHECK();
 }
 
