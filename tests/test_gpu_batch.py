@@ -44,6 +44,30 @@ def test_fused_batch_equals_plain_steps():
     batch.close()
 
 
+def test_fused_batch_tile_rules_leave_results_unchanged(monkeypatch):
+    """The fused batch's DST tile rules (timed over its table launches, or
+    the fixed rule, or forced) change only how transforms share a
+    workgroup: three batched steps give the same bits under each."""
+    from ib_parallel_amd import GpuBatch
+    cases = decks.ellipse_batch_cases(count=4, seed=11, **SMALL)
+    runs = []
+    for env in ({"IB_DST_TUNE": "0"}, {}, {"IB_DST_TILES": "4,4"}):
+        monkeypatch.delenv("IB_DST_TUNE", raising=False)
+        monkeypatch.delenv("IB_DST_TILES", raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        batch = GpuBatch(cases)
+        assert batch.mode == 1
+        for _ in range(3):
+            batch.advance()
+        runs.append((batch.dst_tiles(), [state(s) for s in batch.sims]))
+        batch.close()
+    assert runs[0][0] == (0, 0) and runs[2][0] == (0, 0)  # the environment's choice stays with the handles
+    for rules, st in runs[1:]:
+        for i in range(len(cases)):
+            assert same(st[i], runs[0][1][i]), (rules, i)
+
+
 def test_fused_batch_matches_oracle():
     from ib_parallel_amd import GpuBatch
     from oracle import binding as ob
