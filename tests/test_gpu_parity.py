@@ -350,3 +350,33 @@ def test_forces_over_500_steps_match_the_oracle():
     print(f"500 steps: drag rel {err_cd:.2e}, lift abs / max|C_D| {err_cl:.2e}")
     assert err_cd <= 1e-8 and err_cl <= 1e-8
     g.close()
+
+
+def test_step_without_bodies_matches_oracle():
+    """No bodies (nf = 0: no Schur solve, no FSI loop): three steps from a
+    smooth vortex field on every level agree with the oracle to 1e-10."""
+    p = abi.make_params(48, 40, 3, 0.01, 100.0, 4.0, 1.6, 1.5)
+    g, o = gpu(p, []), orc(p, [])
+    P, L = g.P, g.L
+    i, j = np.meshgrid(np.arange(1, 48), np.arange(1, 40), indexing="xy")
+    blob = (np.exp(-((i - 20.0) ** 2 + (j - 18.0) ** 2) / 30.0) * 1e-3).ravel()
+    w0 = np.concatenate([blob * 0.5 ** k for k in range(L)])
+    for sim in (g, o):
+        w, s_, qx, qy = g.vort2flux(w0)
+        for k in range(L):
+            sim.set_level(k + 1, omega=w[k * P:(k + 1) * P], s=s_[k * P:(k + 1) * P],
+                          qx=qx[k * g.Fx:(k + 1) * g.Fx], qy=qy[k * g.Fy:(k + 1) * g.Fy])
+    for _ in range(3):
+        assert g.advance().status == abi.IB_OK
+        o.advance()
+    for k in range(1, L + 1):
+        assert rel(g.level(k)["omega"], o.level(k)["omega"]) < 1e-10, k
+
+
+def test_grid_beyond_the_dst_size_is_rejected():
+    """A transform that does not fit one workgroup's LDS (2m = 8192 complex
+    points) is refused at create with IB_E_UNSUPPORTED, not launched."""
+    from ib_parallel_amd import IBError
+    with pytest.raises(IBError) as e:
+        gpu(abi.make_params(4096, 64, 1, 0.01, 100.0, 1.0, 0.5, 0.5), [])
+    assert e.value.code == abi.IB_E_UNSUPPORTED
