@@ -98,6 +98,9 @@ struct ib_batch {
   // k > 0 k pairs (halved where the LDS is short).  Tiles never change
   // results.
   int xrule = 0, yrule = 0;
+  // the fused DST launches use the kernels with the generic-radix stage only
+  // when some simulation's row (genx) or column (geny) plan needs it
+  bool genx = false, geny = false;
   int xtile(const ib_handle* h) const {
     return xrule > 0 ? xrule : (xrule < 0 ? (h->px.dev.L <= 512 ? 2 : 1) : h->tiles.rp);
   }
@@ -165,7 +168,7 @@ struct ib_batch {
         es.push_back(xe(h, a, &t));
         l = std::max(l, t);
       }
-      const float ms = time_tab(dst_x_tab_kernel(X_PLAIN), es, l);
+      const float ms = time_tab(dst_x_tab_kernel(X_PLAIN, genx), es, l);
       if (ms < best) {
         best = ms;
         bx = r;
@@ -184,7 +187,7 @@ struct ib_batch {
         es.push_back(ye(h, Y_SCALE2, a, &t));
         l = std::max(l, t);
       }
-      const float ms = time_tab(dst_y_tab_kernel(Y_SCALE2), es, l);
+      const float ms = time_tab(dst_y_tab_kernel(Y_SCALE2, geny), es, l);
       if (ms < best) {
         best = ms;
         by = r;
@@ -233,9 +236,9 @@ struct ib_batch {
                                       h->qy + (long)k * h->g.Fy, bout));
         sbc[i] = bout;
       }
-      plan.add(dst_x_tab_kernel(X_APPLYBC), x1, l1);
-      plan.add(dst_y_tab_kernel(Y_SCALE2), y1, l2);
-      plan.add(dst_x_tab_kernel(X_PLAIN), x2, l3);
+      plan.add(dst_x_tab_kernel(X_APPLYBC, genx), x1, l1);
+      plan.add(dst_y_tab_kernel(Y_SCALE2, geny), y1, l2);
+      plan.add(dst_x_tab_kernel(X_PLAIN, genx), x2, l3);
       plan.add(fluid_tab_kernel(TK_CURL_GETBC), cu);
     }
   }
@@ -291,9 +294,9 @@ struct ib_batch {
       }
       plan.add(fluid_tab_kernel(TK_GET_BC), gb);
       plan.add(fluid_tab_kernel(TK_NL_RHS), nl);
-      plan.add(dst_x_tab_kernel(X_PLAIN), x1, l1);
-      plan.add(dst_y_tab_kernel(Y_PRED), y1, l2);
-      plan.add(dst_x_tab_kernel(X_PLAIN), x2, l3);
+      plan.add(dst_x_tab_kernel(X_PLAIN, genx), x1, l1);
+      plan.add(dst_y_tab_kernel(Y_PRED, geny), y1, l2);
+      plan.add(dst_x_tab_kernel(X_PLAIN, genx), x2, l3);
     }
     plan_vort2flux(Lmax);
     // step 2, num_stat (src/timestepper.F90:457-484): rhsf = regT(q + q0),
@@ -333,9 +336,9 @@ struct ib_batch {
         c.in[0] = h->t1; c.out[0] = h->omega;
         x2.push_back(xe(h, c, &t)); l3 = std::max(l3, t);
       }
-      plan.add(dst_x_tab_kernel(X_ROT), x1, l1);
-      plan.add(dst_y_tab_kernel(Y_SCALE2), y1, l2);
-      plan.add(dst_x_tab_kernel(X_SUBFROM), x2, l3);
+      plan.add(dst_x_tab_kernel(X_ROT, genx), x1, l1);
+      plan.add(dst_y_tab_kernel(Y_SCALE2, geny), y1, l2);
+      plan.add(dst_x_tab_kernel(X_SUBFROM, genx), x2, l3);
     }
     plan_vort2flux(Lmax);
   }
@@ -349,6 +352,11 @@ struct ib_batch {
   }
 
   void build_fused() {
+    genx = geny = false;
+    for (const ib_handle* h : sims) {
+      genx = genx || fft_has_generic(h->px.dev);
+      geny = geny || fft_has_generic(h->py.dev);
+    }
     tune_tiles();
     plan_step();
     arena.alloc(plan.bytes.size());

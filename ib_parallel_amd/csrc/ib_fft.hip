@@ -230,7 +230,10 @@ __device__ void stage_generic(double2* __restrict__ in, double2* __restrict__ ou
 
 // Runs the plan on nf transforms held in a (scratch b) with the twiddle table
 // W staged in LDS; returns the buffer that holds the result.  Caller
-// synchronises before the call.
+// synchronises before the call.  GEN = false leaves out the stage of radix
+// >= 7 (stage_generic, whose registers set the kernel's occupancy): the host
+// launches that form only for plans whose radices are all 2, 3, 4, 5, 8.
+template <bool GEN>
 __device__ __forceinline__ double2* fft_run(double2* a, double2* b, int nf, const FFTPlan& pl, const double2* W) {
   int Ns = 1;
   for (int st = 0; st < pl.nst; ++st) {
@@ -241,7 +244,9 @@ __device__ __forceinline__ double2* fft_run(double2* a, double2* b, int nf, cons
       case 2: stage_fixed<2>(a, b, nf, pl.L, Ns, W); break;
       case 3: stage_fixed<3>(a, b, nf, pl.L, Ns, W); break;
       case 5: stage_fixed<5>(a, b, nf, pl.L, Ns, W); break;
-      default: stage_generic(a, b, nf, pl.L, Ns, R, W); break;
+      default:
+        if constexpr (GEN) stage_generic(a, b, nf, pl.L, Ns, R, W);
+        break;
     }
     __syncthreads();
     double2* t = a;
@@ -278,7 +283,7 @@ __device__ __forceinline__ void load_twiddles(double2* Wl, const FFTPlan& pl) {
 }
 
 // Row pass.  Rows j0 = 0..n-2 (node j = j0+2) of length N = m-1.
-template <int MODE>
+template <int MODE, bool GEN>
 __device__ __forceinline__ void dst_x_tile(const XArgs& a, int bx, int by) {
   extern __shared__ double2 sm[];
   const FFTPlan& pl = a.pl;
@@ -369,7 +374,7 @@ __device__ __forceinline__ void dst_x_tile(const XArgs& a, int bx, int by) {
     A[s * L + M] = make_double2(0.0, 0.0);
   }
   __syncthreads();
-  const double2* Z = fft_run(A, B, rp, pl, Wl);
+  const double2* Z = fft_run<GEN>(A, B, rp, pl, Wl);
   for (int idx = threadIdx.x; idx < rp * N; idx += blockDim.x) {
     const int s = idx / N;
     const int t = idx - s * N;
@@ -386,7 +391,7 @@ __device__ __forceinline__ void dst_x_tile(const XArgs& a, int bx, int by) {
 }
 
 // Column pass.  Columns i0 = 0..m-2 (stride m-1) of length N = n-1.
-template <int MODE>
+template <int MODE, bool GEN>
 __device__ __forceinline__ void dst_y_tile(const YArgs& a, int bx, int by) {
   extern __shared__ double2 sm[];
   const FFTPlan& pl = a.pl;
@@ -433,7 +438,7 @@ __device__ __forceinline__ void dst_y_tile(const YArgs& a, int bx, int by) {
     A[s * L + M] = make_double2(0.0, 0.0);
   }
   __syncthreads();
-  double2* Z = fft_run(A, B, nfft, pl, Wl);
+  double2* Z = fft_run<GEN>(A, B, nfft, pl, Wl);
   if (MODE == Y_SCALE2 || MODE == Y_PRED) {
     double2* O = (Z == A) ? B : A;
     (void)O;
@@ -462,7 +467,7 @@ __device__ __forceinline__ void dst_y_tile(const YArgs& a, int bx, int by) {
     }
     __syncthreads();
     double2* other = (Z == A) ? B : A;
-    Z = fft_run(Z, other, cp, pl, Wl);
+    Z = fft_run<GEN>(Z, other, cp, pl, Wl);
   }
   for (int idx = threadIdx.x; idx < N * w2; idx += blockDim.x) {
     const int t = idx / w2;
@@ -476,23 +481,23 @@ __device__ __forceinline__ void dst_y_tile(const YArgs& a, int bx, int by) {
   }
 }
 
-template <int MODE>
+template <int MODE, bool GEN>
 __global__ void __launch_bounds__(256) k_dst_x(XArgs a) {
-  dst_x_tile<MODE>(a, blockIdx.x, blockIdx.y);
+  dst_x_tile<MODE, GEN>(a, blockIdx.x, blockIdx.y);
 }
-template <int MODE>
+template <int MODE, bool GEN>
 __global__ void __launch_bounds__(256) k_dst_y(YArgs a) {
-  dst_y_tile<MODE>(a, blockIdx.x, blockIdx.y);
+  dst_y_tile<MODE, GEN>(a, blockIdx.x, blockIdx.y);
 }
-template <int MODE>
+template <int MODE, bool GEN>
 struct DstXOp {
   using Args = XArgs;
-  static __device__ __forceinline__ void tile(const XArgs& a, int bx, int by, int) { dst_x_tile<MODE>(a, bx, by); }
+  static __device__ __forceinline__ void tile(const XArgs& a, int bx, int by, int) { dst_x_tile<MODE, GEN>(a, bx, by); }
 };
-template <int MODE>
+template <int MODE, bool GEN>
 struct DstYOp {
   using Args = YArgs;
-  static __device__ __forceinline__ void tile(const YArgs& a, int bx, int by, int) { dst_y_tile<MODE>(a, bx, by); }
+  static __device__ __forceinline__ void tile(const YArgs& a, int bx, int by, int) { dst_y_tile<MODE, GEN>(a, bx, by); }
 };
 
 // ------------------------------------------------------------------------
@@ -540,33 +545,75 @@ void HostFFTPlan::release() {
 
 size_t dst_lds_bytes(const FFTPlan& pl, int nfft) { return (size_t)(2 * nfft + 1) * pl.L * sizeof(double2); }
 
-void dst_set_max_lds(size_t bytes) {
-  const void* fns[] = {(const void*)k_dst_x<X_PLAIN>, (const void*)k_dst_x<X_APPLYBC>,
-                       (const void*)k_dst_x<X_SUBFROM>, (const void*)k_dst_x<X_ROT>,
-                       (const void*)k_dst_x<X_POINTROT>,
-                       (const void*)k_dst_y<Y_PLAIN>,
-                       (const void*)k_dst_y<Y_SCALE2>, (const void*)k_dst_y<Y_PRED>,
-                       dst_x_tab_kernel(X_PLAIN), dst_x_tab_kernel(X_APPLYBC), dst_x_tab_kernel(X_SUBFROM),
-                       dst_x_tab_kernel(X_ROT), dst_y_tab_kernel(Y_SCALE2), dst_y_tab_kernel(Y_PRED)};
-  for (const void* f : fns) IB_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+bool fft_has_generic(const FFTPlan& pl) {
+  for (int st = 0; st < pl.nst; ++st) {
+    switch (pl.rad[st]) {
+      case 8: case 4: case 2: case 3: case 5: break;  // the fixed radices of fft_run
+      default: return true;
+    }
+  }
+  return false;
 }
 
-const void* dst_x_tab_kernel(int mode) {
+namespace {
+template <bool GEN>
+const void* dst_x_kernel_t(int mode) {
   switch (mode) {
-    case X_PLAIN: return (const void*)k_tab<DstXOp<X_PLAIN>>;
-    case X_APPLYBC: return (const void*)k_tab<DstXOp<X_APPLYBC>>;
-    case X_SUBFROM: return (const void*)k_tab<DstXOp<X_SUBFROM>>;
-    case X_ROT: return (const void*)k_tab<DstXOp<X_ROT>>;
+    case X_PLAIN: return (const void*)k_dst_x<X_PLAIN, GEN>;
+    case X_APPLYBC: return (const void*)k_dst_x<X_APPLYBC, GEN>;
+    case X_SUBFROM: return (const void*)k_dst_x<X_SUBFROM, GEN>;
+    case X_ROT: return (const void*)k_dst_x<X_ROT, GEN>;
+    case X_POINTROT: return (const void*)k_dst_x<X_POINTROT, GEN>;
+    default: throw HipError("bad DST row mode");
+  }
+}
+template <bool GEN>
+const void* dst_y_kernel_t(int mode) {
+  switch (mode) {
+    case Y_PLAIN: return (const void*)k_dst_y<Y_PLAIN, GEN>;
+    case Y_SCALE2: return (const void*)k_dst_y<Y_SCALE2, GEN>;
+    case Y_PRED: return (const void*)k_dst_y<Y_PRED, GEN>;
+    default: throw HipError("bad DST column mode");
+  }
+}
+template <bool GEN>
+const void* dst_x_tab_kernel_t(int mode) {
+  switch (mode) {
+    case X_PLAIN: return (const void*)k_tab<DstXOp<X_PLAIN, GEN>>;
+    case X_APPLYBC: return (const void*)k_tab<DstXOp<X_APPLYBC, GEN>>;
+    case X_SUBFROM: return (const void*)k_tab<DstXOp<X_SUBFROM, GEN>>;
+    case X_ROT: return (const void*)k_tab<DstXOp<X_ROT, GEN>>;
     default: throw HipError("DST row pass: no table kernel for this mode");
   }
 }
-
-const void* dst_y_tab_kernel(int mode) {
+template <bool GEN>
+const void* dst_y_tab_kernel_t(int mode) {
   switch (mode) {
-    case Y_SCALE2: return (const void*)k_tab<DstYOp<Y_SCALE2>>;
-    case Y_PRED: return (const void*)k_tab<DstYOp<Y_PRED>>;
+    case Y_SCALE2: return (const void*)k_tab<DstYOp<Y_SCALE2, GEN>>;
+    case Y_PRED: return (const void*)k_tab<DstYOp<Y_PRED, GEN>>;
     default: throw HipError("DST column pass: no table kernel for this mode");
   }
+}
+const void* dst_x_kernel(int mode, bool gen) { return gen ? dst_x_kernel_t<true>(mode) : dst_x_kernel_t<false>(mode); }
+const void* dst_y_kernel(int mode, bool gen) { return gen ? dst_y_kernel_t<true>(mode) : dst_y_kernel_t<false>(mode); }
+}  // namespace
+
+const void* dst_x_tab_kernel(int mode, bool gen) {
+  return gen ? dst_x_tab_kernel_t<true>(mode) : dst_x_tab_kernel_t<false>(mode);
+}
+const void* dst_y_tab_kernel(int mode, bool gen) {
+  return gen ? dst_y_tab_kernel_t<true>(mode) : dst_y_tab_kernel_t<false>(mode);
+}
+
+void dst_set_max_lds(size_t bytes) {
+  std::vector<const void*> fns;
+  for (bool gen : {false, true}) {
+    for (int mode : {X_PLAIN, X_APPLYBC, X_SUBFROM, X_ROT, X_POINTROT}) fns.push_back(dst_x_kernel(mode, gen));
+    for (int mode : {Y_PLAIN, Y_SCALE2, Y_PRED}) fns.push_back(dst_y_kernel(mode, gen));
+    for (int mode : {X_PLAIN, X_APPLYBC, X_SUBFROM, X_ROT}) fns.push_back(dst_x_tab_kernel(mode, gen));
+    for (int mode : {Y_SCALE2, Y_PRED}) fns.push_back(dst_y_tab_kernel(mode, gen));
+  }
+  for (const void* f : fns) IB_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
 TabEntry<XArgs> dst_x_entry(const FFTPlan& pl, const DstLaunch& l, const Geom& g, int rp, size_t max_lds,
@@ -640,15 +687,8 @@ void dst_x(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, cons
            size_t max_lds) {
   size_t lds = 0;
   const TabEntry<XArgs> e = dst_x_entry(pl, l, g, rp, max_lds, &lds);
-  const dim3 grid(e.nbx, e.nby);
-  switch (mode) {
-    case X_PLAIN: hipLaunchKernelGGL(k_dst_x<X_PLAIN>, grid, dim3(256), lds, st, e.a); break;
-    case X_APPLYBC: hipLaunchKernelGGL(k_dst_x<X_APPLYBC>, grid, dim3(256), lds, st, e.a); break;
-    case X_SUBFROM: hipLaunchKernelGGL(k_dst_x<X_SUBFROM>, grid, dim3(256), lds, st, e.a); break;
-    case X_ROT: hipLaunchKernelGGL(k_dst_x<X_ROT>, grid, dim3(256), lds, st, e.a); break;
-    case X_POINTROT: hipLaunchKernelGGL(k_dst_x<X_POINTROT>, grid, dim3(256), lds, st, e.a); break;
-    default: throw HipError("bad DST row mode");
-  }
+  void* args[] = {(void*)&e.a};
+  IB_HIP(hipLaunchKernel(dst_x_kernel(mode, fft_has_generic(pl)), dim3(e.nbx, e.nby), dim3(256), args, lds, st));
   IB_LAUNCH_CHECK();
 }
 
@@ -656,13 +696,8 @@ void dst_y(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, cons
            size_t max_lds) {
   size_t lds = 0;
   const TabEntry<YArgs> e = dst_y_entry(pl, mode, l, g, cp, max_lds, &lds);
-  const dim3 grid(e.nbx, e.nby);
-  switch (mode) {
-    case Y_PLAIN: hipLaunchKernelGGL(k_dst_y<Y_PLAIN>, grid, dim3(256), lds, st, e.a); break;
-    case Y_SCALE2: hipLaunchKernelGGL(k_dst_y<Y_SCALE2>, grid, dim3(256), lds, st, e.a); break;
-    case Y_PRED: hipLaunchKernelGGL(k_dst_y<Y_PRED>, grid, dim3(256), lds, st, e.a); break;
-    default: throw HipError("bad DST column mode");
-  }
+  void* args[] = {(void*)&e.a};
+  IB_HIP(hipLaunchKernel(dst_y_kernel(mode, fft_has_generic(pl)), dim3(e.nbx, e.nby), dim3(256), args, lds, st));
   IB_LAUNCH_CHECK();
 }
 
