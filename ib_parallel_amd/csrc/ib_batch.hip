@@ -98,9 +98,9 @@ struct ib_batch {
   // k > 0 k pairs (halved where the LDS is short).  Tiles never change
   // results.
   int xrule = 0, yrule = 0;
-  // the fused DST launches use the kernels with the generic-radix stage only
-  // when some simulation's row (genx) or column (geny) plan needs it
-  bool genx = false, geny = false;
+  // the fused row-pass launches use the kernels with the generic-radix stage
+  // only when some simulation's row plan needs it
+  bool genx = false;
   int xtile(const ib_handle* h) const {
     return xrule > 0 ? xrule : (xrule < 0 ? (h->px.dev.L <= 512 ? 2 : 1) : h->tiles.rp);
   }
@@ -187,7 +187,7 @@ struct ib_batch {
         es.push_back(ye(h, Y_SCALE2, a, &t));
         l = std::max(l, t);
       }
-      const float ms = time_tab(dst_y_tab_kernel(Y_SCALE2, geny), es, l);
+      const float ms = time_tab(dst_y_tab_kernel(Y_SCALE2), es, l);
       if (ms < best) {
         best = ms;
         by = r;
@@ -237,7 +237,7 @@ struct ib_batch {
         sbc[i] = bout;
       }
       plan.add(dst_x_tab_kernel(X_APPLYBC, genx), x1, l1);
-      plan.add(dst_y_tab_kernel(Y_SCALE2, geny), y1, l2);
+      plan.add(dst_y_tab_kernel(Y_SCALE2), y1, l2);
       plan.add(dst_x_tab_kernel(X_PLAIN, genx), x2, l3);
       plan.add(fluid_tab_kernel(TK_CURL_GETBC), cu);
     }
@@ -295,7 +295,7 @@ struct ib_batch {
       plan.add(fluid_tab_kernel(TK_GET_BC), gb);
       plan.add(fluid_tab_kernel(TK_NL_RHS), nl);
       plan.add(dst_x_tab_kernel(X_PLAIN, genx), x1, l1);
-      plan.add(dst_y_tab_kernel(Y_PRED, geny), y1, l2);
+      plan.add(dst_y_tab_kernel(Y_PRED), y1, l2);
       plan.add(dst_x_tab_kernel(X_PLAIN, genx), x2, l3);
     }
     plan_vort2flux(Lmax);
@@ -337,7 +337,7 @@ struct ib_batch {
         x2.push_back(xe(h, c, &t)); l3 = std::max(l3, t);
       }
       plan.add(dst_x_tab_kernel(X_ROT, genx), x1, l1);
-      plan.add(dst_y_tab_kernel(Y_SCALE2, geny), y1, l2);
+      plan.add(dst_y_tab_kernel(Y_SCALE2), y1, l2);
       plan.add(dst_x_tab_kernel(X_SUBFROM, genx), x2, l3);
     }
     plan_vort2flux(Lmax);
@@ -352,11 +352,8 @@ struct ib_batch {
   }
 
   void build_fused() {
-    genx = geny = false;
-    for (const ib_handle* h : sims) {
-      genx = genx || fft_has_generic(h->px.dev);
-      geny = geny || fft_has_generic(h->py.dev);
-    }
+    genx = false;
+    for (const ib_handle* h : sims) genx = genx || fft_has_generic(h->px.dev);
     tune_tiles();
     plan_step();
     arena.alloc(plan.bytes.size());

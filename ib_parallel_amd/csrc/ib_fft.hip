@@ -233,6 +233,9 @@ __device__ void stage_generic(double2* __restrict__ in, double2* __restrict__ ou
 // synchronises before the call.  GEN = false leaves out the stage of radix
 // >= 7 (stage_generic, whose registers set the kernel's occupancy): the host
 // launches that form only for plans whose radices are all 2, 3, 4, 5, 8.
+// Only the row pass has it: the column pass built without the stage
+// measured slower (its two-input form 23.4 against 16.7 us per launch at
+// 400 x 400, the same LDS-bound occupancy), so it always keeps the stage.
 template <bool GEN>
 __device__ __forceinline__ double2* fft_run(double2* a, double2* b, int nf, const FFTPlan& pl, const double2* W) {
   int Ns = 1;
@@ -391,7 +394,7 @@ __device__ __forceinline__ void dst_x_tile(const XArgs& a, int bx, int by) {
 }
 
 // Column pass.  Columns i0 = 0..m-2 (stride m-1) of length N = n-1.
-template <int MODE, bool GEN>
+template <int MODE>
 __device__ __forceinline__ void dst_y_tile(const YArgs& a, int bx, int by) {
   extern __shared__ double2 sm[];
   const FFTPlan& pl = a.pl;
@@ -438,7 +441,7 @@ __device__ __forceinline__ void dst_y_tile(const YArgs& a, int bx, int by) {
     A[s * L + M] = make_double2(0.0, 0.0);
   }
   __syncthreads();
-  double2* Z = fft_run<GEN>(A, B, nfft, pl, Wl);
+  double2* Z = fft_run<true>(A, B, nfft, pl, Wl);
   if (MODE == Y_SCALE2 || MODE == Y_PRED) {
     double2* O = (Z == A) ? B : A;
     (void)O;
@@ -467,7 +470,7 @@ __device__ __forceinline__ void dst_y_tile(const YArgs& a, int bx, int by) {
     }
     __syncthreads();
     double2* other = (Z == A) ? B : A;
-    Z = fft_run<GEN>(Z, other, cp, pl, Wl);
+    Z = fft_run<true>(Z, other, cp, pl, Wl);
   }
   for (int idx = threadIdx.x; idx < N * w2; idx += blockDim.x) {
     const int t = idx / w2;
@@ -485,19 +488,19 @@ template <int MODE, bool GEN>
 __global__ void __launch_bounds__(256) k_dst_x(XArgs a) {
   dst_x_tile<MODE, GEN>(a, blockIdx.x, blockIdx.y);
 }
-template <int MODE, bool GEN>
+template <int MODE>
 __global__ void __launch_bounds__(256) k_dst_y(YArgs a) {
-  dst_y_tile<MODE, GEN>(a, blockIdx.x, blockIdx.y);
+  dst_y_tile<MODE>(a, blockIdx.x, blockIdx.y);
 }
 template <int MODE, bool GEN>
 struct DstXOp {
   using Args = XArgs;
   static __device__ __forceinline__ void tile(const XArgs& a, int bx, int by, int) { dst_x_tile<MODE, GEN>(a, bx, by); }
 };
-template <int MODE, bool GEN>
+template <int MODE>
 struct DstYOp {
   using Args = YArgs;
-  static __device__ __forceinline__ void tile(const YArgs& a, int bx, int by, int) { dst_y_tile<MODE, GEN>(a, bx, by); }
+  static __device__ __forceinline__ void tile(const YArgs& a, int bx, int by, int) { dst_y_tile<MODE>(a, bx, by); }
 };
 
 // ------------------------------------------------------------------------
@@ -567,12 +570,11 @@ const void* dst_x_kernel_t(int mode) {
     default: throw HipError("bad DST row mode");
   }
 }
-template <bool GEN>
-const void* dst_y_kernel_t(int mode) {
+const void* dst_y_kernel(int mode) {
   switch (mode) {
-    case Y_PLAIN: return (const void*)k_dst_y<Y_PLAIN, GEN>;
-    case Y_SCALE2: return (const void*)k_dst_y<Y_SCALE2, GEN>;
-    case Y_PRED: return (const void*)k_dst_y<Y_PRED, GEN>;
+    case Y_PLAIN: return (const void*)k_dst_y<Y_PLAIN>;
+    case Y_SCALE2: return (const void*)k_dst_y<Y_SCALE2>;
+    case Y_PRED: return (const void*)k_dst_y<Y_PRED>;
     default: throw HipError("bad DST column mode");
   }
 }
@@ -586,33 +588,29 @@ const void* dst_x_tab_kernel_t(int mode) {
     default: throw HipError("DST row pass: no table kernel for this mode");
   }
 }
-template <bool GEN>
-const void* dst_y_tab_kernel_t(int mode) {
-  switch (mode) {
-    case Y_SCALE2: return (const void*)k_tab<DstYOp<Y_SCALE2, GEN>>;
-    case Y_PRED: return (const void*)k_tab<DstYOp<Y_PRED, GEN>>;
-    default: throw HipError("DST column pass: no table kernel for this mode");
-  }
-}
+
 const void* dst_x_kernel(int mode, bool gen) { return gen ? dst_x_kernel_t<true>(mode) : dst_x_kernel_t<false>(mode); }
-const void* dst_y_kernel(int mode, bool gen) { return gen ? dst_y_kernel_t<true>(mode) : dst_y_kernel_t<false>(mode); }
 }  // namespace
 
 const void* dst_x_tab_kernel(int mode, bool gen) {
   return gen ? dst_x_tab_kernel_t<true>(mode) : dst_x_tab_kernel_t<false>(mode);
 }
-const void* dst_y_tab_kernel(int mode, bool gen) {
-  return gen ? dst_y_tab_kernel_t<true>(mode) : dst_y_tab_kernel_t<false>(mode);
+const void* dst_y_tab_kernel(int mode) {
+  switch (mode) {
+    case Y_SCALE2: return (const void*)k_tab<DstYOp<Y_SCALE2>>;
+    case Y_PRED: return (const void*)k_tab<DstYOp<Y_PRED>>;
+    default: throw HipError("DST column pass: no table kernel for this mode");
+  }
 }
 
 void dst_set_max_lds(size_t bytes) {
   std::vector<const void*> fns;
   for (bool gen : {false, true}) {
     for (int mode : {X_PLAIN, X_APPLYBC, X_SUBFROM, X_ROT, X_POINTROT}) fns.push_back(dst_x_kernel(mode, gen));
-    for (int mode : {Y_PLAIN, Y_SCALE2, Y_PRED}) fns.push_back(dst_y_kernel(mode, gen));
     for (int mode : {X_PLAIN, X_APPLYBC, X_SUBFROM, X_ROT}) fns.push_back(dst_x_tab_kernel(mode, gen));
-    for (int mode : {Y_SCALE2, Y_PRED}) fns.push_back(dst_y_tab_kernel(mode, gen));
   }
+  for (int mode : {Y_PLAIN, Y_SCALE2, Y_PRED}) fns.push_back(dst_y_kernel(mode));
+  for (int mode : {Y_SCALE2, Y_PRED}) fns.push_back(dst_y_tab_kernel(mode));
   for (const void* f : fns) IB_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
@@ -697,7 +695,7 @@ void dst_y(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, cons
   size_t lds = 0;
   const TabEntry<YArgs> e = dst_y_entry(pl, mode, l, g, cp, max_lds, &lds);
   void* args[] = {(void*)&e.a};
-  IB_HIP(hipLaunchKernel(dst_y_kernel(mode, fft_has_generic(pl)), dim3(e.nbx, e.nby), dim3(256), args, lds, st));
+  IB_HIP(hipLaunchKernel(dst_y_kernel(mode), dim3(e.nbx, e.nby), dim3(256), args, lds, st));
   IB_LAUNCH_CHECK();
 }
 

@@ -101,10 +101,10 @@ TabEntry<XArgs> dst_x_entry(const FFTPlan& pl, const DstLaunch& l, const Geom& g
 TabEntry<YArgs> dst_y_entry(const FFTPlan& pl, int mode, const DstLaunch& l, const Geom& g, int cp,
                             size_t max_lds, size_t* lds);
 // true when a stage of the plan has a radix other than 2, 3, 4, 5, 8: only
-// the kernels built with the generic-radix stage may run such a plan
+// the row-pass kernels built with the generic-radix stage may run such a plan
 bool fft_has_generic(const FFTPlan& pl);
 const void* dst_x_tab_kernel(int mode, bool gen);
-const void* dst_y_tab_kernel(int mode, bool gen);
+const void* dst_y_tab_kernel(int mode);
 // Raises the dynamic-LDS limit of the DST kernels to `bytes` (call once).
 void dst_set_max_lds(size_t bytes);
 
