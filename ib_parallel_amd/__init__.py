@@ -49,6 +49,15 @@ class GpuSim(Simulation):
         self._check(self._f["schur_info"](self.h, C.byref(mode), C.byref(nfaces), C.byref(builds)))
         return mode.value, nfaces.value, builds.value
 
+    def gmres_info(self):
+        """(form, workgroups, exchange) of the dense GMRES solve: form 0 per-stage
+        kernels, 1 one workgroup, 2 cooperative (nf <= 256), 3 cooperative for
+        larger systems; exchange of the last form-3 solve 1 one XCD's L2, 2
+        device scope, 0 none yet."""
+        f, p, x = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self._f["gmres_info"](self.h, C.byref(f), C.byref(p), C.byref(x)))
+        return f.value, p.value, x.value
+
     def dst_tiles(self):
         """(rp, cp, source): row pairs per workgroup of the DST row pass, column
         pairs of the column pass, 0 fixed rule / 1 timed / 2 forced (IB_DST_TILES)."""

@@ -409,9 +409,9 @@ struct ib_batch {
       int ncu = 0;
       if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, sims[0]->dev) != hipSuccess) ncu = 0;
       long need = 0;
-      for (ib_handle* h : sims) need += h->coopP;
+      for (ib_handle* h : sims) need += h->coopP + h->cwP;
       if (need > ncu / 8)
-        for (ib_handle* h : sims) h->coopP = h->coopR = 0;
+        for (ib_handle* h : sims) h->drop_coop();
     }
     direct = params[0].launch_mode == 1;
     if (mode == 1) build_fused();

@@ -12,6 +12,8 @@
 // convergence reason, so the host may enqueue several iterations ahead of
 // the status it has read (the extra matvecs then write only into basis
 // columns the solution never uses).
+#include <cstdlib>
+
 #include "ib_device.h"
 #include "ib_kernels.h"
 
@@ -784,23 +786,34 @@ __device__ __forceinline__ double gc_reduce4(double v0, double v1, double v2, do
 // the local row whose sum gc_reduce4 leaves in lane l of wavefront q
 __device__ __forceinline__ int gc_reduce4_row(int q, int l) { return 4 * q + (((l & 1) << 1) | ((l >> 1) & 1)); }
 
+// Wait until every vector memory operation this wavefront issued has
+// completed; for the device-scope stores above that means they have
+// reached the coherent level.  A workgroup-scope release fence does not
+// emit this wait on gfx950 (it only orders within the CU), so without it
+// another wavefront's flag store could overtake this wavefront's data.
+__device__ __forceinline__ void gc_wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 // Grid barrier number k (1, 2, ...) over the P workgroups.  Each
-// workgroup's values are device-scope atomic stores; once every wave has
-// waited for its stores (workgroup-scope release) workgroup c publishes
-// flag[c] = k, and wavefront 0 polls the P flags (one lane each) until all
-// have reached k.  Bounded: on a timeout (or another workgroup's) the error
-// word is set and every workgroup leaves.
-__device__ __forceinline__ bool gc_sync(const GmCoopArgs& a, int c, unsigned k, int* s_err) {
+// workgroup's values are device-scope atomic stores.  Every wavefront
+// waits for its own stores to complete (gc_wait_stores) before the
+// workgroup barrier, so when wavefront 0 then publishes flag[c] = k every
+// store of workgroup c is visible at device scope; wavefront 0 polls the P
+// flags (one lane each) until all have reached k, and the others wait at
+// the second workgroup barrier, after which their device-scope loads see
+// the other workgroups' values.  Bounded: on a timeout (or another
+// workgroup's) the error word is set and every workgroup leaves.
+__device__ __forceinline__ bool gc_sync(unsigned* sync, int P, int c, unsigned k, int* s_err) {
+  gc_wait_stores();
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __syncthreads();
   if (threadIdx.x < 64) {
-    unsigned* err = a.sync;
-    unsigned* fl = a.sync + 8;
+    unsigned* err = sync;
+    unsigned* fl = sync + 8;
     const int lane = threadIdx.x;
     if (lane == 0) __hip_atomic_store(fl + c, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     long n = 0;
     while (true) {
-      const unsigned v = lane < a.P ? __hip_atomic_load(fl + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : k;
+      const unsigned v = lane < P ? __hip_atomic_load(fl + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : k;
       if (__all(v >= k)) break;
       if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++n > kCoopSpin) {
         if (lane == 0) {
@@ -918,7 +931,7 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
       gc_st(a.norms + c * 4 + 1, pr);
       gc_st(a.norms + c * 4 + 2, pb);
     }
-    if (!gc_sync(a, c, ++bk, &s_err)) break;
+    if (!gc_sync(a.sync, a.P, c, ++bk, &s_err)) break;
     const double sr = gc_sum(a.norms + 1, 4, a.P), sb = gc_sum(a.norms + 2, 4, a.P);
     const double res0 = sqrt(sr), bn = sqrt(sb);
     if (first) {
@@ -953,7 +966,7 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
           if (sub == 0) gc_st(a.dots + (long)c * (kmax + 2) + j, p);
         }
       }
-      if (!gc_sync(a, c, ++bk, &s_err)) {
+      if (!gc_sync(a.sync, a.P, c, ++bk, &s_err)) {
         ok = false;
         break;
       }
@@ -990,7 +1003,7 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
         pn = gm_block_sum2(pn, red, T);
         if (threadIdx.x == 0) gc_st(a.norms + c * 4, pn);
       }
-      if (!gc_sync(a, c, ++bk, &s_err)) {
+      if (!gc_sync(a.sync, a.P, c, ++bk, &s_err)) {
         ok = false;
         break;
       }
@@ -1082,7 +1095,7 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
       reason = -2;
       break;
     }
-    if (!gc_sync(a, c, ++bk, &s_err)) break;  // every workgroup's rows of x before the next residual
+    if (!gc_sync(a.sync, a.P, c, ++bk, &s_err)) break;  // every workgroup's rows of x before the next residual
   }
   if (threadIdx.x == 0) {
     if (s_err) {
@@ -1118,7 +1131,7 @@ bool gmres_coop_plan(int nf, int kmax, size_t max_lds, int* P, int* R) {
   return false;
 }
 
-constexpr int kCoopSyncDoubles = 24;  // the error word and 32 workgroup flags (unsigned)
+constexpr int kCoopSyncDoubles = 40;  // the error word, 32 workgroup flags and 32 XCD ids (unsigned)
 
 size_t gmres_coop_ws(int nf, int kmax, int P) {
   return kCoopSyncDoubles + (size_t)P * (kmax + 2) + (size_t)4 * P + (size_t)nf + (size_t)P * (kmax + 1) * kmax;
@@ -1135,12 +1148,587 @@ void gmres_coop(hipStream_t st, int nf, int kmax, int maxit, int P, int R, const
   a.norms = a.dots + (size_t)P * (kmax + 2);
   a.wfull = a.norms + (size_t)4 * P;
   a.H = a.wfull + nf;
-  IB_HIP(hipMemsetAsync(ws, 0, kCoopSyncDoubles * sizeof(double), st));
+  // barrier words cleared by a kernel (a memset node misbehaves on graph replays)
+  fill(st, ws, kCoopSyncDoubles, 0.0);
   // an ordinary launch: the barrier is the kernel's own (no runtime grid
   // synchronisation resources), 8P workgroups of which P stay, one per CU
   // on a device of 256 CUs; a workgroup that is not resident in time makes
   // the others leave with reason -3 instead of waiting for ever
   hipLaunchKernelGGL(k_gmres_coop, dim3(8 * P), dim3(kGmT), coop_lds(nf, kmax, R), st, a);
+  IB_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------------
+// The dense solve over P workgroups of one XCD for larger systems: nf <=
+// 1024 and Krylov spaces of up to 1024 vectors (full GMRES of the airfoil +
+// flap system, nf = 800, restart 1000).  The same PETSc iteration as
+// k_gmres_coop, laid out for up to 32 rows per workgroup and long Krylov
+// spaces:
+//  * wavefront q owns local rows 2q and 2q+1; lane l holds S[row][l + 64k],
+//    k < CQ, in registers (nf <= 64 CQ), and gw_reduce2 sums a row pair over
+//    the wavefront;
+//  * the local rows of the basis are stored row-major in LDS (row r of
+//    vector j at Vl[r NL + j]), so thread j forms the partial CGS coefficient
+//    of vector j from conflict-free LDS reads, and the update w -= V h reads
+//    consecutive j across the lanes; vectors j >= NL (past the LDS) are kept
+//    in HBM in the same layout;
+//  * the CGS coefficients are reduced and scattered: workgroup c sums the P
+//    partials of coefficients c, c + P, ... in a fixed butterfly order and
+//    publishes them, and after a third grid barrier every workgroup loads
+//    all of them, so all hold identical h and take the same branches;
+//  * the exchange goes through the L2 of the one XCD the workgroups share
+//    (checked with HW_REG_XCC_ID at the start; device scope otherwise);
+//  * only workgroup 0 keeps the rotated Hessenberg matrix (in HBM).  At the
+//    end of a cycle it runs the back substitution in blocks of kCwBlk
+//    columns (diagonal block by one wavefront from LDS, the rows above by
+//    all threads) and publishes y; every workgroup then adds V y to its rows
+//    of x.
+// ------------------------------------------------------------------------
+constexpr int kCwBlk = 32;
+
+struct GmCoopWArgs {
+  int nf, kmax, maxit, P, R, NL;
+  int l2x;  // 1: exchange through one XCD's L2 when every workgroup is found there
+  const double* S;
+  const double* b;
+  double* x;
+  GmresDev* s;
+  double rtol, atol;
+  unsigned* sync;  // [0] error, [8 + c] barriers passed by workgroup c
+  double* dots;    // P x (kmax + 2) partial CGS coefficients
+  double* norms;   // P x 4 partial squared norms
+  double* wfull;   // nf: w (or r) rows of every workgroup
+  double* Hg;      // kmax x (kmax + 1): the rotated Hessenberg, column it at Hg + it (kmax + 1)
+  double* yg;      // kmax + 1: the back substitution's y
+  double* hg;      // kmax + 1: the summed CGS coefficients of the step
+  double* Vg;      // P x R x (kmax + 1 - NL): basis vectors NL.. of each workgroup's rows
+};
+
+// Every lane holds partial sums v0, v1 of two rows; returns in lane 0 the
+// sum over the 64 lanes of row 0 and in lane 1 that of row 1.  Pairs swap
+// a row (xor 1), then the pairs of a quad are added (xor 2), the quads of a
+// row of 16 (row_ror 4, 8) and the four rows of 16 (xor 16, 32).  Fixed
+// order; the whole wavefront must be active.
+__device__ __forceinline__ double gw_reduce2(double v0, double v1) {
+  const int l = gm_opaque(threadIdx.x) & 63;
+  const bool odd = l & 1;
+  double k = odd ? v1 : v0;
+  k += gm_dpp<0xB1>(odd ? v0 : v1);
+  k += gm_dpp<0x4E>(k);
+  k += gm_dpp<0x124>(k);
+  k += gm_dpp<0x128>(k);
+  k += __shfl_xor(k, 16);
+  k += __shfl_xor(k, 32);
+  return k;
+}
+
+// Butterfly sum over aligned groups of g = 1, 2, 4, ..., 32 consecutive
+// lanes (xor 1 and xor 2 through DPP, xor 4, 8, 16 through lane shuffles):
+// every lane of a group ends with the same bits, ((p0 + p1) + (p2 + p3)) +
+// ... in one fixed order.  The whole wavefront must be active.
+__device__ __forceinline__ double gw_group_sum(double p, int g) {
+  if (g >= 2) p += gm_dpp<0xB1>(p);
+  if (g >= 4) p += gm_dpp<0x4E>(p);
+  if (g >= 8) p += __shfl_xor(p, 4);
+  if (g >= 16) p += __shfl_xor(p, 8);
+  if (g >= 32) p += __shfl_xor(p, 16);
+  return p;
+}
+
+// Lanes per CGS coefficient in the dot phase of step it: as many
+// as the 1024 threads allow, up to 8 (groups never straddle a wavefront).
+__device__ __forceinline__ int gw_lanes_per_coef(int it) {
+  const int n = it + 1;
+  return 8 * n <= kGmT ? 8 : (4 * n <= kGmT ? 4 : (2 * n <= kGmT ? 2 : 1));
+}
+
+// Wavefront-wide sum of p[q * stride] over q < P (lane q loads one value,
+// one memory round trip; fixed reduction order, so every workgroup that
+// runs it gets the same bits).
+
+// Exchange between the workgroups of a cooperative solve.  Loads are
+// device-scope atomic loads: they bypass the CU's L1 and are served by L2
+// (measured on MI355X: ~90 ns per dependent load, against ~56 ns for an L1
+// hit).  Stores: device scope, coherent across XCDs, or with X2 (every
+// workgroup on one XCD, checked at the start of the launch) workgroup-scope
+// stores, which complete at that XCD's L2 -- the one L2 every load of the
+// exchange reads -- instead of at memory.  Each wavefront waits for its
+// stores to complete before a barrier either way.
+template <bool X2>
+__device__ __forceinline__ double xc_ld(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <bool X2>
+__device__ __forceinline__ void xc_st(double* p, double v) {
+  if constexpr (X2) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <bool X2>
+__device__ __forceinline__ unsigned xc_ldu(const unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <bool X2>
+__device__ __forceinline__ void xc_stu(unsigned* p, unsigned v) {
+  if constexpr (X2) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Grid barrier number k over the P workgroups, as gc_sync, with the
+// exchange stores of X2.
+template <bool X2>
+__device__ __forceinline__ bool xc_sync(unsigned* sync, int P, int c, unsigned k, int* s_err) {
+  gc_wait_stores();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    unsigned* err = sync;
+    unsigned* fl = sync + 8;
+    const int lane = threadIdx.x;
+    if (lane == 0) xc_stu<X2>(fl + c, k);
+    long n = 0;
+    while (true) {
+      const unsigned v = lane < P ? xc_ldu<X2>(fl + lane) : k;
+      if (__all(v >= k)) break;
+      if (xc_ldu<X2>(err) != 0u || ++n > kCoopSpin) {
+        if (lane == 0) {
+          xc_stu<X2>(err, 1u);
+          *s_err = 1;
+        }
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+  __syncthreads();
+  return *s_err == 0;
+}
+
+template <bool X2>
+__device__ __forceinline__ double gw_wave_gather(const double* p, long stride, int P, int lane) {
+  const double v = lane < P ? xc_ld<X2>(p + (long)lane * stride) : 0.0;
+  return gm_wave_sum(v);
+}
+
+template <int CQ, bool X2>
+__device__ __forceinline__ void coopw_solve(const GmCoopWArgs& a, int c, unsigned bk, double* dyn, double* red,
+                                            double* s_val, int* s_flag, int* s_err) {
+  const int nf = a.nf, kmax = a.kmax, ld = kmax + 1, T = kGmT, R = a.R, NL = a.NL;
+  const int r0 = c * R, Rc = max(0, min(nf, r0 + R) - r0);
+  const int ldg = ld - NL;  // row stride of the basis vectors kept in HBM
+  double* Vgc = a.Vg + (size_t)c * R * (size_t)max(ldg, 0);
+  double* vfull = dyn;                 // nf: the vector being multiplied
+  double* wl = vfull + nf;             // 32: local rows of w
+  double* hcol = wl + 32;              // kmax + 2: CGS coefficients (y at a cycle's end)
+  double* hrot = hcol + kmax + 2;      // kmax + 2: the new Hessenberg column after the rotations
+  double* lcc = hrot + kmax + 2;       // kmax
+  double* lss = lcc + kmax;            // kmax
+  double* grs = lss + kmax;            // kmax + 1 (y of the back substitution in place)
+  double* stg = grs + kmax + 1;        // kCwBlk^2: a diagonal block of the back substitution
+  double* Vl = stg + kCwBlk * kCwBlk;  // R NL: local rows of basis vectors 0..NL-1, row-major
+  // rows 2 wq and 2 wq + 1 of S, columns ln + 64 k, in registers (0 outside the matrix)
+  double s0[CQ], s1[CQ];
+  {
+    const int wq = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    const int ra = 2 * wq, rb = 2 * wq + 1;
+#pragma unroll
+    for (int k = 0; k < CQ; ++k) {
+      const int col = ln + 64 * k;
+      s0[k] = ra < Rc && col < nf ? a.S[(long)col * nf + r0 + ra] : 0.0;
+      s1[k] = rb < Rc && col < nf ? a.S[(long)col * nf + r0 + rb] : 0.0;
+    }
+  }
+  __syncthreads();
+
+  // wl = S[local rows] vfull (ends with a barrier)
+  auto matvec = [&]() {
+    const int q = gm_opaque(threadIdx.x) >> 6, l = gm_opaque(threadIdx.x) & 63;
+    if (2 * q < Rc) {  // uniform over the wavefront
+      double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+      for (int k = 0; k < CQ; ++k) {
+        const double v = vfull[min(l + 64 * k, nf - 1)];
+        acc0 += s0[k] * v;  // s0, s1 are 0 past the matrix
+        acc1 += s1[k] * v;
+      }
+      const double sum = gw_reduce2(acc0, acc1);
+      const int row = 2 * q + (l & 1);
+      if (l < 2 && row < Rc) wl[row] = sum;
+    }
+    __syncthreads();
+  };
+  // acc0/acc1 += sum_{j = l, l + 64, ... < jn} coef[j] V(ra / rb, j)
+  auto rowpair = [&](const double* coef, int jn, int ra, int rb, int l, double& acc0, double& acc1) {
+    const int ja = min(jn, NL);
+    int j = l;
+#pragma unroll 4
+    for (; j < ja; j += 64) {
+      const double h = coef[j];
+      acc0 += h * Vl[ra * NL + j];
+      acc1 += h * Vl[rb * NL + j];
+    }
+    for (; j < jn; j += 64) {
+      const double h = coef[j];
+      acc0 += h * Vgc[(size_t)ra * ldg + (j - NL)];
+      acc1 += h * Vgc[(size_t)rb * ldg + (j - NL)];
+    }
+  };
+  // the previous rotations applied to the new column, as in k_gmres_coop
+  auto rotate_prev = [&](int it, int lane) {
+    double carry = hcol[0];
+    for (int base = 0; base < it; base += 64) {
+      const int j = base + lane;
+      const bool act = j < it;
+      const double cj0 = act ? lcc[j] : 1.0, sj = act ? lss[j] : 0.0, hn = act ? hcol[j + 1] : 0.0;
+      double A = -sj, B = cj0 * hn;
+      gm_affine_scan(A, B);
+      const double cnext = A * carry + B;
+      const double cj = gm_shift_up1(cnext, carry);
+      if (act) hrot[j] = cj0 * cj + sj * hn;
+      carry = gm_readlane(cnext, min(63, it - 1 - base));
+    }
+    if (lane == 0) hrot[it] = carry;
+  };
+  // a wavefront that owns no rows runs the rotation scan beside the update
+  const bool scan_early = 2 * (T / 64 - 1) >= Rc;
+
+  int its = 0, reason = 0, first = 1;
+  double rn0 = 0.0, ttol = 0.0;
+  while (true) {
+    // ---- cycle start: r = b - S x (x gathered from every workgroup's rows)
+    for (int i = gm_opaque(threadIdx.x); i < nf; i += T) vfull[i] = xc_ld<X2>(a.x + i);
+    __syncthreads();
+    matvec();
+    double pr = 0.0, pb = 0.0;
+    for (int r = gm_opaque(threadIdx.x); r < Rc; r += T) {
+      const double bv = a.b[r0 + r];
+      const double rv = bv - wl[r];
+      xc_st<X2>(a.wfull + r0 + r, rv);
+      pr += rv * rv;
+      pb += bv * bv;
+    }
+    pr = gm_block_sum2(pr, red, T);
+    pb = gm_block_sum2(pb, red, T);
+    if (threadIdx.x == 0) {
+      xc_st<X2>(a.norms + c * 4 + 1, pr);
+      xc_st<X2>(a.norms + c * 4 + 2, pb);
+    }
+    if (!xc_sync<X2>(a.sync, a.P, c, ++bk, s_err)) break;
+    if (threadIdx.x < 64) {
+      const double sr = gw_wave_gather<X2>(a.norms + 1, 4, a.P, threadIdx.x);
+      const double sb = gw_wave_gather<X2>(a.norms + 2, 4, a.P, threadIdx.x);
+      if (threadIdx.x == 0) {
+        s_val[0] = sr;
+        s_val[1] = sb;
+      }
+    }
+    __syncthreads();
+    const double res0 = sqrt(s_val[0]), bn = sqrt(s_val[1]);
+    if (first) {
+      rn0 = bn > 0.0 ? bn : res0;
+      ttol = fmax(a.rtol * rn0, a.atol);
+      first = 0;
+    }
+    if (res0 == 0.0 || res0 <= ttol) reason = 1;
+    else if (res0 >= 1e4 * rn0) reason = -1;
+    if (reason) break;
+    if (threadIdx.x == 0) grs[0] = res0;
+    for (int i = gm_opaque(threadIdx.x); i < nf; i += T) vfull[i] = xc_ld<X2>(a.wfull + i) / res0;
+    for (int r = gm_opaque(threadIdx.x); r < Rc; r += T) {
+      const double v = xc_ld<X2>(a.wfull + r0 + r) / res0;
+      if (NL > 0) Vl[r * NL] = v;
+      else Vgc[(size_t)r * ldg] = v;
+    }
+    __syncthreads();
+    int it = 0;
+    bool ok = true;
+    while (true) {
+      matvec();  // w = S v_it, local rows
+      // partial CGS coefficient <w, v_j> over the local rows: g lanes per
+      // j, lane sub takes rows sub, sub + g, ...
+      const int g = gw_lanes_per_coef(it);
+      {
+        const int t = gm_opaque(threadIdx.x), j = t / g, sub = t - j * g;
+        double p = 0.0;
+        if (j <= it) {
+          if (j < NL) {
+#pragma unroll 4
+            for (int r = sub; r < Rc; r += g) p += wl[r] * Vl[r * NL + j];
+          } else {
+            const double* vg = Vgc + (j - NL);
+#pragma unroll 4
+            for (int r = sub; r < Rc; r += g) p += wl[r] * vg[(size_t)r * ldg];
+          }
+        }
+        p = gw_group_sum(p, g);
+        if (j <= it && sub == 0) xc_st<X2>(a.dots + (long)c * (kmax + 2) + j, p);
+      }
+      if (!xc_sync<X2>(a.sync, a.P, c, ++bk, s_err)) {
+        ok = false;
+        break;
+      }
+      // h_j: workgroup c sums the P partials of j = c, c + P, ... (lane q of
+      // a group of P lanes loads partial q; fixed butterfly over the group)
+      // and publishes it; after a barrier every workgroup loads all h_j.
+      // Each partial is read once, not once per workgroup.
+      {
+        const int t = gm_opaque(threadIdx.x), jj = t / a.P, q = t - jj * a.P;
+        const int j = c + jj * a.P;
+        double h = j <= it ? xc_ld<X2>(a.dots + (long)q * (kmax + 2) + j) : 0.0;
+        h = gw_group_sum(h, a.P);
+        if (j <= it && q == 0) xc_st<X2>(a.hg + j, h);
+      }
+      if (!xc_sync<X2>(a.sync, a.P, c, ++bk, s_err)) {
+        ok = false;
+        break;
+      }
+      for (int j = gm_opaque(threadIdx.x); j <= it; j += T) hcol[j] = xc_ld<X2>(a.hg + j);
+      __syncthreads();
+      // w -= sum_j h_j v_j on the local rows; partial ||w||^2; w rows out
+      {
+        const int q = gm_opaque(threadIdx.x) >> 6, l = gm_opaque(threadIdx.x) & 63;
+        double pn = 0.0;
+        if (2 * q < Rc) {  // uniform over the wavefront
+          const int ra = 2 * q, rb = 2 * q + 1 < Rc ? 2 * q + 1 : 2 * q;
+          double acc0 = 0.0, acc1 = 0.0;
+          rowpair(hcol, it + 1, ra, rb, l, acc0, acc1);
+          const double sum = gw_reduce2(-acc0, -acc1);
+          const int row = 2 * q + (l & 1);
+          if (l < 2 && row < Rc) {
+            const double wn = wl[row] + sum;
+            wl[row] = wn;
+            xc_st<X2>(a.wfull + r0 + row, wn);
+            pn = wn * wn;
+          }
+        } else if (scan_early && q == T / 64 - 1) {
+          rotate_prev(it, l);
+        }
+        pn = gm_block_sum2(pn, red, T);
+        if (threadIdx.x == 0) xc_st<X2>(a.norms + c * 4, pn);
+      }
+      if (!xc_sync<X2>(a.sync, a.P, c, ++bk, s_err)) {
+        ok = false;
+        break;
+      }
+      // this thread's entries of the new vector (full and local rows),
+      // loaded with the norm partials
+      const int t0 = gm_opaque(threadIdx.x);
+      const double wpre = t0 < nf ? xc_ld<X2>(a.wfull + t0) : 0.0;
+      const double wpre_l = t0 < Rc ? xc_ld<X2>(a.wfull + r0 + t0) : 0.0;
+      if (gm_opaque(threadIdx.x) < 64) {
+        const int lane = gm_opaque(threadIdx.x);
+        const double tt = sqrt(gw_wave_gather<X2>(a.norms, 4, a.P, lane));
+        if (!scan_early) rotate_prev(it, lane);
+        if (lane == 0) {
+          hrot[it + 1] = tt;
+          const int hap = tt < 1e-30 && tt < fabs(tt / grs[it]);
+          double res;
+          if (!hap) {
+            const double t1 = sqrt(hrot[it] * hrot[it] + hrot[it + 1] * hrot[it + 1]);
+            const double cc = hrot[it] / t1, sn = hrot[it + 1] / t1;
+            lcc[it] = cc;
+            lss[it] = sn;
+            grs[it + 1] = -sn * grs[it];
+            grs[it] = cc * grs[it];
+            hrot[it] = cc * hrot[it] + sn * hrot[it + 1];
+            res = fabs(grs[it + 1]);
+          } else {
+            res = 0.0;
+          }
+          int rs = 0;
+          if (res <= ttol) rs = 1;
+          else if (res >= 1e4 * rn0) rs = -1;
+          if (hap && !rs) rs = 1;
+          s_flag[0] = rs;
+          s_flag[1] = hap;
+          s_val[0] = tt;
+        }
+      }
+      __syncthreads();
+      reason = s_flag[0];
+      const int hap = s_flag[1];
+      if (c == 0)
+        for (int j = gm_opaque(threadIdx.x); j <= it + 1; j += T) a.Hg[(long)it * ld + j] = hrot[j];
+      if (!hap) {
+        const double inv = 1.0 / s_val[0];
+        if (t0 < nf) vfull[t0] = wpre * inv;
+        for (int i = t0 + T; i < nf; i += T) vfull[i] = xc_ld<X2>(a.wfull + i) * inv;
+        if (t0 < Rc) {
+          const int j = it + 1;
+          if (j < NL) Vl[t0 * NL + j] = wpre_l * inv;
+          else Vgc[(size_t)t0 * ldg + (j - NL)] = wpre_l * inv;
+        }
+      }
+      ++it;
+      ++its;
+      __syncthreads();
+      if (reason || it >= kmax || its >= a.maxit) break;
+    }
+    if (!ok) break;
+    // ---- KSPGMRESBuildSoln: workgroup 0 solves the rotated triangular
+    // system (grs becomes y in place) and publishes y
+    const int top = it - 1;
+    if (c == 0) {
+      for (int kb1 = top; kb1 >= 0; kb1 -= kCwBlk) {
+        const int kb0 = max(0, kb1 - kCwBlk + 1), nb = kb1 - kb0 + 1;
+        // column k of the diagonal block: stg[k kCwBlk + i] = H(kb0 + i, kb0 + k), i <= k
+        for (int q = gm_opaque(threadIdx.x); q < nb * nb; q += T) {
+          const int k = q / nb, i = q - k * nb;
+          if (i <= k) stg[k * kCwBlk + i] = a.Hg[(long)(kb0 + k) * ld + kb0 + i];
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+          const int lane = threadIdx.x;
+          double ti = lane < nb ? grs[kb0 + lane] : 0.0;
+          for (int k = nb - 1; k >= 0; --k) {
+            const double yk = gm_readlane(ti, k) / stg[k * kCwBlk + k];
+            if (lane < k) ti -= stg[k * kCwBlk + lane] * yk;
+            if (lane == k) ti = yk;
+          }
+          if (lane < nb) grs[kb0 + lane] = ti;
+        }
+        __syncthreads();
+        // the rows above the block, columns of the block in descending order
+        for (int i = gm_opaque(threadIdx.x); i < kb0; i += T) {
+          double t = grs[i];
+          for (int k = nb - 1; k >= 0; --k) t -= a.Hg[(long)(kb0 + k) * ld + i] * grs[kb0 + k];
+          grs[i] = t;
+        }
+        __syncthreads();
+      }
+      for (int j = gm_opaque(threadIdx.x); j <= top; j += T) xc_st<X2>(a.yg + j, grs[j]);
+    }
+    if (!xc_sync<X2>(a.sync, a.P, c, ++bk, s_err)) break;
+    for (int j = gm_opaque(threadIdx.x); j <= top; j += T) hcol[j] = xc_ld<X2>(a.yg + j);
+    __syncthreads();
+    // x += V y on the local rows
+    {
+      const int q = gm_opaque(threadIdx.x) >> 6, l = gm_opaque(threadIdx.x) & 63;
+      if (2 * q < Rc) {
+        const int ra = 2 * q, rb = 2 * q + 1 < Rc ? 2 * q + 1 : 2 * q;
+        double acc0 = 0.0, acc1 = 0.0;
+        rowpair(hcol, top + 1, ra, rb, l, acc0, acc1);
+        const double sum = gw_reduce2(acc0, acc1);
+        const int row = 2 * q + (l & 1);
+        if (l < 2 && row < Rc) xc_st<X2>(a.x + r0 + row, xc_ld<X2>(a.x + r0 + row) + sum);
+      }
+    }
+    if (reason) break;
+    if (its >= a.maxit) {
+      reason = -2;
+      break;
+    }
+    if (!xc_sync<X2>(a.sync, a.P, c, ++bk, s_err)) break;  // every workgroup's rows of x before the next residual
+  }
+  if (threadIdx.x == 0) {
+    if (*s_err) {
+      a.s->reason = -3;
+      a.s->its = its;
+    } else if (c == 0) {
+      a.s->its = its;
+      a.s->reason = reason;
+      a.s->rn0 = rn0;
+      a.s->ttol = ttol;
+      a.s->xmode = X2 ? 1 : 2;
+    }
+  }
+}
+
+
+// Offset (in unsigned words of the barrier block) of the XCD ids the
+// workgroups of a cooperative solve publish.
+constexpr int kXccSlot = 40;
+
+// The XCD this wavefront runs on (HW_REG_XCC_ID, bits 3:0).
+__device__ __forceinline__ unsigned gw_xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 0xFu; }
+
+template <int CQ>
+__global__ void __launch_bounds__(kGmT) k_gmres_coopw(GmCoopWArgs a) {
+  if (blockIdx.x % 8 != 0) return;
+  const int c = blockIdx.x / 8;
+  if (c >= a.P) return;
+  extern __shared__ double dyn[];
+  __shared__ double red[kGmT / 64];
+  __shared__ double s_val[2];
+  __shared__ int s_flag[2];
+  __shared__ int s_err;
+  if (threadIdx.x == 0) {
+    s_err = 0;
+    __hip_atomic_store(a.sync + kXccSlot + c, gw_xcc_id() + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // barrier 1 at device scope; then every workgroup reads the P XCD ids
+  // (the same values, so the same decision everywhere)
+  if (xc_sync<false>(a.sync, a.P, c, 1u, &s_err)) {
+    if (threadIdx.x < 64) {
+      const int lane = threadIdx.x;
+      const unsigned v =
+          lane < a.P ? __hip_atomic_load(a.sync + kXccSlot + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+      const unsigned v0 = __builtin_amdgcn_readfirstlane(v);
+      const bool same = a.l2x && __all(lane >= a.P || v == v0);
+      if (lane == 0) s_flag[0] = same;
+    }
+    __syncthreads();
+    if (s_flag[0]) coopw_solve<CQ, true>(a, c, 1u, dyn, red, s_val, s_flag, &s_err);
+    else coopw_solve<CQ, false>(a, c, 1u, dyn, red, s_val, s_flag, &s_err);
+  } else if (threadIdx.x == 0) {
+    a.s->reason = -3;
+    a.s->its = 0;
+  }
+}
+
+// dynamic LDS of k_gmres_coopw without the basis (doubles): vfull nf, wl
+// 32, hcol/hrot kmax+2 each, lcc/lss kmax each, grs kmax+1, stg kCwBlk^2
+static size_t coopw_lds_fixed(int nf, int kmax) {
+  return (size_t)nf + 32 + (size_t)5 * kmax + 5 + (size_t)kCwBlk * kCwBlk;
+}
+
+bool gmres_coopw_plan(int nf, int kmax, size_t max_lds, int ncu, int* P, int* R, int* NL) {
+  static const int cand[] = {8, 16, 32};
+  if (nf < 2 || nf > 64 * 16 || kmax < 1 || kmax + 1 > kGmT) return false;
+  const size_t budget = max_lds > 2048 ? (max_lds - 2048) / sizeof(double) : 0;
+  const size_t fixed = coopw_lds_fixed(nf, kmax);
+  if (fixed >= budget) return false;
+  for (int p : cand) {
+    if (8 * p > ncu) continue;
+    const int r = (nf + p - 1) / p;
+    if (r > 2 * (kGmT / 64) || (p - 1) * r >= nf) continue;
+    const long nl = std::min<long>(kmax + 1, (long)((budget - fixed) / r));
+    if (nl < std::min(kmax + 1, 64)) continue;
+    *P = p;
+    *R = r;
+    *NL = (int)nl;
+    return true;
+  }
+  return false;
+}
+
+size_t gmres_coopw_ws(int nf, int kmax, int P, int R, int NL) {
+  return kCoopSyncDoubles + (size_t)P * (kmax + 2) + (size_t)4 * P + (size_t)nf + (size_t)(kmax + 1) * kmax +
+         (size_t)2 * (kmax + 1) + (size_t)P * R * (size_t)std::max(0, kmax + 1 - NL);
+}
+
+void gmres_coopw(hipStream_t st, int nf, int kmax, int maxit, int P, int R, int NL, const double* S, const double* b,
+                 double* x, GmresDev* s, double rtol, double atol, double* ws) {
+  GmCoopWArgs a{};
+  a.nf = nf; a.kmax = kmax; a.maxit = maxit; a.P = P; a.R = R; a.NL = NL;
+  a.S = S; a.b = b; a.x = x; a.s = s; a.rtol = rtol; a.atol = atol;
+  a.sync = reinterpret_cast<unsigned*>(ws);
+  a.dots = ws + kCoopSyncDoubles;
+  a.norms = a.dots + (size_t)P * (kmax + 2);
+  a.wfull = a.norms + (size_t)4 * P;
+  a.Hg = a.wfull + nf;
+  a.yg = a.Hg + (size_t)(kmax + 1) * kmax;
+  a.hg = a.yg + (kmax + 1);
+  a.Vg = a.hg + (kmax + 1);
+  // IB_GMRES_L2_EXCHANGE=0 keeps the device-scope exchange even when every
+  // workgroup is on one XCD (the results are the same bits either way)
+  const char* e = std::getenv("IB_GMRES_L2_EXCHANGE");
+  a.l2x = e && std::atoi(e) == 0 ? 0 : 1;
+  fill(st, ws, kCoopSyncDoubles, 0.0);
+  const size_t lds = sizeof(double) * (coopw_lds_fixed(nf, kmax) + (size_t)R * NL);
+  // 8P workgroups of which P (0, 8, 16, ...: one XCD) stay, one per CU
+  if (nf <= 64 * 8) hipLaunchKernelGGL(k_gmres_coopw<8>, dim3(8 * P), dim3(kGmT), lds, st, a);
+  else if (nf <= 64 * 13) hipLaunchKernelGGL(k_gmres_coopw<13>, dim3(8 * P), dim3(kGmT), lds, st, a);
+  else hipLaunchKernelGGL(k_gmres_coopw<16>, dim3(8 * P), dim3(kGmT), lds, st, a);
   IB_LAUNCH_CHECK();
 }
 
@@ -1151,6 +1739,12 @@ void gmres_dense_set_max_lds(size_t bytes) {
   IB_HIP(hipFuncSetAttribute((const void*)k_gmres_dense<24>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)(bytes - 1024)));
   IB_HIP(hipFuncSetAttribute((const void*)k_gmres_coop, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(bytes - 1024)));
+  IB_HIP(hipFuncSetAttribute((const void*)k_gmres_coopw<8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(bytes - 1024)));
+  IB_HIP(hipFuncSetAttribute((const void*)k_gmres_coopw<13>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(bytes - 1024)));
+  IB_HIP(hipFuncSetAttribute((const void*)k_gmres_coopw<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)(bytes - 1024)));
 }
 

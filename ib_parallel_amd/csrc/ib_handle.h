@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <map>
 #include <cstdio>
 #include <cstring>
@@ -125,6 +126,10 @@ struct ib_handle {
   // the dense solve over several workgroups (gmres_coop): workgroups, rows
   // per workgroup, workspace (barrier words, partials, w, Hessenberg copies)
   int coopP = 0, coopR = 0;
+  // the same for larger systems (gmres_coopw): workgroups, rows per
+  // workgroup, basis vectors held in LDS
+  int cwP = 0, cwR = 0, cwNL = 0;
+  int last_xmode = 0;  // exchange of the last wide cooperative solve (1 one XCD's L2, 2 device scope)
   DBuf<double> coopws;
   GmresDev* hst = nullptr;  // pinned
   double* hbuf = nullptr;   // pinned readback
@@ -534,14 +539,25 @@ struct ib_handle {
   // the whole solve in one launch on a dense matrix M: several workgroups
   // (gmres_coop) where the split fits, else one workgroup
   bool dense_solve_ok() const {
-    return coopP > 0 || (nf <= kFusedGmresMax && gmres_dense_fits(nf, kmax, max_lds));
+    if (prm.gmres_form == 3) return false;
+    return coopP > 0 || cwP > 0 || (nf <= kFusedGmresMax && gmres_dense_fits(nf, kmax, max_lds));
   }
   void dense_launch(const double* M, const double* b, double* x) {
     if (coopP > 0) gmres_coop(st, nf, kmax, prm.gmres_max_it, coopP, coopR, M, b, x, gst, prm.rtol, prm.atol, coopws);
+    else if (cwP > 0)
+      gmres_coopw(st, nf, kmax, prm.gmres_max_it, cwP, cwR, cwNL, M, b, x, gst, prm.rtol, prm.atol, coopws);
     else gmres_dense(st, nf, kmax, prm.gmres_max_it, M, b, x, V, H, gst, prm.rtol, prm.atol, max_lds);
+  }
+  // drop the cooperative solves (a concurrent batch whose solves would not
+  // all fit on one XCD's CUs) and give back their workspace
+  void drop_coop() {
+    coopP = coopR = 0;
+    cwP = cwR = cwNL = 0;
+    coopws.release();
   }
   // the status of a dense solve already copied to hst
   int dense_check() {
+    if (cwP > 0 && coopP == 0) last_xmode = hst->xmode;
     if (hst->reason == -3) return fail(IB_E_HIP, "GMRES: a grid barrier of the cooperative solve timed out");
     return hst->reason > 0 ? IB_OK : IB_E_NOCONV_GMRES;
   }
@@ -1364,12 +1380,26 @@ struct ib_handle {
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) ncu = 0;
     (void)hipGetLastError();
-    if (prm.gmres_form < 0 || prm.gmres_form > 2) return fail(IB_E_INVAL, "gmres_form must be 0, 1 or 2");
+    if (prm.gmres_form < 0 || prm.gmres_form > 4) return fail(IB_E_INVAL, "gmres_form must be 0 to 4");
     coopP = coopR = 0;
-    if (prm.gmres_form != 1 && nf > 0 && gmres_coop_plan(nf, kmax, max_lds, &coopP, &coopR) && 8 * coopP <= ncu)
+    cwP = cwR = cwNL = 0;
+    const bool coop_ok = prm.gmres_form == 0 || prm.gmres_form == 2 || prm.gmres_form == 4;
+    if (coop_ok && prm.gmres_form != 4 && nf > 0 && gmres_coop_plan(nf, kmax, max_lds, &coopP, &coopR) &&
+        8 * coopP <= ncu)
       coopws.alloc(gmres_coop_ws(nf, kmax, coopP));
-    else
+    else if (coop_ok && nf > 0 && gmres_coopw_plan(nf, kmax, max_lds, ncu, &cwP, &cwR, &cwNL)) {
       coopP = coopR = 0;
+      // IB_COOPW_LDS_VECTORS=k caps the basis vectors kept in LDS (the rest
+      // go to HBM); the results do not depend on it
+      if (const char* e = std::getenv("IB_COOPW_LDS_VECTORS")) {
+        const int cap = std::atoi(e);
+        if (cap >= 2 && cap < cwNL) cwNL = cap;
+      }
+      coopws.alloc(gmres_coopw_ws(nf, kmax, cwP, cwR, cwNL));
+    } else {
+      coopP = coopR = 0;
+      cwP = cwR = cwNL = 0;
+    }
 
     px.init(m - 1);
     py.init(n - 1);

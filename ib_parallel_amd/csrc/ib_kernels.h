@@ -388,7 +388,8 @@ struct GmresDev {
   double ttol, rn0, bnorm, res;
   int its, it, reason, first;
   double tt;  // norm of the last orthogonalised vector (multi-kernel iterations)
-  int hap, pad;
+  int hap;
+  int xmode;  // exchange of the last cooperative solve: 1 through one XCD's L2, 2 device scope
 };
 void gm_start(hipStream_t st, int nf, int kmax, const double* b, const double* Ax, double* V, GmresDev* s,
               double* H, double* grs, double rtol, double atol);
@@ -416,6 +417,14 @@ bool gmres_coop_plan(int nf, int kmax, size_t max_lds, int* P, int* R);
 size_t gmres_coop_ws(int nf, int kmax, int P);
 void gmres_coop(hipStream_t st, int nf, int kmax, int maxit, int P, int R, const double* S, const double* b, double* x,
                 GmresDev* s, double rtol, double atol, double* ws);
+// The dense solve over P workgroups of one XCD for nf <= 1024 and up to 1024
+// Krylov vectors (S rows in registers, basis rows in LDS up to NL vectors,
+// the rest in HBM; three grid barriers per step at most); plan: P, rows per
+// workgroup R and NL, false if no split fits.  ws holds gmres_coopw_ws doubles.
+bool gmres_coopw_plan(int nf, int kmax, size_t max_lds, int ncu, int* P, int* R, int* NL);
+size_t gmres_coopw_ws(int nf, int kmax, int P, int R, int NL);
+void gmres_coopw(hipStream_t st, int nf, int kmax, int maxit, int P, int R, int NL, const double* S, const double* b,
+                 double* x, GmresDev* s, double rtol, double atol, double* ws);
 // pressure post-processing (src/operators_pressure.F90): cell arrays are
 // m x n, i fastest; pressure boundary vectors [left n | right n | bottom m |
 // top m]

@@ -380,6 +380,20 @@ int ib_schur_info(ib_handle* h, int32_t* mode, int32_t* nfaces, int64_t* green_b
   return IB_OK;
 }
 
+int ib_gmres_info(const ib_handle* h, int32_t* form, int32_t* workgroups, int32_t* exchange) {
+  if (!h) return IB_E_INVAL;
+  int f = 0, P = 1;
+  if (h->assembled && h->dense_solve_ok()) {
+    if (h->coopP > 0) f = 2, P = h->coopP;
+    else if (h->cwP > 0) f = 3, P = h->cwP;
+    else f = 1;
+  }
+  if (form) *form = f;
+  if (workgroups) *workgroups = P;
+  if (exchange) *exchange = h->last_xmode;
+  return IB_OK;
+}
+
 int ib_dst_tiles(const ib_handle* h, int32_t* rp, int32_t* cp, int32_t* source) {
   if (!h) return IB_E_INVAL;
   if (rp) *rp = h->tiles.rp;

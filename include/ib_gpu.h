@@ -83,8 +83,13 @@ typedef struct ib_params {
                               would pass 2 GiB; -1: none) */
   int32_t gmres_form;      /* GMRES on a dense Schur matrix: 0 auto (the
                               cooperative multi-workgroup solve where it
-                              fits, else one workgroup), 1 one workgroup,
-                              2 cooperative where it fits */
+                              fits: nf <= 1024 and restart <= 1023; else
+                              one workgroup up to nf = 400; else one kernel
+                              per Arnoldi stage), 1 one workgroup (nf <=
+                              400, else per-stage kernels), 2 cooperative
+                              where it fits, 3 per-stage kernels always,
+                              4 the cooperative solve for larger systems
+                              where it fits (as 0 otherwise) */
   int32_t reserved[2];
 } ib_params;
 
@@ -205,6 +210,13 @@ int ib_schur_info(ib_handle* h, int32_t* mode, int32_t* nfaces, int64_t* green_b
 int ib_dst_tiles(const ib_handle* h, int32_t* rp, int32_t* cp, int32_t* source);
 int ib_kernel_timing_read(ib_handle* h, int32_t kclass, int64_t* launches, double* total_ms,
                           double* total_bytes);
+/* the GMRES solve in use on a dense Schur matrix (no reference counterpart):
+ * form 0 one kernel per Arnoldi stage (or a matrix-free operator), 1 one
+ * workgroup, 2 the cooperative solve for nf <= 256, 3 the cooperative solve
+ * for larger systems; workgroups it spans; exchange of the last form-3
+ * solve: 0 none yet, 1 through one XCD's L2 (all its workgroups were found
+ * on one XCD), 2 device scope (they were not). */
+int ib_gmres_info(const ib_handle* h, int32_t* form, int32_t* workgroups, int32_t* exchange);
 
 /* --- sub-domain approach: ib.inp with sub_domain = T (src/timestepper.F90:
  * 79-97,368-405,602-770; src/grid.F90:320-348,572-951; src/variables.F90:

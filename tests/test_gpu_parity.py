@@ -305,6 +305,79 @@ def test_cooperative_gmres_equals_single_workgroup_gmres(case):
     assert rel(x1, x2) < 1e-8
 
 
+def _gmres_forms(mk, forms, seed=3, env=None, monkeypatch=None, info=None):
+    """Solve S x = S xt from 0.9 xt under each gmres_form on the same S."""
+    out = []
+    for form in forms:
+        if monkeypatch is not None:
+            monkeypatch.delenv("IB_COOPW_LDS_VECTORS", raising=False)
+            monkeypatch.delenv("IB_GMRES_L2_EXCHANGE", raising=False)
+            for k, v in (env or {}).get(form, {}).items():
+                monkeypatch.setenv(k, v)
+        g = gpu(*mk(form))
+        S = g.schur_matrix()
+        rng = np.random.default_rng(seed)
+        xt = rng.standard_normal(g.nf)
+        rhs = S @ xt
+        x, its = g.gmres(rhs, x0=0.9 * xt)
+        out.append((x, its, np.linalg.norm(S @ x - rhs) / np.linalg.norm(rhs)))
+        if info is not None:
+            info.append(g.gmres_info())
+        g.close()
+    return out
+
+
+@pytest.mark.parametrize("case", ("tiny", "plate", "cylinder"))
+def test_wide_cooperative_gmres_equals_other_forms(case):
+    """The cooperative solve laid out for larger systems (gmres_form 4: S
+    rows in registers by row pairs, basis rows in LDS row-major) against
+    the per-stage kernels (gmres_form 3) and the other dense solves (2: the
+    small cooperative one, 1: one workgroup): the same PETSc iteration with
+    partial sums in another order, so the iteration counts agree within one
+    and the solutions to rounding.  The cylinder has nf = 314, past the small
+    cooperative solve (256)."""
+    if case == "tiny":
+        mk = lambda f: decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-12, gmres_form=f, gmres_restart=80)
+        forms = (4, 2, 3)
+    elif case == "plate":
+        mk = lambda f: decks.plate_case(gmres_form=f)
+        forms = (4, 2, 3)
+    else:
+        mk = lambda f: decks.cylinder_case(tol=1e-12, gmres_restart=400, gmres_form=f)
+        forms = (4, 1, 3)
+    out = _gmres_forms(mk, forms)
+    x4, i4, r4 = out[0]
+    assert r4 < 1e-10
+    for x, its, r in out[1:]:
+        assert abs(its - i4) <= 1, (its, i4)
+        assert r < 1e-10
+        assert rel(x4, x) < 1e-8
+
+
+def test_wide_cooperative_gmres_airfoil_flap(monkeypatch):
+    """At the airfoil + flap's nf = 800 with full GMRES (restart 1000): the
+    cooperative solve (the auto choice) against the per-stage kernels; with
+    only 64 basis vectors in LDS (the rest in HBM); and with the exchange
+    between its workgroups at device scope instead of through the L2 of
+    their one XCD.  The last two change no arithmetic: the same bits."""
+    mk = lambda f: decks.airfoil_flap_case(gmres_form=f)
+    info = []
+    env = {4: {"IB_COOPW_LDS_VECTORS": "64"}, 2: {"IB_GMRES_L2_EXCHANGE": "0"}}
+    out = _gmres_forms(mk, (0, 3, 4, 2), env=env, monkeypatch=monkeypatch, info=info)
+    (x0, i0, r0), (x3, i3, r3), (x4, i4, r4), (x2, i2, r2) = out
+    print(f"airfoil+flap GMRES: its {i0} / {i3}, residuals {r0:.2e} / {r3:.2e}, rel x {rel(x0, x3):.2e}; {info}")
+    assert info[0][:2] == (3, 32) and info[1][0] == 0, info
+    # the 32 workgroups land on one XCD, so the exchange goes through its L2
+    assert info[0][2] == 1 and info[2][2] == 1 and info[3][2] == 2, info
+    assert i2 == i0 and np.array_equal(x2, x0)
+    assert i0 > 64, i0
+    # cond(S) ~ 1e8: the true residual may sit a little above the estimate
+    assert r0 < 1e-9 and r3 < 1e-9
+    assert abs(i0 - i3) <= max(2, 0.01 * i3), (i0, i3)
+    assert rel(x0, x3) < 1e-4
+    assert i4 == i0 and np.array_equal(x4, x0)
+
+
 def test_dst_tile_shapes_leave_results_unchanged(monkeypatch):
     """The DST tile shapes (timed at create, forced, or the fixed rule) only
     decide how many transforms share a workgroup's LDS: three steps give the
