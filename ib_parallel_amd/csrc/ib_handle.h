@@ -555,6 +555,18 @@ struct ib_handle {
     cwP = cwR = cwNL = 0;
     coopws.release();
   }
+  // Operand bytes of a dense GMRES solve of its iterations: per Arnoldi step
+  // the matrix (nf^2 doubles), the basis vectors 0..j read twice by
+  // classical Gram-Schmidt (dots and update) and the new vector written,
+  // with j the step's index in its restart cycle; b and x read, x written.
+  // What a solve that streamed its operands from HBM would move; the
+  // cooperative solves keep S and the basis on chip.
+  double gmres_operand_bytes(int its) const {
+    const double n = nf, kc = std::max(kmax, 1);
+    const double cyc = std::floor(its / kc), rem = its - cyc * kc;
+    const double sum_j1 = cyc * kc * (kc + 1) / 2 + rem * (rem + 1) / 2;  // sum of (j + 1)
+    return 8.0 * (its * n * n + 2.0 * n * sum_j1 + its * n + 3.0 * n);
+  }
   // the status of a dense solve already copied to hst
   int dense_check() {
     if (cwP > 0 && coopP == 0) last_xmode = hst->xmode;
@@ -574,11 +586,11 @@ struct ib_handle {
       // dense Schur matrix: the whole solve is one launch.  Algorithmic
       // bytes: S, b read once, x read and written, and one Krylov vector
       // written per iteration (added once the count is known)
-      tim[4].begin(st, 8.0 * ((double)nf * nf + 3.0 * nf));
+      tim[4].begin(st, 0.0);
       dense_launch(Sd, b, x);
       tim[4].end(st);
       const int rc = dense_status();
-      if (tim[4].on) tim[4].total_bytes += 8.0 * (double)nf * (hst->its + 1);
+      if (tim[4].on) tim[4].total_bytes += gmres_operand_bytes(hst->its);
       *its_out = hst->its;
       return rc;
     }
@@ -913,7 +925,7 @@ struct ib_handle {
       int rc = IB_OK;
       if (one_sync) {
         fsi_rhs(st, g, nf, tors_of, tb, ntors, th, dt, delta, w, csr.key, qx, qy, q0x[0], rhsf);
-        tim[4].begin(st, 8.0 * ((double)nf * nf + 3.0 * nf));
+        tim[4].begin(st, 0.0);
         dense_launch(Sd, rhsf, f);
         tim[4].end(st);
         redist_faces(st, g, csr, w, f, hx, hy);
@@ -939,7 +951,7 @@ struct ib_handle {
         if (hflag[2]) csr.build(st);
         rc = dense_check();
         its = hst->its;
-        if (tim[4].on) tim[4].total_bytes += 8.0 * (double)nf * (its + 1);
+        if (tim[4].on) tim[4].total_bytes += gmres_operand_bytes(its);
         ksp_iter += its;
         if (rc != IB_OK) status = rc;
         if (hflag[0]) {

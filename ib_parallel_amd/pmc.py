@@ -1,18 +1,26 @@
-"""HBM traffic per kernel launch from two rocprofv3 counter runs.
+"""Per-kernel HBM traffic and FP64 work from rocprofv3 counter runs, and a
+markdown summary of a --kernel-trace --stats run.
 
-Usage (on a machine with the GPU, each counter in a run of its own):
+Usage (on a machine with the GPU, each counter set in a run of its own):
 
-    rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT/fetch -o run -- \\
-        python bench.py --steps 2 --warmup 0 --no-cpu --pmc-copy-mb 256
-    rocprofv3 --pmc WRITE_SIZE --output-format csv -d OUT/write -o run -- \\
-        python bench.py --steps 2 --warmup 0 --no-cpu --pmc-copy-mb 256
-    python -m ib_parallel_amd.pmc OUT/fetch OUT/write --copy-mb 256 --json profiles/pmc_traffic.json
+    B="python3 -u bench.py --steps 4 --warmup 2 --no-cpu --batch 0 --adjoint-steps 0 \\
+       --adjoint-fsi-steps 0 --window-steps 0 --pmc-copy-mb 256"
+    rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT/fetch -o run -- $B
+    rocprofv3 --pmc WRITE_SIZE --output-format csv -d OUT/write -o run -- $B
+    rocprofv3 --pmc SQ_INSTS_VALU_ADD_F64 SQ_INSTS_VALU_MUL_F64 SQ_INSTS_VALU_FMA_F64 \\
+        SQ_INSTS_VALU_TRANS_F64 --output-format csv -d OUT/f64 -o run -- $B
+    python -m ib_parallel_amd.pmc traffic OUT/fetch OUT/write --copy-mb 256 --f64 OUT/f64 \\
+        --json profiles/pmc_traffic.json
+    python -m ib_parallel_amd.pmc stats OUT/trace        # a --kernel-trace --stats run
 
 FETCH_SIZE and WRITE_SIZE are read in KiB per dispatch.  The device-to-device
-copy of known size that --pmc-copy-mb adds to the same run calibrates them:
-the copy reads and writes exactly copy_mb MiB, so bytes = counter * (copy
-bytes / counter of the copy).  The JSON gives, per kernel, the mean counters
-per dispatch, the two scale factors and the calibrated HBM bytes per launch.
+copy of known size that --pmc-copy-mb adds to the run calibrates them: the
+copy reads and writes exactly copy_mb MiB, so bytes = counter * (copy bytes /
+counter of the copy).  bench.py issues that copy right after its warm-up
+steps, so only the dispatches after the copy (the timed steps; the one-time
+set-up comes before it) are averaged.  FP64 operations per dispatch = 64 *
+(ADD + MUL + TRANS + 2 FMA) wavefront instructions (all 64 lanes counted,
+masked ones included).
 """
 import argparse
 import csv
@@ -23,21 +31,39 @@ import re
 from collections import defaultdict
 
 
-def read_counters(folder, counter):
-    """{kernel name: [value per dispatch]} from a rocprofv3 counter_collection CSV."""
+def read_counters(folder, counters):
+    """{kernel name: {dispatch id: {counter: value}}} from rocprofv3
+    counter_collection CSVs; the pseudo-counter "ns" is the dispatch's
+    duration."""
     files = glob.glob(os.path.join(folder, "**", "*counter_collection*.csv"), recursive=True)
     if not files:
         raise SystemExit(f"no counter_collection CSV under {folder}")
-    per = defaultdict(lambda: defaultdict(float))
+    per = defaultdict(lambda: defaultdict(lambda: defaultdict(float)))
     for f in files:
         with open(f) as fh:
             for row in csv.DictReader(fh):
-                if row.get("Counter_Name", "").strip() != counter:
+                c = row.get("Counter_Name", "").strip()
+                if c not in counters:
                     continue
                 name = row.get("Kernel_Name", "")
-                key = (row.get("Dispatch_Id") or row.get("Correlation_Id") or str(len(per[name])))
-                per[name][key] += float(row["Counter_Value"])  # summed over dimensions of one dispatch
-    return {k: list(v.values()) for k, v in per.items()}
+                key = int(row.get("Dispatch_Id") or row.get("Correlation_Id") or 0)
+                per[name][key][c] += float(row["Counter_Value"])  # summed over dimensions of one dispatch
+                if row.get("Start_Timestamp") and row.get("End_Timestamp"):
+                    per[name][key]["ns"] = float(row["End_Timestamp"]) - float(row["Start_Timestamp"])
+    return per
+
+
+def copy_marker(runs):
+    """Dispatch id of the calibration copy: the longest dispatch of a copy
+    kernel (the runtime also copies small status blocks with the same kernel)."""
+    best = None
+    for k, disp in runs.items():
+        if "copy" not in k.lower():
+            continue
+        for d, v in disp.items():
+            if best is None or v.get("ns", 0.0) > best[1]:
+                best = (d, v.get("ns", 0.0))
+    return None if best is None else best[0]
 
 
 def short(name):
@@ -45,42 +71,123 @@ def short(name):
     return m.group(1) if m else name.split("(")[0][:80]
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("fetch_dir")
-    ap.add_argument("write_dir")
-    ap.add_argument("--copy-mb", type=float, required=True)
-    ap.add_argument("--json", required=True)
-    args = ap.parse_args()
-    fetch = read_counters(args.fetch_dir, "FETCH_SIZE")
-    write = read_counters(args.write_dir, "WRITE_SIZE")
+def _mean(vals):
+    return sum(vals) / len(vals) if vals else 0.0
+
+
+def traffic(args):
+    fetch = read_counters(args.fetch_dir, {"FETCH_SIZE"})
+    write = read_counters(args.write_dir, {"WRITE_SIZE"})
     copy_kib = args.copy_mb * 1024.0
-    # the copy: the kernel whose mean FETCH_SIZE is closest to the copy size
     cands = [k for k in fetch if "copy" in k.lower()]
     if not cands:
         raise SystemExit("no copy kernel in the counter run (pass --pmc-copy-mb to bench.py)")
-    ck = min(cands, key=lambda k: abs(max(fetch[k]) - copy_kib))
-    cf = max(fetch[ck])
-    cw = max(write.get(ck, [0.0]))
+
+    def copy_of(runs, counter):
+        # the dispatch of a copy kernel whose counter is closest to the copy size
+        best = None
+        for k in cands:
+            for d, v in runs.get(k, {}).items():
+                x = v.get(counter, 0.0)
+                if best is None or abs(x - copy_kib) < abs(best[2] - copy_kib):
+                    best = (k, d, x)
+        return best
+    ck, cdf, cf = copy_of(fetch, "FETCH_SIZE")
+    _, cdw, cw = copy_of(write, "WRITE_SIZE")
     sf = copy_kib / cf if cf > 0 else None
     sw = copy_kib / cw if cw > 0 else None
+    f64 = read_counters(args.f64, {"SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_FMA_F64",
+                                   "SQ_INSTS_VALU_TRANS_F64"}) if args.f64 else {}
+    # the marker dispatch of each run: its calibration copy; later dispatches are the timed steps
+    f64_mark = copy_marker(f64) if f64 else None
     out = {"copy": {"kernel": ck, "bytes": copy_kib * 1024.0, "fetch_kib": cf, "write_kib": cw,
-                    "scale_fetch": sf, "scale_write": sw}, "kernels": {}}
-    for k in sorted(set(fetch) | set(write)):
-        if k == ck:
+                    "scale_fetch": sf, "scale_write": sw},
+           "window": "dispatches after the calibration copy (the timed steps)", "kernels": {}}
+    names = defaultdict(list)
+    for k in set(fetch) | set(write):
+        if "copy" not in k.lower():
+            names[short(k)].append(k)
+    for sk, full in sorted(names.items()):
+        fv = [v["FETCH_SIZE"] for k in full for d, v in fetch.get(k, {}).items() if d > cdf]
+        wv = [v["WRITE_SIZE"] for k in full for d, v in write.get(k, {}).items() if d > cdw]
+        if not fv and not wv:
             continue
-        fv, wv = fetch.get(k, []), write.get(k, [])
-        mf = sum(fv) / len(fv) if fv else 0.0
-        mw = sum(wv) / len(wv) if wv else 0.0
-        out["kernels"][short(k)] = {
-            "dispatches": max(len(fv), len(wv)), "fetch_kib": mf, "write_kib": mw,
-            "bytes_per_launch": (mf * (sf or 1.0) + mw * (sw or 1.0)) * 1024.0}
+        mf, mw = _mean(fv), _mean(wv)
+        ent = {"dispatches": max(len(fv), len(wv)), "fetch_kib": mf, "write_kib": mw,
+               "bytes_per_launch": (mf * (sf or 1.0) + mw * (sw or 1.0)) * 1024.0}
+        if f64:
+            ops = []
+            for k in full:
+                for d, v in f64.get(k, {}).items():
+                    if f64_mark is None or d > f64_mark:
+                        ops.append(64.0 * (v["SQ_INSTS_VALU_ADD_F64"] + v["SQ_INSTS_VALU_MUL_F64"]
+                                           + v["SQ_INSTS_VALU_TRANS_F64"] + 2.0 * v["SQ_INSTS_VALU_FMA_F64"]))
+            ent["fp64_ops_per_launch"] = _mean(ops)
+        out["kernels"][sk] = ent
     os.makedirs(os.path.dirname(os.path.abspath(args.json)), exist_ok=True)
     with open(args.json, "w") as fh:
         json.dump(out, fh, indent=1, sort_keys=True)
     print(json.dumps(out["copy"]))
-    for k, v in sorted(out["kernels"].items(), key=lambda kv: -kv[1]["bytes_per_launch"])[:12]:
-        print(f"{k:60s} {v['dispatches']:6d} {v['bytes_per_launch'] / 1e6:12.3f} MB/launch")
+    print("| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM MB/launch (calibrated) | FP64 Mop/launch |")
+    print("|---|---|---|---|---|---|")
+    for k, v in sorted(out["kernels"].items(), key=lambda kv: -kv[1]["bytes_per_launch"] * kv[1]["dispatches"]):
+        fo = v.get("fp64_ops_per_launch")
+        print(f"| {k} | {v['dispatches']} | {v['fetch_kib']:.1f} | {v['write_kib']:.1f} | "
+              f"{v['bytes_per_launch'] / 1e6:.3f} | {'' if fo is None else f'{fo / 1e6:.3f}'} |")
+
+
+def stats(args):
+    """Markdown table of a rocprofv3 --kernel-trace --stats run: the
+    whole-process kernel_stats.csv, or with --after-copy the kernel trace
+    restricted to the dispatches after the calibration copy (the timed
+    steps of a bench.py run with --pmc-copy-mb)."""
+    if args.after_copy:
+        files = glob.glob(os.path.join(args.trace_dir, "**", "*kernel_trace.csv"), recursive=True)
+        if not files:
+            raise SystemExit(f"no kernel_trace.csv under {args.trace_dir}")
+        disp = []
+        with open(files[0]) as fh:
+            for r in csv.DictReader(fh):
+                disp.append((int(r["Dispatch_Id"]), r["Kernel_Name"],
+                             float(r["End_Timestamp"]) - float(r["Start_Timestamp"])))
+        copies = [d for d in disp if "copy" in d[1].lower()]
+        mark = max(copies, key=lambda d: d[2])[0] if copies else -1
+        agg = defaultdict(list)
+        for d, name, ns in disp:
+            if d > mark and "copy" not in name.lower():
+                agg[name].append(ns)
+        rows = [{"Name": k, "Calls": len(v), "TotalDurationNs": sum(v), "AverageNs": sum(v) / len(v),
+                 "MinNs": min(v), "MaxNs": max(v)} for k, v in agg.items()]
+    else:
+        files = glob.glob(os.path.join(args.trace_dir, "**", "*kernel_stats.csv"), recursive=True)
+        if not files:
+            raise SystemExit(f"no kernel_stats.csv under {args.trace_dir}")
+        with open(files[0]) as fh:
+            rows = list(csv.DictReader(fh))
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    print("| kernel | calls | total ms | average us | min us | max us | share |")
+    print("|---|---|---|---|---|---|---|")
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"]))[: args.top]:
+        t = float(r["TotalDurationNs"])
+        print(f"| {short(r['Name'])} | {r['Calls']} | {t / 1e6:.3f} | {float(r['AverageNs']) / 1e3:.2f} | "
+              f"{float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | {t / tot:.3f} |")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_dir")
+    t.add_argument("write_dir")
+    t.add_argument("--copy-mb", type=float, required=True)
+    t.add_argument("--f64", default=None, help="folder of the FP64 instruction counter run")
+    t.add_argument("--json", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("trace_dir")
+    s.add_argument("--top", type=int, default=25)
+    s.add_argument("--after-copy", action="store_true", help="only the dispatches after the calibration copy")
+    args = ap.parse_args()
+    traffic(args) if args.cmd == "traffic" else stats(args)
 
 
 if __name__ == "__main__":

@@ -190,12 +190,13 @@ int ib_op_nonlinear_rot(ib_handle* h, int32_t k, const double* omega, const doub
 /* --- instrumentation (no reference counterpart): per-kernel-class timing
  * with HIP events recorded around each launch on the handle's stream.
  * class 0 = DST row pass, 1 = DST column pass of the time step; 2 and 3 the
- * batched row and column passes that assemble the Schur matrix; 4 the
- * single-workgroup dense GMRES solve.  total_bytes accumulates the
- * algorithmic HBM bytes of the timed launches (DST passes: each FP64 field
- * array read or written once, spectral tables read once per launch; GMRES:
- * S and b read once, x read and written, one Krylov vector written per
- * iteration). --- */
+ * batched row and column passes that assemble the Schur matrix; 4 the dense
+ * GMRES solve (one launch: cooperative or one workgroup).  total_bytes
+ * accumulates the algorithmic bytes of the timed launches (DST passes: each
+ * FP64 field array read or written once, spectral tables read once per
+ * launch; GMRES: the operands of every Arnoldi step -- S, the basis vectors
+ * 0..j twice for classical Gram-Schmidt, the new vector written -- plus b
+ * and x). --- */
 int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable);
 /* the Schur form in use (1 matrix-free, 2 assembled by columns, 3 face-space
  * G), the size of the face set and how many times G has been computed */
