@@ -206,6 +206,7 @@ void ib_handle::adjoint_pos_setup() {
   sync();
   trim_columns();
   Gm.release(); Tm.release();
+  tcache = false;
   gpx.alloc(nf); gpy.alloc(nf);
   adj_pos = true;
 }

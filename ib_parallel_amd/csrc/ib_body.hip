@@ -401,30 +401,34 @@ void map_taps(hipStream_t st, int nf, const int* key, const int* fmap, int* tapi
   IB_LAUNCH_CHECK();
 }
 
-// T(:, j) = sum_q w[j,q] G(:, tapidx[j,q]): one thread per face row b.  A
-// tap outside the face set (tapidx -1, which k_map_taps flags) reads face 0:
-// the FSI loop forms S before it has read that flag and forms it again
-// after rebuilding the set.
+// T(:, j) = sum_q w[j,q] G(:, tapidx[j,q]): one thread per face row b,
+// column j = jl[blockIdx.y] (jl NULL: blockIdx.y).  A tap outside the face
+// set (tapidx -1, which k_map_taps flags) reads face 0: the FSI loop forms S
+// before it has read that flag and forms it again after rebuilding the set.
 __global__ void __launch_bounds__(256) k_green_right(int nF, const double* __restrict__ w,
                                                      const int* __restrict__ tapidx, const double* __restrict__ G,
-                                                     double* __restrict__ T) {
+                                                     const int* __restrict__ jl, double* __restrict__ T) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
+  const int j = jl ? jl[blockIdx.y] : blockIdx.y;
   if (b >= nF) return;
   double acc = 0.0;
   for (int q = 0; q < kTaps; ++q) acc += w[j * kTaps + q] * G[(long)max(tapidx[j * kTaps + q], 0) * nF + b];
   T[(long)j * nF + b] = acc;
 }
 
-// S(i, j) = sum_p w[i,p] T(tapidx[i,p], j) + b_times2(e_j)(i): one thread per row i
-__global__ void __launch_bounds__(256) k_green_left(int nf, int nF, const double* __restrict__ w,
+// S(i, j) = sum_p w[i,p] T(tapidx[i,p], j) + b_times2(e_j)(i): one thread per
+// row i = il[x] (il NULL: i = x) for x < ni, column j = jl[blockIdx.y] (jl
+// NULL: blockIdx.y)
+__global__ void __launch_bounds__(256) k_green_left(int nf, int nF, int ni, const int* __restrict__ il,
+                                                    const int* __restrict__ jl, const double* __restrict__ w,
                                                     const int* __restrict__ tapidx, const double* __restrict__ T,
                                                     const int* __restrict__ tors_of, const TorsBody* __restrict__ tbs,
                                                     const double* __restrict__ theta, double dt, double delta,
                                                     double* __restrict__ S) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
-  if (i >= nf) return;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= ni) return;
+  const int i = il ? il[x] : x;
+  const int j = jl ? jl[blockIdx.y] : blockIdx.y;
   const double* Tj = T + (long)j * nF;
   double acc = 0.0;
   for (int p = 0; p < kTaps; ++p) acc += w[i * kTaps + p] * Tj[max(tapidx[i * kTaps + p], 0)];
@@ -435,11 +439,30 @@ void schur_from_green(hipStream_t st, int nf, int nF, const double* w, const int
                       double* T, const int* tors_of, const TorsBody* tb, const double* theta, double dt,
                       double delta, double* S) {
   if (nf <= 0 || nF <= 0) return;
-  hipLaunchKernelGGL(k_green_right, dim3(ceil_div(nF, 256), nf), dim3(256), 0, st, nF, w, tapidx, G, T);
+  hipLaunchKernelGGL(k_green_right, dim3(ceil_div(nF, 256), nf), dim3(256), 0, st, nF, w, tapidx, G, nullptr, T);
   IB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_green_left, dim3(ceil_div(nf, 256), nf), dim3(256), 0, st, nf, nF, w, tapidx, T, tors_of, tb,
-                     theta, dt, delta, S);
+  hipLaunchKernelGGL(k_green_left, dim3(ceil_div(nf, 256), nf), dim3(256), 0, st, nf, nF, nf, nullptr, nullptr, w,
+                     tapidx, T, tors_of, tb, theta, dt, delta, S);
   IB_LAUNCH_CHECK();
+}
+
+void schur_from_green_moving(hipStream_t st, int nf, int nF, const double* w, const int* tapidx, const double* G,
+                             double* T, const int* tors_of, const TorsBody* tb, const double* theta, double dt,
+                             double delta, double* S, int nmov, const int* mov, int nrig, const int* rig) {
+  if (nf <= 0 || nF <= 0 || nmov <= 0) return;
+  // the moving columns of T
+  hipLaunchKernelGGL(k_green_right, dim3(ceil_div(nF, 256), nmov), dim3(256), 0, st, nF, w, tapidx, G, mov, T);
+  IB_LAUNCH_CHECK();
+  // every row of the moving columns of S
+  hipLaunchKernelGGL(k_green_left, dim3(ceil_div(nf, 256), nmov), dim3(256), 0, st, nf, nF, nf, nullptr, mov, w,
+                     tapidx, T, tors_of, tb, theta, dt, delta, S);
+  IB_LAUNCH_CHECK();
+  // the moving rows of the other columns
+  if (nrig > 0) {
+    hipLaunchKernelGGL(k_green_left, dim3(ceil_div(nmov, 256), nrig), dim3(256), 0, st, nf, nF, nmov, mov, rig, w,
+                       tapidx, T, tors_of, tb, theta, dt, delta, S);
+    IB_LAUNCH_CHECK();
+  }
 }
 
 // redistribute2 on the touched faces: finter = 1/wght * frc where wght > 1e-10

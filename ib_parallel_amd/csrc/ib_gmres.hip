@@ -1178,11 +1178,16 @@ void gmres_coop(hipStream_t st, int nf, int kmax, int maxit, int P, int R, const
 //    all of them, so all hold identical h and take the same branches;
 //  * the exchange goes through the L2 of the one XCD the workgroups share
 //    (checked with HW_REG_XCC_ID at the start; device scope otherwise);
-//  * only workgroup 0 keeps the rotated Hessenberg matrix (in HBM).  At the
-//    end of a cycle it runs the back substitution in blocks of kCwBlk
-//    columns (diagonal block by one wavefront from LDS, the rows above by
-//    all threads) and publishes y; every workgroup then adds V y to its rows
-//    of x.
+//  * the Givens step needs only entry it of the new column after the
+//    previous rotations, which is sum_j alpha_j h_j with alpha updated as
+//    [-s_it alpha, c_it] after each step: one more term in the block
+//    reduction of the norm, instead of a serial scan over the rotations;
+//  * only workgroup 0 keeps the Hessenberg matrix (in HBM, unrotated, with
+//    the rotated diagonal beside it).  At the end of a cycle it applies the
+//    rotations to every column (one thread per column), runs the back
+//    substitution in blocks of kCwBlk columns (diagonal block by one
+//    wavefront from LDS, the rows above by all threads) and publishes y;
+//    every workgroup then adds V y to its rows of x.
 // ------------------------------------------------------------------------
 constexpr int kCwBlk = 32;
 
@@ -1201,6 +1206,7 @@ struct GmCoopWArgs {
   double* Hg;      // kmax x (kmax + 1): the rotated Hessenberg, column it at Hg + it (kmax + 1)
   double* yg;      // kmax + 1: the back substitution's y
   double* hg;      // kmax + 1: the summed CGS coefficients of the step
+  double* Rd;      // kmax + 1: diagonal of the rotated Hessenberg matrix
   double* Vg;      // P x R x (kmax + 1 - NL): basis vectors NL.. of each workgroup's rows
 };
 
@@ -1303,6 +1309,28 @@ __device__ __forceinline__ bool xc_sync(unsigned* sync, int P, int c, unsigned k
   return *s_err == 0;
 }
 
+// Block-wide sums of two values (every thread gets both), in a fixed order:
+// each wavefront's sum (gm_wave_sum), then the 16 wavefront sums by a row of
+// 16 lanes (gm_row_sum16).  red holds at least 2 kGmT / 64 doubles; it is
+// read after the one workgroup barrier here, so the caller must pass another
+// barrier before red is written again.
+__device__ __forceinline__ void gw_block_sum_pair(double& a, double& b, double* red) {
+  const int t = gm_opaque(threadIdx.x), l = t & 63;
+  a = gm_wave_sum(a);
+  b = gm_wave_sum(b);
+  if (l == 0) {
+    red[t >> 6] = a;
+    red[kGmT / 64 + (t >> 6)] = b;
+  }
+  __syncthreads();
+  // lanes 0..15 of every wavefront take the 16 partials of a, lanes 16..31
+  // those of b: one LDS read per lane, then a sum over each row of 16 lanes
+  double v = l < 2 * (kGmT / 64) ? red[l] : 0.0;
+  v = gm_row_sum16(v);
+  a = gm_readlane(v, 0);
+  b = gm_readlane(v, 16);
+}
+
 template <bool X2>
 __device__ __forceinline__ double gw_wave_gather(const double* p, long stride, int P, int lane) {
   const double v = lane < P ? xc_ld<X2>(p + (long)lane * stride) : 0.0;
@@ -1316,11 +1344,10 @@ __device__ __forceinline__ void coopw_solve(const GmCoopWArgs& a, int c, unsigne
   const int r0 = c * R, Rc = max(0, min(nf, r0 + R) - r0);
   const int ldg = ld - NL;  // row stride of the basis vectors kept in HBM
   double* Vgc = a.Vg + (size_t)c * R * (size_t)max(ldg, 0);
-  double* vfull = dyn;                 // nf: the vector being multiplied
-  double* wl = vfull + nf;             // 32: local rows of w
+  double* wl = dyn;                    // 32: local rows of w
   double* hcol = wl + 32;              // kmax + 2: CGS coefficients (y at a cycle's end)
-  double* hrot = hcol + kmax + 2;      // kmax + 2: the new Hessenberg column after the rotations
-  double* lcc = hrot + kmax + 2;       // kmax
+  double* alpha = hcol + kmax + 2;     // kmax + 2: entry it of the rotated column = sum_j alpha_j h_j
+  double* lcc = alpha + kmax + 2;      // kmax
   double* lss = lcc + kmax;            // kmax
   double* grs = lss + kmax;            // kmax + 1 (y of the back substitution in place)
   double* stg = grs + kmax + 1;        // kCwBlk^2: a diagonal block of the back substitution
@@ -1339,22 +1366,31 @@ __device__ __forceinline__ void coopw_solve(const GmCoopWArgs& a, int c, unsigne
   }
   __syncthreads();
 
-  // wl = S[local rows] vfull (ends with a barrier)
-  auto matvec = [&]() {
+  // S[local rows] u for a vector u another workgroup step left in HBM/L2:
+  // every lane loads the CQ entries it multiplies (no LDS copy of u); lanes
+  // 0 and 1 of wavefront q get rows 2q and 2q + 1 (0 in the other lanes)
+  auto matvec_g = [&](const double* u) -> double {
     const int q = gm_opaque(threadIdx.x) >> 6, l = gm_opaque(threadIdx.x) & 63;
+    double sum = 0.0;
     if (2 * q < Rc) {  // uniform over the wavefront
+      double v[CQ];
+#pragma unroll
+      for (int k = 0; k < CQ; ++k) v[k] = xc_ld<X2>(u + min(l + 64 * k, nf - 1));
       double acc0 = 0.0, acc1 = 0.0;
 #pragma unroll
       for (int k = 0; k < CQ; ++k) {
-        const double v = vfull[min(l + 64 * k, nf - 1)];
-        acc0 += s0[k] * v;  // s0, s1 are 0 past the matrix
-        acc1 += s1[k] * v;
+        acc0 += s0[k] * v[k];  // s0, s1 are 0 past the matrix
+        acc1 += s1[k] * v[k];
       }
-      const double sum = gw_reduce2(acc0, acc1);
-      const int row = 2 * q + (l & 1);
-      if (l < 2 && row < Rc) wl[row] = sum;
+      sum = gw_reduce2(acc0, acc1);
     }
-    __syncthreads();
+    return sum;
+  };
+  // lanes 0/1 of the row-pair wavefronts store their row of S u times scale
+  auto store_rows = [&](double sum, double scale) {
+    const int q = gm_opaque(threadIdx.x) >> 6, l = gm_opaque(threadIdx.x) & 63;
+    const int row = 2 * q + (l & 1);
+    if (2 * q < Rc && l < 2 && row < Rc) wl[row] = sum * scale;
   };
   // acc0/acc1 += sum_{j = l, l + 64, ... < jn} coef[j] V(ra / rb, j)
   auto rowpair = [&](const double* coef, int jn, int ra, int rb, int l, double& acc0, double& acc1) {
@@ -1372,32 +1408,12 @@ __device__ __forceinline__ void coopw_solve(const GmCoopWArgs& a, int c, unsigne
       acc1 += h * Vgc[(size_t)rb * ldg + (j - NL)];
     }
   };
-  // the previous rotations applied to the new column, as in k_gmres_coop
-  auto rotate_prev = [&](int it, int lane) {
-    double carry = hcol[0];
-    for (int base = 0; base < it; base += 64) {
-      const int j = base + lane;
-      const bool act = j < it;
-      const double cj0 = act ? lcc[j] : 1.0, sj = act ? lss[j] : 0.0, hn = act ? hcol[j + 1] : 0.0;
-      double A = -sj, B = cj0 * hn;
-      gm_affine_scan(A, B);
-      const double cnext = A * carry + B;
-      const double cj = gm_shift_up1(cnext, carry);
-      if (act) hrot[j] = cj0 * cj + sj * hn;
-      carry = gm_readlane(cnext, min(63, it - 1 - base));
-    }
-    if (lane == 0) hrot[it] = carry;
-  };
-  // a wavefront that owns no rows runs the rotation scan beside the update
-  const bool scan_early = 2 * (T / 64 - 1) >= Rc;
-
   int its = 0, reason = 0, first = 1;
   double rn0 = 0.0, ttol = 0.0;
   while (true) {
-    // ---- cycle start: r = b - S x (x gathered from every workgroup's rows)
-    for (int i = gm_opaque(threadIdx.x); i < nf; i += T) vfull[i] = xc_ld<X2>(a.x + i);
+    // ---- cycle start: r = b - S x (x from every workgroup's rows)
+    store_rows(matvec_g(a.x), 1.0);
     __syncthreads();
-    matvec();
     double pr = 0.0, pb = 0.0;
     for (int r = gm_opaque(threadIdx.x); r < Rc; r += T) {
       const double bv = a.b[r0 + r];
@@ -1431,10 +1447,15 @@ __device__ __forceinline__ void coopw_solve(const GmCoopWArgs& a, int c, unsigne
     if (res0 == 0.0 || res0 <= ttol) reason = 1;
     else if (res0 >= 1e4 * rn0) reason = -1;
     if (reason) break;
-    if (threadIdx.x == 0) grs[0] = res0;
-    for (int i = gm_opaque(threadIdx.x); i < nf; i += T) vfull[i] = xc_ld<X2>(a.wfull + i) / res0;
+    if (threadIdx.x == 0) {
+      grs[0] = res0;
+      alpha[0] = 1.0;
+    }
+    // v_0 = r / res0 and w = S v_0 (the row sums of S r scaled by 1 / res0)
+    const double inv0 = 1.0 / res0;
+    store_rows(matvec_g(a.wfull), inv0);
     for (int r = gm_opaque(threadIdx.x); r < Rc; r += T) {
-      const double v = xc_ld<X2>(a.wfull + r0 + r) / res0;
+      const double v = xc_ld<X2>(a.wfull + r0 + r) * inv0;
       if (NL > 0) Vl[r * NL] = v;
       else Vgc[(size_t)r * ldg] = v;
     }
@@ -1442,7 +1463,7 @@ __device__ __forceinline__ void coopw_solve(const GmCoopWArgs& a, int c, unsigne
     int it = 0;
     bool ok = true;
     while (true) {
-      matvec();  // w = S v_it, local rows
+      // here wl = S v_it on the local rows
       // partial CGS coefficient <w, v_j> over the local rows: g lanes per
       // j, lane sub takes rows sub, sub + g, ...
       const int g = gw_lanes_per_coef(it);
@@ -1483,7 +1504,11 @@ __device__ __forceinline__ void coopw_solve(const GmCoopWArgs& a, int c, unsigne
       }
       for (int j = gm_opaque(threadIdx.x); j <= it; j += T) hcol[j] = xc_ld<X2>(a.hg + j);
       __syncthreads();
-      // w -= sum_j h_j v_j on the local rows; partial ||w||^2; w rows out
+      // w -= sum_j h_j v_j on the local rows; partial ||w||^2; w rows out.
+      // The same block reduction forms sum_j alpha_j h_j, entry it of the
+      // new Hessenberg column after the previous rotations (identical bits
+      // in every workgroup: the same alpha, h and order).
+      double hv;
       {
         const int q = gm_opaque(threadIdx.x) >> 6, l = gm_opaque(threadIdx.x) & 63;
         double pn = 0.0;
@@ -1499,41 +1524,46 @@ __device__ __forceinline__ void coopw_solve(const GmCoopWArgs& a, int c, unsigne
             xc_st<X2>(a.wfull + r0 + row, wn);
             pn = wn * wn;
           }
-        } else if (scan_early && q == T / 64 - 1) {
-          rotate_prev(it, l);
         }
-        pn = gm_block_sum2(pn, red, T);
+        const int t = gm_opaque(threadIdx.x);
+        double ah = t <= it ? alpha[t] * hcol[t] : 0.0;  // it < kGmT
+        gw_block_sum_pair(pn, ah, red);
+        hv = ah;
         if (threadIdx.x == 0) xc_st<X2>(a.norms + c * 4, pn);
       }
       if (!xc_sync<X2>(a.sync, a.P, c, ++bk, s_err)) {
         ok = false;
         break;
       }
-      // this thread's entries of the new vector (full and local rows),
-      // loaded with the norm partials
+      // this thread's local-row entry of the new vector w (unnormalised),
+      // the row sums of S w for the next step (from L2, beside the norm
+      // partials and the Givens step of wavefront 0)
       const int t0 = gm_opaque(threadIdx.x);
-      const double wpre = t0 < nf ? xc_ld<X2>(a.wfull + t0) : 0.0;
       const double wpre_l = t0 < Rc ? xc_ld<X2>(a.wfull + r0 + t0) : 0.0;
+      const double swsum = matvec_g(a.wfull);
       if (gm_opaque(threadIdx.x) < 64) {
         const int lane = gm_opaque(threadIdx.x);
         const double tt = sqrt(gw_wave_gather<X2>(a.norms, 4, a.P, lane));
-        if (!scan_early) rotate_prev(it, lane);
         if (lane == 0) {
-          hrot[it + 1] = tt;
+          // KSPGMRESUpdateHessenberg: tt < min(|tt / grs[it]|, 1e-30) is a
+          // happy breakdown (the quotient is needed only when tt < 1e-30)
           const int hap = tt < 1e-30 && tt < fabs(tt / grs[it]);
-          double res;
+          double res, diag, cc = 1.0, sn = 0.0;
           if (!hap) {
-            const double t1 = sqrt(hrot[it] * hrot[it] + hrot[it + 1] * hrot[it + 1]);
-            const double cc = hrot[it] / t1, sn = hrot[it + 1] / t1;
+            const double t1 = sqrt(hv * hv + tt * tt);
+            cc = hv / t1;
+            sn = tt / t1;
             lcc[it] = cc;
             lss[it] = sn;
             grs[it + 1] = -sn * grs[it];
             grs[it] = cc * grs[it];
-            hrot[it] = cc * hrot[it] + sn * hrot[it + 1];
+            diag = cc * hv + sn * tt;
             res = fabs(grs[it + 1]);
           } else {
+            diag = hv;
             res = 0.0;
           }
+          if (c == 0) a.Rd[it] = diag;
           int rs = 0;
           if (res <= ttol) rs = 1;
           else if (res >= 1e4 * rn0) rs = -1;
@@ -1541,22 +1571,32 @@ __device__ __forceinline__ void coopw_solve(const GmCoopWArgs& a, int c, unsigne
           s_flag[0] = rs;
           s_flag[1] = hap;
           s_val[0] = tt;
+          s_val[1] = sn;
+          red[2 * (kGmT / 64)] = cc;
         }
       }
       __syncthreads();
       reason = s_flag[0];
       const int hap = s_flag[1];
-      if (c == 0)
-        for (int j = gm_opaque(threadIdx.x); j <= it + 1; j += T) a.Hg[(long)it * ld + j] = hrot[j];
+      // workgroup 0 keeps the unrotated column (rows 0..it, then tt); the
+      // rotations are applied to all columns at the cycle's end
+      if (c == 0) {
+        for (int j = gm_opaque(threadIdx.x); j <= it; j += T) a.Hg[(long)it * ld + j] = hcol[j];
+        if (threadIdx.x == 0) a.Hg[(long)it * ld + it + 1] = s_val[0];
+      }
       if (!hap) {
+        // v_(it+1) = w / tt, and S v_(it+1) for the next step; alpha for the
+        // next column: every entry times -s_it, then c_it
         const double inv = 1.0 / s_val[0];
-        if (t0 < nf) vfull[t0] = wpre * inv;
-        for (int i = t0 + T; i < nf; i += T) vfull[i] = xc_ld<X2>(a.wfull + i) * inv;
+        store_rows(swsum, inv);
         if (t0 < Rc) {
           const int j = it + 1;
           if (j < NL) Vl[t0 * NL + j] = wpre_l * inv;
           else Vgc[(size_t)t0 * ldg + (j - NL)] = wpre_l * inv;
         }
+        const double sn = s_val[1];
+        for (int j = gm_opaque(threadIdx.x); j <= it; j += T) alpha[j] = -sn * alpha[j];
+        if (threadIdx.x == 0) alpha[it + 1] = red[2 * (kGmT / 64)];
       }
       ++it;
       ++its;
@@ -1568,6 +1608,19 @@ __device__ __forceinline__ void coopw_solve(const GmCoopWArgs& a, int c, unsigne
     // system (grs becomes y in place) and publishes y
     const int top = it - 1;
     if (c == 0) {
+      // the previous rotations on every column k (thread k; the same
+      // recurrence as KSPGMRESUpdateHessenberg), its diagonal from the step
+      for (int k = gm_opaque(threadIdx.x); k <= top; k += T) {
+        double* col = a.Hg + (long)k * ld;
+        double carry = col[0];
+        for (int j = 0; j < k; ++j) {
+          const double hn = col[j + 1];
+          col[j] = lcc[j] * carry + lss[j] * hn;
+          carry = -lss[j] * carry + lcc[j] * hn;
+        }
+        col[k] = a.Rd[k];
+      }
+      __syncthreads();
       for (int kb1 = top; kb1 >= 0; kb1 -= kCwBlk) {
         const int kb0 = max(0, kb1 - kCwBlk + 1), nb = kb1 - kb0 + 1;
         // column k of the diagonal block: stg[k kCwBlk + i] = H(kb0 + i, kb0 + k), i <= k
@@ -1647,7 +1700,7 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coopw(GmCoopWArgs a) {
   const int c = blockIdx.x / 8;
   if (c >= a.P) return;
   extern __shared__ double dyn[];
-  __shared__ double red[kGmT / 64];
+  __shared__ double red[2 * (kGmT / 64) + 1];
   __shared__ double s_val[2];
   __shared__ int s_flag[2];
   __shared__ int s_err;
@@ -1675,10 +1728,11 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coopw(GmCoopWArgs a) {
   }
 }
 
-// dynamic LDS of k_gmres_coopw without the basis (doubles): vfull nf, wl
-// 32, hcol/hrot kmax+2 each, lcc/lss kmax each, grs kmax+1, stg kCwBlk^2
+// dynamic LDS of k_gmres_coopw without the basis (doubles): wl 32,
+// hcol/alpha kmax+2 each, lcc/lss kmax each, grs kmax+1, stg kCwBlk^2
 static size_t coopw_lds_fixed(int nf, int kmax) {
-  return (size_t)nf + 32 + (size_t)5 * kmax + 5 + (size_t)kCwBlk * kCwBlk;
+  (void)nf;
+  return 32 + (size_t)5 * kmax + 5 + (size_t)kCwBlk * kCwBlk;
 }
 
 bool gmres_coopw_plan(int nf, int kmax, size_t max_lds, int ncu, int* P, int* R, int* NL) {
@@ -1703,7 +1757,7 @@ bool gmres_coopw_plan(int nf, int kmax, size_t max_lds, int ncu, int* P, int* R,
 
 size_t gmres_coopw_ws(int nf, int kmax, int P, int R, int NL) {
   return kCoopSyncDoubles + (size_t)P * (kmax + 2) + (size_t)4 * P + (size_t)nf + (size_t)(kmax + 1) * kmax +
-         (size_t)2 * (kmax + 1) + (size_t)P * R * (size_t)std::max(0, kmax + 1 - NL);
+         (size_t)3 * (kmax + 1) + (size_t)P * R * (size_t)std::max(0, kmax + 1 - NL);
 }
 
 void gmres_coopw(hipStream_t st, int nf, int kmax, int maxit, int P, int R, int NL, const double* S, const double* b,
@@ -1718,7 +1772,8 @@ void gmres_coopw(hipStream_t st, int nf, int kmax, int maxit, int P, int R, int 
   a.Hg = a.wfull + nf;
   a.yg = a.Hg + (size_t)(kmax + 1) * kmax;
   a.hg = a.yg + (kmax + 1);
-  a.Vg = a.hg + (kmax + 1);
+  a.Rd = a.hg + (kmax + 1);
+  a.Vg = a.Rd + (kmax + 1);
   // IB_GMRES_L2_EXCHANGE=0 keeps the device-scope exchange even when every
   // workgroup is on one XCD (the results are the same bits either way)
   const char* e = std::getenv("IB_GMRES_L2_EXCHANGE");

@@ -153,6 +153,13 @@ struct ib_handle {
   int64_t green_builds = 0;
   DBuf<int> Fkeys, fmap, tapidx, face_flag;
   DBuf<double> Gm, Tm;
+  // the face-space S formed again after a weight rebuild: the entries of
+  // torsional bodies move, the others keep their weights; tcache: T's
+  // columns and S's block of the fixed entries are those of the current G,
+  // face set and fixed weights
+  DBuf<int> movidx, rigidx;
+  int nmov = 0, nrig = 0;
+  bool tcache = false;
   // numerically stationary path (num_stat): the inverse of the Cholesky-
   // factored a_times matrix, and a torsional map with no torsional entries
   DBuf<double> Ainv;
@@ -390,7 +397,24 @@ struct ib_handle {
   // new taps' neighbourhoods, so the set is rebuilt once rather than at
   // every change of direction.  S's entries do not depend on the set.
   bool face_grow = false;
+  // S = W G W^T + B.  moving: the weights changed only for the torsional
+  // bodies' entries since the last call, so with a valid cache only their
+  // columns of T and rows and columns of S are formed (the same bits as the
+  // full product; IB_SCHUR_FULL=1 forms everything every time)
+  void form_schur_face(bool moving) {
+    const char* e = std::getenv("IB_SCHUR_FULL");
+    const bool full_always = e && std::atoi(e) == 1;
+    if (moving && tcache && !full_always && nmov > 0 && nmov < nf) {
+      schur_from_green_moving(st, nf, nF, w, tapidx, Gm, Tm, tors_of, tb, th, dt, delta, Sd, nmov, movidx, nrig,
+                              rigidx);
+    } else {
+      schur_from_green(st, nf, nF, w, tapidx, Gm, Tm, tors_of, tb, th, dt, delta, Sd);
+      tcache = true;
+    }
+  }
+
   void build_face_set() {
+    tcache = false;
     // touched faces = unique tap keys of the current weights
     int nr = 0;
     IB_HIP(hipMemcpyAsync(hflag, csr.nruns, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -430,6 +454,7 @@ struct ib_handle {
   }
 
   void compute_green() {
+    tcache = false;
     if (Gm.n < (size_t)nF * nF) Gm.alloc((size_t)nF * nF);
     if (Tm.n < (size_t)nF * nf) Tm.alloc((size_t)nF * nf);
     for (int c0 = 0; c0 < nF; c0 += ncol) {
@@ -458,7 +483,7 @@ struct ib_handle {
       compute_green();
       map_taps(st, nf, csr.key, fmap, tapidx, face_flag);
     }
-    schur_from_green(st, nf, nF, w, tapidx, Gm, Tm, tors_of, tb, th, dt, delta, Sd);
+    form_schur_face(false);
   }
 
   // setup_reg + the sorted tap list; the old touched faces of hx/hy are
@@ -945,7 +970,7 @@ struct ib_handle {
         // the flags, so it runs while the host wakes up; it is formed again
         // below if a tap has left the face set
         IB_HIP(hipEventRecord(ev_flags, st));
-        schur_from_green(st, nf, nF, w, tapidx, Gm, Tm, tors_of, tb, th, dt, delta, Sd);
+        form_schur_face(true);
         IB_HIP(hipEventSynchronize(ev_flags));
         for (auto& t : tim) t.harvest();  // the timed launches all precede the event
         if (hflag[2]) csr.build(st);
@@ -962,7 +987,7 @@ struct ib_handle {
           build_face_set();  // synchronises the stream before G is reallocated
           compute_green();
           map_taps(st, nf, csr.key, fmap, tapidx, face_flag);
-          schur_from_green(st, nf, nF, w, tapidx, Gm, Tm, tors_of, tb, th, dt, delta, Sd);
+          form_schur_face(false);
         }
         err_glob = 0.0;
         for (int t = 0; t < ntors; ++t) err_glob = std::max(err_glob, hbuf[t]);
@@ -1470,6 +1495,14 @@ struct ib_handle {
       IB_HIP(hipMemcpy(yb.p, hyb.data(), sizeof(double) * nf, hipMemcpyHostToDevice));
       IB_HIP(hipMemcpy(comp.p, h_ccomp.data(), sizeof(int) * nf, hipMemcpyHostToDevice));
       IB_HIP(hipMemcpy(tors_of.p, h_tors_of.data(), sizeof(int) * nf, hipMemcpyHostToDevice));
+      std::vector<int> mv, rg;
+      for (int k = 0; k < nf; ++k) (h_tors_of[k] >= 0 ? mv : rg).push_back(k);
+      nmov = (int)mv.size();
+      nrig = (int)rg.size();
+      movidx.alloc(std::max(nmov, 1));
+      rigidx.alloc(std::max(nrig, 1));
+      if (nmov) IB_HIP(hipMemcpy(movidx.p, mv.data(), sizeof(int) * nmov, hipMemcpyHostToDevice));
+      if (nrig) IB_HIP(hipMemcpy(rigidx.p, rg.data(), sizeof(int) * nrig, hipMemcpyHostToDevice));
     }
     fill(st, f, nf, 0.0);
     fill(st, f_rdst, nf, 0.0);
@@ -1547,7 +1580,7 @@ struct ib_handle {
     }
     if (assembled && !sd_on) {
       if (facemode) {
-        schur_from_green(st, nf, nF, w, tapidx, Gm, Tm, tors_of, tb, th, dt, delta, Sd);
+        form_schur_face(false);
       } else {
         assemble_schur();
       }

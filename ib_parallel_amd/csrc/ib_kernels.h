@@ -233,6 +233,13 @@ void map_taps(hipStream_t st, int nf, const int* key, const int* fmap, int* tapi
 void schur_from_green(hipStream_t st, int nf, int nF, const double* w, const int* tapidx, const double* G,
                       double* T, const int* tors_of, const TorsBody* tb, const double* theta, double dt,
                       double delta, double* S);
+// The same after a weight rebuild that moved only the entries mov[0..nmov)
+// (the torsional bodies): the columns of T of those entries, and the
+// entries of S in one of their rows or columns; T's other columns and S's
+// block of the other entries (rig[0..nrig)) are kept as they are.
+void schur_from_green_moving(hipStream_t st, int nf, int nF, const double* w, const int* tapidx, const double* G,
+                             double* T, const int* tors_of, const TorsBody* tb, const double* theta, double dt,
+                             double delta, double* S, int nmov, const int* mov, int nrig, const int* rig);
 // y = S x for the dense nf x nf Schur matrix (column major)
 void gemv(hipStream_t st, int nf, const double* S, const double* x, double* y);
 // Device arguments of reg / redistribute (the CSR of the taps), regT,

@@ -194,6 +194,35 @@ def test_face_space_recompute_when_taps_leave_the_set():
     assert nfaces2 > nfaces and builds2 == 1  # the default margin covers this motion
 
 
+@pytest.mark.parametrize("case", ("tiny", "two_plates"))
+def test_face_space_moving_update_equals_full_assembly(monkeypatch, case):
+    """In the FSI loop only the torsional bodies' weights change, so S = W G
+    W^T + B is formed again only in their rows and columns (and T only in
+    their columns): the steps are bit for bit those with S formed whole
+    after every weight rebuild (IB_SCHUR_FULL=1)."""
+    if case == "tiny":
+        p, b = decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-12, gmres_restart=80, schur_mode=3)
+    else:
+        p, b = decks.tiny_case(m=48, n=40, mgridlev=2, tol=1e-12, gmres_restart=120, schur_mode=3)
+        plate2 = decks.plate_body(npts=5, length=0.3, k_theta=0.05)
+        plate2.x = plate2.x + 1.2
+        plate2.y = plate2.y + 0.4
+        b = b + [plate2]
+    runs = []
+    for full in ("1", "0"):
+        monkeypatch.setenv("IB_SCHUR_FULL", full)
+        g = gpu(p, b)
+        infos = [g.advance() for _ in range(4)]
+        runs.append(([(i.ksp_iter, i.fsi_iter) for i in infos],
+                     [g.level(k)[key] for k in range(1, g.L + 1) for key in ("omega", "qx", "qy")],
+                     g.body()["f"].copy(), g.body()["theta"].copy()))
+        g.close()
+    (i1, f1, b1, t1), (i2, f2, b2, t2) = runs
+    assert i1 == i2
+    assert all(np.array_equal(a, c) for a, c in zip(f1, f2))
+    assert np.array_equal(b1, b2) and np.array_equal(t1, t2)
+
+
 def test_airfoil_flap_keeps_the_face_space_form():
     """schur_mode 0 with a torsional flap picks the face-space form.  For the
     airfoil + flap at 1024x512x5 the default 4-cell face set would pass G's
