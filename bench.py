@@ -257,6 +257,53 @@ def run_batched(args):
     return res
 
 
+def run_c4_batch(args):
+    """The two-body system as a batch: args.c4_batch instances of the
+    airfoil + flap at 1024x512x5 with the flap stiffness spread over
+    [0.02, 0.1] (the concurrent form: each instance's plain step from its
+    own host thread, each cooperative GMRES solve on its own XCD) against
+    the same instances as standalone handles stepped one after another."""
+    import numpy as np
+    from ib_parallel_amd import GpuBatch, GpuSim, decks
+    nb, n = args.c4_batch, args.c4_batch_steps
+    cases = [decks.airfoil_flap_case(k_theta=float(k)) for k in np.linspace(0.02, 0.1, nb)]
+    t0 = time.perf_counter()
+    batch = GpuBatch(cases)
+    t_create = time.perf_counter() - t0
+    pts = sum(gridpts(s) for s in batch.sims)
+    for _ in range(2):
+        batch.advance()
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        batch.advance()
+    device_sync()
+    tb = time.perf_counter() - t0
+    forms = [s.gmres_info() for s in batch.sims]
+    batch.close()
+    plain = [GpuSim(p, b) for p, b in cases]
+    for _ in range(2):
+        for s in plain:
+            s.advance()
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        for s in plain:
+            s.advance()
+    device_sync()
+    ts = time.perf_counter() - t0
+    for s in plain:
+        s.close()
+    return {"workload": f"{nb} x airfoil + torsional flap 1024x512x5, flap K_theta in [0.02, 0.1], tol 1e-10",
+            "instances": nb, "steps": n, "create_s": round(t_create, 1),
+            "value": round(pts * n / tb / 1e6, 3), "unit": "Mgridpt-steps/s",
+            "ms_per_batch_step": round(tb / n * 1e3, 2),
+            "serial_plain_value": round(pts * n / ts / 1e6, 3),
+            "serial_plain_ms_per_step": round(ts / n * 1e3, 2),
+            "speedup_vs_serial_plain": round(ts / tb, 2),
+            "gmres": [{"form": f[0], "workgroups": f[1], "exchange": f[2]} for f in forms]}
+
+
 def _time_adjoint(sim, n, cx, cy, ckpts, **grads):
     """Forward wall time of n plain steps, then the wall time of n adjoint
     steps per segment length in ckpts (None: the whole run taped)."""
@@ -550,6 +597,10 @@ def main():
     ap.add_argument("--adjoint-batch", type=int, default=0,
                     help="instances of the airfoil + flap batch adjoint measurement (0: skip; 8 in the workload)")
     ap.add_argument("--adjoint-batch-steps", type=int, default=4)
+    ap.add_argument("--c4-batch", type=int, default=0,
+                    help="instances of the airfoil + flap batch forward measurement (0: skip; 8 in the workload)")
+    ap.add_argument("--c4-batch-steps", type=int, default=10)
+    ap.add_argument("--c4-batch-only", action="store_true", help="only the airfoil + flap batch measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the library has no multi-GPU path",
@@ -561,6 +612,9 @@ def main():
         if args.adjoint_batch > 0:
             out["adjoint_batch"] = run_adjoint_batch(args)
         print(json.dumps(out), flush=True)
+        return
+    if args.c4_batch_only:
+        print(json.dumps({"c4_batch": run_c4_batch(args)}), flush=True)
         return
     if args.batch_only:
         print(json.dumps({"batched": run_batched(args)}), flush=True)

@@ -169,7 +169,8 @@ class GpuBatch:
         self._check(rc, allow)
         for s, t in zip(self.sims, self.itimes):
             s.itime.value = t
-        return list(self.infos)
+        # copies: the records of the next step reuse the same buffer
+        return [abi.IBStepInfo.from_buffer_copy(i) for i in self.infos]
 
     def adjoint(self, nsteps, cx, cy, ckpt_every=None):
         """ib_adjoint_run of every simulation, concurrently: cx, cy are per

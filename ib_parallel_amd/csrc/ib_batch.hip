@@ -17,6 +17,7 @@
 // plain step on the simulation's own stream.
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <thread>
 
 #include "ib_handle.h"
@@ -64,6 +65,7 @@ std::string g_batch_create_err;
 
 struct ib_batch {
   std::vector<ib_handle*> sims;
+  std::mutex coop_mu;  // turns of the cooperative GMRES solves (concurrent form)
   std::string err;
   int mode = 0;  // 1 fused, 2 concurrent
   hipStream_t st = nullptr;
@@ -402,16 +404,17 @@ struct ib_batch {
     mode = all ? 1 : 2;
     if (mode == 2) {
       // the simulations run at once, and each cooperative GMRES solve keeps
-      // its P workgroups on the CUs of one XCD: when together they need more
-      // than one XCD's CUs, a solve could wait (up to its bounded spin) for
-      // workgroups that cannot be placed, so every simulation uses the
-      // one-workgroup solve instead
+      // its P workgroups, one per CU, on one XCD until it ends.  When the
+      // solves together need more than one XCD's CUs, two of them could wait
+      // for each other's CUs (up to the bounded spin), so they take turns:
+      // each holds the batch's lock from its launch until its work has
+      // completed, while the other simulations' kernels keep running
       int ncu = 0;
       if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, sims[0]->dev) != hipSuccess) ncu = 0;
       long need = 0;
       for (ib_handle* h : sims) need += h->coopP + h->cwP;
       if (need > ncu / 8)
-        for (ib_handle* h : sims) h->drop_coop();
+        for (ib_handle* h : sims) h->coop_lock = &coop_mu;
     }
     direct = params[0].launch_mode == 1;
     if (mode == 1) build_fused();

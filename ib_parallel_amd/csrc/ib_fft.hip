@@ -136,6 +136,10 @@ __device__ __forceinline__ void stage_fixed(const double2* __restrict__ in, doub
 // both X_q = x_0 + U_q - i V_q and X_(R-q) = x_0 + U_q + i V_q: a quarter of
 // the real products of the plain sum.  Each thread holds a tile of 2
 // sub-transforms x 4 values of q, so every LDS read feeds several products.
+// lanes one transform's tiles of the generic stage may spread over (the
+// threads of a one-transform workgroup)
+constexpr int kGenSplitLanes = 256;
+
 __device__ void stage_generic(double2* __restrict__ in, double2* __restrict__ out, int nf, int L, int Ns, int R,
                               const double2* __restrict__ W) {
   const int LR = L / R;
@@ -158,10 +162,14 @@ __device__ void stage_generic(double2* __restrict__ in, double2* __restrict__ ou
   __syncthreads();
   const int nj = (LR + 1) / 2, nq = (H + 4) / 4;  // q = 0..H in groups of 4
   const int ntile = nf * nj * nq;
-  // with few tiles, 2 or 4 adjacent lanes share a tile's sum over r (terms
-  // r = 1 + part, 1 + part + split, ...) and combine it by lane shuffles
+  // with few tiles per transform, 2 or 4 adjacent lanes share a tile's sum
+  // over r (terms r = 1 + part, 1 + part + split, ...) and combine it by
+  // lane shuffles.  The split depends on the transform's own tile count
+  // only (not on how many transforms share the workgroup, nor on its size),
+  // so every tile shape of the DST passes sums in the same order and gives
+  // the same bits.
   int lsplit = 0;
-  while (lsplit < 2 && (ntile << (lsplit + 1)) <= (int)blockDim.x) ++lsplit;
+  while (lsplit < 2 && ((nj * nq) << (lsplit + 1)) <= kGenSplitLanes) ++lsplit;
   const int split = 1 << lsplit;
   for (int tt = threadIdx.x; tt < (ntile << lsplit); tt += blockDim.x) {
     const int t = tt >> lsplit, part = tt & (split - 1);

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <map>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -129,7 +130,15 @@ struct ib_handle {
   // the same for larger systems (gmres_coopw): workgroups, rows per
   // workgroup, basis vectors held in LDS
   int cwP = 0, cwR = 0, cwNL = 0;
-  int last_xmode = 0;  // exchange of the last wide cooperative solve (1 one XCD's L2, 2 device scope)
+  int last_xmode = 0;
+  // set by a concurrent batch whose cooperative solves would not all fit on
+  // one XCD's CUs together: each solve takes the lock from its launch until
+  // its work has completed, so only one is resident at a time
+  std::mutex* coop_lock = nullptr;
+  std::unique_lock<std::mutex> coop_turn() {
+    return (coop_lock && (coopP > 0 || cwP > 0)) ? std::unique_lock<std::mutex>(*coop_lock)
+                                                 : std::unique_lock<std::mutex>();
+  }  // exchange of the last wide cooperative solve (1 one XCD's L2, 2 device scope)
   DBuf<double> coopws;
   GmresDev* hst = nullptr;  // pinned
   double* hbuf = nullptr;   // pinned readback
@@ -573,13 +582,6 @@ struct ib_handle {
       gmres_coopw(st, nf, kmax, prm.gmres_max_it, cwP, cwR, cwNL, M, b, x, gst, prm.rtol, prm.atol, coopws);
     else gmres_dense(st, nf, kmax, prm.gmres_max_it, M, b, x, V, H, gst, prm.rtol, prm.atol, max_lds);
   }
-  // drop the cooperative solves (a concurrent batch whose solves would not
-  // all fit on one XCD's CUs) and give back their workspace
-  void drop_coop() {
-    coopP = coopR = 0;
-    cwP = cwR = cwNL = 0;
-    coopws.release();
-  }
   // Operand bytes of a dense GMRES solve of its iterations: per Arnoldi step
   // the matrix (nf^2 doubles), the basis vectors 0..j read twice by
   // classical Gram-Schmidt (dots and update) and the new vector written,
@@ -603,6 +605,7 @@ struct ib_handle {
     return dense_check();
   }
   int dense_solve(const double* M, const double* b, double* x) {
+    auto turn = coop_turn();
     dense_launch(M, b, x);
     return dense_status();
   }
@@ -611,10 +614,12 @@ struct ib_handle {
       // dense Schur matrix: the whole solve is one launch.  Algorithmic
       // bytes: S, b read once, x read and written, and one Krylov vector
       // written per iteration (added once the count is known)
+      auto turn = coop_turn();
       tim[4].begin(st, 0.0);
       dense_launch(Sd, b, x);
       tim[4].end(st);
       const int rc = dense_status();
+      turn = std::unique_lock<std::mutex>();
       if (tim[4].on) tim[4].total_bytes += gmres_operand_bytes(hst->its);
       *its_out = hst->its;
       return rc;
@@ -950,6 +955,7 @@ struct ib_handle {
       int rc = IB_OK;
       if (one_sync) {
         fsi_rhs(st, g, nf, tors_of, tb, ntors, th, dt, delta, w, csr.key, qx, qy, q0x[0], rhsf);
+        auto turn = coop_turn();
         tim[4].begin(st, 0.0);
         dense_launch(Sd, rhsf, f);
         tim[4].end(st);
@@ -972,6 +978,7 @@ struct ib_handle {
         IB_HIP(hipEventRecord(ev_flags, st));
         form_schur_face(true);
         IB_HIP(hipEventSynchronize(ev_flags));
+        turn = std::unique_lock<std::mutex>();
         for (auto& t : tim) t.harvest();  // the timed launches all precede the event
         if (hflag[2]) csr.build(st);
         rc = dense_check();

@@ -123,17 +123,17 @@ def test_concurrent_batch_equals_plain_steps():
     batch.close()
 
 
-def test_crowded_concurrent_batch_uses_one_workgroup_solves():
-    """17 concurrent torsional simulations would keep 34 cooperative GMRES
-    workgroups on the 32 CUs of one XCD: the batch switches every simulation
-    to the one-workgroup solve, so each equals its plain step with
-    gmres_form = 1 bit for bit."""
+def test_crowded_concurrent_batch_takes_turns_in_cooperative_solves():
+    """17 concurrent torsional simulations, each with a cooperative GMRES
+    solve of 2 workgroups: more than one XCD's 32 CUs together, so the
+    solves take turns (one resident at a time) and every simulation keeps
+    the solve of its plain step, bit for bit."""
     from ib_parallel_amd import GpuBatch, GpuSim
     cases = [decks.tiny_case(m=32, n=24, mgridlev=2, tol=1e-12, re=50.0 + i) for i in range(17)]
     batch = GpuBatch(cases)
     assert batch.mode == 2
-    plain = [GpuSim(*decks.tiny_case(m=32, n=24, mgridlev=2, tol=1e-12, re=50.0 + i, gmres_form=1))
-             for i in range(17)]
+    assert all(s.gmres_info()[0] == 2 for s in batch.sims)
+    plain = [GpuSim(p, b) for p, b in cases]
     for _ in range(2):
         infos = batch.advance()
         refs = [s.advance() for s in plain]

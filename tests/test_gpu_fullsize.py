@@ -1,0 +1,142 @@
+"""The workloads at their full sizes on the GPU.
+
+* The two-body system (elliptic airfoil, 300 points, rigid + torsional flap,
+  100 points) at 1024x512 cells and 5 levels: field invariants on every
+  level, no slip at the rigid airfoil, and the same steps from the per-stage
+  GMRES kernels; then as a batch of 8 instances with the flap stiffness
+  spread over [0.02, 0.1], each bit for bit its plain step.
+* 64 rigid ellipses of uneven size (grids 256..512 per side, 3..5 levels,
+  100..400 points): the fused batch bit for bit each plain step, and the two
+  cheapest instances against the oracle at their full size.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel
+from ib_parallel_amd import abi, decks
+
+pytestmark = pytest.mark.gpu
+
+
+def state(sim):
+    out = []
+    for k in range(1, sim.L + 1):
+        lv = sim.level(k)
+        out += [lv["omega"], lv["s"], lv["qx"], lv["qy"], lv["rhs_old"]]
+    b = sim.body()
+    out += [b["f"], b["f_rdst"], b["xb"], b["yb"], b["theta"], b["thetad"], b["thetadd"]]
+    return out
+
+
+def same(a_list, b_list):
+    return all(a.shape == b.shape and np.array_equal(a, b) for a, b in zip(a_list, b_list))
+
+
+def invariants(sim):
+    """div q = 0 and C^T q = omega on every level (q = C s by construction)."""
+    m, n = sim.m, sim.n
+    for k in range(1, sim.L + 1):
+        lv = sim.level(k)
+        QX = lv["qx"].reshape(n, m + 1)
+        QY = lv["qy"].reshape(n + 1, m)
+        div = QX[:, 1:] - QX[:, :-1] + QY[1:, :] - QY[:-1, :]
+        assert np.abs(div).max() < 1e-12 * np.abs(lv["s"]).max(), k
+        rot = QY[1:-1, 1:] - QY[1:-1, :-1] - QX[1:, 1:-1] + QX[:-1, 1:-1]
+        assert rel(rot.ravel(), lv["omega"]) < 1e-10, k
+
+
+def test_airfoil_flap_full_size():
+    from ib_parallel_amd import GpuSim
+    p, b = decks.airfoil_flap_case()
+    g = GpuSim(p, b)
+    n_air = b[0].npts
+    delta = p.len / p.m
+    infos = [g.advance() for _ in range(3)]
+    assert all(i.status == abi.IB_OK and i.fsi_iter >= 2 for i in infos), [(i.status, i.fsi_iter) for i in infos]
+    invariants(g)
+    # no slip at the rigid airfoil: E(q + q0) = rhsf - S f there, the GMRES
+    # residual (tolerance 1e-10 relative to ||rhsf|| ~ sqrt(nf) delta)
+    lv = g.level(1)
+    e = g.regT(lv["qx"] + delta, lv["qy"])
+    slip = e[: 2 * n_air]  # the airfoil's x then y entries (the force vector goes body by body)
+    print(f"airfoil + flap: max |E(q + q0)| at the airfoil {np.abs(slip).max():.3e} (delta {delta})")
+    assert np.abs(slip).max() < 1e-8 * delta
+    assert g.gmres_info()[0] == 3
+    # the same three steps with the per-stage GMRES kernels: the Krylov
+    # iterations round differently (cond(S) ~ 1e8), so the 1e-10 tolerance
+    # is crossed up to a few per cent of the iterations apart and f agrees
+    # to cond(S) * tol
+    p3, b3 = decks.airfoil_flap_case(gmres_form=3)
+    h = GpuSim(p3, b3)
+    infos3 = [h.advance() for _ in infos]
+    print("GMRES / FSI iterations per step, cooperative:", [(i.ksp_iter, i.fsi_iter) for i in infos],
+          "per-stage kernels:", [(j.ksp_iter, j.fsi_iter) for j in infos3])
+    assert all(j.status == abi.IB_OK for j in infos3)
+    fg, fh = g.forces(), h.forces()
+    print(f"forces rel diff {rel(np.concatenate(fg), np.concatenate(fh)):.2e}, omega rel diff "
+          f"{rel(g.level(1)['omega'], h.level(1)['omega']):.2e}, theta {g.body()['theta']} {h.body()['theta']}")
+    # cond(S) ~ 1.5e8 at tolerance 1e-10: measured 4.6e-7 (forces) and 8.6e-6
+    # (omega, max norm, largest next to the bodies) after three steps
+    assert rel(np.concatenate(fg), np.concatenate(fh)) < 1e-5
+    assert rel(g.level(1)["omega"], h.level(1)["omega"]) < 1e-4
+    g.close()
+    h.close()
+
+
+def test_airfoil_flap_batch_of_8_equals_plain_steps():
+    """The batch of 8 runs each instance's plain step from its own host
+    thread; their cooperative GMRES solves (32 workgroups each, a whole XCD)
+    take turns, so the instances keep the solve their plain steps use."""
+    from ib_parallel_amd import GpuBatch, GpuSim
+    ks = np.linspace(0.02, 0.1, 8)
+    cases = [decks.airfoil_flap_case(k_theta=float(k)) for k in ks]
+    batch = GpuBatch(cases)
+    assert batch.mode == 2
+    # advance() hands back the batch's own step records: copy them each step
+    infos = [[(i.ksp_iter, i.fsi_iter) for i in batch.advance()] for _ in range(2)]
+    got = [state(s) for s in batch.sims]
+    forms = [s.gmres_info() for s in batch.sims]
+    batch.close()
+    assert all(f[0] == 3 for f in forms), forms
+    for i, (p, b) in enumerate(cases):
+        s = GpuSim(p, b)
+        for step in range(2):
+            r = s.advance()
+            assert (r.ksp_iter, r.fsi_iter) == infos[step][i], (i, step)
+        assert same(got[i], state(s)), i
+        s.close()
+
+
+def test_ellipses_64_full_size():
+    from ib_parallel_amd import GpuBatch, GpuSim
+    from oracle import binding as ob
+    cases = decks.ellipse_batch_cases(count=64)
+    batch = GpuBatch(cases)
+    assert batch.mode == 1
+    plain = [GpuSim(p, b) for p, b in cases]
+    for step in range(1, 4):
+        infos = batch.advance()
+        for s in plain:
+            s.advance()
+        assert [i.itime for i in infos] == [step] * 64 and all(i.status == 0 for i in infos)
+    for i in range(64):
+        assert same(state(batch.sims[i]), state(plain[i])), i
+    for s in plain:
+        s.close()
+    batch.close()
+    # the two cheapest instances (grid points x force entries) against the
+    # oracle at their full size, one step each
+    cost = [p.m * p.n * p.mgridlev * 2 * b[0].npts for p, b in cases]
+    ob.set_threads(max(1, min(16, len(os.sched_getaffinity(0)))))
+    for i in np.argsort(cost)[:2]:
+        p, b = cases[i]
+        g, o = GpuSim(p, b), ob.OracleSim(p, b)
+        g.advance()
+        o.advance()
+        for k in range(1, g.L + 1):
+            assert rel(g.level(k)["omega"], o.level(k)["omega"]) < 1e-10, (i, k)
+        assert rel(g.body()["f"], o.body()["f"]) < 1e-8, i
+        g.close()
+        o.close()

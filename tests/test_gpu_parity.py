@@ -407,11 +407,13 @@ def test_wide_cooperative_gmres_airfoil_flap(monkeypatch):
     assert i4 == i0 and np.array_equal(x4, x0)
 
 
-def test_dst_tile_shapes_leave_results_unchanged(monkeypatch):
+@pytest.mark.parametrize("shape", ((64, 48), (76, 44)))
+def test_dst_tile_shapes_leave_results_unchanged(monkeypatch, shape):
     """The DST tile shapes (timed at create, forced, or the fixed rule) only
     decide how many transforms share a workgroup's LDS: three steps give the
-    same bits under each."""
-    p, b = decks.tiny_case(m=64, n=48, mgridlev=2, tol=1e-12, gmres_restart=80)
+    same bits under each.  76 x 44 has FFT lengths 152 = 8 * 19 and 88 = 8 *
+    11, so its transforms take the generic prime-radix stage."""
+    p, b = decks.tiny_case(m=shape[0], n=shape[1], mgridlev=2, tol=1e-12, gmres_restart=80)
     runs = []
     for env in ({"IB_DST_TUNE": "0"}, {}, {"IB_DST_TILES": "4,4"}, {"IB_DST_TILES": "1,2"}):
         monkeypatch.delenv("IB_DST_TUNE", raising=False)
