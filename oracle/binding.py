@@ -7,6 +7,8 @@ import ctypes as C
 import os
 import subprocess
 
+import numpy as np
+
 from ib_parallel_amd import abi
 from ib_parallel_amd.sim import Simulation
 
@@ -33,6 +35,8 @@ EXTRA = {
     "set_threads": (C.c_int, [C.c_int32]),
     "get_threads": (C.c_int, []),
     "op_subdomain_interp": (C.c_int, [C.c_void_p, abi._I, abi._D, abi._D, abi._D]),
+    "op_predictor": (C.c_int, [C.c_void_p, C.c_int32, abi._D, abi._D, abi._D, abi._D, abi._D]),
+    "op_redistribute": (C.c_int, [C.c_void_p, abi._D, abi._D]),
 }
 
 
@@ -65,6 +69,24 @@ class OracleSim(Simulation):
         self._check(self._f["op_subdomain_interp"](self.h, idx.ctypes.data_as(abi._I), abi.dptr(w), abi.dptr(sx),
                                                    abi.dptr(sy)))
         return idx.reshape(self.nf, np_), w.reshape(self.nf, np_), sx, sy
+
+    def predictor(self, k, omega, rhs, rhs_old, rhsbc=None):
+        """fluid_predictor of level k: the new vorticity from omega and the
+        nonlinear terms rhs (now) and rhs_old (previous step)."""
+        c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        rhsbc = np.zeros(self.P) if rhsbc is None else rhsbc
+        omega, rhs, rhs_old, rhsbc = map(c, (omega, rhs, rhs_old, rhsbc))
+        out = np.empty(self.P)
+        d = abi.dptr
+        self._check(self._f["op_predictor"](self.h, k, d(omega), d(rhs), d(rhs_old), d(rhsbc), d(out)))
+        return out
+
+    def redistribute(self, f):
+        """f_rdst = E M^-1 E^T f at the current weights, M = diag(E^T 1)."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        out = np.empty(self.nf)
+        self._check(self._f["op_redistribute"](self.h, abi.dptr(f), abi.dptr(out)))
+        return out
 
     def rot(self, hx, hy):
         import numpy as np

@@ -1746,6 +1746,27 @@ int orc_op_nonlinear_rot(orc_handle* h, int32_t k, const double* omega, const do
   return IB_OK;
 }
 
+/* fluid_predictor of level k (src/operators_fluid.F90:649-663): omega_out =
+ * S[lam1i_k (S[con1_k rhs + con2_k rhs_old + rhsbc] + lam1_k S[omega])] */
+int orc_op_predictor(orc_handle* h, int32_t k, const double* omega, const double* rhs, const double* rhs_old,
+                     const double* rhsbc, double* omega_out) {
+  if (k < 1 || k > h->L) return IB_E_INVAL;
+  std::memcpy(omega_out, omega, sizeof(double) * h->P);
+  h->predictor(omega_out, rhs, rhs_old, rhsbc, k - 1);
+  return IB_OK;
+}
+
+/* redistribute (src/operators_struct.F90:735-834) of a given force vector f
+ * at the current weights; the handle's own f is left as it was */
+int orc_op_redistribute(orc_handle* h, const double* f, double* f_rdst) {
+  std::vector<double> keep = h->f;
+  std::memcpy(h->f.data(), f, sizeof(double) * h->nf);
+  h->redistribute();
+  std::memcpy(f_rdst, h->f_rdst.data(), sizeof(double) * h->nf);
+  h->f = keep;
+  return IB_OK;
+}
+
 int orc_op_gmres(orc_handle* h, const double* b, double* x, int32_t* its) {
   int it = 0;
   int rc = h->gmres(b, x, &it);
