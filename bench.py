@@ -89,11 +89,13 @@ _HIP = None
 
 def _hip():
     """The HIP runtime through ctypes (device synchronisation and the
-    calibration copy; no torch import on the measurement path)."""
+    calibration copy; no torch import on the measurement path): the symbols
+    are looked up through libibgpu.so's handle, so they come from the runtime
+    the library is linked against."""
     global _HIP
     if _HIP is None:
-        import ctypes
-        _HIP = ctypes.CDLL("libamdhip64.so")
+        from ib_parallel_amd import library
+        _HIP = library()
     return _HIP
 
 

@@ -49,6 +49,15 @@ class GpuSim(Simulation):
         self._check(self._f["schur_info"](self.h, C.byref(mode), C.byref(nfaces), C.byref(builds)))
         return mode.value, nfaces.value, builds.value
 
+    def gmres_noalloc(self, b_dev, x_dev):
+        """ib_op_gmres_noalloc on device buffers (addresses of nf doubles; x
+        holds the initial guess and receives the solution): (status, its).
+        The status is returned, not raised, so a caller can see IB_E_INVAL
+        for a workspace above the static maximum."""
+        its = C.c_int32()
+        rc = self._f["op_gmres_noalloc"](self.h, C.c_void_p(b_dev), C.c_void_p(x_dev), C.byref(its))
+        return rc, its.value
+
     def gmres_info(self):
         """(form, workgroups, exchange) of the dense GMRES solve: form 0 per-stage
         kernels, 1 one workgroup, 2 cooperative (nf <= 256), 3 cooperative for

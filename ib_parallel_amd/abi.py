@@ -117,6 +117,8 @@ GPU_ONLY = {
     "schur_info": (C.c_int, [_P, _I, _I, C.POINTER(C.c_int64)]),
     "dst_tiles": (C.c_int, [_P, _I, _I, _I]),
     "gmres_info": (C.c_int, [_P, _I, _I, _I]),
+    # b and x are device addresses
+    "op_gmres_noalloc": (C.c_int, [_P, _P, _P, _I]),
     "kernel_timing": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "kernel_timing_read": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),

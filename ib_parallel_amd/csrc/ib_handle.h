@@ -577,10 +577,22 @@ struct ib_handle {
     return coopP > 0 || cwP > 0 || (nf <= kFusedGmresMax && gmres_dense_fits(nf, kmax, max_lds));
   }
   void dense_launch(const double* M, const double* b, double* x) {
-    if (coopP > 0) gmres_coop(st, nf, kmax, prm.gmres_max_it, coopP, coopR, M, b, x, gst, prm.rtol, prm.atol, coopws);
+    if (coopP > 0 || cwP > 0) dense_launch_ws(M, b, x, coopws, nullptr);
+    else dense_launch_ws(M, b, x, V, H);
+  }
+  // the same solve with its workspace at ws (the cooperative kernels) or at
+  // ws and hw (the one-workgroup kernel's basis and Hessenberg matrix)
+  void dense_launch_ws(const double* M, const double* b, double* x, double* ws, double* hw) {
+    if (coopP > 0) gmres_coop(st, nf, kmax, prm.gmres_max_it, coopP, coopR, M, b, x, gst, prm.rtol, prm.atol, ws);
     else if (cwP > 0)
-      gmres_coopw(st, nf, kmax, prm.gmres_max_it, cwP, cwR, cwNL, M, b, x, gst, prm.rtol, prm.atol, coopws);
-    else gmres_dense(st, nf, kmax, prm.gmres_max_it, M, b, x, V, H, gst, prm.rtol, prm.atol, max_lds);
+      gmres_coopw(st, nf, kmax, prm.gmres_max_it, cwP, cwR, cwNL, M, b, x, gst, prm.rtol, prm.atol, ws);
+    else gmres_dense(st, nf, kmax, prm.gmres_max_it, M, b, x, ws, hw, gst, prm.rtol, prm.atol, max_lds);
+  }
+  // doubles of workspace dense_launch_ws needs (ws and hw back to back)
+  size_t dense_ws_doubles() const {
+    if (coopP > 0) return gmres_coop_ws(nf, kmax, coopP);
+    if (cwP > 0) return gmres_coopw_ws(nf, kmax, cwP, cwR, cwNL);
+    return (size_t)(kmax + 1) * std::max(1, nf) + (size_t)(kmax + 1) * kmax;
   }
   // Operand bytes of a dense GMRES solve of its iterations: per Arnoldi step
   // the matrix (nf^2 doubles), the basis vectors 0..j read twice by

@@ -1,12 +1,42 @@
 // ib_solver.hip -- the C ABI of one simulation (include/ib_gpu.h) over the
 // device-resident handle of ib_handle.h.
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "ib_handle.h"
 
+// The workspace of ib_op_gmres_noalloc: 16 MiB of device memory per GPU the
+// library runs on, held until the process ends (the airfoil + flap system,
+// nf = 800 at restart 1000, needs 11 MiB).  Host code never passes the
+// array's name to a kernel: it stands for no device address on the host.
+constexpr size_t kScratchDoubles = (size_t)2 << 20;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace {
 std::string g_create_err;
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kScratchDevices] = {};
+
+// the device address of `scratch` on device dev, looked up once per device
+// with dev current
+double* scratch_on(int dev) {
+  if (dev < 0 || dev >= kScratchDevices) throw ibg::HipError("ib_op_gmres_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    int cur = 0;
+    IB_HIP(hipGetDevice(&cur));
+    if (cur != dev) IB_HIP(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) IB_HIP(hipSetDevice(cur));
+    IB_HIP(e);
+    g_scratch[dev] = static_cast<double*>(p);
+  }
+  return g_scratch[dev];
+}
 }  // namespace
 
 // ------------------------------------------------------------------------
@@ -351,6 +381,34 @@ int ib_op_gmres(ib_handle* h, const double* b, double* x, int32_t* its) {
   if (h->nf) h->d2h(x, h->opx, h->nf);
   h->sync();
   if (its) *its = it;
+  return rc;
+  API_CATCH(h)
+}
+
+// ib_op_gmres's one-launch dense solve on device buffers, its workspace in
+// the static `scratch` array instead of memory the library allocates.
+int ib_op_gmres_noalloc(ib_handle* h, const double* b, double* x, int32_t* its) {
+  if (!h) return IB_E_INVAL;
+  if (its) *its = 0;
+  if (h->nf == 0) return IB_OK;
+  if (!b || !x) return IB_E_INVAL;
+  API_TRY
+  if (!h->dense_solve_ok())
+    return h->fail(IB_E_UNSUPPORTED, "ib_op_gmres_noalloc: no one-launch dense GMRES for this handle (gmres_form 3, "
+                                     "or a system too large for it)");
+  const size_t need = h->dense_ws_doubles();
+  if (need > kScratchDoubles)
+    return h->fail(IB_E_INVAL, "ib_op_gmres_noalloc: the solve needs " + std::to_string(need * 8) +
+                                   " bytes of workspace, above the static maximum of " +
+                                   std::to_string(kScratchDoubles * 8));
+  if (!h->assembled)
+    return h->fail(IB_E_UNSUPPORTED, "ib_op_gmres_noalloc: the handle's GMRES is matrix-free (no assembled Schur "
+                                     "matrix)");
+  double* ws = scratch_on(h->dev);
+  auto turn = h->coop_turn();
+  h->dense_launch_ws(h->Sd, b, x, ws, ws + (size_t)(h->kmax + 1) * h->nf);
+  const int rc = h->dense_status();
+  if (its) *its = h->hst->its;
   return rc;
   API_CATCH(h)
 }

@@ -182,6 +182,19 @@ int ib_op_schur_matrix(ib_handle* h, double* S);
  * Schur operator with x as the initial guess (in) and solution (out); *its
  * gets the iteration count.  Returns IB_E_NOCONV_GMRES if not converged. */
 int ib_op_gmres(ib_handle* h, const double* b, double* x, int32_t* its);
+/* No counterpart in the reference: ib_op_gmres for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops).  b and x are DEVICE buffers of nf doubles (x: initial guess in,
+ * solution out); the solve is ib_op_gmres's one-launch dense GMRES on the
+ * handle's assembled Schur matrix, with the same results, and its workspace
+ * is a static 16 MiB device array per GPU (held until the process ends)
+ * instead of memory the library allocates.  The call allocates nothing and
+ * waits for the solve.  Returns IB_E_INVAL when the solve needs more than
+ * 16 MiB of workspace, IB_E_UNSUPPORTED when the handle has no assembled
+ * matrix or no one-launch solve (gmres_form 3).  All calls on one GPU share
+ * the array: calls that overlap on one GPU (other threads, other streams)
+ * race and must be serialised by the caller. */
+int ib_op_gmres_noalloc(ib_handle* h, const double* b, double* x, int32_t* its);
 /* nonlinear + rot of one level: rhs = C^T N(omega_k, q_k + q0_k, bc),
  * src/operators_fluid.F90:553-587 and :203-233; bc has nbc entries */
 int ib_op_nonlinear_rot(ib_handle* h, int32_t k, const double* omega, const double* qx,
