@@ -17,6 +17,8 @@
 // plain step on the simulation's own stream.
 #include <cstdlib>
 #include <cstring>
+#include <condition_variable>
+#include <memory>
 #include <mutex>
 #include <thread>
 
@@ -61,11 +63,94 @@ struct OpList {
 
 std::string g_batch_create_err;
 
+// The cooperative GMRES solves of a concurrent batch, run together: each
+// simulation's host thread hands its solve over when it reaches one
+// (solve) and waits; when every simulation still stepping has handed one
+// over, or has finished its step (leave), the last to arrive enqueues the
+// waiting solves, up to one per XCD per launch (gmres_coopw_multi), on its
+// own stream after the others' streams have reached their solves, and each
+// other stream waits for the launch.  Separate launches would not do: a
+// launch holds workgroups on every XCD, and those off its own XCD cannot
+// start (nor the launch end) while another solve holds that XCD's CUs.
+class CoopGroup : public SolveGroup {
+ public:
+  CoopGroup(int n, const std::vector<int>& xcc) : xcc_(xcc), ready_(n), done_(n) {
+    for (int i = 0; i < n; ++i) {
+      IB_HIP(hipEventCreateWithFlags(&ready_[i], hipEventDisableTiming));
+      IB_HIP(hipEventCreateWithFlags(&done_[i], hipEventDisableTiming));
+    }
+  }
+  ~CoopGroup() override {
+    for (hipEvent_t e : ready_) (void)hipEventDestroy(e);
+    for (hipEvent_t e : done_) (void)hipEventDestroy(e);
+  }
+  // n simulations start stepping
+  void enter(int n) {
+    std::lock_guard<std::mutex> lk(mu_);
+    active_ = n;
+    pend_.clear();
+  }
+  // a simulation has finished stepping (st: its stream, idle for it now)
+  void leave(hipStream_t st) {
+    std::lock_guard<std::mutex> lk(mu_);
+    --active_;
+    if (!pend_.empty() && (int)pend_.size() == active_) launch_locked(st);
+  }
+  bool solve(int member, const CoopWJob& job, hipStream_t st) override {
+    std::unique_lock<std::mutex> lk(mu_);
+    if (active_ <= 0) return false;
+    IB_HIP(hipEventRecord(ready_[member], st));
+    pend_.push_back({member, job});
+    const unsigned long long mine = round_;
+    if ((int)pend_.size() == active_) launch_locked(st);
+    else cv_.wait(lk, [&] { return round_ != mine; });
+    const std::string e = err_;
+    lk.unlock();
+    if (!e.empty()) throw HipError(e);
+    IB_HIP(hipStreamWaitEvent(st, done_[member], 0));
+    return true;
+  }
+
+ private:
+  struct Pending {
+    int member;
+    CoopWJob job;
+  };
+  void launch_locked(hipStream_t st) {
+    err_.clear();
+    try {
+      for (const Pending& p : pend_) IB_HIP(hipStreamWaitEvent(st, ready_[p.member], 0));
+      const size_t nx = xcc_.size();
+      for (size_t k0 = 0; k0 < pend_.size(); k0 += nx) {
+        const int n = (int)std::min(nx, pend_.size() - k0);
+        std::vector<CoopWJob> jobs;
+        for (int k = 0; k < n; ++k) jobs.push_back(pend_[k0 + k].job);
+        gmres_coopw_multi(st, n, jobs.data(), xcc_.data(), (int)nx);
+      }
+      for (const Pending& p : pend_) IB_HIP(hipEventRecord(done_[p.member], st));
+    } catch (const std::exception& e) {
+      err_ = e.what();
+    }
+    pend_.clear();
+    ++round_;
+    cv_.notify_all();
+  }
+  std::vector<int> xcc_;
+  std::vector<hipEvent_t> ready_, done_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  int active_ = 0;
+  unsigned long long round_ = 0;
+  std::vector<Pending> pend_;
+  std::string err_;
+};
+
 }  // namespace
 
 struct ib_batch {
   std::vector<ib_handle*> sims;
-  std::mutex coop_mu;  // turns of the cooperative GMRES solves (concurrent form)
+  std::mutex coop_mu[8];  // per XCD: turns of the cooperative GMRES solves (concurrent form)
+  std::unique_ptr<CoopGroup> group;  // concurrent form: the solves run together
   std::string err;
   int mode = 0;  // 1 fused, 2 concurrent
   hipStream_t st = nullptr;
@@ -404,17 +489,37 @@ struct ib_batch {
     mode = all ? 1 : 2;
     if (mode == 2) {
       // the simulations run at once, and each cooperative GMRES solve keeps
-      // its P workgroups, one per CU, on one XCD until it ends.  When the
-      // solves together need more than one XCD's CUs, two of them could wait
-      // for each other's CUs (up to the bounded spin), so they take turns:
-      // each holds the batch's lock from its launch until its work has
-      // completed, while the other simulations' kernels keep running
+      // its P workgroups, one per CU, on one XCD until it ends: simulation i
+      // solves on the device's XCD i mod 8, so up to 8 solves run side by
+      // side.  When the solves of one XCD together need more than its CUs,
+      // two of them could wait for each other's CUs (up to the bounded
+      // spin), so they take turns: each holds that XCD's lock from its
+      // launch until its work has completed, while the other simulations'
+      // kernels keep running
       int ncu = 0;
       if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, sims[0]->dev) != hipSuccess) ncu = 0;
-      long need = 0;
-      for (ib_handle* h : sims) need += h->coopP + h->cwP;
-      if (need > ncu / 8)
-        for (ib_handle* h : sims) h->coop_lock = &coop_mu;
+      const std::vector<int>& xcc = sims[0]->xcc;
+      // solves run together when every simulation has the cooperative solve
+      // for larger systems in the same register layout
+      bool together = sims.size() > 1 && !xcc.empty();
+      for (ib_handle* h : sims)
+        together = together && h->cwP > 0 && h->coopP == 0 && h->xcc == xcc &&
+                   gmres_coopw_multi_ok(h->nf, sims[0]->nf);
+      if (together) {
+        group.reset(new CoopGroup((int)sims.size(), xcc));
+        for (size_t i = 0; i < sims.size(); ++i) {
+          sims[i]->group = group.get();
+          sims[i]->group_member = (int)i;
+        }
+      }
+      const size_t nx = xcc.empty() ? 1 : xcc.size();  // no XCD ids: one shared lock, as one XCD
+      long need[8] = {};
+      for (size_t i = 0; i < sims.size(); ++i) {
+        sims[i]->coop_xcd = xcc.empty() ? -1 : xcc[i % nx];
+        need[i % nx] += sims[i]->coopP + sims[i]->cwP;
+      }
+      for (size_t i = 0; i < sims.size() && !group; ++i)
+        if (need[i % nx] > ncu / (long)(xcc.empty() ? 8 : nx)) sims[i]->coop_lock = &coop_mu[i % nx];
     }
     direct = params[0].launch_mode == 1;
     if (mode == 1) build_fused();
@@ -440,6 +545,7 @@ struct ib_batch {
       std::vector<std::string> ex(n);
       std::vector<std::thread> pool;
       pool.reserve(n);
+      if (group) group->enter(n);
       for (int i = 0; i < n; ++i)
         pool.emplace_back([&, i]() {
           try {
@@ -448,6 +554,7 @@ struct ib_batch {
             ex[i] = e.what();
             rc[i] = IB_E_HIP;
           }
+          if (group) group->leave(sims[i]->st);
         });
       for (auto& t : pool) t.join();
       for (int i = 0; i < n; ++i)
@@ -484,6 +591,7 @@ struct ib_batch {
     std::vector<std::string> ex(n);
     std::vector<std::thread> pool;
     pool.reserve(n);
+    if (group) group->enter(n);
     for (int i = 0; i < n; ++i)
       pool.emplace_back([&, i]() {
         try {
@@ -494,6 +602,7 @@ struct ib_batch {
           ex[i] = e.what();
           rc[i] = IB_E_HIP;
         }
+        if (group) group->leave(sims[i]->st);
       });
     for (auto& t : pool) t.join();
     int first = IB_OK;

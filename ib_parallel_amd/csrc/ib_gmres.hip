@@ -12,7 +12,11 @@
 // convergence reason, so the host may enqueue several iterations ahead of
 // the status it has read (the extra matvecs then write only into basis
 // columns the solution never uses).
+#include <algorithm>
 #include <cstdlib>
+#include <map>
+#include <mutex>
+#include <vector>
 
 #include "ib_device.h"
 #include "ib_kernels.h"
@@ -702,12 +706,13 @@ void gmres_dense(hipStream_t st, int nf, int kmax, int maxit, const double* S, c
 // coefficients of their rows, then the partial squared norm and their rows
 // of w.  Every workgroup sums the partials of all workgroups in the same
 // order and runs the rotations, tests and back substitution on identical
-// values, so all take the same branches; each updates its rows of x.  Only
-// workgroups 0, 8, 16, ... take part: the dispatcher deals workgroups to the
-// 8 XCDs in turn, so they share one L2.  The exchanged values are device-
-// scope atomic stores and loads (coherent without cache write-backs); the
-// barrier waits are bounded: on a timeout every workgroup leaves with
-// reason -3.
+// values, so all take the same branches; each updates its rows of x.  The
+// launch has 8P workgroups, dealt by the dispatcher to the 8 XCDs in turn;
+// the P on the XCD the host names take part (coop_rank), so a solve keeps
+// to one XCD's CUs and L2 and solves on different XCDs run side by side.
+// The exchanged values are device-scope atomic stores and loads (coherent
+// without cache write-backs); the barrier waits are bounded: on a timeout
+// every workgroup leaves with reason -3.
 // ------------------------------------------------------------------------
 // Thread layout of the matrix-vector product and of w -= V h: wavefront q
 // of the 16 owns local rows 4q .. 4q+3 (R <= 64), lane l the columns
@@ -717,8 +722,81 @@ constexpr int kCoopColQ = 4;
 constexpr int kCoopRowsPer16 = 4;  // local rows per 16-lane group in the CGS dots: R <= 64
 constexpr long kCoopSpin = 1L << 22;
 
+// The barrier block of a cooperative solve (unsigned words, cleared before
+// each launch): word 0 the error flag, 8.. the workgroups' barrier counts,
+// kXccSlot.. the XCD ids they publish, kClaimSlot the ranks claimed.
+constexpr int kCoopSyncDoubles = 40;
+constexpr int kXccSlot = 40;
+constexpr int kClaimSlot = 72;
+static_assert(kClaimSlot < 2 * kCoopSyncDoubles, "barrier block too small");
+
+// The XCD this wavefront runs on (HW_REG_XCC_ID, bits 3:0).
+__device__ __forceinline__ unsigned gw_xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 0xFu; }
+
+// The rank of this workgroup in a cooperative solve of P workgroups, or -1
+// if it takes no part.  xcd < 0: workgroups 0, 8, 16, ... (one XCD when the
+// dispatcher starts the launch's round on it); xcd >= 0: the first P
+// workgroups that run on XCD xcd (XCC_ID), ranked in the order they claim.
+// A launch of 8P workgroups puts P on every XCD.
+__device__ __forceinline__ int coop_rank(unsigned* sync, int P, int xcd, int* s_c) {
+  if (xcd < 0) return blockIdx.x % 8 == 0 && (int)(blockIdx.x / 8) < P ? (int)(blockIdx.x / 8) : -1;
+  if (threadIdx.x == 0)
+    *s_c = gw_xcc_id() == (unsigned)xcd
+               ? (int)__hip_atomic_fetch_add(sync + kClaimSlot, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+               : P;
+  __syncthreads();
+  const int c = *s_c;
+  return c < P ? c : -1;
+}
+
+// Before a cooperative launch: the barrier block cleared and the status set
+// to reason -3, which stays if no workgroup takes part.
+__global__ void k_coop_prep(unsigned* sync, GmresDev* s) {
+  if (threadIdx.x < 2 * kCoopSyncDoubles) sync[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) {
+    s->reason = -3;
+    s->its = 0;
+  }
+}
+
+static void coop_prep(hipStream_t st, double* ws, GmresDev* s) {
+  hipLaunchKernelGGL(k_coop_prep, dim3(1), dim3(2 * kCoopSyncDoubles), 0, st, reinterpret_cast<unsigned*>(ws), s);
+  IB_LAUNCH_CHECK();
+}
+
+__global__ void k_xcc_probe(unsigned* ids) {
+  if (threadIdx.x == 0) ids[blockIdx.x] = gw_xcc_id();
+}
+
+std::vector<int> gmres_xcc_ids(hipStream_t st) {
+  static std::mutex mu;
+  static std::map<int, std::vector<int>> known;
+  int dev = 0;
+  IB_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = known.find(dev);
+  if (it != known.end()) return it->second;
+  constexpr int kProbe = 64;
+  unsigned* d = nullptr;
+  IB_HIP(hipMalloc(&d, sizeof(unsigned) * kProbe));
+  unsigned h[kProbe];
+  hipLaunchKernelGGL(k_xcc_probe, dim3(kProbe), dim3(64), 0, st, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(d);
+  IB_HIP(e);
+  std::vector<int> ids;
+  for (unsigned v : h) ids.push_back((int)v);
+  std::sort(ids.begin(), ids.end());
+  ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+  known[dev] = ids;
+  return ids;
+}
+
 struct GmCoopArgs {
   int nf, kmax, maxit, P, R;
+  int xcd;        // the XCD whose workgroups take part (-1: workgroups 0, 8, 16, ...)
   int bsc;        // Hessenberg columns per back-substitution chunk
   const double* S;
   const double* b;
@@ -831,9 +909,9 @@ __device__ __forceinline__ bool gc_sync(unsigned* sync, int P, int c, unsigned k
 }
 
 __global__ void __launch_bounds__(kGmT) k_gmres_coop(GmCoopArgs a) {
-  if (blockIdx.x % 8 != 0) return;
-  const int c = blockIdx.x / 8;
-  if (c >= a.P) return;
+  __shared__ int s_c;
+  const int c = coop_rank(a.sync, a.P, a.xcd, &s_c);
+  if (c < 0) return;
   extern __shared__ double dyn[];
   const int nf = a.nf, kmax = a.kmax, ld = kmax + 1, T = kGmT, R = a.R;
   const int r0 = c * R, Rc = max(0, min(nf, r0 + R) - r0);
@@ -1131,16 +1209,14 @@ bool gmres_coop_plan(int nf, int kmax, size_t max_lds, int* P, int* R) {
   return false;
 }
 
-constexpr int kCoopSyncDoubles = 40;  // the error word, 32 workgroup flags and 32 XCD ids (unsigned)
-
 size_t gmres_coop_ws(int nf, int kmax, int P) {
   return kCoopSyncDoubles + (size_t)P * (kmax + 2) + (size_t)4 * P + (size_t)nf + (size_t)P * (kmax + 1) * kmax;
 }
 
 void gmres_coop(hipStream_t st, int nf, int kmax, int maxit, int P, int R, const double* S, const double* b, double* x,
-                GmresDev* s, double rtol, double atol, double* ws) {
+                GmresDev* s, double rtol, double atol, double* ws, int xcd, int nxcd) {
   GmCoopArgs a{};
-  a.nf = nf; a.kmax = kmax; a.maxit = maxit; a.P = P; a.R = R;
+  a.nf = nf; a.kmax = kmax; a.maxit = maxit; a.P = P; a.R = R; a.xcd = xcd;
   a.bsc = kGmT / (kmax + 1);
   a.S = S; a.b = b; a.x = x; a.s = s; a.rtol = rtol; a.atol = atol;
   a.sync = reinterpret_cast<unsigned*>(ws);
@@ -1149,12 +1225,14 @@ void gmres_coop(hipStream_t st, int nf, int kmax, int maxit, int P, int R, const
   a.wfull = a.norms + (size_t)4 * P;
   a.H = a.wfull + nf;
   // barrier words cleared by a kernel (a memset node misbehaves on graph replays)
-  fill(st, ws, kCoopSyncDoubles, 0.0);
+  coop_prep(st, ws, s);
   // an ordinary launch: the barrier is the kernel's own (no runtime grid
-  // synchronisation resources), 8P workgroups of which P stay, one per CU
-  // on a device of 256 CUs; a workgroup that is not resident in time makes
-  // the others leave with reason -3 instead of waiting for ever
-  hipLaunchKernelGGL(k_gmres_coop, dim3(8 * P), dim3(kGmT), coop_lds(nf, kmax, R), st, a);
+  // synchronisation resources), P workgroups per XCD of which the P on XCD
+  // xcd stay (8P and workgroups 0, 8, 16, ... with xcd < 0), one per CU; a
+  // workgroup that is not resident in time makes the others leave with
+  // reason -3 instead of waiting for ever
+  const int nwg = (xcd < 0 ? 8 : std::max(nxcd, 1)) * P;
+  hipLaunchKernelGGL(k_gmres_coop, dim3(nwg), dim3(kGmT), coop_lds(nf, kmax, R), st, a);
   IB_LAUNCH_CHECK();
 }
 
@@ -1193,6 +1271,7 @@ constexpr int kCwBlk = 32;
 
 struct GmCoopWArgs {
   int nf, kmax, maxit, P, R, NL;
+  int xcd;  // the XCD whose workgroups take part (-1: workgroups 0, 8, 16, ...)
   int l2x;  // 1: exchange through one XCD's L2 when every workgroup is found there
   const double* S;
   const double* b;
@@ -1687,30 +1766,18 @@ __device__ __forceinline__ void coopw_solve(const GmCoopWArgs& a, int c, unsigne
 }
 
 
-// Offset (in unsigned words of the barrier block) of the XCD ids the
-// workgroups of a cooperative solve publish.
-constexpr int kXccSlot = 40;
-
-// The XCD this wavefront runs on (HW_REG_XCC_ID, bits 3:0).
-__device__ __forceinline__ unsigned gw_xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 0xFu; }
-
+// One cooperative solve by the workgroup of rank c (of a.P, all on one XCD
+// when the XCD ids they publish at the first barrier agree).
 template <int CQ>
-__global__ void __launch_bounds__(kGmT) k_gmres_coopw(GmCoopWArgs a) {
-  if (blockIdx.x % 8 != 0) return;
-  const int c = blockIdx.x / 8;
-  if (c >= a.P) return;
-  extern __shared__ double dyn[];
-  __shared__ double red[2 * (kGmT / 64) + 1];
-  __shared__ double s_val[2];
-  __shared__ int s_flag[2];
-  __shared__ int s_err;
+__device__ __forceinline__ void coopw_run(const GmCoopWArgs& a, int c, double* dyn, double* red, double* s_val,
+                                          int* s_flag, int* s_err) {
   if (threadIdx.x == 0) {
-    s_err = 0;
+    *s_err = 0;
     __hip_atomic_store(a.sync + kXccSlot + c, gw_xcc_id() + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   // barrier 1 at device scope; then every workgroup reads the P XCD ids
   // (the same values, so the same decision everywhere)
-  if (xc_sync<false>(a.sync, a.P, c, 1u, &s_err)) {
+  if (xc_sync<false>(a.sync, a.P, c, 1u, s_err)) {
     if (threadIdx.x < 64) {
       const int lane = threadIdx.x;
       const unsigned v =
@@ -1720,9 +1787,71 @@ __global__ void __launch_bounds__(kGmT) k_gmres_coopw(GmCoopWArgs a) {
       if (lane == 0) s_flag[0] = same;
     }
     __syncthreads();
-    if (s_flag[0]) coopw_solve<CQ, true>(a, c, 1u, dyn, red, s_val, s_flag, &s_err);
-    else coopw_solve<CQ, false>(a, c, 1u, dyn, red, s_val, s_flag, &s_err);
+    if (s_flag[0]) coopw_solve<CQ, true>(a, c, 1u, dyn, red, s_val, s_flag, s_err);
+    else coopw_solve<CQ, false>(a, c, 1u, dyn, red, s_val, s_flag, s_err);
   } else if (threadIdx.x == 0) {
+    a.s->reason = -3;
+    a.s->its = 0;
+  }
+}
+
+template <int CQ>
+__global__ void __launch_bounds__(kGmT) k_gmres_coopw(GmCoopWArgs a) {
+  __shared__ int s_c;
+  const int c = coop_rank(a.sync, a.P, a.xcd, &s_c);
+  if (c < 0) return;
+  extern __shared__ double dyn[];
+  __shared__ double red[2 * (kGmT / 64) + 1];
+  __shared__ double s_val[2];
+  __shared__ int s_flag[2];
+  __shared__ int s_err;
+  coopw_run<CQ>(a, c, dyn, red, s_val, s_flag, &s_err);
+}
+
+// Up to kCoopMulti solves in one launch, solve k on XCD a[k].xcd: the
+// workgroups on that XCD claim its ranks, those on XCDs no solve names, or
+// past a solve's P, leave at once.  Every XCD's workgroups belong to this
+// launch, so no solve waits for CUs another solve holds.
+constexpr int kCoopMulti = 8;
+struct GmCoopWMulti {
+  int n;
+  GmCoopWArgs a[kCoopMulti];
+};
+
+template <int CQ>
+__global__ void __launch_bounds__(kGmT) k_gmres_coopw_multi(GmCoopWMulti m) {
+  __shared__ int s_c, s_k;
+  if (threadIdx.x == 0) {
+    const int id = (int)gw_xcc_id();
+    int k = -1;
+    for (int j = 0; j < m.n; ++j)
+      if (m.a[j].xcd == id) k = j;
+    int c = -1;
+    if (k >= 0) {
+      c = (int)__hip_atomic_fetch_add(m.a[k].sync + kClaimSlot, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (c >= m.a[k].P) c = -1;
+    }
+    s_c = c;
+    s_k = k;
+  }
+  __syncthreads();
+  const int c = s_c;
+  if (c < 0) return;
+  const int k = __builtin_amdgcn_readfirstlane(s_k);
+  extern __shared__ double dyn[];
+  __shared__ double red[2 * (kGmT / 64) + 1];
+  __shared__ double s_val[2];
+  __shared__ int s_flag[2];
+  __shared__ int s_err;
+  coopw_run<CQ>(m.a[k], c, dyn, red, s_val, s_flag, &s_err);
+}
+
+// the barrier blocks of the launch's solves cleared and their statuses set
+// to reason -3 (one workgroup per solve)
+__global__ void k_coop_prep_multi(GmCoopWMulti m) {
+  const GmCoopWArgs& a = m.a[blockIdx.x];
+  if (threadIdx.x < 2 * kCoopSyncDoubles) a.sync[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) {
     a.s->reason = -3;
     a.s->its = 0;
   }
@@ -1760,10 +1889,11 @@ size_t gmres_coopw_ws(int nf, int kmax, int P, int R, int NL) {
          (size_t)3 * (kmax + 1) + (size_t)P * R * (size_t)std::max(0, kmax + 1 - NL);
 }
 
-void gmres_coopw(hipStream_t st, int nf, int kmax, int maxit, int P, int R, int NL, const double* S, const double* b,
-                 double* x, GmresDev* s, double rtol, double atol, double* ws) {
+// the kernel arguments of one solve, its workspace laid out in ws
+static GmCoopWArgs coopw_args(int nf, int kmax, int maxit, int P, int R, int NL, const double* S, const double* b,
+                              double* x, GmresDev* s, double rtol, double atol, double* ws, int xcd) {
   GmCoopWArgs a{};
-  a.nf = nf; a.kmax = kmax; a.maxit = maxit; a.P = P; a.R = R; a.NL = NL;
+  a.nf = nf; a.kmax = kmax; a.maxit = maxit; a.P = P; a.R = R; a.NL = NL; a.xcd = xcd;
   a.S = S; a.b = b; a.x = x; a.s = s; a.rtol = rtol; a.atol = atol;
   a.sync = reinterpret_cast<unsigned*>(ws);
   a.dots = ws + kCoopSyncDoubles;
@@ -1778,12 +1908,49 @@ void gmres_coopw(hipStream_t st, int nf, int kmax, int maxit, int P, int R, int 
   // workgroup is on one XCD (the results are the same bits either way)
   const char* e = std::getenv("IB_GMRES_L2_EXCHANGE");
   a.l2x = e && std::atoi(e) == 0 ? 0 : 1;
-  fill(st, ws, kCoopSyncDoubles, 0.0);
+  return a;
+}
+
+// register columns per lane of the kernel instance for nf
+static int coopw_cq(int nf) { return nf <= 64 * 8 ? 8 : nf <= 64 * 13 ? 13 : 16; }
+
+void gmres_coopw(hipStream_t st, int nf, int kmax, int maxit, int P, int R, int NL, const double* S, const double* b,
+                 double* x, GmresDev* s, double rtol, double atol, double* ws, int xcd, int nxcd) {
+  const GmCoopWArgs a = coopw_args(nf, kmax, maxit, P, R, NL, S, b, x, s, rtol, atol, ws, xcd);
+  coop_prep(st, ws, s);
   const size_t lds = sizeof(double) * (coopw_lds_fixed(nf, kmax) + (size_t)R * NL);
-  // 8P workgroups of which P (0, 8, 16, ...: one XCD) stay, one per CU
-  if (nf <= 64 * 8) hipLaunchKernelGGL(k_gmres_coopw<8>, dim3(8 * P), dim3(kGmT), lds, st, a);
-  else if (nf <= 64 * 13) hipLaunchKernelGGL(k_gmres_coopw<13>, dim3(8 * P), dim3(kGmT), lds, st, a);
-  else hipLaunchKernelGGL(k_gmres_coopw<16>, dim3(8 * P), dim3(kGmT), lds, st, a);
+  // P workgroups per XCD, of which the P on XCD xcd stay, one per CU
+  const int nwg = (xcd < 0 ? 8 : std::max(nxcd, 1)) * P;
+  const int cq = coopw_cq(nf);
+  if (cq == 8) hipLaunchKernelGGL(k_gmres_coopw<8>, dim3(nwg), dim3(kGmT), lds, st, a);
+  else if (cq == 13) hipLaunchKernelGGL(k_gmres_coopw<13>, dim3(nwg), dim3(kGmT), lds, st, a);
+  else hipLaunchKernelGGL(k_gmres_coopw<16>, dim3(nwg), dim3(kGmT), lds, st, a);
+  IB_LAUNCH_CHECK();
+}
+
+bool gmres_coopw_multi_ok(int nf_a, int nf_b) { return coopw_cq(nf_a) == coopw_cq(nf_b); }
+
+void gmres_coopw_multi(hipStream_t st, int n, const CoopWJob* jobs, const int* xcds, int nxcd) {
+  if (n < 1 || n > kCoopMulti || n > std::max(nxcd, 1))
+    throw HipError("gmres_coopw_multi: between 1 and min(8, XCDs) solves per launch");
+  GmCoopWMulti m{};
+  m.n = n;
+  size_t lds = 0;
+  int maxP = 0;
+  const int cq = coopw_cq(jobs[0].nf);
+  for (int k = 0; k < n; ++k) {
+    const CoopWJob& j = jobs[k];
+    if (coopw_cq(j.nf) != cq) throw HipError("gmres_coopw_multi: solves of different register layouts");
+    m.a[k] = coopw_args(j.nf, j.kmax, j.maxit, j.P, j.R, j.NL, j.S, j.b, j.x, j.s, j.rtol, j.atol, j.ws, xcds[k]);
+    lds = std::max(lds, sizeof(double) * (coopw_lds_fixed(j.nf, j.kmax) + (size_t)j.R * j.NL));
+    maxP = std::max(maxP, j.P);
+  }
+  hipLaunchKernelGGL(k_coop_prep_multi, dim3(n), dim3(2 * kCoopSyncDoubles), 0, st, m);
+  IB_LAUNCH_CHECK();
+  const int nwg = std::max(nxcd, 1) * maxP;
+  if (cq == 8) hipLaunchKernelGGL(k_gmres_coopw_multi<8>, dim3(nwg), dim3(kGmT), lds, st, m);
+  else if (cq == 13) hipLaunchKernelGGL(k_gmres_coopw_multi<13>, dim3(nwg), dim3(kGmT), lds, st, m);
+  else hipLaunchKernelGGL(k_gmres_coopw_multi<16>, dim3(nwg), dim3(kGmT), lds, st, m);
   IB_LAUNCH_CHECK();
 }
 
@@ -1800,6 +1967,12 @@ void gmres_dense_set_max_lds(size_t bytes) {
   IB_HIP(hipFuncSetAttribute((const void*)k_gmres_coopw<13>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)(bytes - 1024)));
   IB_HIP(hipFuncSetAttribute((const void*)k_gmres_coopw<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(bytes - 1024)));
+  IB_HIP(hipFuncSetAttribute((const void*)k_gmres_coopw_multi<8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(bytes - 1024)));
+  IB_HIP(hipFuncSetAttribute((const void*)k_gmres_coopw_multi<13>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(bytes - 1024)));
+  IB_HIP(hipFuncSetAttribute((const void*)k_gmres_coopw_multi<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)(bytes - 1024)));
 }
 

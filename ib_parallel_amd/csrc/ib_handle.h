@@ -15,6 +15,7 @@
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <map>
@@ -90,6 +91,15 @@ struct KTimer {
   }
 };
 
+// Simulations stepped at once from several host threads whose cooperative
+// GMRES solves run together, one XCD each (ib_batch): solve() takes over
+// member's solve and returns once it is on the device, with st waiting for
+// it; false when the group is not stepping (the caller solves alone).
+struct SolveGroup {
+  virtual ~SolveGroup() = default;
+  virtual bool solve(int member, const CoopWJob& job, hipStream_t st) = 0;
+};
+
 }  // namespace ibg
 
 using namespace ibg;
@@ -135,6 +145,17 @@ struct ib_handle {
   // one XCD's CUs together: each solve takes the lock from its launch until
   // its work has completed, so only one is resident at a time
   std::mutex* coop_lock = nullptr;
+  // the XCD (XCC id) whose CUs run this handle's cooperative solves, of the
+  // device's XCDs xcc; handles take the XCDs in turn as they are created (a
+  // batch assigns its own).  -1: workgroups 0, 8, 16, ... (no XCD ids known)
+  int coop_xcd = -1;
+  std::vector<int> xcc;
+  SolveGroup* group = nullptr;  // set by a batch whose solves run together
+  int group_member = 0;
+  static unsigned next_xcd() {
+    static std::atomic<unsigned> n{0};
+    return n.fetch_add(1u);
+  }
   std::unique_lock<std::mutex> coop_turn() {
     return (coop_lock && (coopP > 0 || cwP > 0)) ? std::unique_lock<std::mutex>(*coop_lock)
                                                  : std::unique_lock<std::mutex>();
@@ -577,15 +598,22 @@ struct ib_handle {
     return coopP > 0 || cwP > 0 || (nf <= kFusedGmresMax && gmres_dense_fits(nf, kmax, max_lds));
   }
   void dense_launch(const double* M, const double* b, double* x) {
+    if (group && cwP > 0 && coopP == 0) {
+      const CoopWJob j{nf, kmax, prm.gmres_max_it, cwP, cwR, cwNL, M, b, x, gst, prm.rtol, prm.atol, coopws};
+      if (group->solve(group_member, j, st)) return;
+    }
     if (coopP > 0 || cwP > 0) dense_launch_ws(M, b, x, coopws, nullptr);
     else dense_launch_ws(M, b, x, V, H);
   }
   // the same solve with its workspace at ws (the cooperative kernels) or at
   // ws and hw (the one-workgroup kernel's basis and Hessenberg matrix)
   void dense_launch_ws(const double* M, const double* b, double* x, double* ws, double* hw) {
-    if (coopP > 0) gmres_coop(st, nf, kmax, prm.gmres_max_it, coopP, coopR, M, b, x, gst, prm.rtol, prm.atol, ws);
+    if (coopP > 0)
+      gmres_coop(st, nf, kmax, prm.gmres_max_it, coopP, coopR, M, b, x, gst, prm.rtol, prm.atol, ws, coop_xcd,
+                 (int)xcc.size());
     else if (cwP > 0)
-      gmres_coopw(st, nf, kmax, prm.gmres_max_it, cwP, cwR, cwNL, M, b, x, gst, prm.rtol, prm.atol, ws);
+      gmres_coopw(st, nf, kmax, prm.gmres_max_it, cwP, cwR, cwNL, M, b, x, gst, prm.rtol, prm.atol, ws, coop_xcd,
+                  (int)xcc.size());
     else gmres_dense(st, nf, kmax, prm.gmres_max_it, M, b, x, ws, hw, gst, prm.rtol, prm.atol, max_lds);
   }
   // doubles of workspace dense_launch_ws needs (ws and hw back to back)
@@ -1439,6 +1467,9 @@ struct ib_handle {
     if (prm.gmres_form < 0 || prm.gmres_form > 4) return fail(IB_E_INVAL, "gmres_form must be 0 to 4");
     coopP = coopR = 0;
     cwP = cwR = cwNL = 0;
+    xcc = gmres_xcc_ids(st);
+    if (xcc.empty() || xcc.size() > 8) xcc.clear();  // no usable ids: workgroups 0, 8, 16, ...
+    coop_xcd = xcc.empty() ? -1 : xcc[next_xcd() % xcc.size()];
     const bool coop_ok = prm.gmres_form == 0 || prm.gmres_form == 2 || prm.gmres_form == 4;
     if (coop_ok && prm.gmres_form != 4 && nf > 0 && gmres_coop_plan(nf, kmax, max_lds, &coopP, &coopR) &&
         8 * coopP <= ncu)

@@ -419,19 +419,38 @@ bool gmres_dense_fits(int nf, int kmax, size_t max_lds);
 // The dense solve over P workgroups of one XCD (rows of S and of the basis
 // on chip, two grid barriers per step); plan: P and rows per workgroup R,
 // false if no split fits.  ws holds gmres_coop_ws doubles.  Status as
-// gmres_dense; reason -3: a grid barrier timed out.
+// gmres_dense; reason -3: a grid barrier timed out.  xcd: the XCC id whose
+// workgroups run the solve (nxcd XCDs on the device), or -1 for workgroups
+// 0, 8, 16, ...
 bool gmres_coop_plan(int nf, int kmax, size_t max_lds, int* P, int* R);
 size_t gmres_coop_ws(int nf, int kmax, int P);
 void gmres_coop(hipStream_t st, int nf, int kmax, int maxit, int P, int R, const double* S, const double* b, double* x,
-                GmresDev* s, double rtol, double atol, double* ws);
+                GmresDev* s, double rtol, double atol, double* ws, int xcd, int nxcd);
 // The dense solve over P workgroups of one XCD for nf <= 1024 and up to 1024
 // Krylov vectors (S rows in registers, basis rows in LDS up to NL vectors,
 // the rest in HBM; three grid barriers per step at most); plan: P, rows per
 // workgroup R and NL, false if no split fits.  ws holds gmres_coopw_ws doubles.
+// One solve of a fused launch of cooperative solves (gmres_coopw's
+// arguments); gmres_coopw_multi runs n <= min(8, nxcd) of them at once,
+// solve k on XCD xcds[k], provided gmres_coopw_multi_ok holds between their
+// nf (the same register layout).
+struct CoopWJob {
+  int nf, kmax, maxit, P, R, NL;
+  const double* S;
+  const double* b;
+  double* x;
+  GmresDev* s;
+  double rtol, atol;
+  double* ws;
+};
+bool gmres_coopw_multi_ok(int nf_a, int nf_b);
+void gmres_coopw_multi(hipStream_t st, int n, const CoopWJob* jobs, const int* xcds, int nxcd);
+// The XCC ids of the device's XCDs (sorted), found once per device.
+std::vector<int> gmres_xcc_ids(hipStream_t st);
 bool gmres_coopw_plan(int nf, int kmax, size_t max_lds, int ncu, int* P, int* R, int* NL);
 size_t gmres_coopw_ws(int nf, int kmax, int P, int R, int NL);
 void gmres_coopw(hipStream_t st, int nf, int kmax, int maxit, int P, int R, int NL, const double* S, const double* b,
-                 double* x, GmresDev* s, double rtol, double atol, double* ws);
+                 double* x, GmresDev* s, double rtol, double atol, double* ws, int xcd, int nxcd);
 // pressure post-processing (src/operators_pressure.F90): cell arrays are
 // m x n, i fastest; pressure boundary vectors [left n | right n | bottom m |
 // top m]

@@ -124,22 +124,26 @@ def test_concurrent_batch_equals_plain_steps():
 
 
 def test_crowded_concurrent_batch_takes_turns_in_cooperative_solves():
-    """17 concurrent torsional simulations, each with a cooperative GMRES
-    solve of 2 workgroups: more than one XCD's 32 CUs together, so the
-    solves take turns (one resident at a time) and every simulation keeps
-    the solve of its plain step, bit for bit."""
+    """33 concurrent torsional simulations, each with a cooperative GMRES
+    solve of 8 workgroups; simulation i solves on XCD i mod 8.  XCD 0 holds
+    five solves (40 workgroups, more than its 32 CUs), which take turns; the
+    other XCDs hold four each (32 workgroups), which run at once.  Every
+    simulation keeps the solve of its plain step, bit for bit, and its
+    workgroups exchange through one XCD's L2."""
     from ib_parallel_amd import GpuBatch, GpuSim
-    cases = [decks.tiny_case(m=32, n=24, mgridlev=2, tol=1e-12, re=50.0 + i) for i in range(17)]
+    cases = [decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-12, gmres_form=4, gmres_restart=80, re=50.0 + i)
+             for i in range(33)]
     batch = GpuBatch(cases)
     assert batch.mode == 2
-    assert all(s.gmres_info()[0] == 2 for s in batch.sims)
+    assert all(s.gmres_info()[:2] == (3, 8) for s in batch.sims)
     plain = [GpuSim(p, b) for p, b in cases]
     for _ in range(2):
         infos = batch.advance()
         refs = [s.advance() for s in plain]
-        for i in range(17):
+        for i in range(33):
             assert (infos[i].ksp_iter, infos[i].fsi_iter) == (refs[i].ksp_iter, refs[i].fsi_iter), i
             assert same(state(batch.sims[i]), state(plain[i])), i
+    assert all(s.gmres_info()[2] == 1 for s in batch.sims)
     batch.close()
 
 
