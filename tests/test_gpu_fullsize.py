@@ -4,7 +4,8 @@
   100 points) at 1024x512 cells and 5 levels: field invariants on every
   level, no slip at the rigid airfoil, and the same steps from the per-stage
   GMRES kernels; then as a batch of 8 instances with the flap stiffness
-  spread over [0.02, 0.1], each bit for bit its plain step.
+  spread over [0.02, 0.1], each bit for bit its plain step, and the batch's
+  adjoint dJ/dK of the mean drag against central differences.
 * 64 rigid ellipses of uneven size (grids 256..512 per side, 3..5 levels,
   100..400 points): the fused batch bit for bit each plain step, and the two
   cheapest instances against the oracle at their full size.
@@ -140,3 +141,46 @@ def test_ellipses_64_full_size():
         assert rel(g.body()["f"], o.body()["f"]) < 1e-8, i
         g.close()
         o.close()
+
+
+def test_airfoil_flap_batch_adjoint_matches_finite_differences():
+    """ib_batch_adjoint over the batch of 8 (flap stiffness over [0.02,
+    0.1]), J = mean drag of the two bodies over 10 steps from rest: J equals
+    the plain forward's, and dJ/dK of three instances matches central
+    differences of the plain forward.  The forward is converged to 1e-10
+    with cond(S) ~ 1.5e8, so J(K) carries steps of ~1e-9 relative where a
+    solve's iteration count changes: differences in K under a few per cent
+    of K measure those, not the derivative (and over 2 steps the whole
+    response to K is of that size).  J is close to linear in K over this
+    range (the adjoint dJ/dK varies by 5e-5 relative between K = 0.02 and
+    0.1), so a wide step h = K / 2 keeps the truncation error far below
+    the noise it averages out; measured agreement 2.4e-4 to 9.3e-4 (1.5e-3
+    with h = 0.1 K and Richardson extrapolation), against 1e-5 for the
+    plate converged to 1e-12."""
+    from ib_parallel_amd import GpuBatch, GpuSim
+    n = 10
+    ks = np.linspace(0.02, 0.1, 8)
+
+    def mean_drag(K):
+        s = GpuSim(*decks.airfoil_flap_case(k_theta=float(K)))
+        acc = 0.0
+        for _ in range(n):
+            assert s.advance().status == abi.IB_OK
+            acc += s.forces()[0].sum()
+        s.close()
+        return acc / n
+
+    batch = GpuBatch([decks.airfoil_flap_case(k_theta=float(k)) for k in ks])
+    J, dre, dK, dg = batch.adjoint(n, [1.0, 1.0] * 8, [0.0, 0.0] * 8)
+    batch.close()
+    assert np.all(dK[0::2] == 0.0)  # the rigid airfoils
+    for i in (0, 3, 7):
+        K = ks[i]
+        Jf = mean_drag(K)
+
+        h = 0.5 * K
+        fdk = (mean_drag(K + h) - mean_drag(K - h)) / (2 * h)
+        print(f"K {K:.4f}: J batch {J[i]:.15e} forward {Jf:.15e}; dJ/dK adjoint {dK[2 * i + 1]:.10e} fd {fdk:.10e}",
+              flush=True)
+        assert abs(J[i] - Jf) <= 1e-12 * abs(Jf)
+        assert abs(dK[2 * i + 1] - fdk) <= 3e-3 * abs(fdk)
