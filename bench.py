@@ -263,8 +263,9 @@ def run_c4_batch(args):
     """The two-body system as a batch: args.c4_batch instances of the
     airfoil + flap at 1024x512x5 with the flap stiffness spread over
     [0.02, 0.1] (the concurrent form: each instance's plain step from its
-    own host thread, each cooperative GMRES solve on its own XCD) against
-    the same instances as standalone handles stepped one after another."""
+    own host thread, the cooperative GMRES solves of all instances in one
+    launch, each on its own XCD) against the same instances as standalone
+    handles stepped one after another; then the batch adjoint."""
     import numpy as np
     from ib_parallel_amd import GpuBatch, GpuSim, decks
     nb, n = args.c4_batch, args.c4_batch_steps
@@ -282,6 +283,28 @@ def run_c4_batch(args):
     device_sync()
     tb = time.perf_counter() - t0
     forms = [s.gmres_info() for s in batch.sims]
+    adj = None
+    if args.adjoint_batch_steps > 0:
+        # each instance with the gradient of its time-mean drag (airfoil +
+        # flap) with respect to the flap stiffness: ib_batch_adjoint of
+        # the same steps (forward pass and reverse sweep, the run taped),
+        # after an untimed set-up call (dG/dRe per instance)
+        na = args.adjoint_batch_steps
+        batch.adjoint(1, [1.0] * (2 * nb), [0.0] * (2 * nb))
+        device_sync()
+        t0 = time.perf_counter()
+        for _ in range(na):
+            batch.advance()
+        device_sync()
+        tf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        J, _, dk, _ = batch.adjoint(na, [1.0] * (2 * nb), [0.0] * (2 * nb))
+        device_sync()
+        ta = time.perf_counter() - t0
+        adj = {"steps": na, "forward_ms_per_batch_step": round(tf / na * 1e3, 2),
+               "adjoint_ms_per_batch_step": round(ta / na * 1e3, 2), "cost_ratio": round(ta / tf, 3),
+               "objective": "mean over the steps of the airfoil + flap drag",
+               "dJ_dK": [float(dk[2 * i + 1]) for i in range(nb)]}
     batch.close()
     plain = [GpuSim(p, b) for p, b in cases]
     for _ in range(2):
@@ -303,7 +326,9 @@ def run_c4_batch(args):
             "serial_plain_value": round(pts * n / ts / 1e6, 3),
             "serial_plain_ms_per_step": round(ts / n * 1e3, 2),
             "speedup_vs_serial_plain": round(ts / tb, 2),
-            "gmres": [{"form": f[0], "workgroups": f[1], "exchange": f[2]} for f in forms]}
+            "solves": "each instance's cooperative GMRES solves handed to one fused launch, one XCD per instance",
+            "gmres": [{"form": f[0], "workgroups": f[1], "exchange": f[2]} for f in forms],
+            "adjoint": adj}
 
 
 def _time_adjoint(sim, n, cx, cy, ckpts, **grads):
@@ -485,9 +510,10 @@ def run_cpu(args, gpu_its_per_step, gpu_fsi_per_step):
     core, warm-up excluded).
 
     Headline workload: one oracle step there takes minutes (matrix-free
-    GMRES, ~0.3-0.6 s per Krylov iteration on 8-16 cores), so the sample is
-    bounded: steps whose GMRES solves stop after K = 1 and K = 2 iterations
-    (FSI loop capped at the GPU's 3 iterations).  A step then costs
+    GMRES, ~0.07-0.3 s per Krylov iteration on 8-16 cores), so the sample is
+    bounded: steps whose GMRES solves stop after K = 4 and K = 40 iterations
+    (FSI loop capped at the GPU's 3 iterations; about 10 s on 16 cores).  A
+    step then costs
     T(K) = F + s (K + 1) t (s solves, each one residual product plus K
     iterations), which gives the fixed cost F (predictor, vort2flux,
     corrector, FSI right-hand sides and body updates) and the cost t of one
@@ -528,16 +554,17 @@ def run_cpu(args, gpu_its_per_step, gpu_fsi_per_step):
     T = {}
     fsi = {}
     t_begin = time.perf_counter()
-    for K in (1, 2):
+    lo, hi = 4, 40
+    for K in (lo, hi):
         pk, bk = decks.airfoil_flap_case(gmres_max_it=K, fsi_max_count=2)
         sim = ob.OracleSim(pk, bk)
-        if K == 1:
+        if K == lo:
             sim.advance(allow_noconv=True)  # warm-up step, excluded
         t0 = time.perf_counter()
         info = sim.advance(allow_noconv=True)
         T[K] = time.perf_counter() - t0
         fsi[K] = info.fsi_iter
-        if K == 2:
+        if K == hi:
             x = np.random.default_rng(0).standard_normal(sim.nf)
 
             def best_of(n):
@@ -553,9 +580,9 @@ def run_cpu(args, gpu_its_per_step, gpu_fsi_per_step):
             ob.set_threads(threads)
         npts = gridpts(sim)
         sim.close()
-    s_solves = fsi[1]
-    t_it = max(T[2] - T[1], 0.0) / s_solves
-    F = T[1] - 2 * s_solves * t_it
+    s_solves = 0.5 * (fsi[lo] + fsi[hi])
+    t_it = max(T[hi] - T[lo], 0.0) / (s_solves * (hi - lo))
+    F = T[lo] - (lo + 1) * s_solves * t_it
     est = F + (gpu_its_per_step + gpu_fsi_per_step) * t_it
     scale1 = t_mv_one / t_mv_all
     value = npts / est / 1e6
@@ -566,7 +593,8 @@ def run_cpu(args, gpu_its_per_step, gpu_fsi_per_step):
                      "schur_product_s": round(t_mv_one, 3), "schur_product_s_all_cores": round(t_mv_all, 3)},
         "sample": (f"oracle (C++ FP64 restatement, matrix-free GMRES as the reference, OpenMP {threads} threads) "
                    f"on the same workload: one warm-up step excluded, then one step each with the GMRES solves "
-                   f"capped at 1 and 2 iterations ({s_solves} FSI iterations): {T[1]:.2f} s and {T[2]:.2f} s, "
+                   f"capped at {lo} and {hi} iterations ({s_solves:g} FSI iterations): {T[lo]:.2f} s and "
+                   f"{T[hi]:.2f} s, "
                    f"so {F:.2f} s fixed + {t_it:.3f} s per Krylov iteration, extrapolated to the GPU's "
                    f"{gpu_its_per_step:.1f} iterations and {gpu_fsi_per_step:.1f} solves per step; one core "
                    f"scaled by one Schur product (best of 2: {t_mv_one:.2f} s on 1 core; best of 3: {t_mv_all:.2f} s "
@@ -599,9 +627,9 @@ def main():
     ap.add_argument("--adjoint-batch", type=int, default=0,
                     help="instances of the airfoil + flap batch adjoint measurement (0: skip; 8 in the workload)")
     ap.add_argument("--adjoint-batch-steps", type=int, default=4)
-    ap.add_argument("--c4-batch", type=int, default=0,
-                    help="instances of the airfoil + flap batch forward measurement (0: skip; 8 in the workload)")
-    ap.add_argument("--c4-batch-steps", type=int, default=10)
+    ap.add_argument("--c4-batch", type=int, default=8,
+                    help="instances of the airfoil + flap batch measurement (0: skip)")
+    ap.add_argument("--c4-batch-steps", type=int, default=5)
     ap.add_argument("--c4-batch-only", action="store_true", help="only the airfoil + flap batch measurement")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -652,6 +680,8 @@ def main():
         out["batched"] = run_batched(args)
     if args.adjoint_steps > 0 or args.adjoint_fsi_steps > 0:
         out["adjoint"] = run_adjoint(args)
+    if args.c4_batch > 0:
+        out["c4_batch"] = run_c4_batch(args)
     if not args.no_cpu:
         cpu = run_cpu(args, g["its"] / args.steps, g["fsi"] / args.steps)
         out["cpu_baseline"] = cpu
