@@ -193,6 +193,8 @@ struct ib_batch {
   }
   int ytile(const ib_handle* h) const { return yrule > 0 ? yrule : (yrule < 0 ? 1 : h->tiles.cp); }
   TabEntry<XArgs> xe(const ib_handle* h, const DstLaunch& l, size_t* lds) const {
+    if (!genx && fft_has_generic(h->px.dev))
+      throw HipError("fused batch: a row plan with a radix >= 7 under the row-pass kernels without that stage");
     return dst_x_entry(h->px.dev, l, h->g, xtile(h), h->max_lds, lds);
   }
   TabEntry<YArgs> ye(const ib_handle* h, int mode, const DstLaunch& l, size_t* lds) const {

@@ -256,7 +256,13 @@ __device__ __forceinline__ double2* fft_run(double2* a, double2* b, int nf, cons
       case 3: stage_fixed<3>(a, b, nf, pl.L, Ns, W); break;
       case 5: stage_fixed<5>(a, b, nf, pl.L, Ns, W); break;
       default:
-        if constexpr (GEN) stage_generic(a, b, nf, pl.L, Ns, R, W);
+        if constexpr (GEN) {
+          stage_generic(a, b, nf, pl.L, Ns, R, W);
+        } else {
+          // a plan this form cannot run (the host never launches one): the
+          // result is NaN, never a stale buffer
+          for (int i = threadIdx.x; i < nf * pl.L; i += blockDim.x) b[i] = make_double2(__builtin_nan(""), __builtin_nan(""));
+        }
         break;
     }
     __syncthreads();
