@@ -14,8 +14,11 @@
 // batched form (ib_batch.hip).
 #pragma once
 
+#include <sys/stat.h>
+
 #include <algorithm>
 #include <atomic>
+#include <cerrno>
 #include <cmath>
 #include <cstdlib>
 #include <map>
@@ -90,6 +93,123 @@ struct KTimer {
     spare.clear();
   }
 };
+
+// input/sd_precomputed.chd, the sub-domain's B as preprocess_sd writes it
+// (src/timestepper.F90:690-695,725-728) and reads it back when the deck sets
+// sub_domain_precomputed (:753-766): Fortran unformatted sequential records
+// (4-byte length markers before and after each, little-endian), per column
+// j = 1..nf_sd one record with its count nz, then one with its nz values and
+// their nz row numbers (1-based entries of the sub-domain).  The integers
+// are PetscInt: 4 bytes in PETSc's default build, 8 with 64-bit indices; the
+// reader takes the size from the first record's length.  The path is
+// relative to the working directory, as in the reference;
+// IB_SD_PRECOMPUTED_FILE names another.
+inline std::string sd_chd_path() {
+  const char* e = std::getenv("IB_SD_PRECOMPUTED_FILE");
+  return (e && *e) ? std::string(e) : std::string("input/sd_precomputed.chd");
+}
+
+// the file is written where its folder exists (a case directory's input)
+inline bool sd_chd_writable(const std::string& path) {
+  const size_t k = path.find_last_of('/');
+  const std::string dir = k == std::string::npos ? std::string(".") : (k == 0 ? std::string("/") : path.substr(0, k));
+  struct stat sb;
+  return ::stat(dir.c_str(), &sb) == 0 && S_ISDIR(sb.st_mode);
+}
+
+inline bool sd_write_chd(const std::string& path, const std::vector<long>& kptr, const std::vector<int>& krow,
+                         const std::vector<double>& kval, std::string* why) {
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) {
+    *why = std::strerror(errno);
+    return false;
+  }
+  bool ok = true;
+  auto put = [&](const void* p, size_t n) { ok = ok && (n == 0 || std::fwrite(p, 1, n, f) == n); };
+  std::vector<int32_t> idx;
+  for (size_t k = 0; k + 1 < kptr.size(); ++k) {
+    const long a = kptr[k], nz = kptr[k + 1] - kptr[k];
+    const int32_t m1 = 4, n32 = (int32_t)nz, m2 = (int32_t)(12 * nz);
+    put(&m1, 4);
+    put(&n32, 4);
+    put(&m1, 4);
+    idx.resize(nz);
+    for (long j = 0; j < nz; ++j) idx[j] = krow[a + j] + 1;
+    put(&m2, 4);
+    put(kval.data() + a, 8 * (size_t)nz);
+    put(idx.data(), 4 * (size_t)nz);
+    put(&m2, 4);
+  }
+  ok = std::fclose(f) == 0 && ok;
+  if (!ok) *why = "write error";
+  return ok;
+}
+
+inline bool sd_read_chd(const std::string& path, int ncol, std::vector<long>& kptr, std::vector<int>& krow,
+                        std::vector<double>& kval, std::string* why) {
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) {
+    *why = std::strerror(errno);
+    return false;
+  }
+  auto bad = [&](const std::string& m) {
+    *why = m;
+    std::fclose(f);
+    return false;
+  };
+  auto rd = [&](void* p, size_t n) { return n == 0 || std::fread(p, 1, n, f) == n; };
+  kptr.assign((size_t)ncol + 1, 0);
+  krow.clear();
+  kval.clear();
+  std::vector<double> v;
+  std::vector<int64_t> r;
+  std::vector<int32_t> r4;
+  std::vector<std::pair<int64_t, double>> col;
+  for (int k = 0; k < ncol; ++k) {
+    const std::string at = "column " + std::to_string(k + 1) + " of " + std::to_string(ncol);
+    int32_t m1 = 0, e1 = 0;
+    if (!rd(&m1, 4)) return bad(at + ": file ends");
+    if (m1 != 4 && m1 != 8) return bad(at + ": a count record of " + std::to_string(m1) + " bytes");
+    int64_t nz = 0;
+    if (m1 == 4) {
+      int32_t t = 0;
+      if (!rd(&t, 4)) return bad(at + ": file ends");
+      nz = t;
+    } else if (!rd(&nz, 8)) {
+      return bad(at + ": file ends");
+    }
+    if (!rd(&e1, 4) || e1 != m1) return bad(at + ": record markers do not match");
+    if (nz < 0 || nz > ncol) return bad(at + ": count " + std::to_string(nz));
+    int32_t m2 = 0, e2 = 0;
+    if (!rd(&m2, 4) || (int64_t)m2 != nz * (8 + m1)) return bad(at + ": the value record does not hold nz entries");
+    v.resize(nz);
+    r.resize(nz);
+    if (!rd(v.data(), 8 * (size_t)nz)) return bad(at + ": file ends");
+    if (m1 == 4) {
+      r4.resize(nz);
+      if (!rd(r4.data(), 4 * (size_t)nz)) return bad(at + ": file ends");
+      for (int64_t j = 0; j < nz; ++j) r[j] = r4[j];
+    } else if (!rd(r.data(), 8 * (size_t)nz)) {
+      return bad(at + ": file ends");
+    }
+    if (!rd(&e2, 4) || e2 != m2) return bad(at + ": record markers do not match");
+    col.resize(nz);
+    for (int64_t j = 0; j < nz; ++j) {
+      if (r[j] < 1 || r[j] > ncol) return bad(at + ": row " + std::to_string(r[j]) + " outside 1.." + std::to_string(ncol));
+      col[j] = {r[j] - 1, v[j]};
+    }
+    std::stable_sort(col.begin(), col.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    for (const auto& e : col) {
+      krow.push_back((int)e.first);
+      kval.push_back(e.second);
+    }
+    kptr[k + 1] = (long)krow.size();
+  }
+  char extra;
+  if (std::fread(&extra, 1, 1, f) == 1) return bad("more data after the " + std::to_string(ncol) + " columns");
+  std::fclose(f);
+  return true;
+}
 
 // Simulations stepped at once from several host threads whose cooperative
 // GMRES solves run together, one XCD each (ib_batch): solve() takes over
@@ -781,6 +901,47 @@ struct ib_handle {
       if (read_err_flag()) return fail(IB_E_INVAL, "sub-domain point regularisation stencil leaves the grid");
       csr_sd.build(st);
     }
+    std::vector<long> kptr(nf_sd + 1, 0);
+    std::vector<int> krow;
+    std::vector<double> kval;
+    const std::string chd = sd_chd_path();
+    if (c.precomputed) {
+      // sub_domain_precomputed: B from the file an earlier run wrote
+      // (src/timestepper.F90:753-766); no sparsity windows are formed
+      std::string why;
+      if (!sd_read_chd(chd, nf_sd, kptr, krow, kval, &why))
+        return fail(IB_E_INVAL, "sub-domain: cannot read the precomputed B from " + chd + ": " + why);
+      sd_nnz_eval = 0;
+    } else {
+      const int rc = sd_compute_b(kptr, krow, kval);
+      if (rc != IB_OK) return rc;
+      // like preprocess_sd (src/timestepper.F90:690-695,725-728): the kept
+      // columns go to the file, where a case directory has an input folder
+      std::string why;
+      if (sd_chd_writable(chd) && !sd_write_chd(chd, kptr, krow, kval, &why))
+        return fail(IB_E_INVAL, "sub-domain: cannot write " + chd + ": " + why);
+    }
+    sd_nnz_kept = (int64_t)krow.size();
+    bptr.alloc(nf_sd + 1);
+    brow.alloc(std::max<size_t>(krow.size(), 1));
+    bval.alloc(std::max<size_t>(kval.size(), 1));
+    IB_HIP(hipMemcpy(bptr.p, kptr.data(), sizeof(long) * (nf_sd + 1), hipMemcpyHostToDevice));
+    if (!krow.empty()) {
+      IB_HIP(hipMemcpy(brow.p, krow.data(), sizeof(int) * krow.size(), hipMemcpyHostToDevice));
+      IB_HIP(hipMemcpy(bval.p, kval.data(), sizeof(double) * kval.size(), hipMemcpyHostToDevice));
+    }
+    rsd.alloc(nf_sd);
+    h0sd.alloc(nf_sd);
+    d_pidx.alloc((size_t)sd_np * std::max(nf, 1));
+    d_pw.alloc((size_t)sd_np * std::max(nf, 1));
+    return sd_build_interp();
+  }
+
+  // B of the sub-domain (get_sparsity_EEt + preprocess_sd): the rows of
+  // each column, the columns by batched unit sources, the drop tolerance;
+  // kept entries by column, rows ascending
+  int sd_compute_b(std::vector<long>& kptr, std::vector<int>& krow, std::vector<double>& kval) {
+    const ib_subdomain& c = sdc;
     // rows of each column: the E E^T windows (half-width sn) of two entries
     // share a column of the window matrix exactly when both entries have the
     // same component and their stencil cells differ by at most 2 sn in x and
@@ -838,9 +999,6 @@ struct ib_handle {
     IB_HIP(hipMemcpyAsync(vals.data(), dvals.p, sizeof(double) * total, hipMemcpyDeviceToHost, st));
     sync();
     // drop tolerance: keep |B(r, j)| > droptol * B(j, j), in row order
-    std::vector<long> kptr(nf_sd + 1, 0);
-    std::vector<int> krow;
-    std::vector<double> kval;
     krow.reserve(total / 2);
     kval.reserve(total / 2);
     for (int k = 0; k < nf_sd; ++k) {
@@ -855,20 +1013,7 @@ struct ib_handle {
       kptr[k + 1] = (long)krow.size();
     }
     sd_nnz_eval = total;
-    sd_nnz_kept = (int64_t)krow.size();
-    bptr.alloc(nf_sd + 1);
-    brow.alloc(std::max<size_t>(krow.size(), 1));
-    bval.alloc(std::max<size_t>(kval.size(), 1));
-    IB_HIP(hipMemcpy(bptr.p, kptr.data(), sizeof(long) * (nf_sd + 1), hipMemcpyHostToDevice));
-    if (!krow.empty()) {
-      IB_HIP(hipMemcpy(brow.p, krow.data(), sizeof(int) * krow.size(), hipMemcpyHostToDevice));
-      IB_HIP(hipMemcpy(bval.p, kval.data(), sizeof(double) * kval.size(), hipMemcpyHostToDevice));
-    }
-    rsd.alloc(nf_sd);
-    h0sd.alloc(nf_sd);
-    d_pidx.alloc((size_t)sd_np * std::max(nf, 1));
-    d_pw.alloc((size_t)sd_np * std::max(nf, 1));
-    return sd_build_interp();
+    return IB_OK;
   }
 
   // setup_reg_subdomain_indices (src/grid.F90:777-951) for the current body

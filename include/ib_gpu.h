@@ -246,7 +246,10 @@ typedef struct ib_subdomain {
   double sdxi, sdxe, sdyi, sdye; /* the rectangle of sub-domain points */
   double delta_sd;               /* their spacing */
   int32_t full_rectangular;      /* 0: the rigid body's points are sub-domain points too */
-  int32_t precomputed;           /* sub_domain_precomputed of the deck (B is computed here either way) */
+  int32_t precomputed;           /* sub_domain_precomputed of the deck: 1 reads B from input/sd_precomputed.chd
+                                    (src/timestepper.F90:753-766; IB_SD_PRECOMPUTED_FILE names another
+                                    path), 0 computes B and writes that file where its folder exists
+                                    (:690-695,725-728) */
   double droptol;                /* drop tolerance of B (0: 7e-3, src/grid.F90:14) */
   int32_t interp_points;         /* points of P per body entry, a square (0: 4, src/grid.F90:13) */
   int32_t support_nz;            /* half-width of the sparsity window (0: 10, src/variables.F90:440) */

@@ -290,6 +290,45 @@ def test_fsi_stiffness_and_pitch_match_finite_differences(nsteps, ckpt):
     assert abs(dg - fdg) <= 1e-5 * abs(fdg)
 
 
+def test_fsi_gradients_match_finite_differences_of_the_oracle():
+    """The same moving-body case from rest, 5 steps in 3 segments: dJ/dK,
+    dJ/dgamma0 and dJ/dRe of the GPU adjoint against central differences of
+    the oracle's forward steps (the CPU restatement of the reference's
+    step), so the gradient is checked against the reference's algorithm
+    without the GPU forward in between."""
+    from oracle import binding as ob
+    p, b = tors_case()
+    L, P = p.mgridlev, (p.m - 1) * (p.n - 1)
+    w0 = np.zeros(L * P)
+    cx, cy = np.array([0.4, 1.0]), np.array([0.2, -0.7])
+    n = 5
+    J, dre, _, dK, dg = sim_from(p, b, w0, w0).adjoint(n, cx, cy, ckpt_every=2, omega_grad=False, body_grad=True)
+
+    def oracle_J(pp, bb):
+        o = ob.OracleSim(pp, bb)
+        acc = 0.0
+        for _ in range(n):
+            assert o.advance().status == abi.IB_OK
+            fx, fy = o.forces(0)
+            acc += float(np.dot(cx, fx) + np.dot(cy, fy))
+        o.close()
+        return acc / n
+
+    assert abs(J - oracle_J(p, b)) <= 1e-9 * abs(J)
+
+    def fd_k(hk):
+        return (oracle_J(*tors_case(k=0.2 + hk)) - oracle_J(*tors_case(k=0.2 - hk))) / (2 * hk)
+    fdk = (4 * fd_k(1e-3) - fd_k(2e-3)) / 3
+    g0 = b[1].gamma0
+    fdg = (oracle_J(*tors_case(g0=g0 + 1e-6)) - oracle_J(*tors_case(g0=g0 - 1e-6))) / 2e-6
+    fdre = (oracle_J(*tors_case(re=50.0 + 1e-2)) - oracle_J(*tors_case(re=50.0 - 1e-2))) / 2e-2
+    print(f"oracle differences: dJ/dK adjoint {dK[1]:.12e} fd {fdk:.12e}; dJ/dgamma0 adjoint {dg[1]:.12e} "
+          f"fd {fdg:.12e}; dJ/dRe adjoint {dre:.12e} fd {fdre:.12e}")
+    assert abs(dK[1] - fdk) <= 1e-5 * abs(fdk)
+    assert abs(dg[1] - fdg) <= 1e-5 * abs(fdg)
+    assert abs(dre - fdre) <= 1e-5 * abs(fdre)
+
+
 def test_fsi_re_omega_and_rigid_positions_match_finite_differences():
     p, b = tors_case()
     L, P = p.mgridlev, (p.m - 1) * (p.n - 1)
@@ -371,8 +410,15 @@ def test_assembled_gmres_form_matches_finite_differences():
 def test_c3_plate_adjoint_matches_finite_differences():
     """The C3 geometry and grid (hinged plate of 101 points, 400x400x3, K =
     0.1, pitch 10 deg, Re 200, dt 1e-3; tolerances 1e-12 for clean
-    differences), 10 steps from rest with a checkpoint every 5: dJ/dK and
-    dJ/dgamma0 of the mean lift against central differences."""
+    differences), 200 steps from rest with a checkpoint every 20 (10
+    segments, 180 steps recomputed): dJ/dK and dJ/dgamma0 of the mean lift
+    against central differences.  Measured over h: the K differences
+    approach the adjoint as h grows (2.1e-5 relative at h = 1e-6, where J
+    moves by 7e-8 against its ~1e-12 convergence noise; 1.2e-6 at 1e-5,
+    2.4e-7 at 1e-3), so h = 1e-3 (1% of K).  The pitch differences are
+    closest at h = 1e-6 (9e-6; 2.9e-5 at 1e-5): a larger turn moves plate
+    points across the radii where deltafnc's REAL(4) literals make the
+    weights jump, and the adjoint differentiates each branch."""
     from ib_parallel_amd import GpuSim
 
     def make(K=0.1, g0=None):
@@ -391,17 +437,17 @@ def test_c3_plate_adjoint_matches_finite_differences():
         s.close()
         return acc / n
 
-    n = 10
+    n = 200
     p, b = make()
     g0 = b[0].gamma0
     s = GpuSim(p, b)
-    J, _, _, dK, dg = s.adjoint(n, [0.0], [1.0], ckpt_every=5, omega_grad=False, body_grad=True)
+    J, _, _, dK, dg = s.adjoint(n, [0.0], [1.0], ckpt_every=20, omega_grad=False, body_grad=True)
     s.close()
     assert abs(J - mean_lift(p, b, n)) <= 1e-12 * abs(J)
-    hK = 1e-3 * 0.1
+    hK = 1e-3
     fdK = (mean_lift(*make(0.1 + hK), n) - mean_lift(*make(0.1 - hK), n)) / (2 * hK)
-    hg = 1e-4
+    hg = 1e-6
     fdg = (mean_lift(*make(g0=g0 + hg), n) - mean_lift(*make(g0=g0 - hg), n)) / (2 * hg)
     print(f"C3 plate: dJ/dK adjoint {dK[0]:.10e} fd {fdK:.10e}; dJ/dgamma0 adjoint {dg[0]:.10e} fd {fdg:.10e}")
     assert abs(dK[0] - fdK) <= 1e-5 * abs(fdK)
-    assert abs(dg[0] - fdg) <= 1e-5 * abs(fdg)
+    assert abs(dg[0] - fdg) <= 2e-5 * abs(fdg)

@@ -146,3 +146,13 @@ def test_driver_runs_a_subdomain_deck(tmp_path):
         assert rel(got[k, 1:], forces[k]) < 1e-8
     it = np.loadtxt(tmp_path / "output" / "iter01.dat")
     assert [tuple(int(v) for v in row[1:]) for row in it] == iters  # ksp_iter 0, as the reference writes it
+    # the run wrote input/sd_precomputed.chd; a second run with
+    # sub_domain_precomputed = T reads B from it and writes the same forces
+    assert (tmp_path / "input" / "sd_precomputed.chd").stat().st_size > 0
+    sd.precomputed = 1
+    decks.write_ib_inp(tmp_path / "input" / "ib.inp", p, istop=3, isave=3, subdomain=sd)
+    first = (tmp_path / "output" / "force01_00.dat").read_bytes()
+    r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert decks.subdomain_of(decks.read_case(str(tmp_path / "input"))[2]).precomputed == 1
+    assert (tmp_path / "output" / "force01_00.dat").read_bytes() == first

@@ -1,5 +1,7 @@
 """Sub-domain approach on the GPU against the oracle (same sparsified B,
 same P, same lumat, same steps)."""
+import struct
+
 import numpy as np
 import pytest
 
@@ -118,3 +120,94 @@ def test_reference_example_deck():
         assert rel(rot.ravel(), lv["omega"]) < 1e-10
     fx, fy = g.forces()
     assert np.all(np.isfinite(fx)) and np.all(np.isfinite(fy)) and fx[0] > 0
+
+
+def read_chd(path):
+    """input/sd_precomputed.chd read by its record layout (Fortran unformatted
+    sequential, 4-byte markers; per column a count record, then values and
+    1-based rows), independently of the library."""
+    raw = open(path, "rb").read()
+    cols, o = [], 0
+
+    def i32(at):
+        return struct.unpack_from("<i", raw, at)[0]
+    while o < len(raw):
+        m = i32(o)
+        nz = int.from_bytes(raw[o + 4:o + 4 + m], "little", signed=True)
+        assert i32(o + 4 + m) == m
+        o += 8 + m
+        m2 = i32(o)
+        assert m2 == nz * (8 + m)
+        vals = np.frombuffer(raw, dtype="<f8", count=nz, offset=o + 4)
+        rows = np.frombuffer(raw, dtype="<i4" if m == 4 else "<i8", count=nz, offset=o + 4 + 8 * nz)
+        assert i32(o + 4 + m2) == m2
+        o += 8 + m2
+        cols.append((vals.copy(), rows.astype(np.int64)))
+    return cols
+
+
+def write_chd(path, cols, isz=4):
+    it = "<i4" if isz == 4 else "<i8"
+    with open(path, "wb") as f:
+        for vals, rows in cols:
+            nz = len(vals)
+            f.write(struct.pack("<i", isz) + np.array([nz], it).tobytes() + struct.pack("<i", isz))
+            m2 = nz * (8 + isz)
+            f.write(struct.pack("<i", m2) + np.asarray(vals, "<f8").tobytes() + np.asarray(rows, it).tobytes() +
+                    struct.pack("<i", m2))
+
+
+def test_precomputed_b_file_round_trip(tmp_path, monkeypatch):
+    """sub_domain_precomputed: a handle that computes B writes
+    input/sd_precomputed.chd in the reference's record layout (where the
+    input folder exists); one created with the flag reads it back instead of
+    computing B and steps bit for bit alike.  The values really come from
+    the file: B scaled by 2 and 3 (rows reversed, 8-byte integers as a
+    64-bit-index PETSc writes them) moves lumat linearly.  A damaged or
+    missing file is an error."""
+    from ib_parallel_amd import GpuSim, IBError
+    monkeypatch.chdir(tmp_path)
+    (tmp_path / "input").mkdir()
+    kw, (x0, x1, y0, y1, d, full, dtol) = CASES["full_rectangle"]
+    p, b = decks.tiny_case(**kw)
+
+    def sim(pre):
+        sd = abi.make_subdomain(x0, x1, y0, y1, d, full_rectangular=full, droptol=dtol, precomputed=pre)
+        return GpuSim(sd_params(p), b, subdomain=sd)
+    g0 = sim(0)
+    nb_sd, nev, nkept = g0.subdomain_info()
+    path = tmp_path / "input" / "sd_precomputed.chd"
+    cols = read_chd(path)
+    assert len(cols) == 2 * nb_sd and sum(len(v) for v, _ in cols) == nkept < nev
+    for j, (v, r) in enumerate(cols):
+        assert np.all(np.diff(r) > 0) and r[0] >= 1 and r[-1] <= 2 * nb_sd
+        assert j + 1 in r  # the diagonal is always kept
+    g1 = sim(1)
+    assert g1.subdomain_info() == (nb_sd, 0, nkept)
+    L0 = g0.subdomain_matrix()
+    assert np.array_equal(g1.subdomain_matrix(), L0)
+    for _ in range(2):
+        i0, i1 = g0.advance(), g1.advance()
+        assert (i0.ksp_iter, i0.fsi_iter, i0.status) == (i1.ksp_iter, i1.fsi_iter, i1.status)
+    for k in range(1, g0.L + 1):
+        assert np.array_equal(g0.level(k)["omega"], g1.level(k)["omega"]), k
+    assert np.array_equal(g0.body()["f"], g1.body()["f"])
+    g0.close()
+    g1.close()
+    scaled = []
+    for c in (2.0, 3.0):
+        write_chd(path, [(c * v[::-1], r[::-1]) for v, r in cols], isz=8)
+        g = sim(1)
+        scaled.append(g.subdomain_matrix())
+        g.close()
+    # lumat = body_mat + P B^T P^T: L3 - L1 = 2 (L2 - L1)
+    assert rel((scaled[1] - L0).ravel(), (2 * (scaled[0] - L0)).ravel()) < 1e-12
+    write_chd(path, cols)
+    path.write_bytes(path.read_bytes()[:-10])
+    with pytest.raises(IBError) as e:
+        sim(1)
+    assert e.value.code == abi.IB_E_INVAL and "sd_precomputed.chd" in str(e.value)
+    path.unlink()
+    with pytest.raises(IBError) as e:
+        sim(1)
+    assert e.value.code == abi.IB_E_INVAL
