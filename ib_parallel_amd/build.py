@@ -41,7 +41,8 @@ def build_lib(force=True, verbose=False):
     if not force and _newer(LIB, deps):
         return LIB
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + HIPFLAGS + ["-I", os.path.join(ROOT, "include")] + srcs + ["-o", tmp]
+    # rocSOLVER (with rocBLAS): the adjoint's LU of the dense Schur matrix
+    cmd = [HIPCC] + HIPFLAGS + ["-I", os.path.join(ROOT, "include")] + srcs + ["-lrocsolver", "-lrocblas", "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -21,6 +21,8 @@
 #include <string>
 #include <vector>
 
+#include <rocsolver/rocsolver.h>
+
 #include "ib_handle.h"
 
 void ib_handle::adjoint_setup(int nsteps, int K) {
@@ -129,9 +131,48 @@ void ib_handle::compute_green_d() {
   dG_build = green_builds;
 }
 
-// GMRES on S^T (SdT), zero initial guess, the deck's tolerances
-int ib_handle::solve_T(const double* b, double* x) {
-  fill(st, x, nf, 0.0);
+// S = P L U of the current Sd for the reverse step's solves with S^T: one
+// factorisation per reverse step (moving bodies) or per run (the fixed S of
+// rigid bodies) in place of a GMRES solve per right side.  The adjoint
+// differentiates the converged forward map, so the exact solve is what it
+// needs; GMRES on S^T took ~1900 iterations per force-adjoint solve on the
+// airfoil + flap system (cond(S) ~ 1.5e8), most of a reverse step.
+void ib_handle::lu_factor() {
+  if (!lu_blas) {
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success) throw HipError("adjoint: rocblas_create_handle failed");
+    lu_blas = h;
+  }
+  rocblas_handle h = static_cast<rocblas_handle>(lu_blas);
+  if (rocblas_set_stream(h, st) != rocblas_status_success) throw HipError("adjoint: rocblas_set_stream failed");
+  if (LUd.n < (size_t)nf * nf) LUd.alloc((size_t)nf * nf);
+  if (lu_piv.n < (size_t)nf) lu_piv.alloc(nf);
+  if (!lu_info.p) lu_info.alloc(1);
+  copy(st, LUd, Sd, (long)nf * nf);
+  if (rocsolver_dgetrf(h, nf, nf, LUd.p, nf, lu_piv.p, lu_info.p) != rocblas_status_success)
+    throw HipError("adjoint: rocsolver_dgetrf failed");
+  int info = 0;
+  IB_HIP(hipMemcpyAsync(&info, lu_info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  sync();
+  if (info != 0) throw HipError("adjoint: the Schur matrix is singular (getrf info " + std::to_string(info) + ")");
+  lu_ok = true;
+}
+
+void ib_handle::lu_release() {
+  if (lu_blas) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(lu_blas));
+  lu_blas = nullptr;
+  lu_ok = false;
+}
+
+int ib_handle::solve_T(const double* b, double* x, bool warm) {
+  if (lu_ok) {
+    copy(st, x, b, nf);
+    if (rocsolver_dgetrs(static_cast<rocblas_handle>(lu_blas), rocblas_operation_transpose, nf, 1, LUd.p, nf,
+                         lu_piv.p, x, nf) != rocblas_status_success)
+      throw HipError("adjoint: rocsolver_dgetrs failed");
+    return IB_OK;
+  }
+  if (!warm) fill(st, x, nf, 0.0);
   if (dense_solve_ok()) return dense_solve(SdT, b, x);
   GmresDev init{};
   init.first = 1;
@@ -180,7 +221,7 @@ void ib_handle::body_restore(const double* bs) {
   if (xfv.n < (size_t)nF) {
     xfv.alloc(nF); vTv.alloc(nF); yfv.alloc(nF); zfv.alloc(nF); lfv.alloc(nF); dgv.alloc(nF); uTv.alloc(nF);
   }
-  transpose(st, nf, Sd, SdT);
+  lu_factor();
   setup_reg_d(st, g, nf, xb, yb, comp, delta, offx, offy, prm.deltafnc_f32_literals, dwx, dwy);
   dthw(st, nf, ntors, tors_of, tb, th, dwx, dwy, dthv);
 }
@@ -417,7 +458,8 @@ int ib_handle::adjoint_step(int sl, bool first) {
     // -<rhsf bar, dS/dRe f>
     if (graphs) replay(120, corrector);
     else corrector();
-    const int rc = solve_T(af, arf);
+    // from the previous reverse step's solution (zero before the first)
+    const int rc = solve_T(af, arf, true);
     if (rc != IB_OK) return fail(rc, "adjoint: GMRES on S^T did not converge");
     gemv(st, nf, dA, f0, af2);
     dot_acc(st, nf, arf, af2, -1.0, adot);
@@ -470,7 +512,11 @@ int ib_handle::fsi_adjoint(int sl, bool first, const double* f0, const double* q
   // f on the face set and G W^T f (hx, hy hold E^T f on the touched faces)
   face_gather(st, g, nF, Fkeys, hx, hy, xfv);
   gemv(st, nF, Gm, xfv, vTv);
-  int rc = solve_T(af, aav);
+  // both solves start from the previous reverse step's solutions (zero
+  // before the first): S and its right sides change little from step to
+  // step, and the sweep visits the steps in the same order for every
+  // segment length, so the results do not depend on the checkpoints
+  int rc = solve_T(af, aav, true);
   if (rc != IB_OK) return fail(rc, "adjoint: GMRES on S^T did not converge");
   double* sc = scal;
   const long zs = 16 * (long)nt;  // per (t, s): BR_t^T Z_s at [2], r1_t^T Z_s at [3]
@@ -479,7 +525,7 @@ int ib_handle::fsi_adjoint(int sl, bool first, const double* f0, const double* q
     const int o = hb.off, n2 = 2 * hb.npts;
     fill(st, zcv, nf, 0.0);
     tors_block(st, tb, th, nt, t, 0, dt, delta, f0, aav, nullptr, zcv, sc + 16 * t);
-    rc = solve_T(zcv, Zv + (long)t * nf);
+    rc = solve_T(zcv, Zv + (long)t * nf, true);
     if (rc != IB_OK) return fail(rc, "adjoint: GMRES on S^T did not converge");
     double* r1 = r1v + (long)t * nf;
     // W G (E'_t^T f) + E'_t (G W^T f - (q* + q0)) + B'_t f - BR'_t sc
@@ -620,6 +666,12 @@ int ib_handle::adjoint_run(int nsteps, int Kin, const double* cx, const double* 
   fill(st, adot, 1, 0.0);
   fill(st, racc, g.P, 0.0);
   fill(st, raccb, g.nbc, 0.0);
+  // the initial guesses of the reverse sweep's S^T solves
+  if (adj_gm) fill(st, arf, nf, 0.0);
+  if (adj_fsi) {
+    fill(st, aav, nf, 0.0);
+    fill(st, Zv, (long)std::max(nt, 1) * nf, 0.0);
+  }
   const int it0 = itime;
   const int nseg = (nsteps + K - 1) / K, last0 = (nseg - 1) * K;
   // the reverse sweep moves the bodies: keep the forward end state
@@ -651,7 +703,15 @@ int ib_handle::adjoint_run(int nsteps, int Kin, const double* cx, const double* 
     copy(st, fin_s, s, LP); copy(st, fin_f, f, nf); copy(st, fin_fr, f_rdst, nf);
     if (adj_fsi) save_body(fin_th, nullptr);
   }
-  // backward, segment by segment
+  // backward, segment by segment; the solves with S^T use S's LU factors,
+  // formed here for the fixed S of rigid bodies and at every reverse step
+  // (body_restore) for moving ones, and dropped when the run ends
+  struct LuGuard {
+    bool& f;
+    explicit LuGuard(bool& x) : f(x) { f = false; }
+    ~LuGuard() { f = false; }
+  } lu_guard(lu_ok);
+  if (adj_gm) lu_factor();
   fill(st, aw, LP, 0.0); fill(st, aro, LP, 0.0);
   fill(st, aqx, CQX, 0.0); fill(st, aqy, CQY, 0.0);
   int status = IB_OK;

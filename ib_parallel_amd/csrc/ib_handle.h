@@ -1480,8 +1480,17 @@ struct ib_handle {
   void adjoint_setup(int nsteps, int K);
   // G's derivative in Re on the current face set (columns with d lam1i/dRe)
   void compute_green_d();
-  // GMRES on S^T (SdT), zero initial guess, the deck's tolerances
-  int solve_T(const double* b, double* x);
+  // the adjoint's solves with S^T: x = S^-T b by the LU factors of S when
+  // lu_factor has run for the current S, else GMRES on S^T (SdT) from x
+  // (zeroed first unless warm) with the deck's tolerances
+  int solve_T(const double* b, double* x, bool warm = false);
+  // S = P L U (rocSOLVER getrf, partial pivoting) of the current Sd
+  void lu_factor();
+  void lu_release();
+  void* lu_blas = nullptr;  // rocblas_handle on st
+  DBuf<double> LUd;
+  DBuf<int> lu_piv, lu_info;
+  bool lu_ok = false;
   // the body at the end of a taped step's FSI loop: theta (and its history),
   // positions, weights, S (and S^T), the weight derivatives
   void body_restore(const double* bs);
@@ -1788,6 +1797,7 @@ struct ib_handle {
 
   ~ib_handle() {
     if (st) (void)hipStreamSynchronize(st);
+    lu_release();
     for (auto& kv : gm_graphs) (void)hipGraphExecDestroy(kv.second);
     for (auto& kv : step_graphs) (void)hipGraphExecDestroy(kv.second);
     if (ev_flags) (void)hipEventDestroy(ev_flags);
