@@ -331,6 +331,37 @@ def run_c4_batch(args):
             "adjoint": adj}
 
 
+def run_flexible(args):
+    """The strongly coupled flexible plate as specified: a beam (family 3) of
+    50 Hermite elements on 101 points, mass ratio 0.2, K_B = 0.1, pitch 10
+    deg, Re 200, 400x400 cells, 4 levels, dt 1e-3, tolerance 1e-10; plain
+    steps after a warm-up."""
+    from ib_parallel_amd import GpuSim, decks
+    p, b = decks.beam_plate_case()
+    sim = GpuSim(p, b)
+    for _ in range(5):
+        sim.advance()
+    device_sync()
+    n = args.flex_steps
+    its = fsi = 0
+    t0 = time.perf_counter()
+    for _ in range(n):
+        info = sim.advance()
+        its += info.ksp_iter
+        fsi += info.fsi_iter
+    device_sync()
+    secs = time.perf_counter() - t0
+    st = sim.beam(0)
+    out = {"workload": "flexible plate (Euler-Bernoulli beam, 50 elements, 101 points, mass ratio 0.2, K_B 0.1, "
+                       "pitch 10 deg) FSI 400x400x4, Re=200, dt=1e-3, tol=1e-10",
+           "steps": n, "warmup": 5, "ms_per_step": round(secs / n * 1e3, 3),
+           "value": round(gridpts(sim) * n / secs / 1e6, 3), "unit": "Mgridpt-steps/s",
+           "gmres_its_per_step": round(its / n, 1), "fsi_its_per_step": round(fsi / n, 2),
+           "max_abs_deflection": float(abs(st["u"][0::2]).max())}
+    sim.close()
+    return out
+
+
 def _time_adjoint(sim, n, cx, cy, ckpts, **grads):
     """Forward wall time of n plain steps, then the wall time of n adjoint
     steps per segment length in ckpts (None: the whole run taped)."""
@@ -631,6 +662,8 @@ def main():
                     help="instances of the airfoil + flap batch measurement (0: skip)")
     ap.add_argument("--c4-batch-steps", type=int, default=5)
     ap.add_argument("--c4-batch-only", action="store_true", help="only the airfoil + flap batch measurement")
+    ap.add_argument("--flex-steps", type=int, default=20,
+                    help="timed steps of the flexible-plate (beam) workload (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the library has no multi-GPU path",
@@ -682,6 +715,8 @@ def main():
         out["adjoint"] = run_adjoint(args)
     if args.c4_batch > 0:
         out["c4_batch"] = run_c4_batch(args)
+    if args.flex_steps > 0:
+        out["flexible_plate"] = run_flexible(args)
     if not args.no_cpu:
         cpu = run_cpu(args, g["its"] / args.steps, g["fsi"] / args.steps)
         out["cpu_baseline"] = cpu

@@ -107,6 +107,7 @@ SIGNATURES = {
     "pressure": (C.c_int, [_P, _D]),
     "op_nonlinear_rot": (C.c_int, [_P, C.c_int32, _D, _D, _D, _D, _D]),
     "op_gmres": (C.c_int, [_P, _D, _D, _I]),
+    "get_beam": (C.c_int, [_P, C.c_int32, _D, _D, _D, _I]),
     "create_subdomain": (C.c_int, [C.POINTER(IBParams), C.POINTER(IBSubdomain), C.c_int32, C.POINTER(IBBody),
                                    C.POINTER(_P)]),
     "subdomain_info": (C.c_int, [_P, _I, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

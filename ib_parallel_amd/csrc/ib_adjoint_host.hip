@@ -633,6 +633,7 @@ int ib_handle::fsi_adjoint(int sl, bool first, const double* f0, const double* q
 int ib_handle::adjoint_run(int nsteps, int Kin, const double* cx, const double* cy, double* J, double* dRe, double* dw0,
                 double* dxb, double* dyb, double* dK, double* dg0) {
   const bool stat_ok = prm.num_stat && ntors == 0 && !sd_on && nf > 0;
+  if (nbeam > 0) return fail(IB_E_UNSUPPORTED, "the adjoint does not cover family-3 (beam) bodies");
   const bool fsi_ok = !prm.num_stat && facemode && !sd_on && nf > 0;
   const bool gm_ok = !prm.num_stat && assembled && !facemode && ntors == 0 && !sd_on && nf > 0;
   if (!stat_ok && !fsi_ok && !gm_ok)

@@ -244,6 +244,14 @@ struct ib_handle {
   // bodies
   int nb = 0, nf = 0, nbodies = 0, ntors = 0;
   std::vector<int> h_cbody, h_ccomp, h_tors_of;
+  std::vector<char> h_moving;  // per entry: a torsional or beam body's
+  // family-3 bodies: host matrices, force-vector offsets, device views into beam_mem
+  int nbeam = 0;
+  std::vector<BeamHost> h_beam;
+  std::vector<int> h_beam_off;
+  std::vector<BeamDev> beams;
+  DBuf<double> beam_mem;
+  int nmoving() const { return ntors + nbeam; }
   std::vector<TorsBody> h_tb;
   DBuf<double> xb, yb, f, f_rdst, w, rhsf, gw, opx;
   DBuf<int> comp, tors_of, err_flag;
@@ -468,7 +476,7 @@ struct ib_handle {
 
   // The column-batch buffers are only needed while columns are formed;
   // forms whose matrix never changes give them back after set-up.
-  bool keeps_columns() const { return !sd_on && (facemode || (assembled && ntors > 0)); }
+  bool keeps_columns() const { return !sd_on && (facemode || (assembled && nmoving() > 0)); }
   void trim_columns() {
     if (keeps_columns() || !Vb.p) return;
     sync();
@@ -561,6 +569,15 @@ struct ib_handle {
       schur_from_green(st, nf, nF, w, tapidx, Gm, Tm, tors_of, tb, th, dt, delta, Sd);
       tcache = true;
     }
+    // both forms rewrite every row and column of the moving entries
+    for (const BeamDev& b : beams) beam_add_schur(st, b, nf, dt, delta, Sd);
+  }
+  // family 3: the structural parts of the FSI iteration
+  void beams_rhs() {
+    for (const BeamDev& b : beams) beam_rhs(st, b, dt, delta, rhsf);
+  }
+  void beams_dx() {
+    for (int k = 0; k < nbeam; ++k) beam_dx(st, beams[k], dt, delta, f, xb, yb, dabs + ntors + k);
   }
 
   void build_face_set() {
@@ -1129,17 +1146,22 @@ struct ib_handle {
       if (itime == 1) body_initial_acc(st, ntors, tb, th, dt, delta, f);
       IB_HIP(hipMemcpyAsync(th + 3 * ntors, th.p, sizeof(double) * 3 * ntors, hipMemcpyDeviceToDevice, st));
     }
+    for (const BeamDev& b : beams) {
+      if (itime == 1) beam_initial_acc(st, b, dt, delta, f);
+      beam_save0(st, b);
+    }
     // moving bodies in the face-space form with a dense solve: the GMRES
     // status, the stencil check of the new weights, the face-set check of the
     // new taps and |d theta| come back in one host synchronisation per
     // iteration
-    const bool one_sync = ntors > 0 && !sd_on && facemode && dense_solve_ok();
+    const bool one_sync = nmoving() > 0 && !sd_on && facemode && dense_solve_ok();
     while (err_glob >= prm.fsi_tol && fsi_count <= prm.fsi_max_count) {
       ++fsi_count;
       int its = 0;
       int rc = IB_OK;
       if (one_sync) {
         fsi_rhs(st, g, nf, tors_of, tb, ntors, th, dt, delta, w, csr.key, qx, qy, q0x[0], rhsf);
+        beams_rhs();
         auto turn = coop_turn();
         tim[4].begin(st, 0.0);
         dense_launch(Sd, rhsf, f);
@@ -1147,13 +1169,14 @@ struct ib_handle {
         redist_faces(st, g, csr, w, f, hx, hy);
         regT(st, g, nf, w, csr.key, hx, hy, 0.0, nullptr, f_rdst);
         body_dx(st, ntors, tb, th, dt, delta, f, xb, yb, dabs);
+        beams_dx();
         // face_flag[0]: a tap outside the face set; [1]: a tap key moved (the
         // sorted tap list is rebuilt after the synchronisation, only then;
         // the next GMRES solve does not read it)
         IB_HIP(hipMemsetAsync(face_flag.p, 0, 2 * sizeof(int), st));
         rebuild_weights(face_flag.p + 1);
         map_taps(st, nf, csr.key, fmap, tapidx, face_flag);
-        IB_HIP(hipMemcpyAsync(hbuf, dabs.p, sizeof(double) * ntors, hipMemcpyDeviceToHost, st));
+        IB_HIP(hipMemcpyAsync(hbuf, dabs.p, sizeof(double) * nmoving(), hipMemcpyDeviceToHost, st));
         IB_HIP(hipMemcpyAsync(hflag, err_flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
         IB_HIP(hipMemcpyAsync(hflag + 1, face_flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
         IB_HIP(hipMemcpyAsync(hst, gst, sizeof(GmresDev), hipMemcpyDeviceToHost, st));
@@ -1182,11 +1205,12 @@ struct ib_handle {
           form_schur_face(false);
         }
         err_glob = 0.0;
-        for (int t = 0; t < ntors; ++t) err_glob = std::max(err_glob, hbuf[t]);
+        for (int t = 0; t < nmoving(); ++t) err_glob = std::max(err_glob, hbuf[t]);
         continue;
       }
       if (!sd_on) {
         fsi_rhs(st, g, nf, tors_of, tb, ntors, th, dt, delta, w, csr.key, qx, qy, q0x[0], rhsf);
+        beams_rhs();
         rc = gmres_dev(rhsf, f, &its);
         ksp_iter += its;
       } else {
@@ -1212,17 +1236,18 @@ struct ib_handle {
         if (ri != IB_OK) return ri;
         err_glob = 0.0;
         for (int t = 0; t < ntors; ++t) err_glob = std::max(err_glob, hbuf[t]);
-      } else if (ntors > 0) {
+      } else if (nmoving() > 0) {
         body_dx(st, ntors, tb, th, dt, delta, f, xb, yb, dabs);
+        beams_dx();
         rebuild_weights();
-        IB_HIP(hipMemcpyAsync(hbuf, dabs.p, sizeof(double) * ntors, hipMemcpyDeviceToHost, st));
+        IB_HIP(hipMemcpyAsync(hbuf, dabs.p, sizeof(double) * nmoving(), hipMemcpyDeviceToHost, st));
         if (read_err_flag()) {
           err = "body point regularisation stencil leaves the grid";
           return IB_E_INVAL;
         }
         refresh_schur();
         err_glob = 0.0;
-        for (int t = 0; t < ntors; ++t) err_glob = std::max(err_glob, hbuf[t]);
+        for (int t = 0; t < nmoving(); ++t) err_glob = std::max(err_glob, hbuf[t]);
       } else {
         err_glob = 0.0;  // rigid bodies only: positions and weights unchanged
       }
@@ -1571,9 +1596,20 @@ struct ib_handle {
     for (int b = 0; b < nbod; ++b) {
       const ib_body& B = bodies[b];
       if (B.npts < 2) return fail(IB_E_INVAL, "a body needs at least 2 points");
-      if (B.fam != 1 && B.fam != 2)
-        return fail(IB_E_UNSUPPORTED, "only body families 1 (rigid) and 2 (torsional) exist in the reference");
+      if (B.fam < 1 || B.fam > 3)
+        return fail(IB_E_UNSUPPORTED, "body families are 1 (rigid), 2 (torsional) and 3 (beam)");
       if (B.fam == 1 && seen_tors) return fail(IB_E_INVAL, "rigid bodies must precede torsional bodies (src/grid.F90:1226)");
+      if (B.fam == 3) {
+        seen_tors = true;
+        if (B.npts < 3 || B.npts % 2 == 0) return fail(IB_E_INVAL, "a beam needs an odd number of points >= 3");
+        if (!(B.itheta > 0) || !(B.ktheta > 0)) return fail(IB_E_INVAL, "a beam needs mass (itheta) and stiffness (ktheta) > 0");
+        if (prm.num_stat || sd) return fail(IB_E_UNSUPPORTED, "beams (family 3) need the GMRES path without num_stat or sub-domains");
+        BeamHost bh;
+        if (!beam_host_setup(B.x, B.y, B.npts, B.itheta, B.ktheta, B.ctheta, dt, &bh))
+          return fail(IB_E_INVAL, "beam matrices are singular");
+        h_beam.push_back(std::move(bh));
+        h_beam_off.push_back(nbb.back());
+      }
       if (B.fam == 2) {
         seen_tors = true;
         if (B.itheta <= 0) return fail(IB_E_INVAL, "torsional body needs itheta > 0");
@@ -1591,6 +1627,7 @@ struct ib_handle {
         h_cbody.push_back(b);
         h_ccomp.push_back(j < B.npts ? 1 : 2);
         h_tors_of.push_back(B.fam == 2 ? (int)h_tb.size() - 1 : -1);
+        h_moving.push_back(B.fam >= 2 ? 1 : 0);
         hxb.push_back(B.x[pnt]);
         hyb.push_back(B.y[pnt]);
       }
@@ -1602,6 +1639,7 @@ struct ib_handle {
     nbodies = nbod;
     nf = 2 * nb;
     ntors = (int)h_tb.size();
+    nbeam = (int)h_beam.size();
 
     IB_HIP(hipGetDevice(&dev));
     IB_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -1700,7 +1738,7 @@ struct ib_handle {
       IB_HIP(hipMemcpy(comp.p, h_ccomp.data(), sizeof(int) * nf, hipMemcpyHostToDevice));
       IB_HIP(hipMemcpy(tors_of.p, h_tors_of.data(), sizeof(int) * nf, hipMemcpyHostToDevice));
       std::vector<int> mv, rg;
-      for (int k = 0; k < nf; ++k) (h_tors_of[k] >= 0 ? mv : rg).push_back(k);
+      for (int k = 0; k < nf; ++k) (h_moving[k] ? mv : rg).push_back(k);
       nmov = (int)mv.size();
       nrig = (int)rg.size();
       movidx.alloc(std::max(nmov, 1));
@@ -1714,8 +1752,39 @@ struct ib_handle {
     tb.alloc(std::max(1, ntors));
     if (ntors > 0) IB_HIP(hipMemcpy(tb.p, h_tb.data(), sizeof(TorsBody) * ntors, hipMemcpyHostToDevice));
     th.alloc((size_t)6 * std::max(1, ntors));
-    dabs.alloc(std::max(1, ntors));
+    dabs.alloc(std::max(1, nmoving()));
     fill(st, th, (long)th.n, 0.0);
+    if (nbeam > 0) {
+      // per beam: M, C, K, K_eff, K_eff^-1, M^-1 (nd^2 each), H (np nd), G
+      // (np^2), X0, Y0, and the state u, ud, udd, u0, ud0, udd0
+      size_t tot = 0;
+      for (const BeamHost& b : h_beam)
+        tot += 6 * (size_t)b.nd * b.nd + (size_t)b.np * b.nd + (size_t)b.np * b.np + 2 * (size_t)b.np + 6 * (size_t)b.nd;
+      beam_mem.alloc(tot);
+      fill(st, beam_mem, (long)tot, 0.0);
+      double* at = beam_mem.p;
+      auto put = [&](const std::vector<double>& v) {
+        double* d = at;
+        IB_HIP(hipMemcpy(d, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
+        at += v.size();
+        return d;
+      };
+      auto take = [&](size_t n) {
+        double* d = at;
+        at += n;
+        return d;
+      };
+      for (int k = 0; k < nbeam; ++k) {
+        const BeamHost& b = h_beam[k];
+        BeamDev d{};
+        d.off = h_beam_off[k]; d.np = b.np; d.nd = b.nd; d.nx = b.nx; d.ny = b.ny;
+        d.M = put(b.M); d.C = put(b.C); d.K = put(b.K); d.Keff = put(b.Keff); d.Keffi = put(b.Keffi);
+        d.Minv = put(b.Minv); d.H = put(b.H); d.G = put(b.G); d.X0 = put(b.X0); d.Y0 = put(b.Y0);
+        d.u = take(b.nd); d.ud = take(b.nd); d.udd = take(b.nd);
+        d.u0 = take(b.nd); d.ud0 = take(b.nd); d.udd0 = take(b.nd);
+        beams.push_back(d);
+      }
+    }
 
     V.alloc((size_t)(kmax + 1) * std::max(1, nf));
     H.alloc((size_t)(kmax + 1) * kmax);
@@ -1723,7 +1792,7 @@ struct ib_handle {
     pnorm.alloc(std::max(1, ceil_div(std::max(nf, 1), 64)));
     IB_HIP(hipMalloc(&gst, sizeof(GmresDev)));
     IB_HIP(hipHostMalloc(&hst, sizeof(GmresDev), hipHostMallocDefault));
-    IB_HIP(hipHostMalloc(&hbuf, sizeof(double) * std::max(64, ntors + 1), hipHostMallocDefault));
+    IB_HIP(hipHostMalloc(&hbuf, sizeof(double) * std::max(64, nmoving() + 1), hipHostMallocDefault));
     IB_HIP(hipHostMalloc(&hflag, 4 * sizeof(int), hipHostMallocDefault));
 
     csr.alloc(nf * kTaps, g.Fx + g.Fy);
@@ -1752,7 +1821,7 @@ struct ib_handle {
       const int rc = num_stat_setup();
       if (rc != IB_OK) return rc;
     } else if (nf > 0) {
-      if (prm.schur_mode == 3 || (prm.schur_mode == 0 && ntors > 0)) {
+      if (prm.schur_mode == 3 || (prm.schur_mode == 0 && nmoving() > 0)) {
         assembled = facemode = true;
       } else if (prm.schur_mode == 2 || (prm.schur_mode == 0 && nf <= kAutoAssembleMax)) {
         assembled = true;
@@ -1782,6 +1851,8 @@ struct ib_handle {
     } else if (assembled && !sd_on) {
       alloc_assembly(nf);
     }
+    if (nbeam > 0 && !facemode)
+      return fail(IB_E_UNSUPPORTED, "beams (family 3) need the face-space Schur form (schur_mode 0 or 3)");
     if (assembled && !sd_on) {
       if (facemode) {
         form_schur_face(false);

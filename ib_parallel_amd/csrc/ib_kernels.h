@@ -445,6 +445,37 @@ struct CoopWJob {
 };
 bool gmres_coopw_multi_ok(int nf_a, int nf_b);
 void gmres_coopw_multi(hipStream_t st, int n, const CoopWJob* jobs, const int* xcds, int nxcd);
+// Family-3 bodies: a deformable plate as a linear Euler-Bernoulli beam of
+// ne = (npts - 1) / 2 Hermite elements, clamped at the first point; nd = 2
+// ne unknowns (normal deflection and slope per free node).  Matrices
+// row-major: M, C, K, K_eff = 4/dt^2 M + 2/dt C + K, their inverses where
+// named, H (np x nd) the shape functions at the points, G = H K_eff^-1 H^T
+// (np x np); n = (nx, ny) the normal of the deck's line.
+struct BeamHost {
+  int np = 0, ne = 0, nd = 0;
+  double nx = 0, ny = 1;
+  std::vector<double> M, C, K, Keff, Keffi, Minv, H, G, X0, Y0;
+};
+struct BeamDev {
+  int off, np, nd;  // entries [off, off + 2 np) of the force vector
+  double nx, ny;
+  const double *M, *C, *K, *Keff, *Keffi, *Minv, *H, *G, *X0, *Y0;
+  double *u, *ud, *udd, *u0, *ud0, *udd0;
+};
+bool beam_host_setup(const double* x, const double* y, int npts, double mass, double EI, double damp, double dt,
+                     BeamHost* out);
+// rhsf += delta Phi (ud0 + 2/dt (u0 - u) - 2/dt K_eff^-1 z)  (compute_rhs)
+void beam_rhs(hipStream_t st, const BeamDev& b, double dt, double delta, double* rhsf);
+// the Newmark update for the force f and the new point positions; *du_abs
+// gets max |du|  (compute_dx + update_xb)
+void beam_dx(hipStream_t st, const BeamDev& b, double dt, double delta, const double* f, double* xb, double* yb,
+             double* du_abs);
+// udd = M^-1 (delta/dt Phi^T f - C ud - K u)  (initial_acc, step 1)
+void beam_initial_acc(hipStream_t st, const BeamDev& b, double dt, double delta, const double* f);
+// (u0, ud0, udd0) = (u, ud, udd) at the start of a step
+void beam_save0(hipStream_t st, const BeamDev& b);
+// S += 2 delta^2/dt^2 Phi K_eff^-1 Phi^T on the beam's entries (b_times2)
+void beam_add_schur(hipStream_t st, const BeamDev& b, int nf, double dt, double delta, double* S);
 // The XCC ids of the device's XCDs (sorted), found once per device.
 std::vector<int> gmres_xcc_ids(hipStream_t st);
 bool gmres_coopw_plan(int nf, int kmax, size_t max_lds, int ncu, int* P, int* R, int* NL);

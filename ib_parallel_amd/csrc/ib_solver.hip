@@ -193,6 +193,21 @@ int ib_get_body(ib_handle* h, double* f, double* f_rdst, double* xb, double* yb,
   API_CATCH(h)
 }
 
+// family 3: beam k's state (no reference counterpart; the reference's
+// family 3 is a stub)
+int ib_get_beam(ib_handle* h, int32_t k, double* u, double* ud, double* udd, int32_t* nd) {
+  if (!h || k < 0 || k >= h->nbeam) return IB_E_INVAL;
+  API_TRY
+  const BeamDev& b = h->beams[k];
+  if (nd) *nd = b.nd;
+  if (u) h->d2h(u, b.u, b.nd);
+  if (ud) h->d2h(ud, b.ud, b.nd);
+  if (udd) h->d2h(udd, b.udd, b.nd);
+  h->sync();
+  return IB_OK;
+  API_CATCH(h)
+}
+
 // calculate_pressure (src/operators_pressure.F90:110-437) at the current
 // state; p receives mgridlev blocks of m*n values (write_pressure layout)
 int ib_pressure(ib_handle* h, double* p) {

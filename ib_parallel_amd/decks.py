@@ -4,6 +4,8 @@
   (src/parameters.F90:51-78).
 * ``read_body``: one input/body.NNN.inp (src/grid.F90:252-288): npts, fam,
   [fam 2: gamma0 (deg), alpha0 (deg), itheta, ktheta, ctheta], then x y per line.
+  Family 3 (a beam; the reference reads no parameters for its stub) takes the
+  same five lines: pitch, alpha0, mass per unit length, EI, damping.
 * ``read_case``: ib.inp + body.001.inp, body.002.inp, ... until one is missing.
 * ``cylinder_case``, ``plate_case``, ``tiny_case``: synthetic workloads with the
   shapes of the named configurations (circular cylinder 200x200x3; hinged
@@ -57,7 +59,7 @@ def read_body(path):
     vals = open(path).read().split()
     npts = int(vals[0]); fam = int(vals[1]); pos = 2
     gamma0 = alpha0 = itheta = ktheta = ctheta = 0.0
-    if fam == 2:
+    if fam in (2, 3):
         gamma0 = float(vals[2].replace("D", "E")) * math.pi / 180
         alpha0 = float(vals[3].replace("D", "E")) * math.pi / 180
         itheta, ktheta, ctheta = (float(v.replace("D", "E")) for v in vals[4:7])
@@ -96,7 +98,7 @@ def subdomain_of(nml):
 def write_body(path, b):
     with open(path, "w") as fh:
         fh.write(f"{b.npts}\n{b.fam}\n")
-        if b.fam == 2:
+        if b.fam in (2, 3):
             fh.write(f"{b.gamma0 * 180 / math.pi:.16f}\n{b.alpha0 * 180 / math.pi:.16f}\n")
             fh.write(f"{b.itheta:.16e}\n{b.ktheta:.16e}\n{b.ctheta:.16e}\n")
         for x, y in zip(b.x, b.y):
@@ -154,6 +156,33 @@ def plate_body(npts=101, length=1.0, pitch_deg=10.0, mass_ratio=0.2, k_theta=0.1
                     itheta=mass_ratio * length ** 3 / 3.0, ktheta=k_theta, ctheta=c_theta)
 
 
+def beam_body(npts=101, length=1.0, pitch_deg=10.0, mass_ratio=0.2, k_b=0.1, damping=0.0):
+    """Flexible flat plate clamped at its leading edge (origin) as a family-3
+    body: a linear Euler-Bernoulli beam of (npts - 1) / 2 Hermite elements.
+
+    Points j = 0..npts-1 at j*ds along angle -pitch (trailing edge below the
+    leading edge, flow in +x).  Mass per unit length = mass ratio (rho_s h /
+    (rho_f L) with L = 1 and unit fluid density), bending stiffness EI = K_B
+    (EI / (rho_f U^2 L^3)), damping C = damping * M.  The deck lines of
+    family 2 carry them: itheta = mass per length, ktheta = EI, ctheta =
+    damping.
+    """
+    g = -math.radians(pitch_deg)
+    ds = length / (npts - 1)
+    r = ds * np.arange(npts)
+    return BodySpec(r * math.cos(g), r * math.sin(g), fam=3, gamma0=g, alpha0=0.0,
+                    itheta=mass_ratio * length, ktheta=k_b, ctheta=damping)
+
+
+def beam_plate_case(m=400, n=400, mgridlev=4, dt=0.001, re=200.0, npts=101, tol=1e-10, gmres_restart=250, **kw):
+    """The flexible-plate FSI workload with the plate as an Euler-Bernoulli
+    beam (50 elements, 101 points, mass ratio 0.2, K_B = 0.1, pitch 10 deg,
+    Re 200) at 400x400 cells, 4 levels, dx = 0.01, dt = 1e-3, tolerance 1e-10."""
+    p = make_params(m, n, mgridlev, dt, re, 4.0, 1.0, 2.0, fsi_tol=tol, atol=tol, rtol=tol,
+                    gmres_restart=gmres_restart, **kw)
+    return p, [beam_body(npts)]
+
+
 def plate_case(m=400, n=400, mgridlev=4, dt=0.001, re=200.0, npts=101, tol=1e-10, gmres_restart=250, **kw):
     """Hinged flexible-plate FSI workload: 400x400 cells, 4 levels, dx=0.01.
 
@@ -202,14 +231,22 @@ def airfoil_flap_case(m=1024, n=512, mgridlev=5, dt=0.0005, re=1000.0, n_airfoil
     return p, [BodySpec(xa, ya, fam=1), flap]
 
 
-def tiny_case(m=32, n=24, mgridlev=2, torsional=True, tol=1e-12, dt=0.01, re=50.0, **kw):
-    """Small grid for oracle-vs-GPU parity: a coarse cylinder plus a hinged plate."""
+def tiny_case(m=32, n=24, mgridlev=2, torsional=True, tol=1e-12, dt=0.01, re=50.0, beam=False, **kw):
+    """Small grid for oracle-vs-GPU parity: a coarse cylinder plus a hinged
+    plate (or, with beam=True, a clamped flexible plate of family 3)."""
     p = make_params(m, n, mgridlev, dt, re, 4.0, 1.6, 1.5, fsi_tol=tol, atol=tol, rtol=tol, **kw)
     dx = 4.0 / m
     nc = max(8, int(round(np.pi * 0.6 / dx)))
     t = 2 * np.pi * np.arange(nc) / nc
     bodies = [BodySpec(0.3 * np.cos(t) - 0.4, 0.3 * np.sin(t), fam=1)]
-    if torsional:
+    if beam:
+        npl = max(5, int(round(0.6 / dx)) + 1)
+        npl += 1 - npl % 2  # odd: two points per element
+        g = -math.radians(15.0)
+        r = np.linspace(0, 0.6, npl)
+        bodies.append(BodySpec(0.2 + r * math.cos(g), 0.05 + r * math.sin(g), fam=3, gamma0=g,
+                               itheta=0.05, ktheta=0.02, ctheta=0.0))
+    elif torsional:
         npl = max(4, int(round(0.6 / dx)) + 1)
         g = -math.radians(15.0)
         r = np.linspace(0, 0.6, npl)

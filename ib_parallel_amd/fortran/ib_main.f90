@@ -164,7 +164,10 @@ contains
     open(unit=8, file='input/body.'//fn//'.inp', form='formatted', status='old')
     read(8, *) d%npts
     read(8, *) d%fam
-    if (d%fam == 2) then
+    ! family 3 (a beam; a stub in the reference, which reads no parameters
+    ! for it) carries the same five lines as family 2: pitch, alpha0, mass
+    ! per unit length, bending stiffness EI, damping
+    if (d%fam == 2 .or. d%fam == 3) then
       read(8, *) d%gamma0; d%gamma0 = d%gamma0 * pi / 180
       read(8, *) d%alpha0; d%alpha0 = d%alpha0 * pi / 180
       read(8, *) d%itheta

@@ -181,6 +181,15 @@ class Simulation:
         self._check(self._f["op_subdomain_matrix"](self.h, abi.dptr(A)))
         return A.T.copy()
 
+    def beam(self, k=0):
+        """State of family-3 body k: u, ud, udd (normal deflection and slope
+        per free node)."""
+        nd = C.c_int32()
+        self._check(self._f["get_beam"](self.h, k, None, None, None, C.byref(nd)))
+        u, ud, udd = np.empty(nd.value), np.empty(nd.value), np.empty(nd.value)
+        self._check(self._f["get_beam"](self.h, k, abi.dptr(u), abi.dptr(ud), abi.dptr(udd), C.byref(nd)))
+        return {"u": u, "ud": ud, "udd": udd}
+
     def gmres(self, b, x0=None, allow_noconv=False):
         """Solve the Schur system (ab_times) by GMRES from x0; returns (x, its)."""
         b = np.ascontiguousarray(b, dtype=np.float64)

@@ -153,6 +153,14 @@ int ib_get_body(ib_handle* h, double* f, double* f_rdst, double* xb, double* yb,
 /* write_force2: per body fx = 2*delta*sum(f_x)/dt, fy likewise (drag, lift).
  * which = 0: fvec, 1: f_rdstvec. fx, fy hold nbodies entries. */
 int ib_get_forces(ib_handle* h, int32_t which, double* fx, double* fy);
+/* Family 3 bodies (deformable: a linear Euler-Bernoulli beam; the
+ * reference declares the family and leaves its routines stubs,
+ * src/operators_struct.F90:104-106,146-148,376-379,472-474): the state of
+ * beam k (0-based, deck order) -- per free node (the first point is
+ * clamped) the deflection normal to the deck's line and the slope, 2 ne
+ * values each for ne = (npts - 1) / 2 elements; *nd gets 2 ne.  Any
+ * pointer may be NULL. */
+int ib_get_beam(ib_handle* h, int32_t k, double* u, double* ud, double* udd, int32_t* nd);
 
 /* --- single operators on the handle's grid (host buffers) --- */
 /* unnormalised 2-D DST-I (FFTW RODFT00 x RODFT00) of `batch` node arrays,

@@ -37,6 +37,8 @@ EXTRA = {
     "op_subdomain_interp": (C.c_int, [C.c_void_p, abi._I, abi._D, abi._D, abi._D]),
     "op_predictor": (C.c_int, [C.c_void_p, C.c_int32, abi._D, abi._D, abi._D, abi._D, abi._D]),
     "op_redistribute": (C.c_int, [C.c_void_p, abi._D, abi._D]),
+    "op_beam_matrices": (C.c_int, [C.c_void_p, C.c_int32, abi._D, abi._D, abi._D]),
+    "op_beam_newmark": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, abi._D, abi._D, abi._D, abi._D]),
 }
 
 
@@ -80,6 +82,24 @@ class OracleSim(Simulation):
         d = abi.dptr
         self._check(self._f["op_predictor"](self.h, k, d(omega), d(rhs), d(rhs_old), d(rhsbc), d(out)))
         return out
+
+    def beam_matrices(self, k=0):
+        """(M, K, H) of beam k: mass and stiffness (nd x nd) and the shape
+        functions at the points (np x nd)."""
+        nd = self.beam(k)["u"].size
+        npts = nd + 1
+        M, K, H = np.empty((nd, nd)), np.empty((nd, nd)), np.empty((npts, nd))
+        self._check(self._f["op_beam_matrices"](self.h, k, abi.dptr(M), abi.dptr(K), abi.dptr(H)))
+        return M, K, H
+
+    def beam_newmark(self, k, nsteps, fb, u, ud, udd):
+        """nsteps Newmark steps of beam k alone under fixed point forces fb
+        (x forces, then y forces); returns (u, ud, udd)."""
+        c = lambda a: np.array(a, dtype=np.float64, copy=True)
+        fb, u, ud, udd = map(c, (fb, u, ud, udd))
+        self._check(self._f["op_beam_newmark"](self.h, k, int(nsteps), abi.dptr(fb), abi.dptr(u), abi.dptr(ud),
+                                               abi.dptr(udd)))
+        return u, ud, udd
 
     def redistribute(self, f):
         """f_rdst = E M^-1 E^T f at the current weights, M = diag(E^T 1)."""

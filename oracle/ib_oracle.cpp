@@ -136,6 +136,136 @@ struct Body {
   std::vector<double> x, y;
 };
 
+/* Family 3, a deformable body.  The reference declares it and leaves every
+ * routine a stub (src/operators_struct.F90:104-106, 146-148, 376-379,
+ * 472-474: "evaluate udd"), so this is new physics in the torsional body's
+ * frame: a linear Euler-Bernoulli beam along the deck's straight line of
+ * npts = 2 ne + 1 points, ne Hermite-cubic elements (nodes at the even
+ * points), clamped at the first point; u holds each free node's deflection
+ * normal to the line and its slope.  Mass per unit length itheta, bending
+ * stiffness EI ktheta, damping C = ctheta M.  The coupling repeats the
+ * torsional body's with theta -> u and BR -> Phi = n H (H_p,i the shape
+ * functions at point p; Phi u moves the points along the normal n):
+ * Newmark average acceleration with K_eff = 4/dt^2 M + 2/dt C + K, the
+ * generalised force Phi^T f delta/dt, b_times2 = 2 delta^2/dt^2 Phi
+ * K_eff^-1 Phi^T, compute_rhs = delta Phi (ud0 + 2/dt (u0 - u) - Jr). */
+struct Beam {
+  int body = 0, np = 0, ne = 0, nd = 0;
+  double L = 0, h = 0, m = 0, EI = 0, c = 0;
+  double nx = 0, ny = 1;
+  std::vector<double> X0, Y0;
+  std::vector<double> M, K, C, Keff, Keffi, Minv; /* nd x nd, row-major */
+  std::vector<double> H;                          /* np x nd, row-major */
+  std::vector<double> G;                          /* np x np: H K_eff^-1 H^T */
+  std::vector<double> u, ud, udd, u0, ud0, udd0;
+};
+
+/* inverse of a dense n x n matrix (row-major) by Gauss-Jordan elimination
+ * with partial pivoting; false if singular */
+bool invert(int n, std::vector<double> a, std::vector<double>& inv) {
+  inv.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::fabs(a[(size_t)r * n + c]) > std::fabs(a[(size_t)piv * n + c])) piv = r;
+    if (a[(size_t)piv * n + c] == 0.0) return false;
+    if (piv != c)
+      for (int k = 0; k < n; ++k) {
+        std::swap(a[(size_t)piv * n + k], a[(size_t)c * n + k]);
+        std::swap(inv[(size_t)piv * n + k], inv[(size_t)c * n + k]);
+      }
+    const double d = 1.0 / a[(size_t)c * n + c];
+    for (int k = 0; k < n; ++k) {
+      a[(size_t)c * n + k] *= d;
+      inv[(size_t)c * n + k] *= d;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const double fct = a[(size_t)r * n + c];
+      if (fct == 0.0) continue;
+      for (int k = 0; k < n; ++k) {
+        a[(size_t)r * n + k] -= fct * a[(size_t)c * n + k];
+        inv[(size_t)r * n + k] -= fct * inv[(size_t)c * n + k];
+      }
+    }
+  }
+  return true;
+}
+
+/* the beam's matrices and shape functions for time step dt */
+bool beam_build(Beam& be, const Body& b, double dt) {
+  be.np = b.npts;
+  be.ne = (b.npts - 1) / 2;
+  be.nd = 2 * be.ne;
+  const double lx = b.x[b.npts - 1] - b.x[0], ly = b.y[b.npts - 1] - b.y[0];
+  be.L = std::sqrt(lx * lx + ly * ly);
+  be.h = be.L / be.ne;
+  be.nx = -ly / be.L;
+  be.ny = lx / be.L;
+  be.m = b.itheta;
+  be.EI = b.ktheta;
+  be.c = b.ctheta;
+  be.X0 = b.x;
+  be.Y0 = b.y;
+  const int nd = be.nd;
+  const double h = be.h;
+  be.M.assign((size_t)nd * nd, 0.0);
+  be.K.assign((size_t)nd * nd, 0.0);
+  const double ke[4][4] = {{12, 6 * h, -12, 6 * h},
+                           {6 * h, 4 * h * h, -6 * h, 2 * h * h},
+                           {-12, -6 * h, 12, -6 * h},
+                           {6 * h, 2 * h * h, -6 * h, 4 * h * h}};
+  const double me[4][4] = {{156, 22 * h, 54, -13 * h},
+                           {22 * h, 4 * h * h, 13 * h, -3 * h * h},
+                           {54, 13 * h, 156, -22 * h},
+                           {-13 * h, -3 * h * h, -22 * h, 4 * h * h}};
+  const double kf = be.EI / (h * h * h), mf = be.m * h / 420.0;
+  /* global index of local dof a of element e (node 0 clamped: -1) */
+  auto gdof = [](int e, int a) { const int node = e + a / 2; return node == 0 ? -1 : 2 * (node - 1) + a % 2; };
+  for (int e = 0; e < be.ne; ++e)
+    for (int a = 0; a < 4; ++a)
+      for (int bb = 0; bb < 4; ++bb) {
+        const int ga = gdof(e, a), gb = gdof(e, bb);
+        if (ga < 0 || gb < 0) continue;
+        be.K[(size_t)ga * nd + gb] += kf * ke[a][bb];
+        be.M[(size_t)ga * nd + gb] += mf * me[a][bb];
+      }
+  be.C.assign((size_t)nd * nd, 0.0);
+  for (size_t i = 0; i < be.C.size(); ++i) be.C[i] = be.c * be.M[i];
+  be.Keff.assign((size_t)nd * nd, 0.0);
+  for (size_t i = 0; i < be.Keff.size(); ++i) be.Keff[i] = 4.0 / (dt * dt) * be.M[i] + 2.0 / dt * be.C[i] + be.K[i];
+  if (!invert(nd, be.Keff, be.Keffi) || !invert(nd, be.M, be.Minv)) return false;
+  be.H.assign((size_t)be.np * nd, 0.0);
+  for (int p = 0; p < be.np; ++p) {
+    const int e = std::min(p / 2, be.ne - 1);
+    const double xi = 0.5 * (double)(p - 2 * e);
+    const double N[4] = {1 - 3 * xi * xi + 2 * xi * xi * xi, h * (xi - 2 * xi * xi + xi * xi * xi),
+                         3 * xi * xi - 2 * xi * xi * xi, h * (-xi * xi + xi * xi * xi)};
+    for (int a = 0; a < 4; ++a) {
+      const int ga = gdof(e, a);
+      if (ga >= 0) be.H[(size_t)p * nd + ga] += N[a];
+    }
+  }
+  be.G.assign((size_t)be.np * be.np, 0.0);
+  std::vector<double> t(nd);
+  for (int q = 0; q < be.np; ++q) {
+    for (int i = 0; i < nd; ++i) {
+      double acc = 0.0;
+      for (int k = 0; k < nd; ++k) acc += be.Keffi[(size_t)i * nd + k] * be.H[(size_t)q * nd + k];
+      t[i] = acc;
+    }
+    for (int p = 0; p < be.np; ++p) {
+      double acc = 0.0;
+      for (int i = 0; i < nd; ++i) acc += be.H[(size_t)p * nd + i] * t[i];
+      be.G[(size_t)p * be.np + q] = acc;
+    }
+  }
+  be.u.assign(nd, 0.0);
+  be.ud = be.u; be.udd = be.u; be.u0 = be.u; be.ud0 = be.u; be.udd0 = be.u;
+  return true;
+}
+
 }  // namespace
 
 struct orc_handle {
@@ -160,6 +290,7 @@ struct orc_handle {
   std::vector<double> xb, yb, f, f_rdst;
   std::vector<int> tors; /* torsional bodies in deck order */
   std::vector<double> theta, thetad, thetadd, theta0, thetad0, thetadd0, dtheta, Mt, Jt, Ct, Ftint;
+  std::vector<Beam> beams; /* family-3 bodies in deck order */
   /* regularisation (src/grid.F90:36-40) */
   int support2 = 3;
   std::vector<double> weight; /* 49 per force entry */
@@ -474,6 +605,84 @@ struct orc_handle {
         y[off + j - 1] = acc * sc;
       }
     }
+    /* family 3: 2 delta^2/dt^2 Phi K_eff^-1 Phi^T, Phi = n H */
+    const double cb = 2.0 * delta * delta / (dt * dt);
+    for (const Beam& be : beams) {
+      const int off = nbb_ind[be.body], np = be.np;
+      for (int j = 0; j < 2 * np; ++j) {
+        const int pj = j % np;
+        const double nj = j < np ? be.nx : be.ny;
+        double acc = 0.0;
+        for (int k = 0; k < 2 * np; ++k) {
+          const int pk = k % np;
+          const double nk = k < np ? be.nx : be.ny;
+          acc += nj * nk * be.G[(size_t)pj * np + pk] * x[off + k];
+        }
+        y[off + j] = cb * acc;
+      }
+    }
+  }
+
+  /* family 3: the f-independent part of the Newmark right side,
+   * M (4/dt^2 u0 + 4/dt ud0 + udd0) + C (ud0 + 2/dt u0) - K_eff u */
+  std::vector<double> beam_z(const Beam& be) const {
+    const int nd = be.nd;
+    std::vector<double> z(nd);
+    for (int i = 0; i < nd; ++i) {
+      double acc = 0.0;
+      for (int k = 0; k < nd; ++k) {
+        const size_t ik = (size_t)i * nd + k;
+        acc += be.M[ik] * (4.0 / (dt * dt) * be.u0[k] + 4.0 / dt * be.ud0[k] + be.udd0[k]) +
+               be.C[ik] * (be.ud0[k] + 2.0 / dt * be.u0[k]) - be.Keff[ik] * be.u[k];
+      }
+      z[i] = acc;
+    }
+    return z;
+  }
+  /* family 3: the generalised force delta/dt Phi^T f of the beam's entries
+   * fb (x forces, then y forces) */
+  std::vector<double> beam_q(const Beam& be, const double* fb) const {
+    std::vector<double> q(be.nd, 0.0);
+    for (int p = 0; p < be.np; ++p) {
+      const double fn = be.nx * fb[p] + be.ny * fb[be.np + p];
+      for (int i = 0; i < be.nd; ++i) q[i] += be.H[(size_t)p * be.nd + i] * fn;
+    }
+    for (double& v : q) v *= delta / dt;
+    return q;
+  }
+  static std::vector<double> matvec(int n, const std::vector<double>& A, const std::vector<double>& x) {
+    std::vector<double> y(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+      double acc = 0.0;
+      for (int k = 0; k < n; ++k) acc += A[(size_t)i * n + k] * x[k];
+      y[i] = acc;
+    }
+    return y;
+  }
+  /* family 3: point positions X0 + n (H u) on both of each point's entries */
+  void beam_positions(const Beam& be) {
+    const int off = nbb_ind[be.body];
+    for (int p = 0; p < be.np; ++p) {
+      double w = 0.0;
+      for (int i = 0; i < be.nd; ++i) w += be.H[(size_t)p * be.nd + i] * be.u[i];
+      const double X = be.X0[p] + be.nx * w, Y = be.Y0[p] + be.ny * w;
+      xb[off + p] = X; xb[off + be.np + p] = X;
+      yb[off + p] = Y; yb[off + be.np + p] = Y;
+    }
+  }
+  /* family 3: one Newmark update of u for the force fb; returns max |du| */
+  double beam_update(Beam& be, const double* fb) {
+    std::vector<double> z = beam_z(be), q = beam_q(be, fb);
+    for (int i = 0; i < be.nd; ++i) z[i] += q[i];
+    const std::vector<double> du = matvec(be.nd, be.Keffi, z);
+    double e = 0.0;
+    for (int i = 0; i < be.nd; ++i) {
+      e = std::max(e, std::fabs(du[i]));
+      be.u[i] += du[i];
+      be.ud[i] = -be.ud0[i] + 2.0 / dt * (be.u[i] - be.u0[i]);
+      be.udd[i] = -be.udd0[i] - 4.0 / dt * be.ud0[i] + 4.0 / (dt * dt) * (be.u[i] - be.u0[i]);
+    }
+    return e;
   }
 
   /* compute_rhs: src/operators_struct.F90:342-398 (gravity = dynamics = 0,
@@ -491,6 +700,19 @@ struct orc_handle {
                    (4.0 / (dt * dt) * Mt[t] + 2.0 / dt * Ct[t] + Jt[t]) * theta[t] + grav + dyn);
       double sc = delta * (rth - jr - thetad[t]);
       for (int j = 1; j <= 2 * b.npts; ++j) y[off + j - 1] = BRval(b, j, theta[t]) * sc;
+    }
+    /* family 3: delta Phi (ud0 + 2/dt (u0 - u) - 2/dt K_eff^-1 z) */
+    for (const Beam& be : beams) {
+      const int off = nbb_ind[be.body];
+      const std::vector<double> jr = matvec(be.nd, be.Keffi, beam_z(be));
+      std::vector<double> v(be.nd);
+      for (int i = 0; i < be.nd; ++i) v[i] = be.ud0[i] + 2.0 / dt * (be.u0[i] - be.u[i]) - 2.0 / dt * jr[i];
+      for (int p = 0; p < be.np; ++p) {
+        double w = 0.0;
+        for (int i = 0; i < be.nd; ++i) w += be.H[(size_t)p * be.nd + i] * v[i];
+        y[off + p] = delta * be.nx * w;
+        y[off + be.np + p] = delta * be.ny * w;
+      }
     }
   }
 
@@ -520,6 +742,10 @@ struct orc_handle {
         yb[off + j - 1] = b.ybf0 + (jj - 1) * b.deltas * s_;
       }
     }
+    for (Beam& be : beams) {
+      e = std::max(e, beam_update(be, f.data() + nbb_ind[be.body]));
+      beam_positions(be);
+    }
     return e;
   }
 
@@ -532,6 +758,13 @@ struct orc_handle {
       for (int j = 1; j <= 2 * b.npts; ++j) brf += BRval(b, j, theta[t]) * f[off + j - 1];
       brf = brf * delta / b.deltas / dt;
       thetadd[t] = 1 / Mt[t] * (-Ct[t] * thetad[t] - Ftint[t] + brf * b.deltas + 0.0 + 0.0);
+    }
+    /* family 3: udd = M^-1 (Phi^T f delta/dt - C ud - K u) */
+    for (Beam& be : beams) {
+      std::vector<double> r = beam_q(be, f.data() + nbb_ind[be.body]);
+      const std::vector<double> cu = matvec(be.nd, be.C, be.ud), ku = matvec(be.nd, be.K, be.u);
+      for (int i = 0; i < be.nd; ++i) r[i] -= cu[i] + ku[i];
+      be.udd = matvec(be.nd, be.Minv, r);
     }
   }
 
@@ -1285,6 +1518,11 @@ struct orc_handle {
     theta0 = theta;
     thetad0 = thetad;
     thetadd0 = thetadd;
+    for (Beam& be : beams) {
+      be.u0 = be.u;
+      be.ud0 = be.ud;
+      be.udd0 = be.udd;
+    }
     std::vector<double> rhsf(nf), yb_(nf);
     while (err_glob >= prm.fsi_tol && fsi_count <= prm.fsi_max_count) {
       ++fsi_count;
@@ -1408,9 +1646,9 @@ struct orc_handle {
       B.npts = bodies[b].npts;
       B.fam = bodies[b].fam;
       if (B.npts < 2) return fail(IB_E_INVAL, "a body needs at least 2 points");
-      if (B.fam != 1 && B.fam != 2) return fail(IB_E_UNSUPPORTED, "only body families 1 (rigid) and 2 (torsional) exist in the reference");
+      if (B.fam < 1 || B.fam > 3) return fail(IB_E_UNSUPPORTED, "body families are 1 (rigid), 2 (torsional), 3 (beam)");
       if (B.fam == 1 && seen_tors) return fail(IB_E_INVAL, "rigid bodies must precede torsional bodies (src/grid.F90:1226)");
-      if (B.fam == 2) seen_tors = true;
+      if (B.fam >= 2) seen_tors = true;
       B.gamma0 = bodies[b].gamma0; B.alpha0 = bodies[b].alpha0;
       B.itheta = bodies[b].itheta; B.ktheta = bodies[b].ktheta; B.ctheta = bodies[b].ctheta;
       B.x.assign(bodies[b].x, bodies[b].x + B.npts);
@@ -1419,6 +1657,16 @@ struct orc_handle {
       B.xbf0 = B.x[0]; B.ybf0 = B.y[0];
       if (B.fam == 2 && B.itheta <= 0) return fail(IB_E_INVAL, "torsional body needs itheta > 0");
       if (B.fam == 2) tors.push_back(b);
+      if (B.fam == 3) {
+        if (B.npts < 3 || B.npts % 2 == 0) return fail(IB_E_INVAL, "a beam needs an odd number of points >= 3");
+        if (!(B.itheta > 0) || !(B.ktheta > 0)) return fail(IB_E_INVAL, "a beam needs mass (itheta) and stiffness (ktheta) > 0");
+        if (prm.num_stat || prm.sub_domain)
+          return fail(IB_E_UNSUPPORTED, "beams (family 3) need the GMRES path without num_stat or sub-domains");
+        Beam be;
+        be.body = b;
+        if (!beam_build(be, B, dt)) return fail(IB_E_INVAL, "beam matrices are singular");
+        beams.push_back(be);
+      }
       bdy.push_back(B);
       nb_ind.push_back(B.npts);
       nbb_ind.push_back(nbb_ind.back() + 2 * B.npts);
@@ -1614,6 +1862,19 @@ int orc_get_body(orc_handle* h, double* f, double* f_rdst, double* xb, double* y
   return IB_OK;
 }
 
+/* family 3 (no reference counterpart): beam k's state u, ud, udd (2 ne
+ * values each: per free node the normal deflection and the slope); *nd
+ * gets 2 ne.  Any pointer may be NULL. */
+int orc_get_beam(orc_handle* h, int32_t k, double* u, double* ud, double* udd, int32_t* nd) {
+  if (k < 0 || k >= (int)h->beams.size()) return IB_E_INVAL;
+  const Beam& be = h->beams[k];
+  if (nd) *nd = be.nd;
+  if (u) std::memcpy(u, be.u.data(), sizeof(double) * be.nd);
+  if (ud) std::memcpy(ud, be.ud.data(), sizeof(double) * be.nd);
+  if (udd) std::memcpy(udd, be.udd.data(), sizeof(double) * be.nd);
+  return IB_OK;
+}
+
 /* calculate_pressure (src/operators_pressure.F90:110-437) at the current
  * state: mgridlev blocks of m*n pressure coefficients (write_pressure) */
 int orc_pressure(orc_handle* h, double* p) {
@@ -1764,6 +2025,41 @@ int orc_op_redistribute(orc_handle* h, const double* f, double* f_rdst) {
   h->redistribute();
   std::memcpy(f_rdst, h->f_rdst.data(), sizeof(double) * h->nf);
   h->f = keep;
+  return IB_OK;
+}
+
+/* family 3, test operators: beam k's mass and stiffness matrices (nd x nd,
+ * row-major) and shape functions H (np x nd: point p's normal deflection
+ * per unit of each unknown) */
+int orc_op_beam_matrices(orc_handle* h, int32_t k, double* M, double* K, double* H) {
+  if (k < 0 || k >= (int)h->beams.size()) return IB_E_INVAL;
+  const Beam& be = h->beams[k];
+  if (M) std::memcpy(M, be.M.data(), sizeof(double) * be.M.size());
+  if (K) std::memcpy(K, be.K.data(), sizeof(double) * be.K.size());
+  if (H) std::memcpy(H, be.H.data(), sizeof(double) * be.H.size());
+  return IB_OK;
+}
+
+/* family 3, test operator: nsteps steps of beam k alone under the fixed
+ * point forces fb (x forces, then y forces, 2 np values), from and back
+ * into (u, ud, udd): the step's Newmark update with each step's start state
+ * as (u0, ud0, udd0); the handle's own beam state is left as it was */
+int orc_op_beam_newmark(orc_handle* h, int32_t k, int32_t nsteps, const double* fb, double* u, double* ud,
+                        double* udd) {
+  if (k < 0 || k >= (int)h->beams.size() || nsteps < 0) return IB_E_INVAL;
+  Beam be = h->beams[k];
+  be.u.assign(u, u + be.nd);
+  be.ud.assign(ud, ud + be.nd);
+  be.udd.assign(udd, udd + be.nd);
+  for (int n = 0; n < nsteps; ++n) {
+    be.u0 = be.u;
+    be.ud0 = be.ud;
+    be.udd0 = be.udd;
+    h->beam_update(be, fb);
+  }
+  std::memcpy(u, be.u.data(), sizeof(double) * be.nd);
+  std::memcpy(ud, be.ud.data(), sizeof(double) * be.nd);
+  std::memcpy(udd, be.udd.data(), sizeof(double) * be.nd);
   return IB_OK;
 }
 

@@ -38,6 +38,10 @@ int orc_pressure(orc_handle* h, double* p);
 int orc_get_body(orc_handle* h, double* f, double* f_rdst, double* xb, double* yb,
                  double* theta, double* thetad, double* thetadd);
 int orc_get_forces(orc_handle* h, int32_t which, double* fx, double* fy);
+int orc_get_beam(orc_handle* h, int32_t k, double* u, double* ud, double* udd, int32_t* nd);
+int orc_op_beam_matrices(orc_handle* h, int32_t k, double* M, double* K, double* H);
+int orc_op_beam_newmark(orc_handle* h, int32_t k, int32_t nsteps, const double* fb, double* u, double* ud,
+                        double* udd);
 int orc_op_dst2d(orc_handle* h, int32_t batch, const double* in, double* out);
 int orc_op_ctci(orc_handle* h, const double* omega, double* s);
 int orc_op_ainv(orc_handle* h, const double* in, double* out);

@@ -55,6 +55,32 @@ def test_driver_matches_oracle(tmp_path):
     assert rel(var["f"], o.body()["f"]) < 1e-10
 
 
+def test_driver_runs_a_beam_deck(tmp_path):
+    """A family-3 body deck (the flexible plate as a beam: the five
+    parameter lines of family 2, read as pitch, alpha0, mass per unit
+    length, EI, damping) through the driver: forces against the oracle on
+    the deck as the driver read it back."""
+    from oracle import binding as ob
+    p, bodies = decks.tiny_case(m=40, n=32, mgridlev=2, tol=1e-12, beam=True)
+    (tmp_path / "input").mkdir()
+    decks.write_ib_inp(tmp_path / "input" / "ib.inp", p, istop=3, isave=3)
+    for i, b in enumerate(bodies):
+        decks.write_body(tmp_path / "input" / f"body.{i + 1:03d}.inp", b)
+    env = dict(os.environ, IB_GMRES_RESTART="80")
+    r = subprocess.run([DRIVER], cwd=tmp_path, env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    p2, b2, _ = decks.read_case(str(tmp_path / "input"))
+    assert b2[1].fam == 3 and b2[1].ktheta == pytest.approx(bodies[1].ktheta)
+    p2.gmres_restart = 80
+    o = ob.OracleSim(p2, b2)
+    got = np.loadtxt(tmp_path / "output" / "force01_00.dat")
+    for k in range(3):
+        info = o.advance()
+        assert info.status == 0
+        fx, fy = o.forces()
+        assert rel(got[k, 1:], np.concatenate([fx, fy])) < 1e-8, k
+
+
 def test_driver_restart_matches_continuous_run(tmp_path):
     """istart > 0: the driver reads output/data00_<istart>.var (read_variables)
     and the resumed steps reproduce the uninterrupted run's."""
