@@ -500,6 +500,60 @@ def copy_calibration(mb):
     hip.hipFree(b)
 
 
+def run_dst_compare(args):
+    """The two DST forms on one grid (one level, no bodies): a handle with
+    the Stockham form and one with the FP64 matrix-core form (IB_DST_FORM),
+    then --dst-reps spectral solves ctci (row pass, scaled column pass, row
+    pass) through each, host copies included (the kernels are what a trace
+    measures).  A third handle reports the autotuner's choice and its
+    per-field times.  With --pmc-copy-mb the calibration copy separates the
+    create-time timing from the measured passes."""
+    import numpy as np
+    from ib_parallel_amd import GpuSim, abi
+    m, n = (int(v) for v in args.dst_compare.lower().split("x"))
+    p = abi.make_params(m, n, 1, 0.01, 100.0, 1.0, 0.5, 0.5)
+    old = os.environ.get("IB_DST_FORM")
+    sims = {}
+    try:
+        for form in ("auto", "fft", "dense"):
+            # a fixed configuration keeps kernel names alike across profiled processes
+            os.environ["IB_DST_FORM"] = f"dense:{args.dst_dense_cfg}" if form == "dense" and args.dst_dense_cfg >= 0 \
+                else form
+            sims[form] = GpuSim(p, [])
+    finally:
+        if old is None:
+            os.environ.pop("IB_DST_FORM", None)
+        else:
+            os.environ["IB_DST_FORM"] = old
+    forms, us = sims["auto"].dst_form()
+    dforms = sims["dense"].dst_form()[0]
+    sims.pop("auto").close()
+    w = np.random.default_rng(0).standard_normal(sims["fft"].P)
+    outs = {f: s.ctci(w) for f, s in sims.items()}  # warm-up
+    if args.pmc_copy_mb > 0:
+        copy_calibration(args.pmc_copy_mb)
+    wall = {}
+    for f, s in sims.items():
+        device_sync()
+        t0 = time.perf_counter()
+        for _ in range(args.dst_reps):
+            outs[f] = s.ctci(w)
+        wall[f] = (time.perf_counter() - t0) / args.dst_reps * 1e3
+    d = outs["dense"] - outs["fft"]
+    names = ["row step", "column step", "row batched", "column batched"]
+    res = {"grid": f"{m}x{n}", "transform_lengths": [m - 1, n - 1], "reps": args.dst_reps,
+           "auto_forms": {k: ("stockham" if f == 0 else f"matrix-core cfg {f - 1}") for k, f in zip(names, forms)},
+           "us_per_field": {k: {"stockham": round(float(us[2 * i]), 3),
+                                "matrix_core": None if us[2 * i + 1] < 0 else round(float(us[2 * i + 1]), 3)}
+                            for i, k in enumerate(names)},
+           "dense_handle_cfgs": list(dforms),
+           "ctci_ms_with_host_copies": {k: round(v, 4) for k, v in wall.items()},
+           "dense_vs_stockham_rel": float(np.linalg.norm(d) / np.linalg.norm(outs["fft"]))}
+    for s in sims.values():
+        s.close()
+    return res
+
+
 def traffic_for(kernel_label):
     """Calibrated HBM bytes per launch of the kernel from the committed
     counter summary (profiles/pmc_traffic.json), or None."""
@@ -664,6 +718,11 @@ def main():
     ap.add_argument("--c4-batch-only", action="store_true", help="only the airfoil + flap batch measurement")
     ap.add_argument("--flex-steps", type=int, default=20,
                     help="timed steps of the flexible-plate (beam) workload (0: skip)")
+    ap.add_argument("--dst-compare", default=None, metavar="MxN",
+                    help="only the comparison of the two DST forms on an M x N grid (one JSON object)")
+    ap.add_argument("--dst-reps", type=int, default=50)
+    ap.add_argument("--dst-dense-cfg", type=int, default=-1,
+                    help="tile configuration of the matrix-core handle in --dst-compare (-1: timed)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the library has no multi-GPU path",
@@ -678,6 +737,9 @@ def main():
         return
     if args.c4_batch_only:
         print(json.dumps({"c4_batch": run_c4_batch(args)}), flush=True)
+        return
+    if args.dst_compare:
+        print(json.dumps({"dst_compare": run_dst_compare(args)}), flush=True)
         return
     if args.batch_only:
         print(json.dumps({"batched": run_batched(args)}), flush=True)

@@ -74,6 +74,16 @@ class GpuSim(Simulation):
         self._check(self._f["dst_tiles"](self.h, C.byref(rp), C.byref(cp), C.byref(src)))
         return rp.value, cp.value, src.value
 
+    def dst_form(self):
+        """(forms, us): forms of the DST passes for (row, column) x (time step,
+        batched assembly), 0 Stockham FFT, k > 0 the FP64 matrix-core form in
+        configuration k-1; us the timed microseconds per field,
+        us[4*batched + 2*column + matrix_core] (negative: not timed)."""
+        f = (C.c_int32 * 4)()
+        us = np.zeros(8)
+        self._check(self._f["dst_form"](self.h, f, us.ctypes.data_as(C.POINTER(C.c_double))))
+        return tuple(f), us
+
     def adjoint(self, nsteps, cx, cy, ckpt_every=None, omega_grad=True, position_grad=False, body_grad=False):
         """Advance nsteps steps and return (J, dJ/dRe, dJ/domega0) for the
         time-mean objective J = mean over the steps of sum_b cx[b] fx_b +

@@ -15,7 +15,7 @@ LIB = os.path.join(PKG, "libibgpu.so")
 FORTRAN = os.path.join(PKG, "fortran")
 DRIVER = os.path.join(PKG, "bin", "ib")
 
-SOURCES = ["ib_fft.hip", "ib_fluid.hip", "ib_body.hip", "ib_beam.hip", "ib_gmres.hip", "ib_pressure.hip", "ib_subdomain.hip",
+SOURCES = ["ib_fft.hip", "ib_dst_dense.hip", "ib_fluid.hip", "ib_body.hip", "ib_beam.hip", "ib_gmres.hip", "ib_pressure.hip", "ib_subdomain.hip",
            "ib_adjoint.hip", "ib_adjoint_host.hip", "ib_solver.hip", "ib_batch.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -524,7 +524,12 @@ struct ib_batch {
         if (need[i % nx] > ncu / (long)(xcc.empty() ? 8 : nx)) sims[i]->coop_lock = &coop_mu[i % nx];
     }
     direct = params[0].launch_mode == 1;
-    if (mode == 1) build_fused();
+    if (mode == 1) {
+      // the fused launches run the Stockham passes for every simulation; a
+      // handle's own passes follow, so that they round alike
+      for (ib_handle* h : sims) h->tiles.xform = h->tiles.yform = 0;
+      build_fused();
+    }
     return IB_OK;
   }
 

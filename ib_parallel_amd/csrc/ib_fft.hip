@@ -714,17 +714,29 @@ void dst_y(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& l, cons
 }
 
 // ------------------------------------------------------------------------
-// Tile shapes of the DST passes, picked by timing.  The row pairs per
-// workgroup of the row pass (rp) and the column pairs of the column pass
+// Form and tile shapes of the DST passes, picked by timing.  The row pairs
+// per workgroup of the row pass (rp) and the column pairs of the column pass
 // (cp) decide only how many transforms share a workgroup's LDS, never a
-// transform's arithmetic, so every choice gives the same bits.  Each
-// candidate that fits the LDS runs a few times on scratch fields between
-// HIP events; the fastest is kept for the process, per grid shape.
-// IB_DST_TILES=rp,cp forces a choice; IB_DST_TUNE=0 keeps the fixed rule
-// (rp 2 up to transform length 512, else 1; cp 1).
+// transform's arithmetic, so every choice gives the same bits.  The
+// matrix-core form (ib_dst_dense.hip) sums in another order: it is taken for
+// a pass class only when it is at least 10% faster, so that a near tie does
+// not flip between processes.  Each candidate that fits runs a few times on
+// scratch fields between HIP events; the fastest is kept for the process,
+// per grid shape.  Two classes are timed: one field per launch (the time
+// step) and a strided batch of 8 fields (the Schur assembly, whose launches
+// cover hundreds of columns).
+//
+// IB_DST_TILES=rp,cp forces the Stockham tiles; IB_DST_TUNE=0 keeps the
+// fixed rule (rp 2 up to transform length 512, else 1; cp 1); neither times
+// anything.  IB_DST_FORM=fft keeps the Stockham form, =dense takes the
+// matrix-core form for both classes (its fastest configuration, or the
+// (T, KS) = (1, 8) configuration when nothing is timed), =dense:k the matrix-core
+// form in configuration k without timing, =auto (the default) times.
 namespace {
 std::mutex g_tile_mu;
-std::map<std::array<long, 5>, DstTiles> g_tiles;
+std::map<std::array<long, 6>, DstTiles> g_tiles;
+constexpr int kDenseDefault = 1;
+constexpr int kTuneBatch = 8;
 
 template <class F>
 float time_launches(hipStream_t st, F&& launch) {
@@ -751,67 +763,148 @@ float time_launches(hipStream_t st, F&& launch) {
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  return best;
+  return best / 4.0f;
 }
+
+// 0 auto, 1 Stockham only, 2 matrix-core form, 3 matrix-core form in
+// configuration *cfg
+int form_rule(int* cfg) {
+  const char* f = std::getenv("IB_DST_FORM");
+  if (!f || !*f || std::strcmp(f, "auto") == 0) return 0;
+  if (std::strcmp(f, "fft") == 0) return 1;
+  if (std::strcmp(f, "dense") == 0) return 2;
+  int k = -1;
+  char tail = 0;
+  if (std::sscanf(f, "dense:%d%c", &k, &tail) == 1 && k >= 0 && k < kDenseCfgs) {
+    *cfg = k;
+    return 3;
+  }
+  throw HipError("IB_DST_FORM must be fft, dense, dense:k (0 <= k < 5) or auto");
+}
+
+struct Scratch {
+  double* p = nullptr;
+  ~Scratch() {
+    if (p) (void)hipFree(p);
+  }
+};
 }  // namespace
 
-DstTiles dst_tiles(hipStream_t st, const FFTPlan& px, const FFTPlan& py, const Geom& g, size_t max_lds) {
+DstTiles dst_tiles(hipStream_t st, const FFTPlan& px, const FFTPlan& py, const DenseMat* dx, const DenseMat* dy,
+                   double* tq, const Geom& g, size_t max_lds) {
   DstTiles t;
   t.rp = px.L <= 512 ? 2 : 1;
   t.cp = 1;
+  int forced_cfg = kDenseDefault;
+  const int rule = form_rule(&forced_cfg);
+  const bool dense_ok = dx && dy && dx->S && dy->S && tq;
+  if (rule >= 2 && !dense_ok) throw HipError("IB_DST_FORM=dense: no sine matrices");
+  const int fixed_form = rule >= 2 ? forced_cfg + 1 : 0;
   if (const char* f = std::getenv("IB_DST_TILES")) {
     int rp = 0, cp = 0;
     if (std::sscanf(f, "%d,%d", &rp, &cp) != 2 || rp < 1 || rp > 8 || cp < 1 || cp > 8)
       throw HipError("IB_DST_TILES must be rp,cp with 1 <= rp, cp <= 8");
     t.rp = rp;  // dst_x_entry / dst_y_entry halve a choice the LDS cannot hold
     t.cp = cp;
+    t.xform = t.yform = t.xform_b = t.yform_b = fixed_form;
     t.source = 2;
     return t;
   }
   if (const char* f = std::getenv("IB_DST_TUNE"))
-    if (std::strcmp(f, "0") == 0) return t;
-  const std::array<long, 5> key{g.m, g.n, px.L, py.L, (long)max_lds};
+    if (std::strcmp(f, "0") == 0) {
+      t.xform = t.yform = t.xform_b = t.yform_b = fixed_form;
+      return t;
+    }
+  if (rule == 3) {  // a forced configuration: the Stockham tiles keep the fixed rule
+    t.xform = t.yform = t.xform_b = t.yform_b = fixed_form;
+    t.source = 2;
+    return t;
+  }
+  const bool try_dense = dense_ok && rule != 1;
+  const std::array<long, 6> key{g.m, g.n, px.L, py.L, (long)max_lds, (long)(rule * 2 + (try_dense ? 1 : 0))};
   std::lock_guard<std::mutex> lock(g_tile_mu);
   auto it = g_tiles.find(key);
   if (it != g_tiles.end()) return it->second;
-  double* in = nullptr;
-  double* out = nullptr;
-  const size_t bytes = sizeof(double) * (size_t)g.P;
-  IB_HIP(hipMalloc(&in, bytes));
-  if (hipMalloc(&out, bytes) != hipSuccess) {
-    (void)hipFree(in);
-    throw HipError("DST tile tuning: out of device memory");
-  }
-  try {
-    IB_HIP(hipMemsetAsync(in, 0, bytes, st));
-    DstLaunch l;
-    l.in[0] = in;
-    l.out[0] = out;
-    float best = 1e30f;
-    for (int rp : {1, 2, 4}) {
-      if (dst_lds_bytes(px, rp) > max_lds) continue;
-      const float ms = time_launches(st, [&] { dst_x(st, px, X_PLAIN, l, g, rp, max_lds); });
-      if (ms < best) {
-        best = ms;
-        t.rp = rp;
-      }
+  const long P = g.P;
+  Scratch in, out;
+  const size_t bytes = sizeof(double) * (size_t)P * kTuneBatch;
+  IB_HIP(hipMalloc(&in.p, bytes));
+  IB_HIP(hipMalloc(&out.p, bytes));
+  IB_HIP(hipMemsetAsync(in.p, 0, bytes, st));
+  // one field per launch: the Stockham tiles, then the matrix-core configurations
+  DstLaunch lx;
+  lx.in[0] = in.p;
+  lx.out[0] = out.p;
+  DstLaunch ly = lx;
+  ly.tab = in.p;  // any table: the values do not change the time
+  ly.div = 1.0;
+  float best = 1e30f;
+  for (int rp : {1, 2, 4}) {
+    if (dst_lds_bytes(px, rp) > max_lds) continue;
+    const float ms = time_launches(st, [&] { dst_x(st, px, X_PLAIN, lx, g, rp, max_lds); });
+    if (ms < best) {
+      best = ms;
+      t.rp = rp;
     }
-    best = 1e30f;
-    for (int cp : {1, 2, 4}) {
-      if (dst_lds_bytes(py, cp) > max_lds) continue;
-      const float ms = time_launches(st, [&] { dst_y(st, py, Y_PLAIN, l, g, cp, max_lds); });
-      if (ms < best) {
-        best = ms;
-        t.cp = cp;
-      }
-    }
-  } catch (...) {
-    (void)hipFree(in);
-    (void)hipFree(out);
-    throw;
   }
-  (void)hipFree(in);
-  (void)hipFree(out);
+  t.us[0] = best * 1e3f;
+  best = 1e30f;
+  for (int cp : {1, 2, 4}) {
+    if (dst_lds_bytes(py, cp) > max_lds) continue;
+    const float ms = time_launches(st, [&] { dst_y(st, py, Y_SCALE2, ly, g, cp, max_lds); });
+    if (ms < best) {
+      best = ms;
+      t.cp = cp;
+    }
+  }
+  t.us[2] = best * 1e3f;
+  // a strided batch of kTuneBatch fields
+  DstLaunch bx = lx;
+  bx.batch = kTuneBatch;
+  bx.strided = true;
+  bx.s_in = bx.s_out = P;
+  DstLaunch by = bx;
+  by.tab = in.p;
+  by.div = 1.0;
+  t.us[4] = time_launches(st, [&] { dst_x(st, px, X_PLAIN, bx, g, t.rp, max_lds); }) * 1e3f / kTuneBatch;
+  t.us[6] = time_launches(st, [&] { dst_y(st, py, Y_SCALE2, by, g, t.cp, max_lds); }) * 1e3f / kTuneBatch;
+  if (try_dense) {
+    // a class is timed only if the matrix-core form could beat the measured
+    // Stockham time at the FP64 matrix peak (78.6 TFLOP/s; 2N flops per
+    // point and product, two products in the column pass); IB_DST_FORM=dense
+    // times every class
+    const double peak = 78.6e6;  // flop per microsecond
+    const double lb[4] = {2.0 * (g.m - 1) * P / peak, 4.0 * (g.n - 1) * P / peak,
+                          2.0 * (g.m - 1) * P / peak, 4.0 * (g.n - 1) * P / peak};
+    float best_us[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+    int cfg[4] = {kDenseDefault, kDenseDefault, kDenseDefault, kDenseDefault};
+    const DstLaunch* ls[4] = {&lx, &ly, &bx, &by};
+    for (int q = 0; q < 4; ++q) {
+      const float fft_us = t.us[2 * q];
+      if (rule != 2 && lb[q] >= fft_us) continue;
+      const int nf = q < 2 ? 1 : kTuneBatch;
+      for (int c = 0; c < kDenseCfgs; ++c) {
+        const float us = 1e3f / nf * time_launches(st, [&] {
+          if (q % 2 == 0) dst_dense_x(st, *dx, X_PLAIN, *ls[q], g, c);
+          else dst_dense_y(st, *dy, Y_SCALE2, *ls[q], g, c, tq, kMaxBatch);
+        });
+        if (us < best_us[q]) {
+          best_us[q] = us;
+          cfg[q] = c;
+        }
+      }
+      t.us[2 * q + 1] = best_us[q];
+    }
+    const auto pick = [&](int q) {
+      if (rule == 2) return cfg[q] + 1;
+      return (t.us[2 * q + 1] >= 0.0f && t.us[2 * q + 1] < 0.9f * t.us[2 * q]) ? cfg[q] + 1 : 0;
+    };
+    t.xform = pick(0);
+    t.yform = pick(1);
+    t.xform_b = pick(2);
+    t.yform_b = pick(3);
+  }
+  IB_HIP(hipStreamSynchronize(st));
   t.source = 1;
   g_tiles[key] = t;
   return t;

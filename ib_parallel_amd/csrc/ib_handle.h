@@ -234,7 +234,9 @@ struct ib_handle {
   int L = 0;
   double delta = 0, dt = 0, Re = 0, offx = 0, offy = 0, normalize = 0;
   HostFFTPlan px, py;
-  DstTiles tiles;  // tile shapes of the step's DST passes (dst_tiles)
+  DstTiles tiles;  // form and tile shapes of the DST passes (dst_tiles)
+  DenseMat dmx, dmy;  // sine matrices of the matrix-core DST form (while a class uses it)
+  DBuf<double> tq;    // kMaxBatch fields: the matrix-core column pass's intermediate
   DBuf<double> laminv, lam1, lam1i;
   std::vector<double> vfac, con1, con2, q0x;
   DBuf<double> omega, s, qx, qy, rhs_old;
@@ -387,15 +389,21 @@ struct ib_handle {
   // passes that assemble the Schur matrix
   static int tclass(int base, const DstLaunch& a) { return base + (a.strided && a.batch > kMaxBatch ? 2 : 0); }
   void xpass(int mode, const DstLaunch& a) {
-    KTimer& t = tim[tclass(0, a)];
+    const int cls = tclass(0, a);
+    KTimer& t = tim[cls];
     t.begin(st, pass_bytes(mode == X_SUBFROM ? 2 : (mode == X_POINTROT ? 0 : 1), 0, 1, a.batch));
-    dst_x(st, px.dev, mode, a, g, tiles.rp, max_lds);
+    const int form = cls == 0 ? tiles.xform : tiles.xform_b;
+    if (form > 0 && mode != X_POINTROT) dst_dense_x(st, dmx, mode, a, g, form - 1);
+    else dst_x(st, px.dev, mode, a, g, tiles.rp, max_lds);
     t.end(st);
   }
   void ypass(int mode, const DstLaunch& a) {
-    KTimer& t = tim[tclass(1, a)];
+    const int cls = tclass(1, a);
+    KTimer& t = tim[cls];
     t.begin(st, pass_bytes(mode == Y_PRED ? 2 : 1, mode == Y_PRED ? 2 : (mode == Y_SCALE2 ? 1 : 0), 1, a.batch));
-    dst_y(st, py.dev, mode, a, g, tiles.cp, max_lds);
+    const int form = cls == 1 ? tiles.yform : tiles.yform_b;
+    if (form > 0) dst_dense_y(st, dmy, mode, a, g, form - 1, tq, (int)(tq.n / (size_t)g.P));
+    else dst_y(st, py.dev, mode, a, g, tiles.cp, max_lds);
     t.end(st);
   }
 
@@ -1684,7 +1692,17 @@ struct ib_handle {
     py.init(n - 1);
     if (dst_lds_bytes(px.dev, 1) > max_lds || dst_lds_bytes(py.dev, 2) > max_lds)
       return fail(IB_E_UNSUPPORTED, "grid too large for the single-workgroup DST (LDS)");
-    tiles = dst_tiles(st, px.dev, py.dev, g, max_lds);
+    // the matrix-core form is timed beside the Stockham passes; its
+    // matrices and intermediate stay only if a pass class takes it
+    dmx.init(m - 1);
+    dmy.init(n - 1);
+    tq.alloc((size_t)kMaxBatch * g.P);
+    tiles = dst_tiles(st, px.dev, py.dev, &dmx, &dmy, tq, g, max_lds);
+    if (tiles.xform == 0 && tiles.xform_b == 0) dmx.release();
+    if (tiles.yform == 0 && tiles.yform_b == 0) {
+      dmy.release();
+      tq.release();
+    }
 
     // setup_fft tables: src/myfft.F90:262-289
     const double pi = 4.0 * std::atan(1.0);
@@ -1876,6 +1894,8 @@ struct ib_handle {
     csr.release();
     px.release();
     py.release();
+    dmx.release();
+    dmy.release();
     pxp.release();
     pyp.release();
     if (gst) (void)hipFree(gst);

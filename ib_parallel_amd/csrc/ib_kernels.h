@@ -85,13 +85,43 @@ void dst_x(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& a, cons
 // y pass over columns of length n-1 (m-1 columns); cp = column pairs per workgroup
 void dst_y(hipStream_t st, const FFTPlan& pl, int mode, const DstLaunch& a, const Geom& g, int cp,
            size_t max_lds);
-// Tile shapes of the DST passes (row pairs per workgroup of the row pass,
-// column pairs of the column pass): timed per grid shape once per process.
-// source: 0 fixed rule, 1 timed, 2 forced by IB_DST_TILES.
+// The DST-I matrix S[t][k] = 2 sin(pi (t+1)(k+1)/(N+1)) in device memory,
+// leading dimension ld (N rounded up to 64), zero past N.
+struct DenseMat {
+  double* S = nullptr;
+  int N = 0, ld = 0;
+  void init(int N);
+  void release();
+};
+// tile shapes of the matrix-core passes: (T, KS) = (1,4), (1,8), (1,16),
+// (2,2), (2,4) -- T x T blocks of 16 x 16 per workgroup, the sum split over
+// KS groups of T x T wavefronts
+constexpr int kDenseCfgs = 5;
+// The DST passes as FP64 matrix-core products (every row mode but
+// X_POINTROT, every column mode).  Y_SCALE2 and Y_PRED run as two products
+// through tq, which holds tq_fields arrays of (m-1)(n-1) values.
+void dst_dense_x(hipStream_t st, const DenseMat& S, int mode, const DstLaunch& a, const Geom& g, int cfg);
+void dst_dense_y(hipStream_t st, const DenseMat& S, int mode, const DstLaunch& a, const Geom& g, int cfg,
+                 double* tq, int tq_fields);
+
+// Form and tile shapes of the DST passes, timed per grid shape once per
+// process.  Stockham form: row pairs per workgroup of the row pass (rp),
+// column pairs of the column pass (cp).  Forms: 0 Stockham, k > 0 the
+// matrix-core form with configuration k-1; xform/yform for the passes of
+// the time step (one field per launch), xform_b/yform_b for the batched
+// passes of the Schur assembly.  us: best measured microseconds per field
+// of {Stockham, matrix-core} x {row, column} x {step, batched} (index
+// 4*batched + 2*column + matrix-core), negative where not timed.
+// source: 0 fixed rule, 1 timed, 2 forced by IB_DST_TILES or IB_DST_FORM.
 struct DstTiles {
   int rp = 1, cp = 1, source = 0;
+  int xform = 0, yform = 0, xform_b = 0, yform_b = 0;
+  float us[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
 };
-DstTiles dst_tiles(hipStream_t st, const FFTPlan& px, const FFTPlan& py, const Geom& g, size_t max_lds);
+// dx/dy (may be null: Stockham only) are the matrices of the row and column
+// lengths; tq holds kMaxBatch arrays of (m-1)(n-1) values for the timing.
+DstTiles dst_tiles(hipStream_t st, const FFTPlan& px, const FFTPlan& py, const DenseMat* dx, const DenseMat* dy,
+                   double* tq, const Geom& g, size_t max_lds);
 // LDS bytes a launch needs
 size_t dst_lds_bytes(const FFTPlan& pl, int nfft);
 // The arguments and grid dst_x / dst_y would launch with, as a table entry,

@@ -475,6 +475,17 @@ int ib_dst_tiles(const ib_handle* h, int32_t* rp, int32_t* cp, int32_t* source) 
   return IB_OK;
 }
 
+int ib_dst_form(const ib_handle* h, int32_t* forms, double* us) {
+  if (!h || !forms) return IB_E_INVAL;
+  forms[0] = h->tiles.xform;
+  forms[1] = h->tiles.yform;
+  forms[2] = h->tiles.xform_b;
+  forms[3] = h->tiles.yform_b;
+  if (us)
+    for (int q = 0; q < 8; ++q) us[q] = h->tiles.us[q];
+  return IB_OK;
+}
+
 int ib_kernel_timing(ib_handle* h, int32_t kclass, int32_t enable) {
   if (!h || kclass < 0 || kclass > 4) return IB_E_INVAL;
   API_TRY

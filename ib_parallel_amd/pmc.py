@@ -8,10 +8,11 @@ Usage (on a machine with the GPU, each counter set in a run of its own):
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT/fetch -o run -- $B
     rocprofv3 --pmc WRITE_SIZE --output-format csv -d OUT/write -o run -- $B
     rocprofv3 --pmc SQ_INSTS_VALU_ADD_F64 SQ_INSTS_VALU_MUL_F64 SQ_INSTS_VALU_FMA_F64 \\
-        SQ_INSTS_VALU_TRANS_F64 --output-format csv -d OUT/f64 -o run -- $B
+        SQ_INSTS_VALU_TRANS_F64 SQ_INSTS_VALU_MFMA_MOPS_F64 --output-format csv -d OUT/f64 -o run -- $B
     python -m ib_parallel_amd.pmc traffic OUT/fetch OUT/write --copy-mb 256 --f64 OUT/f64 \\
         --json profiles/pmc_traffic.json
     python -m ib_parallel_amd.pmc stats OUT/trace        # a --kernel-trace --stats run
+    python -m ib_parallel_amd.pmc rates OUT/trace profiles/pmc_traffic.json   # GB/s and GFLOP/s
 
 FETCH_SIZE and WRITE_SIZE are read in KiB per dispatch.  The device-to-device
 copy of known size that --pmc-copy-mb adds to the run calibrates them: the
@@ -20,7 +21,8 @@ counter of the copy).  bench.py issues that copy right after its warm-up
 steps, so only the dispatches after the copy (the timed steps; the one-time
 set-up comes before it) are averaged.  FP64 operations per dispatch = 64 *
 (ADD + MUL + TRANS + 2 FMA) wavefront instructions (all 64 lanes counted,
-masked ones included).
+masked ones included) + 512 * SQ_INSTS_VALU_MFMA_MOPS_F64 (the matrix-core
+FP64 flops, counted in units of 512) when that counter was collected.
 """
 import argparse
 import csv
@@ -97,7 +99,7 @@ def traffic(args):
     sf = copy_kib / cf if cf > 0 else None
     sw = copy_kib / cw if cw > 0 else None
     f64 = read_counters(args.f64, {"SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_FMA_F64",
-                                   "SQ_INSTS_VALU_TRANS_F64"}) if args.f64 else {}
+                                   "SQ_INSTS_VALU_TRANS_F64", "SQ_INSTS_VALU_MFMA_MOPS_F64"}) if args.f64 else {}
     # the marker dispatch of each run: its calibration copy; later dispatches are the timed steps
     f64_mark = copy_marker(f64) if f64 else None
     out = {"copy": {"kernel": ck, "bytes": copy_kib * 1024.0, "fetch_kib": cf, "write_kib": cw,
@@ -121,7 +123,8 @@ def traffic(args):
                 for d, v in f64.get(k, {}).items():
                     if f64_mark is None or d > f64_mark:
                         ops.append(64.0 * (v["SQ_INSTS_VALU_ADD_F64"] + v["SQ_INSTS_VALU_MUL_F64"]
-                                           + v["SQ_INSTS_VALU_TRANS_F64"] + 2.0 * v["SQ_INSTS_VALU_FMA_F64"]))
+                                           + v["SQ_INSTS_VALU_TRANS_F64"] + 2.0 * v["SQ_INSTS_VALU_FMA_F64"])
+                                   + 512.0 * v["SQ_INSTS_VALU_MFMA_MOPS_F64"])
             ent["fp64_ops_per_launch"] = _mean(ops)
         out["kernels"][sk] = ent
     os.makedirs(os.path.dirname(os.path.abspath(args.json)), exist_ok=True)
@@ -136,26 +139,53 @@ def traffic(args):
               f"{v['bytes_per_launch'] / 1e6:.3f} | {'' if fo is None else f'{fo / 1e6:.3f}'} |")
 
 
+def after_copy_durations(trace_dir):
+    """{kernel name: [ns, ...]} of the dispatches after the calibration copy
+    in a --kernel-trace run."""
+    files = glob.glob(os.path.join(trace_dir, "**", "*kernel_trace.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no kernel_trace.csv under {trace_dir}")
+    disp = []
+    with open(files[0]) as fh:
+        for r in csv.DictReader(fh):
+            disp.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["End_Timestamp"]) - float(r["Start_Timestamp"])))
+    copies = [d for d in disp if "copy" in d[1].lower()]
+    mark = max(copies, key=lambda d: d[2])[0] if copies else -1
+    agg = defaultdict(list)
+    for d, name, ns in disp:
+        if d > mark and "copy" not in name.lower():
+            agg[name].append(ns)
+    return agg
+
+
+def rates(args):
+    """Markdown table joining a --kernel-trace run (durations of the
+    dispatches after the calibration copy) with a traffic JSON (calibrated
+    HBM bytes and FP64 operations per launch): achieved HBM GB/s and FP64
+    GFLOP/s per kernel."""
+    with open(args.traffic_json) as fh:
+        tr = json.load(fh)["kernels"]
+    agg = defaultdict(list)
+    for name, v in after_copy_durations(args.trace_dir).items():
+        agg[short(name)] += v
+    print("| kernel | calls | average us | HBM MB/launch | HBM GB/s | FP64 Mflop/launch | FP64 GFLOP/s |")
+    print("|---|---|---|---|---|---|---|")
+    for k, v in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[: args.top]:
+        us = sum(v) / len(v) / 1e3
+        t = tr.get(k, {})
+        b, f = t.get("bytes_per_launch"), t.get("fp64_ops_per_launch")
+        bs = " | " if b is None else f"{b / 1e6:.3f} | {b / us / 1e3:.0f}"
+        fs = " | " if f is None else f"{f / 1e6:.2f} | {f / us / 1e3:.0f}"
+        print(f"| {k} | {len(v)} | {us:.2f} | {bs} | {fs} |")
+
+
 def stats(args):
     """Markdown table of a rocprofv3 --kernel-trace --stats run: the
     whole-process kernel_stats.csv, or with --after-copy the kernel trace
     restricted to the dispatches after the calibration copy (the timed
     steps of a bench.py run with --pmc-copy-mb)."""
     if args.after_copy:
-        files = glob.glob(os.path.join(args.trace_dir, "**", "*kernel_trace.csv"), recursive=True)
-        if not files:
-            raise SystemExit(f"no kernel_trace.csv under {args.trace_dir}")
-        disp = []
-        with open(files[0]) as fh:
-            for r in csv.DictReader(fh):
-                disp.append((int(r["Dispatch_Id"]), r["Kernel_Name"],
-                             float(r["End_Timestamp"]) - float(r["Start_Timestamp"])))
-        copies = [d for d in disp if "copy" in d[1].lower()]
-        mark = max(copies, key=lambda d: d[2])[0] if copies else -1
-        agg = defaultdict(list)
-        for d, name, ns in disp:
-            if d > mark and "copy" not in name.lower():
-                agg[name].append(ns)
+        agg = after_copy_durations(args.trace_dir)
         rows = [{"Name": k, "Calls": len(v), "TotalDurationNs": sum(v), "AverageNs": sum(v) / len(v),
                  "MinNs": min(v), "MaxNs": max(v)} for k, v in agg.items()]
     else:
@@ -186,8 +216,12 @@ def main():
     s.add_argument("trace_dir")
     s.add_argument("--top", type=int, default=25)
     s.add_argument("--after-copy", action="store_true", help="only the dispatches after the calibration copy")
+    r = sub.add_parser("rates")
+    r.add_argument("trace_dir")
+    r.add_argument("traffic_json")
+    r.add_argument("--top", type=int, default=25)
     args = ap.parse_args()
-    traffic(args) if args.cmd == "traffic" else stats(args)
+    {"traffic": traffic, "stats": stats, "rates": rates}[args.cmd](args)
 
 
 if __name__ == "__main__":

@@ -230,6 +230,17 @@ int ib_schur_info(ib_handle* h, int32_t* mode, int32_t* nfaces, int64_t* green_b
  * tiles never change results, only speed; a choice the LDS cannot hold is
  * halved at launch. */
 int ib_dst_tiles(const ib_handle* h, int32_t* rp, int32_t* cp, int32_t* source);
+/* the form of the handle's DST passes (no reference counterpart): forms[4]
+ * for {row, column} passes x {the time step's one-field launches, the Schur
+ * assembly's batched launches}, 0 the Stockham FFT form, k > 0 the FP64
+ * matrix-core form (the DST-I matrix as a GEMM operand) in tile
+ * configuration k-1 (of 5).  us[8] (may be null): the microseconds per field the
+ * choice was timed at, us[4*batched + 2*column + matrix_core], negative
+ * where not timed.  Chosen at ib_create per grid shape and process: the
+ * matrix-core form only when at least 10% faster; IB_DST_FORM=fft|dense
+ * forces a form.  The two forms sum in different orders (results agree to
+ * rounding, not bit for bit). */
+int ib_dst_form(const ib_handle* h, int32_t* forms, double* us);
 int ib_kernel_timing_read(ib_handle* h, int32_t kclass, int64_t* launches, double* total_ms,
                           double* total_bytes);
 /* the GMRES solve in use on a dense Schur matrix (no reference counterpart):

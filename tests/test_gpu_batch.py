@@ -27,8 +27,10 @@ def same(a_list, b_list):
     return all(a.shape == b.shape and np.array_equal(a, b) for a, b in zip(a_list, b_list))
 
 
-def test_fused_batch_equals_plain_steps():
+def test_fused_batch_equals_plain_steps(monkeypatch):
     from ib_parallel_amd import GpuBatch, GpuSim
+    # the fused launches run the Stockham DST passes: plain handles in that form round alike
+    monkeypatch.setenv("IB_DST_FORM", "fft")
     cases = decks.ellipse_batch_cases(count=5, seed=7, **SMALL)
     assert len({(p.m, p.n, p.mgridlev) for p, _ in cases}) == 5     # uneven sizes
     batch = GpuBatch(cases)
@@ -51,6 +53,7 @@ def test_fused_batch_tile_rules_leave_results_unchanged(monkeypatch):
     from ib_parallel_amd import GpuBatch
     cases = decks.ellipse_batch_cases(count=4, seed=11, **SMALL)
     runs = []
+    monkeypatch.setenv("IB_DST_FORM", "fft")  # the Schur assembly in one form for every run
     for env in ({"IB_DST_TUNE": "0"}, {}, {"IB_DST_TILES": "4,4"}):
         monkeypatch.delenv("IB_DST_TUNE", raising=False)
         monkeypatch.delenv("IB_DST_TILES", raising=False)
@@ -85,10 +88,11 @@ def test_fused_batch_matches_oracle():
     batch.close()
 
 
-def test_fused_batch_restart_first_step_flag():
+def test_fused_batch_restart_first_step_flag(monkeypatch):
     # a simulation resumed at istart > 0 takes the AB2 branch on its first
     # batched step, one at istart = 0 the Euler branch, in the same launch
     from ib_parallel_amd import GpuBatch, GpuSim
+    monkeypatch.setenv("IB_DST_FORM", "fft")  # the fused launches' form, for the plain reference
     cases = decks.ellipse_batch_cases(count=2, seed=3, **SMALL)
     ref = GpuSim(*cases[1])
     ref.advance()

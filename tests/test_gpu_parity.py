@@ -415,6 +415,7 @@ def test_dst_tile_shapes_leave_results_unchanged(monkeypatch, shape):
     11, so its transforms take the generic prime-radix stage."""
     p, b = decks.tiny_case(m=shape[0], n=shape[1], mgridlev=2, tol=1e-12, gmres_restart=80)
     runs = []
+    monkeypatch.setenv("IB_DST_FORM", "fft")  # the Stockham form throughout: its tiles are under test
     for env in ({"IB_DST_TUNE": "0"}, {}, {"IB_DST_TILES": "4,4"}, {"IB_DST_TILES": "1,2"}):
         monkeypatch.delenv("IB_DST_TUNE", raising=False)
         monkeypatch.delenv("IB_DST_TILES", raising=False)
