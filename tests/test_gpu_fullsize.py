@@ -110,9 +110,11 @@ def test_airfoil_flap_batch_of_8_equals_plain_steps():
         s.close()
 
 
-def test_ellipses_64_full_size():
+def test_ellipses_64_full_size(monkeypatch):
     from ib_parallel_amd import GpuBatch, GpuSim
     from oracle import binding as ob
+    # the fused launches run the Stockham DST passes: plain handles in that form round alike
+    monkeypatch.setenv("IB_DST_FORM", "fft")
     cases = decks.ellipse_batch_cases(count=64)
     batch = GpuBatch(cases)
     assert batch.mode == 1

@@ -54,12 +54,18 @@ def test_steps_match_oracle(case):
         assert (ig.ksp_iter, ig.fsi_iter) == (io.ksp_iter, io.fsi_iter)
     # per level against the largest vorticity of the run: a coarse level
     # the flow has not reached after three steps holds only rounding noise
-    # (~1e-18), whose own relative difference means nothing
+    # (~1e-18), whose own relative difference means nothing.  The bound is
+    # 1e-10 or ten times the case's tolerance: the full-rectangle case
+    # (tolerance 1e-10, no drop tolerance) amplifies rounding differences,
+    # measured at 5.6e-11 of the scale with the Stockham DST passes, 3.6e-11
+    # with the matrix-core ones and 4.5e-10 with matrix-core rows and
+    # Stockham columns (the iteration counts agree in each)
     wg = [g.level(k)["omega"] for k in range(1, g.L + 1)]
     wo = [o.level(k)["omega"] for k in range(1, g.L + 1)]
     scale = max(np.abs(w).max() for w in wo)
+    bound = max(1e-10, 10 * CASES[case][0]["tol"])
     for k in range(g.L):
-        assert np.abs(wg[k] - wo[k]).max() <= 1e-10 * scale, k
+        assert np.abs(wg[k] - wo[k]).max() <= bound * scale, k
     bg, bo = g.body(), o.body()
     assert rel(bg["f"], bo["f"]) < 1e-8
     assert rel(bg["theta"], bo["theta"]) < 1e-10
