@@ -41,8 +41,9 @@ def build_lib(force=True, verbose=False):
     if not force and _newer(LIB, deps):
         return LIB
     tmp = LIB + ".tmp"
-    # rocSOLVER (with rocBLAS): the adjoint's LU of the dense Schur matrix
-    cmd = [HIPCC] + HIPFLAGS + ["-I", os.path.join(ROOT, "include")] + srcs + ["-lrocsolver", "-lrocblas", "-o", tmp]
+    # rocSOLVER (with rocBLAS): the adjoint's LU of the dense Schur matrix;
+    # roctx: the host-side ranges of the step's phases
+    cmd = [HIPCC] + HIPFLAGS + ["-I", os.path.join(ROOT, "include")] + srcs + ["-lrocsolver", "-lrocblas", "-lrocprofiler-sdk-roctx", "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

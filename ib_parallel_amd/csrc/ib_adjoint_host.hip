@@ -684,25 +684,28 @@ int ib_handle::adjoint_run(int nsteps, int Kin, const double* cx, const double* 
   } grow_guard(face_grow);
   // forward: checkpoints at the starts of the segments before the last,
   // the last segment taped
-  for (int n = 0; n < nsteps; ++n) {
-    int rc;
-    if (n >= last0) {
-      rc = taped_advance(n - last0);
-    } else {
-      if (n % K == 0) {
-        const long c = n / K;
-        save_state(c_w + c * LP, c_ro + c * LP, c_qx + c * CQX, c_qy + c * CQY);
-        if (adj_fsi) save_body(c_th + c * BS, c_f + c * nf);
+  {
+    Range fwd_range("adjoint: forward pass");
+    for (int n = 0; n < nsteps; ++n) {
+      int rc;
+      if (n >= last0) {
+        rc = taped_advance(n - last0);
+      } else {
+        if (n % K == 0) {
+          const long c = n / K;
+          save_state(c_w + c * LP, c_ro + c * LP, c_qx + c * CQX, c_qy + c * CQY);
+          if (adj_fsi) save_body(c_th + c * BS, c_f + c * nf);
+        }
+        rc = advance();
       }
-      rc = advance();
+      if (rc != IB_OK) return rc;
+      dot_acc(st, nf, cw, f, 1.0, adJ);
     }
-    if (rc != IB_OK) return rc;
-    dot_acc(st, nf, cw, f, 1.0, adJ);
-  }
-  if (keep_end) {
-    save_state(fin_w, fin_ro, fin_qx, fin_qy);
-    copy(st, fin_s, s, LP); copy(st, fin_f, f, nf); copy(st, fin_fr, f_rdst, nf);
-    if (adj_fsi) save_body(fin_th, nullptr);
+    if (keep_end) {
+      save_state(fin_w, fin_ro, fin_qx, fin_qy);
+      copy(st, fin_s, s, LP); copy(st, fin_f, f, nf); copy(st, fin_fr, f_rdst, nf);
+      if (adj_fsi) save_body(fin_th, nullptr);
+    }
   }
   // backward, segment by segment; the solves with S^T use S's LU factors,
   // formed here for the fixed S of rigid bodies and at every reverse step
@@ -719,12 +722,16 @@ int ib_handle::adjoint_run(int nsteps, int Kin, const double* cx, const double* 
   for (int sgi = nseg - 1; sgi >= 0 && status == IB_OK; --sgi) {
     const int n0 = sgi * K, n1 = std::min(nsteps, n0 + K);
     if (sgi < nseg - 1) {
+      Range r("adjoint: segment recomputed");
       load_state(c_w + (long)sgi * LP, c_ro + (long)sgi * LP, c_qx + (long)sgi * CQX, c_qy + (long)sgi * CQY);
       if (adj_fsi) load_body(c_th + (long)sgi * BS, c_f + (long)sgi * nf);
       itime = it0 + n0;
       for (int n = n0; n < n1 && status == IB_OK; ++n) status = taped_advance(n - n0);
     }
-    for (int n = n1 - 1; n >= n0 && status == IB_OK; --n) status = adjoint_step(n - n0, it0 + n + 1 == 1);
+    for (int n = n1 - 1; n >= n0 && status == IB_OK; --n) {
+      Range r("adjoint: reverse step");
+      status = adjoint_step(n - n0, it0 + n + 1 == 1);
+    }
   }
   // the initial q is vort2flux of the initial omega
   if (dw0 && status == IB_OK) v2f_adj(aw, aqx, aqy);

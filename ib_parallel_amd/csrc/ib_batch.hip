@@ -534,6 +534,7 @@ struct ib_batch {
   }
 
   int advance(int32_t* itimes, ib_step_info* infos) {
+    Range batch_range("ib_batch_advance");
     const int n = (int)sims.size();
     std::vector<int> rc(n, IB_OK);
     if (mode == 1) {

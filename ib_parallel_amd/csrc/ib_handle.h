@@ -28,10 +28,23 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "../../include/ib_gpu.h"
 #include "ib_kernels.h"
 
 namespace ibg {
+
+// A host-side range over a phase of the step, seen by `rocprofv3
+// --marker-trace` (a call and nothing more when no tool is attached).
+// Ranges cover the host's enqueue and waits: a phase whose work runs
+// asynchronously ends when its last launch is queued.
+struct Range {
+  explicit Range(const char* name) { roctxRangePush(name); }
+  ~Range() { roctxRangePop(); }
+  Range(const Range&) = delete;
+  Range& operator=(const Range&) = delete;
+};
 
 template <class T>
 struct DBuf {
@@ -1164,6 +1177,7 @@ struct ib_handle {
     // iteration
     const bool one_sync = nmoving() > 0 && !sd_on && facemode && dense_solve_ok();
     while (err_glob >= prm.fsi_tol && fsi_count <= prm.fsi_max_count) {
+      Range iter_range("fsi iteration");
       ++fsi_count;
       int its = 0;
       int rc = IB_OK;
@@ -1193,7 +1207,10 @@ struct ib_handle {
         // below if a tap has left the face set
         IB_HIP(hipEventRecord(ev_flags, st));
         form_schur_face(true);
-        IB_HIP(hipEventSynchronize(ev_flags));
+        {
+          Range w_range("wait: solve and body update");
+          IB_HIP(hipEventSynchronize(ev_flags));
+        }
         turn = std::unique_lock<std::mutex>();
         for (auto& t : tim) t.harvest();  // the timed launches all precede the event
         if (hflag[2]) csr.build(st);
@@ -1459,6 +1476,7 @@ struct ib_handle {
   }
 
   int advance() {
+    Range step_range("ib_advance");
     itime += 1;
     const bool graphs = step_graphs_ok();
     const int first = itime == 1 ? 1 : 0;
@@ -1476,11 +1494,14 @@ struct ib_handle {
       sync();
       return IB_OK;
     }
-    if (graphs) replay(first, [&] { predict(); });
-    else predict();
-    if (tape_ws) copy(st, tape_ws, omega, (long)L * g.P);
-    if (graphs) replay(3, [&] { vort2flux_dev(omega, s, qx, qy); });
-    else vort2flux_dev(omega, s, qx, qy);
+    {
+      Range r("predictor + vort2flux");
+      if (graphs) replay(first, [&] { predict(); });
+      else predict();
+      if (tape_ws) copy(st, tape_ws, omega, (long)L * g.P);
+      if (graphs) replay(3, [&] { vort2flux_dev(omega, s, qx, qy); });
+      else vort2flux_dev(omega, s, qx, qy);
+    }
 
     int status = IB_OK;
     if (prm.num_stat) {
@@ -1490,6 +1511,7 @@ struct ib_handle {
       fsi_iter = 0;
       err_fsi = 0.0;
     } else {
+      Range r("fsi loop");
       status = fsi_fixed_point();
       if (status == IB_E_INVAL) return status;
     }
@@ -1502,6 +1524,7 @@ struct ib_handle {
       copy(st, tape_th + 6 * ntors + nf, yb, nf);
     }
 
+    Range r("corrector + wait");
     if (graphs) replay(2, [&] { correct(); });
     else correct();
     sync();

@@ -13,6 +13,7 @@ Usage (on a machine with the GPU, each counter set in a run of its own):
         --json profiles/pmc_traffic.json
     python -m ib_parallel_amd.pmc stats OUT/trace        # a --kernel-trace --stats run
     python -m ib_parallel_amd.pmc rates OUT/trace profiles/pmc_traffic.json   # GB/s and GFLOP/s
+    python -m ib_parallel_amd.pmc ranges OUT/trace   # a --marker-trace run: the step's phases
 
 FETCH_SIZE and WRITE_SIZE are read in KiB per dispatch.  The device-to-device
 copy of known size that --pmc-copy-mb adds to the run calibrates them: the
@@ -179,6 +180,37 @@ def rates(args):
         print(f"| {k} | {len(v)} | {us:.2f} | {bs} | {fs} |")
 
 
+def ranges(args):
+    """Markdown table of the library's host-side ranges (roctx push/pop)
+    from a rocprofv3 --marker-trace run: calls, total and mean duration per
+    range name; with --after-copy only the ranges that start after the
+    calibration copy's dispatch ends (needs --kernel-trace in the same run)."""
+    files = glob.glob(os.path.join(args.trace_dir, "**", "*marker_api_trace.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no marker_api_trace.csv under {args.trace_dir}")
+    t_copy = None
+    if args.after_copy:
+        kt = glob.glob(os.path.join(args.trace_dir, "**", "*kernel_trace.csv"), recursive=True)
+        if kt:
+            with open(kt[0]) as fh:
+                copies = [(float(r["End_Timestamp"]) - float(r["Start_Timestamp"]), float(r["End_Timestamp"]))
+                          for r in csv.DictReader(fh) if "copy" in r["Kernel_Name"].lower()]
+            if copies:
+                t_copy = max(copies)[1]
+    agg = defaultdict(list)
+    with open(files[0]) as fh:
+        for r in csv.DictReader(fh):
+            name = r.get("Function") or r.get("Message") or r.get("Name") or ""
+            t0, t1 = float(r["Start_Timestamp"]), float(r["End_Timestamp"])
+            if t_copy is not None and t0 < t_copy:
+                continue
+            agg[name].append(t1 - t0)
+    print("| range | calls | total ms | mean us |")
+    print("|---|---|---|---|")
+    for k, v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+        print(f"| {k} | {len(v)} | {sum(v) / 1e6:.3f} | {sum(v) / len(v) / 1e3:.1f} |")
+
+
 def stats(args):
     """Markdown table of a rocprofv3 --kernel-trace --stats run: the
     whole-process kernel_stats.csv, or with --after-copy the kernel trace
@@ -220,8 +252,11 @@ def main():
     r.add_argument("trace_dir")
     r.add_argument("traffic_json")
     r.add_argument("--top", type=int, default=25)
+    g = sub.add_parser("ranges")
+    g.add_argument("trace_dir")
+    g.add_argument("--after-copy", action="store_true", help="only the ranges after the calibration copy")
     args = ap.parse_args()
-    {"traffic": traffic, "stats": stats, "rates": rates}[args.cmd](args)
+    {"traffic": traffic, "stats": stats, "rates": rates, "ranges": ranges}[args.cmd](args)
 
 
 if __name__ == "__main__":

@@ -13,7 +13,9 @@ from ib_parallel_amd import abi
 from ib_parallel_amd.sim import Simulation
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+# IB_ORACLE_LIB selects another build of the same sources (the sanitizer
+# build, liboracle_san.so, in a process that preloads its runtimes)
+LIB = os.environ.get("IB_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 _lib = None
 _fns = None
