@@ -6,7 +6,6 @@
 // with a stable radix sort, run-length encoded, and one thread per touched
 // face sums its contributions in tap order (k, then l, then p) -- the
 // reference's own addition order, so the result is deterministic run to run.
-#include <hipcub/hipcub.hpp>
 
 #include "ib_device.h"
 #include "ib_kernels.h"
@@ -184,61 +183,258 @@ void setup_reg_d(hipStream_t st, const Geom& g, int nf, const double* xb, const 
   IB_LAUNCH_CHECK();
 }
 
-__global__ void k_iota(int* p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = i;
+// ------------------------------------------------------------------------
+// The touched-face list of reg (RegCSR): for every face some tap touches, its
+// taps in tap order.  Entry k's 49 taps cover the 7 x 7 patch of faces around
+// its centre face (tap 24), so the entries touching a face are those centred
+// within 3 cells of it, in its component.  Built without sorting the taps:
+//   1. the entries sorted by centre key (ties by entry index), one workgroup;
+//   2. per tap: every entry whose patch holds the tap's face (7 binary
+//      searches, one per patch row, then a check of that entry's own key for
+//      the face): their count, the lowest of them (whose tap owns the face)
+//      and the tap's rank among them by entry index, which is tap order (an
+//      entry touches a face with one tap);
+//   3. one workgroup scans the owners (run index) and their counts (offsets);
+//   4. each tap writes itself at its owner's offset plus its rank.
+// A tap whose stencil left the grid (key clamped to 0, weight 0, the error
+// flag set) is matched by no entry and owns nothing.
+
+// centre face (component, i, j) of key f; false outside the grid
+struct FaceIJ {
+  int x, i, j, ni, nj;
+};
+__device__ __forceinline__ FaceIJ face_ij(const Geom& g, int f) {
+  FaceIJ r;
+  r.x = f < g.Fx;
+  if (r.x) {
+    r.ni = g.m + 1;
+    r.nj = g.n;
+    r.i = f % r.ni;
+    r.j = f / r.ni;
+  } else {
+    r.ni = g.m;
+    r.nj = g.n + 1;
+    r.i = (f - g.Fx) % r.ni;
+    r.j = (f - g.Fx) / r.ni;
+  }
+  return r;
+}
+__device__ __forceinline__ int face_key(const Geom& g, int x, int i, int j) {
+  return x ? (g.m + 1) * j + i : g.Fx + g.m * j + i;
 }
 
-__global__ void k_zero_int(int* p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
+// 1. bitonic sort of the composite keys (centre key << 32 | entry), n2 a
+// power of two <= kSortLds, padded with the largest key; one workgroup
+constexpr int kSortLds = 16384;
+__global__ void __launch_bounds__(1024) k_sort_centres(int nf, int n2, const int* __restrict__ key,
+                                                       int* __restrict__ skey, int* __restrict__ sent) {
+  __shared__ long long v[kSortLds];
+  for (int q = threadIdx.x; q < n2; q += blockDim.x)
+    v[q] = q < nf ? ((long long)key[(long)q * kTaps + kTaps / 2] << 32) | q : 0x7fffffffffffffffLL;
+  __syncthreads();
+  for (int size = 2; size <= n2; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int q = threadIdx.x; q < n2 / 2; q += blockDim.x) {
+        const int lo = 2 * q - (q & (stride - 1)), hi = lo + stride;  // pair (lo, hi) of this step
+        const bool up = (lo & size) == 0;
+        const long long a = v[lo], b = v[hi];
+        if ((a > b) == up) {
+          v[lo] = b;
+          v[hi] = a;
+        }
+      }
+      __syncthreads();
+    }
+  for (int q = threadIdx.x; q < nf; q += blockDim.x) {
+    skey[q] = (int)(v[q] >> 32);
+    sent[q] = (int)(v[q] & 0xffffffffLL);
+  }
+}
+
+// the same sort for larger n2 in global memory, one launch per step
+__global__ void k_sort_init(int nf, int n2, const int* __restrict__ key, long long* __restrict__ v) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n2) v[q] = q < nf ? ((long long)key[(long)q * kTaps + kTaps / 2] << 32) | q : 0x7fffffffffffffffLL;
+}
+__global__ void k_sort_step(int n2, int size, int stride, long long* __restrict__ v) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n2 / 2) return;
+  const int lo = 2 * q - (q & (stride - 1)), hi = lo + stride;
+  const bool up = (lo & size) == 0;
+  const long long a = v[lo], b = v[hi];
+  if ((a > b) == up) {
+    v[lo] = b;
+    v[hi] = a;
+  }
+}
+__global__ void k_sort_split(int nf, const long long* __restrict__ v, int* __restrict__ skey, int* __restrict__ sent) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nf) return;
+  skey[q] = (int)(v[q] >> 32);
+  sent[q] = (int)(v[q] & 0xffffffffLL);
+}
+
+// Visits every entry e whose patch holds face f (e's tap index of f in te):
+// fn(e, te) for each, in sorted-centre order.
+template <class F>
+__device__ __forceinline__ void touching(const Geom& g, int nf, const int* __restrict__ key,
+                                         const int* __restrict__ skey, const int* __restrict__ sent, int f, F&& fn) {
+  const FaceIJ c = face_ij(g, f);
+  for (int l = -3; l <= 3; ++l) {
+    const int jc = c.j - l;  // the centre row of entries whose patch row l holds f
+    if (jc < 0 || jc >= c.nj) continue;
+    const int i_lo = max(c.i - 3, 0), i_hi = min(c.i + 3, c.ni - 1);
+    const int k_lo = face_key(g, c.x, i_lo, jc), k_hi = face_key(g, c.x, i_hi, jc);
+    int lo = 0, hi = nf;  // first sorted centre >= k_lo
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (skey[mid] < k_lo) lo = mid + 1;
+      else hi = mid;
+    }
+    for (int q = lo; q < nf && skey[q] <= k_hi; ++q) {
+      const int e = sent[q];
+      const int p = c.i - (skey[q] - face_key(g, c.x, 0, jc));  // f's column offset in e's patch
+      const int te = (l + 3) * 7 + (p + 3);
+      if (key[(long)e * kTaps + te] == f) fn(e, te);
+    }
+  }
+}
+
+// 2. per tap: the count of entries touching its face (owners only, else 0),
+// its rank among them, and the owning tap (-1: the tap is matched by none)
+__global__ void k_tap_owner(Geom g, int nf, const int* __restrict__ key, const int* __restrict__ skey,
+                            const int* __restrict__ sent, int* __restrict__ own_cnt, int* __restrict__ rank,
+                            int* __restrict__ otap) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nf * kTaps) return;
+  const int k = t / kTaps, f = key[t];
+  int n = 0, below = 0, lowest = 0x7fffffff, lowest_te = 0;
+  bool self = false;
+  touching(g, nf, key, skey, sent, f, [&](int e, int te) {
+    ++n;
+    below += e < k;
+    self = self || e == k;
+    if (e < lowest) {
+      lowest = e;
+      lowest_te = te;
+    }
+  });
+  own_cnt[t] = (self && lowest == k) ? n : 0;
+  rank[t] = below;
+  otap[t] = self ? lowest * kTaps + lowest_te : -1;
+}
+
+// 3. one workgroup: run index and offset of every owner tap, in tap order;
+// toff[t] = offset (or -1), uniq/cnt/off of each run, nruns
+__global__ void __launch_bounds__(1024) k_tap_scan(int ntaps, const int* __restrict__ key,
+                                                   const int* __restrict__ own_cnt, int* __restrict__ toff,
+                                                   int* __restrict__ uniq, int* __restrict__ cnt,
+                                                   int* __restrict__ off, int* __restrict__ nruns) {
+  __shared__ int sr[1024], so[1024];
+  const int nt = blockDim.x, tid = threadIdx.x;
+  const int chunk = (ntaps + nt - 1) / nt;
+  const int t0 = min(tid * chunk, ntaps), t1 = min(t0 + chunk, ntaps);
+  int r = 0, o = 0;
+  for (int t = t0; t < t1; ++t) {
+    const int c = own_cnt[t];
+    r += c > 0;
+    o += c;
+  }
+  sr[tid] = r;
+  so[tid] = o;
+  __syncthreads();
+  // inclusive scan of the per-thread totals (Hillis-Steele)
+  for (int d = 1; d < nt; d <<= 1) {
+    const int ar = tid >= d ? sr[tid - d] : 0, ao = tid >= d ? so[tid - d] : 0;
+    __syncthreads();
+    sr[tid] += ar;
+    so[tid] += ao;
+    __syncthreads();
+  }
+  r = sr[tid] - r;  // exclusive
+  o = so[tid] - o;
+  for (int t = t0; t < t1; ++t) {
+    const int c = own_cnt[t];
+    if (c > 0) {
+      uniq[r] = key[t];
+      cnt[r] = c;
+      off[r] = o;
+      toff[t] = o;
+      ++r;
+      o += c;
+    } else {
+      toff[t] = -1;
+    }
+  }
+  if (tid == nt - 1) *nruns = sr[nt - 1];
+}
+
+// 4. each matched tap at its owner's offset plus its rank
+__global__ void k_tap_list(int ntaps, const int* __restrict__ rank, const int* __restrict__ otap,
+                           const int* __restrict__ toff, int* __restrict__ taps) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntaps) return;
+  const int o = otap[t];
+  if (o < 0) return;
+  taps[toff[o] + rank[t]] = t;
 }
 
 void RegCSR::alloc(int ntaps_, int max_key) {
+  release();
   ntaps = ntaps_;
-  key_bits = 1;
-  while ((1LL << key_bits) <= (long long)max_key) ++key_bits;
+  (void)max_key;
+  const int nf = ntaps / kTaps;
   const size_t b = sizeof(int) * (size_t)(ntaps > 0 ? ntaps : 1);
   IB_HIP(hipMalloc(&key, b));
-  IB_HIP(hipMalloc(&key_sorted, b));
-  IB_HIP(hipMalloc(&tap, b));
+  IB_HIP(hipMalloc(&own, b));
+  IB_HIP(hipMalloc(&toff, b));
+  IB_HIP(hipMalloc(&rank, b));
+  IB_HIP(hipMalloc(&otap, b));
   IB_HIP(hipMalloc(&tap_sorted, b));
   IB_HIP(hipMalloc(&uniq, b));
   IB_HIP(hipMalloc(&cnt, b));
   IB_HIP(hipMalloc(&off, b));
   IB_HIP(hipMalloc(&nruns, sizeof(int)));
-  size_t t1 = 0, t2 = 0, t3 = 0;
-  IB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, key, key_sorted, tap, tap_sorted, ntaps, 0, key_bits));
-  IB_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, t2, key_sorted, uniq, cnt, nruns, ntaps));
-  IB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t3, cnt, off, ntaps));
-  tmp_bytes = std::max(t1, std::max(t2, t3));
-  IB_HIP(hipMalloc(&tmp, tmp_bytes > 0 ? tmp_bytes : 16));
-  if (ntaps > 0) {
-    hipLaunchKernelGGL(k_iota, dim3(ceil_div(ntaps, 256)), dim3(256), 0, 0, tap, ntaps);
-    IB_LAUNCH_CHECK();
-    IB_HIP(hipDeviceSynchronize());
-  }
+  IB_HIP(hipMalloc(&skey, sizeof(int) * (size_t)std::max(nf, 1)));
+  IB_HIP(hipMalloc(&sent, sizeof(int) * (size_t)std::max(nf, 1)));
+  n2 = 1;
+  while (n2 < nf) n2 <<= 1;
+  if (n2 > kSortLds) IB_HIP(hipMalloc(&sortv, sizeof(long long) * (size_t)n2));
 }
 
 void RegCSR::release() {
-  int* ps[] = {key, key_sorted, tap, tap_sorted, uniq, cnt, off, nruns};
+  int* ps[] = {key, own, toff, rank, otap, tap_sorted, uniq, cnt, off, nruns, skey, sent};
   for (int* p : ps)
     if (p) (void)hipFree(p);
-  if (tmp) (void)hipFree(tmp);
-  key = key_sorted = tap = tap_sorted = uniq = cnt = off = nruns = nullptr;
-  tmp = nullptr;
+  if (sortv) (void)hipFree(sortv);
+  key = own = toff = rank = otap = tap_sorted = uniq = cnt = off = nruns = skey = sent = nullptr;
+  sortv = nullptr;
 }
 
-void RegCSR::build(hipStream_t st) {
+void RegCSR::build(hipStream_t st, const Geom& g) {
   if (ntaps <= 0) return;
-  size_t b = tmp_bytes;
-  IB_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, b, key, key_sorted, tap, tap_sorted, ntaps, 0, key_bits, st));
-  hipLaunchKernelGGL(k_zero_int, dim3(ceil_div(ntaps, 256)), dim3(256), 0, st, cnt, ntaps);
+  const int nf = ntaps / kTaps;
+  if (n2 <= kSortLds) {
+    hipLaunchKernelGGL(k_sort_centres, dim3(1), dim3(1024), 0, st, nf, n2, key, skey, sent);
+    IB_LAUNCH_CHECK();
+  } else {
+    hipLaunchKernelGGL(k_sort_init, dim3(ceil_div(n2, 256)), dim3(256), 0, st, nf, n2, key, sortv);
+    IB_LAUNCH_CHECK();
+    for (int size = 2; size <= n2; size <<= 1)
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        hipLaunchKernelGGL(k_sort_step, dim3(ceil_div(n2 / 2, 256)), dim3(256), 0, st, n2, size, stride, sortv);
+        IB_LAUNCH_CHECK();
+      }
+    hipLaunchKernelGGL(k_sort_split, dim3(ceil_div(nf, 256)), dim3(256), 0, st, nf, sortv, skey, sent);
+    IB_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_tap_owner, dim3(ceil_div(ntaps, 256)), dim3(256), 0, st, g, nf, key, skey, sent, own, rank,
+                     otap);
   IB_LAUNCH_CHECK();
-  b = tmp_bytes;
-  IB_HIP(hipcub::DeviceRunLengthEncode::Encode(tmp, b, key_sorted, uniq, cnt, nruns, ntaps, st));
-  b = tmp_bytes;
-  IB_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, b, cnt, off, ntaps, st));
+  hipLaunchKernelGGL(k_tap_scan, dim3(1), dim3(1024), 0, st, ntaps, key, own, toff, uniq, cnt, off, nruns);
+  IB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_tap_list, dim3(ceil_div(ntaps, 256)), dim3(256), 0, st, ntaps, rank, otap, toff, tap_sorted);
+  IB_LAUNCH_CHECK();
 }
 
 // reg: one thread per touched face, contributions summed in tap order

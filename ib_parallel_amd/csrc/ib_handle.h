@@ -684,7 +684,7 @@ struct ib_handle {
     fill(st, hy, g.Fy, 0.0);
     setup_reg(st, g, nf, xb, yb, comp, delta, offx, offy, prm.deltafnc_f32_literals, w, csr.key, err_flag,
               key_changed);
-    if (!key_changed) csr.build(st);
+    if (!key_changed) csr.build(st, g);
   }
 
   int read_err_flag() {
@@ -937,7 +937,7 @@ struct ib_handle {
       IB_HIP(hipMemsetAsync(err_flag.p, 0, sizeof(int), st));
       setup_reg(st, g, nf_sd, dx_, dy_, dc_, delta, offx, offy, prm.deltafnc_f32_literals, w_sd, csr_sd.key, err_flag);
       if (read_err_flag()) return fail(IB_E_INVAL, "sub-domain point regularisation stencil leaves the grid");
-      csr_sd.build(st);
+      csr_sd.build(st, g);
     }
     std::vector<long> kptr(nf_sd + 1, 0);
     std::vector<int> krow;
@@ -1213,7 +1213,7 @@ struct ib_handle {
         }
         turn = std::unique_lock<std::mutex>();
         for (auto& t : tim) t.harvest();  // the timed launches all precede the event
-        if (hflag[2]) csr.build(st);
+        if (hflag[2]) csr.build(st, g);
         rc = dense_check();
         its = hst->its;
         if (tim[4].on) tim[4].total_bytes += gmres_operand_bytes(its);

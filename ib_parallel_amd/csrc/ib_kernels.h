@@ -223,17 +223,23 @@ const void* fluid_tab_kernel(int kind);
 void setup_reg(hipStream_t st, const Geom& g, int nf, const double* xb, const double* yb, const int* comp,
                double delta, double offx, double offy, int f32lit, double* w, int* key, int* err,
                int* changed = nullptr);
-// sort taps by key (stable) and run-length encode: uniq keys, offsets, count
+// The touched faces of a tap list and each face's taps in tap order: nruns
+// faces uniq[r], their taps tap_sorted[off[r] .. off[r] + cnt[r]).  key holds
+// the tap keys (setup_reg); build() forms the rest from them on the device
+// (hand-written: the entries sorted by centre face, each tap's face owned by
+// the lowest entry touching it, one scan for the offsets).
 struct RegCSR {
   int ntaps = 0;
-  int *key = nullptr, *key_sorted = nullptr, *tap = nullptr, *tap_sorted = nullptr;
+  int *key = nullptr, *tap_sorted = nullptr;
   int *uniq = nullptr, *cnt = nullptr, *off = nullptr, *nruns = nullptr;
-  void* tmp = nullptr;
-  size_t tmp_bytes = 0;
-  int key_bits = 32;
+  int *own = nullptr, *toff = nullptr;    // per tap: count if it owns its face, offset if so
+  int *rank = nullptr, *otap = nullptr;   // per tap: rank among its face's taps, the owning tap
+  int *skey = nullptr, *sent = nullptr;   // entries sorted by centre key
+  long long* sortv = nullptr;             // sort buffer of larger entry counts
+  int n2 = 1;
   void alloc(int ntaps, int max_key);
   void release();
-  void build(hipStream_t st);
+  void build(hipStream_t st, const Geom& g);
 };
 // reg (E^T): hx/hy[face] = sum over taps of w*f[tap/49] in tap order; faces not
 // touched are left as they are (callers zero them).

@@ -64,6 +64,36 @@ def test_ib_ops(pair):
         assert rel(g.nonlinear_rot(k, hx[: g.P], qx, qy, bc), o.nonlinear_rot(k, hx[: g.P], qx, qy, bc)) < 1e-13
 
 
+@pytest.mark.parametrize("npts", (60, 8200))
+def test_reg_touched_face_lists(npts):
+    """reg through the device-built touched-face lists against the oracle:
+    60 points in a disc of 1.5 cells (up to 60 taps on one face), and 8200
+    points on 8 circles (16400 entries: the entry sort runs in global memory,
+    one launch per bitonic step; up to ~90 taps on one face).  Both sides sum
+    each face's taps in tap order (the GPU with fused multiply-adds), so every
+    face agrees to rounding; a tap missing from or added to a face's list
+    would be off by its whole contribution."""
+    from oracle import binding as ob
+    rng = np.random.default_rng(npts)
+    if npts == 60:
+        p, _ = decks.tiny_case(m=64, n=48, mgridlev=2, torsional=False, schur_mode=1)
+        r, a = 0.03 * np.sqrt(rng.random(npts)), 2 * np.pi * rng.random(npts)
+        x, y = 0.11 + r * np.cos(a), 0.07 + r * np.sin(a)
+    else:
+        p = abi.make_params(256, 256, 1, 0.01, 100.0, 4.0, 2.0, 2.0, schur_mode=1)
+        rad = np.repeat(np.linspace(0.2, 1.8, 8), npts // 8)
+        t = 2 * np.pi * (np.tile(np.arange(npts // 8), 8) + 0.5) / (npts // 8)
+        x, y = rad * np.cos(t), rad * np.sin(t)
+    b = [abi.BodySpec(x, y, fam=1)]
+    g, o = gpu(p, b), orc(p, b)
+    f = rng.standard_normal(g.nf)
+    for a_, b_ in zip(g.reg(f), o.reg(f)):
+        assert np.abs(a_ - b_).max() <= 1e-13 * np.abs(b_).max()
+        assert np.array_equal(a_ != 0, b_ != 0)  # the same touched faces
+    g.close()
+    o.close()
+
+
 def test_schur_matrix(pair):
     """Batched column assembly against the oracle's column-by-column
     construction and against the matrix-free product on the GPU."""
