@@ -1430,7 +1430,8 @@ __device__ __forceinline__ void coopw_solve(const GmCoopWArgs& a, int c, unsigne
   double* lss = lcc + kmax;            // kmax
   double* grs = lss + kmax;            // kmax + 1 (y of the back substitution in place)
   double* stg = grs + kmax + 1;        // kCwBlk^2: a diagonal block of the back substitution
-  double* Vl = stg + kCwBlk * kCwBlk;  // R NL: local rows of basis vectors 0..NL-1, row-major
+  double* ub = stg + kCwBlk * kCwBlk;  // nf: the vector of a product, copied once from L2
+  double* Vl = ub + nf;                // R NL: local rows of basis vectors 0..NL-1, row-major
   // rows 2 wq and 2 wq + 1 of S, columns ln + 64 k, in registers (0 outside the matrix)
   double s0[CQ], s1[CQ];
   {
@@ -1445,16 +1446,22 @@ __device__ __forceinline__ void coopw_solve(const GmCoopWArgs& a, int c, unsigne
   }
   __syncthreads();
 
-  // S[local rows] u for a vector u another workgroup step left in HBM/L2:
-  // every lane loads the CQ entries it multiplies (no LDS copy of u); lanes
-  // 0 and 1 of wavefront q get rows 2q and 2q + 1 (0 in the other lanes)
+  // S[local rows] u for a vector u the workgroups left in L2: the
+  // workgroup copies u into LDS once (ub; one load per thread), then every
+  // lane reads the CQ entries it multiplies from there, so the L2 serves u
+  // once per workgroup instead of once per wavefront.  Lanes 0 and 1 of
+  // wavefront q get rows 2q and 2q + 1 (0 in the other lanes).  Every
+  // thread must call it (a workgroup barrier follows the copy); ub is next
+  // written by the following call, after at least one more barrier.
   auto matvec_g = [&](const double* u) -> double {
+    for (int i = gm_opaque(threadIdx.x); i < nf; i += T) ub[i] = xc_ld<X2>(u + i);
+    __syncthreads();
     const int q = gm_opaque(threadIdx.x) >> 6, l = gm_opaque(threadIdx.x) & 63;
     double sum = 0.0;
     if (2 * q < Rc) {  // uniform over the wavefront
       double v[CQ];
 #pragma unroll
-      for (int k = 0; k < CQ; ++k) v[k] = xc_ld<X2>(u + min(l + 64 * k, nf - 1));
+      for (int k = 0; k < CQ; ++k) v[k] = ub[min(l + 64 * k, nf - 1)];
       double acc0 = 0.0, acc1 = 0.0;
 #pragma unroll
       for (int k = 0; k < CQ; ++k) {
@@ -1618,8 +1625,8 @@ __device__ __forceinline__ void coopw_solve(const GmCoopWArgs& a, int c, unsigne
       // the row sums of S w for the next step (from L2, beside the norm
       // partials and the Givens step of wavefront 0)
       const int t0 = gm_opaque(threadIdx.x);
-      const double wpre_l = t0 < Rc ? xc_ld<X2>(a.wfull + r0 + t0) : 0.0;
       const double swsum = matvec_g(a.wfull);
+      const double wpre_l = t0 < Rc ? ub[r0 + t0] : 0.0;
       if (gm_opaque(threadIdx.x) < 64) {
         const int lane = gm_opaque(threadIdx.x);
         const double tt = sqrt(gw_wave_gather<X2>(a.norms, 4, a.P, lane));
@@ -1858,10 +1865,9 @@ __global__ void k_coop_prep_multi(GmCoopWMulti m) {
 }
 
 // dynamic LDS of k_gmres_coopw without the basis (doubles): wl 32,
-// hcol/alpha kmax+2 each, lcc/lss kmax each, grs kmax+1, stg kCwBlk^2
+// hcol/alpha kmax+2 each, lcc/lss kmax each, grs kmax+1, stg kCwBlk^2, ub nf
 static size_t coopw_lds_fixed(int nf, int kmax) {
-  (void)nf;
-  return 32 + (size_t)5 * kmax + 5 + (size_t)kCwBlk * kCwBlk;
+  return 32 + (size_t)5 * kmax + 5 + (size_t)kCwBlk * kCwBlk + (size_t)nf;
 }
 
 bool gmres_coopw_plan(int nf, int kmax, size_t max_lds, int ncu, int* P, int* R, int* NL) {
