@@ -194,7 +194,8 @@ void setup_reg_d(hipStream_t st, const Geom& g, int nf, const double* xb, const 
 //      the face): their count, the lowest of them (whose tap owns the face)
 //      and the tap's rank among them by entry index, which is tap order (an
 //      entry touches a face with one tap);
-//   3. one workgroup scans the owners (run index) and their counts (offsets);
+//   3. a scan of the owners (run index) and their counts (offsets) over
+//      tiles of 1024 taps;
 //   4. each tap writes itself at its owner's offset plus its rank.
 // A tap whose stencil left the grid (key clamped to 0, weight 0, the error
 // flag set) is matched by no entry and owns nothing.
@@ -324,49 +325,79 @@ __global__ void k_tap_owner(Geom g, int nf, const int* __restrict__ key, const i
   otap[t] = self ? lowest * kTaps + lowest_te : -1;
 }
 
-// 3. one workgroup: run index and offset of every owner tap, in tap order;
-// toff[t] = offset (or -1), uniq/cnt/off of each run, nruns
-__global__ void __launch_bounds__(1024) k_tap_scan(int ntaps, const int* __restrict__ key,
-                                                   const int* __restrict__ own_cnt, int* __restrict__ toff,
-                                                   int* __restrict__ uniq, int* __restrict__ cnt,
-                                                   int* __restrict__ off, int* __restrict__ nruns) {
-  __shared__ int sr[1024], so[1024];
-  const int nt = blockDim.x, tid = threadIdx.x;
-  const int chunk = (ntaps + nt - 1) / nt;
-  const int t0 = min(tid * chunk, ntaps), t1 = min(t0 + chunk, ntaps);
-  int r = 0, o = 0;
-  for (int t = t0; t < t1; ++t) {
-    const int c = own_cnt[t];
-    r += c > 0;
-    o += c;
-  }
-  sr[tid] = r;
-  so[tid] = o;
+// 3. run index and offset of every owner tap, in tap order: tiles of 1024
+// taps (one per thread, coalesced); the tile totals of owners and counts,
+// their exclusive scan in one workgroup, then each tile's own scan.
+// toff[t] = offset (or -1), uniq/cnt/off of each run, nruns.
+constexpr int kScanTile = 1024;
+// inclusive scan of (a, b) over the workgroup's 1024 threads (Hillis-Steele)
+__device__ __forceinline__ void block_scan2(int& a, int& b, int* sa, int* sb) {
+  const int tid = threadIdx.x;
+  sa[tid] = a;
+  sb[tid] = b;
   __syncthreads();
-  // inclusive scan of the per-thread totals (Hillis-Steele)
-  for (int d = 1; d < nt; d <<= 1) {
-    const int ar = tid >= d ? sr[tid - d] : 0, ao = tid >= d ? so[tid - d] : 0;
+  for (int d = 1; d < kScanTile; d <<= 1) {
+    const int xa = tid >= d ? sa[tid - d] : 0, xb = tid >= d ? sb[tid - d] : 0;
     __syncthreads();
-    sr[tid] += ar;
-    so[tid] += ao;
+    sa[tid] += xa;
+    sb[tid] += xb;
     __syncthreads();
   }
-  r = sr[tid] - r;  // exclusive
-  o = so[tid] - o;
-  for (int t = t0; t < t1; ++t) {
-    const int c = own_cnt[t];
-    if (c > 0) {
-      uniq[r] = key[t];
-      cnt[r] = c;
-      off[r] = o;
-      toff[t] = o;
-      ++r;
-      o += c;
-    } else {
-      toff[t] = -1;
+  a = sa[tid];
+  b = sb[tid];
+}
+__global__ void __launch_bounds__(kScanTile) k_tap_tile_sum(int ntaps, const int* __restrict__ own_cnt,
+                                                           int* __restrict__ tsum) {
+  __shared__ int sa[kScanTile], sb[kScanTile];
+  const int t = blockIdx.x * kScanTile + threadIdx.x;
+  const int c = t < ntaps ? own_cnt[t] : 0;
+  int r = c > 0, o = c;
+  block_scan2(r, o, sa, sb);
+  if (threadIdx.x == kScanTile - 1) {
+    tsum[2 * blockIdx.x] = r;
+    tsum[2 * blockIdx.x + 1] = o;
+  }
+}
+// one workgroup: exclusive scan of the tile totals in place; nruns
+__global__ void __launch_bounds__(kScanTile) k_tap_tile_scan(int ntiles, int* __restrict__ tsum,
+                                                            int* __restrict__ nruns) {
+  __shared__ int sa[kScanTile], sb[kScanTile];
+  int carry_r = 0, carry_o = 0;
+  for (int b0 = 0; b0 < ntiles; b0 += kScanTile) {  // uniform trip count
+    const int b = b0 + threadIdx.x;
+    const int r0 = b < ntiles ? tsum[2 * b] : 0, o0 = b < ntiles ? tsum[2 * b + 1] : 0;
+    int r = r0, o = o0;
+    block_scan2(r, o, sa, sb);
+    if (b < ntiles) {
+      tsum[2 * b] = carry_r + r - r0;
+      tsum[2 * b + 1] = carry_o + o - o0;
     }
+    carry_r += sa[kScanTile - 1];
+    carry_o += sb[kScanTile - 1];
+    __syncthreads();
   }
-  if (tid == nt - 1) *nruns = sr[nt - 1];
+  if (threadIdx.x == 0) *nruns = carry_r;
+}
+__global__ void __launch_bounds__(kScanTile) k_tap_tile_write(int ntaps, const int* __restrict__ key,
+                                                             const int* __restrict__ own_cnt,
+                                                             const int* __restrict__ tsum, int* __restrict__ toff,
+                                                             int* __restrict__ uniq, int* __restrict__ cnt,
+                                                             int* __restrict__ off) {
+  __shared__ int sa[kScanTile], sb[kScanTile];
+  const int t = blockIdx.x * kScanTile + threadIdx.x;
+  const int c = t < ntaps ? own_cnt[t] : 0;
+  int r = c > 0, o = c;
+  block_scan2(r, o, sa, sb);
+  if (t >= ntaps) return;
+  if (c > 0) {
+    const int rr = tsum[2 * blockIdx.x] + r - 1, oo = tsum[2 * blockIdx.x + 1] + o - c;  // exclusive
+    uniq[rr] = key[t];
+    cnt[rr] = c;
+    off[rr] = oo;
+    toff[t] = oo;
+  } else {
+    toff[t] = -1;
+  }
 }
 
 // 4. each matched tap at its owner's offset plus its rank
@@ -390,6 +421,7 @@ void RegCSR::alloc(int ntaps_, int max_key) {
   IB_HIP(hipMalloc(&toff, b));
   IB_HIP(hipMalloc(&rank, b));
   IB_HIP(hipMalloc(&otap, b));
+  IB_HIP(hipMalloc(&tsum, sizeof(int) * 2 * (size_t)std::max(1, ceil_div(ntaps, kScanTile))));
   IB_HIP(hipMalloc(&tap_sorted, b));
   IB_HIP(hipMalloc(&uniq, b));
   IB_HIP(hipMalloc(&cnt, b));
@@ -403,11 +435,11 @@ void RegCSR::alloc(int ntaps_, int max_key) {
 }
 
 void RegCSR::release() {
-  int* ps[] = {key, own, toff, rank, otap, tap_sorted, uniq, cnt, off, nruns, skey, sent};
+  int* ps[] = {key, own, toff, rank, otap, tsum, tap_sorted, uniq, cnt, off, nruns, skey, sent};
   for (int* p : ps)
     if (p) (void)hipFree(p);
   if (sortv) (void)hipFree(sortv);
-  key = own = toff = rank = otap = tap_sorted = uniq = cnt = off = nruns = skey = sent = nullptr;
+  key = own = toff = rank = otap = tsum = tap_sorted = uniq = cnt = off = nruns = skey = sent = nullptr;
   sortv = nullptr;
 }
 
@@ -431,7 +463,13 @@ void RegCSR::build(hipStream_t st, const Geom& g) {
   hipLaunchKernelGGL(k_tap_owner, dim3(ceil_div(ntaps, 256)), dim3(256), 0, st, g, nf, key, skey, sent, own, rank,
                      otap);
   IB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_tap_scan, dim3(1), dim3(1024), 0, st, ntaps, key, own, toff, uniq, cnt, off, nruns);
+  const int ntiles = ceil_div(ntaps, kScanTile);
+  hipLaunchKernelGGL(k_tap_tile_sum, dim3(ntiles), dim3(kScanTile), 0, st, ntaps, own, tsum);
+  IB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_tap_tile_scan, dim3(1), dim3(kScanTile), 0, st, ntiles, tsum, nruns);
+  IB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_tap_tile_write, dim3(ntiles), dim3(kScanTile), 0, st, ntaps, key, own, tsum, toff, uniq, cnt,
+                     off);
   IB_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_tap_list, dim3(ceil_div(ntaps, 256)), dim3(256), 0, st, ntaps, rank, otap, toff, tap_sorted);
   IB_LAUNCH_CHECK();

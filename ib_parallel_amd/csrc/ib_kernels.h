@@ -234,6 +234,7 @@ struct RegCSR {
   int *uniq = nullptr, *cnt = nullptr, *off = nullptr, *nruns = nullptr;
   int *own = nullptr, *toff = nullptr;    // per tap: count if it owns its face, offset if so
   int *rank = nullptr, *otap = nullptr;   // per tap: rank among its face's taps, the owning tap
+  int* tsum = nullptr;                    // per tile of 1024 taps: owners and counts (then their scan)
   int *skey = nullptr, *sent = nullptr;   // entries sorted by centre key
   long long* sortv = nullptr;             // sort buffer of larger entry counts
   int n2 = 1;
