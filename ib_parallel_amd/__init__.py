@@ -116,6 +116,14 @@ class GpuSim(Simulation):
             out += [dk, dg]
         return tuple(out)
 
+    def hinge_grad(self):
+        """(dJ/dxh, dJ/dyh) per body: the gradient of the last adjoint's J
+        with respect to each torsional body's hinge (0 for the others); needs
+        the last adjoint run with position_grad in the face-space form."""
+        gx, gy = np.empty(self.nbodies), np.empty(self.nbodies)
+        self._check(self._f["adjoint_hinge_grad"](self.h, abi.dptr(gx), abi.dptr(gy)))
+        return gx, gy
+
     def kernel_timing(self, kclass, enable=True):
         self._check(self._f["kernel_timing"](self.h, kclass, 1 if enable else 0))
 

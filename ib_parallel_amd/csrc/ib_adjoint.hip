@@ -662,8 +662,12 @@ __global__ void k_pos_fsi(int nf, const int* __restrict__ tors_of, const int* __
                           const double* __restrict__ f, const double* __restrict__ hb, const double* __restrict__ qt,
                           const double* __restrict__ vT, const double* __restrict__ uT, double* __restrict__ gx,
                           double* __restrict__ gy) {
+  // every entry, torsional ones included: their sum over a torsional body
+  // is the gradient with respect to its hinge (a hinge shift translates the
+  // body's points and changes neither the arms nor the coupling vector)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nf || tors_of[i] >= 0) return;
+  if (i >= nf) return;
+  (void)tors_of;
   double ax = 0.0, ay = 0.0;
   for (int t = 0; t < kTaps; ++t) {
     const int q = i * kTaps + t;

@@ -648,6 +648,7 @@ int ib_handle::adjoint_run(int nsteps, int Kin, const double* cx, const double* 
   adj_gm = gm_ok;
   const int K = adjoint_segment(nsteps, Kin);
   want_pos = dxb || dyb;
+  hinge_ok = false;
   if (want_pos && !adj_fsi) adjoint_pos_setup();
   adjoint_setup(nsteps, K);
   if (want_pos || adj_fsi) {
@@ -759,23 +760,34 @@ int ib_handle::adjoint_run(int nsteps, int Kin, const double* cx, const double* 
   if (dRe) *dRe = hv[1];
   if (want_pos) {
     // per point: the sum over its x-force and y-force entries (both read
-    // the point's coordinates); points in deck order; the points of
-    // torsional bodies (placed by their hinge and angle) get 0
+    // the point's coordinates); points in deck order.  The points of a
+    // torsional body are placed by its hinge and angle: they get 0, and
+    // their sum is the body's hinge gradient (ib_adjoint_hinge_grad)
     std::vector<double> gx(nf), gy(nf);
     d2h(gx.data(), gpx, nf);
     d2h(gy.data(), gpy, nf);
     sync();
+    hinge_gx.assign(nbodies, 0.0);
+    hinge_gy.assign(nbodies, 0.0);
     int pt = 0;
     for (int k = 0; k < nf;) {
       int n = 0;
       while (k + n < nf && h_cbody[k + n] == h_cbody[k] && h_ccomp[k + n] == 1) ++n;
+      const int body = h_cbody[k];
+      const bool tors = adj_fsi && h_tors_of[k] >= 0;  // moving in this form
       for (int j = 0; j < n; ++j) {
-        if (dxb) dxb[pt + j] = gx[k + j] + gx[k + n + j];
-        if (dyb) dyb[pt + j] = gy[k + j] + gy[k + n + j];
+        const double px = gx[k + j] + gx[k + n + j], py = gy[k + j] + gy[k + n + j];
+        if (tors) {
+          hinge_gx[body] += px;
+          hinge_gy[body] += py;
+        }
+        if (dxb) dxb[pt + j] = tors ? 0.0 : px;
+        if (dyb) dyb[pt + j] = tors ? 0.0 : py;
       }
       pt += n;
       k += 2 * n;
     }
+    hinge_ok = adj_fsi;
   }
   // stiffness and pitch: per body in deck order (rigid bodies 0)
   for (int b = 0; b < nbodies; ++b) {

@@ -379,6 +379,9 @@ struct ib_handle {
   DBuf<double> t_th, t_f0, c_th, c_f, fin_th;
   int64_t dG_build = -1;
   std::vector<double> bth, bthd, bthdd, gK, gG0, h_scal;
+  // hinge gradients per body of the last adjoint run with position gradients
+  std::vector<double> hinge_gx, hinge_gy;
+  bool hinge_ok = false;
   // pressure post-processing (allocated on first use): cell-grid DST plans
   // and table, corner products (two levels), rhs, solution, scratch
   bool pres_ready = false;

@@ -507,6 +507,16 @@ int ib_adjoint_run(ib_handle* h, int32_t nsteps, int32_t ckpt_every, const doubl
   API_CATCH(h)
 }
 
+int ib_adjoint_hinge_grad(const ib_handle* h, double* dJ_dxh, double* dJ_dyh) {
+  if (!h || !dJ_dxh || !dJ_dyh) return IB_E_INVAL;
+  if (!h->hinge_ok) return IB_E_INVAL;
+  for (int b = 0; b < h->nbodies; ++b) {
+    dJ_dxh[b] = h->hinge_gx[b];
+    dJ_dyh[b] = h->hinge_gy[b];
+  }
+  return IB_OK;
+}
+
 int ib_kernel_timing_read(ib_handle* h, int32_t kclass, int64_t* launches, double* total_ms, double* total_bytes) {
   if (!h || kclass < 0 || kclass > 4) return IB_E_INVAL;
   API_TRY

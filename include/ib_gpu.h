@@ -344,6 +344,16 @@ int ib_batch_adjoint(ib_batch* b, int32_t nsteps, int32_t ckpt_every, const doub
 int ib_adjoint_run(ib_handle* h, int32_t nsteps, int32_t ckpt_every, const double* cx, const double* cy, double* J,
                    double* dJ_dRe, double* dJ_domega0, double* dJ_dxb, double* dJ_dyb, double* dJ_dK,
                    double* dJ_dgamma0);
+/* the gradient of the last ib_adjoint_run's J with respect to the hinge
+ * (xbf0, ybf0) of every torsional body (no reference counterpart; the
+ * hinge is the body's first deck point, src/grid.F90:252-288, and places its
+ * points with the angle, src/operators_struct.F90:600-620): one entry per
+ * body in deck order, 0 for the others.  A hinge shift translates the
+ * body's points and changes neither its arms nor its coupling vector, so the
+ * gradient is the sum over its points of the position partials.  Needs a
+ * run in the face-space form that asked for position gradients (dJ_dxb or
+ * dJ_dyb not NULL); IB_E_INVAL otherwise. */
+int ib_adjoint_hinge_grad(const ib_handle* h, double* dJ_dxh, double* dJ_dyh);
 
 #ifdef __cplusplus
 }
