@@ -117,12 +117,13 @@ class GpuSim(Simulation):
         return tuple(out)
 
     def hinge_grad(self):
-        """(dJ/dxh, dJ/dyh) per body: the gradient of the last adjoint's J
-        with respect to each torsional body's hinge (0 for the others); needs
-        the last adjoint run with position_grad in the face-space form."""
-        gx, gy = np.empty(self.nbodies), np.empty(self.nbodies)
-        self._check(self._f["adjoint_hinge_grad"](self.h, abi.dptr(gx), abi.dptr(gy)))
-        return gx, gy
+        """(dJ/dxh, dJ/dyh, dJ/dds) per body: the gradient of the last
+        adjoint's J with respect to each torsional body's hinge and point
+        spacing (0 for the others); needs the last adjoint run with
+        position_grad in the face-space form."""
+        gx, gy, gs = np.empty(self.nbodies), np.empty(self.nbodies), np.empty(self.nbodies)
+        self._check(self._f["adjoint_hinge_grad"](self.h, abi.dptr(gx), abi.dptr(gy), abi.dptr(gs)))
+        return gx, gy, gs
 
     def kernel_timing(self, kclass, enable=True):
         self._check(self._f["kernel_timing"](self.h, kclass, 1 if enable else 0))

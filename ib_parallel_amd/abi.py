@@ -118,7 +118,7 @@ GPU_ONLY = {
     "schur_info": (C.c_int, [_P, _I, _I, C.POINTER(C.c_int64)]),
     "dst_tiles": (C.c_int, [_P, _I, _I, _I]),
     "dst_form": (C.c_int, [_P, _I, _D]),
-    "adjoint_hinge_grad": (C.c_int, [_P, _D, _D]),
+    "adjoint_hinge_grad": (C.c_int, [_P, _D, _D, _D]),
     "gmres_info": (C.c_int, [_P, _I, _I, _I]),
     # b and x are device addresses
     "op_gmres_noalloc": (C.c_int, [_P, _P, _P, _I]),

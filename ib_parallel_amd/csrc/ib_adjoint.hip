@@ -661,7 +661,7 @@ __global__ void k_pos_fsi(int nf, const int* __restrict__ tors_of, const int* __
                           const double* __restrict__ dwx, const double* __restrict__ dwy, const double* __restrict__ l,
                           const double* __restrict__ f, const double* __restrict__ hb, const double* __restrict__ qt,
                           const double* __restrict__ vT, const double* __restrict__ uT, double* __restrict__ gx,
-                          double* __restrict__ gy) {
+                          double* __restrict__ gy, double* __restrict__ sx, double* __restrict__ sy) {
   // every entry, torsional ones included: their sum over a torsional body
   // is the gradient with respect to its hinge (a hinge shift translates the
   // body's points and changes neither the arms nor the coupling vector)
@@ -678,14 +678,52 @@ __global__ void k_pos_fsi(int nf, const int* __restrict__ tors_of, const int* __
   }
   gx[i] += ax;
   gy[i] += ay;
+  if (sx) {  // this step's partials alone
+    sx[i] = ax;
+    sy[i] = ay;
+  }
 }
 
 void pos_fsi(hipStream_t st, int nf, const int* tors_of, const int* tapidx, const double* dwx, const double* dwy,
              const double* l, const double* f, const double* hb, const double* qt, const double* vT, const double* uT,
-             double* gx, double* gy) {
+             double* gx, double* gy, double* sx, double* sy) {
   if (nf <= 0) return;
   hipLaunchKernelGGL(k_pos_fsi, dim3(ceil_div(nf, 256)), dim3(256), 0, st, nf, tors_of, tapidx, dwx, dwy, l, f, hb, qt,
-                     vT, uT, gx, gy);
+                     vT, uT, gx, gy, sx, sy);
+  IB_LAUNCH_CHECK();
+}
+
+// The weights' share of dJ/d(ds) for torsional body t at this step: its
+// points sit at the hinge + (j-1) ds (cos, sin)(gamma0 - theta), so the
+// step's position partials (sx, sy of every entry) projected on
+// (j-1)(cos, sin)(gamma0 - theta), summed in a fixed order and added to
+// acc[t] (one workgroup per body).
+__global__ void __launch_bounds__(256) k_ds_weights(const TorsBody* __restrict__ tbs, const double* __restrict__ th,
+                                                    const double* __restrict__ sx, const double* __restrict__ sy,
+                                                    double* __restrict__ acc) {
+  __shared__ double sh[256];
+  const int t = blockIdx.x;
+  const TorsBody b = tbs[t];
+  const double theta = th[t];
+  const double c = cos(b.gamma0 - theta), s = sin(b.gamma0 - theta);
+  double p = 0.0;
+  for (int e = threadIdx.x; e < 2 * b.npts; e += blockDim.x) {
+    const int j1 = e < b.npts ? e : e - b.npts;  // j - 1 of the entry's point
+    p += (double)j1 * (sx[b.off + e] * c + sy[b.off + e] * s);
+  }
+  sh[threadIdx.x] = p;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) acc[t] += sh[0];
+}
+
+void ds_weights(hipStream_t st, const TorsBody* tb, const double* th, int nt, const double* sx, const double* sy,
+                double* acc) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_ds_weights, dim3(nt), dim3(256), 0, st, tb, th, sx, sy, acc);
   IB_LAUNCH_CHECK();
 }
 

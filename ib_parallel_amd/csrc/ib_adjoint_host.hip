@@ -487,7 +487,9 @@ int ib_handle::adjoint_step(int sl, bool first) {
   dot_acc(st, nF, lfv, dgv, -1.0, adot);
   if (want_pos) {
     gemv_t(st, nF, Gm, lfv, uTv);
-    pos_fsi(st, nf, tors_of, tapidx, dwx, dwy, arf, f0, htv, qt, vTv, uTv, gpx, gpy);
+    pos_fsi(st, nf, tors_of, tapidx, dwx, dwy, arf, f0, htv, qt, vTv, uTv, gpx, gpy, ntors ? psx.p : nullptr,
+            ntors ? psy.p : nullptr);
+    if (ntors) ds_weights(st, tb, th, ntors, psx, psy, gdsw);
   }
   if (graphs) replay(112 + fk, predictor);
   else predictor();
@@ -614,6 +616,11 @@ int ib_handle::fsi_adjoint(int sl, bool first, const double* f0, const double* q
     double nb2 = -bthdd[t] + brl1 * ds2 - l2 * dQ2;
     gK[t] += scb * brl1 * brf / (den * den) + brl1 * (-delta * 2.0 / dt * (-Q / den)) - l2 * (-Q / den - c2 / kd * brf);
     gG0[t] += r1l1 + c2 * brpf * l2 + S0(t, 4);
+    // ds: BR is linear in it, so d/d(ds) of (scb/den) BR (BR^T f) - sc BR
+    // and of c2 BR^T f are those terms over ds (sc, Q and c2 do not
+    // depend on it); the weights' share is accumulated on the device
+    const double sc = -delta * (2.0 / dt * Q - thd0 - 2.0 / dt * th0);
+    gDS[t] -= ((2.0 * scb / den * brf - sc) * brl1 + c2 * brf * l2) / b.deltas;
     if (first) {
       // initial_acc: thetadd0 = (-C thetad0 - K theta0 + delta/dt BR(theta0)^T f) / M
       const double brp0 = S0(t, 9);
@@ -621,6 +628,7 @@ int ib_handle::fsi_adjoint(int sl, bool first, const double* f0, const double* q
       nb1 += nb2 * (-Cd / M);
       gK[t] += nb2 * (-th0 / M);
       gG0[t] += nb2 * (-delta / dt * brp0 / M);
+      gDS[t] += nb2 * (delta / dt * S0(t, 8) / M / b.deltas);
       nb2 = 0.0;
     }
     bth[t] = nb0;
@@ -657,6 +665,15 @@ int ib_handle::adjoint_run(int nsteps, int Kin, const double* cx, const double* 
   }
   const int nt = ntors;
   bth.assign(nt, 0.0); bthd.assign(nt, 0.0); bthdd.assign(nt, 0.0); gK.assign(nt, 0.0); gG0.assign(nt, 0.0);
+  gDS.assign(nt, 0.0);
+  if (adj_fsi && nt > 0) {
+    if (psx.n < (size_t)nf) {
+      psx.alloc(nf);
+      psy.alloc(nf);
+    }
+    if (gdsw.n < (size_t)nt) gdsw.alloc(nt);
+    fill(st, gdsw, nt, 0.0);
+  }
   const long LP = (long)L * g.P;
   const long CQX = (long)L * g.Fx, CQY = (long)L * g.Fy, BS = 6 * (long)nt + 2 * (long)nf;
   // objective weights per force entry: write_force2's 2 delta / dt, time mean
@@ -769,6 +786,11 @@ int ib_handle::adjoint_run(int nsteps, int Kin, const double* cx, const double* 
     sync();
     hinge_gx.assign(nbodies, 0.0);
     hinge_gy.assign(nbodies, 0.0);
+    hinge_gds.assign(nbodies, 0.0);
+    std::vector<double> dsw(std::max(1, ntors), 0.0);
+    if (adj_fsi && ntors > 0) d2h(dsw.data(), gdsw, ntors);
+    sync();
+    for (int t = 0; t < ntors && adj_fsi; ++t) hinge_gds[h_cbody[h_tb[t].off]] = gDS[t] + dsw[t];
     int pt = 0;
     for (int k = 0; k < nf;) {
       int n = 0;

@@ -379,9 +379,12 @@ struct ib_handle {
   DBuf<double> t_th, t_f0, c_th, c_f, fin_th;
   int64_t dG_build = -1;
   std::vector<double> bth, bthd, bthdd, gK, gG0, h_scal;
-  // hinge gradients per body of the last adjoint run with position gradients
-  std::vector<double> hinge_gx, hinge_gy;
+  // hinge and point-spacing gradients per body of the last adjoint run with
+  // position gradients; gDS: the spacing's structural share per torsional
+  // body, gdsw its weights' share (device), psx/psy a step's point partials
+  std::vector<double> hinge_gx, hinge_gy, hinge_gds, gDS;
   bool hinge_ok = false;
+  DBuf<double> psx, psy, gdsw;
   // pressure post-processing (allocated on first use): cell-grid DST plans
   // and table, corner products (two levels), rhs, solution, scratch
   bool pres_ready = false;

@@ -396,9 +396,14 @@ void gemv_t(hipStream_t st, int n, const double* G, const double* x, double* y);
 void transpose(hipStream_t st, int n, const double* A, double* AT);
 void tors_block(hipStream_t st, const TorsBody* tb, const double* th, int nt, int t, int at0, double dt, double delta,
                 const double* f, const double* a, double* r1, double* zc, double* s);
+// gx, gy += the step's position partials of every entry; sx, sy (may be
+// null) get them alone
 void pos_fsi(hipStream_t st, int nf, const int* tors_of, const int* tapidx, const double* dwx, const double* dwy,
              const double* l, const double* f, const double* hb, const double* qt, const double* vT, const double* uT,
-             double* gx, double* gy);
+             double* gx, double* gy, double* sx, double* sy);
+// acc[t] += the step's partials of torsional body t projected on d(points)/d(ds)
+void ds_weights(hipStream_t st, const TorsBody* tb, const double* th, int nt, const double* sx, const double* sy,
+                double* acc);
 void dot_set(hipStream_t st, int i0, int i1, const double* a, const double* b, double* out);
 void rot_acc(hipStream_t st, const Geom& g, const double* hx, const double* hy, double* v);
 void rhs_adj(hipStream_t st, long n, const double* r, double c1, double c2, double* ro, double* rb);

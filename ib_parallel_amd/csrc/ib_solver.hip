@@ -507,12 +507,13 @@ int ib_adjoint_run(ib_handle* h, int32_t nsteps, int32_t ckpt_every, const doubl
   API_CATCH(h)
 }
 
-int ib_adjoint_hinge_grad(const ib_handle* h, double* dJ_dxh, double* dJ_dyh) {
-  if (!h || !dJ_dxh || !dJ_dyh) return IB_E_INVAL;
+int ib_adjoint_hinge_grad(const ib_handle* h, double* dJ_dxh, double* dJ_dyh, double* dJ_dds) {
+  if (!h) return IB_E_INVAL;
   if (!h->hinge_ok) return IB_E_INVAL;
   for (int b = 0; b < h->nbodies; ++b) {
-    dJ_dxh[b] = h->hinge_gx[b];
-    dJ_dyh[b] = h->hinge_gy[b];
+    if (dJ_dxh) dJ_dxh[b] = h->hinge_gx[b];
+    if (dJ_dyh) dJ_dyh[b] = h->hinge_gy[b];
+    if (dJ_dds) dJ_dds[b] = h->hinge_gds[b];
   }
   return IB_OK;
 }

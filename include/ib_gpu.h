@@ -345,15 +345,17 @@ int ib_adjoint_run(ib_handle* h, int32_t nsteps, int32_t ckpt_every, const doubl
                    double* dJ_dRe, double* dJ_domega0, double* dJ_dxb, double* dJ_dyb, double* dJ_dK,
                    double* dJ_dgamma0);
 /* the gradient of the last ib_adjoint_run's J with respect to the hinge
- * (xbf0, ybf0) of every torsional body (no reference counterpart; the
- * hinge is the body's first deck point, src/grid.F90:252-288, and places its
+ * (xbf0, ybf0) and the point spacing ds of every torsional body (no
+ * reference counterpart; the hinge is the body's first deck point and ds
+ * the distance between its first two, src/grid.F90:252-288; they place its
  * points with the angle, src/operators_struct.F90:600-620): one entry per
- * body in deck order, 0 for the others.  A hinge shift translates the
- * body's points and changes neither its arms nor its coupling vector, so the
- * gradient is the sum over its points of the position partials.  Needs a
- * run in the face-space form that asked for position gradients (dJ_dxb or
- * dJ_dyb not NULL); IB_E_INVAL otherwise. */
-int ib_adjoint_hinge_grad(const ib_handle* h, double* dJ_dxh, double* dJ_dyh);
+ * body in deck order, 0 for the others; any pointer may be NULL.  A hinge
+ * shift translates the body's points (the sum of their position partials);
+ * ds also scales the arms, so its gradient adds the coupling vector's and
+ * the structural block's terms.  Needs a run in the face-space form that
+ * asked for position gradients (dJ_dxb or dJ_dyb not NULL); IB_E_INVAL
+ * otherwise. */
+int ib_adjoint_hinge_grad(const ib_handle* h, double* dJ_dxh, double* dJ_dyh, double* dJ_dds);
 
 #ifdef __cplusplus
 }

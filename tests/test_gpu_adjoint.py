@@ -359,19 +359,23 @@ def test_fsi_re_omega_and_rigid_positions_match_finite_differences():
     assert abs(ad - fd) <= 1e-5 * abs(fd)
 
 
-def test_fsi_hinge_gradient_matches_finite_differences():
-    """The torsional plate's hinge gradient (ib_adjoint_hinge_grad: the sum
-    over its points of the position partials, since a hinge shift translates
-    the points and leaves the arms alone) against central differences of J
-    with the plate translated by +-h: its deck points, and so its hinge, the
-    first of them.  The plate is first moved off the grid lines (its hinge
-    sits on them in tors_case, where deltafnc's REAL(4) literals jump)."""
+def test_fsi_hinge_and_spacing_gradients_match_finite_differences():
+    """The torsional plate's hinge and point-spacing gradients
+    (ib_adjoint_hinge_grad) against central differences of J: the plate
+    translated by +-h (its deck points, and so its hinge, the first of
+    them), and its deck points scaled about the hinge so that ds changes by
+    +-h.  The hinge gradient is the sum of the points' position partials;
+    ds also scales the arms (the coupling vector and the structural block).
+    The plate is first moved off the grid lines (its hinge sits on them in
+    tors_case, where deltafnc's REAL(4) literals jump)."""
     from ib_parallel_amd import IBError
 
-    def shifted(dx, dy):
+    def shifted(dx, dy, dds=0.0):
         p2, b2 = tors_case()
-        b2[1].x = b2[1].x + 0.0123 + dx
-        b2[1].y = b2[1].y - 0.0071 + dy
+        x, y = b2[1].x + 0.0123, b2[1].y - 0.0071
+        ds = np.hypot(x[1] - x[0], y[1] - y[0])
+        x, y = x[0] + (x - x[0]) * (1 + dds / ds), y[0] + (y - y[0]) * (1 + dds / ds)
+        b2[1].x, b2[1].y = x + dx, y + dy
         return p2, b2
     p, b = shifted(0.0, 0.0)
     L, P = p.mgridlev, (p.m - 1) * (p.n - 1)
@@ -381,8 +385,8 @@ def test_fsi_hinge_gradient_matches_finite_differences():
     n = 4
     s = sim_from(p, b, w0, np.zeros_like(w0))
     s.adjoint(n, cx, cy, ckpt_every=3, omega_grad=False, position_grad=True)
-    gxh, gyh = s.hinge_grad()
-    assert gxh[0] == 0.0 and gyh[0] == 0.0  # the rigid cylinder has no hinge
+    gxh, gyh, gds = s.hinge_grad()
+    assert gxh[0] == 0.0 and gyh[0] == 0.0 and gds[0] == 0.0  # the rigid cylinder has no hinge
     h = 1e-6
     for ex, ey in ((1.0, 0.0), (0.0, 1.0)):
         fd = (tors_J(*shifted(h * ex, h * ey), w0, n, cx, cy) - tors_J(*shifted(-h * ex, -h * ey), w0, n, cx, cy)) \
@@ -390,6 +394,9 @@ def test_fsi_hinge_gradient_matches_finite_differences():
         ad = gxh[1] * ex + gyh[1] * ey
         print(f"dJ/dhinge ({ex}, {ey}): adjoint {ad:.12e} fd {fd:.12e}")
         assert abs(ad - fd) <= 1e-5 * abs(fd)
+    fd = (tors_J(*shifted(0.0, 0.0, h), w0, n, cx, cy) - tors_J(*shifted(0.0, 0.0, -h), w0, n, cx, cy)) / (2 * h)
+    print(f"dJ/dds: adjoint {gds[1]:.12e} fd {fd:.12e}")
+    assert abs(gds[1] - fd) <= 1e-5 * abs(fd)
     # a run without position gradients leaves none to report
     s.adjoint(1, cx, cy, omega_grad=False)
     with pytest.raises(IBError) as e:
