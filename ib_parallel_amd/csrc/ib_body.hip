@@ -635,19 +635,53 @@ void map_taps(hipStream_t st, int nf, const int* key, const int* fmap, int* tapi
   IB_LAUNCH_CHECK();
 }
 
-// T(:, j) = sum_q w[j,q] G(:, tapidx[j,q]): one thread per face row b,
-// column j = jl[blockIdx.y] (jl NULL: blockIdx.y).  A tap outside the face
+// T(:, j) = sum_q w[j,q] G(:, tapidx[j,q]) for the columns j = jl[jq] (jl
+// NULL: j = jq), jq < nj: one thread per face row b.  A tap outside the face
 // set (tapidx -1, which k_map_taps flags) reads face 0: the FSI loop forms S
 // before it has read that flag and forms it again after rebuilding the set.
-__global__ void __launch_bounds__(256) k_green_right(int nF, const double* __restrict__ w,
+// Work item u = tile * nj + jq covers rows 256 tile .. 256 tile + 255 of
+// column j.  The launch has 8 per blocks; the k-th block that the dispatcher
+// deals to XCD x (block 8 k + x) takes u = x per + k, so every XCD forms
+// whole row tiles: a tile's rows of the G columns reach one L2 and serve
+// every column j there.  The 49 loads of a row are issued in two batches
+// before their sum, which runs in tap order.
+constexpr int kGreenHalf = 25;
+__global__ void __launch_bounds__(256) k_green_right(int nF, int nj, long total, long per,
+                                                     const double* __restrict__ w,
                                                      const int* __restrict__ tapidx, const double* __restrict__ G,
                                                      const int* __restrict__ jl, double* __restrict__ T) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = jl ? jl[blockIdx.y] : blockIdx.y;
+  __shared__ long s_col[kTaps];
+  __shared__ double s_w[kTaps];
+  const long u = (long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (u >= total) return;  // the whole block
+  const int tile = (int)(u / nj), jq = (int)(u - (long)tile * nj);
+  const int j = jl ? jl[jq] : jq;
+  if (threadIdx.x < kTaps) {
+    s_col[threadIdx.x] = (long)max(tapidx[j * kTaps + threadIdx.x], 0) * nF;
+    s_w[threadIdx.x] = w[j * kTaps + threadIdx.x];
+  }
+  __syncthreads();
+  const int b = tile * 256 + threadIdx.x;
   if (b >= nF) return;
+  double gv[kGreenHalf];
   double acc = 0.0;
-  for (int q = 0; q < kTaps; ++q) acc += w[j * kTaps + q] * G[(long)max(tapidx[j * kTaps + q], 0) * nF + b];
+#pragma unroll
+  for (int q = 0; q < kGreenHalf; ++q) gv[q] = G[s_col[q] + b];
+#pragma unroll
+  for (int q = 0; q < kGreenHalf; ++q) acc += s_w[q] * gv[q];
+#pragma unroll
+  for (int q = kGreenHalf; q < kTaps; ++q) gv[q - kGreenHalf] = G[s_col[q] + b];
+#pragma unroll
+  for (int q = kGreenHalf; q < kTaps; ++q) acc += s_w[q] * gv[q - kGreenHalf];
   T[(long)j * nF + b] = acc;
+}
+
+static void green_right(hipStream_t st, int nF, int nj, const double* w, const int* tapidx, const double* G,
+                        const int* jl, double* T) {
+  const long total = (long)ceil_div(nF, 256) * nj, per = (total + 7) / 8;
+  hipLaunchKernelGGL(k_green_right, dim3((unsigned)(8 * per)), dim3(256), 0, st, nF, nj, total, per, w, tapidx, G,
+                     jl, T);
+  IB_LAUNCH_CHECK();
 }
 
 // S(i, j) = sum_p w[i,p] T(tapidx[i,p], j) + b_times2(e_j)(i): one thread per
@@ -664,8 +698,38 @@ __global__ void __launch_bounds__(256) k_green_left(int nf, int nF, int ni, cons
   const int i = il ? il[x] : x;
   const int j = jl ? jl[blockIdx.y] : blockIdx.y;
   const double* Tj = T + (long)j * nF;
+  const int* tp = tapidx + (long)i * kTaps;
+  const double* wi = w + (long)i * kTaps;
+  // the row's taps in two batches of independent loads, each summed in tap
+  // order
   double acc = 0.0;
-  for (int p = 0; p < kTaps; ++p) acc += w[i * kTaps + p] * Tj[max(tapidx[i * kTaps + p], 0)];
+  {
+    int ti[kGreenHalf];
+    double wv[kGreenHalf], tv[kGreenHalf];
+#pragma unroll
+    for (int p = 0; p < kGreenHalf; ++p) {
+      ti[p] = tp[p];
+      wv[p] = wi[p];
+    }
+#pragma unroll
+    for (int p = 0; p < kGreenHalf; ++p) tv[p] = Tj[max(ti[p], 0)];
+#pragma unroll
+    for (int p = 0; p < kGreenHalf; ++p) acc += wv[p] * tv[p];
+  }
+  {
+    constexpr int n2 = kTaps - kGreenHalf;
+    int ti[n2];
+    double wv[n2], tv[n2];
+#pragma unroll
+    for (int p = 0; p < n2; ++p) {
+      ti[p] = tp[kGreenHalf + p];
+      wv[p] = wi[kGreenHalf + p];
+    }
+#pragma unroll
+    for (int p = 0; p < n2; ++p) tv[p] = Tj[max(ti[p], 0)];
+#pragma unroll
+    for (int p = 0; p < n2; ++p) acc += wv[p] * tv[p];
+  }
   S[(long)j * nf + i] = 1.0 * acc + btimes_entry(i, j, tors_of, tbs, theta, dt, delta);
 }
 
@@ -673,8 +737,7 @@ void schur_from_green(hipStream_t st, int nf, int nF, const double* w, const int
                       double* T, const int* tors_of, const TorsBody* tb, const double* theta, double dt,
                       double delta, double* S) {
   if (nf <= 0 || nF <= 0) return;
-  hipLaunchKernelGGL(k_green_right, dim3(ceil_div(nF, 256), nf), dim3(256), 0, st, nF, w, tapidx, G, nullptr, T);
-  IB_LAUNCH_CHECK();
+  green_right(st, nF, nf, w, tapidx, G, nullptr, T);
   hipLaunchKernelGGL(k_green_left, dim3(ceil_div(nf, 256), nf), dim3(256), 0, st, nf, nF, nf, nullptr, nullptr, w,
                      tapidx, T, tors_of, tb, theta, dt, delta, S);
   IB_LAUNCH_CHECK();
@@ -685,8 +748,7 @@ void schur_from_green_moving(hipStream_t st, int nf, int nF, const double* w, co
                              double delta, double* S, int nmov, const int* mov, int nrig, const int* rig) {
   if (nf <= 0 || nF <= 0 || nmov <= 0) return;
   // the moving columns of T
-  hipLaunchKernelGGL(k_green_right, dim3(ceil_div(nF, 256), nmov), dim3(256), 0, st, nF, w, tapidx, G, mov, T);
-  IB_LAUNCH_CHECK();
+  green_right(st, nF, nmov, w, tapidx, G, mov, T);
   // every row of the moving columns of S
   hipLaunchKernelGGL(k_green_left, dim3(ceil_div(nf, 256), nmov), dim3(256), 0, st, nf, nF, nf, nullptr, mov, w,
                      tapidx, T, tors_of, tb, theta, dt, delta, S);
