@@ -1625,11 +1625,15 @@ __device__ __forceinline__ void coopw_solve(const GmCoopWArgs& a, int c, unsigne
       // the row sums of S w for the next step (from L2, beside the norm
       // partials and the Givens step of wavefront 0)
       const int t0 = gm_opaque(threadIdx.x);
+      // wavefront 0 loads the P norm partials before the product's vector
+      // (lane q partial q, as gw_wave_gather), so that the two L2 round
+      // trips overlap
+      const double npart = t0 < a.P ? xc_ld<X2>(a.norms + (long)t0 * 4) : 0.0;
       const double swsum = matvec_g(a.wfull);
       const double wpre_l = t0 < Rc ? ub[r0 + t0] : 0.0;
       if (gm_opaque(threadIdx.x) < 64) {
         const int lane = gm_opaque(threadIdx.x);
-        const double tt = sqrt(gw_wave_gather<X2>(a.norms, 4, a.P, lane));
+        const double tt = sqrt(gm_wave_sum(npart));
         if (lane == 0) {
           // KSPGMRESUpdateHessenberg: tt < min(|tt / grs[it]|, 1e-30) is a
           // happy breakdown (the quotient is needed only when tt < 1e-30)
