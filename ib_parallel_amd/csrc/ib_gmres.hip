@@ -1482,7 +1482,7 @@ __device__ __forceinline__ void coopw_solve(const GmCoopWArgs& a, int c, unsigne
   auto rowpair = [&](const double* coef, int jn, int ra, int rb, int l, double& acc0, double& acc1) {
     const int ja = min(jn, NL);
     int j = l;
-#pragma unroll 4
+#pragma unroll 8
     for (; j < ja; j += 64) {
       const double h = coef[j];
       acc0 += h * Vl[ra * NL + j];
@@ -1558,11 +1558,11 @@ __device__ __forceinline__ void coopw_solve(const GmCoopWArgs& a, int c, unsigne
         double p = 0.0;
         if (j <= it) {
           if (j < NL) {
-#pragma unroll 4
+#pragma unroll 8
             for (int r = sub; r < Rc; r += g) p += wl[r] * Vl[r * NL + j];
           } else {
             const double* vg = Vgc + (j - NL);
-#pragma unroll 4
+#pragma unroll 8
             for (int r = sub; r < Rc; r += g) p += wl[r] * vg[(size_t)r * ldg];
           }
         }
