@@ -591,8 +591,8 @@ def _host_cpus():
 
 
 def run_cpu(args, gpu_its_per_step, gpu_fsi_per_step):
-    """The oracle on the host cores (BASELINE.md's plan: all cores and one
-    core, warm-up excluded).
+    """The oracle on the host cores: on all cores and on one core, with the
+    warm-up step excluded.
 
     Headline workload: one oracle step there takes minutes (matrix-free
     GMRES, ~0.07-0.3 s per Krylov iteration on 8-16 cores), so the sample is
