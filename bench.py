@@ -407,7 +407,9 @@ def run_adjoint(args):
     res = {}
     n = args.adjoint_steps
     if n > 0:
-        p, b = decks.cylinder_case(num_stat=1)
+        # --adjoint-launch-mode 1: plain launches instead of the step and
+        # reverse-step graphs (for kernel traces of the reverse sweep)
+        p, b = decks.cylinder_case(num_stat=1, launch_mode=args.adjoint_launch_mode)
         sim = GpuSim(p, b)
         for _ in range(3):
             sim.advance()
@@ -720,6 +722,8 @@ def main():
                     help="timed steps of the flexible-plate (beam) workload (0: skip)")
     ap.add_argument("--dst-compare", default=None, metavar="MxN",
                     help="only the comparison of the two DST forms on an M x N grid (one JSON object)")
+    ap.add_argument("--adjoint-launch-mode", type=int, default=0, choices=[0, 1],
+                    help="launch_mode of the rigid adjoint workload (1: no HIP graphs)")
     ap.add_argument("--dst-reps", type=int, default=50)
     ap.add_argument("--dst-dense-cfg", type=int, default=-1,
                     help="tile configuration of the matrix-core handle in --dst-compare (-1: timed)")

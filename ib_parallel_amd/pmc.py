@@ -140,9 +140,7 @@ def traffic(args):
               f"{v['bytes_per_launch'] / 1e6:.3f} | {'' if fo is None else f'{fo / 1e6:.3f}'} |")
 
 
-def after_copy_durations(trace_dir):
-    """{kernel name: [ns, ...]} of the dispatches after the calibration copy
-    in a --kernel-trace run."""
+def _trace_dispatches(trace_dir):
     files = glob.glob(os.path.join(trace_dir, "**", "*kernel_trace.csv"), recursive=True)
     if not files:
         raise SystemExit(f"no kernel_trace.csv under {trace_dir}")
@@ -150,12 +148,34 @@ def after_copy_durations(trace_dir):
     with open(files[0]) as fh:
         for r in csv.DictReader(fh):
             disp.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["End_Timestamp"]) - float(r["Start_Timestamp"])))
+    return sorted(disp)
+
+
+def after_copy_durations(trace_dir):
+    """{kernel name: [ns, ...]} of the dispatches after the calibration copy
+    in a --kernel-trace run."""
+    disp = _trace_dispatches(trace_dir)
     copies = [d for d in disp if "copy" in d[1].lower()]
     mark = max(copies, key=lambda d: d[2])[0] if copies else -1
     agg = defaultdict(list)
     for d, name, ns in disp:
         if d > mark and "copy" not in name.lower():
             agg[name].append(ns)
+    return agg
+
+
+def window_durations(trace_dir, first, until):
+    """{kernel name: [ns, ...]} of the dispatches from the first one of the
+    kernel named `first` (short name) up to, not including, the next one of
+    the kernel named `until` (to the end of the run when there is none)."""
+    disp = _trace_dispatches(trace_dir)
+    i0 = next((i for i, d in enumerate(disp) if short(d[1]) == first), None)
+    if i0 is None:
+        raise SystemExit(f"no dispatch of {first}")
+    i1 = next((i for i in range(i0 + 1, len(disp)) if until and short(disp[i][1]) == until), len(disp))
+    agg = defaultdict(list)
+    for _, name, ns in disp[i0:i1]:
+        agg[name].append(ns)
     return agg
 
 
@@ -215,9 +235,11 @@ def stats(args):
     """Markdown table of a rocprofv3 --kernel-trace --stats run: the
     whole-process kernel_stats.csv, or with --after-copy the kernel trace
     restricted to the dispatches after the calibration copy (the timed
-    steps of a bench.py run with --pmc-copy-mb)."""
-    if args.after_copy:
-        agg = after_copy_durations(args.trace_dir)
+    steps of a bench.py run with --pmc-copy-mb), or with --first/--until
+    the window between two kernels' dispatches (a phase of the run)."""
+    if args.after_copy or args.first:
+        agg = (window_durations(args.trace_dir, args.first, args.until) if args.first
+               else after_copy_durations(args.trace_dir))
         rows = [{"Name": k, "Calls": len(v), "TotalDurationNs": sum(v), "AverageNs": sum(v) / len(v),
                  "MinNs": min(v), "MaxNs": max(v)} for k, v in agg.items()]
     else:
@@ -248,6 +270,8 @@ def main():
     s.add_argument("trace_dir")
     s.add_argument("--top", type=int, default=25)
     s.add_argument("--after-copy", action="store_true", help="only the dispatches after the calibration copy")
+    s.add_argument("--first", default=None, help="only the dispatches from the first one of this kernel")
+    s.add_argument("--until", default=None, help="... up to the next one of this kernel")
     r = sub.add_parser("rates")
     r.add_argument("trace_dir")
     r.add_argument("traffic_json")

@@ -104,8 +104,23 @@ def test_stats_after_the_copy(tmp_path, capsys):
         (5, KB, 103000, 103500),
         (6, KA, 104000, 108000),
     ])
-    pmc.stats(argparse.Namespace(trace_dir=str(tmp_path / "trace"), after_copy=True, top=10))
+    pmc.stats(argparse.Namespace(trace_dir=str(tmp_path / "trace"), after_copy=True, first=None, until=None, top=10))
     lines = [ln for ln in capsys.readouterr().out.splitlines() if ln.startswith("| k_")]
     # k_alpha: 2 and 4 us -> 2 calls, 6 us total, 3 us mean, share 6 / 6.5
     assert lines[0].split("|")[1:8] == [" k_alpha ", " 2 ", " 0.006 ", " 3.00 ", " 2.00 ", " 4.00 ", " 0.923 "]
     assert lines[1].split("|")[1:4] == [" k_beta<2, true> ", " 1 ", " 0.001 "]
+
+
+def test_stats_between_two_kernels(tmp_path, capsys):
+    write_trace(str(tmp_path / "trace"), [
+        (1, KB, 0, 1000),
+        (2, KA, 2000, 3000),          # the window opens at the first k_alpha
+        (3, KA, 4000, 7000),
+        (4, KB, 8000, 8500),          # ... and closes before this k_beta
+        (5, KA, 9000, 19000),
+    ])
+    pmc.stats(argparse.Namespace(trace_dir=str(tmp_path / "trace"), after_copy=False, first="k_alpha",
+                                 until="k_beta<2, true>", top=10))
+    lines = [ln for ln in capsys.readouterr().out.splitlines() if ln.startswith("| k_")]
+    assert len(lines) == 1
+    assert lines[0].split("|")[1:5] == [" k_alpha ", " 2 ", " 0.004 ", " 2.00 "]
